@@ -1,0 +1,160 @@
+/* taco_tcl4.h — C ABI of the MI355X (gfx950) TCL4 engine.
+ *
+ * Conventions shared by every entry point:
+ *   - complex values are interleaved (re, im) doubles, the byte layout of
+ *     std::complex<double> / Eigen::MatrixXcd;
+ *   - matrices are column-major (Eigen default): A(r, c) = A[r + c*rows];
+ *   - gamma is the [Nt x nf] Gamma table, column-major: gamma[t + Nt*b];
+ *   - pair_to_freq is N x N column-major; ops are num_ops consecutive N x N
+ *     matrices (the coupling operators in the eigenbasis, A_eig);
+ *   - outputs are caller-owned; every call is synchronous unless the name
+ *     ends in _dev, in which case pointers are device pointers and the call
+ *     returns after the work is enqueued on the context stream and
+ *     completed (it synchronises the stream before returning);
+ *   - every function returns TACO_OK (0) or an error code; taco_last_error()
+ *     gives the message of the last failure on the calling thread.
+ *
+ * Error codes mirror the exception types the reference throws
+ * (cpp/src/backend/cuda/tcl4_fused_cuda.cu:820-851, 1107-1216):
+ *   TACO_ERR_INVALID   std::invalid_argument
+ *   TACO_ERR_RANGE     std::out_of_range
+ *   TACO_ERR_RUNTIME   std::runtime_error (missing buckets, device errors)
+ *   TACO_ERR_OVERFLOW  std::overflow_error
+ */
+#ifndef TACO_TCL4_H
+#define TACO_TCL4_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TACO_OK 0
+#define TACO_ERR_INVALID 1
+#define TACO_ERR_RANGE 2
+#define TACO_ERR_RUNTIME 3
+#define TACO_ERR_OVERFLOW 4
+
+/* FCRMethod (cpp/include/taco/tcl4_kernels.hpp:9-12). */
+#define TACO_FCR_CONVOLUTION 0
+#define TACO_FCR_DIRECT 1
+
+typedef struct taco_ctx taco_ctx;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* taco_last_error(void);
+
+/* Library version string. Replaces taco::version() (cpp/src/core/version.cpp:1-7). */
+const char* taco_version(void);
+
+/* One context per host thread, bound to one GPU: it owns a stream, the
+ * twiddle-table cache and a grow-only device workspace, like the reference's
+ * thread_local FusedCudaWorkspace (tcl4_fused_cuda.cu:676-685). */
+int taco_ctx_create(int device, taco_ctx** out);
+int taco_ctx_destroy(taco_ctx* ctx);
+/* The HIP stream (hipStream_t) the context enqueues on. */
+void* taco_ctx_stream(taco_ctx* ctx);
+
+/* Convolution FFT padding factor, process-global like the reference
+ * (set_fcr_fft_pad_factor / get_fcr_fft_pad_factor, tcl4_kernels.cpp:179-185).
+ * Nfft = next_pow2(max(2*Nt-1, factor*Nt)), at least 2. */
+void taco_set_fcr_fft_pad_factor(size_t factor);
+size_t taco_get_fcr_fft_pad_factor(void);
+
+/* Correctly rounded sin/cos of each x[i] (|x| < 2^30): the phase routine of
+ * the F/C/R kernels, exp(+-i*omega*(k*dt)) = (c, +-s).  The reference calls
+ * std::exp of an imaginary argument (tcl4_kernels.cpp:116-120), whose last
+ * bits depend on the platform libm; the engine fixes them to the correctly
+ * rounded values.  ctx == NULL evaluates on the host, else on the GPU. */
+int taco_cr_sincos(taco_ctx* ctx, const double* x, size_t n, double* s, double* c);
+
+/* --------------------------------------------------------------------------
+ * Inputs of the hot path (host).  These produce what the reference's callers
+ * feed the builders: sys::System (system.hpp:201-226) plus build_map
+ * (tcl4.cpp:40-101), C(t) (bcf_fft_fun, correlation_fft.hpp:154-214) and
+ * the Gamma table (compute_trapz_prefix_multi_matrix, gamma.hpp:166-213).
+ * ------------------------------------------------------------------------ */
+
+/* H: N x N Hermitian (lab basis).  A_lab: num_ops x (N x N).  Outputs sized
+ * for nf <= N*N: eps[N] ascending, U[N*N] eigenvectors (columns),
+ * A_eig[num_ops*N*N], omegas[nf], mirror[nf], pair_to_freq[N*N]. */
+int taco_system_build(const double* H, int N, const double* A_lab, int num_ops, double freq_tol,
+                      double* eps, double* U, double* A_eig, int* nf, double* omegas, int* mirror,
+                      int* pair_to_freq, int* zero_index);
+
+/* Spectral density kinds for taco_bcf_fft. */
+#define TACO_J_OHMIC 0          /* J = p0 * w * exp(-w / p1)       (alpha, omega_c) */
+#define TACO_J_DRUDE_LORENTZ 1  /* J = 2 p0 p1 w / (w^2 + p1^2)    (lambda, gamma)  */
+
+/* C(t_n), n = 0..N (N+1 samples); beta may be +inf (zero temperature). */
+int taco_bcf_fft(size_t N, double dt, int kind, double p0, double p1, double beta, double* C_out);
+
+/* Gamma[t + Nc*b] = trapz prefix of exp(i omega_b tau) C(tau), Gamma[0] = 0. */
+int taco_gamma_trapz(const double* C, size_t Nc, double dt, const double* omegas, int nf, double* gamma_out);
+
+/* --------------------------------------------------------------------------
+ * Hot path (GPU).
+ * ------------------------------------------------------------------------ */
+
+/* compute_triple_kernels (tcl4.hpp:37-42; tcl4.cpp:103-183) and
+ * compute_triple_kernels_cuda (tcl4_kernels_cuda.hpp:7-12).  Writes the first
+ * `keep` samples (keep <= Nt; 0 means Nt) of every F/C/R series:
+ * F[(idx3 * keep) + t], idx3 = (i*nf + j)*nf + k.  The FFT length follows the
+ * full Nt, so the kept samples equal the reference's full series.
+ * Only TACO_FCR_CONVOLUTION is supported on the GPU (as in the reference). */
+int taco_triple_kernels(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, const double* omegas,
+                        const int* mirror, double dt, int method, size_t keep, double* F, double* C,
+                        double* R);
+
+/* build_TCL4_generator_cuda_fused_batch (tcl4_fused_cuda.hpp:25-41):
+ * L4(t) for t in tidx[0..ntau) (ntau == 0 means every t < Nt), each an
+ * N^2 x N^2 column-major block of L4_out.  fcr_ms (nullable) receives the
+ * device time of the F/C/R stage, like the reference's cuda_fcr_ms. */
+int taco_l4_batch(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, const double* omegas,
+                  const int* mirror, const int* pair_to_freq, int N, const double* ops, int num_ops,
+                  double dt, int method, const size_t* tidx, size_t ntau, double* L4_out, double* fcr_ms);
+
+/* Same, with gamma/omegas/mirror/pair_to_freq/ops/L4_out as device pointers
+ * (inputs already resident in HBM); tidx stays a host array. */
+int taco_l4_batch_dev(taco_ctx* ctx, const void* d_gamma, size_t Nt, int nf, const void* d_omegas,
+                      const void* d_mirror, const void* d_pair_to_freq, int N, const void* d_ops, int num_ops,
+                      double dt, int method, const size_t* tidx, size_t ntau, void* d_L4_out,
+                      double* fcr_ms);
+
+/* build_TCL4_generator_cuda_fused (tcl4_fused_cuda.hpp:16-21): one L4(t). */
+int taco_l4_single(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, const double* omegas,
+                   const int* mirror, const int* pair_to_freq, int N, const double* ops, int num_ops,
+                   double dt, int method, size_t time_index, double* L4_out);
+
+/* assemble_liouvillian + gw_to_liouvillian on the GPU from nf^3 F/C/R slices
+ * at one time (the device helpers build_mikx_device,
+ * assemble_liouvillian_cuda_device_raw, tcl4_mikx_cuda.hpp:44-46 and
+ * tcl4_assemble_cuda.hpp:36-44, fused).  GW_out (nullable) gets GW. */
+int taco_l4_from_slices(taco_ctx* ctx, const double* Fs, const double* Cs, const double* Rs,
+                        const int* pair_to_freq, int N, int nf, const double* ops, int num_ops,
+                        double* L4_out, double* GW_out);
+
+/* propagate_rk4_dense_serial(L_series[, L_half_series], r, t0, dt, on_sample,
+ * sample_every) (rk4_dense.hpp:85-100; rk4_dense.cpp:246-320).  L_series:
+ * (steps+1) D x D blocks; L_half nullable (then 0.5*(L_i + L_{i+1})).
+ * r: in/out vec(rho) of length D.  samples_out (nullable) receives r at
+ * steps 0, every, 2*every, ... and the final r; *nsamples gets the count. */
+int taco_rk4_propagate(taco_ctx* ctx, const double* L_series, const double* L_half, size_t steps, int D,
+                       double* r, double dt, size_t sample_every, double* samples_out, size_t* nsamples);
+
+/* The drivers' composition (TCL4_spin_boson_example.cpp:284-384): RK4 on
+ * L(t) = L_TCL2(t) + L4(t) (order 4), L_TCL2 (order 2) or -i[diag(eps), .]
+ * (order 0), t = 0..steps, midpoint generator 0.5*(L_n + L_{n+1}).
+ * eps: eigenvalues; ops: A_eig.  r: in/out vec(rho) (eigenbasis).
+ * samples_out (nullable): (steps+1) x N^2 states. */
+int taco_tcl4_propagate(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, const double* omegas,
+                        const int* mirror, const int* pair_to_freq, int N, const double* eps,
+                        const double* ops, int num_ops, double dt, size_t steps, int order, double* r,
+                        double* samples_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TACO_TCL4_H */

@@ -1,0 +1,109 @@
+"""ctypes loader of libtaco_tcl4.so (the HIP engine).
+
+The product has no CPU fallback: if the shared library is missing or cannot
+be loaded, every entry point raises.  Build it with ``python -m taco_amd.build``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_int, c_size_t, c_void_p
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtaco_tcl4.so")
+
+TACO_OK = 0
+TACO_ERR_INVALID = 1
+TACO_ERR_RANGE = 2
+TACO_ERR_RUNTIME = 3
+TACO_ERR_OVERFLOW = 4
+
+_dp = POINTER(c_double)
+_ip = POINTER(c_int)
+_sp = POINTER(c_size_t)
+
+# name -> (restype, argtypes); every symbol declared in include/taco_tcl4.h.
+SIGNATURES = {
+    "taco_last_error": (c_char_p, []),
+    "taco_version": (c_char_p, []),
+    "taco_ctx_create": (c_int, [c_int, POINTER(c_void_p)]),
+    "taco_ctx_destroy": (c_int, [c_void_p]),
+    "taco_ctx_stream": (c_void_p, [c_void_p]),
+    "taco_set_fcr_fft_pad_factor": (None, [c_size_t]),
+    "taco_cr_sincos": (c_int, [c_void_p, _dp, c_size_t, _dp, _dp]),
+    "taco_get_fcr_fft_pad_factor": (c_size_t, []),
+    "taco_system_build": (c_int, [_dp, c_int, _dp, c_int, c_double, _dp, _dp, _dp, _ip, _dp, _ip, _ip, _ip]),
+    "taco_bcf_fft": (c_int, [c_size_t, c_double, c_int, c_double, c_double, c_double, _dp]),
+    "taco_gamma_trapz": (c_int, [_dp, c_size_t, c_double, _dp, c_int, _dp]),
+    "taco_triple_kernels": (c_int, [c_void_p, _dp, c_size_t, c_int, _dp, _ip, c_double, c_int, c_size_t,
+                                    _dp, _dp, _dp]),
+    "taco_l4_batch": (c_int, [c_void_p, _dp, c_size_t, c_int, _dp, _ip, _ip, c_int, _dp, c_int, c_double,
+                              c_int, _sp, c_size_t, _dp, _dp]),
+    "taco_l4_batch_dev": (c_int, [c_void_p, c_void_p, c_size_t, c_int, c_void_p, c_void_p, c_void_p, c_int,
+                                  c_void_p, c_int, c_double, c_int, _sp, c_size_t, c_void_p, _dp]),
+    "taco_l4_single": (c_int, [c_void_p, _dp, c_size_t, c_int, _dp, _ip, _ip, c_int, _dp, c_int, c_double,
+                               c_int, c_size_t, _dp]),
+    "taco_l4_from_slices": (c_int, [c_void_p, _dp, _dp, _dp, _ip, c_int, c_int, _dp, c_int, _dp, _dp]),
+    "taco_rk4_propagate": (c_int, [c_void_p, _dp, _dp, c_size_t, c_int, _dp, c_double, c_size_t, _dp, _sp]),
+    "taco_tcl4_propagate": (c_int, [c_void_p, _dp, c_size_t, c_int, _dp, _ip, _ip, c_int, _dp, _dp, c_int,
+                                    c_double, c_size_t, c_int, _dp, _dp]),
+}
+
+_lib = None
+
+
+class TacoError(RuntimeError):
+    """Base of the engine's errors; ``code`` is the C-ABI status."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+
+
+class TacoInvalidArgument(TacoError, ValueError):
+    """std::invalid_argument in the reference."""
+
+
+class TacoOutOfRange(TacoError, IndexError):
+    """std::out_of_range in the reference."""
+
+
+class TacoOverflow(TacoError, OverflowError):
+    """std::overflow_error in the reference."""
+
+
+_ERR = {TACO_ERR_INVALID: TacoInvalidArgument, TACO_ERR_RANGE: TacoOutOfRange, TACO_ERR_OVERFLOW: TacoOverflow}
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded engine library (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"taco_amd: {LIB_PATH} is not built; run `python -m taco_amd.build` "
+                              "(the engine has no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != TACO_OK:
+        msg = lib().taco_last_error().decode()
+        raise _ERR.get(rc, TacoError)(rc, msg)
+
+
+def dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    return a.ctypes.data_as(_ip)
+
+
+def sptr(a):
+    return a.ctypes.data_as(_sp)

@@ -1,0 +1,1257 @@
+// engine.hip — MI355X (gfx950) TCL4 hot path: F/C/R -> MIKX -> GW -> L4 and
+// RK4, complex FP64, all intermediates resident in HBM.
+//
+// F/C/R (tcl4_kernels.cpp:105-175) need, per triple (i,j,k), the causal
+// convolutions dt*ifft(fft(g_i e^{-i Omega t}) .* fft(g2))[t] with the
+// reference's radix-2 recurrence-twiddle FFT of length Nfft = 2^L.  The
+// engine reproduces that FFT exactly with twiddle tables generated on the
+// host by the same recurrence, in three HBM passes per triple:
+//
+//   phase A  forward stages 1..a of the strided residue groups x[r + m 2^b]
+//            (input generated on the fly: gamma column times the phase),
+//   phase B  forward stages a+1..L of the groups {k = k1 mod 2^a}, the
+//            spectral products with the per-bucket spectra of g2 (F) and
+//            conj(g2) (C), then inverse stages 1..b of the same groups,
+//   phase C  inverse stages b+1..L of the groups {t = q mod 2^b}, keeping
+//            only t < T (the requested window),
+//
+// with a = ceil(L/2), b = L - a.  Every butterfly is the reference's
+// u +/- w*x with the same w, a plain (non-fused) complex product, so the
+// spectra equal the reference's bit for bit.  Groups are processed
+// Stockham-style in LDS, 2048 complex values per workgroup.
+//
+// The per-time stage (tcl4_mikx.cpp:46-136, tcl4_assemble.cpp:29-205) is one
+// fused kernel per time index that gathers M/I/K/X on the fly from the nf^3
+// F/C/R slices (no N^6 X tensor), forms the NAKZWAN sum, symmetrises
+// GW = T + T^H in LDS and writes the reshuffled L4 block.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "engine_internal.hpp"
+
+#pragma clang fp contract(off)
+
+#include "cr_sincos.hpp"
+
+namespace taco_engine {
+
+thread_local std::string g_last_error;
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+static std::size_t g_pad_factor = 0;
+
+#define HIP_OK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            throw Error(TACO_ERR_RUNTIME, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+
+// sin/cos(j/64) double-double table of the correctly rounded phase routine.
+__constant__ double c_crtab[4 * taco_cr::kTabN];
+
+// exp(i * arg) with correctly rounded cos/sin.
+__device__ __forceinline__ double2 cexpi(double arg) {
+    double s, c;
+    taco_cr::sincos_cr(arg, &s, &c, c_crtab);
+    return make_double2(c, s);
+}
+
+constexpr int kBlock = 256;
+constexpr int kGroupElems = 2048;  // complex values per workgroup in the FFT phases
+constexpr int kMaxBfPerThread = kGroupElems / 2 / kBlock;
+
+// Radix-2 Stockham stages on G groups of 2^lg values stored at buf[g*S + pos].
+// Local stage sp uses twiddle twp[g*gstride + (2^(sp-1) - 1) + kk] (conjugated
+// for the inverse transform).  Position mapping per stage: inputs u + 2hk and
+// u + h + 2hk, outputs u + hk and u + hk + 2^(lg-1), h = 2^(lg-sp).
+__device__ void group_stages(double2* buf, int lg, int G, int S, const double2* __restrict__ twp, long gstride,
+                             bool conj) {
+    if (lg <= 0) return;
+    const int half = 1 << (lg - 1);
+    const int nb = G * half;
+    for (int sp = 1; sp <= lg; ++sp) {
+        const int hsh = lg - sp;
+        const int hmask = (1 << hsh) - 1;
+        const int toff = (1 << (sp - 1)) - 1;
+        double2 r0[kMaxBfPerThread], r1[kMaxBfPerThread];
+        int cnt = 0;
+        for (int bf = threadIdx.x; bf < nb; bf += kBlock, ++cnt) {
+            const int grp = bf >> (lg - 1);
+            const int p = bf & (half - 1);
+            const int u = p & hmask;
+            const int kk = p >> hsh;
+            const int i0 = u + (kk << (hsh + 1));
+            const int base = grp * S;
+            double2 w = twp[grp * gstride + toff + kk];
+            if (conj) w.y = -w.y;
+            const double2 E = buf[base + i0];
+            const double2 O = buf[base + i0 + (1 << hsh)];
+            const double2 v = cmul(w, O);
+            r0[cnt] = cadd(E, v);
+            r1[cnt] = csub(E, v);
+        }
+        __syncthreads();
+        cnt = 0;
+        for (int bf = threadIdx.x; bf < nb; bf += kBlock, ++cnt) {
+            const int grp = bf >> (lg - 1);
+            const int p = bf & (half - 1);
+            const int base = grp * S;
+            buf[base + p] = r0[cnt];
+            buf[base + p + half] = r1[cnt];
+        }
+        __syncthreads();
+    }
+}
+
+struct FcrArgs {
+    const double2* gamma;   // [nf][Nt]
+    const double* omegas;   // [nf]
+    const int* mirror;      // [nf]
+    int nf;
+    long Nt;
+    double dt;
+    int L, a, b;
+    long n;
+    const double2* twStd;   // stages 1..max(a,b), standard layout
+    const double2* twB;     // phase B forward stages, per k1 group
+    const double2* twC;     // phase C inverse stages, per q group
+    long tri0;              // first triple of the chunk
+    int njobs;
+    double2* inter1;        // [njobs][2^a][2^b]
+    double2* inter2;        // [2][njobs][2^b][2^a]
+    double2* spec;          // [2 nf][2^a][2^b]   spectra of g_b (even) and conj(g_b) (odd)
+    double2* conv;          // [2][njobs][T]
+    long T;
+};
+
+__device__ __forceinline__ void decode_triple(long tri, int nf, int& i, int& j, int& k) {
+    k = static_cast<int>(tri % nf);
+    j = static_cast<int>((tri / nf) % nf);
+    i = static_cast<int>(tri / (static_cast<long>(nf) * nf));
+}
+
+// Phase A: mode 0 triples (input g_i * phase_minus), mode 1 spectra (g_b or conj g_b).
+__global__ __launch_bounds__(kBlock) void k_phase_a(FcrArgs P, int mode) {
+    __shared__ double2 buf[kGroupElems];
+    const int lg = P.a, gs = 1 << lg;
+    const int ngroups = 1 << P.b;
+    const int G = min(kGroupElems >> lg, ngroups);
+    const int bpj = ngroups / G;
+    const int job = blockIdx.x / bpj;
+    const int r0b = (blockIdx.x % bpj) * G;
+    const double2* src;
+    double omega = 0.0;
+    bool conj_in = false, phase = false;
+    if (mode == 0) {
+        int i, j, k;
+        decode_triple(P.tri0 + job, P.nf, i, j, k);
+        src = P.gamma + static_cast<long>(i) * P.Nt;
+        omega = (P.omegas[i] + P.omegas[j]) + P.omegas[k];
+        phase = true;
+    } else {
+        src = P.gamma + static_cast<long>(job >> 1) * P.Nt;
+        conj_in = (job & 1) != 0;
+    }
+    const int total = G * gs;
+    for (int e = threadIdx.x; e < total; e += kBlock) {
+        const int grp = e % G, pos = e / G;
+        const long t = static_cast<long>(r0b + grp) + (static_cast<long>(pos) << P.b);
+        double2 x = make_double2(0.0, 0.0);
+        if (t < P.Nt) {
+            x = src[t];
+            if (conj_in) x.y = -x.y;
+            if (phase) {
+                const double tk = static_cast<double>(t) * P.dt;
+                x = cmul(x, cconj(cexpi(omega * tk)));
+            }
+        }
+        buf[grp * gs + pos] = x;
+    }
+    __syncthreads();
+    group_stages(buf, lg, G, gs, P.twStd, 0, false);
+    double2* out = P.inter1 + static_cast<long>(job) * P.n;
+    for (int e = threadIdx.x; e < total; e += kBlock) {
+        const int grp = e % G, pos = e / G;
+        out[(static_cast<long>(pos) << P.b) + r0b + grp] = buf[grp * gs + pos];
+    }
+}
+
+// Phase B: forward stages a+1..L; mode 1 stores the spectrum, mode 0 forms
+// the F and C products and runs inverse stages 1..b on each.
+__global__ __launch_bounds__(kBlock) void k_phase_b(FcrArgs P, int mode) {
+    __shared__ double2 X[kGroupElems];
+    __shared__ double2 W[kGroupElems];
+    const int lg = P.b, gs = 1 << lg;
+    const int ngroups = 1 << P.a;
+    const int G = min(kGroupElems >> lg, ngroups);
+    const int bpj = ngroups / G;
+    const int job = blockIdx.x / bpj;
+    const int k1b = (blockIdx.x % bpj) * G;
+    const int total = G * gs;
+    const double2* in = P.inter1 + static_cast<long>(job) * P.n + static_cast<long>(k1b) * gs;
+    for (int e = threadIdx.x; e < total; e += kBlock) X[e] = in[e];
+    __syncthreads();
+    const long bstride = (1L << P.b) - 1;
+    group_stages(X, lg, G, gs, P.twB + static_cast<long>(k1b) * bstride, bstride, false);
+    if (mode == 1) {
+        double2* out = P.spec + static_cast<long>(job) * P.n + static_cast<long>(k1b) * gs;
+        for (int e = threadIdx.x; e < total; e += kBlock) out[e] = X[e];
+        return;
+    }
+    int i, j, k;
+    decode_triple(P.tri0 + job, P.nf, i, j, k);
+    const int bF = P.mirror[j];
+    for (int path = 0; path < 2; ++path) {
+        const double2* sp = P.spec + static_cast<long>(path == 0 ? 2 * bF : 2 * j + 1) * P.n +
+                            static_cast<long>(k1b) * gs;
+        for (int e = threadIdx.x; e < total; e += kBlock) W[e] = cmul(X[e], sp[e]);
+        __syncthreads();
+        group_stages(W, lg, G, gs, P.twStd, 0, true);
+        double2* out = P.inter2 + (static_cast<long>(path) * P.njobs + job) * P.n;
+        for (int e = threadIdx.x; e < total; e += kBlock) {
+            const int grp = e % G, pos = e / G;
+            out[(static_cast<long>(pos) << P.a) + k1b + grp] = W[grp * gs + pos];
+        }
+        __syncthreads();
+    }
+}
+
+// Phase C: inverse stages b+1..L; writes conv[path][job][t] = dt * (y[t] / n), t < T.
+__global__ __launch_bounds__(kBlock) void k_phase_c(FcrArgs P) {
+    __shared__ double2 buf[kGroupElems];
+    const int lg = P.a, gs = 1 << lg;
+    const int ngroups = 1 << P.b;
+    const int G = min(kGroupElems >> lg, ngroups);
+    const int bpj = ngroups / G;
+    const int jp = blockIdx.x / bpj;  // path * njobs + job
+    const int q0 = (blockIdx.x % bpj) * G;
+    const long qmax = q0;  // smallest t in the block is q0
+    if (qmax >= P.T) return;
+    const int total = G * gs;
+    const double2* in = P.inter2 + static_cast<long>(jp) * P.n + static_cast<long>(q0) * gs;
+    for (int e = threadIdx.x; e < total; e += kBlock) buf[e] = in[e];
+    __syncthreads();
+    const long cstride = (1L << P.a) - 1;
+    group_stages(buf, lg, G, gs, P.twC + static_cast<long>(q0) * cstride, cstride, true);
+    const double inv_n = 1.0 / static_cast<double>(P.n);
+    double2* out = P.conv + static_cast<long>(jp) * P.T;
+    for (int e = threadIdx.x; e < total; e += kBlock) {
+        const int grp = e % G, v = e / G;
+        const long t = static_cast<long>(q0 + grp) + (static_cast<long>(v) << P.b);
+        if (t < P.T) {
+            const double2 y = cscale(buf[grp * gs + v], inv_n);
+            out[t] = make_double2(P.dt * y.x, P.dt * y.y);
+        }
+    }
+}
+
+// Inclusive block scan (256 threads = 4 waves of 64) of one double.
+__device__ __forceinline__ double block_scan(double v, double* wsum, double& total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int off = 1; off < 64; off <<= 1) {
+        const double nb = __shfl_up(v, off, 64);
+        if (lane >= off) v += nb;
+    }
+    if (lane == 63) wsum[wid] = v;
+    __syncthreads();
+    double pre = 0.0;
+    for (int w = 0; w < wid; ++w) pre += wsum[w];
+    total = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+    __syncthreads();
+    return pre + v;
+}
+
+struct CombineArgs {
+    int out_mode;        // 0: time-major slices at t2pos; 1: series-major [tri][T]
+    const int* t2pos;    // [T], -1 when t is not requested (mode 0)
+    long nf3;
+    double2* Fo;
+    double2* Co;
+    double2* Ro;
+};
+
+// F/C/R assembly for t < T (tcl4_kernels.cpp:105-175): prefix integrals by
+// block scans carried across 256-sample chunks, combined with the FFT
+// convolution terms of phase C.
+__global__ __launch_bounds__(kBlock) void k_combine(FcrArgs P, CombineArgs O) {
+    __shared__ double wsum[8][4];
+    const int job = blockIdx.x;
+    const long tri = P.tri0 + job;
+    int i, j, k;
+    decode_triple(tri, P.nf, i, j, k);
+    const double omega = (P.omegas[i] + P.omegas[j]) + P.omegas[k];
+    const double2* g1 = P.gamma + static_cast<long>(i) * P.Nt;
+    const double2* g2 = P.gamma + static_cast<long>(j) * P.Nt;
+    const double2* g2m = P.gamma + static_cast<long>(P.mirror[j]) * P.Nt;
+    const double2* convF = P.conv + static_cast<long>(job) * P.T;
+    const double2* convC = P.conv + static_cast<long>(P.njobs + job) * P.T;
+    double carry[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (long t0 = 0; t0 < P.T; t0 += kBlock) {
+        const long t = t0 + threadIdx.x;
+        const bool valid = t < P.T;
+        double2 a1 = make_double2(0, 0), a2 = a1, a3 = a1, a4 = a1, pm = a1, gg1 = a1;
+        if (valid) {
+            const double tk = static_cast<double>(t) * P.dt;
+            const double2 pp = cexpi(omega * tk);
+            pm = cconj(pp);
+            gg1 = g1[t];
+            const double2 gg2 = g2[t];
+            a1 = cmul(g2m[t], pp);           // F: g2m * phase_plus
+            a2 = cmul(cconj(gg2), pp);       // C: conj(g2) * phase_plus
+            a3 = cmul(gg2, pm);              // R: g2 * phase_minus
+            a4 = cmul(cmul(gg1, gg2), pm);   // R: (g1 g2) * phase_minus
+        }
+        double v[8] = {a1.x, a1.y, a2.x, a2.y, a3.x, a3.y, a4.x, a4.y};
+        double tot[8];
+        for (int q = 0; q < 8; ++q) {
+            v[q] = block_scan(v[q], wsum[q], tot[q]) + carry[q];
+            carry[q] += tot[q];
+        }
+        if (!valid) continue;
+        long dst = -1;
+        if (O.out_mode == 0) {
+            const int pos = O.t2pos[t];
+            if (pos < 0) continue;
+            dst = static_cast<long>(pos) * O.nf3 + tri;
+        } else {
+            dst = tri * P.T + t;
+        }
+        const double dt = P.dt;
+        const double2 pF = make_double2(v[0] * dt, v[1] * dt);
+        const double2 pC = make_double2(v[2] * dt, v[3] * dt);
+        const double2 pR = make_double2(v[4] * dt, v[5] * dt);
+        const double2 pP = make_double2(v[6] * dt, v[7] * dt);
+        const double2 g1p = cmul(gg1, pm);
+        O.Fo[dst] = csub(cmul(g1p, pF), convF[t]);
+        O.Co[dst] = csub(cmul(g1p, pC), convC[t]);
+        O.Ro[dst] = csub(cmul(gg1, pR), pP);
+    }
+}
+
+__global__ void k_cr_sincos(const double* x, long n, double* s, double* c) {
+    const long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+    if (i < n) taco_cr::sincos_cr(x[i], s + i, c + i, c_crtab);
+}
+
+// ---------------------------------------------------------------------------
+// Fused MIKX + GW + L4 for one time index per workgroup.
+// ---------------------------------------------------------------------------
+struct L4Args {
+    const double2* Fs;   // [npos][nf3]
+    const double2* Cs;
+    const double2* Rs;
+    const int* p2f;      // N x N column-major
+    const double2* ops;  // num_ops x N x N
+    int N, nf, num_ops;
+    long nf3;
+    double2* L4;         // [npos][N^4]
+    double2* GW;         // optional [npos][N^4]
+};
+
+__global__ __launch_bounds__(kBlock) void k_l4(L4Args A) {
+    extern __shared__ double2 sm[];
+    const int N = A.N, N2 = N * N, N4 = N2 * N2;
+    double2* Tm = sm;                       // N^4
+    double2* opsS = sm + N4;                // num_ops * N^2
+    int* f = reinterpret_cast<int*>(opsS + A.num_ops * N2);  // N^2
+    const long pos = blockIdx.x;
+    for (int e = threadIdx.x; e < A.num_ops * N2; e += kBlock) opsS[e] = A.ops[e];
+    for (int e = threadIdx.x; e < N2; e += kBlock) f[e] = A.p2f[e];
+    __syncthreads();
+    const double2* Fp = A.Fs + pos * A.nf3;
+    const double2* Cp = A.Cs + pos * A.nf3;
+    const double2* Rp = A.Rs + pos * A.nf3;
+    const long nf = A.nf;
+    auto id3 = [nf](int x, int y, int z) { return (static_cast<long>(x) * nf + y) * nf + z; };
+    auto F2 = [f, N](int r, int c) { return f[r + c * N]; };
+    for (int idx = threadIdx.x; idx < N4; idx += kBlock) {
+        const int row_u = idx % N2, col_u = idx / N2;
+        const int n = row_u % N, i = row_u / N, m = col_u % N, j = col_u / N;
+        double2 res = make_double2(0.0, 0.0);
+        for (int iA = 0; iA < A.num_ops; ++iA) {
+            const double2* Ap = opsS + iA * N2;
+            for (int iB = 0; iB < A.num_ops; ++iB) {
+                const double2* Bp = opsS + iB * N2;
+#define AA(r, c) Ap[(r) + (c) * N]
+#define BB(r, c) Bp[(r) + (c) * N]
+                for (int a = 0; a < N; ++a) {
+                    for (int b = 0; b < N; ++b) {
+                        // M(b,i;n,a) = F[f(b,a)][f(b,i)][f(n,b)] - R[f(b,a)][f(n,a)][f(a,i)]
+                        const double2 Mv = csub(Fp[id3(F2(b, a), F2(b, i), F2(n, b))],
+                                                Rp[id3(F2(b, a), F2(n, a), F2(a, i))]);
+                        // I(a,n;i,b) = F[f(a,n)][f(b,i)][f(n,b)]
+                        const double2 Iv = Fp[id3(F2(a, n), F2(b, i), F2(n, b))];
+                        // K(i,b;n,a) = R[f(i,b)][f(n,a)][f(a,i)]
+                        const double2 Kv = Rp[id3(F2(i, b), F2(n, a), F2(a, i))];
+                        const long x1 = id3(F2(a, n), F2(j, b), F2(n, i));
+                        const long x2 = id3(F2(i, a), F2(j, b), F2(n, i));
+                        const long x3 = id3(F2(b, a), F2(j, m), F2(a, i));
+                        const long x4 = id3(F2(i, b), F2(j, m), F2(a, i));
+                        const double2 X1 = cadd(Cp[x1], Rp[x1]);
+                        const double2 X2 = cadd(Cp[x2], Rp[x2]);
+                        const double2 X3 = cadd(Cp[x3], Rp[x3]);
+                        const double2 X4 = cadd(Cp[x4], Rp[x4]);
+                        const double2 t1 = cmul(cmul(cmul(cmul(BB(n, a), AA(a, b)), BB(b, i)), AA(j, m)), Mv);
+                        const double2 t2 = cmul(cmul(cmul(cmul(AA(n, a), BB(a, b)), BB(b, i)), AA(j, m)), Iv);
+                        const double2 t3 = cmul(cmul(cmul(cmul(BB(n, a), BB(a, b)), AA(b, i)), AA(j, m)), Kv);
+                        const double2 t4 = cmul(cmul(cmul(cmul(AA(n, a), BB(a, i)), BB(j, b)), AA(b, m)), X1);
+                        const double2 t5 = cmul(cmul(cmul(cmul(BB(n, a), AA(a, i)), BB(j, b)), AA(b, m)), X2);
+                        const double2 t6 = cmul(cmul(cmul(cmul(AA(n, a), AA(a, b)), BB(b, i)), BB(j, m)), X3);
+                        const double2 t7 = cmul(cmul(cmul(cmul(AA(n, a), BB(a, b)), AA(b, i)), BB(j, m)), X4);
+                        double2 s = csub(t1, t2);
+                        s = cadd(s, t3);
+                        s = cadd(s, t4);
+                        s = csub(s, t5);
+                        s = csub(s, t6);
+                        s = cadd(s, t7);
+                        res = csub(res, s);
+                    }
+                }
+                if (j == m) {
+                    for (int a = 0; a < N; ++a)
+                        for (int b = 0; b < N; ++b)
+                            for (int c = 0; c < N; ++c) {
+                                // M(c,i;a,b) = F[f(c,b)][f(c,i)][f(a,c)] - R[f(c,b)][f(a,b)][f(b,i)]
+                                const double2 Mv = csub(Fp[id3(F2(c, b), F2(c, i), F2(a, c))],
+                                                        Rp[id3(F2(c, b), F2(a, b), F2(b, i))]);
+                                // K(i,c;a,b) = R[f(i,c)][f(a,b)][f(b,i)]
+                                const double2 Kv = Rp[id3(F2(i, c), F2(a, b), F2(b, i))];
+                                // I(b,a;i,c) = F[f(b,a)][f(c,i)][f(a,c)]
+                                const double2 Iv = Fp[id3(F2(b, a), F2(c, i), F2(a, c))];
+                                const double2 add1 = cmul(cmul(cmul(cmul(AA(n, a), BB(a, b)), AA(b, c)), BB(c, i)), Mv);
+                                const double2 add2 = cmul(cmul(cmul(cmul(AA(n, a), BB(a, b)), BB(b, c)), AA(c, i)), Kv);
+                                const double2 nA = make_double2(-AA(n, a).x, -AA(n, a).y);
+                                const double2 add3 = cmul(cmul(cmul(cmul(nA, AA(a, b)), BB(b, c)), BB(c, i)), Iv);
+                                res = cadd(res, cadd(cadd(add1, add2), add3));
+                            }
+                }
+#undef AA
+#undef BB
+            }
+        }
+        Tm[row_u + col_u * N2] = res;
+    }
+    __syncthreads();
+    double2* L = A.L4 + pos * N4;
+    for (int idx = threadIdx.x; idx < N4; idx += kBlock) {
+        const int r = idx % N2, c = idx / N2;
+        const double2 g = cadd(Tm[r + c * N2], cconj(Tm[c + r * N2]));
+        if (A.GW) A.GW[pos * N4 + idx] = g;
+        const int n = r % N, i = r / N, m = c % N, j = c / N;
+        L[(n + m * N) + (i + j * N) * N2] = g;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// RK4 on vec(rho) with prebuilt generators (rk4_dense.cpp:133-160, 246-320),
+// one workgroup, D <= 1024.  Matvecs sum over j in order per row.
+// ---------------------------------------------------------------------------
+__device__ void matvec_rows(const double2* __restrict__ L, const double2* x, double2* y, int D) {
+    for (int r = threadIdx.x; r < D; r += blockDim.x) {
+        double2 s = make_double2(0.0, 0.0);
+        for (int c = 0; c < D; ++c) s = cadd(s, cmul(L[r + static_cast<long>(c) * D], x[c]));
+        y[r] = s;
+    }
+}
+
+// mode 0: Lh given; mode 1: Lh = 0.5*(L0 + L1) formed on the fly per element.
+__device__ void matvec_half(const double2* __restrict__ L0, const double2* __restrict__ L1, const double2* x,
+                            double2* y, int D) {
+    for (int r = threadIdx.x; r < D; r += blockDim.x) {
+        double2 s = make_double2(0.0, 0.0);
+        for (int c = 0; c < D; ++c) {
+            const long e = r + static_cast<long>(c) * D;
+            const double2 h = cscale(cadd(L0[e], L1[e]), 0.5);
+            s = cadd(s, cmul(h, x[c]));
+        }
+        y[r] = s;
+    }
+}
+
+__global__ void k_rk4(const double2* Ls, const double2* Lh, long steps, int D, double2* r, double dt, long every,
+                      double2* samples, long* nsamples) {
+    extern __shared__ double2 sh[];
+    double2* x = sh;
+    double2* k1 = x + D;
+    double2* k2 = k1 + D;
+    double2* k3 = k2 + D;
+    double2* k4 = k3 + D;
+    double2* tmp = k4 + D;
+    for (int e = threadIdx.x; e < D; e += blockDim.x) x[e] = r[e];
+    __syncthreads();
+    const long D2 = static_cast<long>(D) * D;
+    long ns = 0;
+    const double hdt = 0.5 * dt;
+    const double s6 = dt / 6.0;
+    for (long st = 0; st < steps; ++st) {
+        if (samples && st % every == 0) {
+            for (int e = threadIdx.x; e < D; e += blockDim.x) samples[ns * D + e] = x[e];
+            ++ns;
+        }
+        const double2* L0 = Ls + st * D2;
+        const double2* L1 = Ls + (st + 1) * D2;
+        matvec_rows(L0, x, k1, D);
+        __syncthreads();
+        for (int e = threadIdx.x; e < D; e += blockDim.x) tmp[e] = cadd(x[e], cscale(k1[e], hdt));
+        __syncthreads();
+        if (Lh) matvec_rows(Lh + st * D2, tmp, k2, D); else matvec_half(L0, L1, tmp, k2, D);
+        __syncthreads();
+        for (int e = threadIdx.x; e < D; e += blockDim.x) tmp[e] = cadd(x[e], cscale(k2[e], hdt));
+        __syncthreads();
+        if (Lh) matvec_rows(Lh + st * D2, tmp, k3, D); else matvec_half(L0, L1, tmp, k3, D);
+        __syncthreads();
+        for (int e = threadIdx.x; e < D; e += blockDim.x) tmp[e] = cadd(x[e], cscale(k3[e], dt));
+        __syncthreads();
+        matvec_rows(L1, tmp, k4, D);
+        __syncthreads();
+        for (int e = threadIdx.x; e < D; e += blockDim.x) {
+            const double2 acc = cadd(cadd(cadd(k1[e], cscale(k2[e], 2.0)), cscale(k3[e], 2.0)), k4[e]);
+            x[e] = cadd(x[e], cscale(acc, s6));
+        }
+        __syncthreads();
+    }
+    if (samples) {
+        for (int e = threadIdx.x; e < D; e += blockDim.x) samples[ns * D + e] = x[e];
+        ++ns;
+    }
+    for (int e = threadIdx.x; e < D; e += blockDim.x) r[e] = x[e];
+    if (threadIdx.x == 0 && nsamples) *nsamples = ns;
+}
+
+// TCL2 generator at every t (generator.cpp:81-177 with the drivers' single-
+// or diagonal-channel kernels) added onto L_all[t] (which holds L4(t) or 0).
+// One workgroup per time; each thread builds whole columns.
+struct Tcl2Args {
+    const double2* gamma;  // [nf][Nt]
+    long Nt;
+    const double* eps;
+    const double2* ops;    // A_eig
+    const int* p2f;
+    int N, nf, num_ops;
+    double2* L;            // [T][D*D], accumulated into
+    int order;             // 0: unitary only; 2/4: full TCL2
+};
+
+__global__ __launch_bounds__(kBlock) void k_tcl2(Tcl2Args A) {
+    extern __shared__ double2 sh[];
+    const int N = A.N, N2 = N * N, D = N2;
+    const long t = blockIdx.x;
+    double2* H = sh;                 // N x N effective Hamiltonian
+    double2* diss = H + N2;          // per (alpha) jump sums: kept as dense D x D in global instead
+    (void)diss;
+    double2* L = A.L + t * static_cast<long>(D) * D;
+    // H_eff = diag(eps) + hermitize(sum_b sum_a S_b A_b^dag A_b) with S_b = Im gamma(t,b).
+    for (int e = threadIdx.x; e < N2; e += kBlock) {
+        const int r = e % N, c = e / N;
+        double2 h = make_double2(0.0, 0.0), hc = make_double2(0.0, 0.0);
+        if (A.order != 0) {
+            for (int al = 0; al < A.num_ops; ++al) {
+                const double2* Ap = A.ops + al * N2;
+                for (int b = 0; b < A.nf; ++b) {
+                    const double S = A.gamma[static_cast<long>(b) * A.Nt + t].y;
+                    if (S == 0.0) continue;
+                    // (A_b^dag A_b)(r,c) = sum_q conj(A_b(q,r)) A_b(q,c), A_b = A masked to bucket b
+                    double2 s1 = make_double2(0, 0), s2 = make_double2(0, 0);
+                    for (int q = 0; q < N; ++q) {
+                        if (A.p2f[q + r * N] == b && A.p2f[q + c * N] == b)
+                            s1 = cadd(s1, cmul(cconj(Ap[q + r * N]), Ap[q + c * N]));
+                        if (A.p2f[q + c * N] == b && A.p2f[q + r * N] == b)
+                            s2 = cadd(s2, cmul(cconj(Ap[q + c * N]), Ap[q + r * N]));
+                    }
+                    h = cadd(h, cscale(s1, S));
+                    hc = cadd(hc, cscale(s2, S));
+                }
+            }
+        }
+        double2 v = cscale(cadd(h, cconj(hc)), 0.5);
+        if (r == c) v.x += A.eps[r];
+        H[e] = v;
+    }
+    __syncthreads();
+    // L(row=(r1,r2), col=(c1,c2)) with row = r1*N + r2 (kron(I,H) - kron(H^T,I) convention)
+    for (long e = threadIdx.x; e < static_cast<long>(D) * D; e += kBlock) {
+        const int row = static_cast<int>(e % D), col = static_cast<int>(e / D);
+        const int r1 = row / N, r2 = row % N, c1 = col / N, c2 = col % N;
+        double2 left = (r1 == c1) ? H[r2 + c2 * N] : make_double2(0, 0);
+        double2 right = (r2 == c2) ? H[c1 + r1 * N] : make_double2(0, 0);
+        const double2 d = csub(left, right);
+        double2 val = make_double2(d.y, -d.x);  // -i * d
+        if (A.order != 0) {
+            for (int al = 0; al < A.num_ops; ++al) {
+                const double2* Ap = A.ops + al * N2;
+                for (int b = 0; b < A.nf; ++b) {
+                    const double gam = 2.0 * A.gamma[static_cast<long>(b) * A.Nt + t].x;
+                    if (gam == 0.0) continue;
+                    const bool in1 = A.p2f[r1 + c1 * N] == b, in2 = A.p2f[r2 + c2 * N] == b;
+                    double2 jump = make_double2(0, 0);
+                    if (in1 && in2) jump = cmul(cconj(Ap[r1 + c1 * N]), Ap[r2 + c2 * N]);
+                    double2 lt = make_double2(0, 0), rt = make_double2(0, 0);
+                    if (r1 == c1)
+                        for (int q = 0; q < N; ++q)
+                            if (A.p2f[q + r2 * N] == b && A.p2f[q + c2 * N] == b)
+                                lt = cadd(lt, cmul(cconj(Ap[q + r2 * N]), Ap[q + c2 * N]));
+                    if (r2 == c2)
+                        for (int q = 0; q < N; ++q)
+                            if (A.p2f[q + c1 * N] == b && A.p2f[q + r1 * N] == b)
+                                rt = cadd(rt, cmul(cconj(Ap[q + c1 * N]), Ap[q + r1 * N]));
+                    const double2 term = csub(csub(jump, cscale(lt, 0.5)), cscale(rt, 0.5));
+                    val = cadd(val, cscale(term, gam));
+                }
+            }
+        }
+        L[e] = cadd(L[e], val);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Context and device workspace
+// ---------------------------------------------------------------------------
+struct DevBuf {
+    void* p = nullptr;
+    std::size_t bytes = 0;
+    void ensure(std::size_t need) {
+        if (need <= bytes) return;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        if (need == 0) return;
+        HIP_OK(hipMalloc(&p, need));
+        bytes = need;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+struct Twiddles {
+    DevBuf std_, b_, c_;
+    int L = -1;
+};
+
+}  // namespace taco_engine
+
+struct taco_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::map<int, std::unique_ptr<taco_engine::Twiddles>> tw;
+    taco_engine::DevBuf gamma, omegas, mirror, p2f, ops, spec, inter1, inter2, conv, slices, t2pos, l4, gw, misc,
+        rk, eps;
+};
+
+namespace taco_engine {
+
+struct DeviceGuard {
+    int prev = 0;
+    explicit DeviceGuard(int dev) {
+        HIP_OK(hipGetDevice(&prev));
+        if (prev != dev) HIP_OK(hipSetDevice(dev));
+    }
+    ~DeviceGuard() { (void)hipSetDevice(prev); }
+};
+
+static const Twiddles& twiddles_for(taco_ctx* c, int L) {
+    auto it = c->tw.find(L);
+    if (it != c->tw.end()) return *it->second;
+    auto t = std::make_unique<Twiddles>();
+    t->L = L;
+    std::vector<double> tab;
+    make_twiddles(L, tab);  // stage-major standard table, 2^L - 1 entries
+    const int a = (L + 1) / 2, b = L - a;
+    const std::size_t nstd = (std::size_t{1} << std::max(a, b)) - 1;
+    // Phase B forward stages s = a+1..L for group k1 < 2^a: T_s[k1 + v 2^a], v < 2^(s-1-a).
+    const std::size_t bstride = (std::size_t{1} << b) - 1;
+    std::vector<double> tb(2 * (std::size_t{1} << a) * bstride + 2);
+    for (std::size_t k1 = 0; k1 < (std::size_t{1} << a); ++k1)
+        for (int s = a + 1; s <= L; ++s) {
+            const std::size_t cnt = std::size_t{1} << (s - 1 - a);
+            for (std::size_t v = 0; v < cnt; ++v) {
+                const std::size_t src = ((std::size_t{1} << (s - 1)) - 1) + k1 + (v << a);
+                const std::size_t dst = k1 * bstride + (cnt - 1) + v;
+                tb[2 * dst] = tab[2 * src];
+                tb[2 * dst + 1] = tab[2 * src + 1];
+            }
+        }
+    // Phase C inverse stages s = b+1..L for group q < 2^b: T_s[q + v 2^b], v < 2^(s-1-b).
+    const std::size_t cstride = (std::size_t{1} << a) - 1;
+    std::vector<double> tc(2 * (std::size_t{1} << b) * cstride + 2);
+    for (std::size_t q = 0; q < (std::size_t{1} << b); ++q)
+        for (int s = b + 1; s <= L; ++s) {
+            const std::size_t cnt = std::size_t{1} << (s - 1 - b);
+            for (std::size_t v = 0; v < cnt; ++v) {
+                const std::size_t src = ((std::size_t{1} << (s - 1)) - 1) + q + (v << b);
+                const std::size_t dst = q * cstride + (cnt - 1) + v;
+                tc[2 * dst] = tab[2 * src];
+                tc[2 * dst + 1] = tab[2 * src + 1];
+            }
+        }
+    t->std_.ensure(std::max<std::size_t>(nstd, 1) * 16);
+    if (nstd) HIP_OK(hipMemcpy(t->std_.p, tab.data(), nstd * 16, hipMemcpyHostToDevice));
+    t->b_.ensure(tb.size() * 8);
+    HIP_OK(hipMemcpy(t->b_.p, tb.data(), tb.size() * 8, hipMemcpyHostToDevice));
+    t->c_.ensure(tc.size() * 8);
+    HIP_OK(hipMemcpy(t->c_.p, tc.data(), tc.size() * 8, hipMemcpyHostToDevice));
+    auto& ref = *t;
+    c->tw.emplace(L, std::move(t));
+    return ref;
+}
+
+static std::size_t chunk_budget_bytes() {
+    static std::size_t v = [] {
+        const char* s = std::getenv("TACO_FCR_CHUNK_BYTES");
+        return s ? static_cast<std::size_t>(std::strtoull(s, nullptr, 10)) : (std::size_t{3} << 30);
+    }();
+    return v;
+}
+
+// Device-side F/C/R for all nf^3 triples over t < T; outputs per CombineArgs.
+struct FcrPlan {
+    const double2* gamma;
+    std::size_t Nt;
+    int nf;
+    const double* omegas;
+    const int* mirror;
+    double dt;
+    std::size_t T;
+};
+
+static void run_fcr(taco_ctx* c, const FcrPlan& pl, const CombineArgs& out) {
+    const std::size_t n = fcr_nfft(pl.Nt, g_pad_factor);
+    int L = 0;
+    while ((std::size_t{1} << L) < n) ++L;
+    if (L > 22) throw Error(TACO_ERR_INVALID, "FFT length above 2^22 (Nt above 2^21) is not supported");
+    const int a = (L + 1) / 2, b = L - a;
+    const Twiddles& tw = twiddles_for(c, L);
+    const long nf3 = static_cast<long>(pl.nf) * pl.nf * pl.nf;
+    const std::size_t per_job = 3 * n * 16;
+    long chunk = static_cast<long>(std::max<std::size_t>(1, chunk_budget_bytes() / per_job));
+    chunk = std::min(chunk, nf3);
+    const long spec_jobs = 2L * pl.nf;
+    c->spec.ensure(static_cast<std::size_t>(spec_jobs) * n * 16);
+    c->inter1.ensure(static_cast<std::size_t>(std::max(chunk, spec_jobs)) * n * 16);
+    c->inter2.ensure(static_cast<std::size_t>(2 * chunk) * n * 16);
+    c->conv.ensure(static_cast<std::size_t>(2 * chunk) * std::max<std::size_t>(pl.T, 1) * 16);
+    FcrArgs P{};
+    P.gamma = pl.gamma;
+    P.omegas = pl.omegas;
+    P.mirror = pl.mirror;
+    P.nf = pl.nf;
+    P.Nt = static_cast<long>(pl.Nt);
+    P.dt = pl.dt;
+    P.L = L;
+    P.a = a;
+    P.b = b;
+    P.n = static_cast<long>(n);
+    P.twStd = tw.std_.as<double2>();
+    P.twB = tw.b_.as<double2>();
+    P.twC = tw.c_.as<double2>();
+    P.inter1 = c->inter1.as<double2>();
+    P.inter2 = c->inter2.as<double2>();
+    P.spec = c->spec.as<double2>();
+    P.conv = c->conv.as<double2>();
+    P.T = static_cast<long>(pl.T);
+    auto blocks = [](int lg_group, int lg_ngroups) {
+        const int ng = 1 << lg_ngroups;
+        const int G = std::min(kGroupElems >> lg_group, ng);
+        return ng / G;
+    };
+    const int bpjA = blocks(a, b), bpjB = blocks(b, a), bpjC = blocks(a, b);
+    // Spectra of every Gamma column and its conjugate.
+    P.tri0 = 0;
+    P.njobs = static_cast<int>(spec_jobs);
+    hipLaunchKernelGGL(k_phase_a, dim3(static_cast<unsigned>(spec_jobs * bpjA)), dim3(kBlock), 0, c->stream, P, 1);
+    hipLaunchKernelGGL(k_phase_b, dim3(static_cast<unsigned>(spec_jobs * bpjB)), dim3(kBlock), 0, c->stream, P, 1);
+    HIP_OK(hipGetLastError());
+    for (long tri0 = 0; tri0 < nf3; tri0 += chunk) {
+        const long nj = std::min(chunk, nf3 - tri0);
+        P.tri0 = tri0;
+        P.njobs = static_cast<int>(nj);
+        hipLaunchKernelGGL(k_phase_a, dim3(static_cast<unsigned>(nj * bpjA)), dim3(kBlock), 0, c->stream, P, 0);
+        hipLaunchKernelGGL(k_phase_b, dim3(static_cast<unsigned>(nj * bpjB)), dim3(kBlock), 0, c->stream, P, 0);
+        hipLaunchKernelGGL(k_phase_c, dim3(static_cast<unsigned>(2 * nj * bpjC)), dim3(kBlock), 0, c->stream, P);
+        hipLaunchKernelGGL(k_combine, dim3(static_cast<unsigned>(nj)), dim3(kBlock), 0, c->stream, P, out);
+        HIP_OK(hipGetLastError());
+    }
+}
+
+static std::size_t l4_lds_bytes(int N, int num_ops) {
+    const std::size_t N2 = static_cast<std::size_t>(N) * N;
+    return N2 * N2 * 16 + static_cast<std::size_t>(num_ops) * N2 * 16 + N2 * 4;
+}
+
+static void launch_l4(taco_ctx* c, const L4Args& A, long npos) {
+    const std::size_t lds = l4_lds_bytes(A.N, A.num_ops);
+    if (lds > 160 * 1024)
+        throw Error(TACO_ERR_INVALID, "fused L4 kernel supports N^4 * 16 B + ops in 160 KiB of LDS (N <= 8)");
+    if (npos <= 0) return;
+    if (lds > 64 * 1024)
+        HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_l4), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   static_cast<int>(lds)));
+    hipLaunchKernelGGL(k_l4, dim3(static_cast<unsigned>(npos)), dim3(kBlock), lds, c->stream, A);
+    HIP_OK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Validation shared by the builders (tcl4_fused_cuda.cu:820-851, 1107-1216).
+// ---------------------------------------------------------------------------
+static void validate_system(std::size_t Nt, int nf, const int* mirror, const int* p2f, int N, int num_ops,
+                            double dt, int method) {
+    if (method != TACO_FCR_CONVOLUTION)
+        throw Error(TACO_ERR_INVALID, "build_TCL4_generator_hip_fused: only FCRMethod::Convolution is supported");
+    if (!(dt > 0.0)) throw Error(TACO_ERR_INVALID, "dt must be > 0");
+    if (nf <= 0 || Nt == 0) throw Error(TACO_ERR_INVALID, "empty gamma_series");
+    if (N <= 0) throw Error(TACO_ERR_INVALID, "map.N must be > 0");
+    if (num_ops <= 0) throw Error(TACO_ERR_INVALID, "coupling_ops must be non-empty");
+    if (mirror)
+        for (int b = 0; b < nf; ++b)
+            if (mirror[b] < 0 || mirror[b] >= nf) throw Error(TACO_ERR_INVALID, "mirror index out of range");
+    if (p2f)
+        for (int e = 0; e < N * N; ++e) {
+            if (p2f[e] < 0)
+                throw Error(TACO_ERR_RUNTIME, "map.pair_to_freq contains -1 (missing frequency buckets)");
+            if (p2f[e] >= nf) throw Error(TACO_ERR_INVALID, "pair_to_freq entry >= nf");
+        }
+}
+
+struct TimePlan {
+    std::vector<std::size_t> uniq;   // sorted unique times
+    std::vector<int> t2pos;          // [T]
+    std::vector<std::size_t> req2u;  // requested -> uniq position
+    std::size_t T = 0;
+};
+
+static TimePlan plan_times(const std::size_t* tidx, std::size_t ntau, std::size_t Nt) {
+    TimePlan tp;
+    std::vector<std::size_t> req;
+    if (ntau == 0) {
+        req.resize(Nt);
+        std::iota(req.begin(), req.end(), std::size_t{0});
+    } else {
+        req.assign(tidx, tidx + ntau);
+    }
+    for (std::size_t t : req)
+        if (t >= Nt) throw Error(TACO_ERR_RANGE, "time_index out of range");
+    tp.uniq = req;
+    std::sort(tp.uniq.begin(), tp.uniq.end());
+    tp.uniq.erase(std::unique(tp.uniq.begin(), tp.uniq.end()), tp.uniq.end());
+    tp.T = tp.uniq.empty() ? 0 : tp.uniq.back() + 1;
+    tp.t2pos.assign(tp.T, -1);
+    for (std::size_t q = 0; q < tp.uniq.size(); ++q) tp.t2pos[tp.uniq[q]] = static_cast<int>(q);
+    tp.req2u.resize(req.size());
+    for (std::size_t q = 0; q < req.size(); ++q) tp.req2u[q] = static_cast<std::size_t>(tp.t2pos[req[q]]);
+    return tp;
+}
+
+// Core of the batch builders on device-resident inputs.  Writes L4 for the
+// unique times into c->l4 and returns the plan; fcr_ms optional.
+static TimePlan l4_core(taco_ctx* c, const double2* d_gamma, std::size_t Nt, int nf, const double* d_omegas,
+                        const int* d_mirror, const int* d_p2f, int N, const double2* d_ops, int num_ops, double dt,
+                        const std::size_t* tidx, std::size_t ntau, double* fcr_ms) {
+    TimePlan tp = plan_times(tidx, ntau, Nt);
+    const long npos = static_cast<long>(tp.uniq.size());
+    if (npos == 0) return tp;
+    const std::size_t nf3 = static_cast<std::size_t>(nf) * nf * nf;
+    const std::size_t N4 = static_cast<std::size_t>(N) * N * N * N;
+    if (nf3 > 0 && static_cast<std::size_t>(npos) > (~std::size_t{0} / 48) / nf3)
+        throw Error(TACO_ERR_OVERFLOW, "FCR buffer too large");
+    c->slices.ensure(3 * nf3 * static_cast<std::size_t>(npos) * 16);
+    c->t2pos.ensure(tp.T * sizeof(int));
+    HIP_OK(hipMemcpyAsync(c->t2pos.p, tp.t2pos.data(), tp.T * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    c->l4.ensure(N4 * static_cast<std::size_t>(npos) * 16);
+    CombineArgs out{};
+    out.out_mode = 0;
+    out.t2pos = c->t2pos.as<int>();
+    out.nf3 = static_cast<long>(nf3);
+    double2* sl = c->slices.as<double2>();
+    out.Fo = sl;
+    out.Co = sl + nf3 * npos;
+    out.Ro = sl + 2 * nf3 * npos;
+    if (fcr_ms) HIP_OK(hipEventRecord(c->ev0, c->stream));
+    FcrPlan pl{d_gamma, Nt, nf, d_omegas, d_mirror, dt, tp.T};
+    run_fcr(c, pl, out);
+    if (fcr_ms) HIP_OK(hipEventRecord(c->ev1, c->stream));
+    L4Args A{};
+    A.Fs = out.Fo;
+    A.Cs = out.Co;
+    A.Rs = out.Ro;
+    A.p2f = d_p2f;
+    A.ops = d_ops;
+    A.N = N;
+    A.nf = nf;
+    A.num_ops = num_ops;
+    A.nf3 = static_cast<long>(nf3);
+    A.L4 = c->l4.as<double2>();
+    A.GW = nullptr;
+    launch_l4(c, A, npos);
+    if (fcr_ms) {
+        HIP_OK(hipEventSynchronize(c->ev1));
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        *fcr_ms = ms;
+    }
+    return tp;
+}
+
+template <class T>
+static T* upload(taco_ctx* c, DevBuf& buf, const T* host, std::size_t count) {
+    buf.ensure(std::max<std::size_t>(count, 1) * sizeof(T));
+    if (count) HIP_OK(hipMemcpyAsync(buf.p, host, count * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    return buf.as<T>();
+}
+
+static void check_ctx(taco_ctx* c) {
+    if (!c) throw Error(TACO_ERR_INVALID, "null taco_ctx");
+}
+
+}  // namespace taco_engine
+
+using namespace taco_engine;
+
+extern "C" {
+
+const char* taco_last_error(void) { return g_last_error.c_str(); }
+const char* taco_version(void) { return "taco-mi355x 0.1.0 (gfx950)"; }
+
+void taco_set_fcr_fft_pad_factor(size_t factor) { g_pad_factor = factor; }
+size_t taco_get_fcr_fft_pad_factor(void) { return g_pad_factor; }
+
+int taco_ctx_create(int device, taco_ctx** out) {
+    return guarded([&] {
+        if (!out) throw Error(TACO_ERR_INVALID, "null output pointer");
+        int count = 0;
+        HIP_OK(hipGetDeviceCount(&count));
+        if (device < 0 || device >= count) throw Error(TACO_ERR_INVALID, "gpu_id out of range");
+        DeviceGuard g(device);
+        auto c = std::make_unique<taco_ctx>();
+        c->device = device;
+        HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(c_crtab), cr_table(), sizeof(double) * 4 * taco_cr::kTabN, 0,
+                                 hipMemcpyHostToDevice));
+        HIP_OK(hipEventCreate(&c->ev0));
+        HIP_OK(hipEventCreate(&c->ev1));
+        *out = c.release();
+    });
+}
+
+int taco_ctx_destroy(taco_ctx* ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        {
+            DeviceGuard g(ctx->device);
+            (void)hipStreamSynchronize(ctx->stream);
+            (void)hipEventDestroy(ctx->ev0);
+            (void)hipEventDestroy(ctx->ev1);
+            (void)hipStreamDestroy(ctx->stream);
+            ctx->tw.clear();
+        }
+        DeviceGuard g2(ctx->device);
+        delete ctx;
+    });
+}
+
+void* taco_ctx_stream(taco_ctx* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
+
+int taco_cr_sincos(taco_ctx* ctx, const double* x, size_t n, double* s, double* c) {
+    return guarded([&] {
+        if (!ctx) {
+            cr_sincos_host(x, n, s, c);
+            return;
+        }
+        if (n == 0) return;
+        DeviceGuard g(ctx->device);
+        ctx->misc.ensure(3 * n * sizeof(double));
+        double* dx = ctx->misc.as<double>();
+        HIP_OK(hipMemcpyAsync(dx, x, n * 8, hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(k_cr_sincos, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, ctx->stream, dx,
+                           static_cast<long>(n), dx + n, dx + 2 * n);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(s, dx + n, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipMemcpyAsync(c, dx + 2 * n, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int taco_system_build(const double* H, int N, const double* A_lab, int num_ops, double freq_tol, double* eps,
+                      double* U, double* A_eig, int* nf, double* omegas, int* mirror, int* pair_to_freq,
+                      int* zero_index) {
+    return guarded([&] {
+        system_build(H, N, A_lab, num_ops, freq_tol, eps, U, A_eig, nf, omegas, mirror, pair_to_freq,
+                     zero_index);
+    });
+}
+
+int taco_bcf_fft(size_t N, double dt, int kind, double p0, double p1, double beta, double* C_out) {
+    return guarded([&] { bcf_fft(N, dt, kind, p0, p1, beta, C_out); });
+}
+
+int taco_gamma_trapz(const double* C, size_t Nc, double dt, const double* omegas, int nf, double* gamma_out) {
+    return guarded([&] { gamma_trapz(C, Nc, dt, omegas, nf, gamma_out); });
+}
+
+int taco_triple_kernels(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, const double* omegas,
+                        const int* mirror, double dt, int method, size_t keep, double* F, double* C, double* R) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (method != TACO_FCR_CONVOLUTION)
+            throw Error(TACO_ERR_INVALID, "compute_triple_kernels_hip: only FCRMethod::Convolution is supported");
+        if (!(dt > 0.0)) throw Error(TACO_ERR_INVALID, "dt must be > 0");
+        if (nf <= 0 || Nt == 0) throw Error(TACO_ERR_INVALID, "empty gamma_series");
+        for (int b = 0; b < nf; ++b)
+            if (mirror[b] < 0 || mirror[b] >= nf) throw Error(TACO_ERR_INVALID, "mirror index out of range");
+        if (keep == 0) keep = Nt;
+        if (keep > Nt) throw Error(TACO_ERR_RANGE, "keep exceeds gamma_series rows");
+        DeviceGuard g(ctx->device);
+        const std::size_t nf3 = static_cast<std::size_t>(nf) * nf * nf;
+        const double2* dg = upload(ctx, ctx->gamma, reinterpret_cast<const double2*>(gamma), Nt * nf);
+        const double* dom = upload(ctx, ctx->omegas, omegas, static_cast<std::size_t>(nf));
+        const int* dmi = upload(ctx, ctx->mirror, mirror, static_cast<std::size_t>(nf));
+        ctx->slices.ensure(3 * nf3 * keep * 16);
+        CombineArgs out{};
+        out.out_mode = 1;
+        out.t2pos = nullptr;
+        out.nf3 = static_cast<long>(nf3);
+        double2* sl = ctx->slices.as<double2>();
+        out.Fo = sl;
+        out.Co = sl + nf3 * keep;
+        out.Ro = sl + 2 * nf3 * keep;
+        FcrPlan pl{dg, Nt, nf, dom, dmi, dt, keep};
+        run_fcr(ctx, pl, out);
+        HIP_OK(hipMemcpyAsync(F, out.Fo, nf3 * keep * 16, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipMemcpyAsync(C, out.Co, nf3 * keep * 16, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipMemcpyAsync(R, out.Ro, nf3 * keep * 16, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int taco_l4_batch(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, const double* omegas, const int* mirror,
+                  const int* pair_to_freq, int N, const double* ops, int num_ops, double dt, int method,
+                  const size_t* tidx, size_t ntau, double* L4_out, double* fcr_ms) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (fcr_ms) *fcr_ms = 0.0;
+        validate_system(Nt, nf, mirror, pair_to_freq, N, num_ops, dt, method);
+        DeviceGuard g(ctx->device);
+        const std::size_t N2 = static_cast<std::size_t>(N) * N, N4 = N2 * N2;
+        const double2* dg = upload(ctx, ctx->gamma, reinterpret_cast<const double2*>(gamma), Nt * nf);
+        const double* dom = upload(ctx, ctx->omegas, omegas, static_cast<std::size_t>(nf));
+        const int* dmi = upload(ctx, ctx->mirror, mirror, static_cast<std::size_t>(nf));
+        const int* dpf = upload(ctx, ctx->p2f, pair_to_freq, N2);
+        const double2* dops = upload(ctx, ctx->ops, reinterpret_cast<const double2*>(ops), N2 * num_ops);
+        TimePlan tp = l4_core(ctx, dg, Nt, nf, dom, dmi, dpf, N, dops, num_ops, dt, tidx, ntau, fcr_ms);
+        if (tp.uniq.empty()) return;
+        std::vector<double> host(N4 * tp.uniq.size() * 2);
+        HIP_OK(hipMemcpyAsync(host.data(), ctx->l4.p, host.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        for (std::size_t q = 0; q < tp.req2u.size(); ++q)
+            std::memcpy(L4_out + q * N4 * 2, host.data() + tp.req2u[q] * N4 * 2, N4 * 16);
+    });
+}
+
+int taco_l4_batch_dev(taco_ctx* ctx, const void* d_gamma, size_t Nt, int nf, const void* d_omegas,
+                      const void* d_mirror, const void* d_pair_to_freq, int N, const void* d_ops, int num_ops,
+                      double dt, int method, const size_t* tidx, size_t ntau, void* d_L4_out, double* fcr_ms) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (fcr_ms) *fcr_ms = 0.0;
+        validate_system(Nt, nf, nullptr, nullptr, N, num_ops, dt, method);
+        DeviceGuard g(ctx->device);
+        const std::size_t N2 = static_cast<std::size_t>(N) * N, N4 = N2 * N2;
+        {
+            // the kernels index with these tables: validate them on the host
+            std::vector<int> hm(static_cast<std::size_t>(nf)), hp(N2);
+            HIP_OK(hipMemcpy(hm.data(), d_mirror, hm.size() * sizeof(int), hipMemcpyDeviceToHost));
+            HIP_OK(hipMemcpy(hp.data(), d_pair_to_freq, hp.size() * sizeof(int), hipMemcpyDeviceToHost));
+            validate_system(Nt, nf, hm.data(), hp.data(), N, num_ops, dt, method);
+        }
+        TimePlan tp = l4_core(ctx, static_cast<const double2*>(d_gamma), Nt, nf, static_cast<const double*>(d_omegas),
+                              static_cast<const int*>(d_mirror), static_cast<const int*>(d_pair_to_freq), N,
+                              static_cast<const double2*>(d_ops), num_ops, dt, tidx, ntau, fcr_ms);
+        if (tp.uniq.empty()) return;
+        double2* out = static_cast<double2*>(d_L4_out);
+        const bool identity = tp.req2u.size() == tp.uniq.size() &&
+                              std::is_sorted(tp.req2u.begin(), tp.req2u.end());
+        if (identity) {
+            HIP_OK(hipMemcpyAsync(out, ctx->l4.p, N4 * tp.uniq.size() * 16, hipMemcpyDeviceToDevice, ctx->stream));
+        } else {
+            for (std::size_t q = 0; q < tp.req2u.size(); ++q)
+                HIP_OK(hipMemcpyAsync(out + q * N4, ctx->l4.as<double2>() + tp.req2u[q] * N4, N4 * 16,
+                                      hipMemcpyDeviceToDevice, ctx->stream));
+        }
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int taco_l4_single(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, const double* omegas, const int* mirror,
+                   const int* pair_to_freq, int N, const double* ops, int num_ops, double dt, int method,
+                   size_t time_index, double* L4_out) {
+    if (time_index >= Nt) {
+        set_last_error("build_TCL4_generator_hip_fused: time_index out of range");
+        return TACO_ERR_RANGE;
+    }
+    const size_t t = time_index;
+    return taco_l4_batch(ctx, gamma, Nt, nf, omegas, mirror, pair_to_freq, N, ops, num_ops, dt, method, &t, 1,
+                         L4_out, nullptr);
+}
+
+int taco_l4_from_slices(taco_ctx* ctx, const double* Fs, const double* Cs, const double* Rs, const int* pair_to_freq,
+                        int N, int nf, const double* ops, int num_ops, double* L4_out, double* GW_out) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (N <= 0 || nf <= 0) throw Error(TACO_ERR_INVALID, "N and nf must be > 0");
+        if (num_ops <= 0) throw Error(TACO_ERR_INVALID, "assemble_liouvillian: coupling_ops must be non-empty");
+        for (int e = 0; e < N * N; ++e) {
+            if (pair_to_freq[e] < 0)
+                throw Error(TACO_ERR_RUNTIME, "pair_to_freq contains -1 for a required pair");
+            if (pair_to_freq[e] >= nf) throw Error(TACO_ERR_INVALID, "pair_to_freq entry >= nf");
+        }
+        DeviceGuard g(ctx->device);
+        const std::size_t nf3 = static_cast<std::size_t>(nf) * nf * nf;
+        const std::size_t N2 = static_cast<std::size_t>(N) * N, N4 = N2 * N2;
+        ctx->slices.ensure(3 * nf3 * 16);
+        double2* sl = ctx->slices.as<double2>();
+        HIP_OK(hipMemcpyAsync(sl, Fs, nf3 * 16, hipMemcpyHostToDevice, ctx->stream));
+        HIP_OK(hipMemcpyAsync(sl + nf3, Cs, nf3 * 16, hipMemcpyHostToDevice, ctx->stream));
+        HIP_OK(hipMemcpyAsync(sl + 2 * nf3, Rs, nf3 * 16, hipMemcpyHostToDevice, ctx->stream));
+        const int* dpf = upload(ctx, ctx->p2f, pair_to_freq, N2);
+        const double2* dops = upload(ctx, ctx->ops, reinterpret_cast<const double2*>(ops), N2 * num_ops);
+        ctx->l4.ensure(N4 * 16);
+        ctx->gw.ensure(N4 * 16);
+        L4Args A{};
+        A.Fs = sl;
+        A.Cs = sl + nf3;
+        A.Rs = sl + 2 * nf3;
+        A.p2f = dpf;
+        A.ops = dops;
+        A.N = N;
+        A.nf = nf;
+        A.num_ops = num_ops;
+        A.nf3 = static_cast<long>(nf3);
+        A.L4 = ctx->l4.as<double2>();
+        A.GW = ctx->gw.as<double2>();
+        launch_l4(ctx, A, 1);
+        HIP_OK(hipMemcpyAsync(L4_out, A.L4, N4 * 16, hipMemcpyDeviceToHost, ctx->stream));
+        if (GW_out) HIP_OK(hipMemcpyAsync(GW_out, A.GW, N4 * 16, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+static void rk4_launch(taco_ctx* ctx, const double2* dLs, const double2* dLh, std::size_t steps, int D, double2* dr,
+                       double dt, std::size_t every, double2* dsamples, long* dns) {
+    const std::size_t lds = 6 * static_cast<std::size_t>(D) * 16;
+    if (lds > 160 * 1024) throw Error(TACO_ERR_INVALID, "RK4 kernel supports D <= 1706");
+    hipLaunchKernelGGL(k_rk4, dim3(1), dim3(kBlock), lds, ctx->stream, dLs, dLh, static_cast<long>(steps), D, dr, dt,
+                       static_cast<long>(every), dsamples, dns);
+    HIP_OK(hipGetLastError());
+}
+
+int taco_rk4_propagate(taco_ctx* ctx, const double* L_series, const double* L_half, size_t steps, int D, double* r,
+                       double dt, size_t sample_every, double* samples_out, size_t* nsamples) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!(dt > 0.0)) throw Error(TACO_ERR_INVALID, "propagate_rk4_dense: dt must be > 0");
+        if (sample_every == 0) throw Error(TACO_ERR_INVALID, "propagate_rk4_dense: sample_every must be > 0");
+        if (steps < 1) throw Error(TACO_ERR_INVALID, "propagate_rk4_dense: L_series must have at least 2 matrices (endpoints)");
+        if (D <= 0) throw Error(TACO_ERR_INVALID, "propagate_rk4_dense: L_series and r dimension mismatch");
+        DeviceGuard g(ctx->device);
+        const std::size_t D2 = static_cast<std::size_t>(D) * D;
+        const std::size_t nsam = steps / sample_every + 2;
+        const std::size_t bytes = (steps + 1) * D2 * 16 + (L_half ? steps * D2 * 16 : 0) + D * 16 +
+                                  (samples_out ? nsam * D * 16 : 0) + 64;
+        ctx->rk.ensure(bytes);
+        char* base = ctx->rk.as<char>();
+        double2* dLs = reinterpret_cast<double2*>(base);
+        double2* dLh = L_half ? dLs + (steps + 1) * D2 : nullptr;
+        double2* dr = (L_half ? dLh + steps * D2 : dLs + (steps + 1) * D2);
+        double2* ds = samples_out ? dr + D : nullptr;
+        long* dns = reinterpret_cast<long*>((samples_out ? ds + nsam * D : dr + D));
+        HIP_OK(hipMemcpyAsync(dLs, L_series, (steps + 1) * D2 * 16, hipMemcpyHostToDevice, ctx->stream));
+        if (L_half) HIP_OK(hipMemcpyAsync(dLh, L_half, steps * D2 * 16, hipMemcpyHostToDevice, ctx->stream));
+        HIP_OK(hipMemcpyAsync(dr, r, D * 16, hipMemcpyHostToDevice, ctx->stream));
+        rk4_launch(ctx, dLs, dLh, steps, D, dr, dt, sample_every, ds, dns);
+        long ns = 0;
+        HIP_OK(hipMemcpyAsync(r, dr, D * 16, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipMemcpyAsync(&ns, dns, sizeof(long), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        if (samples_out)
+            HIP_OK(hipMemcpy(samples_out, ds, static_cast<std::size_t>(ns) * D * 16, hipMemcpyDeviceToHost));
+        if (nsamples) *nsamples = static_cast<std::size_t>(ns);
+    });
+}
+
+int taco_tcl4_propagate(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, const double* omegas,
+                        const int* mirror, const int* pair_to_freq, int N, const double* eps, const double* ops,
+                        int num_ops, double dt, size_t steps, int order, double* r, double* samples_out) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!(order == 0 || order == 2 || order == 4)) throw Error(TACO_ERR_INVALID, "order must be 0, 2, or 4");
+        validate_system(Nt, nf, mirror, pair_to_freq, N, num_ops, dt, TACO_FCR_CONVOLUTION);
+        if (steps < 1) throw Error(TACO_ERR_INVALID, "steps must be >= 1");
+        if (steps + 1 > Nt) throw Error(TACO_ERR_RANGE, "steps exceed gamma_series rows");
+        DeviceGuard g(ctx->device);
+        const std::size_t N2 = static_cast<std::size_t>(N) * N, N4 = N2 * N2;
+        const double2* dg = upload(ctx, ctx->gamma, reinterpret_cast<const double2*>(gamma), Nt * nf);
+        const double* dom = upload(ctx, ctx->omegas, omegas, static_cast<std::size_t>(nf));
+        const int* dmi = upload(ctx, ctx->mirror, mirror, static_cast<std::size_t>(nf));
+        const int* dpf = upload(ctx, ctx->p2f, pair_to_freq, N2);
+        const double2* dops = upload(ctx, ctx->ops, reinterpret_cast<const double2*>(ops), N2 * num_ops);
+        const double* deps = upload(ctx, ctx->eps, eps, static_cast<std::size_t>(N));
+        const std::size_t T = steps + 1;
+        if (order == 4) {
+            std::vector<std::size_t> all(T);
+            std::iota(all.begin(), all.end(), std::size_t{0});
+            l4_core(ctx, dg, Nt, nf, dom, dmi, dpf, N, dops, num_ops, dt, all.data(), T, nullptr);
+        } else {
+            ctx->l4.ensure(T * N4 * 16);
+            HIP_OK(hipMemsetAsync(ctx->l4.p, 0, T * N4 * 16, ctx->stream));
+        }
+        Tcl2Args A2{};
+        A2.gamma = dg;
+        A2.Nt = static_cast<long>(Nt);
+        A2.eps = deps;
+        A2.ops = dops;
+        A2.p2f = dpf;
+        A2.N = N;
+        A2.nf = nf;
+        A2.num_ops = num_ops;
+        A2.L = ctx->l4.as<double2>();
+        A2.order = order;
+        hipLaunchKernelGGL(k_tcl2, dim3(static_cast<unsigned>(T)), dim3(kBlock), 2 * N2 * 16, ctx->stream, A2);
+        HIP_OK(hipGetLastError());
+        ctx->misc.ensure((T + 1) * N2 * 16 + 64);
+        double2* dr = ctx->misc.as<double2>();
+        double2* ds = dr + N2;
+        HIP_OK(hipMemcpyAsync(dr, r, N2 * 16, hipMemcpyHostToDevice, ctx->stream));
+        rk4_launch(ctx, ctx->l4.as<double2>(), nullptr, steps, static_cast<int>(N2), dr, dt, 1,
+                   samples_out ? ds : nullptr, nullptr);
+        HIP_OK(hipMemcpyAsync(r, dr, N2 * 16, hipMemcpyDeviceToHost, ctx->stream));
+        if (samples_out) HIP_OK(hipMemcpyAsync(samples_out, ds, T * N2 * 16, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+}  // extern "C"

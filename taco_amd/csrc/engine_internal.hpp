@@ -1,0 +1,51 @@
+// engine_internal.hpp — shared declarations of the TCL4 engine sources.
+#pragma once
+
+#include <cstddef>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/taco_tcl4.h"
+
+namespace taco_engine {
+
+// Exception carrying a C-ABI status code; converted at the ABI boundary.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& msg) : std::runtime_error(msg), code(c) {}
+};
+
+std::size_t next_pow2(std::size_t n);
+std::size_t fcr_nfft(std::size_t Nt, std::size_t pad);
+void make_twiddles(int L, std::vector<double>& out);
+void bcf_fft(std::size_t N, double dt, int kind, double p0, double p1, double beta, double* C_out);
+void gamma_trapz(const double* C, std::size_t N, double dt, const double* omegas, int M, double* G);
+void mirror_indices(const double* omegas, int nf, double fidx_tol, int* mirror);
+void system_build(const double* H, int N, const double* A_lab, int num_ops, double freq_tol, double* eps,
+                  double* U, double* A_eig, int* nf_out, double* omegas, int* mirror, int* p2f,
+                  int* zero_index);
+
+void set_last_error(const std::string& msg);
+const double* cr_table();
+void cr_sincos_host(const double* x, std::size_t n, double* s, double* c);
+
+template <class Fn>
+int guarded(Fn&& fn) {
+    try {
+        fn();
+        set_last_error("");
+        return TACO_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_last_error("out of host memory");
+        return TACO_ERR_RUNTIME;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return TACO_ERR_RUNTIME;
+    }
+}
+
+}  // namespace taco_engine

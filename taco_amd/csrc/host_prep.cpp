@@ -1,0 +1,369 @@
+// host_prep.cpp — host-side preparation of the TCL4 hot-path inputs.
+//
+//  * system build: Hermitian eigensystem, Bohr-frequency buckets, pair map,
+//    mirror and zero bucket (system.hpp:37-157, 201-226; tcl4.cpp:40-101);
+//  * C(t) from a spectral density (bcf_fft_fun, correlation_fft.hpp:154-214);
+//  * the trapezoid Gamma table (gamma.hpp:166-213);
+//  * the recurrence twiddle tables consumed by the GPU FFT
+//    (correlation_fft.hpp:52-66).
+//
+// Rounding-sensitive steps keep the reference's operation order; the file
+// is compiled with -ffp-contract=off so no product is fused into an add.
+
+#include "engine_internal.hpp"
+
+#include <quadmath.h>
+
+#include "cr_sincos.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <limits>
+#include <stdexcept>
+#include <vector>
+
+namespace taco_engine {
+
+using cd = std::complex<double>;
+static constexpr double kPi = 3.141592653589793238462643383279502884;
+
+std::size_t next_pow2(std::size_t n) {
+    if (n == 0) return 1;
+    --n;
+    n |= n >> 1;  n |= n >> 2;  n |= n >> 4;
+    n |= n >> 8;  n |= n >> 16; n |= n >> 32;
+    return ++n;
+}
+
+std::size_t fcr_nfft(std::size_t Nt, std::size_t pad) {
+    const std::size_t L = 2 * Nt - 1;
+    std::size_t target = L;
+    if (pad > 0) target = std::max(L, pad * Nt);
+    std::size_t n = next_pow2(target);
+    return n < 2 ? 2 : n;
+}
+
+// Stage twiddles of the radix-2 FFT of length 2^L: for stage s (block length
+// 2^s) entry k < 2^(s-1) is the k-th power of wlen = exp(-2 pi i / 2^s)
+// obtained by repeated complex multiplication from 1, exactly the sequence
+// the reference's inner loop produces (w *= wlen restarts at 1 per block).
+// Table offset of stage s is 2^(s-1) - 1; total 2^L - 1 entries.
+void make_twiddles(int L, std::vector<double>& out) {
+    const std::size_t total = (std::size_t{1} << L) - 1;
+    out.assign(2 * total, 0.0);
+    for (int s = 1; s <= L; ++s) {
+        const std::size_t len = std::size_t{1} << s;
+        const double ang = (-1.0) * 2.0 * kPi / static_cast<double>(len);
+        const double lr = std::cos(ang), li = std::sin(ang);
+        double wr = 1.0, wi = 0.0;
+        const std::size_t off = (len >> 1) - 1;
+        for (std::size_t k = 0; k < (len >> 1); ++k) {
+            out[2 * (off + k)] = wr;
+            out[2 * (off + k) + 1] = wi;
+            const double nr = wr * lr - wi * li;
+            const double ni = wr * li + wi * lr;
+            wr = nr;
+            wi = ni;
+        }
+    }
+}
+
+// Host radix-2 FFT in the reference's form (bit reversal, recurrence twiddles).
+static void fft_host(std::vector<cd>& a, bool forward) {
+    const std::size_t n = a.size();
+    if (n == 0) return;
+    std::size_t j = 0;
+    for (std::size_t i = 1; i < n; ++i) {
+        std::size_t bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) std::swap(a[i], a[j]);
+    }
+    for (std::size_t len = 2; len <= n; len <<= 1) {
+        const double ang = (forward ? -1.0 : +1.0) * 2.0 * kPi / static_cast<double>(len);
+        const double lr = std::cos(ang), li = std::sin(ang);
+        const std::size_t half = len >> 1;
+        for (std::size_t i = 0; i < n; i += len) {
+            double wr = 1.0, wi = 0.0;
+            for (std::size_t k = 0; k < half; ++k) {
+                const cd u = a[i + k];
+                const cd x = a[i + k + half];
+                const cd v(wr * x.real() - wi * x.imag(), wr * x.imag() + wi * x.real());
+                a[i + k] = cd(u.real() + v.real(), u.imag() + v.imag());
+                a[i + k + half] = cd(u.real() - v.real(), u.imag() - v.imag());
+                const double nr = wr * lr - wi * li;
+                const double ni = wr * li + wi * lr;
+                wr = nr;
+                wi = ni;
+            }
+        }
+    }
+    if (!forward) {
+        const double inv_n = 1.0 / static_cast<double>(n);
+        for (auto& x : a) x = cd(x.real() * inv_n, x.imag() * inv_n);
+    }
+}
+
+static double spectral_J(int kind, double p0, double p1, double w) {
+    if (!(w > 0.0)) return 0.0;
+    if (kind == 0) return p0 * w * std::exp(-w / p1);
+    return 2.0 * p0 * p1 * w / (w * w + p1 * p1);
+}
+
+void bcf_fft(std::size_t N, double dt, int kind, double p0, double p1, double beta, double* C_out) {
+    if (!(dt > 0.0)) throw Error(TACO_ERR_INVALID, "bcf_fft: dt must be > 0");
+    if (kind != 0 && kind != 1) throw Error(TACO_ERR_INVALID, "bcf_fft: unknown spectral density kind");
+    const std::size_t Nt = N + 1;
+    std::size_t Nfft = 2 * Nt;
+    if ((Nfft & (Nfft - 1)) != 0) Nfft = next_pow2(Nfft);
+    const double tf = dt * static_cast<double>(Nfft);
+    const double domega = 2.0 * kPi / tf;
+    const double omegaNy = kPi / dt;
+    const std::size_t posCount = Nfft / 2 - 1;
+    std::vector<cd> S(Nfft, cd(0.0, 0.0));
+    if (std::isinf(beta)) {
+        for (std::size_t k = 1; k <= posCount; ++k) {
+            const double w = domega * static_cast<double>(k);
+            S[k] = cd((kPi / 2.0) * spectral_J(kind, p0, p1, w), 0.0);
+        }
+        S[Nfft / 2] = cd(0.25 * kPi * spectral_J(kind, p0, p1, omegaNy), 0.0);
+    } else {
+        const double r = std::exp(-beta * domega);
+        double KMS = r;
+        for (std::size_t k = 1; k <= posCount; ++k) {
+            const double w = domega * static_cast<double>(k);
+            const double Sp = (kPi / 2.0) * spectral_J(kind, p0, p1, w) / (1.0 - KMS);
+            S[k] = cd(Sp, 0.0);
+            S[Nfft - k] = cd(Sp * KMS, 0.0);
+            KMS *= r;
+        }
+        const double ed = r;
+        S[0] = cd(0.5 * (kPi / 2.0) * spectral_J(kind, p0, p1, domega) * (1.0 + ed) / (1.0 - ed), 0.0);
+        const double eNy = std::exp(beta * omegaNy);
+        S[Nfft / 2] = cd(0.25 * kPi * spectral_J(kind, p0, p1, omegaNy) * (1.0 + 2.0 / (eNy - 1.0)), 0.0);
+    }
+    fft_host(S, true);
+    const double scale = domega / kPi;
+    for (std::size_t n = 0; n < Nt; ++n) {
+        C_out[2 * n] = S[n].real() * scale;
+        C_out[2 * n + 1] = S[n].imag() * scale;
+    }
+}
+
+void gamma_trapz(const double* Cin, std::size_t N, double dt, const double* omegas, int M, double* Gout) {
+    if (!(dt > 0.0)) throw Error(TACO_ERR_INVALID, "gamma_trapz: dt must be > 0");
+    if (N == 0 || M <= 0) return;
+    const cd* C = reinterpret_cast<const cd*>(Cin);
+    cd* G = reinterpret_cast<cd*>(Gout);
+    const cd half_dt(dt / 2.0, 0.0);
+    for (int j = 0; j < M; ++j) {
+        cd* col = G + static_cast<std::size_t>(j) * N;
+        col[0] = cd(0.0, 0.0);
+        const cd step = std::exp(cd{0.0, omegas[j] * dt});
+        cd phi(1.0, 0.0), acc(0.0, 0.0), prev = C[0];
+        for (std::size_t k = 1; k < N; ++k) {
+            const cd phi_next = phi * step;
+            acc += half_dt * (phi * prev + phi_next * C[k]);
+            col[k] = acc;
+            phi = phi_next;
+            prev = C[k];
+        }
+    }
+}
+
+// Cyclic complex Jacobi for Hermitian matrices; eigenvalues ascending,
+// eigenvectors normalised with their largest-magnitude entry real positive.
+static void eigh(const cd* Hin, int n, double* w, cd* V) {
+    const std::size_t N = static_cast<std::size_t>(n);
+    std::vector<cd> A(Hin, Hin + N * N);
+    for (std::size_t r = 0; r < N; ++r) {
+        if (std::abs(A[r + r * N].imag()) > 1e-12 * (1.0 + std::abs(A[r + r * N].real())))
+            throw Error(TACO_ERR_INVALID, "Eigensystem: Hamiltonian must be Hermitian");
+        for (std::size_t c = 0; c < N; ++c) V[r + c * N] = (r == c) ? cd(1.0, 0.0) : cd(0.0, 0.0);
+    }
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (std::size_t c = 0; c < N; ++c)
+            for (std::size_t r = 0; r < N; ++r) {
+                tot += std::norm(A[r + c * N]);
+                if (r != c) off += std::norm(A[r + c * N]);
+            }
+        if (off <= 1e-30 * std::max(tot, 1e-300)) break;
+        for (std::size_t p = 0; p < N; ++p)
+            for (std::size_t q = p + 1; q < N; ++q) {
+                const cd apq = A[p + q * N];
+                const double mag = std::abs(apq);
+                if (mag < 1e-300) continue;
+                const double app = A[p + p * N].real(), aqq = A[q + q * N].real();
+                const cd ph = apq / mag;
+                const double theta = 0.5 * std::atan2(2.0 * mag, aqq - app);
+                const double c = std::cos(theta), s = std::sin(theta);
+                const cd gpp(c, 0.0), gpq = s * ph, gqp = -s * std::conj(ph), gqq(c, 0.0);
+                for (std::size_t r = 0; r < N; ++r) {
+                    const cd arp = A[r + p * N], arq = A[r + q * N];
+                    A[r + p * N] = arp * gpp + arq * gqp;
+                    A[r + q * N] = arp * gpq + arq * gqq;
+                }
+                for (std::size_t cc = 0; cc < N; ++cc) {
+                    const cd apc = A[p + cc * N], aqc = A[q + cc * N];
+                    A[p + cc * N] = std::conj(gpp) * apc + std::conj(gqp) * aqc;
+                    A[q + cc * N] = std::conj(gpq) * apc + std::conj(gqq) * aqc;
+                }
+                A[p + q * N] = cd(0.0, 0.0);
+                A[q + p * N] = cd(0.0, 0.0);
+                A[p + p * N] = cd(A[p + p * N].real(), 0.0);
+                A[q + q * N] = cd(A[q + q * N].real(), 0.0);
+                for (std::size_t r = 0; r < N; ++r) {
+                    const cd vrp = V[r + p * N], vrq = V[r + q * N];
+                    V[r + p * N] = vrp * gpp + vrq * gqp;
+                    V[r + q * N] = vrp * gpq + vrq * gqq;
+                }
+            }
+    }
+    std::vector<std::size_t> order(N);
+    for (std::size_t i = 0; i < N; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](std::size_t a, std::size_t b) { return A[a + a * N].real() < A[b + b * N].real(); });
+    std::vector<cd> Vs(N * N);
+    for (std::size_t c = 0; c < N; ++c) {
+        const std::size_t src = order[c];
+        w[c] = A[src + src * N].real();
+        std::size_t imax = 0;
+        double vmax = -1.0;
+        for (std::size_t r = 0; r < N; ++r)
+            if (std::abs(V[r + src * N]) > vmax + 1e-14) {
+                vmax = std::abs(V[r + src * N]);
+                imax = r;
+            }
+        const cd v0 = V[imax + src * N];
+        const cd phase = (std::abs(v0) > 0) ? std::conj(v0) / std::abs(v0) : cd(1.0, 0.0);
+        double nrm = 0.0;
+        for (std::size_t r = 0; r < N; ++r) nrm += std::norm(V[r + src * N]);
+        nrm = std::sqrt(nrm);
+        for (std::size_t r = 0; r < N; ++r) Vs[r + c * N] = V[r + src * N] * phase / nrm;
+    }
+    std::copy(Vs.begin(), Vs.end(), V);
+}
+
+void mirror_indices(const double* omegas, int nf, double fidx_tol, int* mirror) {
+    const double tol = std::max(1e-12, fidx_tol);
+    for (int j = 0; j < nf; ++j) {
+        const double w = omegas[j];
+        if (std::abs(w) <= tol) {
+            mirror[j] = j;
+            continue;
+        }
+        double best = std::numeric_limits<double>::infinity();
+        int best_idx = -1;
+        for (int jp = 0; jp < nf; ++jp) {
+            const double dw = std::abs(omegas[jp] - (-w));
+            if (dw < best) {
+                best = dw;
+                best_idx = jp;
+            }
+        }
+        mirror[j] = best_idx >= 0 ? best_idx : j;
+    }
+}
+
+void system_build(const double* Hin, int N, const double* A_lab, int num_ops, double freq_tol, double* eps,
+                  double* Uout, double* A_eig, int* nf_out, double* omegas, int* mirror, int* p2f,
+                  int* zero_index) {
+    if (N <= 0) throw Error(TACO_ERR_INVALID, "Eigensystem: Hamiltonian dimension must be > 0");
+    if (num_ops < 0) throw Error(TACO_ERR_INVALID, "system_build: num_ops must be >= 0");
+    const std::size_t NN = static_cast<std::size_t>(N);
+    const cd* H = reinterpret_cast<const cd*>(Hin);
+    for (std::size_t r = 0; r < NN; ++r)
+        for (std::size_t c = 0; c < NN; ++c)
+            if (std::abs(H[r + c * NN] - std::conj(H[c + r * NN])) > 1e-12 * (1.0 + std::abs(H[r + c * NN])))
+                throw Error(TACO_ERR_INVALID, "Eigensystem: Hamiltonian must be Hermitian");
+    cd* U = reinterpret_cast<cd*>(Uout);
+    eigh(H, N, eps, U);
+    // build_frequency_buckets (system.hpp:138-157): scan (m, n), first bucket
+    // within tol, representative = first omega seen, then sort by omega.
+    const double tol = std::max(freq_tol, std::numeric_limits<double>::epsilon());
+    struct Bucket {
+        double omega;
+        std::vector<std::pair<int, int>> pairs;
+    };
+    std::vector<Bucket> bk;
+    for (int m = 0; m < N; ++m)
+        for (int n = 0; n < N; ++n) {
+            const double w = eps[m] - eps[n];
+            std::size_t idx = bk.size();
+            for (std::size_t q = 0; q < bk.size(); ++q)
+                if (std::abs(bk[q].omega - w) <= tol) {
+                    idx = q;
+                    break;
+                }
+            if (idx == bk.size()) bk.push_back(Bucket{w, {}});
+            bk[idx].pairs.emplace_back(m, n);
+        }
+    std::sort(bk.begin(), bk.end(), [](const Bucket& a, const Bucket& b) { return a.omega < b.omega; });
+    const int nf = static_cast<int>(bk.size());
+    *nf_out = nf;
+    for (std::size_t e = 0; e < NN * NN; ++e) p2f[e] = -1;
+    for (int b = 0; b < nf; ++b) {
+        omegas[b] = bk[static_cast<std::size_t>(b)].omega;
+        for (const auto& pr : bk[static_cast<std::size_t>(b)].pairs) p2f[pr.first + pr.second * N] = b;
+    }
+    mirror_indices(omegas, nf, tol, mirror);
+    const double mtol = std::max(1e-12, tol);
+    int zi = -1;
+    for (int b = 0; b < nf && zi < 0; ++b)
+        if (std::abs(omegas[b]) <= mtol) zi = b;
+    if (zi < 0) {
+        double best = std::numeric_limits<double>::infinity();
+        for (int b = 0; b < nf; ++b)
+            if (std::abs(omegas[b]) < best) {
+                best = std::abs(omegas[b]);
+                zi = b;
+            }
+    }
+    *zero_index = zi;
+    const cd* Al = reinterpret_cast<const cd*>(A_lab);
+    cd* Ae = reinterpret_cast<cd*>(A_eig);
+    std::vector<cd> tmp(NN * NN);
+    for (int o = 0; o < num_ops; ++o) {
+        const cd* A = Al + static_cast<std::size_t>(o) * NN * NN;
+        cd* E = Ae + static_cast<std::size_t>(o) * NN * NN;
+        for (std::size_t r = 0; r < NN; ++r)
+            for (std::size_t c = 0; c < NN; ++c) {
+                cd s(0.0, 0.0);
+                for (std::size_t q = 0; q < NN; ++q) s += std::conj(U[q + r * NN]) * A[q + c * NN];
+                tmp[r + c * NN] = s;
+            }
+        for (std::size_t r = 0; r < NN; ++r)
+            for (std::size_t c = 0; c < NN; ++c) {
+                cd s(0.0, 0.0);
+                for (std::size_t q = 0; q < NN; ++q) s += tmp[r + q * NN] * U[q + c * NN];
+                E[r + c * NN] = s;
+            }
+    }
+}
+
+// sin(j/64), cos(j/64), j < kTabN, as double-doubles from quad precision.
+const double* cr_table() {
+    static const std::vector<double> tab = [] {
+        std::vector<double> t(4 * taco_cr::kTabN);
+        for (int j = 0; j < taco_cr::kTabN; ++j) {
+            const __float128 a = static_cast<__float128>(j) / 64;
+            const __float128 sv = sinq(a), cv = cosq(a);
+            const double sh = static_cast<double>(sv), ch = static_cast<double>(cv);
+            t[4 * j] = sh;
+            t[4 * j + 1] = static_cast<double>(sv - sh);
+            t[4 * j + 2] = ch;
+            t[4 * j + 3] = static_cast<double>(cv - ch);
+        }
+        return t;
+    }();
+    return tab.data();
+}
+
+void cr_sincos_host(const double* x, std::size_t n, double* s, double* c) {
+    const double* tab = cr_table();
+    for (std::size_t k = 0; k < n; ++k) taco_cr::sincos_cr(x[k], s + k, c + k, tab);
+}
+
+}  // namespace taco_engine

@@ -1,0 +1,435 @@
+"""Host-side mirror of the reference TCL4 interface, served by the HIP engine.
+
+Names, argument meaning and error behaviour follow jcarrots/taco:
+
+* ``Exec`` / ``Backend``              cpp/include/taco/exec.hpp:11-19
+* ``FCRMethod``                        cpp/include/taco/tcl4_kernels.hpp:9-12
+* ``System`` (``sys::System``)         cpp/include/taco/system.hpp:201-226
+* ``build_map`` / ``Tcl4Map``          cpp/include/taco/tcl4.hpp:13-28, cpp/src/tcl/tcl4.cpp:40-101
+* ``compute_triple_kernels``           tcl4.hpp:37-42, tcl4.cpp:103-183
+* ``build_TCL4_generator_hip_fused``   tcl4_fused_cuda.hpp:16-21 (the CUDA name is an alias)
+* ``build_TCL4_generator_hip_fused_batch``  tcl4_fused_cuda.hpp:25-41
+* ``build_correction_series_hip_fused``      tcl4_fused_cuda.hpp:44-48
+* ``build_TCL4_generator`` / ``build_correction_series``  tcl4.hpp:83-95
+* ``propagate_rk4_dense``              cpp/include/taco/rk4_dense.hpp:85-129
+
+Matrices are numpy arrays indexed ``[row, col]``; the engine receives them
+column-major (Eigen's layout).  Exceptions map the reference's types:
+``TacoInvalidArgument`` (ValueError) for std::invalid_argument,
+``TacoOutOfRange`` (IndexError) for std::out_of_range, ``TacoError``
+(RuntimeError) for std::runtime_error, ``TacoOverflow`` for std::overflow_error.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+import threading
+from ctypes import c_double, c_int, c_size_t, c_void_p
+from dataclasses import dataclass, field
+from typing import Callable, Optional, Sequence
+
+import numpy as np
+
+from ._lib import TacoInvalidArgument, check, dptr, iptr, lib, sptr
+
+__all__ = [
+    "Backend", "Exec", "FCRMethod", "System", "Tcl4Map", "TripleKernelSeries", "Ohmic", "DrudeLorentz",
+    "build_map", "bcf_fft_fun", "compute_trapz_prefix_multi_matrix", "compute_triple_kernels",
+    "build_TCL4_generator_hip_fused", "build_TCL4_generator_hip_fused_batch", "build_correction_series_hip_fused",
+    "build_TCL4_generator_cuda_fused", "build_TCL4_generator_cuda_fused_batch", "build_correction_series_cuda_fused",
+    "build_TCL4_generator", "build_correction_series", "assemble_l4_from_slices", "propagate_rk4_dense",
+    "propagate_rk4_dense_serial", "propagate_rk4_dense_omp", "tcl4_propagate", "set_fcr_fft_pad_factor",
+    "get_fcr_fft_pad_factor", "context", "version",
+]
+
+
+class Backend(enum.Enum):
+    Serial = 0
+    Omp = 1
+    Cuda = 2
+    MpiOmp = 3
+    MpiCuda = 4
+    Hip = 5
+
+
+@dataclass
+class Exec:
+    """exec.hpp:13-19.  Serial/Omp/Cuda/Hip requests are all served by the
+    HIP engine on ``gpu_id``; the MPI backends are not part of this engine."""
+    backend: Backend = Backend.Hip
+    threads: int = 0
+    gpu_id: int = 0
+    streams: int = 1
+    pinned: bool = True
+
+
+class FCRMethod(enum.IntEnum):
+    Convolution = 0
+    Direct = 1
+
+
+def version() -> str:
+    return lib().taco_version().decode()
+
+
+def set_fcr_fft_pad_factor(factor: int) -> None:
+    """tcl4_kernels.cpp:179-181 (process-global)."""
+    lib().taco_set_fcr_fft_pad_factor(int(factor))
+
+
+def get_fcr_fft_pad_factor() -> int:
+    return int(lib().taco_get_fcr_fft_pad_factor())
+
+
+# --- contexts: one per (host thread, device), like the reference's thread_local workspace
+_tls = threading.local()
+
+
+class _Ctx:
+    def __init__(self, device: int):
+        h = c_void_p()
+        check(lib().taco_ctx_create(device, ctypes.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().taco_ctx_destroy(self.handle)
+        except Exception:
+            pass
+
+    @property
+    def stream(self) -> int:
+        return int(lib().taco_ctx_stream(self.handle) or 0)
+
+
+def context(device: int = 0) -> _Ctx:
+    cache = getattr(_tls, "ctx", None)
+    if cache is None:
+        cache = _tls.ctx = {}
+    if device not in cache:
+        cache[device] = _Ctx(device)
+    return cache[device]
+
+
+def _ctx_for(exec_: Optional[Exec]) -> _Ctx:
+    e = exec_ or Exec()
+    if e.backend in (Backend.MpiOmp, Backend.MpiCuda):
+        raise TacoInvalidArgument(1, "MPI backends are not provided by the single-GPU HIP engine")
+    return context(e.gpu_id)
+
+
+# --- inputs ------------------------------------------------------------------
+
+def _colmajor(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, np.complex128).T)
+
+
+def _ops_flat(ops) -> np.ndarray:
+    return np.concatenate([_colmajor(o).ravel() for o in ops]) if len(ops) else np.zeros(0, np.complex128)
+
+
+@dataclass
+class Tcl4Map:
+    N: int
+    nf: int
+    omegas: np.ndarray
+    pair_to_freq: np.ndarray
+    mirror_index: np.ndarray
+    zero_index: int
+    time_grid: Sequence[float] = ()
+
+
+@dataclass
+class System:
+    """sys::System: eigensystem of H, Bohr-frequency buckets and A_eig."""
+    H: np.ndarray
+    A_lab: list
+    eps: np.ndarray
+    U: np.ndarray
+    A_eig: list
+    omegas: np.ndarray
+    mirror: np.ndarray
+    pair_to_freq: np.ndarray
+    zero_index: int
+    freq_tol: float = 1e-9
+
+    @property
+    def N(self) -> int:
+        return int(self.H.shape[0])
+
+    @property
+    def nf(self) -> int:
+        return int(len(self.omegas))
+
+    @classmethod
+    def build(cls, H, jump_ops_lab, freq_tol: float = 1e-9) -> "System":
+        H = np.asarray(H, np.complex128)
+        if H.ndim != 2 or H.shape[0] != H.shape[1]:
+            raise TacoInvalidArgument(1, "Eigensystem: Hamiltonian must be square")
+        N = H.shape[0]
+        ops = [np.asarray(o, np.complex128) for o in jump_ops_lab]
+        for o in ops:
+            if o.shape != (N, N):
+                raise TacoInvalidArgument(1, "to_eigen: dimension mismatch")
+        Hf = _colmajor(H).ravel()
+        of = _ops_flat(ops)
+        eps = np.zeros(N)
+        U = np.zeros(N * N, np.complex128)
+        Ae = np.zeros(max(len(ops), 1) * N * N, np.complex128)
+        nf = c_int(0)
+        om = np.zeros(N * N)
+        mi = np.zeros(N * N, np.int32)
+        pf = np.zeros(N * N, np.int32)
+        zi = c_int(0)
+        check(lib().taco_system_build(dptr(Hf), N, dptr(of) if of.size else None, len(ops), freq_tol, dptr(eps),
+                                      dptr(U), dptr(Ae), ctypes.byref(nf), dptr(om), iptr(mi), iptr(pf),
+                                      ctypes.byref(zi)))
+        n = nf.value
+        return cls(H=H, A_lab=ops, eps=eps, U=U.reshape(N, N).T.copy(),
+                   A_eig=[Ae[k * N * N:(k + 1) * N * N].reshape(N, N).T.copy() for k in range(len(ops))],
+                   omegas=om[:n].copy(), mirror=mi[:n].copy(), pair_to_freq=pf.reshape(N, N).T.copy(),
+                   zero_index=zi.value, freq_tol=freq_tol)
+
+    def to_lab(self, A_eig) -> np.ndarray:
+        return self.U @ np.asarray(A_eig) @ self.U.conj().T
+
+    def to_eigen(self, A_lab) -> np.ndarray:
+        return self.U.conj().T @ np.asarray(A_lab) @ self.U
+
+
+def build_map(system: System, time_grid: Sequence[float] = ()) -> Tcl4Map:
+    return Tcl4Map(N=system.N, nf=system.nf, omegas=system.omegas.copy(), pair_to_freq=system.pair_to_freq.copy(),
+                   mirror_index=system.mirror.copy(), zero_index=system.zero_index, time_grid=list(time_grid))
+
+
+@dataclass(frozen=True)
+class Ohmic:
+    """J(w) = alpha * w * exp(-w / omega_c) for w > 0 (TCL4_spin_boson_example.cpp:210-213)."""
+    alpha: float
+    omega_c: float
+    kind: int = field(default=0, init=False)
+
+    @property
+    def params(self):
+        return self.alpha, self.omega_c
+
+
+@dataclass(frozen=True)
+class DrudeLorentz:
+    """J(w) = 2 lambda gamma w / (w^2 + gamma^2) for w > 0."""
+    lam: float
+    gamma: float
+    kind: int = field(default=1, init=False)
+
+    @property
+    def params(self):
+        return self.lam, self.gamma
+
+
+def bcf_fft_fun(N: int, dt: float, J, beta: float):
+    """correlation_fft.hpp:154-214: returns (t[0..N], C[0..N])."""
+    C = np.zeros(N + 1, np.complex128)
+    p0, p1 = J.params
+    check(lib().taco_bcf_fft(N, dt, J.kind, p0, p1, beta, dptr(C)))
+    return np.arange(N + 1) * dt, C
+
+
+def compute_trapz_prefix_multi_matrix(C, dt: float, omegas) -> np.ndarray:
+    """gamma.hpp:166-213: Gamma table [Nt, nf]."""
+    C = np.ascontiguousarray(C, np.complex128)
+    om = np.ascontiguousarray(omegas, np.float64)
+    G = np.zeros((len(om), C.size), np.complex128)
+    check(lib().taco_gamma_trapz(dptr(C), C.size, dt, dptr(om), len(om), dptr(G)))
+    return G.T
+
+
+# --- hot path ----------------------------------------------------------------
+
+@dataclass
+class TripleKernelSeries:
+    """tcl4.hpp:30-35: F[i][j][k](t) as arrays of shape [nf, nf, nf, Nt]."""
+    F: np.ndarray
+    C: np.ndarray
+    R: np.ndarray
+
+
+def _gamma_checked(system: System, gamma_series) -> np.ndarray:
+    g = np.asarray(gamma_series, np.complex128)
+    if g.ndim != 2:
+        raise TacoInvalidArgument(1, "gamma_series must be a [Nt, nf] matrix")
+    if g.shape[1] != system.nf:
+        raise TacoInvalidArgument(1, "gamma_series column count does not match frequency buckets")
+    return g
+
+
+def compute_triple_kernels(system: System, gamma_series, dt: float, nmax: int = 2,
+                           method: FCRMethod = FCRMethod.Convolution, exec: Optional[Exec] = None,
+                           keep: Optional[int] = None) -> TripleKernelSeries:
+    """All nf^3 F/C/R series on the GPU.  ``keep`` (extension) limits the
+    returned samples to t < keep; the values equal the full series' head."""
+    del nmax  # ignored, as in the reference (tcl4.cpp:117)
+    ctx = _ctx_for(exec)
+    g = _gamma_checked(system, gamma_series)
+    Nt, nf = g.shape
+    keep = Nt if keep is None else int(keep)
+    gc = _colmajor(g).ravel()
+    F = np.zeros((nf, nf, nf, keep), np.complex128)
+    C = np.zeros_like(F)
+    R = np.zeros_like(F)
+    om = np.ascontiguousarray(system.omegas, np.float64)
+    mi = np.ascontiguousarray(system.mirror, np.int32)
+    check(lib().taco_triple_kernels(ctx.handle, dptr(gc), Nt, nf, dptr(om), iptr(mi), dt, int(method), keep,
+                                    dptr(F), dptr(C), dptr(R)))
+    return TripleKernelSeries(F, C, R)
+
+
+def _system_arrays(system: System):
+    om = np.ascontiguousarray(system.omegas, np.float64)
+    mi = np.ascontiguousarray(system.mirror, np.int32)
+    pf = np.ascontiguousarray(np.asarray(system.pair_to_freq, np.int32).T.ravel())
+    of = _ops_flat(system.A_eig)
+    return om, mi, pf, of
+
+
+def build_TCL4_generator_hip_fused_batch(system: System, gamma_series, dt: float,
+                                         time_indices: Sequence[int] = (),
+                                         method: FCRMethod = FCRMethod.Convolution, exec: Optional[Exec] = None,
+                                         fcr_ms: Optional[list] = None) -> np.ndarray:
+    """L4(t) for each t in time_indices (empty = all t < Nt): [ntau, N^2, N^2]."""
+    ctx = _ctx_for(exec)
+    g = _gamma_checked(system, gamma_series)
+    Nt, nf = g.shape
+    N = system.N
+    tid = np.ascontiguousarray(np.asarray(list(time_indices), dtype=np.int64), np.uint64)
+    if tid.size and (np.asarray(list(time_indices)) < 0).any():
+        from ._lib import TacoOutOfRange
+        raise TacoOutOfRange(2, "time_index out of range")
+    ntau = tid.size if tid.size else Nt
+    out = np.zeros((ntau, N ** 4), np.complex128)
+    om, mi, pf, of = _system_arrays(system)
+    ms = c_double(0.0)
+    gc = _colmajor(g).ravel()
+    check(lib().taco_l4_batch(ctx.handle, dptr(gc), Nt, nf, dptr(om), iptr(mi), iptr(pf), N,
+                              dptr(of) if of.size else None, len(system.A_eig), dt, int(method),
+                              sptr(tid) if tid.size else None, tid.size, dptr(out), ctypes.byref(ms)))
+    if fcr_ms is not None:
+        fcr_ms.append(ms.value)
+    return out.reshape(ntau, N * N, N * N).transpose(0, 2, 1)
+
+
+def build_TCL4_generator_hip_fused(system: System, gamma_series, dt: float, time_index: int,
+                                   method: FCRMethod = FCRMethod.Convolution,
+                                   exec: Optional[Exec] = None) -> np.ndarray:
+    g = _gamma_checked(system, gamma_series)
+    if time_index < 0 or time_index >= g.shape[0]:
+        from ._lib import TacoOutOfRange
+        raise TacoOutOfRange(2, "build_TCL4_generator_hip_fused: time_index out of range")
+    return build_TCL4_generator_hip_fused_batch(system, g, dt, [time_index], method, exec)[0]
+
+
+def build_correction_series_hip_fused(system: System, gamma_series, dt: float,
+                                      method: FCRMethod = FCRMethod.Convolution,
+                                      exec: Optional[Exec] = None) -> np.ndarray:
+    return build_TCL4_generator_hip_fused_batch(system, gamma_series, dt, (), method, exec)
+
+
+# CUDA names of the reference, served by the same engine.
+build_TCL4_generator_cuda_fused = build_TCL4_generator_hip_fused
+build_TCL4_generator_cuda_fused_batch = build_TCL4_generator_hip_fused_batch
+build_correction_series_cuda_fused = build_correction_series_hip_fused
+
+
+def build_TCL4_generator(system: System, gamma_series, dt: float, time_index: int,
+                         method: FCRMethod = FCRMethod.Convolution, exec: Optional[Exec] = None) -> np.ndarray:
+    """tcl4.cpp:293-320 (Exec dispatch) — always the fused GPU path here."""
+    return build_TCL4_generator_hip_fused(system, gamma_series, dt, time_index, method, exec)
+
+
+def build_correction_series(system: System, gamma_series, dt: float, method: FCRMethod = FCRMethod.Convolution,
+                            exec: Optional[Exec] = None) -> np.ndarray:
+    return build_correction_series_hip_fused(system, gamma_series, dt, method, exec)
+
+
+def assemble_l4_from_slices(Fs, Cs, Rs, pair_to_freq, ops, exec: Optional[Exec] = None):
+    """MIKX -> GW -> L4 at one time from nf^3 slices (idx3 = (i nf + j) nf + k).
+    Returns (L4, GW) as N^2 x N^2 matrices."""
+    ctx = _ctx_for(exec)
+    Fs = np.ascontiguousarray(np.asarray(Fs, np.complex128).ravel())
+    Cs = np.ascontiguousarray(np.asarray(Cs, np.complex128).ravel())
+    Rs = np.ascontiguousarray(np.asarray(Rs, np.complex128).ravel())
+    nf = int(round(Fs.size ** (1.0 / 3.0)))
+    if nf ** 3 != Fs.size or Cs.size != Fs.size or Rs.size != Fs.size:
+        raise TacoInvalidArgument(1, "slices must hold nf^3 values each")
+    p2f = np.asarray(pair_to_freq, np.int32)
+    N = p2f.shape[0]
+    pf = np.ascontiguousarray(p2f.T.ravel())
+    of = _ops_flat(ops)
+    L = np.zeros(N ** 4, np.complex128)
+    GW = np.zeros(N ** 4, np.complex128)
+    check(lib().taco_l4_from_slices(ctx.handle, dptr(Fs), dptr(Cs), dptr(Rs), iptr(pf), N, nf,
+                                    dptr(of) if of.size else None, len(ops), dptr(L), dptr(GW)))
+    return L.reshape(N * N, N * N).T, GW.reshape(N * N, N * N).T
+
+
+def propagate_rk4_dense(L_series, r, t0: float, dt: float,
+                        on_sample: Optional[Callable[[float, np.ndarray], None]] = None, sample_every: int = 1,
+                        L_half_series=None, exec: Optional[Exec] = None) -> np.ndarray:
+    """rk4_dense.cpp:246-320 on the GPU.  Returns the final r; ``on_sample``
+    is called with (t, r) at steps 0, every, ... and at the end."""
+    ctx = _ctx_for(exec)
+    Ls = np.asarray(L_series, np.complex128)
+    if Ls.ndim != 3 or Ls.shape[0] < 2:
+        raise TacoInvalidArgument(1, "propagate_rk4_dense: L_series must have at least 2 matrices (endpoints)")
+    steps = Ls.shape[0] - 1
+    D = Ls.shape[1]
+    if Ls.shape[2] != D:
+        raise TacoInvalidArgument(1, "propagate_rk4_dense: L_series must be square")
+    rv = np.ascontiguousarray(np.asarray(r, np.complex128)).copy()
+    if rv.size != D:
+        raise TacoInvalidArgument(1, "propagate_rk4_dense: L_series and r dimension mismatch")
+    hp = None
+    if L_half_series is not None:
+        Lh = np.asarray(L_half_series, np.complex128)
+        if Lh.shape != (steps, D, D):
+            raise TacoInvalidArgument(1, "propagate_rk4_dense: L_half_series must match L_series size - 1 (midpoints)")
+        Lhf = np.ascontiguousarray(Lh.transpose(0, 2, 1)).ravel()
+        hp = dptr(Lhf)
+    flat = np.ascontiguousarray(Ls.transpose(0, 2, 1)).ravel()
+    samples = None
+    if on_sample is not None:
+        samples = np.zeros((steps // sample_every + 2, D), np.complex128)
+    ns = c_size_t(0)
+    check(lib().taco_rk4_propagate(ctx.handle, dptr(flat), hp, steps, D, dptr(rv), dt, sample_every,
+                                   dptr(samples) if samples is not None else None, ctypes.byref(ns)))
+    if on_sample is not None:
+        for q in range(ns.value):
+            step = q * sample_every if q < ns.value - 1 else steps
+            on_sample(t0 + step * dt, samples[q])
+    return rv
+
+
+propagate_rk4_dense_serial = propagate_rk4_dense
+propagate_rk4_dense_omp = propagate_rk4_dense
+
+
+def tcl4_propagate(system: System, gamma_series, dt: float, steps: int, r0, order: int = 4,
+                   exec: Optional[Exec] = None) -> np.ndarray:
+    """Driver composition (TCL4_spin_boson_example.cpp:284-384): states
+    r(t_0..t_steps) in the eigenbasis, shape [steps+1, N^2]."""
+    ctx = _ctx_for(exec)
+    g = _gamma_checked(system, gamma_series)
+    Nt, nf = g.shape
+    N = system.N
+    om, mi, pf, of = _system_arrays(system)
+    eps = np.ascontiguousarray(system.eps, np.float64)
+    r = np.ascontiguousarray(np.asarray(r0, np.complex128)).copy()
+    if r.size != N * N:
+        raise TacoInvalidArgument(1, "r0 must have N^2 entries")
+    samples = np.zeros((steps + 1, N * N), np.complex128)
+    gc = _colmajor(g).ravel()
+    check(lib().taco_tcl4_propagate(ctx.handle, dptr(gc), Nt, nf, dptr(om), iptr(mi), iptr(pf), N, dptr(eps),
+                                    dptr(of) if of.size else None, len(system.A_eig), dt, steps, order, dptr(r),
+                                    dptr(samples)))
+    return samples

@@ -1,0 +1,209 @@
+"""GPU parity of the F/C/R kernels and the fused L4 builders against the
+oracle, through the C ABI.  Bar: per-t relative Frobenius error <= 1e-10
+(floored at 1e-6 of the series maximum, see tests/metrics.py)."""
+import numpy as np
+import pytest
+
+import taco_amd
+from taco_amd import workloads as W
+from taco_amd._lib import TacoError, TacoInvalidArgument, TacoOutOfRange
+
+from .metrics import max_rel_err, per_t_rel_err, ref_style_err
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def series_t_major(X):
+    nf = X.shape[0]
+    return X.reshape(nf ** 3, -1).T  # [t, nf^3]
+
+
+def smooth_gamma(Nt, nf, seed):
+    """Gamma-like columns: zero at t = 0, saturating, oscillating."""
+    rng = np.random.default_rng(seed)
+    t = np.arange(Nt) * 0.01
+    cols = []
+    for b in range(nf):
+        a, g, w = rng.uniform(0.5, 2), rng.uniform(0.5, 3), rng.uniform(-3, 3)
+        cols.append(a * (1 - np.exp(-(g - 1j * w) * t)) / (g - 1j * w))
+    return np.stack(cols, axis=1)
+
+
+def check_fcr(orc, S, G, dt, keep):
+    """Per-t error of the F/C/R slices.  F(0) and C(0) vanish in exact
+    arithmetic (Gamma(., 0) = 0 kills both terms); the computed values are FFT
+    rounding noise, so t = 0 is checked in absolute terms against the series
+    maximum and t >= 1 with the relative metric."""
+    got = taco_amd.compute_triple_kernels(S, G, dt, keep=keep)
+    ref = orc.triple_kernels(G, S.omegas, S.mirror, dt, keep=keep)
+    errs = {}
+    for name, g, r in zip("FCR", (got.F, got.C, got.R), ref):
+        gt, rt = series_t_major(g), series_t_major(r)
+        scale = np.abs(rt).max() if rt.size else 0.0
+        assert np.abs(gt[0] - rt[0]).max() <= 1e-14 * max(scale, 1e-300), name
+        e = 0.0
+        if gt.shape[0] > 1:
+            per_t, _ = per_t_rel_err(gt[1:], rt[1:])
+            e = float(per_t.max())
+            print(name, "per-t rel err t=1..8:", np.array2string(per_t[:8], precision=2))
+        errs[name] = e
+    return errs
+
+
+def test_triple_kernels_spin_boson_full_grid(orc):
+    w = W.spin_boson()
+    errs = check_fcr(orc, w.system, w.gamma, w.dt, 2001)
+    print("spin_boson F/C/R max per-t rel err", errs)
+    assert max(errs.values()) <= TOL, errs
+
+
+@pytest.mark.parametrize("Nt", [1, 2, 3, 5, 64, 1000, 4097])
+def test_triple_kernels_edge_lengths(orc, Nt):
+    w = W.spin_boson()
+    S = w.system
+    G = smooth_gamma(Nt, S.nf, Nt)
+    errs = check_fcr(orc, S, G, 0.01, Nt)
+    assert max(errs.values()) <= TOL, (Nt, errs)
+
+
+def test_triple_kernels_pad_factor(orc):
+    w = W.spin_boson(Nt=3000)
+    S, G = w.system, w.gamma
+    taco_amd.set_fcr_fft_pad_factor(4)
+    try:
+        got = taco_amd.compute_triple_kernels(S, G, w.dt, keep=500)
+    finally:
+        taco_amd.set_fcr_fft_pad_factor(0)
+    ref = orc.triple_kernels(G, S.omegas, S.mirror, w.dt, keep=500, pad=4)
+    for g, r in zip((got.F, got.C, got.R), ref):
+        assert max_rel_err(series_t_major(g), series_t_major(r))[0] <= TOL
+
+
+def l4_check(orc, w, tidx, tol=TOL):
+    S = w.system
+    G = w.gamma
+    got = taco_amd.build_TCL4_generator_hip_fused_batch(S, G, w.dt, tidx)
+    ref = orc.l4_series(G, S.omegas, S.mirror, S.pair_to_freq, S.A_eig, w.dt, tidx)
+    e, nfloor = max_rel_err(got, ref)
+    per_t, _ = per_t_rel_err(got, ref)
+    print(f"{w.name}: ntau={len(tidx)} max per-t rel err {e:.3e} (floored t: {nfloor}), "
+          f"ref-style {ref_style_err(got, ref):.3e}, first t: {np.array2string(per_t[:6], precision=2)}")
+    assert e <= tol
+    return got, ref
+
+
+def test_l4_spin_boson(orc):
+    w = W.spin_boson()
+    l4_check(orc, w, w.tidx)
+
+
+def test_l4_gue_pair_reduced_grid(orc):
+    w = W.gue_pair(N_bath=20000, tmax=1000)
+    l4_check(orc, w, w.tidx)
+
+
+def test_l4_published_setting_reduced(orc):
+    w = W.published(N_bath=20000, tmax=2000)
+    l4_check(orc, w, w.tidx)
+
+
+@pytest.mark.parametrize("b", [0, 5])
+def test_l4_heterogeneous_small(orc, b):
+    w = W.heterogeneous_system(b, n_time=40, log2_range=(10, 11))
+    if w.meta["d"] > 6:
+        pytest.skip("d = 8 covered by test_l4_d8_small")
+    l4_check(orc, w, w.tidx[:40])
+
+
+def test_l4_d8_small(orc):
+    rng = W.SplitMix64(77)
+    H = W.gue(8, rng)
+    A = W.random_hermitian_unit(8, rng)
+    w = W.Workload("d8", H, [A], taco_amd.DrudeLorentz(0.1, 5.0), 1.0, 0.01, 299, [0, 1, 7, 50, 299])
+    assert w.system.nf == 57
+    l4_check(orc, w, w.tidx)
+
+
+def test_l4_time_list_semantics(orc):
+    w = W.spin_boson(Nt=500)
+    S, G = w.system, w.gamma
+    tidx = [37, 5, 37, 0, 499, 5]
+    got = taco_amd.build_TCL4_generator_hip_fused_batch(S, G, w.dt, tidx)
+    ref = orc.l4_series(G, S.omegas, S.mirror, S.pair_to_freq, S.A_eig, w.dt, tidx)
+    assert max_rel_err(got, ref)[0] <= TOL
+    one = taco_amd.build_TCL4_generator_hip_fused(S, G, w.dt, 37)
+    assert np.array_equal(one, got[0])
+    allt = taco_amd.build_correction_series_hip_fused(S, G[:200], w.dt)
+    assert allt.shape == (200, 4, 4)
+    refall = orc.l4_series(G[:200], S.omegas, S.mirror, S.pair_to_freq, S.A_eig, w.dt, list(range(200)))
+    assert max_rel_err(allt, refall)[0] <= TOL
+    # deterministic: a second run is bitwise identical
+    again = taco_amd.build_TCL4_generator_hip_fused_batch(S, G, w.dt, tidx)
+    assert np.array_equal(again, got)
+
+
+def test_l4_from_slices_fixture(orc):
+    # tests/tcl4_tests.cpp:36-90 synthetic kernels, ops {I, sx}
+    from .test_oracle_mikx_gw import P2F_FIXTURE, fixture_slices
+    ops = [np.eye(2, dtype=complex), np.array([[0, 1], [1, 0]], complex)]
+    for t in (0, 2, 3):
+        F, C, R = fixture_slices(t=t)
+        L4, GW = taco_amd.assemble_l4_from_slices(F, C, R, P2F_FIXTURE, ops)
+        M, I, K, X = orc.mikx(F, C, R, P2F_FIXTURE, 2)
+        GWr = orc.assemble_gw(M, I, K, X, ops, 2)
+        assert np.abs(GW - GWr).max() <= 1e-12 * np.abs(GWr).max()
+        assert np.abs(L4 - orc.gw_to_l4(GWr, 2)).max() <= 1e-12 * np.abs(GWr).max()
+
+
+def test_l4_error_paths():
+    w = W.spin_boson(Nt=64)
+    S, G = w.system, w.gamma
+    with pytest.raises(TacoOutOfRange):
+        taco_amd.build_TCL4_generator_hip_fused_batch(S, G, w.dt, [3, 64])
+    with pytest.raises(TacoOutOfRange):
+        taco_amd.build_TCL4_generator_hip_fused(S, G, w.dt, 64)
+    with pytest.raises(TacoInvalidArgument):
+        taco_amd.build_TCL4_generator_hip_fused_batch(S, G, w.dt, [1], method=taco_amd.FCRMethod.Direct)
+    with pytest.raises(TacoInvalidArgument):
+        taco_amd.build_TCL4_generator_hip_fused_batch(S, G, -1.0, [1])
+    with pytest.raises(TacoInvalidArgument):
+        taco_amd.build_TCL4_generator_hip_fused_batch(S, G[:, :2], w.dt, [1])
+    import copy
+    S2 = copy.deepcopy(S)
+    S2.pair_to_freq[0, 1] = -1
+    with pytest.raises(TacoError, match="pair_to_freq"):
+        taco_amd.build_TCL4_generator_hip_fused_batch(S2, G, w.dt, [1])
+    S3 = copy.deepcopy(S)
+    S3.A_eig = []
+    with pytest.raises(TacoInvalidArgument):
+        taco_amd.build_TCL4_generator_hip_fused_batch(S3, G, w.dt, [1])
+    assert taco_amd.build_TCL4_generator_hip_fused_batch(S, G, w.dt, [5]).shape == (1, 4, 4)
+
+
+def test_device_cr_sincos_bitwise_equals_host():
+    import ctypes
+    from taco_amd import _lib
+    ctx = taco_amd.context(0)
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.uniform(-1e5, 1e5, 500000), rng.uniform(-3, 3, 200000), [0.0, -0.0, np.pi]])
+    out = []
+    for h in (ctx.handle, None):
+        s = np.zeros_like(x)
+        c = np.zeros_like(x)
+        _lib.check(_lib.lib().taco_cr_sincos(h, _lib.dptr(x), x.size, _lib.dptr(s), _lib.dptr(c)))
+        out.append((s, c))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+def test_fft_path_is_bitwise_at_t0(orc):
+    """With identical phases, F(0) and C(0) (pure FFT rounding noise, since
+    Gamma(., 0) = 0 zeroes the prefix term) equal the oracle's bit for bit:
+    the three-pass FFT performs the reference's butterflies exactly."""
+    w = W.spin_boson()
+    S, G = w.system, w.gamma
+    got = taco_amd.compute_triple_kernels(S, G, w.dt, keep=3)
+    ref = orc.triple_kernels(G, S.omegas, S.mirror, w.dt, keep=3)
+    assert np.array_equal(got.F[..., 0], ref[0][..., 0])
+    assert np.array_equal(got.C[..., 0], ref[1][..., 0])
+    assert np.array_equal(got.R[..., 0], ref[2][..., 0])

@@ -1,0 +1,68 @@
+"""GPU parity of RK4 propagation and the TCL2+TCL4 driver composition."""
+import numpy as np
+import pytest
+from scipy.linalg import expm
+
+import taco_amd
+from taco_amd import workloads as W
+
+from .metrics import max_rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("D,steps", [(4, 50), (16, 200), (64, 30)])
+def test_rk4_matches_oracle(orc, D, steps):
+    rng = np.random.default_rng(D)
+    base = (rng.standard_normal((D, D)) + 1j * rng.standard_normal((D, D))) * 0.3
+    Ls = np.array([base * np.cos(0.01 * s) for s in range(steps + 1)])
+    r0 = rng.standard_normal(D) + 1j * rng.standard_normal(D)
+    got_samples = []
+    r = taco_amd.propagate_rk4_dense(Ls, r0, 0.0, 0.01, on_sample=lambda t, v: got_samples.append(v.copy()),
+                                     sample_every=7)
+    rr, ref_samples = orc.propagate(Ls, r0, 0.01, every=7)
+    assert np.abs(r - rr).max() <= 1e-12 * np.abs(rr).max()
+    assert len(got_samples) == len(ref_samples)
+    assert max_rel_err(np.array(got_samples), ref_samples)[0] <= 1e-12
+
+
+def test_rk4_closed_forms():
+    D = 4
+    L = np.diag([0.1 - 0.03j, -0.07 + 0.05j, 0.02, -0.04 - 0.02j])
+    L[0, 2], L[2, 0], L[1, 3] = 0.03 + 0.01j, -0.02 + 0.02j, 0.01 - 0.04j
+    r0 = np.array([1.0, 0.5 - 0.2j, -0.7 + 0.1j, 0.1 + 0.3j])
+    r = taco_amd.propagate_rk4_dense(np.repeat(L[None], 501, 0), r0, 0.0, 0.0025)
+    expect = expm(1.25 * L) @ r0
+    assert np.linalg.norm(r - expect) / max(1, np.linalg.norm(expect)) <= 1e-6
+    ts = np.arange(101) * 0.01
+    Ls = np.array([t * np.eye(3) for t in ts], complex)
+    Lh = np.array([(t + 0.005) * np.eye(3) for t in ts[:-1]], complex)
+    r0 = np.array([0.3 + 0.1j, -0.2 + 0.4j, 0.9 - 0.3j])
+    r = taco_amd.propagate_rk4_dense(Ls, r0, 0.0, 0.01, L_half_series=Lh)
+    assert np.linalg.norm(r - np.exp(0.5) * r0) / np.linalg.norm(r0) <= 1e-6
+
+
+@pytest.mark.parametrize("order", [0, 2, 4])
+def test_tcl4_propagate_spin_boson(orc, order):
+    w = W.spin_boson()
+    S = w.system
+    r0 = np.zeros(4, complex)
+    r0[0] = 1.0
+    got = taco_amd.tcl4_propagate(S, w.gamma, w.dt, w.steps, r0, order=order)
+    ref = orc.tcl4_propagate(w.gamma, S.omegas, S.mirror, S.pair_to_freq, S.eps, S.A_eig, w.dt, w.steps, r0,
+                             order=order)
+    e, _ = max_rel_err(got, ref)
+    print(f"order {order}: max per-t rho rel err {e:.3e}")
+    assert e <= 1e-10
+    tr = got[:, 0] + got[:, 3]
+    assert np.abs(tr - 1).max() < 1e-12
+
+
+def test_tcl4_propagate_spin_pair_degenerate(orc):
+    w = W.spin_pair(Nt=4096, steps=400)
+    S = w.system
+    r0 = np.zeros(16, complex)
+    r0[0] = 1.0
+    got = taco_amd.tcl4_propagate(S, w.gamma, w.dt, w.steps, r0)
+    ref = orc.tcl4_propagate(w.gamma, S.omegas, S.mirror, S.pair_to_freq, S.eps, S.A_eig, w.dt, w.steps, r0)
+    assert max_rel_err(got, ref)[0] <= 1e-10

@@ -1,0 +1,88 @@
+"""Pins of the oracle's system builder, C(t) and Gamma producers (CPU only).
+
+References: system.hpp:37-226, tcl4.cpp:40-101, correlation_fft.hpp:154-214,
+gamma.hpp:166-213.
+"""
+import numpy as np
+import pytest
+
+from taco_amd import workloads as W
+
+
+@pytest.mark.parametrize("H,nf", [
+    (0.5 * W.SZ + 0.5 * W.SX, 3),
+    (W.kron(0.5 * W.SZ + 0.5 * W.SX, W.I2) + W.kron(W.I2, 0.5 * W.SZ + 0.5 * W.SX), 5),
+])
+def test_system_build_known_bucket_counts(orc, H, nf):
+    S = orc.system_build(H, [np.eye(H.shape[0], dtype=complex)])
+    assert S["nf"] == nf
+    U = S["U"]
+    assert np.abs(H @ U - U @ np.diag(S["eps"])).max() < 1e-13
+    assert np.abs(U.conj().T @ U - np.eye(len(U))).max() < 1e-13
+    assert np.all(np.diff(S["eps"]) >= 0)
+    om = S["omegas"]
+    assert np.all(np.diff(om) > 0)
+    # mirror: omega[mirror[b]] = -omega[b]
+    assert np.abs(om[S["mirror"]] + om).max() < 1e-9
+    assert abs(om[S["zero_index"]]) < 1e-9
+    # every pair is bucketed by its Bohr frequency
+    eps = S["eps"]
+    for m in range(len(eps)):
+        for n in range(len(eps)):
+            assert abs(om[S["pair_to_freq"][m, n]] - (eps[m] - eps[n])) <= 1e-9
+
+
+@pytest.mark.parametrize("d,nf", [(2, 3), (3, 7), (4, 13), (6, 31), (8, 57)])
+def test_gue_bucket_counts(orc, d, nf):
+    rng = W.SplitMix64(100 + d)
+    H = W.gue(d, rng)
+    S = orc.system_build(H, [W.random_hermitian_unit(d, rng)])
+    assert S["nf"] == nf == d * d - d + 1
+    A = S["A_eig"][0]
+    assert np.abs(A - A.conj().T).max() < 1e-13
+
+
+def test_tfim_bucket_count(orc):
+    H, A = W.tfim(4, 1.0, 0.77)
+    assert orc.system_build(H, [A])["nf"] == 81
+
+
+def test_gamma_trapz_geometric_closed_form(orc):
+    # C(t) = exp(-g t): trapz sum is (dt/2)(1 + z)(1 - z^k)/(1 - z), z = e^{(i w - g) dt}
+    g, dt, n = 0.7, 0.01, 4000
+    C = np.exp(-g * np.arange(n) * dt).astype(complex)
+    om = np.array([-2.0, 0.0, 3.5])
+    G = orc.gamma_trapz(C, dt, om)
+    k = np.arange(n)
+    for b, w in enumerate(om):
+        z = np.exp((1j * w - g) * dt)
+        ref = 0.5 * dt * (1 + z) * (1 - z ** k) / (1 - z)
+        assert np.abs(G[:, b] - ref).max() < 1e-12 * np.abs(ref).max()
+    assert np.all(G[0] == 0)
+
+
+@pytest.mark.parametrize("beta", [0.5, np.inf])
+def test_bcf_matches_bruteforce_dft_of_kms_spectrum(orc, beta):
+    N, dt, alpha, wc = 31, 0.05, 0.3, 4.0
+    Nt = N + 1
+    Nfft = 64
+    domega = 2 * np.pi / (dt * Nfft)
+    J = lambda w: alpha * w * np.exp(-w / wc) if w > 0 else 0.0  # noqa: E731
+    S = np.zeros(Nfft, complex)
+    wNy = np.pi / dt
+    if np.isinf(beta):
+        for k in range(1, Nfft // 2):
+            S[k] = np.pi / 2 * J(k * domega)
+        S[Nfft // 2] = 0.25 * np.pi * J(wNy)
+    else:
+        r = np.exp(-beta * domega)
+        for k in range(1, Nfft // 2):
+            kms = r ** k
+            S[k] = np.pi / 2 * J(k * domega) / (1 - kms)
+            S[Nfft - k] = S[k] * kms
+        S[0] = 0.5 * np.pi / 2 * J(domega) * (1 + r) / (1 - r)
+        S[Nfft // 2] = 0.25 * np.pi * J(wNy) * (1 + 2 / (np.exp(beta * wNy) - 1))
+    n = np.arange(Nfft)
+    ref = (np.exp(-2j * np.pi * np.outer(n, n) / Nfft) @ S)[:Nt] * domega / np.pi
+    got = orc.bcf_fft(N, dt, beta, 0, alpha, wc)
+    assert np.abs(got - ref).max() < 1e-13 * np.abs(ref).max()
