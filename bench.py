@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""Benchmark of the TCL4 hot path on one MI355X.
+
+Workload (default): the plain GPU drop-in at scale — d = 4 GUE Hamiltonian
+(seed 2), A = sz(x)I + I(x)sz, Ohmic bath (alpha 0.1, omega_c 10, beta 0.5),
+bath grid N = 200000 (Nt = 200001 Gamma rows, Nfft = 2^19), dt = 0.002,
+L4(t) at t = 0..10000.  One step is one fused batch build
+(F/C/R -> MIKX -> GW -> L4 for all 10001 times) from inputs already resident
+in HBM.  The value is L4 time-points per second.
+
+The CPU baseline leg times the oracle (the C++ restatement of the reference
+serial/OpenMP path, with the platform libm phases the reference calls) on the
+host cores, and uses the oracle as the checker of the GPU result (per-t
+relative Frobenius error of L4 against the correctly rounded-phase oracle).
+
+Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "L4 time-points/s batched vs plain at ≤1e-10 rel err; adjoint grads/s"
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--workload", default="gue_pair", choices=["gue_pair", "published", "spin_boson"])
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / checker leg")
+    p.add_argument("--cpu-threads", type=int, default=0)
+    p.add_argument("--reduced", action="store_true", help="small grid (for a quick functional run)")
+    return p.parse_args()
+
+
+def make_workload(name: str, reduced: bool):
+    from taco_amd import workloads as W
+
+    if name == "gue_pair":
+        return W.gue_pair(N_bath=20000, tmax=1000) if reduced else W.gue_pair()
+    if name == "published":
+        return W.published(N_bath=20000, tmax=2000) if reduced else W.published()
+    return W.spin_boson()
+
+
+def cpu_baseline_leg(w, L4_gpu: np.ndarray, threads: int) -> tuple[dict, dict]:
+    """Time the oracle on the host cores and check the GPU result with it."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from metrics import per_t_rel_err, ref_style_err
+    from oracle import oracle
+
+    S, G = w.system, w.gamma
+    args = (G, S.omegas, S.mirror, S.pair_to_freq, S.A_eig, w.dt, w.tidx)
+    with oracle.phase_mode(oracle.PHASE_LIBM):
+        t0 = time.perf_counter()
+        ref_libm = oracle.l4_series(*args, nthreads=threads)
+        cpu_s = time.perf_counter() - t0
+    with oracle.phase_mode(oracle.PHASE_CR):
+        ref_cr = oracle.l4_series(*args, nthreads=threads)
+    e_cr, nfl = per_t_rel_err(L4_gpu, ref_cr)
+    e_libm, _ = per_t_rel_err(L4_gpu, ref_libm)
+    e_ref_libm_vs_cr, _ = per_t_rel_err(ref_libm, ref_cr)
+    ntau = len(w.tidx)
+    baseline = {
+        "value": ntau / cpu_s,
+        "unit": "L4 time-points/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"full workload: all {S.nf ** 3} F/C/R triples (Nfft {oracle.fcr_nfft(G.shape[0])}) and "
+                  f"{ntau} L4 times, oracle with libm phases, OpenMP {threads} threads, {cpu_s:.1f} s",
+        "seconds": cpu_s,
+    }
+    parity = {
+        "max_per_t_rel_err_vs_oracle_cr_phases": float(e_cr.max()),
+        "floored_t": nfl,
+        "ref_style_max_abs_over_max1": ref_style_err(L4_gpu, ref_cr),
+        "max_per_t_rel_err_vs_oracle_libm_phases": float(e_libm.max()),
+        "max_per_t_rel_err_vs_oracle_libm_phases_t_ge_20": float(e_libm[20:].max()) if ntau > 20 else None,
+        "oracle_libm_vs_cr_phases_max_per_t": float(e_ref_libm_vs_cr.max()),
+        "pass_1e-10": bool(e_cr.max() <= 1e-10),
+    }
+    return baseline, parity
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this engine runs on one GPU; --gpus {a.gpus} is not supported (use --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import taco_amd
+    from taco_amd import tcl4
+
+    assert torch.cuda.is_available(), "bench needs an MI355X"
+    w = make_workload(a.workload, a.reduced)
+    S, G = w.system, w.gamma
+    ntau = len(w.tidx)
+    dsys = tcl4.DeviceSystem(S, G, 0)
+    out = torch.empty((ntau, S.N ** 4), dtype=torch.complex128, device="cuda:0")
+    print(f"[bench] workload {w.name}: d={S.N} nf={S.nf} Nt={G.shape[0]} ntau={ntau}; warmup {a.warmup}",
+          file=sys.stderr, flush=True)
+    for _ in range(a.warmup):
+        tcl4.build_l4_dev(dsys, w.dt, w.tidx, out)
+    torch.cuda.synchronize()
+    with tcl4.Profile(0) as prof:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            tcl4.build_l4_dev(dsys, w.dt, w.tidx, out)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        kern = prof.read()
+    print(f"[bench] timed {a.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+    ms_per_step = 1000.0 * elapsed / a.steps
+    value = ntau * a.steps / elapsed
+    kernels = {}
+    for name, k in kern.items():
+        kernels[name] = {
+            "ms_per_step": k["ms"] / a.steps,
+            "launches_per_step": k["launches"] / a.steps,
+            "avg_launch_us": 1000.0 * k["ms"] / k["launches"],
+            "achieved_GBps": k["algo_bytes"] / (k["ms"] * 1e-3) / 1e9 if k["ms"] > 0 else None,
+        }
+    dom = max(kern, key=lambda n: kern[n]["ms"])
+    kd = kern[dom]
+    achieved = kd["algo_bytes"] / (kd["ms"] * 1e-3) / 1e9
+    roofline = {
+        "kernel": dom,
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": PEAK_HBM_GBS,
+        "unit": "GB/s",
+        "frac": achieved / PEAK_HBM_GBS,
+        "traffic": None,
+        "algo_bytes_per_launch": kd["algo_bytes"] / kd["launches"],
+        "avg_launch_us": 1000.0 * kd["ms"] / kd["launches"],
+    }
+    L4 = out.cpu().numpy().reshape(ntau, S.N ** 2, S.N ** 2).transpose(0, 2, 1)
+    cpu, parity = None, None
+    if not a.no_cpu:
+        threads = a.cpu_threads or len(os.sched_getaffinity(0))
+        print(f"[bench] cpu baseline / checker leg on {threads} threads", file=sys.stderr, flush=True)
+        cpu, parity = cpu_baseline_leg(w, L4, threads)
+        print(f"[bench] cpu leg done in {cpu['seconds']:.1f} s", file=sys.stderr, flush=True)
+    nfft = 1 << int(np.ceil(np.log2(2 * G.shape[0] - 1)))
+    rec = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "L4 time-points/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{w.name}: plain per-system fused L4 build, d={S.N}, nf={S.nf}, Nt={G.shape[0]}, "
+                        f"Nfft={nfft}, dt={w.dt}, tidx 0..{w.tidx[-1]} ({ntau} L4 points), inputs resident in HBM",
+            "path": "plain",
+            "phases": "correctly rounded",
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "parity": parity,
+        "kernels": kernels,
+    }
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

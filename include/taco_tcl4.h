@@ -62,6 +62,17 @@ void* taco_ctx_stream(taco_ctx* ctx);
 void taco_set_fcr_fft_pad_factor(size_t factor);
 size_t taco_get_fcr_fft_pad_factor(void);
 
+/* Per-kernel HIP-event profiler of a context (off by default).  Kernel ids:
+ * 0 phase A, 1 phase B, 2 phase C, 3 F/C/R combine, 4 fused L4,
+ * 5 per-bucket spectra, 6 TCL2, 7 RK4.  taco_profile_get returns the summed
+ * device time, launch count and algorithmic HBM bytes (compulsory reads and
+ * writes of each launch) since the last reset.  No reference counterpart:
+ * it replaces the reference's one-shot TACO_CUDA_*_PROFILE timers
+ * (tcl4_fcr_kernels.cu:135-138, tcl4_assemble_cuda.cu:68-71). */
+int taco_profile_enable(taco_ctx* ctx, int on);
+int taco_profile_reset(taco_ctx* ctx);
+int taco_profile_get(taco_ctx* ctx, int kernel, double* total_ms, long* launches, double* algo_bytes);
+
 /* Correctly rounded sin/cos of each x[i] (|x| < 2^30): the phase routine of
  * the F/C/R kernels, exp(+-i*omega*(k*dt)) = (c, +-s).  The reference calls
  * std::exp of an imaginary argument (tcl4_kernels.cpp:116-120), whose last

@@ -31,6 +31,9 @@ SIGNATURES = {
     "taco_ctx_stream": (c_void_p, [c_void_p]),
     "taco_set_fcr_fft_pad_factor": (None, [c_size_t]),
     "taco_cr_sincos": (c_int, [c_void_p, _dp, c_size_t, _dp, _dp]),
+    "taco_profile_enable": (c_int, [c_void_p, c_int]),
+    "taco_profile_reset": (c_int, [c_void_p]),
+    "taco_profile_get": (c_int, [c_void_p, c_int, _dp, POINTER(ctypes.c_long), _dp]),
     "taco_get_fcr_fft_pad_factor": (c_size_t, []),
     "taco_system_build": (c_int, [_dp, c_int, _dp, c_int, c_double, _dp, _dp, _dp, _ip, _dp, _ip, _ip, _ip]),
     "taco_bcf_fft": (c_int, [c_size_t, c_double, c_int, c_double, c_double, c_double, _dp]),

@@ -287,6 +287,7 @@ __device__ __forceinline__ double block_scan(double v, double* wsum, double& tot
 struct CombineArgs {
     int out_mode;        // 0: time-major slices at t2pos; 1: series-major [tri][T]
     const int* t2pos;    // [T], -1 when t is not requested (mode 0)
+    long npos;           // number of kept times (mode 0)
     long nf3;
     double2* Fo;
     double2* Co;
@@ -655,10 +656,64 @@ struct Twiddles {
 
 }  // namespace taco_engine
 
+namespace taco_engine {
+// Kernel classes timed by the optional per-launch HIP-event profiler.
+enum KernelId { K_PHASE_A = 0, K_PHASE_B, K_PHASE_C, K_COMBINE, K_L4, K_SPECTRA, K_TCL2, K_RK4, K_COUNT };
+
+struct Profiler {
+    bool on = false;
+    struct Pending {
+        int id;
+        hipEvent_t a, b;
+        double bytes;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    double ms[K_COUNT] = {};
+    double bytes[K_COUNT] = {};
+    long launches[K_COUNT] = {};
+    hipEvent_t get() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        HIP_OK(hipEventCreate(&e));
+        return e;
+    }
+    void resolve() {
+        for (auto& p : pending) {
+            HIP_OK(hipEventSynchronize(p.b));
+            float t = 0.f;
+            HIP_OK(hipEventElapsedTime(&t, p.a, p.b));
+            ms[p.id] += t;
+            bytes[p.id] += p.bytes;
+            launches[p.id] += 1;
+            pool.push_back(p.a);
+            pool.push_back(p.b);
+        }
+        pending.clear();
+    }
+    void reset() {
+        resolve();
+        for (int k = 0; k < K_COUNT; ++k) ms[k] = bytes[k] = 0, launches[k] = 0;
+    }
+    ~Profiler() {
+        for (auto& p : pending) {
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
+        }
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+}  // namespace taco_engine
+
 struct taco_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    taco_engine::Profiler prof;
     std::map<int, std::unique_ptr<taco_engine::Twiddles>> tw;
     taco_engine::DevBuf gamma, omegas, mirror, p2f, ops, spec, inter1, inter2, conv, slices, t2pos, l4, gw, misc,
         rk, eps;
@@ -674,6 +729,21 @@ struct DeviceGuard {
     }
     ~DeviceGuard() { (void)hipSetDevice(prev); }
 };
+
+// Brackets a launch with profiler events on the context stream when enabled.
+// `bytes` is the launch's algorithmic HBM traffic (compulsory reads + writes).
+template <class Fn>
+static void timed(taco_ctx* c, int id, double bytes, Fn&& launch) {
+    if (!c->prof.on) {
+        launch();
+        return;
+    }
+    hipEvent_t a = c->prof.get(), b = c->prof.get();
+    HIP_OK(hipEventRecord(a, c->stream));
+    launch();
+    HIP_OK(hipEventRecord(b, c->stream));
+    c->prof.pending.push_back({id, a, b, bytes});
+}
 
 static const Twiddles& twiddles_for(taco_ctx* c, int L) {
     auto it = c->tw.find(L);
@@ -784,17 +854,39 @@ static void run_fcr(taco_ctx* c, const FcrPlan& pl, const CombineArgs& out) {
     // Spectra of every Gamma column and its conjugate.
     P.tri0 = 0;
     P.njobs = static_cast<int>(spec_jobs);
-    hipLaunchKernelGGL(k_phase_a, dim3(static_cast<unsigned>(spec_jobs * bpjA)), dim3(kBlock), 0, c->stream, P, 1);
-    hipLaunchKernelGGL(k_phase_b, dim3(static_cast<unsigned>(spec_jobs * bpjB)), dim3(kBlock), 0, c->stream, P, 1);
+    const double cn = static_cast<double>(n) * 16.0;      // one complex series of length Nfft
+    const double cNt = static_cast<double>(pl.Nt) * 16.0;
+    const double cT = static_cast<double>(pl.T) * 16.0;
+    timed(c, K_SPECTRA, spec_jobs * (cNt + 3 * cn), [&] {
+        hipLaunchKernelGGL(k_phase_a, dim3(static_cast<unsigned>(spec_jobs * bpjA)), dim3(kBlock), 0, c->stream, P,
+                           1);
+        hipLaunchKernelGGL(k_phase_b, dim3(static_cast<unsigned>(spec_jobs * bpjB)), dim3(kBlock), 0, c->stream, P,
+                           1);
+    });
     HIP_OK(hipGetLastError());
     for (long tri0 = 0; tri0 < nf3; tri0 += chunk) {
         const long nj = std::min(chunk, nf3 - tri0);
         P.tri0 = tri0;
         P.njobs = static_cast<int>(nj);
-        hipLaunchKernelGGL(k_phase_a, dim3(static_cast<unsigned>(nj * bpjA)), dim3(kBlock), 0, c->stream, P, 0);
-        hipLaunchKernelGGL(k_phase_b, dim3(static_cast<unsigned>(nj * bpjB)), dim3(kBlock), 0, c->stream, P, 0);
-        hipLaunchKernelGGL(k_phase_c, dim3(static_cast<unsigned>(2 * nj * bpjC)), dim3(kBlock), 0, c->stream, P);
-        hipLaunchKernelGGL(k_combine, dim3(static_cast<unsigned>(nj)), dim3(kBlock), 0, c->stream, P, out);
+        // phase A: read the Gamma column, write the half-transformed signal
+        timed(c, K_PHASE_A, nj * (cNt + cn), [&] {
+            hipLaunchKernelGGL(k_phase_a, dim3(static_cast<unsigned>(nj * bpjA)), dim3(kBlock), 0, c->stream, P, 0);
+        });
+        // phase B: read it and two spectra, write two half-inverse products
+        timed(c, K_PHASE_B, nj * 5 * cn, [&] {
+            hipLaunchKernelGGL(k_phase_b, dim3(static_cast<unsigned>(nj * bpjB)), dim3(kBlock), 0, c->stream, P, 0);
+        });
+        // phase C: read both, write the two convolutions for t < T
+        timed(c, K_PHASE_C, nj * 2 * (cn + cT), [&] {
+            hipLaunchKernelGGL(k_phase_c, dim3(static_cast<unsigned>(2 * nj * bpjC)), dim3(kBlock), 0, c->stream,
+                               P);
+        });
+        // combine: read three Gamma columns and two convolutions over t < T,
+        // write F/C/R at the kept times
+        const double kept = out.out_mode == 0 ? static_cast<double>(out.npos) : static_cast<double>(pl.T);
+        timed(c, K_COMBINE, nj * (5 * cT + 3 * 16.0 * kept), [&] {
+            hipLaunchKernelGGL(k_combine, dim3(static_cast<unsigned>(nj)), dim3(kBlock), 0, c->stream, P, out);
+        });
         HIP_OK(hipGetLastError());
     }
 }
@@ -812,7 +904,11 @@ static void launch_l4(taco_ctx* c, const L4Args& A, long npos) {
     if (lds > 64 * 1024)
         HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_l4), hipFuncAttributeMaxDynamicSharedMemorySize,
                                    static_cast<int>(lds)));
-    hipLaunchKernelGGL(k_l4, dim3(static_cast<unsigned>(npos)), dim3(kBlock), lds, c->stream, A);
+    const double N4 = static_cast<double>(A.N) * A.N * A.N * A.N;
+    // per time: read the three nf^3 slices, write one N^4 block
+    timed(c, K_L4, npos * (3.0 * A.nf3 + N4) * 16.0, [&] {
+        hipLaunchKernelGGL(k_l4, dim3(static_cast<unsigned>(npos)), dim3(kBlock), lds, c->stream, A);
+    });
     HIP_OK(hipGetLastError());
 }
 
@@ -886,6 +982,7 @@ static TimePlan l4_core(taco_ctx* c, const double2* d_gamma, std::size_t Nt, int
     CombineArgs out{};
     out.out_mode = 0;
     out.t2pos = c->t2pos.as<int>();
+    out.npos = npos;
     out.nf3 = static_cast<long>(nf3);
     double2* sl = c->slices.as<double2>();
     out.Fo = sl;
@@ -975,6 +1072,34 @@ int taco_ctx_destroy(taco_ctx* ctx) {
 }
 
 void* taco_ctx_stream(taco_ctx* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
+
+int taco_profile_enable(taco_ctx* ctx, int on) {
+    return guarded([&] {
+        check_ctx(ctx);
+        ctx->prof.resolve();
+        ctx->prof.on = on != 0;
+    });
+}
+
+int taco_profile_reset(taco_ctx* ctx) {
+    return guarded([&] {
+        check_ctx(ctx);
+        DeviceGuard g(ctx->device);
+        ctx->prof.reset();
+    });
+}
+
+int taco_profile_get(taco_ctx* ctx, int kernel, double* total_ms, long* launches, double* algo_bytes) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (kernel < 0 || kernel >= K_COUNT) throw Error(TACO_ERR_RANGE, "unknown kernel id");
+        DeviceGuard g(ctx->device);
+        ctx->prof.resolve();
+        if (total_ms) *total_ms = ctx->prof.ms[kernel];
+        if (launches) *launches = ctx->prof.launches[kernel];
+        if (algo_bytes) *algo_bytes = ctx->prof.bytes[kernel];
+    });
+}
 
 int taco_cr_sincos(taco_ctx* ctx, const double* x, size_t n, double* s, double* c) {
     return guarded([&] {

@@ -433,3 +433,76 @@ def tcl4_propagate(system: System, gamma_series, dt: float, steps: int, r0, orde
                                     dptr(of) if of.size else None, len(system.A_eig), dt, steps, order, dptr(r),
                                     dptr(samples)))
     return samples
+
+
+# --- device-resident inputs (torch tensors as HBM buffers) -----------------------
+
+class DeviceSystem:
+    """System arrays and a Gamma table resident on one GPU.
+
+    torch only provides the device buffers; every kernel is the engine's."""
+
+    def __init__(self, system: System, gamma_series, device: int = 0):
+        import torch
+
+        g = _gamma_checked(system, gamma_series)
+        self.device = device
+        self.N = system.N
+        self.nf = system.nf
+        self.Nt = g.shape[0]
+        self.num_ops = len(system.A_eig)
+        dev = torch.device("cuda", device)
+        om, mi, pf, of = _system_arrays(system)
+        self.gamma = torch.from_numpy(_colmajor(g).ravel().copy()).to(dev)  # [nf][Nt]
+        self.omegas = torch.from_numpy(om.copy()).to(dev)
+        self.mirror = torch.from_numpy(mi.copy()).to(dev)
+        self.pair_to_freq = torch.from_numpy(pf.copy()).to(dev)
+        self.ops = torch.from_numpy(of.copy()).to(dev)
+        torch.cuda.synchronize(dev)
+
+
+def build_l4_dev(dsys: DeviceSystem, dt: float, time_indices: Sequence[int], out=None,
+                 method: FCRMethod = FCRMethod.Convolution):
+    """L4(t) for t in time_indices into a device tensor [ntau, N^4] (column-major blocks)."""
+    import torch
+
+    ctx = context(dsys.device)
+    tid = np.ascontiguousarray(np.asarray(list(time_indices), dtype=np.int64), np.uint64)
+    ntau = tid.size if tid.size else dsys.Nt
+    N4 = dsys.N ** 4
+    if out is None:
+        out = torch.empty((ntau, N4), dtype=torch.complex128, device=torch.device("cuda", dsys.device))
+    assert out.is_contiguous() and out.numel() == ntau * N4 and out.dtype == torch.complex128
+    check(lib().taco_l4_batch_dev(ctx.handle, c_void_p(dsys.gamma.data_ptr()), dsys.Nt, dsys.nf,
+                                  c_void_p(dsys.omegas.data_ptr()), c_void_p(dsys.mirror.data_ptr()),
+                                  c_void_p(dsys.pair_to_freq.data_ptr()), dsys.N, c_void_p(dsys.ops.data_ptr()),
+                                  dsys.num_ops, dt, int(method), sptr(tid) if tid.size else None, tid.size,
+                                  c_void_p(out.data_ptr()), None))
+    return out
+
+
+KERNEL_NAMES = ["k_phase_a", "k_phase_b", "k_phase_c", "k_combine", "k_l4", "spectra", "k_tcl2", "k_rk4"]
+
+
+class Profile:
+    """Per-kernel HIP-event timing of one context (taco_profile_* of the C ABI)."""
+
+    def __init__(self, device: int = 0):
+        self.ctx = context(device)
+
+    def __enter__(self):
+        check(lib().taco_profile_reset(self.ctx.handle))
+        check(lib().taco_profile_enable(self.ctx.handle, 1))
+        return self
+
+    def __exit__(self, *exc):
+        check(lib().taco_profile_enable(self.ctx.handle, 0))
+
+    def read(self) -> dict:
+        out = {}
+        for k, name in enumerate(KERNEL_NAMES):
+            ms, n, b = c_double(0), ctypes.c_long(0), c_double(0)
+            check(lib().taco_profile_get(self.ctx.handle, k, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(b)))
+            if n.value:
+                out[name] = {"ms": ms.value, "launches": n.value, "algo_bytes": b.value}
+        return out
