@@ -35,6 +35,10 @@ constexpr double kP1 = 0x1.1a62633145c07p-54;
 constexpr double kP2 = -0x1.f1976b7ed8fbcp-110;
 constexpr double kTwoOverPi = 0x1.45f306dc9c883p-1;
 constexpr int kTabN = 52;  // sin(j/64), cos(j/64) for j = 0..51, as (hi, lo, hi, lo)
+// 1/6, 1/24, 1/120 as double-doubles (hi = RN(1/n), lo = RN(1/n - hi)).
+constexpr double kInv6Hi = 0x1.5555555555555p-3, kInv6Lo = 0x1.5555555555555p-57;
+constexpr double kInv24Hi = 0x1.5555555555555p-5, kInv24Lo = 0x1.5555555555555p-59;
+constexpr double kInv120Hi = 0x1.1111111111111p-7, kInv120Lo = 0x1.1111111111111p-63;
 
 struct dd {
     double hi, lo;
@@ -103,11 +107,11 @@ TACO_HD inline void sincos_cr(double x, double* s_out, double* c_out, const doub
     const double x2 = d2.hi;
     const double x3 = x2 * x2 * x2;
     // sin d = d - d*v, v = d2/6 - d4/120 + d2^3/5040 - d2^4/362880
-    dd v = dd_add(dd_div_d(d2, 6.0), dd_neg(dd_div_d(d4, 120.0)));
+    dd v = dd_add(dd_mul(d2, dd{kInv6Hi, kInv6Lo}), dd_neg(dd_mul(d4, dd{kInv120Hi, kInv120Lo})));
     v = dd_add(v, dd{x3 * (1.0 / 5040.0) - x3 * x2 * (1.0 / 362880.0), 0.0});
     const dd sd = dd_add(d, dd_neg(dd_mul(d, v)));
     // cos d = 1 - w, w = d2/2 - d4/24 + d2^3/720 - d2^4/40320 + d2^5/3628800
-    dd w = dd_add(dd{d2.hi * 0.5, d2.lo * 0.5}, dd_neg(dd_div_d(d4, 24.0)));
+    dd w = dd_add(dd{d2.hi * 0.5, d2.lo * 0.5}, dd_neg(dd_mul(d4, dd{kInv24Hi, kInv24Lo})));
     w = dd_add(w, dd{x3 * (1.0 / 720.0) - x3 * x2 * (1.0 / 40320.0) + x3 * x2 * x2 * (1.0 / 3628800.0), 0.0});
     const dd cd = dd_add(dd{1.0, 0.0}, dd_neg(w));
     dd sr = sd, cr = cd;

@@ -80,49 +80,126 @@ __device__ __forceinline__ double2 cexpi(double arg) {
 }
 
 constexpr int kBlock = 256;
-constexpr int kGroupElems = 2048;  // complex values per workgroup in the FFT phases
-constexpr int kMaxBfPerThread = kGroupElems / 2 / kBlock;
 
-// Radix-2 Stockham stages on G groups of 2^lg values stored at buf[g*S + pos].
-// Local stage sp uses twiddle twp[g*gstride + (2^(sp-1) - 1) + kk] (conjugated
-// for the inverse transform).  Position mapping per stage: inputs u + 2hk and
-// u + h + 2hk, outputs u + hk and u + hk + 2^(lg-1), h = 2^(lg-sp).
-__device__ void group_stages(double2* buf, int lg, int G, int S, const double2* __restrict__ twp, long gstride,
-                             bool conj) {
+// FFT phases: 512-thread workgroups on a tile of 8192 complex values (G
+// groups of 2^lg) in LDS.  Stages run as radix-2^r register passes: a thread
+// loads the 2^r values of a closed butterfly set, performs r radix-2 stages
+// on them (each butterfly exactly u +/- w*x with the stage's table
+// twiddle), and writes them back, Stockham-style.  Two barriers per pass.
+constexpr int kTile = 8192;
+constexpr int kFT = 512;
+constexpr int kPT = kTile / kFT;                       // values per thread
+constexpr int kLds = 9216;                             // LDS tile incl. padding
+
+// Physical LDS slot of (group, position): one pad slot per 16 values and an
+// odd group stride keep strided accesses off a single bank.
+struct TileLayout {
+    int lg, gs, G, GS;
+    __device__ TileLayout(int lg_, int G_) : lg(lg_), gs(1 << lg_), G(G_), GS((1 << lg_) + ((1 << lg_) >> 4) + 1) {}
+    __device__ __forceinline__ int at(int grp, int pos) const { return grp * GS + pos + (pos >> 4); }
+};
+
+__host__ __device__ inline int tile_groups(int lg, int ngroups) {
+    int G = kTile >> lg;
+    if (G > ngroups) G = ngroups;
+    const int GS = (1 << lg) + ((1 << lg) >> 4) + 1;
+    while (G > 1 && G * GS > kLds) G >>= 1;
+    return G;
+}
+
+// Passes of r stages; local stage sp (1-based) uses twiddle
+// twp[grp*gstride + (2^(sp-1) - 1) + kk], conjugated for inverse transforms.
+template <int RL>
+__device__ __forceinline__ void tile_pass(double2* buf, const TileLayout& T, int s_lo,
+                                          const double2* __restrict__ twp, long gstride, bool conj) {
+    constexpr int R = 1 << RL;
+    constexpr int SETS = kPT / R;
+    const int lg = T.lg;
+    const int u0bits = lg - s_lo - RL + 1;
+    const int spg = 1 << (lg - RL);
+    const int nsets = T.G * spg;
+    double2 X[SETS][R];
+#pragma unroll
+    for (int q = 0; q < SETS; ++q) {
+        const int set = threadIdx.x + q * kFT;
+        if (set < nsets) {
+            const int grp = set >> (lg - RL);
+            const int rem = set & (spg - 1);
+            const int u0 = rem & ((1 << u0bits) - 1), k = rem >> u0bits;
+            const int pos0 = u0 + ((k * R) << u0bits);
+#pragma unroll
+            for (int m = 0; m < R; ++m) X[q][m] = buf[T.at(grp, pos0 + (m << u0bits))];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < SETS; ++q) {
+        const int set = threadIdx.x + q * kFT;
+        if (set < nsets) {
+            const int grp = set >> (lg - RL);
+            const int rem = set & (spg - 1);
+            const int u0 = rem & ((1 << u0bits) - 1), k = rem >> u0bits;
+            const double2* tg = twp + grp * gstride;
+#pragma unroll
+            for (int i = 0; i < RL; ++i) {
+                const int h = R >> (i + 1);
+                const double2* Tw = tg + ((1 << (s_lo + i - 1)) - 1);
+                double2 Y[R];
+#pragma unroll
+                for (int j = 0; j < (1 << i); ++j) {
+                    double2 w = Tw[k + (j << (s_lo - 1))];
+                    if (conj) w.y = -w.y;
+#pragma unroll
+                    for (int m = 0; m < h; ++m) {
+                        const double2 E = X[q][m + 2 * h * j];
+                        const double2 O = X[q][m + h + 2 * h * j];
+                        const double2 v = cmul(w, O);
+                        Y[m + h * j] = cadd(E, v);
+                        Y[m + h * j + R / 2] = csub(E, v);
+                    }
+                }
+#pragma unroll
+                for (int m = 0; m < R; ++m) X[q][m] = Y[m];
+            }
+#pragma unroll
+            for (int j = 0; j < R; ++j) buf[T.at(grp, u0 + ((k + (j << (s_lo - 1))) << u0bits))] = X[q][j];
+        }
+    }
+    __syncthreads();
+}
+
+// All lg stages of the tile: ceil(lg/4) passes, stage counts as even as possible.
+__device__ void tile_fft(double2* buf, const TileLayout& T, const double2* __restrict__ twp, long gstride,
+                         bool conj) {
+    const int lg = T.lg;
     if (lg <= 0) return;
-    const int half = 1 << (lg - 1);
-    const int nb = G * half;
-    for (int sp = 1; sp <= lg; ++sp) {
-        const int hsh = lg - sp;
-        const int hmask = (1 << hsh) - 1;
-        const int toff = (1 << (sp - 1)) - 1;
-        double2 r0[kMaxBfPerThread], r1[kMaxBfPerThread];
-        int cnt = 0;
-        for (int bf = threadIdx.x; bf < nb; bf += kBlock, ++cnt) {
-            const int grp = bf >> (lg - 1);
-            const int p = bf & (half - 1);
-            const int u = p & hmask;
-            const int kk = p >> hsh;
-            const int i0 = u + (kk << (hsh + 1));
-            const int base = grp * S;
-            double2 w = twp[grp * gstride + toff + kk];
-            if (conj) w.y = -w.y;
-            const double2 E = buf[base + i0];
-            const double2 O = buf[base + i0 + (1 << hsh)];
-            const double2 v = cmul(w, O);
-            r0[cnt] = cadd(E, v);
-            r1[cnt] = csub(E, v);
+    const int npass = (lg + 3) / 4;
+    const int base = lg / npass, extra = lg % npass;
+    int s_lo = 1;
+    for (int p = 0; p < npass; ++p) {
+        const int RL = base + (p < extra ? 1 : 0);
+        switch (RL) {
+            case 1: tile_pass<1>(buf, T, s_lo, twp, gstride, conj); break;
+            case 2: tile_pass<2>(buf, T, s_lo, twp, gstride, conj); break;
+            case 3: tile_pass<3>(buf, T, s_lo, twp, gstride, conj); break;
+            default: tile_pass<4>(buf, T, s_lo, twp, gstride, conj); break;
         }
-        __syncthreads();
-        cnt = 0;
-        for (int bf = threadIdx.x; bf < nb; bf += kBlock, ++cnt) {
-            const int grp = bf >> (lg - 1);
-            const int p = bf & (half - 1);
-            const int base = grp * S;
-            buf[base + p] = r0[cnt];
-            buf[base + p + half] = r1[cnt];
-        }
-        __syncthreads();
+        s_lo += RL;
+    }
+}
+
+// Block -> (job, group block).  Workgroups are dispatched to the 8 XCDs
+// round-robin by id; when possible each XCD keeps a fixed eighth of the
+// group blocks for every job, so its L2 holds only that eighth of the
+// per-group twiddle tables.
+__device__ __forceinline__ void xcd_map(int bid, int bpj, int& job, int& gb) {
+    if ((bpj & 7) == 0) {
+        const int x = bid & 7, i = bid >> 3, per = bpj >> 3;
+        gb = x + 8 * (i % per);
+        job = i / per;
+    } else {
+        job = bid / bpj;
+        gb = bid % bpj;
     }
 }
 
@@ -154,14 +231,14 @@ __device__ __forceinline__ void decode_triple(long tri, int nf, int& i, int& j, 
 }
 
 // Phase A: mode 0 triples (input g_i * phase_minus), mode 1 spectra (g_b or conj g_b).
-__global__ __launch_bounds__(kBlock) void k_phase_a(FcrArgs P, int mode) {
-    __shared__ double2 buf[kGroupElems];
-    const int lg = P.a, gs = 1 << lg;
+__global__ __launch_bounds__(kFT) void k_phase_a(FcrArgs P, int mode) {
+    __shared__ double2 buf[kLds];
     const int ngroups = 1 << P.b;
-    const int G = min(kGroupElems >> lg, ngroups);
-    const int bpj = ngroups / G;
-    const int job = blockIdx.x / bpj;
-    const int r0b = (blockIdx.x % bpj) * G;
+    const TileLayout Tl(P.a, tile_groups(P.a, ngroups));
+    const int G = Tl.G, gs = Tl.gs;
+    int job, gb;
+    xcd_map(blockIdx.x, ngroups / G, job, gb);
+    const int r0b = gb * G;
     const double2* src;
     double omega = 0.0;
     bool conj_in = false, phase = false;
@@ -176,7 +253,7 @@ __global__ __launch_bounds__(kBlock) void k_phase_a(FcrArgs P, int mode) {
         conj_in = (job & 1) != 0;
     }
     const int total = G * gs;
-    for (int e = threadIdx.x; e < total; e += kBlock) {
+    for (int e = threadIdx.x; e < total; e += kFT) {
         const int grp = e % G, pos = e / G;
         const long t = static_cast<long>(r0b + grp) + (static_cast<long>(pos) << P.b);
         double2 x = make_double2(0.0, 0.0);
@@ -188,81 +265,90 @@ __global__ __launch_bounds__(kBlock) void k_phase_a(FcrArgs P, int mode) {
                 x = cmul(x, cconj(cexpi(omega * tk)));
             }
         }
-        buf[grp * gs + pos] = x;
+        buf[Tl.at(grp, pos)] = x;
     }
     __syncthreads();
-    group_stages(buf, lg, G, gs, P.twStd, 0, false);
+    tile_fft(buf, Tl, P.twStd, 0, false);
     double2* out = P.inter1 + static_cast<long>(job) * P.n;
-    for (int e = threadIdx.x; e < total; e += kBlock) {
+    for (int e = threadIdx.x; e < total; e += kFT) {
         const int grp = e % G, pos = e / G;
-        out[(static_cast<long>(pos) << P.b) + r0b + grp] = buf[grp * gs + pos];
+        out[(static_cast<long>(pos) << P.b) + r0b + grp] = buf[Tl.at(grp, pos)];
     }
 }
 
 // Phase B: forward stages a+1..L; mode 1 stores the spectrum, mode 0 forms
 // the F and C products and runs inverse stages 1..b on each.
-__global__ __launch_bounds__(kBlock) void k_phase_b(FcrArgs P, int mode) {
-    __shared__ double2 X[kGroupElems];
-    __shared__ double2 W[kGroupElems];
-    const int lg = P.b, gs = 1 << lg;
+__global__ __launch_bounds__(kFT) void k_phase_b(FcrArgs P, int mode) {
+    __shared__ double2 buf[kLds];
     const int ngroups = 1 << P.a;
-    const int G = min(kGroupElems >> lg, ngroups);
-    const int bpj = ngroups / G;
-    const int job = blockIdx.x / bpj;
-    const int k1b = (blockIdx.x % bpj) * G;
+    const TileLayout Tl(P.b, tile_groups(P.b, ngroups));
+    const int G = Tl.G, gs = Tl.gs;
+    int job, gb;
+    xcd_map(blockIdx.x, ngroups / G, job, gb);
+    const int k1b = gb * G;
     const int total = G * gs;
     const double2* in = P.inter1 + static_cast<long>(job) * P.n + static_cast<long>(k1b) * gs;
-    for (int e = threadIdx.x; e < total; e += kBlock) X[e] = in[e];
+    for (int e = threadIdx.x; e < total; e += kFT) buf[Tl.at(e >> Tl.lg, e & (gs - 1))] = in[e];
     __syncthreads();
     const long bstride = (1L << P.b) - 1;
-    group_stages(X, lg, G, gs, P.twB + static_cast<long>(k1b) * bstride, bstride, false);
+    tile_fft(buf, Tl, P.twB + static_cast<long>(k1b) * bstride, bstride, false);
     if (mode == 1) {
         double2* out = P.spec + static_cast<long>(job) * P.n + static_cast<long>(k1b) * gs;
-        for (int e = threadIdx.x; e < total; e += kBlock) out[e] = X[e];
+        for (int e = threadIdx.x; e < total; e += kFT) out[e] = buf[Tl.at(e >> Tl.lg, e & (gs - 1))];
         return;
     }
+    double2 X[kPT];
+#pragma unroll
+    for (int q = 0; q < kPT; ++q) {
+        const int e = threadIdx.x + q * kFT;
+        if (e < total) X[q] = buf[Tl.at(e >> Tl.lg, e & (gs - 1))];
+    }
+    __syncthreads();
     int i, j, k;
     decode_triple(P.tri0 + job, P.nf, i, j, k);
     const int bF = P.mirror[j];
     for (int path = 0; path < 2; ++path) {
         const double2* sp = P.spec + static_cast<long>(path == 0 ? 2 * bF : 2 * j + 1) * P.n +
                             static_cast<long>(k1b) * gs;
-        for (int e = threadIdx.x; e < total; e += kBlock) W[e] = cmul(X[e], sp[e]);
+#pragma unroll
+        for (int q = 0; q < kPT; ++q) {
+            const int e = threadIdx.x + q * kFT;
+            if (e < total) buf[Tl.at(e >> Tl.lg, e & (gs - 1))] = cmul(X[q], sp[e]);
+        }
         __syncthreads();
-        group_stages(W, lg, G, gs, P.twStd, 0, true);
+        tile_fft(buf, Tl, P.twStd, 0, true);
         double2* out = P.inter2 + (static_cast<long>(path) * P.njobs + job) * P.n;
-        for (int e = threadIdx.x; e < total; e += kBlock) {
+        for (int e = threadIdx.x; e < total; e += kFT) {
             const int grp = e % G, pos = e / G;
-            out[(static_cast<long>(pos) << P.a) + k1b + grp] = W[grp * gs + pos];
+            out[(static_cast<long>(pos) << P.a) + k1b + grp] = buf[Tl.at(grp, pos)];
         }
         __syncthreads();
     }
 }
 
 // Phase C: inverse stages b+1..L; writes conv[path][job][t] = dt * (y[t] / n), t < T.
-__global__ __launch_bounds__(kBlock) void k_phase_c(FcrArgs P) {
-    __shared__ double2 buf[kGroupElems];
-    const int lg = P.a, gs = 1 << lg;
+__global__ __launch_bounds__(kFT) void k_phase_c(FcrArgs P) {
+    __shared__ double2 buf[kLds];
     const int ngroups = 1 << P.b;
-    const int G = min(kGroupElems >> lg, ngroups);
-    const int bpj = ngroups / G;
-    const int jp = blockIdx.x / bpj;  // path * njobs + job
-    const int q0 = (blockIdx.x % bpj) * G;
-    const long qmax = q0;  // smallest t in the block is q0
-    if (qmax >= P.T) return;
+    const TileLayout Tl(P.a, tile_groups(P.a, ngroups));
+    const int G = Tl.G, gs = Tl.gs;
+    int jp, gb;  // jp = path * njobs + job
+    xcd_map(blockIdx.x, ngroups / G, jp, gb);
+    const int q0 = gb * G;
+    if (q0 >= P.T) return;
     const int total = G * gs;
     const double2* in = P.inter2 + static_cast<long>(jp) * P.n + static_cast<long>(q0) * gs;
-    for (int e = threadIdx.x; e < total; e += kBlock) buf[e] = in[e];
+    for (int e = threadIdx.x; e < total; e += kFT) buf[Tl.at(e >> Tl.lg, e & (gs - 1))] = in[e];
     __syncthreads();
     const long cstride = (1L << P.a) - 1;
-    group_stages(buf, lg, G, gs, P.twC + static_cast<long>(q0) * cstride, cstride, true);
+    tile_fft(buf, Tl, P.twC + static_cast<long>(q0) * cstride, cstride, true);
     const double inv_n = 1.0 / static_cast<double>(P.n);
     double2* out = P.conv + static_cast<long>(jp) * P.T;
-    for (int e = threadIdx.x; e < total; e += kBlock) {
+    for (int e = threadIdx.x; e < total; e += kFT) {
         const int grp = e % G, v = e / G;
         const long t = static_cast<long>(q0 + grp) + (static_cast<long>(v) << P.b);
         if (t < P.T) {
-            const double2 y = cscale(buf[grp * gs + v], inv_n);
+            const double2 y = cscale(buf[Tl.at(grp, v)], inv_n);
             out[t] = make_double2(P.dt * y.x, P.dt * y.y);
         }
     }
@@ -847,8 +933,7 @@ static void run_fcr(taco_ctx* c, const FcrPlan& pl, const CombineArgs& out) {
     P.T = static_cast<long>(pl.T);
     auto blocks = [](int lg_group, int lg_ngroups) {
         const int ng = 1 << lg_ngroups;
-        const int G = std::min(kGroupElems >> lg_group, ng);
-        return ng / G;
+        return ng / tile_groups(lg_group, ng);
     };
     const int bpjA = blocks(a, b), bpjB = blocks(b, a), bpjC = blocks(a, b);
     // Spectra of every Gamma column and its conjugate.
@@ -858,9 +943,9 @@ static void run_fcr(taco_ctx* c, const FcrPlan& pl, const CombineArgs& out) {
     const double cNt = static_cast<double>(pl.Nt) * 16.0;
     const double cT = static_cast<double>(pl.T) * 16.0;
     timed(c, K_SPECTRA, spec_jobs * (cNt + 3 * cn), [&] {
-        hipLaunchKernelGGL(k_phase_a, dim3(static_cast<unsigned>(spec_jobs * bpjA)), dim3(kBlock), 0, c->stream, P,
+        hipLaunchKernelGGL(k_phase_a, dim3(static_cast<unsigned>(spec_jobs * bpjA)), dim3(kFT), 0, c->stream, P,
                            1);
-        hipLaunchKernelGGL(k_phase_b, dim3(static_cast<unsigned>(spec_jobs * bpjB)), dim3(kBlock), 0, c->stream, P,
+        hipLaunchKernelGGL(k_phase_b, dim3(static_cast<unsigned>(spec_jobs * bpjB)), dim3(kFT), 0, c->stream, P,
                            1);
     });
     HIP_OK(hipGetLastError());
@@ -870,15 +955,15 @@ static void run_fcr(taco_ctx* c, const FcrPlan& pl, const CombineArgs& out) {
         P.njobs = static_cast<int>(nj);
         // phase A: read the Gamma column, write the half-transformed signal
         timed(c, K_PHASE_A, nj * (cNt + cn), [&] {
-            hipLaunchKernelGGL(k_phase_a, dim3(static_cast<unsigned>(nj * bpjA)), dim3(kBlock), 0, c->stream, P, 0);
+            hipLaunchKernelGGL(k_phase_a, dim3(static_cast<unsigned>(nj * bpjA)), dim3(kFT), 0, c->stream, P, 0);
         });
         // phase B: read it and two spectra, write two half-inverse products
         timed(c, K_PHASE_B, nj * 5 * cn, [&] {
-            hipLaunchKernelGGL(k_phase_b, dim3(static_cast<unsigned>(nj * bpjB)), dim3(kBlock), 0, c->stream, P, 0);
+            hipLaunchKernelGGL(k_phase_b, dim3(static_cast<unsigned>(nj * bpjB)), dim3(kFT), 0, c->stream, P, 0);
         });
         // phase C: read both, write the two convolutions for t < T
         timed(c, K_PHASE_C, nj * 2 * (cn + cT), [&] {
-            hipLaunchKernelGGL(k_phase_c, dim3(static_cast<unsigned>(2 * nj * bpjC)), dim3(kBlock), 0, c->stream,
+            hipLaunchKernelGGL(k_phase_c, dim3(static_cast<unsigned>(2 * nj * bpjC)), dim3(kFT), 0, c->stream,
                                P);
         });
         // combine: read three Gamma columns and two convolutions over t < T,
