@@ -28,6 +28,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -86,10 +87,20 @@ constexpr int kBlock = 256;
 // loads the 2^r values of a closed butterfly set, performs r radix-2 stages
 // on them (each butterfly exactly u +/- w*x with the stage's table
 // twiddle), and writes them back, Stockham-style.  Two barriers per pass.
-constexpr int kTile = 8192;
-constexpr int kFT = 512;
-constexpr int kPT = kTile / kFT;                       // values per thread
-constexpr int kLds = 9216;                             // LDS tile incl. padding
+// Tile configurations (values per workgroup, threads); the autotuner picks
+// one per FFT length and phase.  kLds leaves room for the padding slots.
+template <int TILE, int NT>
+struct FftCfg {
+    static constexpr int kTile = TILE;
+    static constexpr int kFT = NT;
+    static constexpr int kPT = TILE / NT;  // values per thread (8 or 16)
+    static constexpr int kLds = TILE + TILE / 8;
+};
+using Cfg8k = FftCfg<8192, 512>;
+using Cfg4k = FftCfg<4096, 256>;
+using Cfg4kW = FftCfg<4096, 512>;
+using Cfg2k = FftCfg<2048, 256>;
+constexpr int kNumFftCfg = 4;
 
 // Physical LDS slot of (group, position): one pad slot per 16 values and an
 // odd group stride keep strided accesses off a single bank.
@@ -99,21 +110,24 @@ struct TileLayout {
     __device__ __forceinline__ int at(int grp, int pos) const { return grp * GS + pos + (pos >> 4); }
 };
 
+template <class C>
 __host__ __device__ inline int tile_groups(int lg, int ngroups) {
-    int G = kTile >> lg;
+    int G = C::kTile >> lg;
+    if (G < 1) G = 1;
     if (G > ngroups) G = ngroups;
     const int GS = (1 << lg) + ((1 << lg) >> 4) + 1;
-    while (G > 1 && G * GS > kLds) G >>= 1;
+    while (G > 1 && G * GS > C::kLds) G >>= 1;
     return G;
 }
 
 // Passes of r stages; local stage sp (1-based) uses twiddle
 // twp[grp*gstride + (2^(sp-1) - 1) + kk], conjugated for inverse transforms.
-template <int RL>
+template <class C, int RL>
 __device__ __forceinline__ void tile_pass(double2* buf, const TileLayout& T, int s_lo,
                                           const double2* __restrict__ twp, long gstride, bool conj) {
+    constexpr int kFT = C::kFT;
     constexpr int R = 1 << RL;
-    constexpr int SETS = kPT / R;
+    constexpr int SETS = C::kPT / R;
     const int lg = T.lg;
     const int u0bits = lg - s_lo - RL + 1;
     const int spg = 1 << (lg - RL);
@@ -168,21 +182,26 @@ __device__ __forceinline__ void tile_pass(double2* buf, const TileLayout& T, int
     __syncthreads();
 }
 
-// All lg stages of the tile: ceil(lg/4) passes, stage counts as even as possible.
+// All lg stages of the tile: passes of at most log2(values per thread)
+// stages, stage counts as even as possible.
+template <class C>
 __device__ void tile_fft(double2* buf, const TileLayout& T, const double2* __restrict__ twp, long gstride,
                          bool conj) {
     const int lg = T.lg;
     if (lg <= 0) return;
-    const int npass = (lg + 3) / 4;
+    constexpr int maxr = C::kPT >= 16 ? 4 : 3;
+    const int npass = (lg + maxr - 1) / maxr;
     const int base = lg / npass, extra = lg % npass;
     int s_lo = 1;
     for (int p = 0; p < npass; ++p) {
         const int RL = base + (p < extra ? 1 : 0);
         switch (RL) {
-            case 1: tile_pass<1>(buf, T, s_lo, twp, gstride, conj); break;
-            case 2: tile_pass<2>(buf, T, s_lo, twp, gstride, conj); break;
-            case 3: tile_pass<3>(buf, T, s_lo, twp, gstride, conj); break;
-            default: tile_pass<4>(buf, T, s_lo, twp, gstride, conj); break;
+            case 1: tile_pass<C, 1>(buf, T, s_lo, twp, gstride, conj); break;
+            case 2: tile_pass<C, 2>(buf, T, s_lo, twp, gstride, conj); break;
+            case 3: tile_pass<C, 3>(buf, T, s_lo, twp, gstride, conj); break;
+            default:
+                if constexpr (C::kPT >= 16) tile_pass<C, 4>(buf, T, s_lo, twp, gstride, conj);
+                break;
         }
         s_lo += RL;
     }
@@ -231,10 +250,12 @@ __device__ __forceinline__ void decode_triple(long tri, int nf, int& i, int& j, 
 }
 
 // Phase A: mode 0 triples (input g_i * phase_minus), mode 1 spectra (g_b or conj g_b).
-__global__ __launch_bounds__(kFT) void k_phase_a(FcrArgs P, int mode) {
-    __shared__ double2 buf[kLds];
+template <class Cfg>
+__global__ __launch_bounds__(Cfg::kFT) void k_phase_a(FcrArgs P, int mode) {
+    constexpr int kFT = Cfg::kFT;
+    __shared__ double2 buf[Cfg::kLds];
     const int ngroups = 1 << P.b;
-    const TileLayout Tl(P.a, tile_groups(P.a, ngroups));
+    const TileLayout Tl(P.a, tile_groups<Cfg>(P.a, ngroups));
     const int G = Tl.G, gs = Tl.gs;
     int job, gb;
     xcd_map(blockIdx.x, ngroups / G, job, gb);
@@ -268,7 +289,7 @@ __global__ __launch_bounds__(kFT) void k_phase_a(FcrArgs P, int mode) {
         buf[Tl.at(grp, pos)] = x;
     }
     __syncthreads();
-    tile_fft(buf, Tl, P.twStd, 0, false);
+    tile_fft<Cfg>(buf, Tl, P.twStd, 0, false);
     double2* out = P.inter1 + static_cast<long>(job) * P.n;
     for (int e = threadIdx.x; e < total; e += kFT) {
         const int grp = e % G, pos = e / G;
@@ -278,10 +299,13 @@ __global__ __launch_bounds__(kFT) void k_phase_a(FcrArgs P, int mode) {
 
 // Phase B: forward stages a+1..L; mode 1 stores the spectrum, mode 0 forms
 // the F and C products and runs inverse stages 1..b on each.
-__global__ __launch_bounds__(kFT) void k_phase_b(FcrArgs P, int mode) {
-    __shared__ double2 buf[kLds];
+template <class Cfg>
+__global__ __launch_bounds__(Cfg::kFT) void k_phase_b(FcrArgs P, int mode) {
+    constexpr int kFT = Cfg::kFT;
+    constexpr int kPT = Cfg::kPT;
+    __shared__ double2 buf[Cfg::kLds];
     const int ngroups = 1 << P.a;
-    const TileLayout Tl(P.b, tile_groups(P.b, ngroups));
+    const TileLayout Tl(P.b, tile_groups<Cfg>(P.b, ngroups));
     const int G = Tl.G, gs = Tl.gs;
     int job, gb;
     xcd_map(blockIdx.x, ngroups / G, job, gb);
@@ -291,7 +315,7 @@ __global__ __launch_bounds__(kFT) void k_phase_b(FcrArgs P, int mode) {
     for (int e = threadIdx.x; e < total; e += kFT) buf[Tl.at(e >> Tl.lg, e & (gs - 1))] = in[e];
     __syncthreads();
     const long bstride = (1L << P.b) - 1;
-    tile_fft(buf, Tl, P.twB + static_cast<long>(k1b) * bstride, bstride, false);
+    tile_fft<Cfg>(buf, Tl, P.twB + static_cast<long>(k1b) * bstride, bstride, false);
     if (mode == 1) {
         double2* out = P.spec + static_cast<long>(job) * P.n + static_cast<long>(k1b) * gs;
         for (int e = threadIdx.x; e < total; e += kFT) out[e] = buf[Tl.at(e >> Tl.lg, e & (gs - 1))];
@@ -316,7 +340,7 @@ __global__ __launch_bounds__(kFT) void k_phase_b(FcrArgs P, int mode) {
             if (e < total) buf[Tl.at(e >> Tl.lg, e & (gs - 1))] = cmul(X[q], sp[e]);
         }
         __syncthreads();
-        tile_fft(buf, Tl, P.twStd, 0, true);
+        tile_fft<Cfg>(buf, Tl, P.twStd, 0, true);
         double2* out = P.inter2 + (static_cast<long>(path) * P.njobs + job) * P.n;
         for (int e = threadIdx.x; e < total; e += kFT) {
             const int grp = e % G, pos = e / G;
@@ -327,10 +351,12 @@ __global__ __launch_bounds__(kFT) void k_phase_b(FcrArgs P, int mode) {
 }
 
 // Phase C: inverse stages b+1..L; writes conv[path][job][t] = dt * (y[t] / n), t < T.
-__global__ __launch_bounds__(kFT) void k_phase_c(FcrArgs P) {
-    __shared__ double2 buf[kLds];
+template <class Cfg>
+__global__ __launch_bounds__(Cfg::kFT) void k_phase_c(FcrArgs P) {
+    constexpr int kFT = Cfg::kFT;
+    __shared__ double2 buf[Cfg::kLds];
     const int ngroups = 1 << P.b;
-    const TileLayout Tl(P.a, tile_groups(P.a, ngroups));
+    const TileLayout Tl(P.a, tile_groups<Cfg>(P.a, ngroups));
     const int G = Tl.G, gs = Tl.gs;
     int jp, gb;  // jp = path * njobs + job
     xcd_map(blockIdx.x, ngroups / G, jp, gb);
@@ -341,7 +367,7 @@ __global__ __launch_bounds__(kFT) void k_phase_c(FcrArgs P) {
     for (int e = threadIdx.x; e < total; e += kFT) buf[Tl.at(e >> Tl.lg, e & (gs - 1))] = in[e];
     __syncthreads();
     const long cstride = (1L << P.a) - 1;
-    tile_fft(buf, Tl, P.twC + static_cast<long>(q0) * cstride, cstride, true);
+    tile_fft<Cfg>(buf, Tl, P.twC + static_cast<long>(q0) * cstride, cstride, true);
     const double inv_n = 1.0 / static_cast<double>(P.n);
     double2* out = P.conv + static_cast<long>(jp) * P.T;
     for (int e = threadIdx.x; e < total; e += kFT) {
@@ -800,6 +826,7 @@ struct taco_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     taco_engine::Profiler prof;
+    std::map<int, std::array<int, 3>> fft_choice;  // per FFT log-length: tile config of phases A, B, C
     std::map<int, std::unique_ptr<taco_engine::Twiddles>> tw;
     taco_engine::DevBuf gamma, omegas, mirror, p2f, ops, spec, inter1, inter2, conv, slices, t2pos, l4, gw, misc,
         rk, eps;
@@ -877,12 +904,86 @@ static const Twiddles& twiddles_for(taco_ctx* c, int L) {
     return ref;
 }
 
+// Launch phase kernels for a tile configuration index.
+enum Phase { PH_A_TRIPLE, PH_A_SPEC, PH_B_TRIPLE, PH_B_SPEC, PH_C };
+
+template <class Cfg>
+static void launch_phase_cfg(Phase ph, const FcrArgs& P, long njobs, hipStream_t st) {
+    const int ngA = 1 << P.b, ngB = 1 << P.a;
+    switch (ph) {
+        case PH_A_TRIPLE:
+        case PH_A_SPEC: {
+            const long blocks = njobs * (ngA / tile_groups<Cfg>(P.a, ngA));
+            hipLaunchKernelGGL(k_phase_a<Cfg>, dim3(static_cast<unsigned>(blocks)), dim3(Cfg::kFT), 0, st, P,
+                               ph == PH_A_SPEC ? 1 : 0);
+            break;
+        }
+        case PH_B_TRIPLE:
+        case PH_B_SPEC: {
+            const long blocks = njobs * (ngB / tile_groups<Cfg>(P.b, ngB));
+            hipLaunchKernelGGL(k_phase_b<Cfg>, dim3(static_cast<unsigned>(blocks)), dim3(Cfg::kFT), 0, st, P,
+                               ph == PH_B_SPEC ? 1 : 0);
+            break;
+        }
+        case PH_C: {
+            const long blocks = 2 * njobs * (ngA / tile_groups<Cfg>(P.a, ngA));
+            hipLaunchKernelGGL(k_phase_c<Cfg>, dim3(static_cast<unsigned>(blocks)), dim3(Cfg::kFT), 0, st, P);
+            break;
+        }
+    }
+}
+
+static void launch_phase(int cfg, Phase ph, const FcrArgs& P, long njobs, hipStream_t st) {
+    switch (cfg) {
+        case 0: launch_phase_cfg<Cfg8k>(ph, P, njobs, st); break;
+        case 1: launch_phase_cfg<Cfg4k>(ph, P, njobs, st); break;
+        case 2: launch_phase_cfg<Cfg4kW>(ph, P, njobs, st); break;
+        default: launch_phase_cfg<Cfg2k>(ph, P, njobs, st); break;
+    }
+}
+
+static int env_int(const char* name, int dflt) {
+    const char* v = std::getenv(name);
+    return v ? std::atoi(v) : dflt;
+}
+
 static std::size_t chunk_budget_bytes() {
     static std::size_t v = [] {
         const char* s = std::getenv("TACO_FCR_CHUNK_BYTES");
         return s ? static_cast<std::size_t>(std::strtoull(s, nullptr, 10)) : (std::size_t{3} << 30);
     }();
     return v;
+}
+
+// Times every tile configuration on this chunk, phase by phase in pipeline
+// order (each run rewrites the same values), and records the fastest.
+static void autotune(taco_ctx* c, const FcrArgs& P, long nj, std::array<int, 3>& choice) {
+    const Phase phases[3] = {PH_A_TRIPLE, PH_B_TRIPLE, PH_C};
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    for (int ph = 0; ph < 3; ++ph) {
+        float best = 0.f;
+        int best_cfg = 0;
+        for (int cfg = 0; cfg < kNumFftCfg; ++cfg) {
+            launch_phase(cfg, phases[ph], P, nj, c->stream);  // warm
+            HIP_OK(hipEventRecord(e0, c->stream));
+            launch_phase(cfg, phases[ph], P, nj, c->stream);
+            HIP_OK(hipEventRecord(e1, c->stream));
+            HIP_OK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+            if (cfg == 0 || ms < best) {
+                best = ms;
+                best_cfg = cfg;
+            }
+        }
+        choice[ph] = best_cfg;
+        launch_phase(best_cfg, phases[ph], P, nj, c->stream);  // leave this phase's output in place
+    }
+    HIP_OK(hipGetLastError());
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
 }
 
 // Device-side F/C/R for all nf^3 triples over t < T; outputs per CombineArgs.
@@ -931,41 +1032,37 @@ static void run_fcr(taco_ctx* c, const FcrPlan& pl, const CombineArgs& out) {
     P.spec = c->spec.as<double2>();
     P.conv = c->conv.as<double2>();
     P.T = static_cast<long>(pl.T);
-    auto blocks = [](int lg_group, int lg_ngroups) {
-        const int ng = 1 << lg_ngroups;
-        return ng / tile_groups(lg_group, ng);
-    };
-    const int bpjA = blocks(a, b), bpjB = blocks(b, a), bpjC = blocks(a, b);
+    // Tile configuration per phase: TACO_FFT_TILE=<0..3> forces one; otherwise
+    // the first chunk of each FFT length is run with every configuration
+    // (identical butterflies, identical results) and the fastest is kept.
+    auto it = c->fft_choice.find(L);
+    if (it == c->fft_choice.end()) it = c->fft_choice.emplace(L, std::array<int, 3>{-1, -1, -1}).first;
+    auto& choice = it->second;
+    const int forced = env_int("TACO_FFT_TILE", -1);
+    if (forced >= 0 && forced < kNumFftCfg) choice = {forced, forced, forced};
     // Spectra of every Gamma column and its conjugate.
     P.tri0 = 0;
     P.njobs = static_cast<int>(spec_jobs);
     const double cn = static_cast<double>(n) * 16.0;      // one complex series of length Nfft
     const double cNt = static_cast<double>(pl.Nt) * 16.0;
     const double cT = static_cast<double>(pl.T) * 16.0;
+    const int cs = choice[0] >= 0 ? choice[0] : 0;
     timed(c, K_SPECTRA, spec_jobs * (cNt + 3 * cn), [&] {
-        hipLaunchKernelGGL(k_phase_a, dim3(static_cast<unsigned>(spec_jobs * bpjA)), dim3(kFT), 0, c->stream, P,
-                           1);
-        hipLaunchKernelGGL(k_phase_b, dim3(static_cast<unsigned>(spec_jobs * bpjB)), dim3(kFT), 0, c->stream, P,
-                           1);
+        launch_phase(cs, PH_A_SPEC, P, spec_jobs, c->stream);
+        launch_phase(choice[1] >= 0 ? choice[1] : 0, PH_B_SPEC, P, spec_jobs, c->stream);
     });
     HIP_OK(hipGetLastError());
     for (long tri0 = 0; tri0 < nf3; tri0 += chunk) {
         const long nj = std::min(chunk, nf3 - tri0);
         P.tri0 = tri0;
         P.njobs = static_cast<int>(nj);
+        if (choice[0] < 0 || choice[1] < 0 || choice[2] < 0) autotune(c, P, nj, choice);
         // phase A: read the Gamma column, write the half-transformed signal
-        timed(c, K_PHASE_A, nj * (cNt + cn), [&] {
-            hipLaunchKernelGGL(k_phase_a, dim3(static_cast<unsigned>(nj * bpjA)), dim3(kFT), 0, c->stream, P, 0);
-        });
+        timed(c, K_PHASE_A, nj * (cNt + cn), [&] { launch_phase(choice[0], PH_A_TRIPLE, P, nj, c->stream); });
         // phase B: read it and two spectra, write two half-inverse products
-        timed(c, K_PHASE_B, nj * 5 * cn, [&] {
-            hipLaunchKernelGGL(k_phase_b, dim3(static_cast<unsigned>(nj * bpjB)), dim3(kFT), 0, c->stream, P, 0);
-        });
+        timed(c, K_PHASE_B, nj * 5 * cn, [&] { launch_phase(choice[1], PH_B_TRIPLE, P, nj, c->stream); });
         // phase C: read both, write the two convolutions for t < T
-        timed(c, K_PHASE_C, nj * 2 * (cn + cT), [&] {
-            hipLaunchKernelGGL(k_phase_c, dim3(static_cast<unsigned>(2 * nj * bpjC)), dim3(kFT), 0, c->stream,
-                               P);
-        });
+        timed(c, K_PHASE_C, nj * 2 * (cn + cT), [&] { launch_phase(choice[2], PH_C, P, nj, c->stream); });
         // combine: read three Gamma columns and two convolutions over t < T,
         // write F/C/R at the kept times
         const double kept = out.out_mode == 0 ? static_cast<double>(out.npos) : static_cast<double>(pl.T);

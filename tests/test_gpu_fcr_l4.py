@@ -207,3 +207,25 @@ def test_fft_path_is_bitwise_at_t0(orc):
     assert np.array_equal(got.F[..., 0], ref[0][..., 0])
     assert np.array_equal(got.C[..., 0], ref[1][..., 0])
     assert np.array_equal(got.R[..., 0], ref[2][..., 0])
+
+
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3])
+def test_fft_tile_configs_bitwise_identical(orc, cfg):
+    """Every FFT tile configuration the autotuner may pick performs the same
+    butterflies: results are bit-identical across configurations."""
+    import os
+    w = W.gue_pair(N_bath=3000, tmax=300)
+    S, G = w.system, w.gamma
+    os.environ["TACO_FFT_TILE"] = str(cfg)
+    try:
+        got = taco_amd.build_TCL4_generator_hip_fused_batch(S, G, w.dt, w.tidx)
+    finally:
+        del os.environ["TACO_FFT_TILE"]
+    os.environ["TACO_FFT_TILE"] = "0"
+    try:
+        ref = taco_amd.build_TCL4_generator_hip_fused_batch(S, G, w.dt, w.tidx)
+    finally:
+        del os.environ["TACO_FFT_TILE"]
+    assert np.array_equal(got, ref)
+    oref = orc.l4_series(G, S.omegas, S.mirror, S.pair_to_freq, S.A_eig, w.dt, w.tidx)
+    assert max_rel_err(got, oref)[0] <= TOL
