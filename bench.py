@@ -41,6 +41,9 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / checker leg")
     p.add_argument("--cpu-threads", type=int, default=0)
     p.add_argument("--reduced", action="store_true", help="small grid (for a quick functional run)")
+    p.add_argument("--copy-bytes", type=int, default=0,
+                   help="after the timed steps, run one plain device copy of this many bytes (the reference "
+                        "kernel for scaling hardware traffic counters)")
     return p.parse_args()
 
 
@@ -124,6 +127,14 @@ def main():
         elapsed = time.perf_counter() - t0
         kern = prof.read()
     print(f"[bench] timed {a.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+    if a.copy_bytes:
+        n = a.copy_bytes // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     ms_per_step = 1000.0 * elapsed / a.steps
     value = ntau * a.steps / elapsed
     kernels = {}

@@ -29,6 +29,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cstdint>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -234,8 +235,13 @@ struct FcrArgs {
     const double2* twStd;   // stages 1..max(a,b), standard layout
     const double2* twB;     // phase B forward stages, per k1 group
     const double2* twC;     // phase C inverse stages, per q group
-    long tri0;              // first triple of the chunk
-    int njobs;
+    const int* job_tri;     // sorted position -> triple id (triples grouped by equal Omega)
+    const int* cls_start;   // [ncls + 1] first sorted position of each Omega class
+    long job0;              // first sorted position of the FFT chunk
+    int njobs;              // triples in the FFT chunk
+    int cls0;               // first class touching the chunk
+    long comb0;             // first sorted position of the combine batch
+    long cconv;             // combine-batch capacity (conv slots)
     double2* inter1;        // [njobs][2^a][2^b]
     double2* inter2;        // [2][njobs][2^b][2^a]
     double2* spec;          // [2 nf][2^a][2^b]   spectra of g_b (even) and conj(g_b) (odd)
@@ -249,9 +255,9 @@ __device__ __forceinline__ void decode_triple(long tri, int nf, int& i, int& j, 
     i = static_cast<int>(tri / (static_cast<long>(nf) * nf));
 }
 
-// Phase A: mode 0 triples (input g_i * phase_minus), mode 1 spectra (g_b or conj g_b).
+// Phase A of the per-bucket spectra: input g_b (even job) or conj(g_b) (odd job).
 template <class Cfg>
-__global__ __launch_bounds__(Cfg::kFT) void k_phase_a(FcrArgs P, int mode) {
+__global__ __launch_bounds__(Cfg::kFT) void k_phase_a_spec(FcrArgs P) {
     constexpr int kFT = Cfg::kFT;
     __shared__ double2 buf[Cfg::kLds];
     const int ngroups = 1 << P.b;
@@ -260,19 +266,8 @@ __global__ __launch_bounds__(Cfg::kFT) void k_phase_a(FcrArgs P, int mode) {
     int job, gb;
     xcd_map(blockIdx.x, ngroups / G, job, gb);
     const int r0b = gb * G;
-    const double2* src;
-    double omega = 0.0;
-    bool conj_in = false, phase = false;
-    if (mode == 0) {
-        int i, j, k;
-        decode_triple(P.tri0 + job, P.nf, i, j, k);
-        src = P.gamma + static_cast<long>(i) * P.Nt;
-        omega = (P.omegas[i] + P.omegas[j]) + P.omegas[k];
-        phase = true;
-    } else {
-        src = P.gamma + static_cast<long>(job >> 1) * P.Nt;
-        conj_in = (job & 1) != 0;
-    }
+    const double2* src = P.gamma + static_cast<long>(job >> 1) * P.Nt;
+    const bool conj_in = (job & 1) != 0;
     const int total = G * gs;
     for (int e = threadIdx.x; e < total; e += kFT) {
         const int grp = e % G, pos = e / G;
@@ -281,10 +276,6 @@ __global__ __launch_bounds__(Cfg::kFT) void k_phase_a(FcrArgs P, int mode) {
         if (t < P.Nt) {
             x = src[t];
             if (conj_in) x.y = -x.y;
-            if (phase) {
-                const double tk = static_cast<double>(t) * P.dt;
-                x = cmul(x, cconj(cexpi(omega * tk)));
-            }
         }
         buf[Tl.at(grp, pos)] = x;
     }
@@ -294,6 +285,61 @@ __global__ __launch_bounds__(Cfg::kFT) void k_phase_a(FcrArgs P, int mode) {
     for (int e = threadIdx.x; e < total; e += kFT) {
         const int grp = e % G, pos = e / G;
         out[(static_cast<long>(pos) << P.b) + r0b + grp] = buf[Tl.at(grp, pos)];
+    }
+}
+
+// Phase A of the triples: one workgroup per (Omega class, group block).  The
+// phases exp(-i Omega t_k) of the tile are computed once and applied to the
+// Gamma column g_i of every triple of the class inside the chunk.
+template <class Cfg>
+__global__ __launch_bounds__(Cfg::kFT) void k_phase_a(FcrArgs P) {
+    constexpr int kFT = Cfg::kFT;
+    constexpr int kPT = Cfg::kPT;
+    __shared__ double2 buf[Cfg::kLds];
+    const int ngroups = 1 << P.b;
+    const TileLayout Tl(P.a, tile_groups<Cfg>(P.a, ngroups));
+    const int G = Tl.G, gs = Tl.gs;
+    int cj, gb;
+    xcd_map(blockIdx.x, ngroups / G, cj, gb);
+    const int cls = P.cls0 + cj;
+    const long pbeg = max(static_cast<long>(P.cls_start[cls]), P.job0);
+    const long pend = min(static_cast<long>(P.cls_start[cls + 1]), P.job0 + P.njobs);
+    if (pbeg >= pend) return;
+    const int r0b = gb * G;
+    int i0, j0, k0;
+    decode_triple(P.job_tri[pbeg], P.nf, i0, j0, k0);
+    const double omega = (P.omegas[i0] + P.omegas[j0]) + P.omegas[k0];
+    const int total = G * gs;
+    double2 ph[kPT];
+#pragma unroll
+    for (int q = 0; q < kPT; ++q) {
+        const int e = threadIdx.x + q * kFT;
+        const int grp = e % G, pos = e / G;
+        const long t = static_cast<long>(r0b + grp) + (static_cast<long>(pos) << P.b);
+        ph[q] = make_double2(0.0, 0.0);
+        if (e < total && t < P.Nt) ph[q] = cconj(cexpi(omega * (static_cast<double>(t) * P.dt)));
+    }
+    for (long p = pbeg; p < pend; ++p) {
+        int i, j, k;
+        decode_triple(P.job_tri[p], P.nf, i, j, k);
+        const double2* src = P.gamma + static_cast<long>(i) * P.Nt;
+#pragma unroll
+        for (int q = 0; q < kPT; ++q) {
+            const int e = threadIdx.x + q * kFT;
+            if (e < total) {
+                const int grp = e % G, pos = e / G;
+                const long t = static_cast<long>(r0b + grp) + (static_cast<long>(pos) << P.b);
+                buf[Tl.at(grp, pos)] = (t < P.Nt) ? cmul(src[t], ph[q]) : make_double2(0.0, 0.0);
+            }
+        }
+        __syncthreads();
+        tile_fft<Cfg>(buf, Tl, P.twStd, 0, false);
+        double2* out = P.inter1 + (p - P.job0) * P.n;
+        for (int e = threadIdx.x; e < total; e += kFT) {
+            const int grp = e % G, pos = e / G;
+            out[(static_cast<long>(pos) << P.b) + r0b + grp] = buf[Tl.at(grp, pos)];
+        }
+        __syncthreads();
     }
 }
 
@@ -329,7 +375,7 @@ __global__ __launch_bounds__(Cfg::kFT) void k_phase_b(FcrArgs P, int mode) {
     }
     __syncthreads();
     int i, j, k;
-    decode_triple(P.tri0 + job, P.nf, i, j, k);
+    decode_triple(P.job_tri[P.job0 + job], P.nf, i, j, k);
     const int bF = P.mirror[j];
     for (int path = 0; path < 2; ++path) {
         const double2* sp = P.spec + static_cast<long>(path == 0 ? 2 * bF : 2 * j + 1) * P.n +
@@ -369,7 +415,8 @@ __global__ __launch_bounds__(Cfg::kFT) void k_phase_c(FcrArgs P) {
     const long cstride = (1L << P.a) - 1;
     tile_fft<Cfg>(buf, Tl, P.twC + static_cast<long>(q0) * cstride, cstride, true);
     const double inv_n = 1.0 / static_cast<double>(P.n);
-    double2* out = P.conv + static_cast<long>(jp) * P.T;
+    const int path = jp / P.njobs, job = jp % P.njobs;
+    double2* out = P.conv + (path * P.cconv + (P.job0 + job - P.comb0)) * P.T;
     for (int e = threadIdx.x; e < total; e += kFT) {
         const int grp = e % G, v = e / G;
         const long t = static_cast<long>(q0 + grp) + (static_cast<long>(v) << P.b);
@@ -411,16 +458,16 @@ struct CombineArgs {
 // convolution terms of phase C.
 __global__ __launch_bounds__(kBlock) void k_combine(FcrArgs P, CombineArgs O) {
     __shared__ double wsum[8][4];
-    const int job = blockIdx.x;
-    const long tri = P.tri0 + job;
+    const int slot = blockIdx.x;
+    const long tri = P.job_tri[P.comb0 + slot];
     int i, j, k;
     decode_triple(tri, P.nf, i, j, k);
     const double omega = (P.omegas[i] + P.omegas[j]) + P.omegas[k];
     const double2* g1 = P.gamma + static_cast<long>(i) * P.Nt;
     const double2* g2 = P.gamma + static_cast<long>(j) * P.Nt;
     const double2* g2m = P.gamma + static_cast<long>(P.mirror[j]) * P.Nt;
-    const double2* convF = P.conv + static_cast<long>(job) * P.T;
-    const double2* convC = P.conv + static_cast<long>(P.njobs + job) * P.T;
+    const double2* convF = P.conv + static_cast<long>(slot) * P.T;
+    const double2* convC = P.conv + (P.cconv + slot) * P.T;
     double carry[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (long t0 = 0; t0 < P.T; t0 += kBlock) {
         const long t = t0 + threadIdx.x;
@@ -829,7 +876,7 @@ struct taco_ctx {
     std::map<int, std::array<int, 3>> fft_choice;  // per FFT log-length: tile config of phases A, B, C
     std::map<int, std::unique_ptr<taco_engine::Twiddles>> tw;
     taco_engine::DevBuf gamma, omegas, mirror, p2f, ops, spec, inter1, inter2, conv, slices, t2pos, l4, gw, misc,
-        rk, eps;
+        rk, eps, jobtri, clsstart;
 };
 
 namespace taco_engine {
@@ -911,11 +958,14 @@ template <class Cfg>
 static void launch_phase_cfg(Phase ph, const FcrArgs& P, long njobs, hipStream_t st) {
     const int ngA = 1 << P.b, ngB = 1 << P.a;
     switch (ph) {
-        case PH_A_TRIPLE:
+        case PH_A_TRIPLE: {  // njobs = number of Omega classes touching the chunk
+            const long blocks = njobs * (ngA / tile_groups<Cfg>(P.a, ngA));
+            hipLaunchKernelGGL(k_phase_a<Cfg>, dim3(static_cast<unsigned>(blocks)), dim3(Cfg::kFT), 0, st, P);
+            break;
+        }
         case PH_A_SPEC: {
             const long blocks = njobs * (ngA / tile_groups<Cfg>(P.a, ngA));
-            hipLaunchKernelGGL(k_phase_a<Cfg>, dim3(static_cast<unsigned>(blocks)), dim3(Cfg::kFT), 0, st, P,
-                               ph == PH_A_SPEC ? 1 : 0);
+            hipLaunchKernelGGL(k_phase_a_spec<Cfg>, dim3(static_cast<unsigned>(blocks)), dim3(Cfg::kFT), 0, st, P);
             break;
         }
         case PH_B_TRIPLE:
@@ -957,8 +1007,9 @@ static std::size_t chunk_budget_bytes() {
 
 // Times every tile configuration on this chunk, phase by phase in pipeline
 // order (each run rewrites the same values), and records the fastest.
-static void autotune(taco_ctx* c, const FcrArgs& P, long nj, std::array<int, 3>& choice) {
+static void autotune(taco_ctx* c, const FcrArgs& P, long ncls, long nj, std::array<int, 3>& choice) {
     const Phase phases[3] = {PH_A_TRIPLE, PH_B_TRIPLE, PH_C};
+    const long units[3] = {ncls, nj, nj};
     hipEvent_t e0, e1;
     HIP_OK(hipEventCreate(&e0));
     HIP_OK(hipEventCreate(&e1));
@@ -966,9 +1017,9 @@ static void autotune(taco_ctx* c, const FcrArgs& P, long nj, std::array<int, 3>&
         float best = 0.f;
         int best_cfg = 0;
         for (int cfg = 0; cfg < kNumFftCfg; ++cfg) {
-            launch_phase(cfg, phases[ph], P, nj, c->stream);  // warm
+            launch_phase(cfg, phases[ph], P, units[ph], c->stream);  // warm
             HIP_OK(hipEventRecord(e0, c->stream));
-            launch_phase(cfg, phases[ph], P, nj, c->stream);
+            launch_phase(cfg, phases[ph], P, units[ph], c->stream);
             HIP_OK(hipEventRecord(e1, c->stream));
             HIP_OK(hipEventSynchronize(e1));
             float ms = 0.f;
@@ -979,7 +1030,7 @@ static void autotune(taco_ctx* c, const FcrArgs& P, long nj, std::array<int, 3>&
             }
         }
         choice[ph] = best_cfg;
-        launch_phase(best_cfg, phases[ph], P, nj, c->stream);  // leave this phase's output in place
+        launch_phase(best_cfg, phases[ph], P, units[ph], c->stream);  // leave this phase's output in place
     }
     HIP_OK(hipGetLastError());
     (void)hipEventDestroy(e0);
@@ -991,11 +1042,17 @@ struct FcrPlan {
     const double2* gamma;
     std::size_t Nt;
     int nf;
-    const double* omegas;
+    const double* omegas;    // device
+    const double* h_omegas;  // host copy (groups the triples by Omega)
     const int* mirror;
     double dt;
     std::size_t T;
 };
+
+static std::size_t env_bytes(const char* name, std::size_t dflt) {
+    const char* v = std::getenv(name);
+    return v ? static_cast<std::size_t>(std::strtoull(v, nullptr, 10)) : dflt;
+}
 
 static void run_fcr(taco_ctx* c, const FcrPlan& pl, const CombineArgs& out) {
     const std::size_t n = fcr_nfft(pl.Nt, g_pad_factor);
@@ -1004,20 +1061,48 @@ static void run_fcr(taco_ctx* c, const FcrPlan& pl, const CombineArgs& out) {
     if (L > 22) throw Error(TACO_ERR_INVALID, "FFT length above 2^22 (Nt above 2^21) is not supported");
     const int a = (L + 1) / 2, b = L - a;
     const Twiddles& tw = twiddles_for(c, L);
-    const long nf3 = static_cast<long>(pl.nf) * pl.nf * pl.nf;
+    const int nf = pl.nf;
+    const long nf3 = static_cast<long>(nf) * nf * nf;
+
+    // Group the triples by the bit pattern of Omega = (w_i + w_j) + w_k (the
+    // kernels' expression): one class shares its phases exp(-i Omega t_k).
+    std::vector<int> order(static_cast<std::size_t>(nf3));
+    std::vector<std::uint64_t> key(static_cast<std::size_t>(nf3));
+    for (long t = 0; t < nf3; ++t) {
+        const int k = static_cast<int>(t % nf), j = static_cast<int>((t / nf) % nf), i = static_cast<int>(t / (nf * nf));
+        const double om = (pl.h_omegas[i] + pl.h_omegas[j]) + pl.h_omegas[k];
+        std::memcpy(&key[static_cast<std::size_t>(t)], &om, 8);
+        order[static_cast<std::size_t>(t)] = static_cast<int>(t);
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key[x] < key[y]; });
+    std::vector<int> cls_start;
+    for (long p = 0; p < nf3; ++p)
+        if (p == 0 || key[order[p]] != key[order[p - 1]]) cls_start.push_back(static_cast<int>(p));
+    cls_start.push_back(static_cast<int>(nf3));
+    c->jobtri.ensure(order.size() * sizeof(int));
+    c->clsstart.ensure(cls_start.size() * sizeof(int));
+    HIP_OK(hipMemcpyAsync(c->jobtri.p, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipMemcpyAsync(c->clsstart.p, cls_start.data(), cls_start.size() * sizeof(int), hipMemcpyHostToDevice,
+                          c->stream));
+
+    // FFT chunk (intermediates of nj triples) and combine batch (conv slots).
     const std::size_t per_job = 3 * n * 16;
     long chunk = static_cast<long>(std::max<std::size_t>(1, chunk_budget_bytes() / per_job));
     chunk = std::min(chunk, nf3);
-    const long spec_jobs = 2L * pl.nf;
+    const std::size_t Tpad = std::max<std::size_t>(pl.T, 1);
+    long cconv = static_cast<long>(env_bytes("TACO_CONV_BYTES", std::size_t{4} << 30) / (2 * Tpad * 16));
+    cconv = std::max(chunk, (cconv / chunk) * chunk);
+    cconv = std::min(cconv, nf3);
+    const long spec_jobs = 2L * nf;
     c->spec.ensure(static_cast<std::size_t>(spec_jobs) * n * 16);
     c->inter1.ensure(static_cast<std::size_t>(std::max(chunk, spec_jobs)) * n * 16);
     c->inter2.ensure(static_cast<std::size_t>(2 * chunk) * n * 16);
-    c->conv.ensure(static_cast<std::size_t>(2 * chunk) * std::max<std::size_t>(pl.T, 1) * 16);
+    c->conv.ensure(static_cast<std::size_t>(2 * cconv) * Tpad * 16);
     FcrArgs P{};
     P.gamma = pl.gamma;
     P.omegas = pl.omegas;
     P.mirror = pl.mirror;
-    P.nf = pl.nf;
+    P.nf = nf;
     P.Nt = static_cast<long>(pl.Nt);
     P.dt = pl.dt;
     P.L = L;
@@ -1027,10 +1112,13 @@ static void run_fcr(taco_ctx* c, const FcrPlan& pl, const CombineArgs& out) {
     P.twStd = tw.std_.as<double2>();
     P.twB = tw.b_.as<double2>();
     P.twC = tw.c_.as<double2>();
+    P.job_tri = c->jobtri.as<int>();
+    P.cls_start = c->clsstart.as<int>();
     P.inter1 = c->inter1.as<double2>();
     P.inter2 = c->inter2.as<double2>();
     P.spec = c->spec.as<double2>();
     P.conv = c->conv.as<double2>();
+    P.cconv = cconv;
     P.T = static_cast<long>(pl.T);
     // Tile configuration per phase: TACO_FFT_TILE=<0..3> forces one; otherwise
     // the first chunk of each FFT length is run with every configuration
@@ -1040,34 +1128,43 @@ static void run_fcr(taco_ctx* c, const FcrPlan& pl, const CombineArgs& out) {
     auto& choice = it->second;
     const int forced = env_int("TACO_FFT_TILE", -1);
     if (forced >= 0 && forced < kNumFftCfg) choice = {forced, forced, forced};
-    // Spectra of every Gamma column and its conjugate.
-    P.tri0 = 0;
-    P.njobs = static_cast<int>(spec_jobs);
     const double cn = static_cast<double>(n) * 16.0;      // one complex series of length Nfft
     const double cNt = static_cast<double>(pl.Nt) * 16.0;
     const double cT = static_cast<double>(pl.T) * 16.0;
-    const int cs = choice[0] >= 0 ? choice[0] : 0;
+    // Spectra of every Gamma column and its conjugate.
+    P.njobs = static_cast<int>(spec_jobs);
     timed(c, K_SPECTRA, spec_jobs * (cNt + 3 * cn), [&] {
-        launch_phase(cs, PH_A_SPEC, P, spec_jobs, c->stream);
+        launch_phase(choice[0] >= 0 ? choice[0] : 0, PH_A_SPEC, P, spec_jobs, c->stream);
         launch_phase(choice[1] >= 0 ? choice[1] : 0, PH_B_SPEC, P, spec_jobs, c->stream);
     });
     HIP_OK(hipGetLastError());
-    for (long tri0 = 0; tri0 < nf3; tri0 += chunk) {
-        const long nj = std::min(chunk, nf3 - tri0);
-        P.tri0 = tri0;
-        P.njobs = static_cast<int>(nj);
-        if (choice[0] < 0 || choice[1] < 0 || choice[2] < 0) autotune(c, P, nj, choice);
-        // phase A: read the Gamma column, write the half-transformed signal
-        timed(c, K_PHASE_A, nj * (cNt + cn), [&] { launch_phase(choice[0], PH_A_TRIPLE, P, nj, c->stream); });
-        // phase B: read it and two spectra, write two half-inverse products
-        timed(c, K_PHASE_B, nj * 5 * cn, [&] { launch_phase(choice[1], PH_B_TRIPLE, P, nj, c->stream); });
-        // phase C: read both, write the two convolutions for t < T
-        timed(c, K_PHASE_C, nj * 2 * (cn + cT), [&] { launch_phase(choice[2], PH_C, P, nj, c->stream); });
+    auto class_of = [&](long pos) {
+        return static_cast<int>(std::upper_bound(cls_start.begin(), cls_start.end(), static_cast<int>(pos)) -
+                                cls_start.begin()) - 1;
+    };
+    for (long comb0 = 0; comb0 < nf3; comb0 += cconv) {
+        const long cb = std::min(cconv, nf3 - comb0);
+        P.comb0 = comb0;
+        for (long job0 = comb0; job0 < comb0 + cb; job0 += chunk) {
+            const long nj = std::min(chunk, comb0 + cb - job0);
+            P.job0 = job0;
+            P.njobs = static_cast<int>(nj);
+            P.cls0 = class_of(job0);
+            const long ncls = class_of(job0 + nj - 1) - P.cls0 + 1;
+            if (choice[0] < 0 || choice[1] < 0 || choice[2] < 0) autotune(c, P, ncls, nj, choice);
+            // phase A: read the Gamma column, write the half-transformed signal
+            timed(c, K_PHASE_A, nj * (cNt + cn), [&] { launch_phase(choice[0], PH_A_TRIPLE, P, ncls, c->stream); });
+            // phase B: read it and two spectra, write two half-inverse products
+            timed(c, K_PHASE_B, nj * 5 * cn, [&] { launch_phase(choice[1], PH_B_TRIPLE, P, nj, c->stream); });
+            // phase C: read both, write the two convolutions for t < T
+            timed(c, K_PHASE_C, nj * 2 * (cn + cT), [&] { launch_phase(choice[2], PH_C, P, nj, c->stream); });
+            HIP_OK(hipGetLastError());
+        }
         // combine: read three Gamma columns and two convolutions over t < T,
         // write F/C/R at the kept times
         const double kept = out.out_mode == 0 ? static_cast<double>(out.npos) : static_cast<double>(pl.T);
-        timed(c, K_COMBINE, nj * (5 * cT + 3 * 16.0 * kept), [&] {
-            hipLaunchKernelGGL(k_combine, dim3(static_cast<unsigned>(nj)), dim3(kBlock), 0, c->stream, P, out);
+        timed(c, K_COMBINE, cb * (5 * cT + 3 * 16.0 * kept), [&] {
+            hipLaunchKernelGGL(k_combine, dim3(static_cast<unsigned>(cb)), dim3(kBlock), 0, c->stream, P, out);
         });
         HIP_OK(hipGetLastError());
     }
@@ -1148,8 +1245,8 @@ static TimePlan plan_times(const std::size_t* tidx, std::size_t ntau, std::size_
 // Core of the batch builders on device-resident inputs.  Writes L4 for the
 // unique times into c->l4 and returns the plan; fcr_ms optional.
 static TimePlan l4_core(taco_ctx* c, const double2* d_gamma, std::size_t Nt, int nf, const double* d_omegas,
-                        const int* d_mirror, const int* d_p2f, int N, const double2* d_ops, int num_ops, double dt,
-                        const std::size_t* tidx, std::size_t ntau, double* fcr_ms) {
+                        const double* h_omegas, const int* d_mirror, const int* d_p2f, int N, const double2* d_ops,
+                        int num_ops, double dt, const std::size_t* tidx, std::size_t ntau, double* fcr_ms) {
     TimePlan tp = plan_times(tidx, ntau, Nt);
     const long npos = static_cast<long>(tp.uniq.size());
     if (npos == 0) return tp;
@@ -1171,7 +1268,7 @@ static TimePlan l4_core(taco_ctx* c, const double2* d_gamma, std::size_t Nt, int
     out.Co = sl + nf3 * npos;
     out.Ro = sl + 2 * nf3 * npos;
     if (fcr_ms) HIP_OK(hipEventRecord(c->ev0, c->stream));
-    FcrPlan pl{d_gamma, Nt, nf, d_omegas, d_mirror, dt, tp.T};
+    FcrPlan pl{d_gamma, Nt, nf, d_omegas, h_omegas, d_mirror, dt, tp.T};
     run_fcr(c, pl, out);
     if (fcr_ms) HIP_OK(hipEventRecord(c->ev1, c->stream));
     L4Args A{};
@@ -1346,7 +1443,7 @@ int taco_triple_kernels(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, c
         out.Fo = sl;
         out.Co = sl + nf3 * keep;
         out.Ro = sl + 2 * nf3 * keep;
-        FcrPlan pl{dg, Nt, nf, dom, dmi, dt, keep};
+        FcrPlan pl{dg, Nt, nf, dom, omegas, dmi, dt, keep};
         run_fcr(ctx, pl, out);
         HIP_OK(hipMemcpyAsync(F, out.Fo, nf3 * keep * 16, hipMemcpyDeviceToHost, ctx->stream));
         HIP_OK(hipMemcpyAsync(C, out.Co, nf3 * keep * 16, hipMemcpyDeviceToHost, ctx->stream));
@@ -1369,7 +1466,7 @@ int taco_l4_batch(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, const d
         const int* dmi = upload(ctx, ctx->mirror, mirror, static_cast<std::size_t>(nf));
         const int* dpf = upload(ctx, ctx->p2f, pair_to_freq, N2);
         const double2* dops = upload(ctx, ctx->ops, reinterpret_cast<const double2*>(ops), N2 * num_ops);
-        TimePlan tp = l4_core(ctx, dg, Nt, nf, dom, dmi, dpf, N, dops, num_ops, dt, tidx, ntau, fcr_ms);
+        TimePlan tp = l4_core(ctx, dg, Nt, nf, dom, omegas, dmi, dpf, N, dops, num_ops, dt, tidx, ntau, fcr_ms);
         if (tp.uniq.empty()) return;
         std::vector<double> host(N4 * tp.uniq.size() * 2);
         HIP_OK(hipMemcpyAsync(host.data(), ctx->l4.p, host.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -1388,15 +1485,15 @@ int taco_l4_batch_dev(taco_ctx* ctx, const void* d_gamma, size_t Nt, int nf, con
         validate_system(Nt, nf, nullptr, nullptr, N, num_ops, dt, method);
         DeviceGuard g(ctx->device);
         const std::size_t N2 = static_cast<std::size_t>(N) * N, N4 = N2 * N2;
-        {
-            // the kernels index with these tables: validate them on the host
-            std::vector<int> hm(static_cast<std::size_t>(nf)), hp(N2);
-            HIP_OK(hipMemcpy(hm.data(), d_mirror, hm.size() * sizeof(int), hipMemcpyDeviceToHost));
-            HIP_OK(hipMemcpy(hp.data(), d_pair_to_freq, hp.size() * sizeof(int), hipMemcpyDeviceToHost));
-            validate_system(Nt, nf, hm.data(), hp.data(), N, num_ops, dt, method);
-        }
+        // the kernels index with these tables: validate them on the host
+        std::vector<int> hm(static_cast<std::size_t>(nf)), hp(N2);
+        std::vector<double> hom(static_cast<std::size_t>(nf));
+        HIP_OK(hipMemcpy(hm.data(), d_mirror, hm.size() * sizeof(int), hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(hp.data(), d_pair_to_freq, hp.size() * sizeof(int), hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(hom.data(), d_omegas, hom.size() * sizeof(double), hipMemcpyDeviceToHost));
+        validate_system(Nt, nf, hm.data(), hp.data(), N, num_ops, dt, method);
         TimePlan tp = l4_core(ctx, static_cast<const double2*>(d_gamma), Nt, nf, static_cast<const double*>(d_omegas),
-                              static_cast<const int*>(d_mirror), static_cast<const int*>(d_pair_to_freq), N,
+                              hom.data(), static_cast<const int*>(d_mirror), static_cast<const int*>(d_pair_to_freq), N,
                               static_cast<const double2*>(d_ops), num_ops, dt, tidx, ntau, fcr_ms);
         if (tp.uniq.empty()) return;
         double2* out = static_cast<double2*>(d_L4_out);
@@ -1531,7 +1628,7 @@ int taco_tcl4_propagate(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, c
         if (order == 4) {
             std::vector<std::size_t> all(T);
             std::iota(all.begin(), all.end(), std::size_t{0});
-            l4_core(ctx, dg, Nt, nf, dom, dmi, dpf, N, dops, num_ops, dt, all.data(), T, nullptr);
+            l4_core(ctx, dg, Nt, nf, dom, omegas, dmi, dpf, N, dops, num_ops, dt, all.data(), T, nullptr);
         } else {
             ctx->l4.ensure(T * N4 * 16);
             HIP_OK(hipMemsetAsync(ctx->l4.p, 0, T * N4 * 16, ctx->stream));
