@@ -37,7 +37,9 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--workload", default="gue_pair", choices=["gue_pair", "published", "spin_boson"])
+    p.add_argument("--workload", default="gue_pair", choices=["gue_pair", "published", "spin_boson", "batched"])
+    p.add_argument("--systems", type=int, default=256, help="batched workload: number of systems")
+    p.add_argument("--no-plain", action="store_true", help="batched workload: skip the plain per-system leg")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / checker leg")
     p.add_argument("--cpu-threads", type=int, default=0)
     p.add_argument("--reduced", action="store_true", help="small grid (for a quick functional run)")
@@ -96,12 +98,130 @@ def cpu_baseline_leg(w, L4_gpu: np.ndarray, threads: int) -> tuple[dict, dict]:
     return baseline, parity
 
 
+def kernel_table(kern: dict, steps: int) -> tuple[dict, dict]:
+    kernels = {}
+    for name, k in kern.items():
+        kernels[name] = {
+            "ms_per_step": k["ms"] / steps,
+            "launches_per_step": k["launches"] / steps,
+            "avg_launch_us": 1000.0 * k["ms"] / k["launches"],
+            "achieved_GBps": k["algo_bytes"] / (k["ms"] * 1e-3) / 1e9 if k["ms"] > 0 else None,
+        }
+    dom = max(kern, key=lambda n: kern[n]["ms"])
+    kd = kern[dom]
+    achieved = kd["algo_bytes"] / (kd["ms"] * 1e-3) / 1e9
+    roofline = {
+        "kernel": dom,
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": PEAK_HBM_GBS,
+        "unit": "GB/s",
+        "frac": achieved / PEAK_HBM_GBS,
+        "traffic": None,
+        "algo_bytes_per_launch": kd["algo_bytes"] / kd["launches"],
+        "avg_launch_us": 1000.0 * kd["ms"] / kd["launches"],
+    }
+    return kernels, roofline
+
+
+def batched_main(a):
+    """Many independent systems (d in {2,3,4,6,8}, Ohmic or Drude-Lorentz,
+    Nt = 2^14..2^18, 1000 L4 times at stride 1..16, 1000 RK4 steps of
+    TCL2 + L4 each): one batch (inputs resident in HBM, replayed hipGraph)
+    against the plain per-system path (the host-API builders, one system at a
+    time: L4 at tidx, then the RK4 run)."""
+    import torch
+
+    import taco_amd
+    from taco_amd import tcl4, workloads as W
+
+    assert torch.cuda.is_available(), "bench needs an MI355X"
+    nsys = a.systems
+    t0 = time.perf_counter()
+    ws = [W.heterogeneous_system(b) for b in range(nsys)]
+    items = []
+    for w in ws:
+        r0 = np.zeros(w.system.N ** 2, complex)
+        r0[0] = 1.0
+        items.append(taco_amd.BatchItem(w.system, w.gamma, w.dt, w.tidx, steps=w.steps, r0=r0, order=4))
+    dims = {}
+    for w in ws:
+        dims[w.meta["d"]] = dims.get(w.meta["d"], 0) + 1
+    print(f"[bench] batched: {nsys} systems, d counts {dims}, host prep {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    b = taco_amd.Batch(items)
+    print(f"[bench] batch created in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    with tcl4.Profile(0) as prof:
+        t0 = time.perf_counter()
+        b.run()  # eager: tunes the FFT tiles; profiled per kernel
+        eager_s = time.perf_counter() - t0
+        kern = prof.read()
+    print(f"[bench] eager batch run {eager_s:.2f} s", file=sys.stderr, flush=True)
+    for _ in range(a.warmup):
+        b.run(graph=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        b.run(graph=True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    n_l4 = sum(len(it.tidx) for it in items)
+    value = n_l4 * a.steps / elapsed
+    print(f"[bench] graph replays: {a.steps} in {elapsed:.2f} s -> {value:.0f} L4/s", file=sys.stderr, flush=True)
+    kernels, roofline = kernel_table(kern, 1)
+    plain, check = None, None
+    if not a.no_plain:
+        t0 = time.perf_counter()
+        mism = 0
+        for s, it in enumerate(items):
+            L4 = taco_amd.build_TCL4_generator_hip_fused_batch(it.system, it.gamma, it.dt, list(it.tidx))
+            rho = taco_amd.tcl4_propagate(it.system, it.gamma, it.dt, it.steps, it.r0, order=4)
+            mism += int(not (np.array_equal(L4, b.l4(s)) and np.array_equal(rho, b.rho(s))))
+        plain_s = time.perf_counter() - t0
+        plain = {"value": n_l4 / plain_s, "seconds": plain_s,
+                 "batched_over_plain": (n_l4 / plain_s) and value / (n_l4 / plain_s)}
+        check = {"systems_bitwise_equal_to_plain": nsys - mism, "systems": nsys}
+        print(f"[bench] plain per-system path {plain_s:.2f} s; {nsys - mism}/{nsys} systems bitwise equal",
+              file=sys.stderr, flush=True)
+    rec = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "L4 time-points/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1000.0 * elapsed / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128",
+        "data": "synthetic",
+        "config": {
+            "workload": f"batched: {nsys} systems (seed 3+b), d counts {dims}, Nt 2^14..2^18, dt 0.005, "
+                        f"1000 L4 times each (stride 1..16) + 1000 RK4 steps of TCL2+L4, inputs resident in HBM",
+            "path": "batched (hipGraph replay)",
+            "phases": "correctly rounded",
+        },
+        "plain": plain,
+        "check": check,
+        "eager_run_s": eager_s,
+        "roofline": roofline,
+        "cpu_baseline": None,
+        "kernels": kernels,
+    }
+    b.close()
+    print(json.dumps(rec), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {a.gpus} is not supported (use --gpus 1)",
               file=sys.stderr)
         sys.exit(2)
+    if a.workload == "batched":
+        return batched_main(a)
     import torch
 
     import taco_amd
@@ -137,28 +257,13 @@ def main():
         del src, dst
     ms_per_step = 1000.0 * elapsed / a.steps
     value = ntau * a.steps / elapsed
-    kernels = {}
-    for name, k in kern.items():
-        kernels[name] = {
-            "ms_per_step": k["ms"] / a.steps,
-            "launches_per_step": k["launches"] / a.steps,
-            "avg_launch_us": 1000.0 * k["ms"] / k["launches"],
-            "achieved_GBps": k["algo_bytes"] / (k["ms"] * 1e-3) / 1e9 if k["ms"] > 0 else None,
-        }
-    dom = max(kern, key=lambda n: kern[n]["ms"])
-    kd = kern[dom]
-    achieved = kd["algo_bytes"] / (kd["ms"] * 1e-3) / 1e9
-    roofline = {
-        "kernel": dom,
-        "bound": "hbm",
-        "achieved": achieved,
-        "peak": PEAK_HBM_GBS,
-        "unit": "GB/s",
-        "frac": achieved / PEAK_HBM_GBS,
-        "traffic": None,
-        "algo_bytes_per_launch": kd["algo_bytes"] / kd["launches"],
-        "avg_launch_us": 1000.0 * kd["ms"] / kd["launches"],
-    }
+    kernels, roofline = kernel_table(kern, a.steps)
+    from taco_amd import _lib
+    choice = np.zeros(3, np.int32)
+    nfft = 1 << int(np.ceil(np.log2(2 * G.shape[0] - 1)))
+    _lib.check(_lib.lib().taco_fft_tile_choice(tcl4.context(0).handle, int(np.log2(nfft)), _lib.iptr(choice)))
+    tiles = ["8192/512", "4096/256", "4096/512", "2048/256"]
+    fft_tiles = {ph: tiles[c] if c >= 0 else None for ph, c in zip("ABC", choice)}
     L4 = out.cpu().numpy().reshape(ntau, S.N ** 2, S.N ** 2).transpose(0, 2, 1)
     cpu, parity = None, None
     if not a.no_cpu:
@@ -166,7 +271,6 @@ def main():
         print(f"[bench] cpu baseline / checker leg on {threads} threads", file=sys.stderr, flush=True)
         cpu, parity = cpu_baseline_leg(w, L4, threads)
         print(f"[bench] cpu leg done in {cpu['seconds']:.1f} s", file=sys.stderr, flush=True)
-    nfft = 1 << int(np.ceil(np.log2(2 * G.shape[0] - 1)))
     rec = {
         "metric": METRIC,
         "value": value,
@@ -185,6 +289,7 @@ def main():
                         f"Nfft={nfft}, dt={w.dt}, tidx 0..{w.tidx[-1]} ({ntau} L4 points), inputs resident in HBM",
             "path": "plain",
             "phases": "correctly rounded",
+            "fft_tiles": fft_tiles,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

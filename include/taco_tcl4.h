@@ -73,6 +73,13 @@ int taco_profile_enable(taco_ctx* ctx, int on);
 int taco_profile_reset(taco_ctx* ctx);
 int taco_profile_get(taco_ctx* ctx, int kernel, double* total_ms, long* launches, double* algo_bytes);
 
+/* Tile configuration the autotuner chose for FFT length 2^log2_nfft in this
+ * context, per phase A, B, C (0: 8192 values/512 threads, 1: 4096/256,
+ * 2: 4096/512, 3: 2048/256; -1 if not tuned yet).  TACO_FFT_TILE=<0..3>
+ * forces one.  No reference counterpart (the reference's cuFFT plans are
+ * internal, tcl4_fcr_kernels.cu:602-622). */
+int taco_fft_tile_choice(taco_ctx* ctx, int log2_nfft, int* cfg3);
+
 /* Correctly rounded sin/cos of each x[i] (|x| < 2^30): the phase routine of
  * the F/C/R kernels, exp(+-i*omega*(k*dt)) = (c, +-s).  The reference calls
  * std::exp of an imaginary argument (tcl4_kernels.cpp:116-120), whose last
@@ -163,6 +170,55 @@ int taco_tcl4_propagate(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, c
                         const int* mirror, const int* pair_to_freq, int N, const double* eps,
                         const double* ops, int num_ops, double dt, size_t steps, int order, double* r,
                         double* samples_out);
+
+/* ---- Batched runs: many independent systems in one device pass ----------
+ * Each system brings its own N, buckets, Gamma table (own Nt), dt, time-index
+ * set and RK4 run; systems share nothing but the context.  The reference
+ * runs such sets one system at a time, each call through
+ * build_TCL4_generator_cuda_fused_batch (tcl4_fused_cuda.cu:1086-1522) and a
+ * separate propagate_rk4_dense (rk4_dense.cpp:246-320); a batch computes the
+ * same values per system (same kernels, same arithmetic) with every input,
+ * job list and output resident in HBM, no host round trip between systems,
+ * the TCL4 F/C/R work done once for the union of the L4 times and the RK4
+ * window, and one RK4 launch stepping every system. */
+typedef struct taco_batch_system {
+    const double* gamma;        /* [Nt x nf] Gamma table, column-major */
+    size_t Nt;
+    int nf;
+    const double* omegas;       /* [nf] */
+    const int* mirror;          /* [nf] */
+    const int* pair_to_freq;    /* N x N column-major */
+    int N;
+    const double* ops;          /* num_ops N x N blocks (A_eig) */
+    int num_ops;
+    double dt;
+    const size_t* tidx;         /* L4 output times (may be NULL when ntau = 0) */
+    size_t ntau;
+    size_t steps;               /* RK4 steps (0: no propagation) */
+    int order;                  /* generator of the RK4 run: 4 (TCL2 + L4), 2, 0 */
+    const double* eps;          /* [N] eigenvalues (needed when steps > 0) */
+    const double* r0;           /* [N^2] initial vec(rho), eigenbasis (needed when steps > 0) */
+} taco_batch_system;
+
+typedef struct taco_batch taco_batch;
+
+/* Validates and uploads every system, prepares the job lists and sizes the
+ * workspace.  The host arrays are only read during this call. */
+int taco_batch_create(taco_ctx* ctx, const taco_batch_system* systems, int nsys, taco_batch** out);
+/* Runs the whole batch and synchronises.  use_graph = 1 captures the batch
+ * into a hipGraph on the first call after an eager run (which also tunes the
+ * FFT tiles) and replays it afterwards; results are identical either way. */
+int taco_batch_run(taco_batch* b, int use_graph);
+/* System s's L4 at its tidx, request order: ntau blocks of N^2 x N^2. */
+int taco_batch_l4(taco_batch* b, int s, double* L4_out);
+/* System s's RK4 states at steps 0..steps ((steps+1) x N^2, nullable) and the
+ * final state (N^2, nullable). */
+int taco_batch_rho(taco_batch* b, int s, double* samples_out, double* r_out);
+int taco_batch_destroy(taco_batch* b);
+/* One-shot create + eager run + copy-out + destroy.  L4_out[s] (ntau x N^4)
+ * and rho_out[s] ((steps+1) x N^2) may be NULL. */
+int taco_batched_run(taco_ctx* ctx, const taco_batch_system* systems, int nsys, double* const* L4_out,
+                     double* const* rho_out);
 
 #ifdef __cplusplus
 }

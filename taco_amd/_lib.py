@@ -22,6 +22,17 @@ _dp = POINTER(c_double)
 _ip = POINTER(c_int)
 _sp = POINTER(c_size_t)
 
+
+
+class BatchSystem(ctypes.Structure):
+    """taco_batch_system of include/taco_tcl4.h."""
+    _fields_ = [
+        ("gamma", _dp), ("Nt", c_size_t), ("nf", c_int), ("omegas", _dp), ("mirror", _ip),
+        ("pair_to_freq", _ip), ("N", c_int), ("ops", _dp), ("num_ops", c_int), ("dt", c_double),
+        ("tidx", _sp), ("ntau", c_size_t), ("steps", c_size_t), ("order", c_int), ("eps", _dp), ("r0", _dp),
+    ]
+
+
 # name -> (restype, argtypes); every symbol declared in include/taco_tcl4.h.
 SIGNATURES = {
     "taco_last_error": (c_char_p, []),
@@ -32,6 +43,7 @@ SIGNATURES = {
     "taco_set_fcr_fft_pad_factor": (None, [c_size_t]),
     "taco_cr_sincos": (c_int, [c_void_p, _dp, c_size_t, _dp, _dp]),
     "taco_profile_enable": (c_int, [c_void_p, c_int]),
+    "taco_fft_tile_choice": (c_int, [c_void_p, c_int, _ip]),
     "taco_profile_reset": (c_int, [c_void_p]),
     "taco_profile_get": (c_int, [c_void_p, c_int, _dp, POINTER(ctypes.c_long), _dp]),
     "taco_get_fcr_fft_pad_factor": (c_size_t, []),
@@ -50,6 +62,12 @@ SIGNATURES = {
     "taco_rk4_propagate": (c_int, [c_void_p, _dp, _dp, c_size_t, c_int, _dp, c_double, c_size_t, _dp, _sp]),
     "taco_tcl4_propagate": (c_int, [c_void_p, _dp, c_size_t, c_int, _dp, _ip, _ip, c_int, _dp, _dp, c_int,
                                     c_double, c_size_t, c_int, _dp, _dp]),
+    "taco_batch_create": (c_int, [c_void_p, POINTER(BatchSystem), c_int, POINTER(c_void_p)]),
+    "taco_batch_run": (c_int, [c_void_p, c_int]),
+    "taco_batch_l4": (c_int, [c_void_p, c_int, _dp]),
+    "taco_batch_rho": (c_int, [c_void_p, c_int, _dp, _dp]),
+    "taco_batch_destroy": (c_int, [c_void_p]),
+    "taco_batched_run": (c_int, [c_void_p, POINTER(BatchSystem), c_int, POINTER(_dp), POINTER(_dp)]),
 }
 
 _lib = None

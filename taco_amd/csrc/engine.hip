@@ -236,6 +236,8 @@ struct FcrArgs {
     const double2* twB;     // phase B forward stages, per k1 group
     const double2* twC;     // phase C inverse stages, per q group
     const int* job_tri;     // sorted position -> triple id (triples grouped by equal Omega)
+    const unsigned char* job_mask;  // per sorted position: bit0 F needed, bit1 C needed, bit2 R needed
+    long nfft_jobs;         // positions [0, nfft_jobs) need FFTs; the rest only R
     const int* cls_start;   // [ncls + 1] first sorted position of each Omega class
     long job0;              // first sorted position of the FFT chunk
     int njobs;              // triples in the FFT chunk
@@ -377,7 +379,9 @@ __global__ __launch_bounds__(Cfg::kFT) void k_phase_b(FcrArgs P, int mode) {
     int i, j, k;
     decode_triple(P.job_tri[P.job0 + job], P.nf, i, j, k);
     const int bF = P.mirror[j];
+    const int mask = P.job_mask[P.job0 + job];
     for (int path = 0; path < 2; ++path) {
+        if (!(mask & (1 << path))) continue;
         const double2* sp = P.spec + static_cast<long>(path == 0 ? 2 * bF : 2 * j + 1) * P.n +
                             static_cast<long>(k1b) * gs;
 #pragma unroll
@@ -408,6 +412,7 @@ __global__ __launch_bounds__(Cfg::kFT) void k_phase_c(FcrArgs P) {
     xcd_map(blockIdx.x, ngroups / G, jp, gb);
     const int q0 = gb * G;
     if (q0 >= P.T) return;
+    if (!(P.job_mask[P.job0 + jp % P.njobs] & (1 << (jp / P.njobs)))) return;
     const int total = G * gs;
     const double2* in = P.inter2 + static_cast<long>(jp) * P.n + static_cast<long>(q0) * gs;
     for (int e = threadIdx.x; e < total; e += kFT) buf[Tl.at(e >> Tl.lg, e & (gs - 1))] = in[e];
@@ -459,7 +464,10 @@ struct CombineArgs {
 __global__ __launch_bounds__(kBlock) void k_combine(FcrArgs P, CombineArgs O) {
     __shared__ double wsum[8][4];
     const int slot = blockIdx.x;
-    const long tri = P.job_tri[P.comb0 + slot];
+    const long cpos = P.comb0 + slot;
+    const long tri = P.job_tri[cpos];
+    const int mask = P.job_mask[cpos];
+    const bool has_conv = cpos < P.nfft_jobs;
     int i, j, k;
     decode_triple(tri, P.nf, i, j, k);
     const double omega = (P.omegas[i] + P.omegas[j]) + P.omegas[k];
@@ -505,9 +513,9 @@ __global__ __launch_bounds__(kBlock) void k_combine(FcrArgs P, CombineArgs O) {
         const double2 pR = make_double2(v[4] * dt, v[5] * dt);
         const double2 pP = make_double2(v[6] * dt, v[7] * dt);
         const double2 g1p = cmul(gg1, pm);
-        O.Fo[dst] = csub(cmul(g1p, pF), convF[t]);
-        O.Co[dst] = csub(cmul(g1p, pC), convC[t]);
-        O.Ro[dst] = csub(cmul(gg1, pR), pP);
+        if (has_conv && (mask & 1)) O.Fo[dst] = csub(cmul(g1p, pF), convF[t]);
+        if (has_conv && (mask & 2)) O.Co[dst] = csub(cmul(g1p, pC), convC[t]);
+        if (mask & 4) O.Ro[dst] = csub(cmul(gg1, pR), pP);
     }
 }
 
@@ -651,16 +659,17 @@ __device__ void matvec_half(const double2* __restrict__ L0, const double2* __res
     }
 }
 
-__global__ void k_rk4(const double2* Ls, const double2* Lh, long steps, int D, double2* r, double dt, long every,
-                      double2* samples, long* nsamples) {
-    extern __shared__ double2 sh[];
+// One system's RK4 run by the whole workgroup; x, k1..k4, tmp in LDS.  Reads
+// the initial vector from r_in and writes the final one to r_out.
+__device__ void rk4_body(double2* sh, const double2* Ls, const double2* Lh, long steps, int D, const double2* r_in,
+                         double2* r_out, double dt, long every, double2* samples, long* nsamples) {
     double2* x = sh;
     double2* k1 = x + D;
     double2* k2 = k1 + D;
     double2* k3 = k2 + D;
     double2* k4 = k3 + D;
     double2* tmp = k4 + D;
-    for (int e = threadIdx.x; e < D; e += blockDim.x) x[e] = r[e];
+    for (int e = threadIdx.x; e < D; e += blockDim.x) x[e] = r_in[e];
     __syncthreads();
     const long D2 = static_cast<long>(D) * D;
     long ns = 0;
@@ -697,8 +706,43 @@ __global__ void k_rk4(const double2* Ls, const double2* Lh, long steps, int D, d
         for (int e = threadIdx.x; e < D; e += blockDim.x) samples[ns * D + e] = x[e];
         ++ns;
     }
-    for (int e = threadIdx.x; e < D; e += blockDim.x) r[e] = x[e];
+    for (int e = threadIdx.x; e < D; e += blockDim.x) r_out[e] = x[e];
     if (threadIdx.x == 0 && nsamples) *nsamples = ns;
+}
+
+__global__ void k_rk4(const double2* Ls, const double2* Lh, long steps, int D, double2* r, double dt, long every,
+                      double2* samples, long* nsamples) {
+    extern __shared__ double2 sh[];
+    rk4_body(sh, Ls, Lh, steps, D, r, r, dt, every, samples, nsamples);
+}
+
+// Many independent systems, one workgroup each, in a single launch.
+struct Rk4Sys {
+    const double2* Ls;   // [steps + 1][D x D]
+    const double2* r0;   // [D]
+    double2* r;          // [D] final vector
+    double2* samples;    // [steps + 1][D]
+    long steps;
+    double dt;
+    int D;
+};
+
+__global__ void k_rk4_batch(const Rk4Sys* sys) {
+    extern __shared__ double2 sh[];
+    const Rk4Sys S = sys[blockIdx.x];
+    rk4_body(sh, S.Ls, nullptr, S.steps, S.D, S.r0, S.r, S.dt, 1, S.samples, nullptr);
+}
+
+// dst[q] = src[rows[q]] for rows of `len` complex values.
+__global__ __launch_bounds__(kBlock) void k_gather_rows(const double2* src, const int* rows, long len, double2* dst) {
+    const double2* s = src + static_cast<long>(rows[blockIdx.x]) * len;
+    double2* d = dst + static_cast<long>(blockIdx.x) * len;
+    for (long e = threadIdx.x; e < len; e += kBlock) d[e] = s[e];
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_zero(double2* dst, long count) {
+    const long i = blockIdx.x * static_cast<long>(kBlock) + threadIdx.x;
+    if (i < count) dst[i] = make_double2(0.0, 0.0);
 }
 
 // TCL2 generator at every t (generator.cpp:81-177 with the drivers' single-
@@ -876,7 +920,8 @@ struct taco_ctx {
     std::map<int, std::array<int, 3>> fft_choice;  // per FFT log-length: tile config of phases A, B, C
     std::map<int, std::unique_ptr<taco_engine::Twiddles>> tw;
     taco_engine::DevBuf gamma, omegas, mirror, p2f, ops, spec, inter1, inter2, conv, slices, t2pos, l4, gw, misc,
-        rk, eps, jobtri, clsstart;
+        rk, eps, jobtri, clsstart, jobmask;
+    std::size_t l4_lds_set = 0;  // dynamic LDS limit set on the fused L4 kernel
 };
 
 namespace taco_engine {
@@ -1047,14 +1092,82 @@ struct FcrPlan {
     const int* mirror;
     double dt;
     std::size_t T;
+    const std::vector<unsigned char>* need;  // per triple: bit0 F, bit1 C, bit2 R (null: all)
 };
+
+// Triples the L4 assembly reads (tcl4_mikx.cpp:101-132 gathers through the
+// NAKZWAN sums, tcl4_assemble.cpp:240-304): F at the M1 and I patterns, R at
+// the M2 and K patterns and wherever X is read, C wherever X is read.  Only
+// these need the FFT work; every other slice entry is never touched.
+static std::vector<unsigned char> needed_triples(const int* p2f, int N, int nf) {
+    const long nf_ = nf;
+    std::vector<unsigned char> need(static_cast<std::size_t>(nf_ * nf_ * nf_), 0);
+    auto f = [&](int r, int c) { return static_cast<long>(p2f[r + c * N]); };
+    auto mark = [&](long x, long y, long z, unsigned char bit) { need[static_cast<std::size_t>((x * nf_ + y) * nf_ + z)] |= bit; };
+    for (int p = 0; p < N; ++p)
+        for (int q = 0; q < N; ++q)
+            for (int u = 0; u < N; ++u)
+                for (int v = 0; v < N; ++v) {
+                    // off-diagonal terms with (b, a, i, n) = (p, q, u, v)
+                    mark(f(p, q), f(p, u), f(v, p), 1);  // M: F[f(b,a)][f(b,i)][f(n,b)]
+                    mark(f(p, q), f(v, q), f(q, u), 4);  // M: R[f(b,a)][f(n,a)][f(a,i)]
+                    mark(f(q, v), f(p, u), f(v, p), 1);  // I: F[f(a,n)][f(b,i)][f(n,b)]
+                    mark(f(u, p), f(v, q), f(q, u), 4);  // K: R[f(i,b)][f(n,a)][f(a,i)]
+                    // diagonal terms with (c, b, a, i) = (p, q, u, v)
+                    mark(f(p, q), f(p, v), f(u, p), 1);  // M: F[f(c,b)][f(c,i)][f(a,c)]
+                    mark(f(p, q), f(u, q), f(q, v), 4);  // M: R[f(c,b)][f(a,b)][f(b,i)]
+                    mark(f(v, p), f(u, q), f(q, v), 4);  // K: R[f(i,c)][f(a,b)][f(b,i)]
+                    mark(f(q, u), f(p, v), f(u, p), 1);  // I: F[f(b,a)][f(c,i)][f(a,c)]
+                }
+    // X = C + R at (f(a,n), f(j,b), f(n,i)), (f(i,a), f(j,b), f(n,i)),
+    // (f(b,a), f(j,m), f(a,i)), (f(i,b), f(j,m), f(a,i)): an outer pair times
+    // any middle bucket.
+    std::vector<char> mid(static_cast<std::size_t>(nf), 0), outer(static_cast<std::size_t>(nf_ * nf_), 0);
+    for (int x = 0; x < N; ++x)
+        for (int y = 0; y < N; ++y) mid[static_cast<std::size_t>(f(x, y))] = 1;
+    // With (x, y, z) = (a, n, i) the first two patterns read the outer pairs
+    // (f(x,y), f(y,z)) and (f(z,x), f(y,z)); with (x, y, z) = (b, a, i) the
+    // last two read the same two forms.
+    for (int x = 0; x < N; ++x)
+        for (int y = 0; y < N; ++y)
+            for (int z = 0; z < N; ++z) {
+                outer[static_cast<std::size_t>(f(x, y) * nf_ + f(y, z))] = 1;
+                outer[static_cast<std::size_t>(f(z, x) * nf_ + f(y, z))] = 1;
+            }
+    for (long o1 = 0; o1 < nf_; ++o1)
+        for (long o3 = 0; o3 < nf_; ++o3)
+            if (outer[static_cast<std::size_t>(o1 * nf_ + o3)])
+                for (long m = 0; m < nf_; ++m)
+                    if (mid[static_cast<std::size_t>(m)]) mark(o1, m, o3, 6);
+    return need;
+}
 
 static std::size_t env_bytes(const char* name, std::size_t dflt) {
     const char* v = std::getenv(name);
     return v ? static_cast<std::size_t>(std::strtoull(v, nullptr, 10)) : dflt;
 }
 
-static void run_fcr(taco_ctx* c, const FcrPlan& pl, const CombineArgs& out) {
+// One F/C/R pass made ready to launch: the job lists live in device memory,
+// the launch geometry on the host.  Workspace pointers are taken from the
+// context at launch time, so passes prepared one after another can share it.
+struct FcrJob {
+    FcrArgs P{};
+    std::vector<int> cls_start;  // host copy of the Omega-class starts
+    std::vector<long> paths;     // [ncomb + 1] prefix count of the F and C paths (traffic accounting)
+    std::vector<long> outs;      // [ncomb + 1] prefix count of the F, C and R outputs
+    long nfft_jobs = 0, ncomb = 0, chunk = 1, cconv = 1, spec_jobs = 0;
+    std::size_t n = 0, Nt = 0, T = 0;
+    int L = 0;
+};
+
+// Where a pass keeps its device job lists.
+struct FcrLists {
+    DevBuf* jobtri;
+    DevBuf* jobmask;
+    DevBuf* clsstart;
+};
+
+static FcrJob fcr_prepare(taco_ctx* c, const FcrPlan& pl, const FcrLists& lists) {
     const std::size_t n = fcr_nfft(pl.Nt, g_pad_factor);
     int L = 0;
     while ((std::size_t{1} << L) < n) ++L;
@@ -1064,41 +1177,60 @@ static void run_fcr(taco_ctx* c, const FcrPlan& pl, const CombineArgs& out) {
     const int nf = pl.nf;
     const long nf3 = static_cast<long>(nf) * nf * nf;
 
-    // Group the triples by the bit pattern of Omega = (w_i + w_j) + w_k (the
-    // kernels' expression): one class shares its phases exp(-i Omega t_k).
-    std::vector<int> order(static_cast<std::size_t>(nf3));
+    // Triples to compute: every one (compute_triple_kernels) or the ones the
+    // L4 assembly reads.  Those needing F or C get FFTs and are grouped by the
+    // bit pattern of Omega = (w_i + w_j) + w_k (the kernels' expression) so one
+    // class shares its phases exp(-i Omega t_k); R-only triples follow.
+    std::vector<int> order;
+    std::vector<unsigned char> mask;
     std::vector<std::uint64_t> key(static_cast<std::size_t>(nf3));
     for (long t = 0; t < nf3; ++t) {
         const int k = static_cast<int>(t % nf), j = static_cast<int>((t / nf) % nf), i = static_cast<int>(t / (nf * nf));
         const double om = (pl.h_omegas[i] + pl.h_omegas[j]) + pl.h_omegas[k];
         std::memcpy(&key[static_cast<std::size_t>(t)], &om, 8);
-        order[static_cast<std::size_t>(t)] = static_cast<int>(t);
+        const unsigned char m = pl.need ? (*pl.need)[static_cast<std::size_t>(t)] : 7;
+        if (m & 3) order.push_back(static_cast<int>(t));
     }
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key[x] < key[y]; });
+    const long nfft_jobs = static_cast<long>(order.size());
+    for (long t = 0; t < nf3; ++t) {
+        const unsigned char m = pl.need ? (*pl.need)[static_cast<std::size_t>(t)] : 7;
+        if (m == 4) order.push_back(static_cast<int>(t));
+    }
+    const long ncomb = static_cast<long>(order.size());
+    mask.resize(order.size());
+    for (std::size_t q = 0; q < order.size(); ++q)
+        mask[q] = pl.need ? (*pl.need)[static_cast<std::size_t>(order[q])] : 7;
     std::vector<int> cls_start;
-    for (long p = 0; p < nf3; ++p)
+    for (long p = 0; p < nfft_jobs; ++p)
         if (p == 0 || key[order[p]] != key[order[p - 1]]) cls_start.push_back(static_cast<int>(p));
-    cls_start.push_back(static_cast<int>(nf3));
-    c->jobtri.ensure(order.size() * sizeof(int));
-    c->clsstart.ensure(cls_start.size() * sizeof(int));
-    HIP_OK(hipMemcpyAsync(c->jobtri.p, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    HIP_OK(hipMemcpyAsync(c->clsstart.p, cls_start.data(), cls_start.size() * sizeof(int), hipMemcpyHostToDevice,
-                          c->stream));
+    cls_start.push_back(static_cast<int>(nfft_jobs));
+    lists.jobtri->ensure(std::max<std::size_t>(order.size(), 1) * sizeof(int));
+    lists.jobmask->ensure(std::max<std::size_t>(mask.size(), 1));
+    lists.clsstart->ensure(cls_start.size() * sizeof(int));
+    if (!order.empty()) {
+        HIP_OK(hipMemcpyAsync(lists.jobtri->p, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice,
+                              c->stream));
+        HIP_OK(hipMemcpyAsync(lists.jobmask->p, mask.data(), mask.size(), hipMemcpyHostToDevice, c->stream));
+    }
+    HIP_OK(hipMemcpyAsync(lists.clsstart->p, cls_start.data(), cls_start.size() * sizeof(int),
+                          hipMemcpyHostToDevice, c->stream));
 
     // FFT chunk (intermediates of nj triples) and combine batch (conv slots).
     const std::size_t per_job = 3 * n * 16;
     long chunk = static_cast<long>(std::max<std::size_t>(1, chunk_budget_bytes() / per_job));
-    chunk = std::min(chunk, nf3);
+    chunk = std::max(1L, std::min(chunk, nfft_jobs));
     const std::size_t Tpad = std::max<std::size_t>(pl.T, 1);
     long cconv = static_cast<long>(env_bytes("TACO_CONV_BYTES", std::size_t{4} << 30) / (2 * Tpad * 16));
     cconv = std::max(chunk, (cconv / chunk) * chunk);
-    cconv = std::min(cconv, nf3);
+    cconv = std::max(1L, std::min(cconv, ncomb));
     const long spec_jobs = 2L * nf;
     c->spec.ensure(static_cast<std::size_t>(spec_jobs) * n * 16);
     c->inter1.ensure(static_cast<std::size_t>(std::max(chunk, spec_jobs)) * n * 16);
     c->inter2.ensure(static_cast<std::size_t>(2 * chunk) * n * 16);
     c->conv.ensure(static_cast<std::size_t>(2 * cconv) * Tpad * 16);
-    FcrArgs P{};
+    FcrJob J;
+    FcrArgs& P = J.P;
     P.gamma = pl.gamma;
     P.omegas = pl.omegas;
     P.mirror = pl.mirror;
@@ -1112,14 +1244,45 @@ static void run_fcr(taco_ctx* c, const FcrPlan& pl, const CombineArgs& out) {
     P.twStd = tw.std_.as<double2>();
     P.twB = tw.b_.as<double2>();
     P.twC = tw.c_.as<double2>();
-    P.job_tri = c->jobtri.as<int>();
-    P.cls_start = c->clsstart.as<int>();
+    P.job_tri = lists.jobtri->as<int>();
+    P.job_mask = lists.jobmask->as<unsigned char>();
+    P.nfft_jobs = nfft_jobs;
+    P.cls_start = lists.clsstart->as<int>();
+    P.cconv = cconv;
+    P.T = static_cast<long>(pl.T);
+    J.cls_start = std::move(cls_start);
+    J.paths.assign(static_cast<std::size_t>(ncomb) + 1, 0);
+    J.outs.assign(static_cast<std::size_t>(ncomb) + 1, 0);
+    for (long p = 0; p < ncomb; ++p) {
+        const int fc = p < nfft_jobs ? (mask[p] & 1) + ((mask[p] >> 1) & 1) : 0;
+        J.paths[p + 1] = J.paths[p] + fc;
+        J.outs[p + 1] = J.outs[p] + fc + ((mask[p] >> 2) & 1);
+    }
+    J.nfft_jobs = nfft_jobs;
+    J.ncomb = ncomb;
+    J.chunk = chunk;
+    J.cconv = cconv;
+    J.spec_jobs = spec_jobs;
+    J.n = n;
+    J.Nt = pl.Nt;
+    J.T = pl.T;
+    J.L = L;
+    return J;
+}
+
+// Launches one prepared pass on the context stream.  The first pass of an
+// FFT length picks the tile configurations (autotune synchronises the
+// stream); later passes of that length only launch kernels.
+static void fcr_launch(taco_ctx* c, FcrJob& J, const CombineArgs& out) {
+    FcrArgs& P = J.P;
     P.inter1 = c->inter1.as<double2>();
     P.inter2 = c->inter2.as<double2>();
     P.spec = c->spec.as<double2>();
     P.conv = c->conv.as<double2>();
-    P.cconv = cconv;
-    P.T = static_cast<long>(pl.T);
+    const int L = J.L;
+    const long nfft_jobs = J.nfft_jobs, ncomb = J.ncomb, chunk = J.chunk, cconv = J.cconv;
+    const long spec_jobs = J.spec_jobs;
+    const std::vector<int>& cls_start = J.cls_start;
     // Tile configuration per phase: TACO_FFT_TILE=<0..3> forces one; otherwise
     // the first chunk of each FFT length is run with every configuration
     // (identical butterflies, identical results) and the fastest is kept.
@@ -1128,9 +1291,9 @@ static void run_fcr(taco_ctx* c, const FcrPlan& pl, const CombineArgs& out) {
     auto& choice = it->second;
     const int forced = env_int("TACO_FFT_TILE", -1);
     if (forced >= 0 && forced < kNumFftCfg) choice = {forced, forced, forced};
-    const double cn = static_cast<double>(n) * 16.0;      // one complex series of length Nfft
-    const double cNt = static_cast<double>(pl.Nt) * 16.0;
-    const double cT = static_cast<double>(pl.T) * 16.0;
+    const double cn = static_cast<double>(J.n) * 16.0;      // one complex series of length Nfft
+    const double cNt = static_cast<double>(J.Nt) * 16.0;
+    const double cT = static_cast<double>(J.T) * 16.0;
     // Spectra of every Gamma column and its conjugate.
     P.njobs = static_cast<int>(spec_jobs);
     timed(c, K_SPECTRA, spec_jobs * (cNt + 3 * cn), [&] {
@@ -1142,11 +1305,12 @@ static void run_fcr(taco_ctx* c, const FcrPlan& pl, const CombineArgs& out) {
         return static_cast<int>(std::upper_bound(cls_start.begin(), cls_start.end(), static_cast<int>(pos)) -
                                 cls_start.begin()) - 1;
     };
-    for (long comb0 = 0; comb0 < nf3; comb0 += cconv) {
-        const long cb = std::min(cconv, nf3 - comb0);
+    for (long comb0 = 0; comb0 < ncomb; comb0 += cconv) {
+        const long cb = std::min(cconv, ncomb - comb0);
         P.comb0 = comb0;
-        for (long job0 = comb0; job0 < comb0 + cb; job0 += chunk) {
-            const long nj = std::min(chunk, comb0 + cb - job0);
+        const long fft_end = std::min(comb0 + cb, nfft_jobs);
+        for (long job0 = comb0; job0 < fft_end; job0 += chunk) {
+            const long nj = std::min(chunk, fft_end - job0);
             P.job0 = job0;
             P.njobs = static_cast<int>(nj);
             P.cls0 = class_of(job0);
@@ -1154,20 +1318,29 @@ static void run_fcr(taco_ctx* c, const FcrPlan& pl, const CombineArgs& out) {
             if (choice[0] < 0 || choice[1] < 0 || choice[2] < 0) autotune(c, P, ncls, nj, choice);
             // phase A: read the Gamma column, write the half-transformed signal
             timed(c, K_PHASE_A, nj * (cNt + cn), [&] { launch_phase(choice[0], PH_A_TRIPLE, P, ncls, c->stream); });
-            // phase B: read it and two spectra, write two half-inverse products
-            timed(c, K_PHASE_B, nj * 5 * cn, [&] { launch_phase(choice[1], PH_B_TRIPLE, P, nj, c->stream); });
-            // phase C: read both, write the two convolutions for t < T
-            timed(c, K_PHASE_C, nj * 2 * (cn + cT), [&] { launch_phase(choice[2], PH_C, P, nj, c->stream); });
+            // F and C paths of the chunk (one or both per triple)
+            const double np_ = static_cast<double>(J.paths[job0 + nj] - J.paths[job0]);
+            // phase B: read it and one spectrum per path, write one half-inverse product per path
+            timed(c, K_PHASE_B, nj * cn + np_ * 2 * cn, [&] { launch_phase(choice[1], PH_B_TRIPLE, P, nj, c->stream); });
+            // phase C: read each path's product, write its convolution for t < T
+            timed(c, K_PHASE_C, np_ * (cn + cT), [&] { launch_phase(choice[2], PH_C, P, nj, c->stream); });
             HIP_OK(hipGetLastError());
         }
-        // combine: read three Gamma columns and two convolutions over t < T,
-        // write F/C/R at the kept times
-        const double kept = out.out_mode == 0 ? static_cast<double>(out.npos) : static_cast<double>(pl.T);
-        timed(c, K_COMBINE, cb * (5 * cT + 3 * 16.0 * kept), [&] {
+        // combine: read three Gamma columns and each path's convolution over
+        // t < T, write the needed F/C/R values at the kept times
+        const double kept = out.out_mode == 0 ? static_cast<double>(out.npos) : static_cast<double>(J.T);
+        const double cpaths = static_cast<double>(J.paths[comb0 + cb] - J.paths[comb0]);
+        const double couts = static_cast<double>(J.outs[comb0 + cb] - J.outs[comb0]);
+        timed(c, K_COMBINE, cb * 3 * cT + cpaths * cT + couts * 16.0 * kept, [&] {
             hipLaunchKernelGGL(k_combine, dim3(static_cast<unsigned>(cb)), dim3(kBlock), 0, c->stream, P, out);
         });
         HIP_OK(hipGetLastError());
     }
+}
+
+static void run_fcr(taco_ctx* c, const FcrPlan& pl, const CombineArgs& out) {
+    FcrJob J = fcr_prepare(c, pl, FcrLists{&c->jobtri, &c->jobmask, &c->clsstart});
+    fcr_launch(c, J, out);
 }
 
 static std::size_t l4_lds_bytes(int N, int num_ops) {
@@ -1175,14 +1348,22 @@ static std::size_t l4_lds_bytes(int N, int num_ops) {
     return N2 * N2 * 16 + static_cast<std::size_t>(num_ops) * N2 * 16 + N2 * 4;
 }
 
+// Raises the fused L4 kernel's dynamic LDS limit when a launch needs more than
+// 64 KiB; once per context and size, so a captured graph never calls it.
+static void set_l4_lds(taco_ctx* c, std::size_t lds) {
+    if (lds > 64 * 1024 && lds > c->l4_lds_set) {
+        HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_l4), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   static_cast<int>(lds)));
+        c->l4_lds_set = lds;
+    }
+}
+
 static void launch_l4(taco_ctx* c, const L4Args& A, long npos) {
     const std::size_t lds = l4_lds_bytes(A.N, A.num_ops);
     if (lds > 160 * 1024)
         throw Error(TACO_ERR_INVALID, "fused L4 kernel supports N^4 * 16 B + ops in 160 KiB of LDS (N <= 8)");
     if (npos <= 0) return;
-    if (lds > 64 * 1024)
-        HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_l4), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   static_cast<int>(lds)));
+    set_l4_lds(c, lds);
     const double N4 = static_cast<double>(A.N) * A.N * A.N * A.N;
     // per time: read the three nf^3 slices, write one N^4 block
     timed(c, K_L4, npos * (3.0 * A.nf3 + N4) * 16.0, [&] {
@@ -1245,8 +1426,9 @@ static TimePlan plan_times(const std::size_t* tidx, std::size_t ntau, std::size_
 // Core of the batch builders on device-resident inputs.  Writes L4 for the
 // unique times into c->l4 and returns the plan; fcr_ms optional.
 static TimePlan l4_core(taco_ctx* c, const double2* d_gamma, std::size_t Nt, int nf, const double* d_omegas,
-                        const double* h_omegas, const int* d_mirror, const int* d_p2f, int N, const double2* d_ops,
-                        int num_ops, double dt, const std::size_t* tidx, std::size_t ntau, double* fcr_ms) {
+                        const double* h_omegas, const int* d_mirror, const int* d_p2f, const int* h_p2f, int N,
+                        const double2* d_ops, int num_ops, double dt, const std::size_t* tidx, std::size_t ntau,
+                        double* fcr_ms) {
     TimePlan tp = plan_times(tidx, ntau, Nt);
     const long npos = static_cast<long>(tp.uniq.size());
     if (npos == 0) return tp;
@@ -1267,8 +1449,11 @@ static TimePlan l4_core(taco_ctx* c, const double2* d_gamma, std::size_t Nt, int
     out.Fo = sl;
     out.Co = sl + nf3 * npos;
     out.Ro = sl + 2 * nf3 * npos;
+    const std::vector<unsigned char> need = needed_triples(h_p2f, N, nf);
     if (fcr_ms) HIP_OK(hipEventRecord(c->ev0, c->stream));
-    FcrPlan pl{d_gamma, Nt, nf, d_omegas, h_omegas, d_mirror, dt, tp.T};
+    if (env_int("TACO_DEBUG_POISON", 0))  // NaN-fill the slices: an entry L4 reads but nobody wrote shows up
+        HIP_OK(hipMemsetAsync(c->slices.p, 0xff, 3 * nf3 * static_cast<std::size_t>(npos) * 16, c->stream));
+    FcrPlan pl{d_gamma, Nt, nf, d_omegas, h_omegas, d_mirror, dt, tp.T, &need};
     run_fcr(c, pl, out);
     if (fcr_ms) HIP_OK(hipEventRecord(c->ev1, c->stream));
     L4Args A{};
@@ -1302,6 +1487,112 @@ static T* upload(taco_ctx* c, DevBuf& buf, const T* host, std::size_t count) {
 
 static void check_ctx(taco_ctx* c) {
     if (!c) throw Error(TACO_ERR_INVALID, "null taco_ctx");
+}
+
+// ---------------------------------------------------------------------------
+// Batched run: many independent systems (uneven N, bath, grid length, time
+// set) with every input, list and output resident in HBM.  Each system's
+// F/C/R pass, fused L4 and TCL2 assembly are enqueued back to back with no
+// host round trip, and one RK4 launch steps every system (a workgroup each).
+// The whole sequence is captured once into a hipGraph and replayed.
+// ---------------------------------------------------------------------------
+struct BatchSys {
+    std::size_t Nt = 0, T = 0, ntau = 0, steps = 0;
+    int nf = 0, N = 0, num_ops = 0, order = 4;
+    double dt = 0.0;
+    long npos = 0;
+    DevBuf gamma, omegas, mirror, p2f, ops, eps, r0;
+    DevBuf jobtri, jobmask, clsstart, t2pos, rows_out, rows_gen;
+    DevBuf l4out;  // [ntau][N^4] at the requested times, request order
+    DevBuf lgen;   // [steps + 1][N^4] generator series of the RK4 run
+    DevBuf rho;    // [steps + 1][N^2] samples, and the final vector after them
+    FcrJob fcr;
+};
+
+struct BatchRun {
+    std::vector<std::unique_ptr<BatchSys>> sys;
+    DevBuf rkargs;
+    long nrk = 0;
+    std::size_t rk_lds = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool ran_eager = false;
+    ~BatchRun() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+// Enqueues one system of a batch: F/C/R at its kept times, fused L4 into the
+// context scratch, rows copied to the outputs, TCL2 added to the generator.
+static void batch_enqueue_system(taco_ctx* c, BatchSys& S) {
+    const std::size_t N2 = static_cast<std::size_t>(S.N) * S.N, N4 = N2 * N2;
+    const std::size_t nf3 = static_cast<std::size_t>(S.nf) * S.nf * S.nf;
+    if (S.npos > 0) {
+        CombineArgs out{};
+        out.out_mode = 0;
+        out.t2pos = S.t2pos.as<int>();
+        out.npos = S.npos;
+        out.nf3 = static_cast<long>(nf3);
+        double2* sl = c->slices.as<double2>();
+        out.Fo = sl;
+        out.Co = sl + nf3 * S.npos;
+        out.Ro = sl + 2 * nf3 * S.npos;
+        fcr_launch(c, S.fcr, out);
+        L4Args A{};
+        A.Fs = out.Fo;
+        A.Cs = out.Co;
+        A.Rs = out.Ro;
+        A.p2f = S.p2f.as<int>();
+        A.ops = S.ops.as<double2>();
+        A.N = S.N;
+        A.nf = S.nf;
+        A.num_ops = S.num_ops;
+        A.nf3 = static_cast<long>(nf3);
+        A.L4 = c->l4.as<double2>();
+        A.GW = nullptr;
+        launch_l4(c, A, S.npos);
+        if (S.ntau)
+            hipLaunchKernelGGL(k_gather_rows, dim3(static_cast<unsigned>(S.ntau)), dim3(kBlock), 0, c->stream,
+                               c->l4.as<double2>(), S.rows_out.as<int>(), static_cast<long>(N4),
+                               S.l4out.as<double2>());
+    }
+    if (S.steps) {
+        const long gen = static_cast<long>((S.steps + 1) * N4);
+        if (S.order == 4)
+            hipLaunchKernelGGL(k_gather_rows, dim3(static_cast<unsigned>(S.steps + 1)), dim3(kBlock), 0, c->stream,
+                               c->l4.as<double2>(), S.rows_gen.as<int>(), static_cast<long>(N4), S.lgen.as<double2>());
+        else
+            hipLaunchKernelGGL(k_fill_zero, dim3(static_cast<unsigned>((gen + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                               c->stream, S.lgen.as<double2>(), gen);
+        Tcl2Args A2{};
+        A2.gamma = S.gamma.as<double2>();
+        A2.Nt = static_cast<long>(S.Nt);
+        A2.eps = S.eps.as<double>();
+        A2.ops = S.ops.as<double2>();
+        A2.p2f = S.p2f.as<int>();
+        A2.N = S.N;
+        A2.nf = S.nf;
+        A2.num_ops = S.num_ops;
+        A2.L = S.lgen.as<double2>();
+        A2.order = S.order;
+        timed(c, K_TCL2, static_cast<double>(gen) * 32.0, [&] {
+            hipLaunchKernelGGL(k_tcl2, dim3(static_cast<unsigned>(S.steps + 1)), dim3(kBlock), 2 * N2 * 16, c->stream,
+                               A2);
+        });
+    }
+    HIP_OK(hipGetLastError());
+}
+
+static void batch_enqueue(taco_ctx* c, BatchRun& B) {
+    for (auto& s : B.sys) batch_enqueue_system(c, *s);
+    if (B.nrk) {
+        timed(c, K_RK4, 0.0, [&] {
+            hipLaunchKernelGGL(k_rk4_batch, dim3(static_cast<unsigned>(B.nrk)), dim3(kBlock), B.rk_lds, c->stream,
+                               B.rkargs.as<Rk4Sys>());
+        });
+        HIP_OK(hipGetLastError());
+    }
 }
 
 }  // namespace taco_engine
@@ -1380,6 +1671,14 @@ int taco_profile_get(taco_ctx* ctx, int kernel, double* total_ms, long* launches
     });
 }
 
+int taco_fft_tile_choice(taco_ctx* ctx, int log2_nfft, int* cfg3) {
+    return guarded([&] {
+        check_ctx(ctx);
+        auto it = ctx->fft_choice.find(log2_nfft);
+        for (int k = 0; k < 3; ++k) cfg3[k] = it == ctx->fft_choice.end() ? -1 : it->second[k];
+    });
+}
+
 int taco_cr_sincos(taco_ctx* ctx, const double* x, size_t n, double* s, double* c) {
     return guarded([&] {
         if (!ctx) {
@@ -1443,7 +1742,7 @@ int taco_triple_kernels(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, c
         out.Fo = sl;
         out.Co = sl + nf3 * keep;
         out.Ro = sl + 2 * nf3 * keep;
-        FcrPlan pl{dg, Nt, nf, dom, omegas, dmi, dt, keep};
+        FcrPlan pl{dg, Nt, nf, dom, omegas, dmi, dt, keep, nullptr};
         run_fcr(ctx, pl, out);
         HIP_OK(hipMemcpyAsync(F, out.Fo, nf3 * keep * 16, hipMemcpyDeviceToHost, ctx->stream));
         HIP_OK(hipMemcpyAsync(C, out.Co, nf3 * keep * 16, hipMemcpyDeviceToHost, ctx->stream));
@@ -1466,7 +1765,8 @@ int taco_l4_batch(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, const d
         const int* dmi = upload(ctx, ctx->mirror, mirror, static_cast<std::size_t>(nf));
         const int* dpf = upload(ctx, ctx->p2f, pair_to_freq, N2);
         const double2* dops = upload(ctx, ctx->ops, reinterpret_cast<const double2*>(ops), N2 * num_ops);
-        TimePlan tp = l4_core(ctx, dg, Nt, nf, dom, omegas, dmi, dpf, N, dops, num_ops, dt, tidx, ntau, fcr_ms);
+        TimePlan tp = l4_core(ctx, dg, Nt, nf, dom, omegas, dmi, dpf, pair_to_freq, N, dops, num_ops, dt, tidx, ntau,
+                              fcr_ms);
         if (tp.uniq.empty()) return;
         std::vector<double> host(N4 * tp.uniq.size() * 2);
         HIP_OK(hipMemcpyAsync(host.data(), ctx->l4.p, host.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -1493,7 +1793,8 @@ int taco_l4_batch_dev(taco_ctx* ctx, const void* d_gamma, size_t Nt, int nf, con
         HIP_OK(hipMemcpy(hom.data(), d_omegas, hom.size() * sizeof(double), hipMemcpyDeviceToHost));
         validate_system(Nt, nf, hm.data(), hp.data(), N, num_ops, dt, method);
         TimePlan tp = l4_core(ctx, static_cast<const double2*>(d_gamma), Nt, nf, static_cast<const double*>(d_omegas),
-                              hom.data(), static_cast<const int*>(d_mirror), static_cast<const int*>(d_pair_to_freq), N,
+                              hom.data(), static_cast<const int*>(d_mirror), static_cast<const int*>(d_pair_to_freq),
+                              hp.data(), N,
                               static_cast<const double2*>(d_ops), num_ops, dt, tidx, ntau, fcr_ms);
         if (tp.uniq.empty()) return;
         double2* out = static_cast<double2*>(d_L4_out);
@@ -1628,7 +1929,8 @@ int taco_tcl4_propagate(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, c
         if (order == 4) {
             std::vector<std::size_t> all(T);
             std::iota(all.begin(), all.end(), std::size_t{0});
-            l4_core(ctx, dg, Nt, nf, dom, omegas, dmi, dpf, N, dops, num_ops, dt, all.data(), T, nullptr);
+            l4_core(ctx, dg, Nt, nf, dom, omegas, dmi, dpf, pair_to_freq, N, dops, num_ops, dt, all.data(), T,
+                    nullptr);
         } else {
             ctx->l4.ensure(T * N4 * 16);
             HIP_OK(hipMemsetAsync(ctx->l4.p, 0, T * N4 * 16, ctx->stream));
@@ -1656,6 +1958,200 @@ int taco_tcl4_propagate(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, c
         if (samples_out) HIP_OK(hipMemcpyAsync(samples_out, ds, T * N2 * 16, hipMemcpyDeviceToHost, ctx->stream));
         HIP_OK(hipStreamSynchronize(ctx->stream));
     });
+}
+
+struct taco_batch {
+    taco_ctx* ctx = nullptr;
+    taco_engine::BatchRun run;
+};
+
+int taco_batch_create(taco_ctx* ctx, const taco_batch_system* systems, int nsys, taco_batch** out) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!out) throw Error(TACO_ERR_INVALID, "null output handle");
+        *out = nullptr;
+        if (nsys < 0 || (nsys > 0 && !systems)) throw Error(TACO_ERR_INVALID, "invalid system list");
+        DeviceGuard g(ctx->device);
+        auto b = std::make_unique<taco_batch>();
+        b->ctx = ctx;
+        BatchRun& B = b->run;
+        std::vector<Rk4Sys> rk;
+        for (int q = 0; q < nsys; ++q) {
+            const taco_batch_system& d = systems[q];
+            validate_system(d.Nt, d.nf, d.mirror, d.pair_to_freq, d.N, d.num_ops, d.dt, TACO_FCR_CONVOLUTION);
+            if (!d.gamma || !d.omegas || !d.mirror || !d.pair_to_freq || !d.ops)
+                throw Error(TACO_ERR_INVALID, "system inputs must be non-null");
+            if (!(d.order == 0 || d.order == 2 || d.order == 4)) throw Error(TACO_ERR_INVALID, "order must be 0, 2, or 4");
+            if (d.ntau && !d.tidx) throw Error(TACO_ERR_INVALID, "null time index list");
+            if (d.steps && (!d.r0 || !d.eps)) throw Error(TACO_ERR_INVALID, "RK4 needs r0 and eps");
+            if (d.steps + 1 > d.Nt && d.steps) throw Error(TACO_ERR_RANGE, "steps exceed gamma_series rows");
+            auto S = std::make_unique<BatchSys>();
+            S->Nt = d.Nt;
+            S->nf = d.nf;
+            S->N = d.N;
+            S->num_ops = d.num_ops;
+            S->dt = d.dt;
+            S->ntau = d.ntau;
+            S->steps = d.steps;
+            S->order = d.order;
+            const std::size_t N2 = static_cast<std::size_t>(d.N) * d.N, N4 = N2 * N2;
+            auto put = [&](DevBuf& buf, const void* host, std::size_t bytes) {
+                buf.ensure(std::max<std::size_t>(bytes, 16));
+                if (bytes) HIP_OK(hipMemcpy(buf.p, host, bytes, hipMemcpyHostToDevice));
+            };
+            put(S->gamma, d.gamma, d.Nt * d.nf * 16);
+            put(S->omegas, d.omegas, d.nf * sizeof(double));
+            put(S->mirror, d.mirror, d.nf * sizeof(int));
+            put(S->p2f, d.pair_to_freq, N2 * sizeof(int));
+            put(S->ops, d.ops, N2 * d.num_ops * 16);
+            if (d.steps) {
+                put(S->eps, d.eps, d.N * sizeof(double));
+                put(S->r0, d.r0, N2 * 16);
+            }
+            // L4 times: the requested ones and, for a TCL4 run, every RK4 step.
+            std::vector<std::size_t> want(d.tidx, d.tidx + d.ntau);
+            if (d.steps && d.order == 4)
+                for (std::size_t t = 0; t <= d.steps; ++t) want.push_back(t);
+            if (!want.empty()) {
+                const TimePlan tp = plan_times(want.data(), want.size(), d.Nt);
+                S->npos = static_cast<long>(tp.uniq.size());
+                S->T = tp.T;
+                put(S->t2pos, tp.t2pos.data(), tp.T * sizeof(int));
+                std::vector<int> rows(tp.req2u.begin(), tp.req2u.begin() + d.ntau);
+                put(S->rows_out, rows.data(), rows.size() * sizeof(int));
+                if (d.steps && d.order == 4) {
+                    std::vector<int> gen(tp.req2u.begin() + d.ntau, tp.req2u.end());
+                    put(S->rows_gen, gen.data(), gen.size() * sizeof(int));
+                }
+                const std::vector<unsigned char> need = needed_triples(d.pair_to_freq, d.N, d.nf);
+                FcrPlan pl{S->gamma.as<double2>(), d.Nt, d.nf, S->omegas.as<double>(), d.omegas, S->mirror.as<int>(),
+                           d.dt, tp.T, &need};
+                S->fcr = fcr_prepare(ctx, pl, FcrLists{&S->jobtri, &S->jobmask, &S->clsstart});
+                const std::size_t nf3 = static_cast<std::size_t>(d.nf) * d.nf * d.nf;
+                if (static_cast<std::size_t>(S->npos) > (~std::size_t{0} / 48) / nf3)
+                    throw Error(TACO_ERR_OVERFLOW, "FCR buffer too large");
+                ctx->slices.ensure(3 * nf3 * static_cast<std::size_t>(S->npos) * 16);
+                ctx->l4.ensure(N4 * static_cast<std::size_t>(S->npos) * 16);
+                const std::size_t lds = l4_lds_bytes(d.N, d.num_ops);
+                if (lds > 160 * 1024)
+                    throw Error(TACO_ERR_INVALID, "fused L4 kernel supports N^4 * 16 B + ops in 160 KiB of LDS (N <= 8)");
+                set_l4_lds(ctx, lds);
+            }
+            S->l4out.ensure(std::max<std::size_t>(d.ntau * N4 * 16, 16));
+            if (d.steps) {
+                S->lgen.ensure((d.steps + 1) * N4 * 16);
+                S->rho.ensure((d.steps + 2) * N2 * 16);
+                Rk4Sys r{};
+                r.Ls = S->lgen.as<double2>();
+                r.r0 = S->r0.as<double2>();
+                r.samples = S->rho.as<double2>();
+                r.r = r.samples + (d.steps + 1) * N2;
+                r.steps = static_cast<long>(d.steps);
+                r.dt = d.dt;
+                r.D = static_cast<int>(N2);
+                rk.push_back(r);
+                B.rk_lds = std::max(B.rk_lds, 6 * N2 * 16);
+            }
+            B.sys.push_back(std::move(S));
+        }
+        B.nrk = static_cast<long>(rk.size());
+        if (B.nrk) {
+            if (B.rk_lds > 160 * 1024) throw Error(TACO_ERR_INVALID, "RK4 kernel supports D <= 1706");
+            if (B.rk_lds > 64 * 1024)
+                HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_rk4_batch),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(B.rk_lds)));
+            B.rkargs.ensure(rk.size() * sizeof(Rk4Sys));
+            HIP_OK(hipMemcpy(B.rkargs.p, rk.data(), rk.size() * sizeof(Rk4Sys), hipMemcpyHostToDevice));
+        }
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        *out = b.release();
+    });
+}
+
+int taco_batch_run(taco_batch* b, int use_graph) {
+    return guarded([&] {
+        if (!b) throw Error(TACO_ERR_INVALID, "null taco_batch");
+        taco_ctx* c = b->ctx;
+        BatchRun& B = b->run;
+        DeviceGuard g(c->device);
+        if (!use_graph || !B.ran_eager) {
+            // The first run is eager: it picks the FFT tile configurations
+            // (autotune synchronises) that a captured graph then keeps.
+            batch_enqueue(c, B);
+            HIP_OK(hipStreamSynchronize(c->stream));
+            B.ran_eager = true;
+            if (!use_graph) return;
+        }
+        if (!B.exec) {
+            const bool prof = c->prof.on;
+            c->prof.on = false;
+            HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            try {
+                batch_enqueue(c, B);
+            } catch (...) {
+                hipGraph_t junk = nullptr;
+                (void)hipStreamEndCapture(c->stream, &junk);
+                if (junk) (void)hipGraphDestroy(junk);
+                c->prof.on = prof;
+                throw;
+            }
+            c->prof.on = prof;
+            HIP_OK(hipStreamEndCapture(c->stream, &B.graph));
+            HIP_OK(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+        }
+        HIP_OK(hipGraphLaunch(B.exec, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int taco_batch_l4(taco_batch* b, int s, double* L4_out) {
+    return guarded([&] {
+        if (!b) throw Error(TACO_ERR_INVALID, "null taco_batch");
+        if (s < 0 || s >= static_cast<int>(b->run.sys.size())) throw Error(TACO_ERR_RANGE, "system index out of range");
+        const BatchSys& S = *b->run.sys[static_cast<std::size_t>(s)];
+        const std::size_t N2 = static_cast<std::size_t>(S.N) * S.N;
+        DeviceGuard g(b->ctx->device);
+        if (S.ntau) HIP_OK(hipMemcpy(L4_out, S.l4out.p, S.ntau * N2 * N2 * 16, hipMemcpyDeviceToHost));
+    });
+}
+
+int taco_batch_rho(taco_batch* b, int s, double* samples_out, double* r_out) {
+    return guarded([&] {
+        if (!b) throw Error(TACO_ERR_INVALID, "null taco_batch");
+        if (s < 0 || s >= static_cast<int>(b->run.sys.size())) throw Error(TACO_ERR_RANGE, "system index out of range");
+        const BatchSys& S = *b->run.sys[static_cast<std::size_t>(s)];
+        if (!S.steps) throw Error(TACO_ERR_INVALID, "system has no RK4 steps");
+        const std::size_t N2 = static_cast<std::size_t>(S.N) * S.N;
+        DeviceGuard g(b->ctx->device);
+        if (samples_out) HIP_OK(hipMemcpy(samples_out, S.rho.p, (S.steps + 1) * N2 * 16, hipMemcpyDeviceToHost));
+        if (r_out)
+            HIP_OK(hipMemcpy(r_out, S.rho.as<double2>() + (S.steps + 1) * N2, N2 * 16, hipMemcpyDeviceToHost));
+    });
+}
+
+int taco_batch_destroy(taco_batch* b) {
+    return guarded([&] {
+        if (!b) return;
+        DeviceGuard g(b->ctx->device);
+        (void)hipStreamSynchronize(b->ctx->stream);
+        delete b;
+    });
+}
+
+int taco_batched_run(taco_ctx* ctx, const taco_batch_system* systems, int nsys, double* const* L4_out,
+                     double* const* rho_out) {
+    taco_batch* b = nullptr;
+    int rc = taco_batch_create(ctx, systems, nsys, &b);
+    if (rc) return rc;
+    rc = taco_batch_run(b, 0);
+    for (int s = 0; rc == TACO_OK && s < nsys; ++s) {
+        if (L4_out && L4_out[s]) rc = taco_batch_l4(b, s, L4_out[s]);
+        if (rc == TACO_OK && rho_out && rho_out[s] && systems[s].steps) rc = taco_batch_rho(b, s, rho_out[s], nullptr);
+    }
+    const std::string msg = rc ? std::string(taco_last_error()) : std::string();
+    const int rc2 = taco_batch_destroy(b);
+    if (rc) set_last_error(msg.c_str());  // keep the failure's message past the clean-up
+    return rc ? rc : rc2;
 }
 
 }  // extern "C"

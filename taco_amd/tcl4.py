@@ -39,7 +39,7 @@ __all__ = [
     "build_TCL4_generator_cuda_fused", "build_TCL4_generator_cuda_fused_batch", "build_correction_series_cuda_fused",
     "build_TCL4_generator", "build_correction_series", "assemble_l4_from_slices", "propagate_rk4_dense",
     "propagate_rk4_dense_serial", "propagate_rk4_dense_omp", "tcl4_propagate", "set_fcr_fft_pad_factor",
-    "get_fcr_fft_pad_factor", "context", "version",
+    "get_fcr_fft_pad_factor", "context", "version", "BatchItem", "Batch", "batched_run",
 ]
 
 
@@ -433,6 +433,99 @@ def tcl4_propagate(system: System, gamma_series, dt: float, steps: int, r0, orde
                                     dptr(of) if of.size else None, len(system.A_eig), dt, steps, order, dptr(r),
                                     dptr(samples)))
     return samples
+
+
+# --- batched runs: many independent systems in one device pass ------------------
+
+@dataclass
+class BatchItem:
+    """One system of a batch: its System, Gamma table, dt, L4 output times
+    and RK4 run (``steps`` steps of the ``order`` generator from ``r0``)."""
+    system: System
+    gamma: np.ndarray
+    dt: float
+    tidx: Sequence[int] = ()
+    steps: int = 0
+    r0: Optional[np.ndarray] = None
+    order: int = 4
+
+
+class Batch:
+    """A prepared batch (taco_batch_* of the C ABI).  ``run()`` computes every
+    system; ``run(graph=True)`` replays a captured hipGraph after the first
+    (eager) run.  Per system the values equal the plain builders'
+    (build_TCL4_generator_hip_fused_batch, tcl4_propagate) bit for bit."""
+
+    def __init__(self, items: Sequence[BatchItem], exec: Optional[Exec] = None):
+        from ._lib import BatchSystem
+
+        self.ctx = _ctx_for(exec)
+        self.items = list(items)
+        self._keep = []
+        descs = (BatchSystem * max(len(self.items), 1))()
+        for q, it in enumerate(self.items):
+            S = it.system
+            g = _gamma_checked(S, it.gamma)
+            om, mi, pf, of = _system_arrays(S)
+            gc = _colmajor(g).ravel()
+            tid = np.ascontiguousarray(np.asarray(list(it.tidx), dtype=np.int64), np.uint64)
+            eps = np.ascontiguousarray(S.eps, np.float64)
+            r0 = (np.ascontiguousarray(np.asarray(it.r0, np.complex128).ravel()) if it.r0 is not None
+                  else np.zeros(S.N * S.N, np.complex128))
+            if it.steps and r0.size != S.N * S.N:
+                raise TacoInvalidArgument(1, "r0 must have N^2 entries")
+            self._keep += [gc, om, mi, pf, of, tid, eps, r0]
+            d = descs[q]
+            d.gamma, d.Nt, d.nf = dptr(gc), g.shape[0], g.shape[1]
+            d.omegas, d.mirror, d.pair_to_freq, d.N = dptr(om), iptr(mi), iptr(pf), S.N
+            d.ops, d.num_ops, d.dt = (dptr(of) if of.size else None), len(S.A_eig), float(it.dt)
+            d.tidx, d.ntau = (sptr(tid) if tid.size else None), tid.size
+            d.steps, d.order, d.eps, d.r0 = int(it.steps), int(it.order), dptr(eps), dptr(r0)
+        h = c_void_p()
+        check(lib().taco_batch_create(self.ctx.handle, descs, len(self.items), ctypes.byref(h)))
+        self.handle = h
+        self._keep = []  # the engine copied every input
+
+    def run(self, graph: bool = False) -> "Batch":
+        check(lib().taco_batch_run(self.handle, 1 if graph else 0))
+        return self
+
+    def l4(self, s: int) -> np.ndarray:
+        """System s's L4 at its tidx: [ntau, N^2, N^2]."""
+        it = self.items[s]
+        n, N2 = len(it.tidx), it.system.N ** 2
+        out = np.zeros(n * N2 * N2, np.complex128)
+        check(lib().taco_batch_l4(self.handle, s, dptr(out)))
+        return out.reshape(n, N2, N2).transpose(0, 2, 1)
+
+    def rho(self, s: int) -> np.ndarray:
+        """System s's states at steps 0..steps: [steps+1, N^2] (eigenbasis)."""
+        it = self.items[s]
+        out = np.zeros((it.steps + 1, it.system.N ** 2), np.complex128)
+        check(lib().taco_batch_rho(self.handle, s, dptr(out), None))
+        return out
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            check(lib().taco_batch_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batched_run(items: Sequence[BatchItem], exec: Optional[Exec] = None):
+    """One-shot batch: returns ([L4 per system], [rho per system or None])."""
+    b = Batch(items, exec)
+    try:
+        b.run()
+        return ([b.l4(s) for s in range(len(items))],
+                [b.rho(s) if it.steps else None for s, it in enumerate(items)])
+    finally:
+        b.close()
 
 
 # --- device-resident inputs (torch tensors as HBM buffers) -----------------------

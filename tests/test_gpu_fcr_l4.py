@@ -156,6 +156,30 @@ def test_l4_from_slices_fixture(orc):
         assert np.abs(L4 - orc.gw_to_l4(GWr, 2)).max() <= 1e-12 * np.abs(GWr).max()
 
 
+@pytest.mark.parametrize("case", ["gue_pair", "d8"])
+def test_l4_pruned_triples_match_full_slices(orc, case, monkeypatch):
+    """The L4 builders compute only the triples the GW sums read (F, C and R
+    separately).  With the slice buffer NaN-filled first, an entry read but
+    never written would show up; the result equals L4 assembled from the full
+    nf^3 slices of compute_triple_kernels bit for bit."""
+    if case == "gue_pair":
+        w = W.gue_pair(N_bath=3000, tmax=300)
+    else:
+        rng = W.SplitMix64(77)
+        w = W.Workload("d8", W.gue(8, rng), [W.random_hermitian_unit(8, rng)], taco_amd.DrudeLorentz(0.1, 5.0),
+                       1.0, 0.01, 199, [0, 3, 199])
+    S, G = w.system, w.gamma
+    tidx = list(w.tidx)[:6]
+    monkeypatch.setenv("TACO_DEBUG_POISON", "1")
+    got = taco_amd.build_TCL4_generator_hip_fused_batch(S, G, w.dt, tidx)
+    assert np.isfinite(got).all()
+    full = taco_amd.compute_triple_kernels(S, G, w.dt, keep=max(tidx) + 1)
+    for q, t in enumerate(tidx):
+        L4, _ = taco_amd.assemble_l4_from_slices(full.F[..., t], full.C[..., t], full.R[..., t], S.pair_to_freq,
+                                                 S.A_eig)
+        assert np.array_equal(got[q], L4), (case, t)
+
+
 def test_l4_error_paths():
     w = W.spin_boson(Nt=64)
     S, G = w.system, w.gamma
