@@ -37,7 +37,8 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--workload", default="gue_pair", choices=["gue_pair", "published", "spin_boson", "batched"])
+    p.add_argument("--workload", default="gue_pair",
+                   choices=["gue_pair", "published", "spin_boson", "batched", "adjoint"])
     p.add_argument("--systems", type=int, default=256, help="batched workload: number of systems")
     p.add_argument("--no-plain", action="store_true", help="batched workload: skip the plain per-system leg")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / checker leg")
@@ -140,7 +141,9 @@ def batched_main(a):
     t0 = time.perf_counter()
     ws = [W.heterogeneous_system(b) for b in range(nsys)]
     items = []
-    for w in ws:
+    for q, w in enumerate(ws):
+        if q % 32 == 0:
+            print(f"[bench] host prep (Gamma tables) system {q}", file=sys.stderr, flush=True)
         r0 = np.zeros(w.system.N ** 2, complex)
         r0[0] = 1.0
         items.append(taco_amd.BatchItem(w.system, w.gamma, w.dt, w.tidx, steps=w.steps, r0=r0, order=4))
@@ -175,6 +178,8 @@ def batched_main(a):
         t0 = time.perf_counter()
         mism = 0
         for s, it in enumerate(items):
+            if s % 32 == 0:
+                print(f"[bench] plain leg system {s}", file=sys.stderr, flush=True)
             L4 = taco_amd.build_TCL4_generator_hip_fused_batch(it.system, it.gamma, it.dt, list(it.tidx))
             rho = taco_amd.tcl4_propagate(it.system, it.gamma, it.dt, it.steps, it.r0, order=4)
             mism += int(not (np.array_equal(L4, b.l4(s)) and np.array_equal(rho, b.rho(s))))
@@ -214,6 +219,114 @@ def batched_main(a):
     print(json.dumps(rec), flush=True)
 
 
+def adjoint_main(a):
+    """Gradient of the spin-pair loss (d = 4, H1 = eps/2 sz + Delta/2 sx,
+    A = sz(x)I + I(x)sz, Ohmic bath alpha 0.05, omega_c 5, beta 1, bath grid
+    2^16, dt 0.005, 4000 RK4 steps of TCL2 + L4, checkpoints every 64 steps,
+    loss sum_t ||rho_lab(t) - rho_target(t)||_F^2 with the target from
+    parameters moved by 5% (seed 4)) with respect to (alpha, omega_c, beta,
+    eps, Delta).  One step = one forward + reverse pass; the value is
+    gradients per second.  The CPU leg times central finite differences of
+    the oracle pipeline (10 forwards) and checks the gradient against them."""
+    import torch
+
+    from taco_amd import adjoint as AD
+
+    assert torch.cuda.is_available(), "bench needs an MI355X"
+    prob = AD.SpinPairProblem()
+    p0 = np.array([0.05, 5.0, 1.0, 1.0, 1.0])
+    u = np.where(np.random.default_rng(4).uniform(size=5) < 0.5, -1.0, 1.0)
+    with torch.no_grad():
+        target = prob.rho_lab(torch.tensor(p0 * (1 + 0.05 * u), dtype=torch.float64)).detach()
+    print("[bench] adjoint: target built", file=sys.stderr, flush=True)
+
+    def fwd():
+        with torch.no_grad():
+            return prob.loss(torch.tensor(p0, dtype=torch.float64), target)
+
+    def grad():
+        params = torch.tensor(p0, dtype=torch.float64, requires_grad=True)
+        loss = prob.loss(params, target)
+        (g,) = torch.autograd.grad(loss, params)
+        return float(loss), g.numpy()
+
+    for _ in range(a.warmup):
+        fwd()
+        grad()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        fwd()
+    t_fwd = (time.perf_counter() - t0) / a.steps
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        loss, g = grad()
+    t_grad = (time.perf_counter() - t0) / a.steps
+    print(f"[bench] forward {t_fwd * 1e3:.1f} ms, forward+reverse {t_grad * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    # reverse-pass kernels of one gradient
+    from taco_amd import tcl4
+    with tcl4.Profile(0) as prof:
+        grad()
+        kern = prof.read()
+    kernels, roofline = kernel_table(kern, 1)
+    cpu, check = None, None
+    if not a.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle import oracle
+        from test_gpu_adjoint import oracle_spin_pair_rho_lab
+
+        threads = a.cpu_threads or len(os.sched_getaffinity(0))
+        oracle.set_phase_mode(oracle.PHASE_CR)
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+        tn = target.numpy()
+
+        def oloss(p):
+            d = oracle_spin_pair_rho_lab(oracle, prob, p) - tn
+            return float(np.sum(d.real ** 2 + d.imag ** 2))
+
+        t0 = time.perf_counter()
+        fdg = np.zeros(5)
+        for q in range(5):
+            h = 1e-6 * abs(p0[q])
+            pp, pm = p0.copy(), p0.copy()
+            pp[q] += h
+            pm[q] -= h
+            fdg[q] = (oloss(pp) - oloss(pm)) / (2 * h)
+            print(f"[bench] oracle FD {prob.names[q]} done", file=sys.stderr, flush=True)
+        cpu_s = time.perf_counter() - t0
+        rel = np.abs(g - fdg) / np.abs(fdg)
+        cpu = {"value": 1.0 / cpu_s, "unit": "gradients/s", "cores": threads, "kind": "port",
+               "sample": f"central finite differences of the oracle pipeline (10 forwards, h = 1e-6 relative), "
+                         f"{cpu_s:.1f} s"}
+        check = {"params": list(prob.names), "adjoint": g.tolist(), "oracle_fd": fdg.tolist(),
+                 "max_rel_diff": float(rel.max()), "loss": loss}
+    rec = {
+        "metric": METRIC,
+        "value": 1.0 / t_grad,
+        "unit": "gradients/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * t_grad,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128",
+        "data": "synthetic",
+        "config": {
+            "workload": "adjoint: d=4 spin pair, Ohmic (0.05, 5, 1), Nt=2^16, dt=0.005, 4000 RK4 steps of TCL2+L4, "
+                        "checkpoints every 64, 5 parameters (alpha, omega_c, beta, eps, Delta)",
+            "path": "tape forward + hand-written reverse",
+        },
+        "forward_ms": 1e3 * t_fwd,
+        "gradient_over_forward": t_grad / t_fwd,
+        "check": check,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+    }
+    print(json.dumps(rec), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -222,6 +335,8 @@ def main():
         sys.exit(2)
     if a.workload == "batched":
         return batched_main(a)
+    if a.workload == "adjoint":
+        return adjoint_main(a)
     import torch
 
     import taco_amd

@@ -220,6 +220,33 @@ int taco_batch_destroy(taco_batch* b);
 int taco_batched_run(taco_ctx* ctx, const taco_batch_system* systems, int nsys, double* const* L4_out,
                      double* const* rho_out);
 
+/* ---- Derivatives (no reference counterpart: the reference has none) ------
+ * Reverse mode through RK4 -> L(t) = L_TCL2(t) + L4(t) -> fused L4 -> F/C/R,
+ * for the composition of taco_tcl4_propagate.  taco_tape_forward runs it,
+ * writes the states r_0..r_steps ((steps+1) x N^2, nullable) and keeps on the
+ * device what the reverse pass reads: the F/C/R slices of every t <= steps,
+ * the generators L(t) and the RK4 states every checkpoint_every steps (the
+ * states in between are recomputed from those checkpoints).
+ * taco_tape_adjoint takes samples_bar = dLoss/dr_n ((steps+1) x N^2) and
+ * returns the cotangents of the inputs: gamma_bar [Nt x nf] (rows past steps
+ * are 0), omega_bar [nf] (through the phases exp(-+i Omega t) of F/C/R),
+ * ops_bar [num_ops x N^2], eps_bar [N], r0_bar [N^2]; any may be NULL.
+ * Complex cotangents follow dLoss = Re sum conj(z_bar) dz (z_bar =
+ * dLoss/dRe z + i dLoss/dIm z).  A tape can be reversed any number of times. */
+typedef struct taco_tape taco_tape;
+int taco_tape_forward(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, const double* omegas,
+                      const int* mirror, const int* pair_to_freq, int N, const double* eps, const double* ops,
+                      int num_ops, double dt, size_t steps, int order, const double* r0, size_t checkpoint_every,
+                      double* samples_out, taco_tape** out);
+int taco_tape_adjoint(taco_tape* tape, const double* samples_bar, double* gamma_bar, double* omega_bar,
+                      double* ops_bar, double* eps_bar, double* r0_bar);
+int taco_tape_destroy(taco_tape* tape);
+/* One-shot taco_tape_forward + taco_tape_adjoint + taco_tape_destroy. */
+int taco_adjoint(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, const double* omegas, const int* mirror,
+                 const int* pair_to_freq, int N, const double* eps, const double* ops, int num_ops, double dt,
+                 size_t steps, int order, const double* r0, size_t checkpoint_every, const double* samples_bar,
+                 double* gamma_bar, double* omega_bar, double* ops_bar, double* eps_bar, double* r0_bar);
+
 #ifdef __cplusplus
 }
 #endif

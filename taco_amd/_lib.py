@@ -68,6 +68,12 @@ SIGNATURES = {
     "taco_batch_rho": (c_int, [c_void_p, c_int, _dp, _dp]),
     "taco_batch_destroy": (c_int, [c_void_p]),
     "taco_batched_run": (c_int, [c_void_p, POINTER(BatchSystem), c_int, POINTER(_dp), POINTER(_dp)]),
+    "taco_tape_forward": (c_int, [c_void_p, _dp, c_size_t, c_int, _dp, _ip, _ip, c_int, _dp, _dp, c_int, c_double,
+                                  c_size_t, c_int, _dp, c_size_t, _dp, POINTER(c_void_p)]),
+    "taco_tape_adjoint": (c_int, [c_void_p, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "taco_tape_destroy": (c_int, [c_void_p]),
+    "taco_adjoint": (c_int, [c_void_p, _dp, c_size_t, c_int, _dp, _ip, _ip, c_int, _dp, _dp, c_int, c_double,
+                             c_size_t, c_int, _dp, c_size_t, _dp, _dp, _dp, _dp, _dp, _dp]),
 }
 
 _lib = None

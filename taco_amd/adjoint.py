@@ -1,0 +1,347 @@
+"""Gradients of the TCL4 pipeline (no reference counterpart: jcarrots/taco has
+no derivatives).
+
+* ``Tape`` wraps the engine's forward tape and hand-written reverse pass
+  (taco_tape_* of the C ABI): RK4 -> L(t) = L_TCL2(t) + L4(t) -> fused L4 ->
+  F/C/R, with RK4 checkpoints every K steps.
+* ``TCL4Propagate`` is the torch.autograd.Function over it: inputs (Gamma
+  table, bucket frequencies, A_eig, eigenvalues, r0), output the states
+  r_0..r_steps of ``tcl4_propagate`` (the drivers' composition,
+  TCL4_spin_boson_example.cpp:284-384).
+* ``BcfFun``, ``GammaFun``, ``EighFun`` and ``BucketOmegas`` carry the
+  parameter chain in front of it: bath spectral density -> C(t)
+  (correlation_fft.hpp:154-214), trapz Gamma (gamma.hpp:166-213), the
+  eigensystem of H (system.hpp:37-157).  Their forwards are the engine's host
+  routines (bit-identical to the oracle's); their backwards are written out
+  below.
+* ``SpinPairProblem`` is the differentiation workload: H = H1 (x) I + I (x) H1,
+  H1 = eps/2 sz + Delta/2 sx, A = sz (x) I + I (x) sz, Ohmic bath
+  (alpha, omega_c, beta), loss sum_t ||rho_lab(t) - rho_target(t)||_F^2.
+
+Complex cotangents follow torch's convention for real losses:
+z.grad = dLoss/dRe z + i dLoss/dIm z.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import c_void_p
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+from ._lib import TacoInvalidArgument, check, dptr, iptr, lib
+from .tcl4 import Exec, Ohmic, DrudeLorentz, System, _ctx_for, bcf_fft_fun, compute_trapz_prefix_multi_matrix
+
+__all__ = ["Tape", "TCL4Propagate", "BcfFun", "GammaFun", "EighFun", "BucketOmegas", "SpinPairProblem"]
+
+
+def _cm(a) -> np.ndarray:
+    """Row-major numpy matrix -> column-major flat (Eigen's layout)."""
+    return np.ascontiguousarray(np.asarray(a, np.complex128).T).ravel()
+
+
+class Tape:
+    """One forward run kept for the reverse pass (taco_tape_forward)."""
+
+    def __init__(self, gamma, omegas, mirror, pair_to_freq, ops, eps, dt: float, steps: int, r0,
+                 order: int = 4, checkpoint_every: int = 64, exec: Optional[Exec] = None):
+        self.ctx = _ctx_for(exec)
+        g = np.asarray(gamma, np.complex128)
+        self.Nt, self.nf = g.shape
+        p2f = np.asarray(pair_to_freq, np.int32)
+        self.N = p2f.shape[0]
+        self.num_ops = len(ops)
+        self.steps = int(steps)
+        gc = np.ascontiguousarray(g.T).ravel()
+        om = np.ascontiguousarray(omegas, np.float64)
+        mi = np.ascontiguousarray(mirror, np.int32)
+        pf = np.ascontiguousarray(p2f.T).ravel()
+        of = np.concatenate([_cm(o) for o in ops])
+        ev = np.ascontiguousarray(eps, np.float64)
+        r = np.ascontiguousarray(np.asarray(r0, np.complex128).ravel())
+        if r.size != self.N ** 2:
+            raise TacoInvalidArgument(1, "r0 must have N^2 entries")
+        self.samples = np.zeros((self.steps + 1, self.N ** 2), np.complex128)
+        h = c_void_p()
+        check(lib().taco_tape_forward(self.ctx.handle, dptr(gc), self.Nt, self.nf, dptr(om), iptr(mi), iptr(pf),
+                                      self.N, dptr(ev), dptr(of), self.num_ops, float(dt), self.steps, int(order),
+                                      dptr(r), int(checkpoint_every), dptr(self.samples), ctypes.byref(h)))
+        self.handle = h
+
+    def adjoint(self, samples_bar) -> dict:
+        sb = np.ascontiguousarray(np.asarray(samples_bar, np.complex128).reshape(self.steps + 1, self.N ** 2))
+        gb = np.zeros(self.Nt * self.nf, np.complex128)
+        ob = np.zeros(self.nf)
+        ab = np.zeros(self.num_ops * self.N ** 2, np.complex128)
+        eb = np.zeros(self.N)
+        rb = np.zeros(self.N ** 2, np.complex128)
+        check(lib().taco_tape_adjoint(self.handle, dptr(sb), dptr(gb), dptr(ob), dptr(ab), dptr(eb), dptr(rb)))
+        N = self.N
+        return {
+            "gamma": gb.reshape(self.nf, self.Nt).T.copy(),
+            "omegas": ob,
+            "ops": [ab[k * N * N:(k + 1) * N * N].reshape(N, N).T.copy() for k in range(self.num_ops)],
+            "eps": eb,
+            "r0": rb,
+        }
+
+    def close(self):
+        if getattr(self, "handle", None):
+            check(lib().taco_tape_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class TCL4Propagate(torch.autograd.Function):
+    """r_0..r_steps of the TCL4 run as a differentiable function of
+    (gamma [Nt, nf], omegas [nf], ops [num_ops, N, N], eps [N], r0 [N^2]).
+    ``meta`` = (mirror, pair_to_freq, dt, steps, order, checkpoint_every,
+    exec); the discrete bucket structure carries no gradient."""
+
+    @staticmethod
+    def forward(ctx, gamma, omegas, ops, eps, r0, meta):
+        mirror, p2f, dt, steps, order, K, ex = meta
+        tape = Tape(gamma.detach().numpy(), omegas.detach().numpy(), mirror, p2f,
+                    list(ops.detach().numpy()), eps.detach().numpy(), dt, steps, r0.detach().numpy(), order, K, ex)
+        ctx.tape = tape
+        return torch.from_numpy(tape.samples.copy())
+
+    @staticmethod
+    def backward(ctx, gs):
+        res = ctx.tape.adjoint(gs.detach().numpy())
+        return (torch.from_numpy(res["gamma"]), torch.from_numpy(res["omegas"]),
+                torch.from_numpy(np.stack(res["ops"])), torch.from_numpy(res["eps"]), torch.from_numpy(res["r0"]),
+                None)
+
+
+def _spectral(kind: int, p0: float, p1: float, w: np.ndarray):
+    """J(w) and dJ/dp0, dJ/dp1 for w > 0 (spectral_J of the bath models)."""
+    w = np.asarray(w, float)
+    pos = w > 0
+    J = np.zeros_like(w)
+    d0 = np.zeros_like(w)
+    d1 = np.zeros_like(w)
+    if kind == 0:  # Ohmic p0 w exp(-w/p1)
+        e = np.exp(-w[pos] / p1)
+        J[pos] = p0 * w[pos] * e
+        d0[pos] = w[pos] * e
+        d1[pos] = J[pos] * w[pos] / (p1 * p1)
+    else:  # Drude-Lorentz 2 p0 p1 w / (w^2 + p1^2)
+        den = w[pos] ** 2 + p1 * p1
+        J[pos] = 2.0 * p0 * p1 * w[pos] / den
+        d0[pos] = 2.0 * p1 * w[pos] / den
+        d1[pos] = 2.0 * p0 * w[pos] * (w[pos] ** 2 - p1 * p1) / (den * den)
+    return J, d0, d1
+
+
+class BcfFun(torch.autograd.Function):
+    """C(t_n), n < Nt = N + 1, of bcf_fft_fun (correlation_fft.hpp:154-214) as
+    a function of (p0, p1, beta) for a finite beta.  C = (dw/pi) FFT(S) with
+    S the sampled KMS spectrum, linear in S: the backward is
+    Sbar_k = (dw/pi) Re sum_n Cbar_n e^{+2 pi i k n / Nfft} and
+    p_bar = sum_k Sbar_k dS_k/dp."""
+
+    @staticmethod
+    def forward(ctx, p0, p1, beta, meta):
+        kind, N, dt = meta
+        bath = Ohmic(float(p0), float(p1)) if kind == 0 else DrudeLorentz(float(p0), float(p1))
+        _, C = bcf_fft_fun(N, dt, bath, float(beta))
+        ctx.save = (kind, N, dt, float(p0), float(p1), float(beta))
+        return torch.from_numpy(C)
+
+    @staticmethod
+    def backward(ctx, Cbar):
+        kind, N, dt, p0, p1, beta = ctx.save
+        if not np.isfinite(beta):
+            raise TacoInvalidArgument(1, "BcfFun backward needs a finite beta")
+        Nt = N + 1
+        Nfft = 2 * Nt
+        if Nfft & (Nfft - 1):
+            Nfft = 1 << int(Nfft - 1).bit_length()
+        dw = 2.0 * np.pi / (dt * Nfft)
+        wny = np.pi / dt
+        cb = np.zeros(Nfft, np.complex128)
+        cb[:Nt] = Cbar.detach().numpy()
+        Sbar = (dw / np.pi) * (Nfft * np.fft.ifft(cb)).real
+        k = np.arange(1, Nfft // 2)
+        w = dw * k
+        J, dJ0, dJ1 = _spectral(kind, p0, p1, w)
+        kms = np.exp(-beta * w)
+        f = (np.pi / 2.0) / (1.0 - kms)
+        Sp = f * J
+        # S_k = Sp, S_{Nfft-k} = Sp * kms
+        dSp_dJ = f
+        dSp_db = (np.pi / 2.0) * J * (-w * kms) / (1.0 - kms) ** 2
+        g0 = Sbar[k] * dSp_dJ * dJ0 + Sbar[Nfft - k] * dSp_dJ * kms * dJ0
+        g1 = Sbar[k] * dSp_dJ * dJ1 + Sbar[Nfft - k] * dSp_dJ * kms * dJ1
+        gb = Sbar[k] * dSp_db + Sbar[Nfft - k] * (dSp_db * kms + Sp * (-w * kms))
+        # S_0 = 0.5 (pi/2) J(dw) (1 + r) / (1 - r), r = exp(-beta dw)
+        r = np.exp(-beta * dw)
+        J0, a0, a1 = (x[0] for x in _spectral(kind, p0, p1, np.array([dw])))
+        q0 = 0.5 * (np.pi / 2.0) * (1.0 + r) / (1.0 - r)
+        s0b = Sbar[0]
+        g0 = g0.sum() + s0b * q0 * a0
+        g1 = g1.sum() + s0b * q0 * a1
+        gb = gb.sum() + s0b * 0.5 * (np.pi / 2.0) * J0 * (-2.0 * dw * r / (1.0 - r) ** 2)
+        # S_Ny = 0.25 pi J(wny) (1 + 2 / (e^{beta wny} - 1))
+        Jn, n0, n1 = (x[0] for x in _spectral(kind, p0, p1, np.array([wny])))
+        xN = np.exp(-beta * wny)  # 2 / (e^{beta wny} - 1) = 2 xN / (1 - xN)
+        qn = 0.25 * np.pi * (1.0 + 2.0 * xN / (1.0 - xN))
+        sNb = Sbar[Nfft // 2]
+        g0 += sNb * qn * n0
+        g1 += sNb * qn * n1
+        gb += sNb * 0.25 * np.pi * Jn * (-2.0 * wny * xN / (1.0 - xN) ** 2)
+        t = lambda v: torch.tensor(float(v), dtype=torch.float64)
+        return t(g0), t(g1), t(gb), None
+
+
+class GammaFun(torch.autograd.Function):
+    """Gamma_b(t_k) = (dt/2) sum_{m=1..k} (phi^{m-1} C_{m-1} + phi^m C_m),
+    phi = e^{i w_b dt} (compute_trapz_prefix_multi_matrix, gamma.hpp:166-213),
+    as a function of C [Nt] and w [nf]."""
+
+    @staticmethod
+    def forward(ctx, C, omegas, dt):
+        Cn = C.detach().numpy()
+        om = omegas.detach().numpy()
+        G = compute_trapz_prefix_multi_matrix(Cn, dt, om)
+        ctx.save = (Cn, om, dt)
+        return torch.from_numpy(G)
+
+    @staticmethod
+    def backward(ctx, Gbar):
+        Cn, om, dt = ctx.save
+        Gb = Gbar.detach().numpy()  # [Nt, nf]
+        Nt = Cn.size
+        m = np.arange(Nt)
+        # Q_b(m): weight-summed cotangents; Q(0) = sum_{k>=1} Gb, Q(m) = Gb(m) + 2 sum_{k>m} Gb
+        suf = np.cumsum(Gb[::-1], axis=0)[::-1]  # sum_{k>=m}
+        Q = np.empty_like(Gb)
+        Q[0] = suf[1] if Nt > 1 else 0.0
+        if Nt > 1:
+            after = np.vstack([suf[2:], np.zeros((1, Gb.shape[1]))]) if Nt > 2 else np.zeros((1, Gb.shape[1]))
+            Q[1:] = Gb[1:] + 2.0 * after
+        phi = np.exp(1j * np.outer(m * dt, om))  # [Nt, nf]
+        Cbar = (dt / 2.0) * np.sum(np.conj(phi) * Q, axis=1)
+        dphi = (1j * (m * dt))[:, None] * phi * Cn[:, None]
+        wbar = (dt / 2.0) * np.real(np.sum(np.conj(Q) * dphi, axis=0))
+        return torch.from_numpy(Cbar), torch.from_numpy(wbar), None
+
+
+class EighFun(torch.autograd.Function):
+    """(eps, U) of a Hermitian H with the engine's eigensolver (the System
+    build, system.hpp:37-51), ascending eigenvalues.  Backward: with
+    G = U^H U_bar, K_ij = G_ij / (e_j - e_i) for e_i != e_j, K_ii = eps_bar_i,
+    0 inside degenerate blocks (the loss must not depend on the basis chosen
+    inside a degenerate eigenspace, which holds for lab-frame losses), and
+    H_bar = herm(U K U^H)."""
+
+    @staticmethod
+    def forward(ctx, H, A_lab, tol):
+        S = System.build(H.detach().numpy(), [np.asarray(a) for a in A_lab], tol)
+        ctx.save = (S.eps.copy(), S.U.copy(), tol)
+        ctx.system = S
+        return torch.from_numpy(S.eps.copy()), torch.from_numpy(S.U.copy())
+
+    @staticmethod
+    def backward(ctx, eb, Ub):
+        e, U, tol = ctx.save
+        N = e.size
+        G = U.conj().T @ (Ub.detach().numpy() if Ub is not None else np.zeros((N, N), complex))
+        E = e[None, :] - e[:, None]
+        K = np.zeros((N, N), complex)
+        off = np.abs(E) > max(tol, 1e-12) * max(1.0, np.abs(e).max())
+        K[off] = G[off] / E[off]
+        K[np.diag_indices(N)] = eb.detach().numpy() if eb is not None else 0.0
+        Hb = U @ K @ U.conj().T
+        Hb = 0.5 * (Hb + Hb.conj().T)
+        return torch.from_numpy(Hb), None, None
+
+
+class BucketOmegas(torch.autograd.Function):
+    """Bucket frequencies w_b (the System's values) as a function of the
+    eigenvalues: w_b = e_m - e_n for a representative pair (m, n) with
+    pair_to_freq[m, n] = b (every pair of a bucket moves with it here)."""
+
+    @staticmethod
+    def forward(ctx, e, omegas, p2f):
+        N = p2f.shape[0]
+        rep = {}
+        for n in range(N):
+            for m in range(N):
+                rep.setdefault(int(p2f[m, n]), (m, n))
+        ctx.rep = [rep[b] for b in range(len(omegas))]
+        ctx.N = N
+        return torch.from_numpy(np.asarray(omegas, np.float64).copy())
+
+    @staticmethod
+    def backward(ctx, wb):
+        eb = np.zeros(ctx.N)
+        for b, (m, n) in enumerate(ctx.rep):
+            eb[m] += float(wb[b])
+            eb[n] -= float(wb[b])
+        return torch.from_numpy(eb), None, None
+
+
+SZ = np.array([[1, 0], [0, -1]], complex)
+SX = np.array([[0, 1], [1, 0]], complex)
+I2 = np.eye(2, dtype=complex)
+
+
+@dataclass
+class SpinPairProblem:
+    """Gradient workload: d = 4 spin pair, Ohmic bath, RK4 of TCL2 + L4 from
+    rho0 = |0><0| in the eigenbasis, loss sum_t ||rho_lab(t) - target(t)||_F^2
+    over t = 0..steps (lab basis, which does not depend on the eigenbasis
+    chosen)."""
+    Nt: int = 2 ** 16
+    dt: float = 0.005
+    steps: int = 4000
+    checkpoint_every: int = 64
+    order: int = 4
+    freq_tol: float = 1e-9
+
+    names = ("alpha", "omega_c", "beta", "eps", "delta")
+
+    def hamiltonian(self, eps, delta):
+        sz = torch.from_numpy(SZ)
+        sx = torch.from_numpy(SX)
+        i2 = torch.from_numpy(I2)
+        H1 = 0.5 * eps * sz + 0.5 * delta * sx
+        return torch.kron(H1, i2) + torch.kron(i2, H1)
+
+    @staticmethod
+    def coupling():
+        return np.kron(SZ, I2) + np.kron(I2, SZ)
+
+    def rho_lab(self, params: torch.Tensor, exec: Optional[Exec] = None) -> torch.Tensor:
+        """rho_lab(t) [steps+1, N, N] as a differentiable function of
+        params = (alpha, omega_c, beta, eps, delta)."""
+        alpha, wc, beta, eps, delta = params[0], params[1], params[2], params[3], params[4]
+        H = self.hamiltonian(eps.to(torch.complex128), delta.to(torch.complex128))
+        A = self.coupling()
+        e, U = EighFun.apply(H, [A], self.freq_tol)
+        S = System.build(H.detach().numpy(), [A], self.freq_tol)
+        om = BucketOmegas.apply(e, S.omegas, S.pair_to_freq)
+        C = BcfFun.apply(alpha, wc, beta, (0, self.Nt - 1, self.dt))
+        G = GammaFun.apply(C, om, self.dt)
+        At = torch.from_numpy(A)
+        Aeig = (U.conj().T @ At @ U).unsqueeze(0)
+        N = S.N
+        r0 = torch.zeros(N * N, dtype=torch.complex128)
+        r0[0] = 1.0
+        meta = (S.mirror, S.pair_to_freq, self.dt, self.steps, self.order, self.checkpoint_every, exec)
+        r = TCL4Propagate.apply(G, om, Aeig, e, r0, meta)
+        rho = r.reshape(-1, N, N).transpose(1, 2)  # vec column-major: rho[n, m] = r[n + m N]
+        return U @ rho @ U.conj().T
+
+    def loss(self, params: torch.Tensor, target: torch.Tensor, exec: Optional[Exec] = None) -> torch.Tensor:
+        d = self.rho_lab(params, exec) - target
+        return (d.real ** 2 + d.imag ** 2).sum()

@@ -1,0 +1,535 @@
+// Reverse-mode kernels of the TCL4 pipeline: RK4 -> fused L4 -> F/C/R.
+//
+// Included once by engine.hip after the forward kernels (it uses their device
+// helpers).  The reference has no derivative code; these are the hand-written
+// adjoints of the forward passes above.  Convention for every complex
+// cotangent z_bar: dLoss = Re sum conj(z_bar) dz, i.e. z_bar = dLoss/dRe z +
+// i dLoss/dIm z (torch's convention for real losses).
+#pragma once
+
+// y = L^H x (column r of L conjugated), summed over c in order.
+__device__ void matvec_h(const double2* __restrict__ L, const double2* x, double2* y, int D) {
+    for (int r = threadIdx.x; r < D; r += blockDim.x) {
+        double2 s = make_double2(0.0, 0.0);
+        const double2* col = L + static_cast<long>(r) * D;
+        for (int c = 0; c < D; ++c) s = cadd(s, cmul(cconj(col[c]), x[c]));
+        y[r] = s;
+    }
+}
+
+// y = (0.5 (L0 + L1))^H x.
+__device__ void matvec_h_half(const double2* __restrict__ L0, const double2* __restrict__ L1, const double2* x,
+                              double2* y, int D) {
+    for (int r = threadIdx.x; r < D; r += blockDim.x) {
+        double2 s = make_double2(0.0, 0.0);
+        const long o = static_cast<long>(r) * D;
+        for (int c = 0; c < D; ++c) s = cadd(s, cmul(cconj(cscale(cadd(L0[o + c], L1[o + c]), 0.5)), x[c]));
+        y[r] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// RK4 adjoint.  Forward step n (rk4_step): k1 = L0 r, u1 = r + h k1,
+// k2 = Lh u1, u2 = r + h k2, k3 = Lh u2, u3 = r + dt k3, k4 = L1 u3,
+// r' = r + s (k1 + 2 k2 + 2 k3 + k4), h = dt/2, s = dt/6, Lh = (L0 + L1)/2.
+// Backwards from lam = dLoss/dr_steps: each step maps lam_{n+1} to
+// lam_n = M_n^H lam_{n+1} + gbar_n and adds the cotangents of L_n and L_{n+1}
+// into Lbar.  The states r_n are recomputed segment by segment (K steps)
+// from checkpoints ck[q] = r_{qK}, with the forward step's own arithmetic.
+// One workgroup; LDS: 12 vectors of D.
+// ---------------------------------------------------------------------------
+struct Rk4AdjArgs {
+    const double2* Ls;    // [steps + 1][D * D] generators
+    const double2* ck;    // [ceil(steps / K) + 1][D] checkpoint states r_{qK}
+    const double2* gbar;  // [steps + 1][D] dLoss/dr_n
+    double2* Lbar;        // [steps + 1][D * D], zero on entry
+    double2* seg;         // [K + 1][D] recomputed states of one segment
+    double2* r0bar;       // [D]
+    long steps;
+    int D;
+    int K;
+    double dt;
+};
+
+__global__ __launch_bounds__(kBlock) void k_rk4_adjoint(Rk4AdjArgs A) {
+    extern __shared__ double2 sh[];
+    const int D = A.D;
+    double2* x = sh;          // state being recomputed / r_n
+    double2* k1 = x + D;
+    double2* k2 = k1 + D;
+    double2* k3 = k2 + D;
+    double2* k4 = k3 + D;
+    double2* tmp = k4 + D;
+    double2* lam = tmp + D;   // cotangent of r_{n+1}, then of r_n
+    double2* u1 = lam + D;
+    double2* u2 = u1 + D;
+    double2* u3 = u2 + D;
+    double2* kb1 = u3 + D;    // cotangents of k1..k4 (kb4 = s lam is formed in place)
+    double2* kb2 = kb1 + D;
+    const long D2 = static_cast<long>(D) * D;
+    const double h = 0.5 * A.dt, s6 = A.dt / 6.0;
+    for (int e = threadIdx.x; e < D; e += blockDim.x) lam[e] = A.gbar[A.steps * D + e];
+    __syncthreads();
+    const long nseg = (A.steps + A.K - 1) / A.K;
+    for (long sg = nseg - 1; sg >= 0; --sg) {
+        const long s0 = sg * A.K, s1 = min(s0 + A.K, A.steps);
+        // recompute r_{s0} .. r_{s1 - 1} from the checkpoint
+        for (int e = threadIdx.x; e < D; e += blockDim.x) x[e] = A.ck[sg * D + e];
+        __syncthreads();
+        for (long n = s0; n < s1; ++n) {
+            for (int e = threadIdx.x; e < D; e += blockDim.x) A.seg[(n - s0) * D + e] = x[e];
+            __syncthreads();
+            if (n + 1 < s1) rk4_step(A.Ls + n * D2, A.Ls + (n + 1) * D2, nullptr, x, k1, k2, k3, k4, tmp, D, A.dt);
+        }
+        for (long n = s1 - 1; n >= s0; --n) {
+            const double2* L0 = A.Ls + n * D2;
+            const double2* L1 = A.Ls + (n + 1) * D2;
+            for (int e = threadIdx.x; e < D; e += blockDim.x) x[e] = A.seg[(n - s0) * D + e];
+            __syncthreads();
+            // stage vectors of the forward step
+            matvec_rows(L0, x, k1, D);
+            __syncthreads();
+            for (int e = threadIdx.x; e < D; e += blockDim.x) u1[e] = cadd(x[e], cscale(k1[e], h));
+            __syncthreads();
+            matvec_half(L0, L1, u1, k2, D);
+            __syncthreads();
+            for (int e = threadIdx.x; e < D; e += blockDim.x) u2[e] = cadd(x[e], cscale(k2[e], h));
+            __syncthreads();
+            matvec_half(L0, L1, u2, k3, D);
+            __syncthreads();
+            for (int e = threadIdx.x; e < D; e += blockDim.x) u3[e] = cadd(x[e], cscale(k3[e], A.dt));
+            __syncthreads();
+            // reverse: r' = r + s (k1 + 2 k2 + 2 k3 + k4)
+            for (int e = threadIdx.x; e < D; e += blockDim.x) k4[e] = cscale(lam[e], s6);      // kb4
+            __syncthreads();
+            matvec_h(L1, k4, tmp, D);                                                          // ub3
+            __syncthreads();
+            for (int e = threadIdx.x; e < D; e += blockDim.x) {
+                k3[e] = cadd(cscale(lam[e], 2.0 * s6), cscale(tmp[e], A.dt));                  // kb3
+                k1[e] = cadd(lam[e], tmp[e]);                                                   // rb so far
+            }
+            __syncthreads();
+            matvec_h_half(L0, L1, k3, tmp, D);                                                 // ub2
+            __syncthreads();
+            for (int e = threadIdx.x; e < D; e += blockDim.x) {
+                kb2[e] = cadd(cscale(lam[e], 2.0 * s6), cscale(tmp[e], h));
+                k1[e] = cadd(k1[e], tmp[e]);
+            }
+            __syncthreads();
+            matvec_h_half(L0, L1, kb2, tmp, D);                                                // ub1
+            __syncthreads();
+            for (int e = threadIdx.x; e < D; e += blockDim.x) {
+                kb1[e] = cadd(cscale(lam[e], s6), cscale(tmp[e], h));
+                k1[e] = cadd(k1[e], tmp[e]);
+            }
+            __syncthreads();
+            matvec_h(L0, kb1, tmp, D);
+            __syncthreads();
+            // generator cotangents: Lbar_h = kb3 u2^H + kb2 u1^H, split half/half
+            double2* Lb0 = A.Lbar + n * D2;
+            double2* Lb1 = A.Lbar + (n + 1) * D2;
+            for (long e = threadIdx.x; e < D2; e += blockDim.x) {
+                const int r = static_cast<int>(e % D), c = static_cast<int>(e / D);
+                const double2 lh = cadd(cmul(k3[r], cconj(u2[c])), cmul(kb2[r], cconj(u1[c])));
+                Lb0[e] = cadd(Lb0[e], cadd(cmul(kb1[r], cconj(x[c])), cscale(lh, 0.5)));
+                Lb1[e] = cadd(Lb1[e], cadd(cmul(k4[r], cconj(u3[c])), cscale(lh, 0.5)));
+            }
+            for (int e = threadIdx.x; e < D; e += blockDim.x) lam[e] = cadd(cadd(k1[e], tmp[e]), A.gbar[n * D + e]);
+            __syncthreads();
+        }
+    }
+    for (int e = threadIdx.x; e < D; e += blockDim.x) A.r0bar[e] = lam[e];
+}
+
+// ---------------------------------------------------------------------------
+// Fused L4 adjoint, one workgroup per time t.  From Lbar(t) (the RK4 layout of
+// L4(n,m;i,j) = GW(n,i;m,j)) it forms Tbar = GWbar + GWbar^H (GW = T + T^H)
+// and, for every NAKZWAN term sigma * e1 e2 e3 e4 * S of T(n,i;m,j), adds
+// sigma Tbar conj(e1 e2 e3 e4) to the slice entries S gathers from and
+// sigma Tbar conj(S * prod of the other three) to each operator entry.  Slice
+// cotangents accumulate in LDS (or global when they do not fit) with atomic
+// adds; operator cotangents in LDS, written per time.
+// ---------------------------------------------------------------------------
+struct L4AdjArgs {
+    const double2* Lbar;  // [T][N^4]
+    const double2* Fs;    // [T][nf3] forward slices
+    const double2* Cs;
+    const double2* Rs;
+    const int* p2f;
+    const double2* ops;
+    int N, nf, num_ops;
+    long nf3;
+    double2* sbar;        // [T][3][nf3]; zero on entry unless lds_slices
+    double2* Abar;        // [T][num_ops][N^2]
+    int lds_slices;
+};
+
+__device__ __forceinline__ void atomic_cadd(double2* p, double2 v) {
+    atomicAdd(&p->x, v.x);
+    atomicAdd(&p->y, v.y);
+}
+
+__global__ __launch_bounds__(kBlock) void k_l4_adjoint(L4AdjArgs A) {
+    extern __shared__ double2 sm[];
+    const int N = A.N, N2 = N * N, N4 = N2 * N2;
+    const long t = blockIdx.x;
+    double2* Tb = sm;                                   // N^4
+    double2* opsS = Tb + N4;                            // num_ops N^2
+    double2* Ab = opsS + A.num_ops * N2;                // num_ops N^2
+    double2* sb = Ab + A.num_ops * N2;                  // 3 nf3 (lds_slices)
+    int* f = reinterpret_cast<int*>(sb + (A.lds_slices ? 3 * A.nf3 : 0));
+    const double2* Lb = A.Lbar + t * N4;
+    for (int idx = threadIdx.x; idx < N4; idx += kBlock) {
+        const int ru = idx % N2, cu = idx / N2;
+        const int n = ru % N, i = ru / N, m = cu % N, j = cu / N;
+        const double2 g1 = Lb[(n + m * N) + (i + j * N) * N2];
+        const double2 g2 = Lb[(m + n * N) + (j + i * N) * N2];
+        Tb[idx] = cadd(g1, cconj(g2));
+    }
+    for (int e = threadIdx.x; e < A.num_ops * N2; e += kBlock) {
+        opsS[e] = A.ops[e];
+        Ab[e] = make_double2(0.0, 0.0);
+    }
+    if (A.lds_slices)
+        for (long e = threadIdx.x; e < 3 * A.nf3; e += kBlock) sb[e] = make_double2(0.0, 0.0);
+    for (int e = threadIdx.x; e < N2; e += kBlock) f[e] = A.p2f[e];
+    __syncthreads();
+    const double2* Fp = A.Fs + t * A.nf3;
+    const double2* Cp = A.Cs + t * A.nf3;
+    const double2* Rp = A.Rs + t * A.nf3;
+    double2* sF = A.lds_slices ? sb : A.sbar + t * 3 * A.nf3;
+    double2* sC = sF + A.nf3;
+    double2* sR = sC + A.nf3;
+    const long nf = A.nf;
+    auto id3 = [nf](int x, int y, int z) { return (static_cast<long>(x) * nf + y) * nf + z; };
+    auto F2 = [f, N](int r, int c) { return f[r + c * N]; };
+    // One term sigma * e1 e2 e3 e4 * S with S = sum of up to two slice
+    // entries (w1 * slice1[x1] + w2 * slice2[x2]); d1..d4 are the operator
+    // cotangent slots of e1..e4.
+    auto term = [&](double sg, double2 Tv, double2 e1, double2 e2, double2 e3, double2 e4, double2* d1, double2* d2,
+                    double2* d3, double2* d4, double2 S, double2* s1, long x1, double w1, double2* s2, long x2,
+                    double w2) {
+        const double2 p12 = cmul(e1, e2), p34 = cmul(e3, e4);
+        const double2 Sb = cscale(cmul(Tv, cconj(cmul(p12, p34))), sg);
+        atomic_cadd(s1 + x1, cscale(Sb, w1));
+        if (s2) atomic_cadd(s2 + x2, cscale(Sb, w2));
+        const double2 TS = cscale(cmul(Tv, cconj(S)), sg);
+        atomic_cadd(d1, cmul(TS, cconj(cmul(e2, p34))));
+        atomic_cadd(d2, cmul(TS, cconj(cmul(e1, p34))));
+        atomic_cadd(d3, cmul(TS, cconj(cmul(p12, e4))));
+        atomic_cadd(d4, cmul(TS, cconj(cmul(p12, e3))));
+    };
+    for (int idx = threadIdx.x; idx < N4; idx += kBlock) {
+        const int row_u = idx % N2, col_u = idx / N2;
+        const int n = row_u % N, i = row_u / N, m = col_u % N, j = col_u / N;
+        const double2 Tv = Tb[idx];
+        if (Tv.x == 0.0 && Tv.y == 0.0) continue;
+        for (int iA = 0; iA < A.num_ops; ++iA) {
+            const double2* Ap = opsS + iA * N2;
+            double2* AbA = Ab + iA * N2;
+            for (int iB = 0; iB < A.num_ops; ++iB) {
+                const double2* Bp = opsS + iB * N2;
+                double2* AbB = Ab + iB * N2;
+#define AA(r, c) Ap[(r) + (c) * N]
+#define BB(r, c) Bp[(r) + (c) * N]
+#define DA(r, c) (AbA + (r) + (c) * N)
+#define DB(r, c) (AbB + (r) + (c) * N)
+                for (int a = 0; a < N; ++a) {
+                    for (int b = 0; b < N; ++b) {
+                        const long xM1 = id3(F2(b, a), F2(b, i), F2(n, b)), xM2 = id3(F2(b, a), F2(n, a), F2(a, i));
+                        const long xI = id3(F2(a, n), F2(b, i), F2(n, b));
+                        const long xK = id3(F2(i, b), F2(n, a), F2(a, i));
+                        const long x1 = id3(F2(a, n), F2(j, b), F2(n, i));
+                        const long x2 = id3(F2(i, a), F2(j, b), F2(n, i));
+                        const long x3 = id3(F2(b, a), F2(j, m), F2(a, i));
+                        const long x4 = id3(F2(i, b), F2(j, m), F2(a, i));
+                        const double2 Mv = csub(Fp[xM1], Rp[xM2]);
+                        const double2 Iv = Fp[xI], Kv = Rp[xK];
+                        const double2 X1 = cadd(Cp[x1], Rp[x1]), X2 = cadd(Cp[x2], Rp[x2]);
+                        const double2 X3 = cadd(Cp[x3], Rp[x3]), X4 = cadd(Cp[x4], Rp[x4]);
+                        // T = -(t1 - t2 + t3 + t4 - t5 - t6 + t7)
+                        term(-1.0, Tv, BB(n, a), AA(a, b), BB(b, i), AA(j, m), DB(n, a), DA(a, b), DB(b, i), DA(j, m),
+                             Mv, sF, xM1, 1.0, sR, xM2, -1.0);
+                        term(+1.0, Tv, AA(n, a), BB(a, b), BB(b, i), AA(j, m), DA(n, a), DB(a, b), DB(b, i), DA(j, m),
+                             Iv, sF, xI, 1.0, nullptr, 0, 0.0);
+                        term(-1.0, Tv, BB(n, a), BB(a, b), AA(b, i), AA(j, m), DB(n, a), DB(a, b), DA(b, i), DA(j, m),
+                             Kv, sR, xK, 1.0, nullptr, 0, 0.0);
+                        term(-1.0, Tv, AA(n, a), BB(a, i), BB(j, b), AA(b, m), DA(n, a), DB(a, i), DB(j, b), DA(b, m),
+                             X1, sC, x1, 1.0, sR, x1, 1.0);
+                        term(+1.0, Tv, BB(n, a), AA(a, i), BB(j, b), AA(b, m), DB(n, a), DA(a, i), DB(j, b), DA(b, m),
+                             X2, sC, x2, 1.0, sR, x2, 1.0);
+                        term(+1.0, Tv, AA(n, a), AA(a, b), BB(b, i), BB(j, m), DA(n, a), DA(a, b), DB(b, i), DB(j, m),
+                             X3, sC, x3, 1.0, sR, x3, 1.0);
+                        term(-1.0, Tv, AA(n, a), BB(a, b), AA(b, i), BB(j, m), DA(n, a), DB(a, b), DA(b, i), DB(j, m),
+                             X4, sC, x4, 1.0, sR, x4, 1.0);
+                    }
+                }
+                if (j == m) {
+                    for (int a = 0; a < N; ++a)
+                        for (int b = 0; b < N; ++b)
+                            for (int c = 0; c < N; ++c) {
+                                const long yM1 = id3(F2(c, b), F2(c, i), F2(a, c));
+                                const long yM2 = id3(F2(c, b), F2(a, b), F2(b, i));
+                                const long yK = id3(F2(i, c), F2(a, b), F2(b, i));
+                                const long yI = id3(F2(b, a), F2(c, i), F2(a, c));
+                                const double2 Mv = csub(Fp[yM1], Rp[yM2]);
+                                const double2 Kv = Rp[yK], Iv = Fp[yI];
+                                term(+1.0, Tv, AA(n, a), BB(a, b), AA(b, c), BB(c, i), DA(n, a), DB(a, b), DA(b, c),
+                                     DB(c, i), Mv, sF, yM1, 1.0, sR, yM2, -1.0);
+                                term(+1.0, Tv, AA(n, a), BB(a, b), BB(b, c), AA(c, i), DA(n, a), DB(a, b), DB(b, c),
+                                     DA(c, i), Kv, sR, yK, 1.0, nullptr, 0, 0.0);
+                                term(-1.0, Tv, AA(n, a), AA(a, b), BB(b, c), BB(c, i), DA(n, a), DA(a, b), DB(b, c),
+                                     DB(c, i), Iv, sF, yI, 1.0, nullptr, 0, 0.0);
+                            }
+                }
+#undef AA
+#undef BB
+#undef DA
+#undef DB
+            }
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < A.num_ops * N2; e += kBlock) A.Abar[t * A.num_ops * N2 + e] = Ab[e];
+    if (A.lds_slices)
+        for (long e = threadIdx.x; e < 3 * A.nf3; e += kBlock) A.sbar[t * 3 * A.nf3 + e] = sb[e];
+}
+
+// ---------------------------------------------------------------------------
+// F/C/R adjoint per triple (t < T), the reverse of k_combine and the
+// convolutions of phases A-C.  With g1 = Gamma_i, g2 = Gamma_j,
+// gm = Gamma_mirror(j), pp = exp(+i Omega t dt), pm = conj(pp), w = g1 pm:
+//   F = w PF - convF,  PF = dt cumsum(gm pp),       convF = dt (w * gm)
+//   C = w PC - convC,  PC = dt cumsum(conj(g2) pp), convC = dt (w * conj g2)
+//   R = g1 P3 - P4,    P3 = dt cumsum(g2 pm),       P4 = dt cumsum(g1 g2 pm)
+// (tcl4_kernels.cpp:105-175).  Only t < T enters, so the convolution
+// adjoints are direct correlations over [0, T).  Per position p of the
+// combine list the scratch holds 11 series of length T:
+//   0 w, 1 Fbar, 2 Cbar, 3 wbar, 4 g1bar, 5 g2bar, 6 gmbar, 7..10 P-bars.
+// ---------------------------------------------------------------------------
+struct FcrAdjArgs {
+    const double2* gamma;   // [nf][Nt]
+    const double* omegas;
+    const int* mirror;
+    int nf;
+    long Nt;
+    double dt;
+    long T;
+    const int* job_tri;     // combine-list position -> triple
+    const unsigned char* job_mask;
+    long ncomb;
+    const double2* sbar;    // [T][3][nf3]
+    long nf3;
+    double2* scr;           // [ncomb][11][T]
+    double* om;             // [ncomb] Omega cotangent
+};
+
+__device__ __forceinline__ double block_sum(double v, double* ws) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double tot = 0.0;
+    if (threadIdx.x == 0) tot = (ws[0] + ws[1]) + (ws[2] + ws[3]);
+    return tot;  // valid in thread 0
+}
+
+// Forward prefix scans and reverse suffix scans of one triple.
+__global__ __launch_bounds__(kBlock) void k_fcr_adj_scan(FcrAdjArgs P) {
+    __shared__ double wsum[8][4];
+    __shared__ double wred[4];
+    const long p = blockIdx.x;
+    const long tri = P.job_tri[p];
+    const int mask = P.job_mask[p];
+    int i, j, k;
+    decode_triple(tri, P.nf, i, j, k);
+    const double omega = (P.omegas[i] + P.omegas[j]) + P.omegas[k];
+    const double2* g1 = P.gamma + static_cast<long>(i) * P.Nt;
+    const double2* g2 = P.gamma + static_cast<long>(j) * P.Nt;
+    const double2* gm = P.gamma + static_cast<long>(P.mirror[j]) * P.Nt;
+    const long T = P.T;
+    double2* S = P.scr + p * 11 * T;
+    double2 *W = S, *FB = S + T, *CB = S + 2 * T, *WB = S + 3 * T, *G1 = S + 4 * T, *G2 = S + 5 * T,
+            *GM = S + 6 * T, *PbF = S + 7 * T, *PbC = S + 8 * T, *Pb3 = S + 9 * T, *Pb4 = S + 10 * T;
+    const double dt = P.dt;
+    double carry[6] = {0, 0, 0, 0, 0, 0};
+    for (long t0 = 0; t0 < T; t0 += kBlock) {
+        const long t = t0 + threadIdx.x;
+        const bool valid = t < T;
+        double2 a1 = make_double2(0, 0), a2 = a1, a3 = a1;
+        double2 pp = a1, pm = a1, x1 = a1, x2 = a1;
+        if (valid) {
+            pp = cexpi(omega * (static_cast<double>(t) * dt));
+            pm = cconj(pp);
+            x1 = g1[t];
+            x2 = g2[t];
+            a1 = cmul(gm[t], pp);
+            a2 = cmul(cconj(x2), pp);
+            a3 = cmul(x2, pm);
+        }
+        double v[6] = {a1.x, a1.y, a2.x, a2.y, a3.x, a3.y};
+        double tot[6];
+        for (int q = 0; q < 6; ++q) {
+            v[q] = block_scan(v[q], wsum[q], tot[q]) + carry[q];
+            carry[q] += tot[q];
+        }
+        if (!valid) continue;
+        const double2 PF = make_double2(v[0] * dt, v[1] * dt);
+        const double2 PC = make_double2(v[2] * dt, v[3] * dt);
+        const double2 P3 = make_double2(v[4] * dt, v[5] * dt);
+        const double2* sb = P.sbar + t * 3 * P.nf3;
+        const double2 z = make_double2(0.0, 0.0);
+        const double2 Fb = (mask & 1) ? sb[tri] : z;
+        const double2 Cb = (mask & 2) ? sb[P.nf3 + tri] : z;
+        const double2 Rb = (mask & 4) ? sb[2 * P.nf3 + tri] : z;
+        const double2 w = cmul(x1, pm);
+        W[t] = w;
+        FB[t] = Fb;
+        CB[t] = Cb;
+        WB[t] = cadd(cmul(Fb, cconj(PF)), cmul(Cb, cconj(PC)));
+        G1[t] = cmul(Rb, cconj(P3));
+        PbF[t] = cmul(Fb, cconj(w));
+        PbC[t] = cmul(Cb, cconj(w));
+        Pb3[t] = cmul(Rb, cconj(x1));
+        Pb4[t] = make_double2(-Rb.x, -Rb.y);
+    }
+    __syncthreads();
+    // suffix sums ybar_m = dt sum_{t >= m} Pbar_t, last chunk first
+    double sc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double om_part = 0.0;
+    const long nch = (T + kBlock - 1) / kBlock;
+    for (long ch = nch - 1; ch >= 0; --ch) {
+        const long t = ch * kBlock + (kBlock - 1 - threadIdx.x);
+        const bool valid = t < T;
+        double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (valid) {
+            v[0] = PbF[t].x; v[1] = PbF[t].y; v[2] = PbC[t].x; v[3] = PbC[t].y;
+            v[4] = Pb3[t].x; v[5] = Pb3[t].y; v[6] = Pb4[t].x; v[7] = Pb4[t].y;
+        }
+        double tot[8];
+        for (int q = 0; q < 8; ++q) {
+            v[q] = block_scan(v[q], wsum[q], tot[q]) + sc[q];
+            sc[q] += tot[q];
+        }
+        if (!valid) continue;
+        const double2 yF = make_double2(v[0] * dt, v[1] * dt), yC = make_double2(v[2] * dt, v[3] * dt);
+        const double2 y3 = make_double2(v[4] * dt, v[5] * dt), y4 = make_double2(v[6] * dt, v[7] * dt);
+        const double tdt = static_cast<double>(t) * dt;
+        const double2 pp = cexpi(omega * tdt), pm = cconj(pp);
+        const double2 x1 = g1[t], x2 = g2[t], xm = gm[t];
+        GM[t] = cmul(yF, cconj(pp));
+        G2[t] = cadd(cadd(cmul(cconj(yC), pp), cmul(y3, pp)), cmul(y4, cconj(cmul(x1, pm))));
+        G1[t] = cadd(G1[t], cmul(y4, cconj(cmul(x2, pm))));
+        const double2 ppb = cadd(cmul(yF, cconj(xm)), cmul(yC, x2));
+        const double2 pmb = cadd(cmul(y3, cconj(x2)), cmul(y4, cconj(cmul(x1, x2))));
+        // d pp / d Omega = i t dt pp, d pm / d Omega = -i t dt pm
+        const double2 dpp = make_double2(-tdt * pp.y, tdt * pp.x);
+        const double2 dpm = make_double2(tdt * pm.y, -tdt * pm.x);
+        om_part += (ppb.x * dpp.x + ppb.y * dpp.y) + (pmb.x * dpm.x + pmb.y * dpm.y);
+    }
+    const double tot = block_sum(om_part, wred);
+    if (threadIdx.x == 0) P.om[p] = tot;
+}
+
+// Correlation adjoints of the two convolutions for one block of 256 lags u:
+//   wbar_u  += dt sum_{t>=u} (-Fbar_t conj(gm_{t-u}) - Cbar_t g2_{t-u})
+//   gmbar_u += dt sum_{t>=u} -Fbar_t conj(w_{t-u})
+//   g2bar_u += conj(dt sum_{t>=u} -Cbar_t conj(w_{t-u}))
+__global__ __launch_bounds__(kBlock) void k_fcr_adj_corr(FcrAdjArgs P) {
+    __shared__ double2 fb[kBlock], cb[kBlock];
+    __shared__ double2 wgm[2 * kBlock], wg2[2 * kBlock], ww[2 * kBlock];
+    const long p = blockIdx.x;
+    const int mask = P.job_mask[p];
+    if (!(mask & 3)) return;
+    const long tri = P.job_tri[p];
+    int i, j, k;
+    decode_triple(tri, P.nf, i, j, k);
+    const double2* g2 = P.gamma + static_cast<long>(j) * P.Nt;
+    const double2* gm = P.gamma + static_cast<long>(P.mirror[j]) * P.Nt;
+    const long T = P.T;
+    double2* S = P.scr + p * 11 * T;
+    const double2 *W = S, *FB = S + T, *CB = S + 2 * T;
+    double2 *WB = S + 3 * T, *G2 = S + 5 * T, *GM = S + 6 * T;
+    const long ub0 = static_cast<long>(blockIdx.y) * kBlock;
+    const long u = ub0 + threadIdx.x;
+    double2 xw = make_double2(0, 0), xm = xw, xc = xw;
+    for (long t0 = ub0; t0 < T; t0 += kBlock) {
+        // window of lags d = t - u for t in [t0, t0 + 256), u in [ub0, ub0 + 256):
+        // d in [t0 - ub0 - 255, t0 - ub0 + 255], stored at d - (t0 - ub0 - 255)
+        const long dlo = t0 - ub0 - (kBlock - 1);
+        for (int e = threadIdx.x; e < 2 * kBlock; e += kBlock) {
+            const long d = dlo + e;
+            const bool ok = d >= 0 && d < T;
+            wgm[e] = ok ? gm[d] : make_double2(0, 0);
+            wg2[e] = ok ? g2[d] : make_double2(0, 0);
+            ww[e] = ok ? W[d] : make_double2(0, 0);
+        }
+        {
+            const long t = t0 + threadIdx.x;
+            fb[threadIdx.x] = t < T ? FB[t] : make_double2(0, 0);
+            cb[threadIdx.x] = t < T ? CB[t] : make_double2(0, 0);
+        }
+        __syncthreads();
+        if (u < T) {
+            const int qlo = static_cast<int>(max(0L, u - t0));
+            const int qhi = static_cast<int>(min(static_cast<long>(kBlock), T - t0));
+            for (int q = qlo; q < qhi; ++q) {
+                const int e = static_cast<int>(t0 + q - u - dlo);
+                const double2 f = fb[q], c = cb[q];
+                xw = csub(xw, cadd(cmul(f, cconj(wgm[e])), cmul(c, wg2[e])));
+                xm = csub(xm, cmul(f, cconj(ww[e])));
+                xc = csub(xc, cmul(c, cconj(ww[e])));
+            }
+        }
+        __syncthreads();
+    }
+    if (u < T) {
+        const double dt = P.dt;
+        WB[u] = cadd(WB[u], cscale(xw, dt));
+        GM[u] = cadd(GM[u], cscale(xm, dt));
+        G2[u] = cadd(G2[u], cconj(cscale(xc, dt)));
+    }
+}
+
+// g1bar += wbar pp; Omega cotangent of w = g1 pm.
+__global__ __launch_bounds__(kBlock) void k_fcr_adj_fin(FcrAdjArgs P) {
+    __shared__ double wred[4];
+    const long p = blockIdx.x;
+    const long tri = P.job_tri[p];
+    int i, j, k;
+    decode_triple(tri, P.nf, i, j, k);
+    const double omega = (P.omegas[i] + P.omegas[j]) + P.omegas[k];
+    const double2* g1 = P.gamma + static_cast<long>(i) * P.Nt;
+    const long T = P.T;
+    double2* S = P.scr + p * 11 * T;
+    const double2* WB = S + 3 * T;
+    double2* G1 = S + 4 * T;
+    double om_part = 0.0;
+    for (long t = threadIdx.x; t < T; t += kBlock) {
+        const double tdt = static_cast<double>(t) * P.dt;
+        const double2 pp = cexpi(omega * tdt), pm = cconj(pp);
+        const double2 wb = WB[t];
+        G1[t] = cadd(G1[t], cmul(wb, pp));
+        const double2 pmb = cmul(wb, cconj(g1[t]));
+        const double2 dpm = make_double2(tdt * pm.y, -tdt * pm.x);
+        om_part += pmb.x * dpm.x + pmb.y * dpm.y;
+    }
+    const double tot = block_sum(om_part, wred);
+    if (threadIdx.x == 0) P.om[p] += tot;
+}
+
+// Gamma cotangent per bucket: gbar[b][t] = sum over the bucket's entries
+// (position, role) of series role (4 g1bar, 5 g2bar, 6 gmbar), in list order.
+__global__ __launch_bounds__(kBlock) void k_bucket_reduce(const double2* scr, long T, const int* ent_start,
+                                                          const int* ent, long Nt, double2* gbar) {
+    const int b = blockIdx.y;
+    const long t = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (t >= T) return;
+    double2 s = make_double2(0.0, 0.0);
+    for (int q = ent_start[b]; q < ent_start[b + 1]; ++q) {
+        const int code = ent[q];
+        const long p = code / 3, role = 4 + code % 3;
+        s = cadd(s, scr[(p * 11 + role) * T + t]);
+    }
+    gbar[static_cast<long>(b) * Nt + t] = s;
+}

@@ -659,6 +659,33 @@ __device__ void matvec_half(const double2* __restrict__ L0, const double2* __res
     }
 }
 
+// One RK4 step x <- x + dt/6 (k1 + 2 k2 + 2 k3 + k4) by the whole workgroup
+// (rk4_dense.cpp:133-160); Lh null: midpoint 0.5 (L0 + L1) per element.
+__device__ void rk4_step(const double2* L0, const double2* L1, const double2* Lh, double2* x, double2* k1,
+                         double2* k2, double2* k3, double2* k4, double2* tmp, int D, double dt) {
+    const double hdt = 0.5 * dt;
+    const double s6 = dt / 6.0;
+    matvec_rows(L0, x, k1, D);
+    __syncthreads();
+    for (int e = threadIdx.x; e < D; e += blockDim.x) tmp[e] = cadd(x[e], cscale(k1[e], hdt));
+    __syncthreads();
+    if (Lh) matvec_rows(Lh, tmp, k2, D); else matvec_half(L0, L1, tmp, k2, D);
+    __syncthreads();
+    for (int e = threadIdx.x; e < D; e += blockDim.x) tmp[e] = cadd(x[e], cscale(k2[e], hdt));
+    __syncthreads();
+    if (Lh) matvec_rows(Lh, tmp, k3, D); else matvec_half(L0, L1, tmp, k3, D);
+    __syncthreads();
+    for (int e = threadIdx.x; e < D; e += blockDim.x) tmp[e] = cadd(x[e], cscale(k3[e], dt));
+    __syncthreads();
+    matvec_rows(L1, tmp, k4, D);
+    __syncthreads();
+    for (int e = threadIdx.x; e < D; e += blockDim.x) {
+        const double2 acc = cadd(cadd(cadd(k1[e], cscale(k2[e], 2.0)), cscale(k3[e], 2.0)), k4[e]);
+        x[e] = cadd(x[e], cscale(acc, s6));
+    }
+    __syncthreads();
+}
+
 // One system's RK4 run by the whole workgroup; x, k1..k4, tmp in LDS.  Reads
 // the initial vector from r_in and writes the final one to r_out.
 __device__ void rk4_body(double2* sh, const double2* Ls, const double2* Lh, long steps, int D, const double2* r_in,
@@ -673,34 +700,12 @@ __device__ void rk4_body(double2* sh, const double2* Ls, const double2* Lh, long
     __syncthreads();
     const long D2 = static_cast<long>(D) * D;
     long ns = 0;
-    const double hdt = 0.5 * dt;
-    const double s6 = dt / 6.0;
     for (long st = 0; st < steps; ++st) {
         if (samples && st % every == 0) {
             for (int e = threadIdx.x; e < D; e += blockDim.x) samples[ns * D + e] = x[e];
             ++ns;
         }
-        const double2* L0 = Ls + st * D2;
-        const double2* L1 = Ls + (st + 1) * D2;
-        matvec_rows(L0, x, k1, D);
-        __syncthreads();
-        for (int e = threadIdx.x; e < D; e += blockDim.x) tmp[e] = cadd(x[e], cscale(k1[e], hdt));
-        __syncthreads();
-        if (Lh) matvec_rows(Lh + st * D2, tmp, k2, D); else matvec_half(L0, L1, tmp, k2, D);
-        __syncthreads();
-        for (int e = threadIdx.x; e < D; e += blockDim.x) tmp[e] = cadd(x[e], cscale(k2[e], hdt));
-        __syncthreads();
-        if (Lh) matvec_rows(Lh + st * D2, tmp, k3, D); else matvec_half(L0, L1, tmp, k3, D);
-        __syncthreads();
-        for (int e = threadIdx.x; e < D; e += blockDim.x) tmp[e] = cadd(x[e], cscale(k3[e], dt));
-        __syncthreads();
-        matvec_rows(L1, tmp, k4, D);
-        __syncthreads();
-        for (int e = threadIdx.x; e < D; e += blockDim.x) {
-            const double2 acc = cadd(cadd(cadd(k1[e], cscale(k2[e], 2.0)), cscale(k3[e], 2.0)), k4[e]);
-            x[e] = cadd(x[e], cscale(acc, s6));
-        }
-        __syncthreads();
+        rk4_step(Ls + st * D2, Ls + (st + 1) * D2, Lh ? Lh + st * D2 : nullptr, x, k1, k2, k3, k4, tmp, D, dt);
     }
     if (samples) {
         for (int e = threadIdx.x; e < D; e += blockDim.x) samples[ns * D + e] = x[e];
@@ -856,6 +861,9 @@ struct Twiddles {
     DevBuf std_, b_, c_;
     int L = -1;
 };
+
+// Reverse-mode kernels (device code, same namespace as the forward kernels).
+#include "adjoint_kernels.hpp"
 
 }  // namespace taco_engine
 
@@ -1155,6 +1163,8 @@ struct FcrJob {
     std::vector<int> cls_start;  // host copy of the Omega-class starts
     std::vector<long> paths;     // [ncomb + 1] prefix count of the F and C paths (traffic accounting)
     std::vector<long> outs;      // [ncomb + 1] prefix count of the F, C and R outputs
+    std::vector<int> order;      // host copy of the combine list (position -> triple)
+    std::vector<unsigned char> mask;
     long nfft_jobs = 0, ncomb = 0, chunk = 1, cconv = 1, spec_jobs = 0;
     std::size_t n = 0, Nt = 0, T = 0;
     int L = 0;
@@ -1258,6 +1268,8 @@ static FcrJob fcr_prepare(taco_ctx* c, const FcrPlan& pl, const FcrLists& lists)
         J.paths[p + 1] = J.paths[p] + fc;
         J.outs[p + 1] = J.outs[p] + fc + ((mask[p] >> 2) & 1);
     }
+    J.order = order;
+    J.mask = mask;
     J.nfft_jobs = nfft_jobs;
     J.ncomb = ncomb;
     J.chunk = chunk;
@@ -1597,6 +1609,290 @@ static void batch_enqueue(taco_ctx* c, BatchRun& B) {
 
 }  // namespace taco_engine
 
+
+// ---------------------------------------------------------------------------
+// Forward tape and reverse pass (no reference counterpart: the reference has
+// no derivatives).  The tape keeps on the device what the reverse pass reads:
+// the F/C/R slices at every t <= steps, the generators L(t), and the RK4
+// states every K steps (checkpoints; the states between are recomputed).
+// ---------------------------------------------------------------------------
+struct taco_tape {
+    taco_ctx* ctx = nullptr;
+    std::size_t Nt = 0, T = 0, steps = 0;
+    int nf = 0, N = 0, num_ops = 0, order = 4, K = 64;
+    double dt = 0.0;
+    std::vector<std::complex<double>> h_gamma;  // [nf][T] rows t < T
+    std::vector<double> h_eps, h_omegas;
+    std::vector<int> h_p2f, h_mirror;
+    std::vector<std::complex<double>> h_ops;
+    taco_engine::DevBuf gamma, omegas, mirror, p2f, ops, eps, t2pos, jobtri, jobmask, clsstart, slices, lgen, ck;
+    taco_engine::FcrJob fcr;
+};
+
+namespace taco_engine {
+
+static void rk4_launch(taco_ctx* ctx, const double2* dLs, const double2* dLh, std::size_t steps, int D, double2* dr,
+                       double dt, std::size_t every, double2* dsamples, long* dns) {
+    const std::size_t lds = 6 * static_cast<std::size_t>(D) * 16;
+    if (lds > 160 * 1024) throw Error(TACO_ERR_INVALID, "RK4 kernel supports D <= 1706");
+    hipLaunchKernelGGL(k_rk4, dim3(1), dim3(kBlock), lds, ctx->stream, dLs, dLh, static_cast<long>(steps), D, dr, dt,
+                       static_cast<long>(every), dsamples, dns);
+    HIP_OK(hipGetLastError());
+}
+
+static std::size_t l4_adj_lds(int N, int num_ops, std::size_t nf3, bool lds_slices) {
+    const std::size_t N2 = static_cast<std::size_t>(N) * N;
+    return (N2 * N2 + 2 * num_ops * N2 + (lds_slices ? 3 * nf3 : 0)) * 16 + N2 * 4;
+}
+
+static void set_fn_lds(const void* fn, std::size_t lds) {
+    if (lds > 64 * 1024)
+        HIP_OK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+}
+
+static void tape_forward(taco_tape& tp, const double* r0, double* samples_out) {
+    taco_ctx* c = tp.ctx;
+    const std::size_t N2 = static_cast<std::size_t>(tp.N) * tp.N, D = N2, N4 = N2 * N2;
+    const std::size_t T = tp.T;
+    const std::size_t nf3 = static_cast<std::size_t>(tp.nf) * tp.nf * tp.nf;
+    tp.lgen.ensure(T * N4 * 16);
+    if (tp.order == 4) {
+        std::vector<int> t2pos(T);
+        std::iota(t2pos.begin(), t2pos.end(), 0);
+        tp.t2pos.ensure(T * sizeof(int));
+        HIP_OK(hipMemcpy(tp.t2pos.p, t2pos.data(), T * sizeof(int), hipMemcpyHostToDevice));
+        const std::vector<unsigned char> need = needed_triples(tp.h_p2f.data(), tp.N, tp.nf);
+        FcrPlan pl{tp.gamma.as<double2>(), tp.Nt, tp.nf, tp.omegas.as<double>(), tp.h_omegas.data(),
+                   tp.mirror.as<int>(), tp.dt, T, &need};
+        tp.fcr = fcr_prepare(c, pl, FcrLists{&tp.jobtri, &tp.jobmask, &tp.clsstart});
+        if (T > (~std::size_t{0} / 48) / nf3) throw Error(TACO_ERR_OVERFLOW, "FCR buffer too large");
+        tp.slices.ensure(3 * nf3 * T * 16);
+        CombineArgs out{};
+        out.out_mode = 0;
+        out.t2pos = tp.t2pos.as<int>();
+        out.npos = static_cast<long>(T);
+        out.nf3 = static_cast<long>(nf3);
+        double2* sl = tp.slices.as<double2>();
+        out.Fo = sl;
+        out.Co = sl + nf3 * T;
+        out.Ro = sl + 2 * nf3 * T;
+        fcr_launch(c, tp.fcr, out);
+        L4Args A{};
+        A.Fs = out.Fo;
+        A.Cs = out.Co;
+        A.Rs = out.Ro;
+        A.p2f = tp.p2f.as<int>();
+        A.ops = tp.ops.as<double2>();
+        A.N = tp.N;
+        A.nf = tp.nf;
+        A.num_ops = tp.num_ops;
+        A.nf3 = static_cast<long>(nf3);
+        A.L4 = tp.lgen.as<double2>();
+        A.GW = nullptr;
+        launch_l4(c, A, static_cast<long>(T));
+    } else {
+        const long cnt = static_cast<long>(T * N4);
+        hipLaunchKernelGGL(k_fill_zero, dim3(static_cast<unsigned>((cnt + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                           c->stream, tp.lgen.as<double2>(), cnt);
+    }
+    Tcl2Args A2{};
+    A2.gamma = tp.gamma.as<double2>();
+    A2.Nt = static_cast<long>(tp.Nt);
+    A2.eps = tp.eps.as<double>();
+    A2.ops = tp.ops.as<double2>();
+    A2.p2f = tp.p2f.as<int>();
+    A2.N = tp.N;
+    A2.nf = tp.nf;
+    A2.num_ops = tp.num_ops;
+    A2.L = tp.lgen.as<double2>();
+    A2.order = tp.order;
+    hipLaunchKernelGGL(k_tcl2, dim3(static_cast<unsigned>(T)), dim3(kBlock), 2 * N2 * 16, c->stream, A2);
+    HIP_OK(hipGetLastError());
+    // RK4 with every state sampled; checkpoints r_{qK} kept on the tape.
+    c->misc.ensure((T + 1) * D * 16 + 64);
+    double2* dr = c->misc.as<double2>();
+    double2* ds = dr + D;
+    HIP_OK(hipMemcpyAsync(dr, r0, D * 16, hipMemcpyHostToDevice, c->stream));
+    rk4_launch(c, tp.lgen.as<double2>(), nullptr, tp.steps, static_cast<int>(D), dr, tp.dt, 1, ds, nullptr);
+    const std::size_t nseg = (tp.steps + tp.K - 1) / tp.K;
+    std::vector<int> rows(nseg);
+    for (std::size_t q = 0; q < nseg; ++q) rows[q] = static_cast<int>(q * tp.K);
+    tp.ck.ensure(std::max<std::size_t>(nseg, 1) * D * 16 + nseg * sizeof(int) + 16);
+    int* drows = reinterpret_cast<int*>(tp.ck.as<double2>() + nseg * D);
+    HIP_OK(hipMemcpyAsync(drows, rows.data(), nseg * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_gather_rows, dim3(static_cast<unsigned>(nseg)), dim3(kBlock), 0, c->stream, ds, drows,
+                       static_cast<long>(D), tp.ck.as<double2>());
+    HIP_OK(hipGetLastError());
+    if (samples_out) HIP_OK(hipMemcpyAsync(samples_out, ds, T * D * 16, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+}
+
+static void tape_adjoint(taco_tape& tp, const double* samples_bar, double* gamma_bar, double* omega_bar,
+                         double* ops_bar, double* eps_bar, double* r0_bar) {
+    taco_ctx* c = tp.ctx;
+    const std::size_t N2 = static_cast<std::size_t>(tp.N) * tp.N, D = N2, D2 = D * D;
+    const std::size_t T = tp.T;
+    const std::size_t nf3 = static_cast<std::size_t>(tp.nf) * tp.nf * tp.nf;
+    const std::size_t K = static_cast<std::size_t>(tp.K);
+    // device layout in the context's rk buffer: gbar, Lbar, seg, r0bar
+    c->rk.ensure((T * D + T * D2 + (K + 1) * D + D) * 16);
+    double2* gb = c->rk.as<double2>();
+    double2* Lb = gb + T * D;
+    double2* seg = Lb + T * D2;
+    double2* rb = seg + (K + 1) * D;
+    HIP_OK(hipMemcpyAsync(gb, samples_bar, T * D * 16, hipMemcpyHostToDevice, c->stream));
+    const long cntL = static_cast<long>(T * D2);
+    hipLaunchKernelGGL(k_fill_zero, dim3(static_cast<unsigned>((cntL + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       c->stream, Lb, cntL);
+    Rk4AdjArgs R{};
+    R.Ls = tp.lgen.as<double2>();
+    R.ck = tp.ck.as<double2>();
+    R.gbar = gb;
+    R.Lbar = Lb;
+    R.seg = seg;
+    R.r0bar = rb;
+    R.steps = static_cast<long>(tp.steps);
+    R.D = static_cast<int>(D);
+    R.K = tp.K;
+    R.dt = tp.dt;
+    const std::size_t rk_lds = 12 * D * 16;
+    if (rk_lds > 160 * 1024) throw Error(TACO_ERR_INVALID, "adjoint RK4 kernel supports D <= 853");
+    set_fn_lds(reinterpret_cast<const void*>(k_rk4_adjoint), rk_lds);
+    timed(c, K_RK4, 0.0, [&] {
+        hipLaunchKernelGGL(k_rk4_adjoint, dim3(1), dim3(kBlock), rk_lds, c->stream, R);
+    });
+    HIP_OK(hipGetLastError());
+    std::vector<std::complex<double>> gbar_h(static_cast<std::size_t>(tp.nf) * T, {0.0, 0.0});
+    std::vector<std::complex<double>> opsb(static_cast<std::size_t>(tp.num_ops) * N2, {0.0, 0.0});
+    std::vector<double> omb(static_cast<std::size_t>(tp.nf), 0.0), epsb(static_cast<std::size_t>(tp.N), 0.0);
+    if (tp.order == 4) {
+        const FcrJob& J = tp.fcr;
+        const long ncomb = J.ncomb;
+        // L4 adjoint: slice cotangents [T][3][nf3], operator partials [T][ops N^2]
+        const bool lds_sl = l4_adj_lds(tp.N, tp.num_ops, nf3, true) <= 96 * 1024;
+        const std::size_t lds = l4_adj_lds(tp.N, tp.num_ops, nf3, lds_sl);
+        if (lds > 160 * 1024) throw Error(TACO_ERR_INVALID, "L4 adjoint kernel supports N <= 8");
+        set_fn_lds(reinterpret_cast<const void*>(k_l4_adjoint), lds);
+        const std::size_t sbar_n = T * 3 * nf3, abar_n = T * tp.num_ops * N2;
+        const std::size_t nscr = static_cast<std::size_t>(ncomb) * 11 * T;
+        c->inter2.ensure((sbar_n + abar_n + nscr + static_cast<std::size_t>(tp.nf) * T) * 16 +
+                         static_cast<std::size_t>(ncomb) * 8 + 64);
+        double2* sbar = c->inter2.as<double2>();
+        double2* abar = sbar + sbar_n;
+        double2* scr = abar + abar_n;
+        double2* gb4 = scr + nscr;
+        double* om = reinterpret_cast<double*>(gb4 + static_cast<std::size_t>(tp.nf) * T);
+        if (!lds_sl)
+            hipLaunchKernelGGL(k_fill_zero, dim3(static_cast<unsigned>((sbar_n + kBlock - 1) / kBlock)), dim3(kBlock),
+                               0, c->stream, sbar, static_cast<long>(sbar_n));
+        L4AdjArgs A{};
+        A.Lbar = Lb;
+        A.Fs = tp.slices.as<double2>();
+        A.Cs = A.Fs + nf3 * T;
+        A.Rs = A.Fs + 2 * nf3 * T;
+        A.p2f = tp.p2f.as<int>();
+        A.ops = tp.ops.as<double2>();
+        A.N = tp.N;
+        A.nf = tp.nf;
+        A.num_ops = tp.num_ops;
+        A.nf3 = static_cast<long>(nf3);
+        A.sbar = sbar;
+        A.Abar = abar;
+        A.lds_slices = lds_sl ? 1 : 0;
+        timed(c, K_L4, 0.0, [&] {
+            hipLaunchKernelGGL(k_l4_adjoint, dim3(static_cast<unsigned>(T)), dim3(kBlock), lds, c->stream, A);
+        });
+        HIP_OK(hipGetLastError());
+        if (ncomb > 0) {
+            FcrAdjArgs P{};
+            P.gamma = tp.gamma.as<double2>();
+            P.omegas = tp.omegas.as<double>();
+            P.mirror = tp.mirror.as<int>();
+            P.nf = tp.nf;
+            P.Nt = static_cast<long>(tp.Nt);
+            P.dt = tp.dt;
+            P.T = static_cast<long>(T);
+            P.job_tri = tp.jobtri.as<int>();
+            P.job_mask = tp.jobmask.as<unsigned char>();
+            P.ncomb = ncomb;
+            P.sbar = sbar;
+            P.nf3 = static_cast<long>(nf3);
+            P.scr = scr;
+            P.om = om;
+            const unsigned nub = static_cast<unsigned>((T + kBlock - 1) / kBlock);
+            timed(c, K_COMBINE, 0.0, [&] {
+                hipLaunchKernelGGL(k_fcr_adj_scan, dim3(static_cast<unsigned>(ncomb)), dim3(kBlock), 0, c->stream, P);
+                hipLaunchKernelGGL(k_fcr_adj_corr, dim3(static_cast<unsigned>(ncomb), nub), dim3(kBlock), 0,
+                                   c->stream, P);
+                hipLaunchKernelGGL(k_fcr_adj_fin, dim3(static_cast<unsigned>(ncomb)), dim3(kBlock), 0, c->stream, P);
+            });
+            HIP_OK(hipGetLastError());
+            // bucket lists: (position, role) with role 0 g1 = Gamma_i, 1 g2 = Gamma_j, 2 gm = Gamma_mirror(j)
+            std::vector<std::vector<int>> lists(static_cast<std::size_t>(tp.nf));
+            for (long q = 0; q < ncomb; ++q) {
+                const int tri = J.order[static_cast<std::size_t>(q)];
+                const int i = tri / (tp.nf * tp.nf), j = (tri / tp.nf) % tp.nf;
+                lists[static_cast<std::size_t>(i)].push_back(static_cast<int>(q * 3 + 0));
+                lists[static_cast<std::size_t>(j)].push_back(static_cast<int>(q * 3 + 1));
+                lists[static_cast<std::size_t>(tp.h_mirror[static_cast<std::size_t>(j)])].push_back(
+                    static_cast<int>(q * 3 + 2));
+            }
+            std::vector<int> start(1, 0), ent;
+            for (auto& l : lists) {
+                ent.insert(ent.end(), l.begin(), l.end());
+                start.push_back(static_cast<int>(ent.size()));
+            }
+            c->misc.ensure((start.size() + ent.size()) * sizeof(int) + 16);
+            int* dstart = c->misc.as<int>();
+            int* dent = dstart + start.size();
+            HIP_OK(hipMemcpyAsync(dstart, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+            HIP_OK(hipMemcpyAsync(dent, ent.data(), ent.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+            hipLaunchKernelGGL(k_bucket_reduce, dim3(nub, static_cast<unsigned>(tp.nf)), dim3(kBlock), 0, c->stream,
+                               scr, static_cast<long>(T), dstart, dent, static_cast<long>(T), gb4);
+            HIP_OK(hipGetLastError());
+            HIP_OK(hipMemcpyAsync(gbar_h.data(), gb4, gbar_h.size() * 16, hipMemcpyDeviceToHost, c->stream));
+            std::vector<double> omp(static_cast<std::size_t>(ncomb));
+            HIP_OK(hipMemcpyAsync(omp.data(), om, omp.size() * 8, hipMemcpyDeviceToHost, c->stream));
+            HIP_OK(hipStreamSynchronize(c->stream));
+            for (long q = 0; q < ncomb; ++q) {
+                const int tri = J.order[static_cast<std::size_t>(q)];
+                const int k = tri % tp.nf, j = (tri / tp.nf) % tp.nf, i = tri / (tp.nf * tp.nf);
+                const double v = omp[static_cast<std::size_t>(q)];
+                omb[static_cast<std::size_t>(i)] += v;
+                omb[static_cast<std::size_t>(j)] += v;
+                omb[static_cast<std::size_t>(k)] += v;
+            }
+        }
+        // operator cotangents of the L4 part: per-time partials summed in time order
+        std::vector<std::complex<double>> ap(abar_n);
+        HIP_OK(hipMemcpyAsync(ap.data(), abar, abar_n * 16, hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        for (std::size_t t = 0; t < T; ++t)
+            for (std::size_t e = 0; e < opsb.size(); ++e) opsb[e] += ap[t * opsb.size() + e];
+    }
+    // TCL2 part on the host from Lbar
+    std::vector<std::complex<double>> Lh(T * D2);
+    HIP_OK(hipMemcpyAsync(Lh.data(), Lb, T * D2 * 16, hipMemcpyDeviceToHost, c->stream));
+    std::vector<std::complex<double>> r0b(D);
+    HIP_OK(hipMemcpyAsync(r0b.data(), rb, D * 16, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    tcl2_adjoint(Lh.data(), T, tp.h_gamma.data(), T, tp.h_eps.data(), tp.h_ops.data(), tp.num_ops, tp.N,
+                 tp.h_p2f.data(), tp.nf, tp.order, gbar_h.data(), T, opsb.data(), epsb.data());
+    if (gamma_bar) {
+        auto* g = reinterpret_cast<std::complex<double>*>(gamma_bar);
+        std::fill(g, g + tp.Nt * static_cast<std::size_t>(tp.nf), std::complex<double>(0.0, 0.0));
+        for (int b = 0; b < tp.nf; ++b)
+            for (std::size_t t = 0; t < T; ++t)
+                g[static_cast<std::size_t>(b) * tp.Nt + t] = gbar_h[static_cast<std::size_t>(b) * T + t];
+    }
+    if (omega_bar) std::copy(omb.begin(), omb.end(), omega_bar);
+    if (ops_bar) std::copy(opsb.begin(), opsb.end(), reinterpret_cast<std::complex<double>*>(ops_bar));
+    if (eps_bar) std::copy(epsb.begin(), epsb.end(), eps_bar);
+    if (r0_bar) std::copy(r0b.begin(), r0b.end(), reinterpret_cast<std::complex<double>*>(r0_bar));
+}
+
+}  // namespace taco_engine
+
 using namespace taco_engine;
 
 extern "C" {
@@ -1863,15 +2159,6 @@ int taco_l4_from_slices(taco_ctx* ctx, const double* Fs, const double* Cs, const
         if (GW_out) HIP_OK(hipMemcpyAsync(GW_out, A.GW, N4 * 16, hipMemcpyDeviceToHost, ctx->stream));
         HIP_OK(hipStreamSynchronize(ctx->stream));
     });
-}
-
-static void rk4_launch(taco_ctx* ctx, const double2* dLs, const double2* dLh, std::size_t steps, int D, double2* dr,
-                       double dt, std::size_t every, double2* dsamples, long* dns) {
-    const std::size_t lds = 6 * static_cast<std::size_t>(D) * 16;
-    if (lds > 160 * 1024) throw Error(TACO_ERR_INVALID, "RK4 kernel supports D <= 1706");
-    hipLaunchKernelGGL(k_rk4, dim3(1), dim3(kBlock), lds, ctx->stream, dLs, dLh, static_cast<long>(steps), D, dr, dt,
-                       static_cast<long>(every), dsamples, dns);
-    HIP_OK(hipGetLastError());
 }
 
 int taco_rk4_propagate(taco_ctx* ctx, const double* L_series, const double* L_half, size_t steps, int D, double* r,
@@ -2151,6 +2438,94 @@ int taco_batched_run(taco_ctx* ctx, const taco_batch_system* systems, int nsys, 
     const std::string msg = rc ? std::string(taco_last_error()) : std::string();
     const int rc2 = taco_batch_destroy(b);
     if (rc) set_last_error(msg.c_str());  // keep the failure's message past the clean-up
+    return rc ? rc : rc2;
+}
+
+int taco_tape_forward(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, const double* omegas, const int* mirror,
+                      const int* pair_to_freq, int N, const double* eps, const double* ops, int num_ops, double dt,
+                      size_t steps, int order, const double* r0, size_t checkpoint_every, double* samples_out,
+                      taco_tape** out) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!out) throw Error(TACO_ERR_INVALID, "null output handle");
+        *out = nullptr;
+        if (!(order == 0 || order == 2 || order == 4)) throw Error(TACO_ERR_INVALID, "order must be 0, 2, or 4");
+        validate_system(Nt, nf, mirror, pair_to_freq, N, num_ops, dt, TACO_FCR_CONVOLUTION);
+        if (!gamma || !omegas || !mirror || !pair_to_freq || !ops || !eps || !r0)
+            throw Error(TACO_ERR_INVALID, "tape inputs must be non-null");
+        if (steps < 1) throw Error(TACO_ERR_INVALID, "steps must be >= 1");
+        if (steps + 1 > Nt) throw Error(TACO_ERR_RANGE, "steps exceed gamma_series rows");
+        if (checkpoint_every < 1) throw Error(TACO_ERR_INVALID, "checkpoint_every must be >= 1");
+        DeviceGuard g(ctx->device);
+        auto tp = std::make_unique<taco_tape>();
+        tp->ctx = ctx;
+        tp->Nt = Nt;
+        tp->T = steps + 1;
+        tp->steps = steps;
+        tp->nf = nf;
+        tp->N = N;
+        tp->num_ops = num_ops;
+        tp->order = order;
+        tp->K = static_cast<int>(std::min<std::size_t>(checkpoint_every, steps));
+        tp->dt = dt;
+        const std::size_t N2 = static_cast<std::size_t>(N) * N;
+        const auto* gc = reinterpret_cast<const std::complex<double>*>(gamma);
+        tp->h_gamma.resize(static_cast<std::size_t>(nf) * tp->T);
+        for (int b = 0; b < nf; ++b)
+            std::copy(gc + static_cast<std::size_t>(b) * Nt, gc + static_cast<std::size_t>(b) * Nt + tp->T,
+                      tp->h_gamma.begin() + static_cast<std::size_t>(b) * tp->T);
+        tp->h_eps.assign(eps, eps + N);
+        tp->h_omegas.assign(omegas, omegas + nf);
+        tp->h_p2f.assign(pair_to_freq, pair_to_freq + N2);
+        tp->h_mirror.assign(mirror, mirror + nf);
+        const auto* oc = reinterpret_cast<const std::complex<double>*>(ops);
+        tp->h_ops.assign(oc, oc + N2 * num_ops);
+        auto put = [&](DevBuf& buf, const void* host, std::size_t bytes) {
+            buf.ensure(std::max<std::size_t>(bytes, 16));
+            HIP_OK(hipMemcpy(buf.p, host, bytes, hipMemcpyHostToDevice));
+        };
+        put(tp->gamma, gamma, Nt * nf * 16);
+        put(tp->omegas, omegas, nf * sizeof(double));
+        put(tp->mirror, mirror, nf * sizeof(int));
+        put(tp->p2f, pair_to_freq, N2 * sizeof(int));
+        put(tp->ops, ops, N2 * num_ops * 16);
+        put(tp->eps, eps, N * sizeof(double));
+        tape_forward(*tp, r0, samples_out);
+        *out = tp.release();
+    });
+}
+
+int taco_tape_adjoint(taco_tape* tape, const double* samples_bar, double* gamma_bar, double* omega_bar,
+                      double* ops_bar, double* eps_bar, double* r0_bar) {
+    return guarded([&] {
+        if (!tape) throw Error(TACO_ERR_INVALID, "null taco_tape");
+        if (!samples_bar) throw Error(TACO_ERR_INVALID, "null samples_bar");
+        DeviceGuard g(tape->ctx->device);
+        tape_adjoint(*tape, samples_bar, gamma_bar, omega_bar, ops_bar, eps_bar, r0_bar);
+    });
+}
+
+int taco_tape_destroy(taco_tape* tape) {
+    return guarded([&] {
+        if (!tape) return;
+        DeviceGuard g(tape->ctx->device);
+        (void)hipStreamSynchronize(tape->ctx->stream);
+        delete tape;
+    });
+}
+
+int taco_adjoint(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, const double* omegas, const int* mirror,
+                 const int* pair_to_freq, int N, const double* eps, const double* ops, int num_ops, double dt,
+                 size_t steps, int order, const double* r0, size_t checkpoint_every, const double* samples_bar,
+                 double* gamma_bar, double* omega_bar, double* ops_bar, double* eps_bar, double* r0_bar) {
+    taco_tape* tp = nullptr;
+    int rc = taco_tape_forward(ctx, gamma, Nt, nf, omegas, mirror, pair_to_freq, N, eps, ops, num_ops, dt, steps,
+                               order, r0, checkpoint_every, nullptr, &tp);
+    if (rc) return rc;
+    rc = taco_tape_adjoint(tp, samples_bar, gamma_bar, omega_bar, ops_bar, eps_bar, r0_bar);
+    const std::string msg = rc ? std::string(taco_last_error()) : std::string();
+    const int rc2 = taco_tape_destroy(tp);
+    if (rc) set_last_error(msg.c_str());
     return rc ? rc : rc2;
 }
 

@@ -1,6 +1,7 @@
 // engine_internal.hpp — shared declarations of the TCL4 engine sources.
 #pragma once
 
+#include <complex>
 #include <cstddef>
 #include <stdexcept>
 #include <string>
@@ -25,6 +26,14 @@ void mirror_indices(const double* omegas, int nf, double fidx_tol, int* mirror);
 void system_build(const double* H, int N, const double* A_lab, int num_ops, double freq_tol, double* eps,
                   double* U, double* A_eig, int* nf_out, double* omegas, int* mirror, int* p2f,
                   int* zero_index);
+
+// Reverse of the TCL2 generator (adjoint_host.cpp); complex values are
+// interleaved doubles.  Adds into gamma_bar ([nf][gbld], rows t < T),
+// ops_bar and eps_bar.
+void tcl2_adjoint(const std::complex<double>* Lbar, std::size_t T, const std::complex<double>* gamma,
+                  std::size_t gld, const double* eps, const std::complex<double>* ops, int num_ops, int N,
+                  const int* p2f, int nf, int order, std::complex<double>* gamma_bar, std::size_t gbld,
+                  std::complex<double>* ops_bar, double* eps_bar);
 
 void set_last_error(const std::string& msg);
 const double* cr_table();

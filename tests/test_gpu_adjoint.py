@@ -1,0 +1,154 @@
+"""GPU checks of the hand-written reverse pass (taco_tape_* of the C ABI) and
+of the parameter gradients built on it.
+
+* The tape's forward equals tcl4_propagate bit for bit.
+* The vector-Jacobian product of r_0..r_steps with respect to every engine
+  input (Gamma table, bucket frequencies, A_eig, eigenvalues, r0) matches
+  central finite differences of the oracle's forward along random
+  directions: a dropped term, a sign or an index error in any of the RK4,
+  L4, F/C/R or TCL2 adjoints shows up here.
+* The gradient of the spin-pair loss with respect to (alpha, omega_c, beta,
+  eps, Delta) matches central finite differences (h = 1e-6 relative) of the
+  oracle pipeline (system build, bath correlation, trapz Gamma, TCL4 run).
+
+Finite differences limit the comparison: the bar is 1e-6 relative (the
+printed values show the actual agreement)."""
+import numpy as np
+import pytest
+import torch
+
+import taco_amd
+from taco_amd import adjoint as AD
+from taco_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+FD_TOL = 1e-6
+
+
+def r0_for(N):
+    r = np.zeros(N * N, complex)
+    r[0] = 1.0
+    return r
+
+
+def cases():
+    out = []
+    sb = W.spin_boson(Nt=1024, tmax=200, tstep=1)
+    out.append(("spin_boson", sb.system, sb.gamma, sb.dt, 200, 64))
+    sp = W.spin_pair(Nt=2048, steps=300)
+    out.append(("spin_pair", sp.system, sp.gamma, sp.dt, 300, 64))
+    rng = W.SplitMix64(11)
+    H = W.gue(3, rng)
+    A1, A2 = W.random_hermitian_unit(3, rng), W.random_hermitian_unit(3, rng)
+    g2 = W.Workload("gue3_two_ops", H, [A1, A2], taco_amd.DrudeLorentz(0.08, 4.0), 0.7, 0.01, 511,
+                    list(range(151)), steps=150)
+    out.append(("gue3_two_ops", g2.system, g2.gamma, g2.dt, 150, 32))
+    return out
+
+
+CASES = cases()
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_tape_forward_equals_tcl4_propagate(case):
+    name, S, G, dt, steps, K = case
+    tp = AD.Tape(G, S.omegas, S.mirror, S.pair_to_freq, S.A_eig, S.eps, dt, steps, r0_for(S.N), 4, K)
+    ref = taco_amd.tcl4_propagate(S, G, dt, steps, r0_for(S.N))
+    assert np.array_equal(tp.samples, ref)
+    tp.close()
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_vjp_matches_oracle_finite_differences(orc, case):
+    name, S, G, dt, steps, K = case
+    N = S.N
+    rng = np.random.default_rng(5)
+    Wt = rng.standard_normal((steps + 1, N * N)) + 1j * rng.standard_normal((steps + 1, N * N))
+    tp = AD.Tape(G, S.omegas, S.mirror, S.pair_to_freq, S.A_eig, S.eps, dt, steps, r0_for(N), 4, K)
+    g = tp.adjoint(Wt)
+    tp.close()
+
+    def loss(Gx=G, om=S.omegas, ops=S.A_eig, eps=S.eps, r0=r0_for(N)):
+        s = orc.tcl4_propagate(Gx, om, S.mirror, S.pair_to_freq, eps, ops, dt, steps, r0)
+        return float(np.real(np.vdot(Wt, s)))
+
+    l0 = abs(loss())
+
+    def check(label, got, f, h):
+        # central differences: truncation O(h^2), rounding ~1e-14 |loss| / h
+        ref = (f(h) - f(-h)) / (2 * h)
+        tol = FD_TOL * abs(ref) + 1e-14 * l0 / h
+        print(f"{name} {label}: vjp {got:.12e} fd {ref:.12e} diff {abs(got - ref):.2e} tol {tol:.2e}")
+        assert abs(got - ref) <= tol, label
+
+    dG = (rng.standard_normal(G.shape) + 1j * rng.standard_normal(G.shape)) * np.abs(G).max()
+    check("gamma", float(np.real(np.vdot(g["gamma"], dG))), lambda s: loss(Gx=G + s * dG), 1e-5)
+    dw = rng.standard_normal(S.nf)
+    check("omegas", float(g["omegas"] @ dw), lambda s: loss(om=S.omegas + s * dw), 1e-4)
+    dA = [rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N)) for _ in S.A_eig]
+    got = sum(float(np.real(np.vdot(a, d))) for a, d in zip(g["ops"], dA))
+    check("ops", got, lambda s: loss(ops=[A + s * d for A, d in zip(S.A_eig, dA)]), 1e-5)
+    de = rng.standard_normal(N)
+    check("eps", float(g["eps"] @ de), lambda s: loss(eps=S.eps + s * de), 1e-4)
+    dr = rng.standard_normal(N * N) + 1j * rng.standard_normal(N * N)
+    check("r0", float(np.real(np.vdot(g["r0"], dr))), lambda s: loss(r0=r0_for(N) + s * dr), 1e-2)
+
+
+def test_tape_adjoint_repeatable():
+    name, S, G, dt, steps, K = CASES[1]
+    N = S.N
+    Wt = np.ones((steps + 1, N * N), complex)
+    tp = AD.Tape(G, S.omegas, S.mirror, S.pair_to_freq, S.A_eig, S.eps, dt, steps, r0_for(N), 4, K)
+    a, b = tp.adjoint(Wt), tp.adjoint(Wt)
+    tp.close()
+    for key in ("gamma", "omegas", "eps", "r0"):
+        assert np.allclose(a[key], b[key], rtol=1e-12, atol=1e-14 * np.abs(a[key]).max()), key
+    # the reverse of the checkpointed run equals the reverse with every state kept
+    tp1 = AD.Tape(G, S.omegas, S.mirror, S.pair_to_freq, S.A_eig, S.eps, dt, steps, r0_for(N), 4, 1)
+    c = tp1.adjoint(Wt)
+    tp1.close()
+    for key in ("gamma", "omegas", "eps", "r0"):
+        assert np.allclose(a[key], c[key], rtol=1e-12, atol=1e-14 * np.abs(a[key]).max()), key
+
+
+def oracle_spin_pair_rho_lab(orc, prob, p):
+    alpha, wc, beta, eps, delta = p
+    H1 = 0.5 * eps * AD.SZ + 0.5 * delta * AD.SX
+    H = np.kron(H1, AD.I2) + np.kron(AD.I2, H1)
+    A = prob.coupling()
+    sysd = orc.system_build(H, [A], prob.freq_tol)
+    C = orc.bcf_fft(prob.Nt - 1, prob.dt, beta, 0, alpha, wc)
+    G = orc.gamma_trapz(C, prob.dt, sysd["omegas"])
+    r0 = r0_for(sysd["N"])
+    s = orc.tcl4_propagate(G, sysd["omegas"], sysd["mirror"], sysd["pair_to_freq"], sysd["eps"], sysd["A_eig"],
+                           prob.dt, prob.steps, r0, order=prob.order)
+    N = sysd["N"]
+    rho = s.reshape(-1, N, N).transpose(0, 2, 1)
+    U = sysd["U"]
+    return U @ rho @ U.conj().T
+
+
+def test_spin_pair_parameter_gradient_matches_oracle_fd(orc):
+    prob = AD.SpinPairProblem(Nt=2 ** 12, steps=400, checkpoint_every=64)
+    p0 = np.array([0.05, 5.0, 1.0, 1.0, 1.0])
+    u = np.where(np.random.default_rng(4).uniform(size=5) < 0.5, -1.0, 1.0)
+    target = oracle_spin_pair_rho_lab(orc, prob, p0 * (1 + 0.05 * u))
+    params = torch.tensor(p0, dtype=torch.float64, requires_grad=True)
+    loss = prob.loss(params, torch.from_numpy(target))
+    (grad,) = torch.autograd.grad(loss, params)
+
+    def oloss(p):
+        d = oracle_spin_pair_rho_lab(orc, prob, p) - target
+        return float(np.sum(d.real ** 2 + d.imag ** 2))
+
+    l0 = oloss(p0)
+    assert abs(float(loss) - l0) <= 1e-10 * l0
+    for q, name in enumerate(prob.names):
+        h = 1e-6 * abs(p0[q])
+        pp, pm = p0.copy(), p0.copy()
+        pp[q] += h
+        pm[q] -= h
+        ref = (oloss(pp) - oloss(pm)) / (2 * h)
+        got = float(grad[q])
+        print(f"d loss / d {name}: adjoint {got:.12e} fd {ref:.12e} rel {abs(got - ref) / abs(ref):.2e}")
+        assert abs(got - ref) <= FD_TOL * abs(ref), name
