@@ -373,6 +373,14 @@ def main():
     ms_per_step = 1000.0 * elapsed / a.steps
     value = ntau * a.steps / elapsed
     kernels, roofline = kernel_table(kern, a.steps)
+    # HBM traffic of the dominant kernel measured by rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE passes of this workload (committed summary), when present
+    tf = os.path.join(ROOT, "profiles", "fcr_l4_gue_pair_d4_traffic.json")
+    if a.workload == "gue_pair" and not a.reduced and os.path.exists(tf):
+        t = json.load(open(tf))
+        if t.get("kernel") == roofline["kernel"]:
+            roofline["traffic"] = t["traffic_bytes_per_launch"]
+            roofline["traffic_source"] = t["source"]
     from taco_amd import _lib
     choice = np.zeros(3, np.int32)
     nfft = 1 << int(np.ceil(np.log2(2 * G.shape[0] - 1)))
