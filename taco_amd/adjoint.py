@@ -218,20 +218,26 @@ class GammaFun(torch.autograd.Function):
     @staticmethod
     def backward(ctx, Gbar):
         Cn, om, dt = ctx.save
-        Gb = Gbar.detach().numpy()  # [Nt, nf]
+        Gb_full = Gbar.detach().numpy()  # [Nt, nf]
         Nt = Cn.size
-        m = np.arange(Nt)
-        # Q_b(m): weight-summed cotangents; Q(0) = sum_{k>=1} Gb, Q(m) = Gb(m) + 2 sum_{k>m} Gb
-        suf = np.cumsum(Gb[::-1], axis=0)[::-1]  # sum_{k>=m}
-        Q = np.empty_like(Gb)
-        Q[0] = suf[1] if Nt > 1 else 0.0
-        if Nt > 1:
-            after = np.vstack([suf[2:], np.zeros((1, Gb.shape[1]))]) if Nt > 2 else np.zeros((1, Gb.shape[1]))
-            Q[1:] = Gb[1:] + 2.0 * after
-        phi = np.exp(1j * np.outer(m * dt, om))  # [Nt, nf]
-        Cbar = (dt / 2.0) * np.sum(np.conj(phi) * Q, axis=1)
-        dphi = (1j * (m * dt))[:, None] * phi * Cn[:, None]
-        wbar = (dt / 2.0) * np.real(np.sum(np.conj(Q) * dphi, axis=0))
+        # Gamma(k) depends on C_m for m <= k only: rows past the last nonzero
+        # cotangent contribute nothing
+        nz = np.flatnonzero(np.any(Gb_full != 0, axis=1))
+        T = int(nz[-1]) + 1 if nz.size else 0
+        Cbar = np.zeros(Nt, np.complex128)
+        wbar = np.zeros(om.size)
+        if T > 0:
+            Gb = Gb_full[:T]
+            m = np.arange(T)
+            # Q_b(m): weight-summed cotangents; Q(0) = sum_{k>=1} Gb, Q(m) = Gb(m) + 2 sum_{k>m} Gb
+            suf = np.vstack([np.cumsum(Gb[::-1], axis=0)[::-1], np.zeros((1, Gb.shape[1]))])  # sum_{k>=m}
+            Q = np.empty_like(Gb)
+            Q[0] = suf[1]
+            Q[1:] = Gb[1:] + 2.0 * suf[2:]
+            phi = np.exp(1j * np.outer(m * dt, om))  # [T, nf]
+            Cbar[:T] = (dt / 2.0) * np.sum(np.conj(phi) * Q, axis=1)
+            dphi = (1j * (m * dt))[:, None] * phi * Cn[:T, None]
+            wbar = (dt / 2.0) * np.real(np.sum(np.conj(Q) * dphi, axis=0))
         return torch.from_numpy(Cbar), torch.from_numpy(wbar), None
 
 

@@ -28,7 +28,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
 ]
-CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra"]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fopenmp", "-Wall", "-Wextra"]
 
 
 def _sources(ext: str) -> list[str]:
@@ -54,7 +54,7 @@ def build(verbose: bool = False) -> str:
         obj = os.path.join(BUILD_DIR, os.path.basename(src) + ".o")
         run([hipcc, *HIP_FLAGS, "-I", INCLUDE, "-c", src, "-o", obj])
         objs.append(obj)
-    run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp", *objs, "-lquadmath"])
+    run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp", *objs, "-lquadmath", "-lgomp"])
     os.replace(LIB + ".tmp", LIB)
     return LIB
 

@@ -7,23 +7,30 @@
 // i dLoss/dIm z (torch's convention for real losses).
 #pragma once
 
-// y = L^H x (column r of L conjugated), summed over c in order.
-__device__ void matvec_h(const double2* __restrict__ L, const double2* x, double2* y, int D) {
-    for (int r = threadIdx.x; r < D; r += blockDim.x) {
+// y = L^H x (column r of L conjugated, column stride ld), summed over c in order.
+template <int DT>
+__device__ __forceinline__ void matvec_h(const double2* __restrict__ L, int ld, const double2* x, double2* y, int D) {
+    const int n = DT ? DT : D;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
         double2 s = make_double2(0.0, 0.0);
-        const double2* col = L + static_cast<long>(r) * D;
-        for (int c = 0; c < D; ++c) s = cadd(s, cmul(cconj(col[c]), x[c]));
+        const double2* col = L + static_cast<long>(r) * ld;
+#pragma unroll
+        for (int c = 0; c < (DT ? DT : n); ++c) s = cadd(s, cmul(cconj(col[c]), x[c]));
         y[r] = s;
     }
 }
 
 // y = (0.5 (L0 + L1))^H x.
-__device__ void matvec_h_half(const double2* __restrict__ L0, const double2* __restrict__ L1, const double2* x,
-                              double2* y, int D) {
-    for (int r = threadIdx.x; r < D; r += blockDim.x) {
+template <int DT>
+__device__ __forceinline__ void matvec_h_half(const double2* __restrict__ L0, const double2* __restrict__ L1, int ld,
+                                              const double2* x, double2* y, int D) {
+    const int n = DT ? DT : D;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
         double2 s = make_double2(0.0, 0.0);
-        const long o = static_cast<long>(r) * D;
-        for (int c = 0; c < D; ++c) s = cadd(s, cmul(cconj(cscale(cadd(L0[o + c], L1[o + c]), 0.5)), x[c]));
+        const long o = static_cast<long>(r) * ld;
+#pragma unroll
+        for (int c = 0; c < (DT ? DT : n); ++c)
+            s = cadd(s, cmul(cconj(cscale(cadd(L0[o + c], L1[o + c]), 0.5)), x[c]));
         y[r] = s;
     }
 }
@@ -32,113 +39,206 @@ __device__ void matvec_h_half(const double2* __restrict__ L0, const double2* __r
 // RK4 adjoint.  Forward step n (rk4_step): k1 = L0 r, u1 = r + h k1,
 // k2 = Lh u1, u2 = r + h k2, k3 = Lh u2, u3 = r + dt k3, k4 = L1 u3,
 // r' = r + s (k1 + 2 k2 + 2 k3 + k4), h = dt/2, s = dt/6, Lh = (L0 + L1)/2.
-// Backwards from lam = dLoss/dr_steps: each step maps lam_{n+1} to
-// lam_n = M_n^H lam_{n+1} + gbar_n and adds the cotangents of L_n and L_{n+1}
-// into Lbar.  The states r_n are recomputed segment by segment (K steps)
-// from checkpoints ck[q] = r_{qK}, with the forward step's own arithmetic.
-// One workgroup; LDS: 12 vectors of D.
+// Two launches.  k_rk4_recompute: one workgroup per checkpoint segment (K
+// steps) recomputes, from the checkpoint ck[q] = r_{qK} and with the forward
+// step's own arithmetic, the states r_n and stage points u1..u3 of its
+// segment; segments are independent, so all run at once.  k_rk4_adjoint:
+// one workgroup walks n = steps-1 .. 0, mapping lam_{n+1} to
+// lam_n = M_n^H lam_{n+1} + gbar_n and forming the cotangents of L_n and
+// L_{n+1}.  LDS: vectors of D, then (nstage >= 2) a window of generators
+// staged with column stride D + 1.  Lbar(t) collects the L1 part of step t-1
+// and the L0 part of step t: the latter is carried in registers
+// (D^2 <= 16 x 256) and each Lbar(t) stored once; otherwise Lbar (zero on
+// entry) is accumulated in place.
 // ---------------------------------------------------------------------------
 struct Rk4AdjArgs {
     const double2* Ls;    // [steps + 1][D * D] generators
-    const double2* ck;    // [ceil(steps / K) + 1][D] checkpoint states r_{qK}
+    const double2* ck;    // [ceil(steps / K)][D] checkpoint states r_{qK}
     const double2* gbar;  // [steps + 1][D] dLoss/dr_n
-    double2* Lbar;        // [steps + 1][D * D], zero on entry
-    double2* seg;         // [K + 1][D] recomputed states of one segment
+    double2* Lbar;        // [steps + 1][D * D]
+    double2* seg;         // [steps][4][D] state r_n and stage points u1, u2, u3 of step n
     double2* r0bar;       // [D]
     long steps;
     int D;
     int K;
     double dt;
+    int nstage;
 };
 
-__global__ __launch_bounds__(kBlock) void k_rk4_adjoint(Rk4AdjArgs A) {
-    extern __shared__ double2 sh[];
+constexpr int kRkCarry = 16;
+
+// Makes generators L_a..L_b resident in the LDS window M (column stride
+// D + 1), restaging the window [first, last] when they are not.
+struct GenWindow {
+    long lo = 0, hi = -1;
+    __device__ __forceinline__ void need(double2* M, const double2* Ls, long a, long b, long first, long last, int D) {
+        if (a >= lo && b <= hi) return;
+        __syncthreads();
+        lo = first;
+        hi = last;
+        stage_mats(M, Ls, lo, static_cast<int>(hi - lo + 1), D);
+        __syncthreads();
+    }
+};
+
+template <int DT, bool STAGED>
+__device__ __forceinline__ void rk4_recompute_body(double2* sh, const Rk4AdjArgs& A, long sg) {
     const int D = A.D;
-    double2* x = sh;          // state being recomputed / r_n
+    double2* x = sh;
     double2* k1 = x + D;
     double2* k2 = k1 + D;
     double2* k3 = k2 + D;
     double2* k4 = k3 + D;
     double2* tmp = k4 + D;
+    double2* M = tmp + D;
+    const long D2 = static_cast<long>(D) * D;
+    const int ld = STAGED ? D + 1 : D;
+    GenWindow w;
+    const long s0 = sg * A.K, s1 = min(s0 + A.K, A.steps);
+    for (int e = threadIdx.x; e < D; e += blockDim.x) x[e] = A.ck[sg * D + e];
+    __syncthreads();
+    for (long n = s0; n < s1; ++n) {
+        double2* row = A.seg + n * 4 * D;
+        for (int e = threadIdx.x; e < D; e += blockDim.x) row[e] = x[e];
+        if constexpr (STAGED) {
+            w.need(M, A.Ls, n, n + 1, n, min(n + A.nstage - 1, s1), D);
+            rk4_step<DT>(M + (n - w.lo) * D * ld, M + (n + 1 - w.lo) * D * ld, ld, nullptr, x, k1, k2, k3, k4, tmp,
+                         D, A.dt, row + D);
+        } else {
+            rk4_step<DT>(A.Ls + n * D2, A.Ls + (n + 1) * D2, ld, nullptr, x, k1, k2, k3, k4, tmp, D, A.dt, row + D);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_rk4_recompute(Rk4AdjArgs A) {
+    extern __shared__ double2 sh[];
+    const long sg = blockIdx.x;
+    if (A.nstage >= 2) {
+#define TACO_RK4_CALL(DT) rk4_recompute_body<DT, true>(sh, A, sg)
+        TACO_RK4_DISPATCH(A.D, TACO_RK4_CALL)
+#undef TACO_RK4_CALL
+    } else {
+#define TACO_RK4_CALL(DT) rk4_recompute_body<DT, false>(sh, A, sg)
+        TACO_RK4_DISPATCH(A.D, TACO_RK4_CALL)
+#undef TACO_RK4_CALL
+    }
+}
+
+template <int DT, bool STAGED>
+__device__ __forceinline__ void rk4_adjoint_body(double2* sh, const Rk4AdjArgs& A) {
+    const int D = A.D;
+    double2* x = sh;          // r_n
+    double2* k3 = x + D;      // cotangent of k3
+    double2* k4 = k3 + D;     // cotangent of k4
+    double2* tmp = k4 + D;
     double2* lam = tmp + D;   // cotangent of r_{n+1}, then of r_n
     double2* u1 = lam + D;
     double2* u2 = u1 + D;
     double2* u3 = u2 + D;
-    double2* kb1 = u3 + D;    // cotangents of k1..k4 (kb4 = s lam is formed in place)
+    double2* kb1 = u3 + D;    // cotangents of k1, k2
     double2* kb2 = kb1 + D;
+    double2* rb = kb2 + D;    // cotangent of r_n being accumulated
+    double2* M = rb + D;      // staged generators
     const long D2 = static_cast<long>(D) * D;
+    const int ld = STAGED ? D + 1 : D;
+    const bool carry = D2 <= static_cast<long>(kRkCarry) * blockDim.x;
+    double2 Lc[kRkCarry];     // L0 part of the step after the current one (carry mode)
+#pragma unroll
+    for (int q = 0; q < kRkCarry; ++q) Lc[q] = make_double2(0.0, 0.0);
+    GenWindow w;
     const double h = 0.5 * A.dt, s6 = A.dt / 6.0;
     for (int e = threadIdx.x; e < D; e += blockDim.x) lam[e] = A.gbar[A.steps * D + e];
     __syncthreads();
-    const long nseg = (A.steps + A.K - 1) / A.K;
-    for (long sg = nseg - 1; sg >= 0; --sg) {
-        const long s0 = sg * A.K, s1 = min(s0 + A.K, A.steps);
-        // recompute r_{s0} .. r_{s1 - 1} from the checkpoint
-        for (int e = threadIdx.x; e < D; e += blockDim.x) x[e] = A.ck[sg * D + e];
-        __syncthreads();
-        for (long n = s0; n < s1; ++n) {
-            for (int e = threadIdx.x; e < D; e += blockDim.x) A.seg[(n - s0) * D + e] = x[e];
-            __syncthreads();
-            if (n + 1 < s1) rk4_step(A.Ls + n * D2, A.Ls + (n + 1) * D2, nullptr, x, k1, k2, k3, k4, tmp, D, A.dt);
+    for (long n = A.steps - 1; n >= 0; --n) {
+        const double2* L0;
+        const double2* L1;
+        if constexpr (STAGED) {
+            w.need(M, A.Ls, n, n + 1, max(0L, n + 2 - A.nstage), n + 1, D);
+            L0 = M + (n - w.lo) * D * ld;
+            L1 = L0 + D * ld;
+        } else {
+            L0 = A.Ls + n * D2;
+            L1 = L0 + D2;
         }
-        for (long n = s1 - 1; n >= s0; --n) {
-            const double2* L0 = A.Ls + n * D2;
-            const double2* L1 = A.Ls + (n + 1) * D2;
-            for (int e = threadIdx.x; e < D; e += blockDim.x) x[e] = A.seg[(n - s0) * D + e];
-            __syncthreads();
-            // stage vectors of the forward step
-            matvec_rows(L0, x, k1, D);
-            __syncthreads();
-            for (int e = threadIdx.x; e < D; e += blockDim.x) u1[e] = cadd(x[e], cscale(k1[e], h));
-            __syncthreads();
-            matvec_half(L0, L1, u1, k2, D);
-            __syncthreads();
-            for (int e = threadIdx.x; e < D; e += blockDim.x) u2[e] = cadd(x[e], cscale(k2[e], h));
-            __syncthreads();
-            matvec_half(L0, L1, u2, k3, D);
-            __syncthreads();
-            for (int e = threadIdx.x; e < D; e += blockDim.x) u3[e] = cadd(x[e], cscale(k3[e], A.dt));
-            __syncthreads();
-            // reverse: r' = r + s (k1 + 2 k2 + 2 k3 + k4)
-            for (int e = threadIdx.x; e < D; e += blockDim.x) k4[e] = cscale(lam[e], s6);      // kb4
-            __syncthreads();
-            matvec_h(L1, k4, tmp, D);                                                          // ub3
-            __syncthreads();
-            for (int e = threadIdx.x; e < D; e += blockDim.x) {
-                k3[e] = cadd(cscale(lam[e], 2.0 * s6), cscale(tmp[e], A.dt));                  // kb3
-                k1[e] = cadd(lam[e], tmp[e]);                                                   // rb so far
+        const double2* row = A.seg + n * 4 * D;
+        for (int e = threadIdx.x; e < D; e += blockDim.x) {
+            x[e] = row[e];
+            u1[e] = row[D + e];
+            u2[e] = row[2 * D + e];
+            u3[e] = row[3 * D + e];
+            k4[e] = cscale(lam[e], s6);                                                    // kb4
+        }
+        __syncthreads();
+        // reverse: r' = r + s (k1 + 2 k2 + 2 k3 + k4)
+        matvec_h<DT>(L1, ld, k4, tmp, D);                                                  // ub3
+        __syncthreads();
+        for (int e = threadIdx.x; e < D; e += blockDim.x) {
+            k3[e] = cadd(cscale(lam[e], 2.0 * s6), cscale(tmp[e], A.dt));                  // kb3
+            rb[e] = cadd(lam[e], tmp[e]);
+        }
+        __syncthreads();
+        matvec_h_half<DT>(L0, L1, ld, k3, tmp, D);                                         // ub2
+        __syncthreads();
+        for (int e = threadIdx.x; e < D; e += blockDim.x) {
+            kb2[e] = cadd(cscale(lam[e], 2.0 * s6), cscale(tmp[e], h));
+            rb[e] = cadd(rb[e], tmp[e]);
+        }
+        __syncthreads();
+        matvec_h_half<DT>(L0, L1, ld, kb2, tmp, D);                                        // ub1
+        __syncthreads();
+        for (int e = threadIdx.x; e < D; e += blockDim.x) {
+            kb1[e] = cadd(cscale(lam[e], s6), cscale(tmp[e], h));
+            rb[e] = cadd(rb[e], tmp[e]);
+        }
+        __syncthreads();
+        matvec_h<DT>(L0, ld, kb1, tmp, D);
+        __syncthreads();
+        // generator cotangents: Lbar_h = kb3 u2^H + kb2 u1^H, split half/half
+        double2* Lb0 = A.Lbar + n * D2;
+        double2* Lb1 = A.Lbar + (n + 1) * D2;
+        if (carry) {
+#pragma unroll
+            for (int q = 0; q < kRkCarry; ++q) {
+                const long e = threadIdx.x + static_cast<long>(q) * blockDim.x;
+                if (e < D2) {
+                    const int r = static_cast<int>(e % D), c = static_cast<int>(e / D);
+                    const double2 lh = cadd(cmul(k3[r], cconj(u2[c])), cmul(kb2[r], cconj(u1[c])));
+                    Lb1[e] = cadd(Lc[q], cadd(cmul(k4[r], cconj(u3[c])), cscale(lh, 0.5)));
+                    Lc[q] = cadd(cmul(kb1[r], cconj(x[c])), cscale(lh, 0.5));
+                }
             }
-            __syncthreads();
-            matvec_h_half(L0, L1, k3, tmp, D);                                                 // ub2
-            __syncthreads();
-            for (int e = threadIdx.x; e < D; e += blockDim.x) {
-                kb2[e] = cadd(cscale(lam[e], 2.0 * s6), cscale(tmp[e], h));
-                k1[e] = cadd(k1[e], tmp[e]);
-            }
-            __syncthreads();
-            matvec_h_half(L0, L1, kb2, tmp, D);                                                // ub1
-            __syncthreads();
-            for (int e = threadIdx.x; e < D; e += blockDim.x) {
-                kb1[e] = cadd(cscale(lam[e], s6), cscale(tmp[e], h));
-                k1[e] = cadd(k1[e], tmp[e]);
-            }
-            __syncthreads();
-            matvec_h(L0, kb1, tmp, D);
-            __syncthreads();
-            // generator cotangents: Lbar_h = kb3 u2^H + kb2 u1^H, split half/half
-            double2* Lb0 = A.Lbar + n * D2;
-            double2* Lb1 = A.Lbar + (n + 1) * D2;
+        } else {
             for (long e = threadIdx.x; e < D2; e += blockDim.x) {
                 const int r = static_cast<int>(e % D), c = static_cast<int>(e / D);
                 const double2 lh = cadd(cmul(k3[r], cconj(u2[c])), cmul(kb2[r], cconj(u1[c])));
                 Lb0[e] = cadd(Lb0[e], cadd(cmul(kb1[r], cconj(x[c])), cscale(lh, 0.5)));
                 Lb1[e] = cadd(Lb1[e], cadd(cmul(k4[r], cconj(u3[c])), cscale(lh, 0.5)));
             }
-            for (int e = threadIdx.x; e < D; e += blockDim.x) lam[e] = cadd(cadd(k1[e], tmp[e]), A.gbar[n * D + e]);
-            __syncthreads();
+        }
+        for (int e = threadIdx.x; e < D; e += blockDim.x) lam[e] = cadd(cadd(rb[e], tmp[e]), A.gbar[n * D + e]);
+        __syncthreads();
+    }
+    if (carry) {
+#pragma unroll
+        for (int q = 0; q < kRkCarry; ++q) {
+            const long e = threadIdx.x + static_cast<long>(q) * blockDim.x;
+            if (e < D2) A.Lbar[e] = Lc[q];
         }
     }
     for (int e = threadIdx.x; e < D; e += blockDim.x) A.r0bar[e] = lam[e];
+}
+
+__global__ __launch_bounds__(kBlock) void k_rk4_adjoint(Rk4AdjArgs A) {
+    extern __shared__ double2 sh[];
+    if (A.nstage >= 2) {
+#define TACO_RK4_CALL(DT) rk4_adjoint_body<DT, true>(sh, A)
+        TACO_RK4_DISPATCH(A.D, TACO_RK4_CALL)
+#undef TACO_RK4_CALL
+    } else {
+#define TACO_RK4_CALL(DT) rk4_adjoint_body<DT, false>(sh, A)
+        TACO_RK4_DISPATCH(A.D, TACO_RK4_CALL)
+#undef TACO_RK4_CALL
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -637,21 +637,36 @@ __global__ __launch_bounds__(kBlock) void k_l4(L4Args A) {
 // RK4 on vec(rho) with prebuilt generators (rk4_dense.cpp:133-160, 246-320),
 // one workgroup, D <= 1024.  Matvecs sum over j in order per row.
 // ---------------------------------------------------------------------------
-__device__ void matvec_rows(const double2* __restrict__ L, const double2* x, double2* y, int D) {
-    for (int r = threadIdx.x; r < D; r += blockDim.x) {
+// Matrices are read as L[r + c * ld]: ld = D in global memory, D + 1 when
+// staged in LDS (the padded column stride spreads the column reads of L^H
+// products over the banks).  Everything down to the kernels is inlined and
+// each code path sees either LDS or global matrices, so the compiler knows
+// the address space of every access (LDS reads, not flat ones).  Every product sums over c in order per row, so
+// staged and unstaged runs, and every instantiation below, give identical
+// values.  DT > 0 fixes D at compile time (the row loop is unrolled, so its
+// loads issue together); DT = 0 is the general case.
+template <int DT>
+__device__ __forceinline__ void matvec_rows(const double2* __restrict__ L, int ld, const double2* x, double2* y,
+                                            int D) {
+    const int n = DT ? DT : D;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
         double2 s = make_double2(0.0, 0.0);
-        for (int c = 0; c < D; ++c) s = cadd(s, cmul(L[r + static_cast<long>(c) * D], x[c]));
+#pragma unroll
+        for (int c = 0; c < (DT ? DT : n); ++c) s = cadd(s, cmul(L[r + static_cast<long>(c) * ld], x[c]));
         y[r] = s;
     }
 }
 
-// mode 0: Lh given; mode 1: Lh = 0.5*(L0 + L1) formed on the fly per element.
-__device__ void matvec_half(const double2* __restrict__ L0, const double2* __restrict__ L1, const double2* x,
-                            double2* y, int D) {
-    for (int r = threadIdx.x; r < D; r += blockDim.x) {
+// y = 0.5 (L0 + L1) x, the midpoint generator formed per element.
+template <int DT>
+__device__ __forceinline__ void matvec_half(const double2* __restrict__ L0, const double2* __restrict__ L1, int ld,
+                                            const double2* x, double2* y, int D) {
+    const int n = DT ? DT : D;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
         double2 s = make_double2(0.0, 0.0);
-        for (int c = 0; c < D; ++c) {
-            const long e = r + static_cast<long>(c) * D;
+#pragma unroll
+        for (int c = 0; c < (DT ? DT : n); ++c) {
+            const long e = r + static_cast<long>(c) * ld;
             const double2 h = cscale(cadd(L0[e], L1[e]), 0.5);
             s = cadd(s, cmul(h, x[c]));
         }
@@ -661,23 +676,36 @@ __device__ void matvec_half(const double2* __restrict__ L0, const double2* __res
 
 // One RK4 step x <- x + dt/6 (k1 + 2 k2 + 2 k3 + k4) by the whole workgroup
 // (rk4_dense.cpp:133-160); Lh null: midpoint 0.5 (L0 + L1) per element.
-__device__ void rk4_step(const double2* L0, const double2* L1, const double2* Lh, double2* x, double2* k1,
-                         double2* k2, double2* k3, double2* k4, double2* tmp, int D, double dt) {
+// L0 and L1 have column stride ld; Lh (global) has D.
+template <int DT>
+__device__ __forceinline__ void rk4_step(const double2* L0, const double2* L1, int ld, const double2* Lh, double2* x,
+                                         double2* k1, double2* k2, double2* k3, double2* k4, double2* tmp, int D,
+                                         double dt, double2* stash = nullptr) {
+    // stash (optional): receives the stage points u1, u2, u3 (3 x D)
     const double hdt = 0.5 * dt;
     const double s6 = dt / 6.0;
-    matvec_rows(L0, x, k1, D);
+    matvec_rows<DT>(L0, ld, x, k1, D);
     __syncthreads();
-    for (int e = threadIdx.x; e < D; e += blockDim.x) tmp[e] = cadd(x[e], cscale(k1[e], hdt));
+    for (int e = threadIdx.x; e < D; e += blockDim.x) {
+        tmp[e] = cadd(x[e], cscale(k1[e], hdt));
+        if (stash) stash[e] = tmp[e];
+    }
     __syncthreads();
-    if (Lh) matvec_rows(Lh, tmp, k2, D); else matvec_half(L0, L1, tmp, k2, D);
+    if (Lh) matvec_rows<DT>(Lh, D, tmp, k2, D); else matvec_half<DT>(L0, L1, ld, tmp, k2, D);
     __syncthreads();
-    for (int e = threadIdx.x; e < D; e += blockDim.x) tmp[e] = cadd(x[e], cscale(k2[e], hdt));
+    for (int e = threadIdx.x; e < D; e += blockDim.x) {
+        tmp[e] = cadd(x[e], cscale(k2[e], hdt));
+        if (stash) stash[D + e] = tmp[e];
+    }
     __syncthreads();
-    if (Lh) matvec_rows(Lh, tmp, k3, D); else matvec_half(L0, L1, tmp, k3, D);
+    if (Lh) matvec_rows<DT>(Lh, D, tmp, k3, D); else matvec_half<DT>(L0, L1, ld, tmp, k3, D);
     __syncthreads();
-    for (int e = threadIdx.x; e < D; e += blockDim.x) tmp[e] = cadd(x[e], cscale(k3[e], dt));
+    for (int e = threadIdx.x; e < D; e += blockDim.x) {
+        tmp[e] = cadd(x[e], cscale(k3[e], dt));
+        if (stash) stash[2 * D + e] = tmp[e];
+    }
     __syncthreads();
-    matvec_rows(L1, tmp, k4, D);
+    matvec_rows<DT>(L1, ld, tmp, k4, D);
     __syncthreads();
     for (int e = threadIdx.x; e < D; e += blockDim.x) {
         const double2 acc = cadd(cadd(cadd(k1[e], cscale(k2[e], 2.0)), cscale(k3[e], 2.0)), k4[e]);
@@ -686,26 +714,61 @@ __device__ void rk4_step(const double2* L0, const double2* L1, const double2* Lh
     __syncthreads();
 }
 
-// One system's RK4 run by the whole workgroup; x, k1..k4, tmp in LDS.  Reads
-// the initial vector from r_in and writes the final one to r_out.
-__device__ void rk4_body(double2* sh, const double2* Ls, const double2* Lh, long steps, int D, const double2* r_in,
-                         double2* r_out, double dt, long every, double2* samples, long* nsamples) {
+// Copies generators L_first .. L_first+count-1 into LDS with column stride D + 1.
+__device__ __forceinline__ void stage_mats(double2* dst, const double2* Ls, long first, int count, int D) {
+    const long D2 = static_cast<long>(D) * D;
+    const int ld = D + 1;
+    const double2* src = Ls + first * D2;
+    for (long e = threadIdx.x; e < count * D2; e += blockDim.x) {
+        const long m = e / D2, w = e % D2;
+        const int r = static_cast<int>(w % D), c = static_cast<int>(w / D);
+        dst[m * D * ld + r + static_cast<long>(c) * ld] = src[e];
+    }
+}
+
+// One system's RK4 run by the whole workgroup; x, k1..k4, tmp in LDS, then
+// (nstage >= 2, no Lh) the generators staged nstage at a time.  Reads the
+// initial vector from r_in and writes the final one to r_out.
+template <int DT>
+__device__ __forceinline__ void rk4_body(double2* sh, int nstage, const double2* Ls, const double2* Lh, long steps, int D,
+                         const double2* r_in, double2* r_out, double dt, long every, double2* samples,
+                         long* nsamples) {
     double2* x = sh;
     double2* k1 = x + D;
     double2* k2 = k1 + D;
     double2* k3 = k2 + D;
     double2* k4 = k3 + D;
     double2* tmp = k4 + D;
+    double2* M = tmp + D;
     for (int e = threadIdx.x; e < D; e += blockDim.x) x[e] = r_in[e];
     __syncthreads();
     const long D2 = static_cast<long>(D) * D;
+    const int ld = D + 1;
     long ns = 0;
-    for (long st = 0; st < steps; ++st) {
-        if (samples && st % every == 0) {
-            for (int e = threadIdx.x; e < D; e += blockDim.x) samples[ns * D + e] = x[e];
-            ++ns;
+    if (nstage >= 2 && !Lh) {
+        for (long s = 0; s < steps; s += nstage - 1) {
+            const int cnt = static_cast<int>(min(static_cast<long>(nstage), steps - s + 1));
+            stage_mats(M, Ls, s, cnt, D);
+            __syncthreads();
+            for (int q = 0; q + 1 < cnt; ++q) {
+                const long st = s + q;
+                if (samples && st % every == 0) {
+                    for (int e = threadIdx.x; e < D; e += blockDim.x) samples[ns * D + e] = x[e];
+                    ++ns;
+                }
+                rk4_step<DT>(M + static_cast<long>(q) * D * ld, M + static_cast<long>(q + 1) * D * ld, ld, nullptr,
+                             x, k1, k2, k3, k4, tmp, D, dt);
+            }
         }
-        rk4_step(Ls + st * D2, Ls + (st + 1) * D2, Lh ? Lh + st * D2 : nullptr, x, k1, k2, k3, k4, tmp, D, dt);
+    } else {
+        for (long st = 0; st < steps; ++st) {
+            if (samples && st % every == 0) {
+                for (int e = threadIdx.x; e < D; e += blockDim.x) samples[ns * D + e] = x[e];
+                ++ns;
+            }
+            rk4_step<DT>(Ls + st * D2, Ls + (st + 1) * D2, D, Lh ? Lh + st * D2 : nullptr, x, k1, k2, k3, k4, tmp,
+                         D, dt);
+        }
     }
     if (samples) {
         for (int e = threadIdx.x; e < D; e += blockDim.x) samples[ns * D + e] = x[e];
@@ -715,15 +778,31 @@ __device__ void rk4_body(double2* sh, const double2* Ls, const double2* Lh, long
     if (threadIdx.x == 0 && nsamples) *nsamples = ns;
 }
 
-__global__ void k_rk4(const double2* Ls, const double2* Lh, long steps, int D, double2* r, double dt, long every,
-                      double2* samples, long* nsamples) {
+// Dispatch on D = N^2 for the sizes of N = 2..8 (unrolled rows), else general.
+#define TACO_RK4_DISPATCH(D, CALL)      \
+    switch (D) {                        \
+        case 4: CALL(4); break;         \
+        case 9: CALL(9); break;         \
+        case 16: CALL(16); break;       \
+        case 25: CALL(25); break;       \
+        case 36: CALL(36); break;       \
+        case 49: CALL(49); break;       \
+        case 64: CALL(64); break;       \
+        default: CALL(0); break;        \
+    }
+
+__global__ __launch_bounds__(kBlock) void k_rk4(const double2* Ls, const double2* Lh, long steps, int D, double2* r,
+                                                double dt, long every, double2* samples, long* nsamples, int nstage) {
     extern __shared__ double2 sh[];
-    rk4_body(sh, Ls, Lh, steps, D, r, r, dt, every, samples, nsamples);
+#define TACO_RK4_CALL(DT) rk4_body<DT>(sh, nstage, Ls, Lh, steps, D, r, r, dt, every, samples, nsamples)
+    TACO_RK4_DISPATCH(D, TACO_RK4_CALL)
+#undef TACO_RK4_CALL
 }
 
 // Many independent systems, one workgroup each, in a single launch.
 struct Rk4Sys {
     const double2* Ls;   // [steps + 1][D x D]
+    int nstage;          // generators staged in LDS at a time (0: read from global)
     const double2* r0;   // [D]
     double2* r;          // [D] final vector
     double2* samples;    // [steps + 1][D]
@@ -732,10 +811,12 @@ struct Rk4Sys {
     int D;
 };
 
-__global__ void k_rk4_batch(const Rk4Sys* sys) {
+__global__ __launch_bounds__(kBlock) void k_rk4_batch(const Rk4Sys* sys) {
     extern __shared__ double2 sh[];
     const Rk4Sys S = sys[blockIdx.x];
-    rk4_body(sh, S.Ls, nullptr, S.steps, S.D, S.r0, S.r, S.dt, 1, S.samples, nullptr);
+#define TACO_RK4_CALL(DT) rk4_body<DT>(sh, S.nstage, S.Ls, nullptr, S.steps, S.D, S.r0, S.r, S.dt, 1, S.samples, nullptr)
+    TACO_RK4_DISPATCH(S.D, TACO_RK4_CALL)
+#undef TACO_RK4_CALL
 }
 
 // dst[q] = src[rows[q]] for rows of `len` complex values.
@@ -1526,6 +1607,7 @@ struct BatchRun {
     DevBuf rkargs;
     long nrk = 0;
     std::size_t rk_lds = 0;
+    int rk_block = 64;  // raised to rk4_block(D) of the systems
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool ran_eager = false;
@@ -1600,7 +1682,7 @@ static void batch_enqueue(taco_ctx* c, BatchRun& B) {
     for (auto& s : B.sys) batch_enqueue_system(c, *s);
     if (B.nrk) {
         timed(c, K_RK4, 0.0, [&] {
-            hipLaunchKernelGGL(k_rk4_batch, dim3(static_cast<unsigned>(B.nrk)), dim3(kBlock), B.rk_lds, c->stream,
+            hipLaunchKernelGGL(k_rk4_batch, dim3(static_cast<unsigned>(B.nrk)), dim3(B.rk_block), B.rk_lds, c->stream,
                                B.rkargs.as<Rk4Sys>());
         });
         HIP_OK(hipGetLastError());
@@ -1631,12 +1713,42 @@ struct taco_tape {
 
 namespace taco_engine {
 
+// LDS of the RK4 workgroups: `vecs` vectors of D, then generators staged
+// nstage at a time (column stride D + 1) within the 160 KiB of a CU.
+constexpr std::size_t kRkLds = 160 * 1024;
+
+static int rk4_nstage(int D, int vecs, long max_mats) {
+    const std::size_t per = static_cast<std::size_t>(D) * (D + 1) * 16;
+    const std::size_t base = static_cast<std::size_t>(vecs) * D * 16;
+    if (base + 2 * per > kRkLds) return 0;
+    long n = static_cast<long>((kRkLds - base) / per);
+    n = std::min(n, std::min(max_mats, 256L));
+    return n >= 2 ? static_cast<int>(n) : 0;
+}
+
+static std::size_t rk4_lds(int D, int vecs, int nstage) {
+    return (static_cast<std::size_t>(vecs) * D + static_cast<std::size_t>(nstage) * D * (D + 1)) * 16;
+}
+
+// Threads of an RK4 workgroup: TACO_RK4_BLOCK (64..256, multiple of 64),
+// else 256 (measured faster than one wave for D = 16..64).
+static int env_int(const char* name, int dflt);
+static int rk4_block(int D) {
+    (void)D;
+    const int b = env_int("TACO_RK4_BLOCK", 0);
+    if (b >= 64 && b <= kBlock && b % 64 == 0) return b;
+    return kBlock;
+}
+
 static void rk4_launch(taco_ctx* ctx, const double2* dLs, const double2* dLh, std::size_t steps, int D, double2* dr,
                        double dt, std::size_t every, double2* dsamples, long* dns) {
-    const std::size_t lds = 6 * static_cast<std::size_t>(D) * 16;
-    if (lds > 160 * 1024) throw Error(TACO_ERR_INVALID, "RK4 kernel supports D <= 1706");
-    hipLaunchKernelGGL(k_rk4, dim3(1), dim3(kBlock), lds, ctx->stream, dLs, dLh, static_cast<long>(steps), D, dr, dt,
-                       static_cast<long>(every), dsamples, dns);
+    const int nstage = dLh ? 0 : rk4_nstage(D, 6, static_cast<long>(steps) + 1);
+    const std::size_t lds = rk4_lds(D, 6, nstage);
+    if (lds > kRkLds) throw Error(TACO_ERR_INVALID, "RK4 kernel supports D <= 1706");
+    timed(ctx, K_RK4, 0.0, [&] {
+        hipLaunchKernelGGL(k_rk4, dim3(1), dim3(rk4_block(D)), lds, ctx->stream, dLs, dLh, static_cast<long>(steps), D,
+                           dr, dt, static_cast<long>(every), dsamples, dns, nstage);
+    });
     HIP_OK(hipGetLastError());
 }
 
@@ -1735,15 +1847,13 @@ static void tape_adjoint(taco_tape& tp, const double* samples_bar, double* gamma
     const std::size_t nf3 = static_cast<std::size_t>(tp.nf) * tp.nf * tp.nf;
     const std::size_t K = static_cast<std::size_t>(tp.K);
     // device layout in the context's rk buffer: gbar, Lbar, seg, r0bar
-    c->rk.ensure((T * D + T * D2 + (K + 1) * D + D) * 16);
+    const std::size_t steps = tp.steps;
+    c->rk.ensure((T * D + T * D2 + 4 * steps * D + D) * 16);
     double2* gb = c->rk.as<double2>();
     double2* Lb = gb + T * D;
     double2* seg = Lb + T * D2;
-    double2* rb = seg + (K + 1) * D;
+    double2* rb = seg + 4 * steps * D;
     HIP_OK(hipMemcpyAsync(gb, samples_bar, T * D * 16, hipMemcpyHostToDevice, c->stream));
-    const long cntL = static_cast<long>(T * D2);
-    hipLaunchKernelGGL(k_fill_zero, dim3(static_cast<unsigned>((cntL + kBlock - 1) / kBlock)), dim3(kBlock), 0,
-                       c->stream, Lb, cntL);
     Rk4AdjArgs R{};
     R.Ls = tp.lgen.as<double2>();
     R.ck = tp.ck.as<double2>();
@@ -1751,15 +1861,27 @@ static void tape_adjoint(taco_tape& tp, const double* samples_bar, double* gamma
     R.Lbar = Lb;
     R.seg = seg;
     R.r0bar = rb;
-    R.steps = static_cast<long>(tp.steps);
+    R.steps = static_cast<long>(steps);
     R.D = static_cast<int>(D);
     R.K = tp.K;
     R.dt = tp.dt;
-    const std::size_t rk_lds = 12 * D * 16;
-    if (rk_lds > 160 * 1024) throw Error(TACO_ERR_INVALID, "adjoint RK4 kernel supports D <= 853");
-    set_fn_lds(reinterpret_cast<const void*>(k_rk4_adjoint), rk_lds);
+    const bool carry = D2 <= static_cast<std::size_t>(kRkCarry) * kBlock;
+    if (!carry) {
+        const long cntL = static_cast<long>(T * D2);
+        hipLaunchKernelGGL(k_fill_zero, dim3(static_cast<unsigned>((cntL + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                           c->stream, Lb, cntL);
+    }
+    const std::size_t nseg = (steps + K - 1) / K;
+    Rk4AdjArgs Rf = R;
+    Rf.nstage = rk4_nstage(static_cast<int>(D), 6, static_cast<long>(K) + 1);
+    const std::size_t lds_f = rk4_lds(static_cast<int>(D), 6, Rf.nstage);
+    R.nstage = rk4_nstage(static_cast<int>(D), 12, static_cast<long>(steps) + 1);
+    const std::size_t lds_b = rk4_lds(static_cast<int>(D), 12, R.nstage);
+    if (lds_b > kRkLds || lds_f > kRkLds) throw Error(TACO_ERR_INVALID, "adjoint RK4 kernel supports D <= 853");
     timed(c, K_RK4, 0.0, [&] {
-        hipLaunchKernelGGL(k_rk4_adjoint, dim3(1), dim3(kBlock), rk_lds, c->stream, R);
+        // every segment's states and stage points from its checkpoint, all segments at once
+        hipLaunchKernelGGL(k_rk4_recompute, dim3(static_cast<unsigned>(nseg)), dim3(kBlock), lds_f, c->stream, Rf);
+        hipLaunchKernelGGL(k_rk4_adjoint, dim3(1), dim3(kBlock), lds_b, c->stream, R);
     });
     HIP_OK(hipGetLastError());
     std::vector<std::complex<double>> gbar_h(static_cast<std::size_t>(tp.nf) * T, {0.0, 0.0});
@@ -1917,6 +2039,10 @@ int taco_ctx_create(int device, taco_ctx** out) {
                                  hipMemcpyHostToDevice));
         HIP_OK(hipEventCreate(&c->ev0));
         HIP_OK(hipEventCreate(&c->ev1));
+        for (const void* fn : {reinterpret_cast<const void*>(k_rk4), reinterpret_cast<const void*>(k_rk4_batch),
+                               reinterpret_cast<const void*>(k_rk4_adjoint),
+                               reinterpret_cast<const void*>(k_rk4_recompute)})
+            HIP_OK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kRkLds)));
         *out = c.release();
     });
 }
@@ -2336,17 +2462,16 @@ int taco_batch_create(taco_ctx* ctx, const taco_batch_system* systems, int nsys,
                 r.steps = static_cast<long>(d.steps);
                 r.dt = d.dt;
                 r.D = static_cast<int>(N2);
+                r.nstage = rk4_nstage(r.D, 6, static_cast<long>(d.steps) + 1);
                 rk.push_back(r);
-                B.rk_lds = std::max(B.rk_lds, 6 * N2 * 16);
+                B.rk_lds = std::max(B.rk_lds, rk4_lds(r.D, 6, r.nstage));
+                B.rk_block = std::max(B.rk_block, rk4_block(r.D));
             }
             B.sys.push_back(std::move(S));
         }
         B.nrk = static_cast<long>(rk.size());
         if (B.nrk) {
-            if (B.rk_lds > 160 * 1024) throw Error(TACO_ERR_INVALID, "RK4 kernel supports D <= 1706");
-            if (B.rk_lds > 64 * 1024)
-                HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_rk4_batch),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(B.rk_lds)));
+            if (B.rk_lds > kRkLds) throw Error(TACO_ERR_INVALID, "RK4 kernel supports D <= 1706");
             B.rkargs.ensure(rk.size() * sizeof(Rk4Sys));
             HIP_OK(hipMemcpy(B.rkargs.p, rk.data(), rk.size() * sizeof(Rk4Sys), hipMemcpyHostToDevice));
         }
