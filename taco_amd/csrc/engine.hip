@@ -186,7 +186,7 @@ __device__ __forceinline__ void tile_pass(double2* buf, const TileLayout& T, int
 // All lg stages of the tile: passes of at most log2(values per thread)
 // stages, stage counts as even as possible.
 template <class C>
-__device__ void tile_fft(double2* buf, const TileLayout& T, const double2* __restrict__ twp, long gstride,
+__device__ __forceinline__ void tile_fft(double2* buf, const TileLayout& T, const double2* __restrict__ twp, long gstride,
                          bool conj) {
     const int lg = T.lg;
     if (lg <= 0) return;
@@ -239,6 +239,7 @@ struct FcrArgs {
     const unsigned char* job_mask;  // per sorted position: bit0 F needed, bit1 C needed, bit2 R needed
     long nfft_jobs;         // positions [0, nfft_jobs) need FFTs; the rest only R
     const int* cls_start;   // [ncls + 1] first sorted position of each Omega class
+    const int* bperm;       // phase B processing order: grid slot (per chunk) -> sorted position
     long job0;              // first sorted position of the FFT chunk
     int njobs;              // triples in the FFT chunk
     int cls0;               // first class touching the chunk
@@ -357,6 +358,7 @@ __global__ __launch_bounds__(Cfg::kFT) void k_phase_b(FcrArgs P, int mode) {
     const int G = Tl.G, gs = Tl.gs;
     int job, gb;
     xcd_map(blockIdx.x, ngroups / G, job, gb);
+    if (mode == 0) job = P.bperm[P.job0 + job] - static_cast<int>(P.job0);
     const int k1b = gb * G;
     const int total = G * gs;
     const double2* in = P.inter1 + static_cast<long>(job) * P.n + static_cast<long>(k1b) * gs;
@@ -401,35 +403,68 @@ __global__ __launch_bounds__(Cfg::kFT) void k_phase_b(FcrArgs P, int mode) {
 }
 
 // Phase C: inverse stages b+1..L; writes conv[path][job][t] = dt * (y[t] / n), t < T.
+// Each workgroup takes tiles blockIdx.x and blockIdx.x + gridDim.x (same XCD
+// when the grid is a multiple of 8); the second tile's input is loaded into
+// registers while the first is transformed.
 template <class Cfg>
-__global__ __launch_bounds__(Cfg::kFT) void k_phase_c(FcrArgs P) {
+__global__ __launch_bounds__(Cfg::kFT) void k_phase_c(FcrArgs P, int ntiles) {
     constexpr int kFT = Cfg::kFT;
+    constexpr int kPT = Cfg::kPT;
     __shared__ double2 buf[Cfg::kLds];
     const int ngroups = 1 << P.b;
     const TileLayout Tl(P.a, tile_groups<Cfg>(P.a, ngroups));
     const int G = Tl.G, gs = Tl.gs;
-    int jp, gb;  // jp = path * njobs + job
-    xcd_map(blockIdx.x, ngroups / G, jp, gb);
-    const int q0 = gb * G;
-    if (q0 >= P.T) return;
-    if (!(P.job_mask[P.job0 + jp % P.njobs] & (1 << (jp / P.njobs)))) return;
     const int total = G * gs;
-    const double2* in = P.inter2 + static_cast<long>(jp) * P.n + static_cast<long>(q0) * gs;
-    for (int e = threadIdx.x; e < total; e += kFT) buf[Tl.at(e >> Tl.lg, e & (gs - 1))] = in[e];
-    __syncthreads();
     const long cstride = (1L << P.a) - 1;
-    tile_fft<Cfg>(buf, Tl, P.twC + static_cast<long>(q0) * cstride, cstride, true);
     const double inv_n = 1.0 / static_cast<double>(P.n);
-    const int path = jp / P.njobs, job = jp % P.njobs;
-    double2* out = P.conv + (path * P.cconv + (P.job0 + job - P.comb0)) * P.T;
-    for (int e = threadIdx.x; e < total; e += kFT) {
-        const int grp = e % G, v = e / G;
-        const long t = static_cast<long>(q0 + grp) + (static_cast<long>(v) << P.b);
-        if (t < P.T) {
-            const double2 y = cscale(buf[Tl.at(grp, v)], inv_n);
-            out[t] = make_double2(P.dt * y.x, P.dt * y.y);
+    int jpA = 0, jpB = 0, q0A = 0, q0B = 0;
+    bool liveA = false, liveB = false;
+    {
+        const int bA = blockIdx.x, bB = blockIdx.x + gridDim.x;
+        int gb;
+        if (bA < ntiles) {
+            xcd_map(bA, ngroups / G, jpA, gb);  // jp = path * njobs + job
+            q0A = gb * G;
+            liveA = q0A < P.T && (P.job_mask[P.job0 + jpA % P.njobs] & (1 << (jpA / P.njobs)));
+        }
+        if (bB < ntiles) {
+            xcd_map(bB, ngroups / G, jpB, gb);
+            q0B = gb * G;
+            liveB = q0B < P.T && (P.job_mask[P.job0 + jpB % P.njobs] & (1 << (jpB / P.njobs)));
         }
     }
+    double2 vA[kPT], vB[kPT];
+#pragma unroll
+    for (int q = 0; q < kPT; ++q) {
+        const int e = threadIdx.x + q * kFT;
+        vA[q] = (liveA && e < total) ? P.inter2[static_cast<long>(jpA) * P.n + static_cast<long>(q0A) * gs + e]
+                                     : make_double2(0.0, 0.0);
+        vB[q] = (liveB && e < total) ? P.inter2[static_cast<long>(jpB) * P.n + static_cast<long>(q0B) * gs + e]
+                                     : make_double2(0.0, 0.0);
+    }
+#define TACO_PHASE_C_TILE(JP, Q0, V)                                                                         \
+    {                                                                                                         \
+        _Pragma("unroll") for (int q = 0; q < kPT; ++q) {                                                     \
+            const int e = threadIdx.x + q * kFT;                                                              \
+            if (e < total) buf[Tl.at(e >> Tl.lg, e & (gs - 1))] = V[q];                                       \
+        }                                                                                                     \
+        __syncthreads();                                                                                      \
+        tile_fft<Cfg>(buf, Tl, P.twC + static_cast<long>(Q0) * cstride, cstride, true);                       \
+        const int path = (JP) / P.njobs, job = (JP) % P.njobs;                                                \
+        double2* out = P.conv + (path * P.cconv + (P.job0 + job - P.comb0)) * P.T;                            \
+        for (int e = threadIdx.x; e < total; e += kFT) {                                                      \
+            const int grp = e % G, v2 = e / G;                                                                \
+            const long tt = static_cast<long>((Q0) + grp) + (static_cast<long>(v2) << P.b);                   \
+            if (tt < P.T) {                                                                                   \
+                const double2 y = cscale(buf[Tl.at(grp, v2)], inv_n);                                         \
+                out[tt] = make_double2(P.dt * y.x, P.dt * y.y);                                               \
+            }                                                                                                 \
+        }                                                                                                     \
+        __syncthreads();                                                                                      \
+    }
+    if (liveA) TACO_PHASE_C_TILE(jpA, q0A, vA)
+    if (liveB) TACO_PHASE_C_TILE(jpB, q0B, vB)
+#undef TACO_PHASE_C_TILE
 }
 
 // Inclusive block scan (256 threads = 4 waves of 64) of one double.
@@ -1110,8 +1145,10 @@ static void launch_phase_cfg(Phase ph, const FcrArgs& P, long njobs, hipStream_t
             break;
         }
         case PH_C: {
-            const long blocks = 2 * njobs * (ngA / tile_groups<Cfg>(P.a, ngA));
-            hipLaunchKernelGGL(k_phase_c<Cfg>, dim3(static_cast<unsigned>(blocks)), dim3(Cfg::kFT), 0, st, P);
+            const long tiles = 2 * njobs * (ngA / tile_groups<Cfg>(P.a, ngA));
+            const long blocks = (tiles + 1) / 2;
+            hipLaunchKernelGGL(k_phase_c<Cfg>, dim3(static_cast<unsigned>(blocks)), dim3(Cfg::kFT), 0, st, P,
+                               static_cast<int>(tiles));
             break;
         }
     }
@@ -1292,25 +1329,47 @@ static FcrJob fcr_prepare(taco_ctx* c, const FcrPlan& pl, const FcrLists& lists)
     mask.resize(order.size());
     for (std::size_t q = 0; q < order.size(); ++q)
         mask[q] = pl.need ? (*pl.need)[static_cast<std::size_t>(order[q])] : 7;
+    // Class starts; a class longer than TACO_CLASS_MAX triples (default 4) is
+    // cut into pieces, each recomputing the same phases, so no phase-A
+    // workgroup loops much longer than the others.
+    const int cmax = std::max(1, env_int("TACO_CLASS_MAX", 4));
     std::vector<int> cls_start;
-    for (long p = 0; p < nfft_jobs; ++p)
-        if (p == 0 || key[order[p]] != key[order[p - 1]]) cls_start.push_back(static_cast<int>(p));
+    for (long p = 0, run = 0; p < nfft_jobs; ++p, ++run) {
+        if (p == 0 || key[order[p]] != key[order[p - 1]]) run = 0;
+        if (run % cmax == 0) cls_start.push_back(static_cast<int>(p));
+    }
     cls_start.push_back(static_cast<int>(nfft_jobs));
-    lists.jobtri->ensure(std::max<std::size_t>(order.size(), 1) * sizeof(int));
+    // Phase B visits the FFT jobs of each chunk grouped by j, so the triples
+    // running at the same time read the same two spectra (of g_mirror(j) and
+    // conj(g_j)) and share them through L2.  The chunk length is fixed here
+    // (chunks start at multiples of it).
+    const std::size_t per_job_b = 3 * n * 16;
+    long chunk_len = static_cast<long>(std::max<std::size_t>(1, chunk_budget_bytes() / per_job_b));
+    chunk_len = std::max(1L, std::min(chunk_len, nfft_jobs));
+    std::vector<int> bperm(static_cast<std::size_t>(nfft_jobs));
+    for (long c0 = 0; c0 < nfft_jobs; c0 += chunk_len) {
+        const long c1 = std::min(nfft_jobs, c0 + chunk_len);
+        std::iota(bperm.begin() + c0, bperm.begin() + c1, static_cast<int>(c0));
+        std::stable_sort(bperm.begin() + c0, bperm.begin() + c1, [&](int x, int y) {
+            return (order[x] / nf) % nf < (order[y] / nf) % nf;
+        });
+    }
+    lists.jobtri->ensure(std::max<std::size_t>(order.size() + bperm.size(), 1) * sizeof(int));
     lists.jobmask->ensure(std::max<std::size_t>(mask.size(), 1));
     lists.clsstart->ensure(cls_start.size() * sizeof(int));
     if (!order.empty()) {
         HIP_OK(hipMemcpyAsync(lists.jobtri->p, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice,
                               c->stream));
         HIP_OK(hipMemcpyAsync(lists.jobmask->p, mask.data(), mask.size(), hipMemcpyHostToDevice, c->stream));
+        if (!bperm.empty())
+            HIP_OK(hipMemcpyAsync(lists.jobtri->as<int>() + order.size(), bperm.data(), bperm.size() * sizeof(int),
+                                  hipMemcpyHostToDevice, c->stream));
     }
     HIP_OK(hipMemcpyAsync(lists.clsstart->p, cls_start.data(), cls_start.size() * sizeof(int),
                           hipMemcpyHostToDevice, c->stream));
 
     // FFT chunk (intermediates of nj triples) and combine batch (conv slots).
-    const std::size_t per_job = 3 * n * 16;
-    long chunk = static_cast<long>(std::max<std::size_t>(1, chunk_budget_bytes() / per_job));
-    chunk = std::max(1L, std::min(chunk, nfft_jobs));
+    const long chunk = chunk_len;
     const std::size_t Tpad = std::max<std::size_t>(pl.T, 1);
     long cconv = static_cast<long>(env_bytes("TACO_CONV_BYTES", std::size_t{4} << 30) / (2 * Tpad * 16));
     cconv = std::max(chunk, (cconv / chunk) * chunk);
@@ -1336,6 +1395,7 @@ static FcrJob fcr_prepare(taco_ctx* c, const FcrPlan& pl, const FcrLists& lists)
     P.twB = tw.b_.as<double2>();
     P.twC = tw.c_.as<double2>();
     P.job_tri = lists.jobtri->as<int>();
+    P.bperm = P.job_tri + order.size();
     P.job_mask = lists.jobmask->as<unsigned char>();
     P.nfft_jobs = nfft_jobs;
     P.cls_start = lists.clsstart->as<int>();
