@@ -253,3 +253,29 @@ def test_fft_tile_configs_bitwise_identical(orc, cfg):
     assert np.array_equal(got, ref)
     oref = orc.l4_series(G, S.omegas, S.mirror, S.pair_to_freq, S.A_eig, w.dt, w.tidx)
     assert max_rel_err(got, oref)[0] <= TOL
+
+
+@pytest.mark.parametrize("case", ["gue_pair", "d8"])
+def test_l4_factorised_matches_oracle(orc, case, monkeypatch):
+    """The factorised L4 (O(N^5 + N^3 nf) per time; the kernel for N > 8)
+    forced on systems the LDS kernel also handles: same values to rounding."""
+    if case == "gue_pair":
+        w = W.gue_pair(N_bath=3000, tmax=300)
+        tidx = w.tidx
+    else:
+        rng = W.SplitMix64(77)
+        w = W.Workload("d8", W.gue(8, rng), [W.random_hermitian_unit(8, rng)], taco_amd.DrudeLorentz(0.1, 5.0),
+                       1.0, 0.01, 299, [0, 1, 7, 50, 299])
+        tidx = w.tidx
+    monkeypatch.setenv("TACO_L4_FACTORISED", "1")
+    l4_check(orc, w, tidx)
+
+
+@pytest.mark.parametrize("h", [0.7, 1.0])
+def test_l4_tfim_d16_matches_oracle(orc, h):
+    """d = 16 (4-spin transverse-field Ising chain, nf = 81 at generic h, 57
+    at h = 1): the factorised kernel against the oracle's O(N^6) NAKZWAN sum."""
+    w = W.tfim_workload(h, Nt=64, tmax=19)
+    S = w.system
+    assert S.N == 16 and S.nf == (57 if h == 1.0 else 81)
+    l4_check(orc, w, [0, 5, 19])
