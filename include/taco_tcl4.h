@@ -75,8 +75,8 @@ int taco_profile_get(taco_ctx* ctx, int kernel, double* total_ms, long* launches
 
 /* Tile configuration the autotuner chose for FFT length 2^log2_nfft in this
  * context, per phase A, B, C (0: 8192 values/512 threads, 1: 4096/256,
- * 2: 4096/512, 3: 2048/256; -1 if not tuned yet).  TACO_FFT_TILE=<0..3>
- * forces one.  No reference counterpart (the reference's cuFFT plans are
+ * 2: 4096/512, 3: 2048/256, 4: 4096/1024, 5: 2048/512; -1 if not tuned
+ * yet).  TACO_FFT_TILE=<0..5> forces one.  No reference counterpart (the reference's cuFFT plans are
  * internal, tcl4_fcr_kernels.cu:602-622). */
 int taco_fft_tile_choice(taco_ctx* ctx, int log2_nfft, int* cfg3);
 

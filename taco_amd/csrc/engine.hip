@@ -101,7 +101,9 @@ using Cfg8k = FftCfg<8192, 512>;
 using Cfg4k = FftCfg<4096, 256>;
 using Cfg4kW = FftCfg<4096, 512>;
 using Cfg2k = FftCfg<2048, 256>;
-constexpr int kNumFftCfg = 4;
+using Cfg4kX = FftCfg<4096, 1024>;
+using Cfg2kW = FftCfg<2048, 512>;
+constexpr int kNumFftCfg = 6;
 
 // Physical LDS slot of (group, position): one pad slot per 16 values and an
 // odd group stride keep strided accesses off a single bank.
@@ -190,7 +192,7 @@ __device__ __forceinline__ void tile_fft(double2* buf, const TileLayout& T, cons
                          bool conj) {
     const int lg = T.lg;
     if (lg <= 0) return;
-    constexpr int maxr = C::kPT >= 16 ? 4 : 3;
+    constexpr int maxr = C::kPT >= 16 ? 4 : (C::kPT >= 8 ? 3 : 2);
     const int npass = (lg + maxr - 1) / maxr;
     const int base = lg / npass, extra = lg % npass;
     int s_lo = 1;
@@ -199,7 +201,9 @@ __device__ __forceinline__ void tile_fft(double2* buf, const TileLayout& T, cons
         switch (RL) {
             case 1: tile_pass<C, 1>(buf, T, s_lo, twp, gstride, conj); break;
             case 2: tile_pass<C, 2>(buf, T, s_lo, twp, gstride, conj); break;
-            case 3: tile_pass<C, 3>(buf, T, s_lo, twp, gstride, conj); break;
+            case 3:
+                if constexpr (C::kPT >= 8) tile_pass<C, 3>(buf, T, s_lo, twp, gstride, conj);
+                break;
             default:
                 if constexpr (C::kPT >= 16) tile_pass<C, 4>(buf, T, s_lo, twp, gstride, conj);
                 break;
@@ -1312,7 +1316,9 @@ static void launch_phase(int cfg, Phase ph, const FcrArgs& P, long njobs, hipStr
         case 0: launch_phase_cfg<Cfg8k>(ph, P, njobs, st); break;
         case 1: launch_phase_cfg<Cfg4k>(ph, P, njobs, st); break;
         case 2: launch_phase_cfg<Cfg4kW>(ph, P, njobs, st); break;
-        default: launch_phase_cfg<Cfg2k>(ph, P, njobs, st); break;
+        case 3: launch_phase_cfg<Cfg2k>(ph, P, njobs, st); break;
+        case 4: launch_phase_cfg<Cfg4kX>(ph, P, njobs, st); break;
+        default: launch_phase_cfg<Cfg2kW>(ph, P, njobs, st); break;
     }
 }
 
