@@ -110,6 +110,17 @@ __device__ __forceinline__ void rk4_recompute_body(double2* sh, const Rk4AdjArgs
     }
 }
 
+// Row-group recompute (rk4_rows with the stage points stashed) for the sizes
+// that have one: the same arithmetic as the forward run, so the states equal
+// the forward's bit for bit.  One workgroup per checkpoint segment.
+template <int DT>
+__global__ __launch_bounds__(64 * RowGroup<DT>::W) void k_rk4_recompute_rows(Rk4AdjArgs A) {
+    extern __shared__ double2 sh[];
+    const long sg = blockIdx.x;
+    const long s0 = sg * A.K, s1 = min(s0 + A.K, A.steps);
+    rk4_rows<DT, true>(sh, A.Ls, s0, s1, A.ck + sg * DT, nullptr, A.dt, 1, nullptr, nullptr, A.seg);
+}
+
 __global__ __launch_bounds__(kBlock) void k_rk4_recompute(Rk4AdjArgs A) {
     extern __shared__ double2 sh[];
     const long sg = blockIdx.x;
@@ -226,6 +237,13 @@ __device__ __forceinline__ void rk4_adjoint_body(double2* sh, const Rk4AdjArgs& 
         }
     }
     for (int e = threadIdx.x; e < D; e += blockDim.x) A.r0bar[e] = lam[e];
+}
+
+// Row-group reverse walk for D <= 16 (one wave of 64 threads).
+template <int DT>
+__global__ __launch_bounds__(64) void k_rk4_adjoint_rows(Rk4AdjArgs A) {
+    extern __shared__ double2 sh[];
+    rk4_rows_adjoint<DT>(sh, A);
 }
 
 __global__ __launch_bounds__(kBlock) void k_rk4_adjoint(Rk4AdjArgs A) {
@@ -393,6 +411,269 @@ __global__ __launch_bounds__(kBlock) void k_l4_adjoint(L4AdjArgs A) {
     for (int e = threadIdx.x; e < A.num_ops * N2; e += kBlock) A.Abar[t * A.num_ops * N2 + e] = Ab[e];
     if (A.lds_slices)
         for (long e = threadIdx.x; e < 3 * A.nf3; e += kBlock) A.sbar[t * 3 * A.nf3 + e] = sb[e];
+}
+
+// ---------------------------------------------------------------------------
+// Factorised L4 adjoint: the reverse of k_l4f, O(N^5 + N^3 nf) per time
+// instead of the O(N^6) term walk above.  Per operator pair (A, B) the
+// forward builds
+//   Q(a,i,g)  = sum_b A(a,b) B(b,i) X(f(b,a),g,f(a,i)) - B(a,b) A(b,i) X(f(i,b),g,f(a,i)),
+//   S123(n,i) = sum_{a,b} -B(n,a)A(a,b)B(b,i) M + A(n,a)B(a,b)B(b,i) I - B(n,a)B(a,b)A(b,i) K,
+//   S67(n,i,g) = sum_a A(n,a) Q(a,i,g),  D(n,i) = -sum_a A(n,a) S123(a,i),
+//   T(n,i;m,j) = A(j,m) S123(n,i) + B(j,m) S67(n,i,f(j,m))
+//                - sum_b B(j,b) A(b,m) Q(n,i,f(j,b)) + d_jm D(n,i),
+// with X = C + R, M = F(f(b,a),f(b,i),f(n,b)) - R(f(b,a),f(n,a),f(a,i)),
+// I = F(f(a,n),f(b,i),f(n,b)), K = R(f(i,b),f(n,a),f(a,i)) (the E
+// intermediate of k_l4f equals -S123, and its Wd equals Q).  The reverse
+// walks these back: Tbar -> (S123bar, S67bar, Dbar, U = A^H-contracted
+// Tbar) -> Qbar, S123bar -> slice and operator cotangents.  Slice cotangents
+// accumulate with atomic adds (LDS when 3 nf^3 fits, else global); operator
+// cotangents with LDS atomic adds of per-thread partial sums.  Each block
+// walks times blockIdx.x, blockIdx.x + gridDim.x, ... with its own scratch.
+// ---------------------------------------------------------------------------
+struct L4fAdjArgs {
+    const double2* Lbar;  // [T][N^4]
+    const double2* Fs;    // [T][nf3] forward slices
+    const double2* Cs;
+    const double2* Rs;
+    const int* p2f;
+    const double2* ops;
+    int N, nf, num_ops;
+    long nf3;
+    long T;
+    double2* sbar;        // [T][3][nf3]; zero on entry unless lds_slices
+    double2* Abar;        // [T][num_ops][N^2]
+    int lds_slices;
+    double2* scr;         // [gridDim.x][scr_per]
+    long scr_per;         // 4 N^2 nf + 3 N^2 + N^4
+};
+
+// LDS_SLICES: slice cotangents accumulate in LDS (else in global memory);
+// fixed per instantiation so every access has a known address space.
+template <bool LDS_SLICES>
+__device__ __forceinline__ void l4f_adjoint_body(double2* sm, const L4fAdjArgs& A) {
+    const int N = A.N, N2 = N * N, N4 = N2 * N2, nf = A.nf;
+    const long N2nf = static_cast<long>(N2) * nf;
+    double2* Tb = sm;                                   // N^4
+    double2* opsS = Tb + N4;                            // num_ops N^2
+    double2* Ab = opsS + A.num_ops * N2;                // num_ops N^2
+    double2* sb = Ab + A.num_ops * N2;                  // 3 nf3 (LDS_SLICES)
+    int* f = reinterpret_cast<int*>(sb + (LDS_SLICES ? 3 * A.nf3 : 0));
+    for (int e = threadIdx.x; e < A.num_ops * N2; e += kBlock) opsS[e] = A.ops[e];
+    for (int e = threadIdx.x; e < N2; e += kBlock) f[e] = A.p2f[e];
+    double2* Q = A.scr + static_cast<long>(blockIdx.x) * A.scr_per;  // Q(a,i,g) at a + i N + g N^2
+    double2* S67 = Q + N2nf;
+    double2* S67b = S67 + N2nf;
+    double2* Qb = S67b + N2nf;
+    double2* S123 = Qb + N2nf;                          // N^2
+    double2* S123b = S123 + N2;
+    double2* Db = S123b + N2;
+    double2* U = Db + N2;                               // U(n,i,b,j) at (n + i N) + (b + j N) N^2
+    const long nfl = nf;
+    auto id3 = [nfl](int x, int y, int z) { return (static_cast<long>(x) * nfl + y) * nfl + z; };
+    auto F2 = [f, N](int r, int c) { return f[r + c * N]; };
+    for (long t = blockIdx.x; t < A.T; t += gridDim.x) {
+        __syncthreads();
+        const double2* Lb = A.Lbar + t * N4;
+        for (int idx = threadIdx.x; idx < N4; idx += kBlock) {
+            const int ru = idx % N2, cu = idx / N2;
+            const int n = ru % N, i = ru / N, m = cu % N, j = cu / N;
+            Tb[idx] = cadd(Lb[(n + m * N) + (i + j * N) * N2], cconj(Lb[(m + n * N) + (j + i * N) * N2]));
+        }
+        for (int e = threadIdx.x; e < A.num_ops * N2; e += kBlock) Ab[e] = make_double2(0.0, 0.0);
+        if constexpr (LDS_SLICES)
+            for (long e = threadIdx.x; e < 3 * A.nf3; e += kBlock) sb[e] = make_double2(0.0, 0.0);
+        __syncthreads();
+        const double2* Fp = A.Fs + t * A.nf3;
+        const double2* Cp = A.Cs + t * A.nf3;
+        const double2* Rp = A.Rs + t * A.nf3;
+        double2* sF = LDS_SLICES ? sb : A.sbar + t * 3 * A.nf3;
+        double2* sC = sF + A.nf3;
+        double2* sR = sC + A.nf3;
+        for (int iA = 0; iA < A.num_ops; ++iA)
+            for (int iB = 0; iB < A.num_ops; ++iB) {
+                const double2* Ap = opsS + iA * N2;
+                const double2* Bp = opsS + iB * N2;
+                double2* AbA = Ab + iA * N2;
+                double2* AbB = Ab + iB * N2;
+#define AA(r, c) Ap[(r) + (c) * N]
+#define BB(r, c) Bp[(r) + (c) * N]
+#define TB(n, i, m, j) Tb[((n) + (i) * N) + ((m) + (j) * N) * N2]
+                // 1: forward Q and S123
+                for (long e = threadIdx.x; e < N2nf; e += kBlock) {
+                    const int a = static_cast<int>(e % N), i = static_cast<int>((e / N) % N);
+                    const int g = static_cast<int>(e / N2);
+                    double2 y = make_double2(0.0, 0.0);
+                    for (int b = 0; b < N; ++b) {
+                        const long x1 = id3(F2(b, a), g, F2(a, i)), x2 = id3(F2(i, b), g, F2(a, i));
+                        y = cadd(y, cmul(cmul(AA(a, b), BB(b, i)), cadd(Cp[x1], Rp[x1])));
+                        y = csub(y, cmul(cmul(BB(a, b), AA(b, i)), cadd(Cp[x2], Rp[x2])));
+                    }
+                    Q[e] = y;
+                }
+                for (int e = threadIdx.x; e < N2; e += kBlock) {
+                    const int n = e % N, i = e / N;
+                    double2 s = make_double2(0.0, 0.0);
+                    for (int a = 0; a < N; ++a)
+                        for (int b = 0; b < N; ++b) {
+                            const double2 Mv = csub(Fp[id3(F2(b, a), F2(b, i), F2(n, b))],
+                                                    Rp[id3(F2(b, a), F2(n, a), F2(a, i))]);
+                            const double2 Iv = Fp[id3(F2(a, n), F2(b, i), F2(n, b))];
+                            const double2 Kv = Rp[id3(F2(i, b), F2(n, a), F2(a, i))];
+                            s = csub(s, cmul(cmul(cmul(BB(n, a), AA(a, b)), BB(b, i)), Mv));
+                            s = cadd(s, cmul(cmul(cmul(AA(n, a), BB(a, b)), BB(b, i)), Iv));
+                            s = csub(s, cmul(cmul(cmul(BB(n, a), BB(a, b)), AA(b, i)), Kv));
+                        }
+                    S123[e] = s;
+                }
+                __syncthreads();
+                // 2: S67; S123bar, Dbar, S67bar, U from Tbar; A-bar of the A(j,m) S123 term
+                for (long e = threadIdx.x; e < N2nf; e += kBlock) {
+                    const int n = static_cast<int>(e % N);
+                    const long rest = e - n;
+                    double2 s = make_double2(0.0, 0.0);
+                    for (int a = 0; a < N; ++a) s = cadd(s, cmul(AA(n, a), Q[rest + a]));
+                    S67[e] = s;
+                    const int ni = static_cast<int>(e % N2), g = static_cast<int>(e / N2);
+                    double2 sb7 = make_double2(0.0, 0.0);
+                    for (int j = 0; j < N; ++j)
+                        for (int m = 0; m < N; ++m)
+                            if (F2(j, m) == g) sb7 = cadd(sb7, cmul(cconj(BB(j, m)), Tb[ni + (m + j * N) * N2]));
+                    S67b[e] = sb7;
+                }
+                for (int e = threadIdx.x; e < N2; e += kBlock) {
+                    double2 s = make_double2(0.0, 0.0), d = s;
+                    for (int j = 0; j < N; ++j) {
+                        for (int m = 0; m < N; ++m) s = cadd(s, cmul(cconj(AA(j, m)), Tb[e + (m + j * N) * N2]));
+                        d = cadd(d, Tb[e + (j + j * N) * N2]);
+                    }
+                    S123b[e] = s;
+                    Db[e] = d;
+                }
+                for (int e = threadIdx.x; e < N4; e += kBlock) {
+                    const int r = e % N2, c = e / N2, b = c % N, j = c / N;
+                    double2 s = make_double2(0.0, 0.0);
+                    for (int m = 0; m < N; ++m) s = cadd(s, cmul(cconj(AA(b, m)), Tb[r + (m + j * N) * N2]));
+                    U[e] = s;
+                }
+                for (int e = threadIdx.x; e < N2; e += kBlock) {
+                    const int j = e % N, m = e / N;
+                    double2 s = make_double2(0.0, 0.0);
+                    for (int r = 0; r < N2; ++r) s = cadd(s, cmul(cconj(S123[r]), Tb[r + (m + j * N) * N2]));
+                    atomic_cadd(AbA + j + m * N, s);
+                }
+                __syncthreads();
+                // 3: Qbar; S123bar gets the D term; operator cotangents of the S67,
+                // Q and D terms of T and of S67 = A Q and D = -A S123
+                for (long e = threadIdx.x; e < N2nf; e += kBlock) {
+                    const int a = static_cast<int>(e % N), i = static_cast<int>((e / N) % N);
+                    const int g = static_cast<int>(e / N2);
+                    double2 s = make_double2(0.0, 0.0);
+                    for (int n = 0; n < N; ++n) s = cadd(s, cmul(cconj(AA(n, a)), S67b[n + i * N + g * N2]));
+                    for (int j = 0; j < N; ++j)
+                        for (int b = 0; b < N; ++b)
+                            if (F2(j, b) == g) s = csub(s, cmul(cconj(BB(j, b)), U[(a + i * N) + (b + j * N) * N2]));
+                    Qb[e] = s;
+                }
+                for (int e = threadIdx.x; e < N2; e += kBlock) {
+                    const int a = e % N, i = e / N;
+                    double2 s = S123b[e];
+                    for (int n = 0; n < N; ++n) s = csub(s, cmul(cconj(AA(n, a)), Db[n + i * N]));
+                    S123b[e] = s;
+                }
+                for (int e = threadIdx.x; e < N2; e += kBlock) {
+                    const int j = e % N, m = e / N;
+                    const long go = static_cast<long>(F2(j, m)) * N2;
+                    double2 s = make_double2(0.0, 0.0);
+                    for (int r = 0; r < N2; ++r) s = cadd(s, cmul(cconj(S67[go + r]), Tb[r + (m + j * N) * N2]));
+                    atomic_cadd(AbB + j + m * N, s);
+                }
+                for (int e = threadIdx.x; e < N2 * N; e += kBlock) {
+                    const int j = e % N, b = (e / N) % N, m = e / N2;
+                    const long go = static_cast<long>(F2(j, b)) * N2;
+                    double2 v = make_double2(0.0, 0.0);
+                    for (int r = 0; r < N2; ++r) v = cadd(v, cmul(cconj(Q[go + r]), Tb[r + (m + j * N) * N2]));
+                    atomic_cadd(AbA + b + m * N, cscale(cmul(cconj(BB(j, b)), v), -1.0));
+                    atomic_cadd(AbB + j + b * N, cscale(cmul(cconj(AA(b, m)), v), -1.0));
+                }
+                for (int e = threadIdx.x; e < N2; e += kBlock) {
+                    const int n = e % N, a = e / N;
+                    double2 s = make_double2(0.0, 0.0);
+                    for (int i = 0; i < N; ++i) {
+                        s = csub(s, cmul(cconj(S123[a + i * N]), Db[n + i * N]));
+                        for (int g = 0; g < nf; ++g)
+                            s = cadd(s, cmul(cconj(Q[a + i * N + g * N2]), S67b[n + i * N + g * N2]));
+                    }
+                    atomic_cadd(AbA + n + a * N, s);
+                }
+                __syncthreads();
+                // 4: slice and operator cotangents of Q and of S123
+                for (long e = threadIdx.x; e < N2nf; e += kBlock) {
+                    const double2 qb = Qb[e];
+                    if (qb.x == 0.0 && qb.y == 0.0) continue;
+                    const int a = static_cast<int>(e % N), i = static_cast<int>((e / N) % N);
+                    const int g = static_cast<int>(e / N2);
+                    for (int b = 0; b < N; ++b) {
+                        const long x1 = id3(F2(b, a), g, F2(a, i)), x2 = id3(F2(i, b), g, F2(a, i));
+                        const double2 X1 = cadd(Cp[x1], Rp[x1]), X2 = cadd(Cp[x2], Rp[x2]);
+                        const double2 s1 = cmul(cconj(cmul(AA(a, b), BB(b, i))), qb);
+                        const double2 s2 = cscale(cmul(cconj(cmul(BB(a, b), AA(b, i))), qb), -1.0);
+                        atomic_cadd(sC + x1, s1);
+                        atomic_cadd(sR + x1, s1);
+                        atomic_cadd(sC + x2, s2);
+                        atomic_cadd(sR + x2, s2);
+                        atomic_cadd(AbA + a + b * N, cmul(cconj(cmul(BB(b, i), X1)), qb));
+                        atomic_cadd(AbB + b + i * N, cmul(cconj(cmul(AA(a, b), X1)), qb));
+                        atomic_cadd(AbB + a + b * N, cscale(cmul(cconj(cmul(AA(b, i), X2)), qb), -1.0));
+                        atomic_cadd(AbA + b + i * N, cscale(cmul(cconj(cmul(BB(a, b), X2)), qb), -1.0));
+                    }
+                }
+                for (int e = threadIdx.x; e < N2; e += kBlock) {
+                    const double2 sv = S123b[e];
+                    if (sv.x == 0.0 && sv.y == 0.0) continue;
+                    const int n = e % N, i = e / N;
+                    for (int a = 0; a < N; ++a)
+                        for (int b = 0; b < N; ++b) {
+                            const long p1 = id3(F2(b, a), F2(b, i), F2(n, b)), q1 = id3(F2(b, a), F2(n, a), F2(a, i));
+                            const long p2 = id3(F2(a, n), F2(b, i), F2(n, b));
+                            const long q3 = id3(F2(i, b), F2(n, a), F2(a, i));
+                            const double2 Mv = csub(Fp[p1], Rp[q1]), Iv = Fp[p2], Kv = Rp[q3];
+                            // -B(n,a) A(a,b) B(b,i) M
+                            const double2 w1 = cmul(cconj(cmul(cmul(BB(n, a), AA(a, b)), BB(b, i))), sv);
+                            atomic_cadd(sF + p1, cscale(w1, -1.0));
+                            atomic_cadd(sR + q1, w1);
+                            atomic_cadd(AbB + n + a * N, cscale(cmul(cconj(cmul(cmul(AA(a, b), BB(b, i)), Mv)), sv), -1.0));
+                            atomic_cadd(AbA + a + b * N, cscale(cmul(cconj(cmul(cmul(BB(n, a), BB(b, i)), Mv)), sv), -1.0));
+                            atomic_cadd(AbB + b + i * N, cscale(cmul(cconj(cmul(cmul(BB(n, a), AA(a, b)), Mv)), sv), -1.0));
+                            // +A(n,a) B(a,b) B(b,i) I
+                            atomic_cadd(sF + p2, cmul(cconj(cmul(cmul(AA(n, a), BB(a, b)), BB(b, i))), sv));
+                            atomic_cadd(AbA + n + a * N, cmul(cconj(cmul(cmul(BB(a, b), BB(b, i)), Iv)), sv));
+                            atomic_cadd(AbB + a + b * N, cmul(cconj(cmul(cmul(AA(n, a), BB(b, i)), Iv)), sv));
+                            atomic_cadd(AbB + b + i * N, cmul(cconj(cmul(cmul(AA(n, a), BB(a, b)), Iv)), sv));
+                            // -B(n,a) B(a,b) A(b,i) K
+                            atomic_cadd(sR + q3, cscale(cmul(cconj(cmul(cmul(BB(n, a), BB(a, b)), AA(b, i))), sv), -1.0));
+                            atomic_cadd(AbB + n + a * N, cscale(cmul(cconj(cmul(cmul(BB(a, b), AA(b, i)), Kv)), sv), -1.0));
+                            atomic_cadd(AbB + a + b * N, cscale(cmul(cconj(cmul(cmul(BB(n, a), AA(b, i)), Kv)), sv), -1.0));
+                            atomic_cadd(AbA + b + i * N, cscale(cmul(cconj(cmul(cmul(BB(n, a), BB(a, b)), Kv)), sv), -1.0));
+                        }
+                }
+                __syncthreads();
+#undef AA
+#undef BB
+#undef TB
+            }
+        for (int e = threadIdx.x; e < A.num_ops * N2; e += kBlock) A.Abar[t * A.num_ops * N2 + e] = Ab[e];
+        if constexpr (LDS_SLICES)
+            for (long e = threadIdx.x; e < 3 * A.nf3; e += kBlock) A.sbar[t * 3 * A.nf3 + e] = sb[e];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_l4f_adjoint(L4fAdjArgs A) {
+    extern __shared__ double2 sm[];
+    if (A.lds_slices)
+        l4f_adjoint_body<true>(sm, A);
+    else
+        l4f_adjoint_body<false>(sm, A);
 }
 
 // ---------------------------------------------------------------------------

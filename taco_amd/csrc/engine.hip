@@ -38,6 +38,7 @@
 #include <mutex>
 #include <numeric>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "engine_internal.hpp"
@@ -983,6 +984,50 @@ __device__ __forceinline__ void rk4_body(double2* sh, int nstage, const double2*
         default: CALL(0); break;        \
     }
 
+#include "rk4_rows.hpp"
+
+// Largest RK4 workgroup: the row-group path at D = 64.
+constexpr int kRkBlockMax = 512;
+
+// Row-group path for the sizes that have one (no separate midpoint series),
+// else the general path above.
+template <int DT>
+__device__ __forceinline__ void rk4_any(double2* sh, int nstage, const double2* Ls, const double2* Lh, long steps,
+                                        int D, const double2* r_in, double2* r_out, double dt, long every,
+                                        double2* samples, long* nsamples) {
+    if constexpr (DT != 0) {
+        if (!Lh) {
+            rk4_rows<DT, false>(sh, Ls, 0, steps, r_in, r_out, dt, every, samples, nsamples, nullptr);
+            return;
+        }
+    }
+    rk4_body<DT>(sh, nstage, Ls, Lh, steps, D, r_in, r_out, dt, every, samples, nsamples);
+}
+
+// One system on the row-group path (D = N^2, N = 2..8), 64 W threads.
+template <int DT>
+__global__ __launch_bounds__(64 * RowGroup<DT>::W) void k_rk4_rows(const double2* Ls, long steps, double2* r,
+                                                                   double dt, long every, double2* samples,
+                                                                   long* nsamples) {
+    extern __shared__ double2 sh[];
+    rk4_rows<DT, false>(sh, Ls, 0, steps, r, r, dt, every, samples, nsamples, nullptr);
+}
+
+// Calls f(std::integral_constant<int, D>) for a D with a row-group path.
+template <class F>
+static bool rows_dispatch(int D, F&& f) {
+    switch (D) {
+        case 4: f(std::integral_constant<int, 4>{}); return true;
+        case 9: f(std::integral_constant<int, 9>{}); return true;
+        case 16: f(std::integral_constant<int, 16>{}); return true;
+        case 25: f(std::integral_constant<int, 25>{}); return true;
+        case 36: f(std::integral_constant<int, 36>{}); return true;
+        case 49: f(std::integral_constant<int, 49>{}); return true;
+        case 64: f(std::integral_constant<int, 64>{}); return true;
+        default: return false;
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_rk4(const double2* Ls, const double2* Lh, long steps, int D, double2* r,
                                                 double dt, long every, double2* samples, long* nsamples, int nstage) {
     extern __shared__ double2 sh[];
@@ -1003,10 +1048,10 @@ struct Rk4Sys {
     int D;
 };
 
-__global__ __launch_bounds__(kBlock) void k_rk4_batch(const Rk4Sys* sys) {
+__global__ __launch_bounds__(kRkBlockMax) void k_rk4_batch(const Rk4Sys* sys) {
     extern __shared__ double2 sh[];
     const Rk4Sys S = sys[blockIdx.x];
-#define TACO_RK4_CALL(DT) rk4_body<DT>(sh, S.nstage, S.Ls, nullptr, S.steps, S.D, S.r0, S.r, S.dt, 1, S.samples, nullptr)
+#define TACO_RK4_CALL(DT) rk4_any<DT>(sh, S.nstage, S.Ls, nullptr, S.steps, S.D, S.r0, S.r, S.dt, 1, S.samples, nullptr)
     TACO_RK4_DISPATCH(S.D, TACO_RK4_CALL)
 #undef TACO_RK4_CALL
 }
@@ -1875,7 +1920,7 @@ struct BatchRun {
     DevBuf rkargs;
     long nrk = 0;
     std::size_t rk_lds = 0;
-    int rk_block = 64;  // raised to rk4_block(D) of the systems
+    int rk_block = 64;  // raised to the workgroup size each system needs
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool ran_eager = false;
@@ -2010,6 +2055,17 @@ static int rk4_block(int D) {
 
 static void rk4_launch(taco_ctx* ctx, const double2* dLs, const double2* dLh, std::size_t steps, int D, double2* dr,
                        double dt, std::size_t every, double2* dsamples, long* dns) {
+    if (!dLh && rk4_rows_threads(D) > 0) {
+        timed(ctx, K_RK4, 0.0, [&] {
+            rows_dispatch(D, [&](auto dc) {
+                constexpr int DT = decltype(dc)::value;
+                hipLaunchKernelGGL(k_rk4_rows<DT>, dim3(1), dim3(rk4_rows_threads(DT)), 3 * DT * 16, ctx->stream, dLs,
+                                   static_cast<long>(steps), dr, dt, static_cast<long>(every), dsamples, dns);
+            });
+        });
+        HIP_OK(hipGetLastError());
+        return;
+    }
     const int nstage = dLh ? 0 : rk4_nstage(D, 6, static_cast<long>(steps) + 1);
     const std::size_t lds = rk4_lds(D, 6, nstage);
     if (lds > kRkLds) throw Error(TACO_ERR_INVALID, "RK4 kernel supports D <= 1706");
@@ -2140,16 +2196,32 @@ static void tape_adjoint(taco_tape& tp, const double* samples_bar, double* gamma
                            c->stream, Lb, cntL);
     }
     const std::size_t nseg = (steps + K - 1) / K;
+    // row-group kernels where D has them (no generator staging), else the general ones
+    const int Di = static_cast<int>(D);
     Rk4AdjArgs Rf = R;
-    Rf.nstage = rk4_nstage(static_cast<int>(D), 6, static_cast<long>(K) + 1);
-    const std::size_t lds_f = rk4_lds(static_cast<int>(D), 6, Rf.nstage);
-    R.nstage = rk4_nstage(static_cast<int>(D), 12, static_cast<long>(steps) + 1);
-    const std::size_t lds_b = rk4_lds(static_cast<int>(D), 12, R.nstage);
+    Rf.nstage = rk4_nstage(Di, 6, static_cast<long>(K) + 1);
+    const std::size_t lds_f = rk4_lds(Di, 6, Rf.nstage);
+    R.nstage = rk4_nstage(Di, 12, static_cast<long>(steps) + 1);
+    const std::size_t lds_b = rk4_lds(Di, 12, R.nstage);
     if (lds_b > kRkLds || lds_f > kRkLds) throw Error(TACO_ERR_INVALID, "adjoint RK4 kernel supports D <= 853");
     timed(c, K_RK4, 0.0, [&] {
         // every segment's states and stage points from its checkpoint, all segments at once
-        hipLaunchKernelGGL(k_rk4_recompute, dim3(static_cast<unsigned>(nseg)), dim3(kBlock), lds_f, c->stream, Rf);
-        hipLaunchKernelGGL(k_rk4_adjoint, dim3(1), dim3(kBlock), lds_b, c->stream, R);
+        const bool rows_f = rows_dispatch(Di, [&](auto dc) {
+            constexpr int DT = decltype(dc)::value;
+            hipLaunchKernelGGL(k_rk4_recompute_rows<DT>, dim3(static_cast<unsigned>(nseg)), dim3(rk4_rows_threads(DT)),
+                               3 * DT * 16, c->stream, Rf);
+        });
+        if (!rows_f)
+            hipLaunchKernelGGL(k_rk4_recompute, dim3(static_cast<unsigned>(nseg)), dim3(kBlock), lds_f, c->stream, Rf);
+        if (Di == 4 || Di == 9 || Di == 16) {
+            rows_dispatch(Di, [&](auto dc) {
+                constexpr int DT = decltype(dc)::value;
+                if constexpr (DT <= 16)
+                    hipLaunchKernelGGL(k_rk4_adjoint_rows<DT>, dim3(1), dim3(64), 5 * DT * 16, c->stream, R);
+            });
+        } else {
+            hipLaunchKernelGGL(k_rk4_adjoint, dim3(1), dim3(kBlock), lds_b, c->stream, R);
+        }
     });
     HIP_OK(hipGetLastError());
     std::vector<std::complex<double>> gbar_h(static_cast<std::size_t>(tp.nf) * T, {0.0, 0.0});
@@ -2165,13 +2237,22 @@ static void tape_adjoint(taco_tape& tp, const double* samples_bar, double* gamma
         set_fn_lds(reinterpret_cast<const void*>(k_l4_adjoint), lds);
         const std::size_t sbar_n = T * 3 * nf3, abar_n = T * tp.num_ops * N2;
         const std::size_t nscr = static_cast<std::size_t>(ncomb) * 11 * T;
-        c->inter2.ensure((sbar_n + abar_n + nscr + static_cast<std::size_t>(tp.nf) * T) * 16 +
+        // factorised L4 adjoint (default; TACO_L4_ADJ_KERNEL=direct: the O(N^6) term walk)
+        const bool l4_direct = [] {
+            const char* v = std::getenv("TACO_L4_ADJ_KERNEL");
+            return v && std::string(v) == "direct";
+        }();
+        const unsigned l4_grid = static_cast<unsigned>(std::min<std::size_t>(T, 1024));
+        const std::size_t l4_per = 4 * N2 * static_cast<std::size_t>(tp.nf) + 3 * N2 + N2 * N2;
+        const std::size_t l4_scr = l4_direct ? 0 : l4_grid * l4_per;
+        c->inter2.ensure((sbar_n + abar_n + nscr + static_cast<std::size_t>(tp.nf) * T + l4_scr) * 16 +
                          static_cast<std::size_t>(ncomb) * 8 + 64);
         double2* sbar = c->inter2.as<double2>();
         double2* abar = sbar + sbar_n;
         double2* scr = abar + abar_n;
         double2* gb4 = scr + nscr;
-        double* om = reinterpret_cast<double*>(gb4 + static_cast<std::size_t>(tp.nf) * T);
+        double2* l4s = gb4 + static_cast<std::size_t>(tp.nf) * T;
+        double* om = reinterpret_cast<double*>(l4s + l4_scr);
         if (!lds_sl)
             hipLaunchKernelGGL(k_fill_zero, dim3(static_cast<unsigned>((sbar_n + kBlock - 1) / kBlock)), dim3(kBlock),
                                0, c->stream, sbar, static_cast<long>(sbar_n));
@@ -2189,9 +2270,33 @@ static void tape_adjoint(taco_tape& tp, const double* samples_bar, double* gamma
         A.sbar = sbar;
         A.Abar = abar;
         A.lds_slices = lds_sl ? 1 : 0;
-        timed(c, K_L4, 0.0, [&] {
-            hipLaunchKernelGGL(k_l4_adjoint, dim3(static_cast<unsigned>(T)), dim3(kBlock), lds, c->stream, A);
-        });
+        if (l4_direct) {
+            timed(c, K_L4, 0.0, [&] {
+                hipLaunchKernelGGL(k_l4_adjoint, dim3(static_cast<unsigned>(T)), dim3(kBlock), lds, c->stream, A);
+            });
+        } else {
+            L4fAdjArgs Af{};
+            Af.Lbar = A.Lbar;
+            Af.Fs = A.Fs;
+            Af.Cs = A.Cs;
+            Af.Rs = A.Rs;
+            Af.p2f = A.p2f;
+            Af.ops = A.ops;
+            Af.N = A.N;
+            Af.nf = A.nf;
+            Af.num_ops = A.num_ops;
+            Af.nf3 = A.nf3;
+            Af.T = static_cast<long>(T);
+            Af.sbar = sbar;
+            Af.Abar = abar;
+            Af.lds_slices = A.lds_slices;
+            Af.scr = l4s;
+            Af.scr_per = static_cast<long>(l4_per);
+            set_fn_lds(reinterpret_cast<const void*>(k_l4f_adjoint), lds);
+            timed(c, K_L4, 0.0, [&] {
+                hipLaunchKernelGGL(k_l4f_adjoint, dim3(l4_grid), dim3(kBlock), lds, c->stream, Af);
+            });
+        }
         HIP_OK(hipGetLastError());
         if (ncomb > 0) {
             FcrAdjArgs P{};
@@ -2730,10 +2835,11 @@ int taco_batch_create(taco_ctx* ctx, const taco_batch_system* systems, int nsys,
                 r.steps = static_cast<long>(d.steps);
                 r.dt = d.dt;
                 r.D = static_cast<int>(N2);
-                r.nstage = rk4_nstage(r.D, 6, static_cast<long>(d.steps) + 1);
+                const bool rows = rk4_rows_threads(r.D) > 0;
+                r.nstage = rows ? 0 : rk4_nstage(r.D, 6, static_cast<long>(d.steps) + 1);
                 rk.push_back(r);
                 B.rk_lds = std::max(B.rk_lds, rk4_lds(r.D, 6, r.nstage));
-                B.rk_block = std::max(B.rk_block, rk4_block(r.D));
+                B.rk_block = std::max(B.rk_block, rows ? rk4_rows_threads(r.D) : rk4_block(r.D));
             }
             B.sys.push_back(std::move(S));
         }

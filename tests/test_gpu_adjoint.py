@@ -111,6 +111,31 @@ def test_tape_adjoint_repeatable():
         assert np.allclose(a[key], c[key], rtol=1e-12, atol=1e-14 * np.abs(a[key]).max()), key
 
 
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_factorised_l4_adjoint_matches_term_walk(monkeypatch, case):
+    # the default (factorised) L4 reverse against the O(N^6) term walk kept
+    # behind TACO_L4_ADJ_KERNEL=direct: same cotangents up to summation order
+    name, S, G, dt, steps, K = case
+    N = S.N
+    rng = np.random.default_rng(9)
+    Wt = rng.standard_normal((steps + 1, N * N)) + 1j * rng.standard_normal((steps + 1, N * N))
+    out = []
+    for kernel in ("factorised", "direct"):
+        if kernel == "direct":
+            monkeypatch.setenv("TACO_L4_ADJ_KERNEL", "direct")
+        tp = AD.Tape(G, S.omegas, S.mirror, S.pair_to_freq, S.A_eig, S.eps, dt, steps, r0_for(N), 4, K)
+        out.append(tp.adjoint(Wt))
+        tp.close()
+    a, b = out
+    for key in ("gamma", "omegas", "eps", "r0"):
+        scale = np.abs(b[key]).max()
+        err = np.abs(a[key] - b[key]).max() / scale
+        print(f"{name} {key}: max diff / max {err:.2e}")
+        assert err <= 1e-12, key
+    for x, y in zip(a["ops"], b["ops"]):
+        assert np.abs(x - y).max() <= 1e-12 * np.abs(y).max()
+
+
 def oracle_spin_pair_rho_lab(orc, prob, p):
     alpha, wc, beta, eps, delta = p
     H1 = 0.5 * eps * AD.SZ + 0.5 * delta * AD.SX
