@@ -1,4 +1,4 @@
-// adjoint_host.cpp — reverse of the TCL2 generator (host side).
+// adjoint_host.cpp — host side of the TCL2 generator reverse.
 //
 // The forward generator is k_tcl2 / the drivers' TCL2 composition
 // (generator.cpp:81-177 with diagonal channels, TCL4_spin_boson_example.cpp:
@@ -9,10 +9,10 @@
 //   Hls += S AdA,  H = (Hls + Hls^H)/2 + diag(eps),
 //   L[row, col] += -i (d(r1,c1) H[r2,c2] - d(r2,c2) H[c1,r1]),
 // row = r1 N + r2, col = c1 N + c2, L stored L[row + col D].
-// Given Lbar(t) for t < T it returns dLoss/dGamma (rows t < T), and adds the
-// cotangents of the operators and of eps.  Complex cotangents follow
-// dLoss = Re sum conj(z_bar) dz.
-#include <algorithm>
+// The per-time work of the reverse runs on the device (k_tcl2_adj_*); here
+// are the per-(operator, bucket) parts it needs and the operator cotangents
+// formed from its time sums W_b = sum_t gam_b Lbar(t), V_b = sum_t S_b X(t).
+// Complex cotangents follow dLoss = Re sum conj(z_bar) dz.
 #include <complex>
 #include <cstddef>
 #include <vector>
@@ -23,147 +23,82 @@ namespace taco_engine {
 
 using cd = std::complex<double>;
 
-void tcl2_adjoint(const cd* Lbar, std::size_t T, const cd* gamma, std::size_t gld, const double* eps, const cd* ops,
-                  int num_ops, int N, const int* p2f, int nf, int order, cd* gamma_bar, std::size_t gbld,
-                  cd* ops_bar, double* eps_bar) {
-    (void)eps;
+Tcl2Parts tcl2_parts(const cd* ops, int num_ops, int N, const int* p2f, int nf) {
     const std::size_t NN = static_cast<std::size_t>(N), N2 = NN * NN, D = N2, D2 = D * D;
-    const cd I(0.0, 1.0);
-    // Per (al, b): P, AdA and the dissipator direction K = dL/dgam.
-    struct Part {
-        std::vector<cd> P, AdA, K;
-        bool any = false;
-    };
-    std::vector<Part> parts(static_cast<std::size_t>(num_ops) * nf);
+    Tcl2Parts out;
     for (int al = 0; al < num_ops; ++al)
         for (int b = 0; b < nf; ++b) {
-            Part& q = parts[static_cast<std::size_t>(al) * nf + b];
-            q.P.assign(N2, cd(0.0, 0.0));
+            std::vector<cd> P(N2, cd(0.0, 0.0));
             const cd* A = ops + static_cast<std::size_t>(al) * N2;
+            bool any = false;
             for (std::size_t e = 0; e < N2; ++e)
                 if (p2f[e] == b) {
-                    q.P[e] = A[e];
-                    q.any = q.any || A[e] != cd(0.0, 0.0);
+                    P[e] = A[e];
+                    any = any || A[e] != cd(0.0, 0.0);
                 }
-            if (!q.any) continue;
-            q.AdA.assign(N2, cd(0.0, 0.0));
+            if (!any) continue;
+            std::vector<cd> AdA(N2, cd(0.0, 0.0));
             for (std::size_t r = 0; r < NN; ++r)
                 for (std::size_t c = 0; c < NN; ++c) {
                     cd s(0.0, 0.0);
-                    for (std::size_t m = 0; m < NN; ++m) s += std::conj(q.P[m + r * NN]) * q.P[m + c * NN];
-                    q.AdA[r + c * NN] = s;
+                    for (std::size_t m = 0; m < NN; ++m) s += std::conj(P[m + r * NN]) * P[m + c * NN];
+                    AdA[r + c * NN] = s;
                 }
-            q.K.assign(D2, cd(0.0, 0.0));
+            const std::size_t k0 = out.K.size();
+            out.K.resize(k0 + D2);
             for (std::size_t c1 = 0; c1 < NN; ++c1)
                 for (std::size_t c2 = 0; c2 < NN; ++c2)
                     for (std::size_t r1 = 0; r1 < NN; ++r1)
                         for (std::size_t r2 = 0; r2 < NN; ++r2) {
                             const std::size_t row = r1 * NN + r2, col = c1 * NN + c2;
-                            cd v = std::conj(q.P[r1 + c1 * NN]) * q.P[r2 + c2 * NN];
-                            if (r1 == c1) v -= 0.5 * q.AdA[r2 + c2 * NN];
-                            if (r2 == c2) v -= 0.5 * q.AdA[c1 + r1 * NN];
-                            q.K[row + col * D] = v;
+                            cd v = std::conj(P[r1 + c1 * NN]) * P[r2 + c2 * NN];
+                            if (r1 == c1) v -= 0.5 * AdA[r2 + c2 * NN];
+                            if (r2 == c2) v -= 0.5 * AdA[c1 + r1 * NN];
+                            out.K[k0 + row + col * D] = v;
                         }
+            out.P.insert(out.P.end(), P.begin(), P.end());
+            out.AdA.insert(out.AdA.end(), AdA.begin(), AdA.end());
+            out.al.push_back(al);
+            out.b.push_back(b);
         }
-    // Accumulated over t: W = sum gam Lbar (per al, b), V = sum S Hlsbar.
-    // Times are split into at most 64 fixed blocks, summed in parallel and the
-    // block partials added in block order (same values for any thread count).
-    const std::size_t nblk = std::min<std::size_t>(T, 64), per = nblk ? (T + nblk - 1) / nblk : 0;
-    std::vector<cd> Wp(nblk * parts.size() * D2, cd(0.0, 0.0)), Vp(nblk * parts.size() * N2, cd(0.0, 0.0));
-    std::vector<double> epsp(nblk * NN, 0.0);
-#pragma omp parallel for schedule(dynamic)
-    for (std::ptrdiff_t bk = 0; bk < static_cast<std::ptrdiff_t>(nblk); ++bk) {
-        std::vector<cd> Hb(N2), Xb(N2);
-        cd* Wb = Wp.data() + static_cast<std::size_t>(bk) * parts.size() * D2;
-        cd* Vb = Vp.data() + static_cast<std::size_t>(bk) * parts.size() * N2;
-        double* eb = epsp.data() + static_cast<std::size_t>(bk) * NN;
-        const std::size_t t_end = std::min(T, (static_cast<std::size_t>(bk) + 1) * per);
-        for (std::size_t t = static_cast<std::size_t>(bk) * per; t < t_end; ++t) {
-            const cd* Lb = Lbar + t * D2;
-            // unitary part: Hbar from Lbar
-            for (auto& v : Hb) v = cd(0.0, 0.0);
-            for (std::size_t r1 = 0; r1 < NN; ++r1)
-                for (std::size_t r2 = 0; r2 < NN; ++r2) {
-                    const std::size_t row = r1 * NN + r2;
-                    for (std::size_t c2 = 0; c2 < NN; ++c2)  // r1 == c1
-                        Hb[r2 + c2 * NN] += I * Lb[row + (r1 * NN + c2) * D];
-                    for (std::size_t c1 = 0; c1 < NN; ++c1)  // r2 == c2
-                        Hb[c1 + r1 * NN] -= I * Lb[row + (c1 * NN + r2) * D];
-                }
-            for (std::size_t n = 0; n < NN; ++n) eb[n] += Hb[n + n * NN].real();
-            if (order == 0) continue;
-            for (std::size_t r = 0; r < NN; ++r)
-                for (std::size_t c = 0; c < NN; ++c)
-                    Xb[r + c * NN] = 0.5 * (Hb[r + c * NN] + std::conj(Hb[c + r * NN]));
-            for (int b = 0; b < nf; ++b) {
-                const cd G = gamma[static_cast<std::size_t>(b) * gld + t];
-                const double gam = 2.0 * G.real(), S = G.imag();
-                double gamb = 0.0, Sb = 0.0;
-                for (int al = 0; al < num_ops; ++al) {
-                    const std::size_t pi = static_cast<std::size_t>(al) * nf + b;
-                    const Part& q = parts[pi];
-                    if (!q.any) continue;
-                    double g = 0.0;
-                    for (std::size_t e = 0; e < D2; ++e) g += (std::conj(Lb[e]) * q.K[e]).real();
-                    gamb += g;
-                    double sv = 0.0;
-                    for (std::size_t e = 0; e < N2; ++e) sv += (std::conj(Xb[e]) * q.AdA[e]).real();
-                    Sb += sv;
-                    cd* W = Wb + pi * D2;
-                    for (std::size_t e = 0; e < D2; ++e) W[e] += gam * Lb[e];
-                    cd* V = Vb + pi * N2;
-                    for (std::size_t e = 0; e < N2; ++e) V[e] += S * Xb[e];
-                }
-                // Gamma = gam/2 + i S: dLoss/dRe = 2 gamb, dLoss/dIm = Sb
-                gamma_bar[static_cast<std::size_t>(b) * gbld + t] += cd(2.0 * gamb, Sb);
-            }
-        }
-    }
-    std::vector<cd> W(parts.size() * D2, cd(0.0, 0.0)), V(parts.size() * N2, cd(0.0, 0.0));
-    std::vector<double> epsb(NN, 0.0);
-    for (std::size_t bk = 0; bk < nblk; ++bk) {
-        for (std::size_t e = 0; e < W.size(); ++e) W[e] += Wp[bk * W.size() + e];
-        for (std::size_t e = 0; e < V.size(); ++e) V[e] += Vp[bk * V.size() + e];
-        for (std::size_t n = 0; n < NN; ++n) epsb[n] += epsp[bk * NN + n];
-    }
-    for (std::size_t n = 0; n < NN; ++n) eps_bar[n] += epsb[n];
-    if (order == 0) return;
-    // Operator cotangents from the accumulated weights.
+    return out;
+}
+
+void tcl2_ops_adjoint(const Tcl2Parts& parts, const cd* W, const cd* V, int N, const int* p2f, cd* ops_bar) {
+    const std::size_t NN = static_cast<std::size_t>(N), N2 = NN * NN, D = N2, D2 = D * D;
     std::vector<cd> AdAb(N2), Pb(N2);
-    for (int al = 0; al < num_ops; ++al)
-        for (int b = 0; b < nf; ++b) {
-            const std::size_t pi = static_cast<std::size_t>(al) * nf + b;
-            const Part& q = parts[pi];
-            if (!q.any) continue;
-            const cd* Wq = W.data() + pi * D2;
-            const cd* Vq = V.data() + pi * N2;
-            for (std::size_t e = 0; e < N2; ++e) AdAb[e] = Vq[e];  // from Hls += S AdA
-            for (auto& v : Pb) v = cd(0.0, 0.0);
-            for (std::size_t c1 = 0; c1 < NN; ++c1)
-                for (std::size_t c2 = 0; c2 < NN; ++c2)
-                    for (std::size_t r1 = 0; r1 < NN; ++r1)
-                        for (std::size_t r2 = 0; r2 < NN; ++r2) {
-                            const std::size_t row = r1 * NN + r2, col = c1 * NN + c2;
-                            const cd w = Wq[row + col * D];
-                            // jump conj(P[r1,c1]) P[r2,c2]
-                            Pb[r2 + c2 * NN] += w * q.P[r1 + c1 * NN];
-                            Pb[r1 + c1 * NN] += std::conj(w) * q.P[r2 + c2 * NN];
-                            if (r1 == c1) AdAb[r2 + c2 * NN] -= 0.5 * w;
-                            if (r2 == c2) AdAb[c1 + r1 * NN] -= 0.5 * w;
-                        }
-            // AdA[r,c] = sum_m conj(P[m,r]) P[m,c]
-            for (std::size_t r = 0; r < NN; ++r)
-                for (std::size_t c = 0; c < NN; ++c) {
-                    const cd a = AdAb[r + c * NN];
-                    for (std::size_t m = 0; m < NN; ++m) {
-                        Pb[m + c * NN] += a * q.P[m + r * NN];
-                        Pb[m + r * NN] += std::conj(a) * q.P[m + c * NN];
+    for (std::size_t q = 0; q < parts.al.size(); ++q) {
+        const int b = parts.b[q];
+        const cd* P = parts.P.data() + q * N2;
+        const cd* Wq = W + static_cast<std::size_t>(b) * D2;
+        const cd* Vq = V + static_cast<std::size_t>(b) * N2;
+        for (std::size_t e = 0; e < N2; ++e) AdAb[e] = Vq[e];  // from Hls += S AdA
+        for (auto& v : Pb) v = cd(0.0, 0.0);
+        for (std::size_t c1 = 0; c1 < NN; ++c1)
+            for (std::size_t c2 = 0; c2 < NN; ++c2)
+                for (std::size_t r1 = 0; r1 < NN; ++r1)
+                    for (std::size_t r2 = 0; r2 < NN; ++r2) {
+                        const std::size_t row = r1 * NN + r2, col = c1 * NN + c2;
+                        const cd w = Wq[row + col * D];
+                        // jump conj(P[r1,c1]) P[r2,c2]
+                        Pb[r2 + c2 * NN] += w * P[r1 + c1 * NN];
+                        Pb[r1 + c1 * NN] += std::conj(w) * P[r2 + c2 * NN];
+                        if (r1 == c1) AdAb[r2 + c2 * NN] -= 0.5 * w;
+                        if (r2 == c2) AdAb[c1 + r1 * NN] -= 0.5 * w;
                     }
+        // AdA[r,c] = sum_m conj(P[m,r]) P[m,c]
+        for (std::size_t r = 0; r < NN; ++r)
+            for (std::size_t c = 0; c < NN; ++c) {
+                const cd a = AdAb[r + c * NN];
+                for (std::size_t m = 0; m < NN; ++m) {
+                    Pb[m + c * NN] += a * P[m + r * NN];
+                    Pb[m + r * NN] += std::conj(a) * P[m + c * NN];
                 }
-            cd* Ab = ops_bar + static_cast<std::size_t>(al) * N2;
-            for (std::size_t e = 0; e < N2; ++e)
-                if (p2f[e] == b) Ab[e] += Pb[e];
-        }
+            }
+        cd* Ab = ops_bar + static_cast<std::size_t>(parts.al[q]) * N2;
+        for (std::size_t e = 0; e < N2; ++e)
+            if (p2f[e] == b) Ab[e] += Pb[e];
+    }
 }
 
 }  // namespace taco_engine

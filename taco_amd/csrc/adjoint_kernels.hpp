@@ -914,3 +914,127 @@ __global__ __launch_bounds__(kBlock) void k_bucket_reduce(const double2* scr, lo
     }
     gbar[static_cast<long>(b) * Nt + t] = s;
 }
+
+// ---------------------------------------------------------------------------
+// TCL2 generator adjoint, the reverse of k_tcl2 (generator.cpp:81-177 with
+// diagonal channels).  With P = A masked to bucket b, AdA = P^H P,
+// gam = 2 Re Gamma_b(t), S = Im Gamma_b(t) the forward adds per (operator,
+// bucket) part gam K (K = dL/dgam, the dissipator direction) and the
+// unitary part of H = diag(eps) + hermitize(sum S AdA).  Per time t
+// (k_tcl2_adj_t, one workgroup each):
+//   Hbar(p,q) = i sum_r Lbar[(r,p),(r,q)] - i sum_r Lbar[(q,r),(p,r)]
+//   (row (r1,r2) = r1 N + r2), eps parts Re Hbar(n,n), X = (Hbar + Hbar^H)/2,
+//   Gamma_b cotangent (2 sum_parts Re<Lbar,K>, sum_parts Re<X,AdA>).
+// Time sums (k_tcl2_adj_w per chunk of times, k_tcl2_adj_wsum over chunks
+// in order): W_b = sum_t gam_b Lbar(t), V_b = sum_t S_b X(t) and
+// sum_t Re Hbar(n,n); the host forms the operator cotangents from them.
+// ---------------------------------------------------------------------------
+struct Tcl2AdjArgs {
+    const double2* Lbar;   // [T][D^2]
+    const double2* gamma;  // [nf][Nt]
+    long Nt, T;
+    const double2* K;      // [npart][D^2]
+    const double2* AdA;    // [npart][N^2]
+    const int* part_b;     // [npart] bucket of each part
+    int npart, N, nf, order;
+    double2* gbar;         // [nf][T]
+    double2* X;            // [T][N^2]
+    double* epsd;          // [T][N]
+    int nchunk;
+    long nout;             // nf D^2 + nf N^2 + N
+    double2* part;         // [nchunk][nout]
+    double2* wsum;         // [nout]: W_b, V_b, eps sums (in .x)
+};
+
+__global__ __launch_bounds__(kBlock) void k_tcl2_adj_t(Tcl2AdjArgs A) {
+    __shared__ double wred[4];
+    extern __shared__ double2 sh[];
+    const int N = A.N, N2 = N * N, D = N2;
+    const long D2 = static_cast<long>(D) * D;
+    const long t = blockIdx.x;
+    const double2* Lb = A.Lbar + t * D2;
+    double2* Hb = sh;
+    double2* Xb = sh + N2;
+    for (int e = threadIdx.x; e < N2; e += kBlock) {
+        const int p = e % N, q = e / N;
+        double2 h = make_double2(0.0, 0.0);
+        for (int r = 0; r < N; ++r) {
+            const double2 a = Lb[(r * N + p) + static_cast<long>(r * N + q) * D];
+            const double2 b = Lb[(q * N + r) + static_cast<long>(p * N + r) * D];
+            h = cadd(h, make_double2(-a.y, a.x));  // + i a
+            h = cadd(h, make_double2(b.y, -b.x));  // - i b
+        }
+        Hb[e] = h;
+    }
+    __syncthreads();
+    for (int n = threadIdx.x; n < N; n += kBlock) A.epsd[t * N + n] = Hb[n + n * N].x;
+    if (A.order == 0) return;
+    for (int e = threadIdx.x; e < N2; e += kBlock) {
+        const int r = e % N, c = e / N;
+        const double2 x = cscale(cadd(Hb[e], cconj(Hb[c + r * N])), 0.5);
+        Xb[e] = x;
+        A.X[t * N2 + e] = x;
+    }
+    __syncthreads();
+    for (int b = 0; b < A.nf; ++b) {
+        double gp = 0.0, sp = 0.0;
+        bool any = false;
+        for (int pi = 0; pi < A.npart; ++pi) {
+            if (A.part_b[pi] != b) continue;
+            any = true;
+            const double2* K = A.K + pi * D2;
+            for (long e = threadIdx.x; e < D2; e += kBlock) {
+                const double2 l = Lb[e], k = K[e];
+                gp += l.x * k.x + l.y * k.y;
+            }
+            const double2* M = A.AdA + static_cast<long>(pi) * N2;
+            for (int e = threadIdx.x; e < N2; e += kBlock) {
+                const double2 x = Xb[e], m = M[e];
+                sp += x.x * m.x + x.y * m.y;
+            }
+        }
+        double2 out = make_double2(0.0, 0.0);
+        if (any) {
+            const double g = block_sum(gp, wred);
+            const double sv = block_sum(sp, wred);
+            out = make_double2(2.0 * g, sv);
+        }
+        if (threadIdx.x == 0) A.gbar[static_cast<long>(b) * A.T + t] = out;
+    }
+}
+
+// Chunk sums over times of W_b, V_b and the eps parts (grid: outputs x chunks).
+__global__ __launch_bounds__(kBlock) void k_tcl2_adj_w(Tcl2AdjArgs A) {
+    const long o = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (o >= A.nout) return;
+    const int N = A.N, N2 = N * N;
+    const long D2 = static_cast<long>(N2) * N2;
+    const long per = (A.T + A.nchunk - 1) / A.nchunk;
+    const long t0 = blockIdx.y * per, t1 = min(A.T, t0 + per);
+    double2 s = make_double2(0.0, 0.0);
+    const long nW = A.nf * D2, nV = static_cast<long>(A.nf) * N2;
+    if (o < nW) {
+        if (A.order != 0) {
+            const long b = o / D2, e = o % D2;
+            for (long t = t0; t < t1; ++t)
+                s = cadd(s, cscale(A.Lbar[t * D2 + e], 2.0 * A.gamma[b * A.Nt + t].x));
+        }
+    } else if (o < nW + nV) {
+        if (A.order != 0) {
+            const long b = (o - nW) / N2, e = (o - nW) % N2;
+            for (long t = t0; t < t1; ++t) s = cadd(s, cscale(A.X[t * N2 + e], A.gamma[b * A.Nt + t].y));
+        }
+    } else {
+        const long n = o - nW - nV;
+        for (long t = t0; t < t1; ++t) s.x += A.epsd[t * N + n];
+    }
+    A.part[static_cast<long>(blockIdx.y) * A.nout + o] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_tcl2_adj_wsum(Tcl2AdjArgs A) {
+    const long o = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (o >= A.nout) return;
+    double2 s = make_double2(0.0, 0.0);
+    for (int c = 0; c < A.nchunk; ++c) s = cadd(s, A.part[static_cast<long>(c) * A.nout + o]);
+    A.wsum[o] = s;
+}

@@ -2172,11 +2172,32 @@ static void tape_adjoint(taco_tape& tp, const double* samples_bar, double* gamma
     const std::size_t K = static_cast<std::size_t>(tp.K);
     // device layout in the context's rk buffer: gbar, Lbar, seg, r0bar
     const std::size_t steps = tp.steps;
-    c->rk.ensure((T * D + T * D2 + 4 * steps * D + D) * 16);
+    // TCL2 reverse: per-(operator, bucket) parts and the device time sums
+    const Tcl2Parts parts = tcl2_parts(tp.h_ops.data(), tp.num_ops, tp.N, tp.h_p2f.data(), tp.nf);
+    const std::size_t npart = parts.al.size();
+    const std::size_t nfz = static_cast<std::size_t>(tp.nf);
+    const std::size_t nout = nfz * D2 + nfz * N2 + static_cast<std::size_t>(tp.N);
+    const int nchunk = static_cast<int>(std::max<std::size_t>(1, std::min<std::size_t>(64, (std::size_t{1} << 22) / nout)));
+    const std::size_t n_tcl2 = T * N2 + (T * tp.N + 1) / 2 + nfz * T + nchunk * nout + nout + npart * (D2 + N2) +
+                               (npart + 1) / 4 + 1;
+    c->rk.ensure((T * D + T * D2 + 4 * steps * D + D + n_tcl2) * 16);
     double2* gb = c->rk.as<double2>();
     double2* Lb = gb + T * D;
     double2* seg = Lb + T * D2;
     double2* rb = seg + 4 * steps * D;
+    double2* t2X = rb + D;
+    double* t2eps = reinterpret_cast<double*>(t2X + T * N2);
+    double2* t2g = t2X + T * N2 + (T * tp.N + 1) / 2;
+    double2* t2part = t2g + nfz * T;
+    double2* t2w = t2part + nchunk * nout;
+    double2* t2K = t2w + nout;
+    double2* t2A = t2K + npart * D2;
+    int* t2b = reinterpret_cast<int*>(t2A + npart * N2);
+    if (npart) {
+        HIP_OK(hipMemcpyAsync(t2K, parts.K.data(), npart * D2 * 16, hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipMemcpyAsync(t2A, parts.AdA.data(), npart * N2 * 16, hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipMemcpyAsync(t2b, parts.b.data(), npart * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    }
     HIP_OK(hipMemcpyAsync(gb, samples_bar, T * D * 16, hipMemcpyHostToDevice, c->stream));
     Rk4AdjArgs R{};
     R.Ls = tp.lgen.as<double2>();
@@ -2365,14 +2386,43 @@ static void tape_adjoint(taco_tape& tp, const double* samples_bar, double* gamma
         for (std::size_t t = 0; t < T; ++t)
             for (std::size_t e = 0; e < opsb.size(); ++e) opsb[e] += ap[t * opsb.size() + e];
     }
-    // TCL2 part on the host from Lbar
-    std::vector<std::complex<double>> Lh(T * D2);
-    HIP_OK(hipMemcpyAsync(Lh.data(), Lb, T * D2 * 16, hipMemcpyDeviceToHost, c->stream));
-    std::vector<std::complex<double>> r0b(D);
+    // TCL2 part: per-time reverse and time sums on the device, operator
+    // cotangents on the host from the sums
+    Tcl2AdjArgs Q{};
+    Q.Lbar = Lb;
+    Q.gamma = tp.gamma.as<double2>();
+    Q.Nt = static_cast<long>(tp.Nt);
+    Q.T = static_cast<long>(T);
+    Q.K = t2K;
+    Q.AdA = t2A;
+    Q.part_b = t2b;
+    Q.npart = static_cast<int>(npart);
+    Q.N = tp.N;
+    Q.nf = tp.nf;
+    Q.order = tp.order;
+    Q.gbar = t2g;
+    Q.X = t2X;
+    Q.epsd = t2eps;
+    Q.nchunk = nchunk;
+    Q.nout = static_cast<long>(nout);
+    Q.part = t2part;
+    Q.wsum = t2w;
+    const unsigned nob = static_cast<unsigned>((nout + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_tcl2_adj_t, dim3(static_cast<unsigned>(T)), dim3(kBlock), 2 * N2 * 16, c->stream, Q);
+    hipLaunchKernelGGL(k_tcl2_adj_w, dim3(nob, static_cast<unsigned>(nchunk)), dim3(kBlock), 0, c->stream, Q);
+    hipLaunchKernelGGL(k_tcl2_adj_wsum, dim3(nob), dim3(kBlock), 0, c->stream, Q);
+    HIP_OK(hipGetLastError());
+    std::vector<std::complex<double>> g2(nfz * T), wv(nout), r0b(D);
+    HIP_OK(hipMemcpyAsync(g2.data(), t2g, g2.size() * 16, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipMemcpyAsync(wv.data(), t2w, nout * 16, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipMemcpyAsync(r0b.data(), rb, D * 16, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
-    tcl2_adjoint(Lh.data(), T, tp.h_gamma.data(), T, tp.h_eps.data(), tp.h_ops.data(), tp.num_ops, tp.N,
-                 tp.h_p2f.data(), tp.nf, tp.order, gbar_h.data(), T, opsb.data(), epsb.data());
+    const std::size_t nW = nfz * D2, nV = nfz * N2;
+    for (int n = 0; n < tp.N; ++n) epsb[static_cast<std::size_t>(n)] += wv[nW + nV + static_cast<std::size_t>(n)].real();
+    if (tp.order != 0) {
+        for (std::size_t e = 0; e < g2.size(); ++e) gbar_h[e] += g2[e];
+        tcl2_ops_adjoint(parts, wv.data(), wv.data() + nW, tp.N, tp.h_p2f.data(), opsb.data());
+    }
     if (gamma_bar) {
         auto* g = reinterpret_cast<std::complex<double>*>(gamma_bar);
         std::fill(g, g + tp.Nt * static_cast<std::size_t>(tp.nf), std::complex<double>(0.0, 0.0));

@@ -27,13 +27,18 @@ void system_build(const double* H, int N, const double* A_lab, int num_ops, doub
                   double* U, double* A_eig, int* nf_out, double* omegas, int* mirror, int* p2f,
                   int* zero_index);
 
-// Reverse of the TCL2 generator (adjoint_host.cpp); complex values are
-// interleaved doubles.  Adds into gamma_bar ([nf][gbld], rows t < T),
-// ops_bar and eps_bar.
-void tcl2_adjoint(const std::complex<double>* Lbar, std::size_t T, const std::complex<double>* gamma,
-                  std::size_t gld, const double* eps, const std::complex<double>* ops, int num_ops, int N,
-                  const int* p2f, int nf, int order, std::complex<double>* gamma_bar, std::size_t gbld,
-                  std::complex<double>* ops_bar, double* eps_bar);
+// Reverse of the TCL2 generator, host side (adjoint_host.cpp).  Parts: one
+// per (operator, bucket) with a nonzero masked operator P: K = dL/dgam
+// [D^2], AdA = P^H P [N^2], P [N^2]; complex values column-major.
+struct Tcl2Parts {
+    std::vector<std::complex<double>> K, AdA, P;
+    std::vector<int> al, b;
+};
+Tcl2Parts tcl2_parts(const std::complex<double>* ops, int num_ops, int N, const int* p2f, int nf);
+// Adds the operator cotangents formed from the time sums W [nf][D^2] and
+// V [nf][N^2] of the device reverse to ops_bar [num_ops][N^2].
+void tcl2_ops_adjoint(const Tcl2Parts& parts, const std::complex<double>* W, const std::complex<double>* V, int N,
+                      const int* p2f, std::complex<double>* ops_bar);
 
 void set_last_error(const std::string& msg);
 const double* cr_table();
