@@ -1170,6 +1170,10 @@ struct DevBuf {
     }
     template <class T>
     T* as() const { return static_cast<T*>(p); }
+    void swap(DevBuf& o) {
+        std::swap(p, o.p);
+        std::swap(bytes, o.bytes);
+    }
     ~DevBuf() {
         if (p) (void)hipFree(p);
     }
@@ -1178,6 +1182,20 @@ struct DevBuf {
 struct Twiddles {
     DevBuf std_, b_, c_;
     int L = -1;
+};
+
+// Device buffers of one tape.  A destroyed tape hands them back to its
+// context, and the next tape on the context takes them over, so repeated
+// gradients run without device allocations once the sizes have settled.
+struct TapeBufs {
+    DevBuf gamma, omegas, mirror, p2f, ops, eps, t2pos, jobtri, jobmask, clsstart, slices, lgen, ck;
+    void swap(TapeBufs& o) {
+        for (auto [a, b] : {std::pair{&gamma, &o.gamma}, {&omegas, &o.omegas}, {&mirror, &o.mirror}, {&p2f, &o.p2f},
+                            {&ops, &o.ops}, {&eps, &o.eps}, {&t2pos, &o.t2pos}, {&jobtri, &o.jobtri},
+                            {&jobmask, &o.jobmask}, {&clsstart, &o.clsstart}, {&slices, &o.slices},
+                            {&lgen, &o.lgen}, {&ck, &o.ck}})
+            a->swap(*b);
+    }
 };
 
 // Reverse-mode kernels (device code, same namespace as the forward kernels).
@@ -1248,6 +1266,7 @@ struct taco_ctx {
     taco_engine::DevBuf gamma, omegas, mirror, p2f, ops, spec, inter1, inter2, conv, slices, t2pos, l4, gw, misc,
         rk, eps, jobtri, clsstart, jobmask;
     std::size_t l4_lds_set = 0;  // dynamic LDS limit set on the fused L4 kernel
+    std::vector<std::unique_ptr<taco_engine::TapeBufs>> tape_pool;  // buffers of destroyed tapes (at most 2)
 };
 
 namespace taco_engine {
@@ -1921,6 +1940,7 @@ struct BatchRun {
     long nrk = 0;
     std::size_t rk_lds = 0;
     int rk_block = 64;  // raised to the workgroup size each system needs
+    double rk_bytes = 0.0;  // algorithmic bytes of the RK4 launch (generators read, samples written)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool ran_eager = false;
@@ -1994,7 +2014,7 @@ static void batch_enqueue_system(taco_ctx* c, BatchSys& S) {
 static void batch_enqueue(taco_ctx* c, BatchRun& B) {
     for (auto& s : B.sys) batch_enqueue_system(c, *s);
     if (B.nrk) {
-        timed(c, K_RK4, 0.0, [&] {
+        timed(c, K_RK4, B.rk_bytes, [&] {
             hipLaunchKernelGGL(k_rk4_batch, dim3(static_cast<unsigned>(B.nrk)), dim3(B.rk_block), B.rk_lds, c->stream,
                                B.rkargs.as<Rk4Sys>());
         });
@@ -2020,7 +2040,11 @@ struct taco_tape {
     std::vector<double> h_eps, h_omegas;
     std::vector<int> h_p2f, h_mirror;
     std::vector<std::complex<double>> h_ops;
-    taco_engine::DevBuf gamma, omegas, mirror, p2f, ops, eps, t2pos, jobtri, jobmask, clsstart, slices, lgen, ck;
+    taco_engine::TapeBufs bufs;
+    taco_engine::DevBuf &gamma = bufs.gamma, &omegas = bufs.omegas, &mirror = bufs.mirror, &p2f = bufs.p2f,
+                        &ops = bufs.ops, &eps = bufs.eps, &t2pos = bufs.t2pos, &jobtri = bufs.jobtri,
+                        &jobmask = bufs.jobmask, &clsstart = bufs.clsstart, &slices = bufs.slices, &lgen = bufs.lgen,
+                        &ck = bufs.ck;
     taco_engine::FcrJob fcr;
 };
 
@@ -2055,8 +2079,11 @@ static int rk4_block(int D) {
 
 static void rk4_launch(taco_ctx* ctx, const double2* dLs, const double2* dLh, std::size_t steps, int D, double2* dr,
                        double dt, std::size_t every, double2* dsamples, long* dns) {
+    // algorithmic bytes: every generator read once, every sample written once
+    const double gen_bytes = static_cast<double>(steps + 1) * D * D * 16.0 * (dLh ? 2.0 : 1.0);
+    const double rk_bytes = gen_bytes + (dsamples ? static_cast<double>(steps / every + 2) * D * 16.0 : 0.0);
     if (!dLh && rk4_rows_threads(D) > 0) {
-        timed(ctx, K_RK4, 0.0, [&] {
+        timed(ctx, K_RK4, rk_bytes, [&] {
             rows_dispatch(D, [&](auto dc) {
                 constexpr int DT = decltype(dc)::value;
                 hipLaunchKernelGGL(k_rk4_rows<DT>, dim3(1), dim3(rk4_rows_threads(DT)), 3 * DT * 16, ctx->stream, dLs,
@@ -2069,7 +2096,7 @@ static void rk4_launch(taco_ctx* ctx, const double2* dLs, const double2* dLh, st
     const int nstage = dLh ? 0 : rk4_nstage(D, 6, static_cast<long>(steps) + 1);
     const std::size_t lds = rk4_lds(D, 6, nstage);
     if (lds > kRkLds) throw Error(TACO_ERR_INVALID, "RK4 kernel supports D <= 1706");
-    timed(ctx, K_RK4, 0.0, [&] {
+    timed(ctx, K_RK4, rk_bytes, [&] {
         hipLaunchKernelGGL(k_rk4, dim3(1), dim3(rk4_block(D)), lds, ctx->stream, dLs, dLh, static_cast<long>(steps), D,
                            dr, dt, static_cast<long>(every), dsamples, dns, nstage);
     });
@@ -2225,7 +2252,11 @@ static void tape_adjoint(taco_tape& tp, const double* samples_bar, double* gamma
     R.nstage = rk4_nstage(Di, 12, static_cast<long>(steps) + 1);
     const std::size_t lds_b = rk4_lds(Di, 12, R.nstage);
     if (lds_b > kRkLds || lds_f > kRkLds) throw Error(TACO_ERR_INVALID, "adjoint RK4 kernel supports D <= 853");
-    timed(c, K_RK4, 0.0, [&] {
+    // algorithmic bytes of the recompute and the walk: generators read twice,
+    // stage points written and read, gbar read, Lbar written
+    const double rev_bytes = 2.0 * static_cast<double>(steps + 1) * D2 * 16.0 + 8.0 * steps * D * 16.0 +
+                             static_cast<double>(T) * (D + D2) * 16.0;
+    timed(c, K_RK4, rev_bytes, [&] {
         // every segment's states and stage points from its checkpoint, all segments at once
         const bool rows_f = rows_dispatch(Di, [&](auto dc) {
             constexpr int DT = decltype(dc)::value;
@@ -2890,6 +2921,7 @@ int taco_batch_create(taco_ctx* ctx, const taco_batch_system* systems, int nsys,
                 rk.push_back(r);
                 B.rk_lds = std::max(B.rk_lds, rk4_lds(r.D, 6, r.nstage));
                 B.rk_block = std::max(B.rk_block, rows ? rk4_rows_threads(r.D) : rk4_block(r.D));
+                B.rk_bytes += static_cast<double>(d.steps + 1) * r.D * (r.D + 1) * 16.0;
             }
             B.sys.push_back(std::move(S));
         }
@@ -3008,6 +3040,10 @@ int taco_tape_forward(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, con
         DeviceGuard g(ctx->device);
         auto tp = std::make_unique<taco_tape>();
         tp->ctx = ctx;
+        if (!ctx->tape_pool.empty()) {
+            tp->bufs.swap(*ctx->tape_pool.back());
+            ctx->tape_pool.pop_back();
+        }
         tp->Nt = Nt;
         tp->T = steps + 1;
         tp->steps = steps;
@@ -3059,6 +3095,11 @@ int taco_tape_destroy(taco_tape* tape) {
         if (!tape) return;
         DeviceGuard g(tape->ctx->device);
         (void)hipStreamSynchronize(tape->ctx->stream);
+        auto& pool = tape->ctx->tape_pool;
+        if (pool.size() < 2) {
+            pool.push_back(std::make_unique<TapeBufs>());
+            pool.back()->swap(tape->bufs);
+        }
         delete tape;
     });
 }
