@@ -108,11 +108,20 @@ constexpr int kNumFftCfg = 6;
 
 // Physical LDS slot of (group, position): one pad slot per 16 values and an
 // odd group stride keep strided accesses off a single bank.
+// G is a power of two (tile_groups halves it), so the (group, position) split
+// of a group-minor element index is a mask and a shift.
 struct TileLayout {
-    int lg, gs, G, GS;
-    __device__ TileLayout(int lg_, int G_) : lg(lg_), gs(1 << lg_), G(G_), GS((1 << lg_) + ((1 << lg_) >> 4) + 1) {}
+    int lg, gs, G, GS, lG;
+    __device__ TileLayout(int lg_, int G_)
+        : lg(lg_), gs(1 << lg_), G(G_), GS((1 << lg_) + ((1 << lg_) >> 4) + 1), lG(__ffs(G_) - 1) {}
     __device__ __forceinline__ int at(int grp, int pos) const { return grp * GS + pos + (pos >> 4); }
+    __device__ __forceinline__ int grp_of(int e) const { return e & (G - 1); }
+    __device__ __forceinline__ int pos_of(int e) const { return e >> lG; }
 };
+
+// Phase B with 8-value tiles runs at 4 waves per SIMD (two 512-thread or
+// four 256-thread workgroups per CU, as the LDS allows): at most 128 VGPRs.
+#define TACO_FFT_OCC(Cfg) __attribute__((amdgpu_waves_per_eu((Cfg::kPT <= 8 ? 4 : 1), 8)))
 
 template <class C>
 __host__ __device__ inline int tile_groups(int lg, int ngroups) {
@@ -126,17 +135,40 @@ __host__ __device__ inline int tile_groups(int lg, int ngroups) {
 
 // Passes of r stages; local stage sp (1-based) uses twiddle
 // twp[grp*gstride + (2^(sp-1) - 1) + kk], conjugated for inverse transforms.
-template <class C, int RL>
+template <class C, int RL, bool PRE>
 __device__ __forceinline__ void tile_pass(double2* buf, const TileLayout& T, int s_lo,
                                           const double2* __restrict__ twp, long gstride, bool conj) {
     constexpr int kFT = C::kFT;
     constexpr int R = 1 << RL;
     constexpr int SETS = C::kPT / R;
+    // PRE (8 values per thread): the pass loads its sets' 2^RL - 1 twiddles
+    // into registers before the tile's LDS reads, so the loads' latency
+    // overlaps the reads and the barrier; otherwise (kernels with no
+    // registers to spare) they are loaded as they go.
+    constexpr bool kPre = PRE && RL <= 3 && C::kPT <= 8;
     const int lg = T.lg;
     const int u0bits = lg - s_lo - RL + 1;
     const int spg = 1 << (lg - RL);
     const int nsets = T.G * spg;
     double2 X[SETS][R];
+    double2 W[SETS][kPre ? R - 1 : 1];
+    if constexpr (kPre) {
+#pragma unroll
+        for (int q = 0; q < SETS; ++q) {
+            const int set = threadIdx.x + q * kFT;
+            if (set < nsets) {
+                const int grp = set >> (lg - RL);
+                const int k = (set & (spg - 1)) >> u0bits;
+                const double2* tg = twp + grp * gstride;
+#pragma unroll
+                for (int i = 0; i < RL; ++i) {
+                    const double2* Tw = tg + ((1 << (s_lo + i - 1)) - 1);
+#pragma unroll
+                    for (int j = 0; j < (1 << i); ++j) W[q][(1 << i) - 1 + j] = Tw[k + (j << (s_lo - 1))];
+                }
+            }
+        }
+    }
 #pragma unroll
     for (int q = 0; q < SETS; ++q) {
         const int set = threadIdx.x + q * kFT;
@@ -165,7 +197,11 @@ __device__ __forceinline__ void tile_pass(double2* buf, const TileLayout& T, int
                 double2 Y[R];
 #pragma unroll
                 for (int j = 0; j < (1 << i); ++j) {
-                    double2 w = Tw[k + (j << (s_lo - 1))];
+                    double2 w;
+                    if constexpr (kPre)
+                        w = W[q][(1 << i) - 1 + j];
+                    else
+                        w = Tw[k + (j << (s_lo - 1))];
                     if (conj) w.y = -w.y;
 #pragma unroll
                     for (int m = 0; m < h; ++m) {
@@ -188,7 +224,7 @@ __device__ __forceinline__ void tile_pass(double2* buf, const TileLayout& T, int
 
 // All lg stages of the tile: passes of at most log2(values per thread)
 // stages, stage counts as even as possible.
-template <class C>
+template <class C, bool PRE = false>
 __device__ __forceinline__ void tile_fft(double2* buf, const TileLayout& T, const double2* __restrict__ twp, long gstride,
                          bool conj) {
     const int lg = T.lg;
@@ -200,13 +236,13 @@ __device__ __forceinline__ void tile_fft(double2* buf, const TileLayout& T, cons
     for (int p = 0; p < npass; ++p) {
         const int RL = base + (p < extra ? 1 : 0);
         switch (RL) {
-            case 1: tile_pass<C, 1>(buf, T, s_lo, twp, gstride, conj); break;
-            case 2: tile_pass<C, 2>(buf, T, s_lo, twp, gstride, conj); break;
+            case 1: tile_pass<C, 1, PRE>(buf, T, s_lo, twp, gstride, conj); break;
+            case 2: tile_pass<C, 2, PRE>(buf, T, s_lo, twp, gstride, conj); break;
             case 3:
-                if constexpr (C::kPT >= 8) tile_pass<C, 3>(buf, T, s_lo, twp, gstride, conj);
+                if constexpr (C::kPT >= 8) tile_pass<C, 3, PRE>(buf, T, s_lo, twp, gstride, conj);
                 break;
             default:
-                if constexpr (C::kPT >= 16) tile_pass<C, 4>(buf, T, s_lo, twp, gstride, conj);
+                if constexpr (C::kPT >= 16) tile_pass<C, 4, PRE>(buf, T, s_lo, twp, gstride, conj);
                 break;
         }
         s_lo += RL;
@@ -278,7 +314,7 @@ __global__ __launch_bounds__(Cfg::kFT) void k_phase_a_spec(FcrArgs P) {
     const bool conj_in = (job & 1) != 0;
     const int total = G * gs;
     for (int e = threadIdx.x; e < total; e += kFT) {
-        const int grp = e % G, pos = e / G;
+        const int grp = Tl.grp_of(e), pos = Tl.pos_of(e);
         const long t = static_cast<long>(r0b + grp) + (static_cast<long>(pos) << P.b);
         double2 x = make_double2(0.0, 0.0);
         if (t < P.Nt) {
@@ -291,7 +327,7 @@ __global__ __launch_bounds__(Cfg::kFT) void k_phase_a_spec(FcrArgs P) {
     tile_fft<Cfg>(buf, Tl, P.twStd, 0, false);
     double2* out = P.inter1 + static_cast<long>(job) * P.n;
     for (int e = threadIdx.x; e < total; e += kFT) {
-        const int grp = e % G, pos = e / G;
+        const int grp = Tl.grp_of(e), pos = Tl.pos_of(e);
         out[(static_cast<long>(pos) << P.b) + r0b + grp] = buf[Tl.at(grp, pos)];
     }
 }
@@ -322,7 +358,7 @@ __global__ __launch_bounds__(Cfg::kFT) void k_phase_a(FcrArgs P) {
 #pragma unroll
     for (int q = 0; q < kPT; ++q) {
         const int e = threadIdx.x + q * kFT;
-        const int grp = e % G, pos = e / G;
+        const int grp = Tl.grp_of(e), pos = Tl.pos_of(e);
         const long t = static_cast<long>(r0b + grp) + (static_cast<long>(pos) << P.b);
         ph[q] = make_double2(0.0, 0.0);
         if (e < total && t < P.Nt) ph[q] = cconj(cexpi(omega * (static_cast<double>(t) * P.dt)));
@@ -335,7 +371,7 @@ __global__ __launch_bounds__(Cfg::kFT) void k_phase_a(FcrArgs P) {
         for (int q = 0; q < kPT; ++q) {
             const int e = threadIdx.x + q * kFT;
             if (e < total) {
-                const int grp = e % G, pos = e / G;
+                const int grp = Tl.grp_of(e), pos = Tl.pos_of(e);
                 const long t = static_cast<long>(r0b + grp) + (static_cast<long>(pos) << P.b);
                 buf[Tl.at(grp, pos)] = (t < P.Nt) ? cmul(src[t], ph[q]) : make_double2(0.0, 0.0);
             }
@@ -344,7 +380,7 @@ __global__ __launch_bounds__(Cfg::kFT) void k_phase_a(FcrArgs P) {
         tile_fft<Cfg>(buf, Tl, P.twStd, 0, false);
         double2* out = P.inter1 + (p - P.job0) * P.n;
         for (int e = threadIdx.x; e < total; e += kFT) {
-            const int grp = e % G, pos = e / G;
+            const int grp = Tl.grp_of(e), pos = Tl.pos_of(e);
             out[(static_cast<long>(pos) << P.b) + r0b + grp] = buf[Tl.at(grp, pos)];
         }
         __syncthreads();
@@ -354,7 +390,7 @@ __global__ __launch_bounds__(Cfg::kFT) void k_phase_a(FcrArgs P) {
 // Phase B: forward stages a+1..L; mode 1 stores the spectrum, mode 0 forms
 // the F and C products and runs inverse stages 1..b on each.
 template <class Cfg>
-__global__ __launch_bounds__(Cfg::kFT) void k_phase_b(FcrArgs P, int mode) {
+__global__ __launch_bounds__(Cfg::kFT) TACO_FFT_OCC(Cfg) void k_phase_b(FcrArgs P, int mode) {
     constexpr int kFT = Cfg::kFT;
     constexpr int kPT = Cfg::kPT;
     __shared__ double2 buf[Cfg::kLds];
@@ -370,7 +406,7 @@ __global__ __launch_bounds__(Cfg::kFT) void k_phase_b(FcrArgs P, int mode) {
     for (int e = threadIdx.x; e < total; e += kFT) buf[Tl.at(e >> Tl.lg, e & (gs - 1))] = in[e];
     __syncthreads();
     const long bstride = (1L << P.b) - 1;
-    tile_fft<Cfg>(buf, Tl, P.twB + static_cast<long>(k1b) * bstride, bstride, false);
+    tile_fft<Cfg, true>(buf, Tl, P.twB + static_cast<long>(k1b) * bstride, bstride, false);
     if (mode == 1) {
         double2* out = P.spec + static_cast<long>(job) * P.n + static_cast<long>(k1b) * gs;
         for (int e = threadIdx.x; e < total; e += kFT) out[e] = buf[Tl.at(e >> Tl.lg, e & (gs - 1))];
@@ -397,10 +433,10 @@ __global__ __launch_bounds__(Cfg::kFT) void k_phase_b(FcrArgs P, int mode) {
             if (e < total) buf[Tl.at(e >> Tl.lg, e & (gs - 1))] = cmul(X[q], sp[e]);
         }
         __syncthreads();
-        tile_fft<Cfg>(buf, Tl, P.twStd, 0, true);
+        tile_fft<Cfg, true>(buf, Tl, P.twStd, 0, true);
         double2* out = P.inter2 + (static_cast<long>(path) * P.njobs + job) * P.n;
         for (int e = threadIdx.x; e < total; e += kFT) {
-            const int grp = e % G, pos = e / G;
+            const int grp = Tl.grp_of(e), pos = Tl.pos_of(e);
             out[(static_cast<long>(pos) << P.a) + k1b + grp] = buf[Tl.at(grp, pos)];
         }
         __syncthreads();
@@ -457,8 +493,11 @@ __global__ __launch_bounds__(Cfg::kFT) void k_phase_c(FcrArgs P, int ntiles) {
         tile_fft<Cfg>(buf, Tl, P.twC + static_cast<long>(Q0) * cstride, cstride, true);                       \
         const int path = (JP) / P.njobs, job = (JP) % P.njobs;                                                \
         double2* out = P.conv + (path * P.cconv + (P.job0 + job - P.comb0)) * P.T;                            \
-        for (int e = threadIdx.x; e < total; e += kFT) {                                                      \
-            const int grp = e % G, v2 = e / G;                                                                \
+        /* only rows v2 with (Q0) + v2 * 2^b < T are written */                                              \
+        const long vrows = min(static_cast<long>(gs), (P.T - (Q0) + (1L << P.b) - 1) >> P.b);                  \
+        const int etot = min(total, G * static_cast<int>(vrows));                                             \
+        for (int e = threadIdx.x; e < etot; e += kFT) {                                                       \
+            const int grp = Tl.grp_of(e), v2 = Tl.pos_of(e);                                                  \
             const long tt = static_cast<long>((Q0) + grp) + (static_cast<long>(v2) << P.b);                   \
             if (tt < P.T) {                                                                                   \
                 const double2 y = cscale(buf[Tl.at(grp, v2)], inv_n);                                         \
@@ -1394,7 +1433,7 @@ static int env_int(const char* name, int dflt) {
 static std::size_t chunk_budget_bytes() {
     static std::size_t v = [] {
         const char* s = std::getenv("TACO_FCR_CHUNK_BYTES");
-        return s ? static_cast<std::size_t>(std::strtoull(s, nullptr, 10)) : (std::size_t{3} << 30);
+        return s ? static_cast<std::size_t>(std::strtoull(s, nullptr, 10)) : (std::size_t{12} << 30);
     }();
     return v;
 }
