@@ -385,7 +385,7 @@ def main():
     choice = np.zeros(3, np.int32)
     nfft = 1 << int(np.ceil(np.log2(2 * G.shape[0] - 1)))
     _lib.check(_lib.lib().taco_fft_tile_choice(tcl4.context(0).handle, int(np.log2(nfft)), _lib.iptr(choice)))
-    tiles = ["8192/512", "4096/256", "4096/512", "2048/256", "4096/1024", "2048/512"]
+    tiles = ["8192/512", "4096/256", "4096/512", "2048/256", "4096/1024", "2048/512", "8192/1024"]
     fft_tiles = {ph: tiles[c] if c >= 0 else None for ph, c in zip("ABC", choice)}
     L4 = out.cpu().numpy().reshape(ntau, S.N ** 2, S.N ** 2).transpose(0, 2, 1)
     cpu, parity = None, None

@@ -104,7 +104,8 @@ using Cfg4kW = FftCfg<4096, 512>;
 using Cfg2k = FftCfg<2048, 256>;
 using Cfg4kX = FftCfg<4096, 1024>;
 using Cfg2kW = FftCfg<2048, 512>;
-constexpr int kNumFftCfg = 6;
+using Cfg8kX = FftCfg<8192, 1024>;
+constexpr int kNumFftCfg = 7;
 
 // Physical LDS slot of (group, position): one pad slot per 16 values and an
 // odd group stride keep strided accesses off a single bank.
@@ -135,9 +136,11 @@ __host__ __device__ inline int tile_groups(int lg, int ngroups) {
 
 // Passes of r stages; local stage sp (1-based) uses twiddle
 // twp[grp*gstride + (2^(sp-1) - 1) + kk], conjugated for inverse transforms.
+// Sets whose butterfly index k is kmax or above produce no value a later
+// pass or the caller reads and are skipped.
 template <class C, int RL, bool PRE>
 __device__ __forceinline__ void tile_pass(double2* buf, const TileLayout& T, int s_lo,
-                                          const double2* __restrict__ twp, long gstride, bool conj) {
+                                          const double2* __restrict__ twp, long gstride, bool conj, int kmax) {
     constexpr int kFT = C::kFT;
     constexpr int R = 1 << RL;
     constexpr int SETS = C::kPT / R;
@@ -156,9 +159,9 @@ __device__ __forceinline__ void tile_pass(double2* buf, const TileLayout& T, int
 #pragma unroll
         for (int q = 0; q < SETS; ++q) {
             const int set = threadIdx.x + q * kFT;
-            if (set < nsets) {
+            const int k = (set & (spg - 1)) >> u0bits;
+            if (set < nsets && k < kmax) {
                 const int grp = set >> (lg - RL);
-                const int k = (set & (spg - 1)) >> u0bits;
                 const double2* tg = twp + grp * gstride;
 #pragma unroll
                 for (int i = 0; i < RL; ++i) {
@@ -172,10 +175,10 @@ __device__ __forceinline__ void tile_pass(double2* buf, const TileLayout& T, int
 #pragma unroll
     for (int q = 0; q < SETS; ++q) {
         const int set = threadIdx.x + q * kFT;
-        if (set < nsets) {
+        const int rem = set & (spg - 1);
+        const int u0 = rem & ((1 << u0bits) - 1), k = rem >> u0bits;
+        if (set < nsets && k < kmax) {
             const int grp = set >> (lg - RL);
-            const int rem = set & (spg - 1);
-            const int u0 = rem & ((1 << u0bits) - 1), k = rem >> u0bits;
             const int pos0 = u0 + ((k * R) << u0bits);
 #pragma unroll
             for (int m = 0; m < R; ++m) X[q][m] = buf[T.at(grp, pos0 + (m << u0bits))];
@@ -185,10 +188,10 @@ __device__ __forceinline__ void tile_pass(double2* buf, const TileLayout& T, int
 #pragma unroll
     for (int q = 0; q < SETS; ++q) {
         const int set = threadIdx.x + q * kFT;
-        if (set < nsets) {
+        const int rem = set & (spg - 1);
+        const int u0 = rem & ((1 << u0bits) - 1), k = rem >> u0bits;
+        if (set < nsets && k < kmax) {
             const int grp = set >> (lg - RL);
-            const int rem = set & (spg - 1);
-            const int u0 = rem & ((1 << u0bits) - 1), k = rem >> u0bits;
             const double2* tg = twp + grp * gstride;
 #pragma unroll
             for (int i = 0; i < RL; ++i) {
@@ -223,26 +226,43 @@ __device__ __forceinline__ void tile_pass(double2* buf, const TileLayout& T, int
 }
 
 // All lg stages of the tile: passes of at most log2(values per thread)
-// stages, stage counts as even as possible.
+// stages, stage counts as even as possible.  need: only positions below it
+// are read afterwards; each pass then runs only the sets that feed them (a
+// set of butterfly index k writes positions from k * 2^u0bits on and reads
+// the block of R * 2^u0bits positions from k * R * 2^u0bits), which leaves
+// every kept value bit-identical.
 template <class C, bool PRE = false>
 __device__ __forceinline__ void tile_fft(double2* buf, const TileLayout& T, const double2* __restrict__ twp, long gstride,
-                         bool conj) {
+                                         bool conj, int need = 1 << 30) {
     const int lg = T.lg;
     if (lg <= 0) return;
     constexpr int maxr = C::kPT >= 16 ? 4 : (C::kPT >= 8 ? 3 : 2);
     const int npass = (lg + maxr - 1) / maxr;
     const int base = lg / npass, extra = lg % npass;
+    // Sets of pass p that are needed: walk the bound back from the last pass.
+    auto kmax_of = [&](int p) {
+        int bout = min(need, 1 << lg), s_hi = lg + 1, K = 0;
+        for (int pp = npass - 1; pp >= p; --pp) {
+            const int RL = base + (pp < extra ? 1 : 0);
+            const int s_lo = s_hi - RL, ub = lg - s_lo - RL + 1;
+            K = (bout + (1 << ub) - 1) >> ub;
+            bout = min(K << (ub + RL), 1 << lg);
+            s_hi = s_lo;
+        }
+        return K;
+    };
     int s_lo = 1;
     for (int p = 0; p < npass; ++p) {
         const int RL = base + (p < extra ? 1 : 0);
+        const int kmax = need >= (1 << lg) ? (1 << 30) : kmax_of(p);
         switch (RL) {
-            case 1: tile_pass<C, 1, PRE>(buf, T, s_lo, twp, gstride, conj); break;
-            case 2: tile_pass<C, 2, PRE>(buf, T, s_lo, twp, gstride, conj); break;
+            case 1: tile_pass<C, 1, PRE>(buf, T, s_lo, twp, gstride, conj, kmax); break;
+            case 2: tile_pass<C, 2, PRE>(buf, T, s_lo, twp, gstride, conj, kmax); break;
             case 3:
-                if constexpr (C::kPT >= 8) tile_pass<C, 3, PRE>(buf, T, s_lo, twp, gstride, conj);
+                if constexpr (C::kPT >= 8) tile_pass<C, 3, PRE>(buf, T, s_lo, twp, gstride, conj, kmax);
                 break;
             default:
-                if constexpr (C::kPT >= 16) tile_pass<C, 4, PRE>(buf, T, s_lo, twp, gstride, conj);
+                if constexpr (C::kPT >= 16) tile_pass<C, 4, PRE>(buf, T, s_lo, twp, gstride, conj, kmax);
                 break;
         }
         s_lo += RL;
@@ -490,12 +510,12 @@ __global__ __launch_bounds__(Cfg::kFT) void k_phase_c(FcrArgs P, int ntiles) {
             if (e < total) buf[Tl.at(e >> Tl.lg, e & (gs - 1))] = V[q];                                       \
         }                                                                                                     \
         __syncthreads();                                                                                      \
-        tile_fft<Cfg>(buf, Tl, P.twC + static_cast<long>(Q0) * cstride, cstride, true);                       \
+        /* only rows v2 with (Q0) + v2 * 2^b < T are kept */                                                 \
+        const int vrows = static_cast<int>(min(static_cast<long>(gs), (P.T - (Q0) + (1L << P.b) - 1) >> P.b));  \
+        tile_fft<Cfg>(buf, Tl, P.twC + static_cast<long>(Q0) * cstride, cstride, true, vrows);                \
         const int path = (JP) / P.njobs, job = (JP) % P.njobs;                                                \
         double2* out = P.conv + (path * P.cconv + (P.job0 + job - P.comb0)) * P.T;                            \
-        /* only rows v2 with (Q0) + v2 * 2^b < T are written */                                              \
-        const long vrows = min(static_cast<long>(gs), (P.T - (Q0) + (1L << P.b) - 1) >> P.b);                  \
-        const int etot = min(total, G * static_cast<int>(vrows));                                             \
+        const int etot = min(total, G * vrows);                                                               \
         for (int e = threadIdx.x; e < etot; e += kFT) {                                                       \
             const int grp = Tl.grp_of(e), v2 = Tl.pos_of(e);                                                  \
             const long tt = static_cast<long>((Q0) + grp) + (static_cast<long>(v2) << P.b);                   \
@@ -1421,7 +1441,8 @@ static void launch_phase(int cfg, Phase ph, const FcrArgs& P, long njobs, hipStr
         case 2: launch_phase_cfg<Cfg4kW>(ph, P, njobs, st); break;
         case 3: launch_phase_cfg<Cfg2k>(ph, P, njobs, st); break;
         case 4: launch_phase_cfg<Cfg4kX>(ph, P, njobs, st); break;
-        default: launch_phase_cfg<Cfg2kW>(ph, P, njobs, st); break;
+        case 5: launch_phase_cfg<Cfg2kW>(ph, P, njobs, st); break;
+        default: launch_phase_cfg<Cfg8kX>(ph, P, njobs, st); break;
     }
 }
 
@@ -1698,7 +1719,7 @@ static void fcr_launch(taco_ctx* c, FcrJob& J, const CombineArgs& out) {
     const long nfft_jobs = J.nfft_jobs, ncomb = J.ncomb, chunk = J.chunk, cconv = J.cconv;
     const long spec_jobs = J.spec_jobs;
     const std::vector<int>& cls_start = J.cls_start;
-    // Tile configuration per phase: TACO_FFT_TILE=<0..3> forces one; otherwise
+    // Tile configuration per phase: TACO_FFT_TILE=<0..6> forces one; otherwise
     // the first chunk of each FFT length is run with every configuration
     // (identical butterflies, identical results) and the fastest is kept.
     auto it = c->fft_choice.find(L);

@@ -233,7 +233,7 @@ def test_fft_path_is_bitwise_at_t0(orc):
     assert np.array_equal(got.R[..., 0], ref[2][..., 0])
 
 
-@pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4, 5, 6])
 def test_fft_tile_configs_bitwise_identical(orc, cfg):
     """Every FFT tile configuration the autotuner may pick performs the same
     butterflies: results are bit-identical across configurations."""
