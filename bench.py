@@ -38,8 +38,9 @@ def parse():
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--workload", default="gue_pair",
-                   choices=["gue_pair", "published", "spin_boson", "batched", "adjoint"])
-    p.add_argument("--systems", type=int, default=256, help="batched workload: number of systems")
+                   choices=["gue_pair", "published", "spin_boson", "batched", "adjoint", "tfim_adjoint"])
+    p.add_argument("--systems", type=int, default=None,
+                   help="batched workload: number of systems (256); tfim_adjoint: parameter sets (4)")
     p.add_argument("--no-plain", action="store_true", help="batched workload: skip the plain per-system leg")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / checker leg")
     p.add_argument("--cpu-threads", type=int, default=0)
@@ -137,7 +138,7 @@ def batched_main(a):
     from taco_amd import tcl4, workloads as W
 
     assert torch.cuda.is_available(), "bench needs an MI355X"
-    nsys = a.systems
+    nsys = a.systems or 256
     t0 = time.perf_counter()
     ws = [W.heterogeneous_system(b) for b in range(nsys)]
     items = []
@@ -327,6 +328,107 @@ def adjoint_main(a):
     print(json.dumps(rec), flush=True)
 
 
+def tfim_adjoint_main(a):
+    """Gradients of the d = 16 loss (open 4-spin transverse-field Ising chain,
+    J = 1, h drawn from U[0.5, 1.5] (seed 5), A = sum sz_i, Drude-Lorentz
+    bath lambda 0.05, gamma 5, beta 1) with respect to (lambda, gamma, beta,
+    J, h), one parameter set after another (--systems sets, default 4).  The
+    window is reduced from the 2^20-point grid and 20001 times of the full
+    configuration (its forward F/C/R pass alone moves ~10 TB per parameter
+    set) to a 2^12-point grid and 200 RK4 steps of TCL2 + L4, checkpoints
+    every 64.  One step = one gradient per parameter set (forward + reverse);
+    the value is gradients per second.  The gradient of the first set is
+    checked against central differences (h = 1e-6 relative) of the same GPU
+    pipeline, whose d = 16 forward the tests pin to the oracle; the oracle's
+    own d = 16 forward at this size (all 531441 F/C/R triples at Nfft 2^13
+    plus 201 O(N^6) L4 evaluations) is too slow to time as a baseline."""
+    import torch
+
+    from taco_amd import adjoint as AD
+
+    assert torch.cuda.is_available(), "bench needs an MI355X"
+    nsets = a.systems or 4
+    rng = np.random.default_rng(5)
+    hs = rng.uniform(0.5, 1.5, size=nsets)
+    prob = AD.TfimChainProblem(Nt=2 ** 12, dt=0.005, steps=200, checkpoint_every=64)
+    P0 = [np.array([0.05, 5.0, 1.0, 1.0, h]) for h in hs]
+    u = np.where(np.random.default_rng(4).uniform(size=5) < 0.5, -1.0, 1.0)
+    with torch.no_grad():
+        targets = [prob.rho_lab(torch.tensor(p * (1 + 0.05 * u), dtype=torch.float64)).detach() for p in P0]
+    print(f"[bench] tfim_adjoint: {nsets} targets built", file=sys.stderr, flush=True)
+
+    def fwd(q):
+        with torch.no_grad():
+            return float(prob.loss(torch.tensor(P0[q], dtype=torch.float64), targets[q]))
+
+    def grad(q):
+        params = torch.tensor(P0[q], dtype=torch.float64, requires_grad=True)
+        loss = prob.loss(params, targets[q])
+        (g,) = torch.autograd.grad(loss, params)
+        return float(loss), g.numpy()
+
+    for _ in range(a.warmup):
+        fwd(0)
+        grad(0)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        for q in range(nsets):
+            fwd(q)
+    t_fwd = (time.perf_counter() - t0) / (a.steps * nsets)
+    t0 = time.perf_counter()
+    grads = []
+    for _ in range(a.steps):
+        grads = [grad(q) for q in range(nsets)]
+    t_grad = (time.perf_counter() - t0) / (a.steps * nsets)
+    print(f"[bench] forward {t_fwd * 1e3:.1f} ms, forward+reverse {t_grad * 1e3:.1f} ms per set", file=sys.stderr,
+          flush=True)
+    from taco_amd import tcl4
+    with tcl4.Profile(0) as prof:
+        grad(0)
+        kern = prof.read()
+    kernels, roofline = kernel_table(kern, 1)
+    loss0, g0 = grads[0]
+    fdg = np.zeros(5)
+    for q in range(5):
+        h = 1e-6 * abs(P0[0][q])
+        pp, pm = P0[0].copy(), P0[0].copy()
+        pp[q] += h
+        pm[q] -= h
+        with torch.no_grad():
+            lp = float(prob.loss(torch.tensor(pp, dtype=torch.float64), targets[0]))
+            lm = float(prob.loss(torch.tensor(pm, dtype=torch.float64), targets[0]))
+        fdg[q] = (lp - lm) / (2 * h)
+    rel = np.abs(g0 - fdg) / np.abs(fdg)
+    rec = {
+        "metric": METRIC,
+        "value": 1.0 / t_grad,
+        "unit": "gradients/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * t_grad * nsets,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128",
+        "data": "synthetic",
+        "config": {
+            "workload": f"tfim_adjoint (reduced window): d=16 open TFIM chain (nf=81), J=1, h~U[0.5,1.5] seed 5, "
+                        f"{nsets} parameter sets, Drude-Lorentz (0.05, 5, 1), Nt=2^12, dt=0.005, 200 RK4 steps of "
+                        f"TCL2+L4, checkpoints every 64, 5 parameters (lambda, gamma, beta, J, h)",
+            "path": "tape forward + hand-written reverse, one parameter set after another",
+        },
+        "forward_ms": 1e3 * t_fwd,
+        "gradient_over_forward": t_grad / t_fwd,
+        "check": {"params": list(prob.names), "h": float(hs[0]), "adjoint": g0.tolist(), "gpu_fd": fdg.tolist(),
+                  "max_rel_diff": float(rel.max()), "loss": loss0},
+        "roofline": roofline,
+        "cpu_baseline": None,
+        "kernels": kernels,
+    }
+    print(json.dumps(rec), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -337,6 +439,8 @@ def main():
         return batched_main(a)
     if a.workload == "adjoint":
         return adjoint_main(a)
+    if a.workload == "tfim_adjoint":
+        return tfim_adjoint_main(a)
     import torch
 
     import taco_amd

@@ -332,21 +332,92 @@ class SpinPairProblem:
         params = (alpha, omega_c, beta, eps, delta)."""
         alpha, wc, beta, eps, delta = params[0], params[1], params[2], params[3], params[4]
         H = self.hamiltonian(eps.to(torch.complex128), delta.to(torch.complex128))
-        A = self.coupling()
-        e, U = EighFun.apply(H, [A], self.freq_tol)
-        S = System.build(H.detach().numpy(), [A], self.freq_tol)
-        om = BucketOmegas.apply(e, S.omegas, S.pair_to_freq)
-        C = BcfFun.apply(alpha, wc, beta, (0, self.Nt - 1, self.dt))
-        G = GammaFun.apply(C, om, self.dt)
-        At = torch.from_numpy(A)
-        Aeig = (U.conj().T @ At @ U).unsqueeze(0)
-        N = S.N
-        r0 = torch.zeros(N * N, dtype=torch.complex128)
-        r0[0] = 1.0
-        meta = (S.mirror, S.pair_to_freq, self.dt, self.steps, self.order, self.checkpoint_every, exec)
-        r = TCL4Propagate.apply(G, om, Aeig, e, r0, meta)
-        rho = r.reshape(-1, N, N).transpose(1, 2)  # vec column-major: rho[n, m] = r[n + m N]
-        return U @ rho @ U.conj().T
+        return _rho_lab(self, H, self.coupling(), 0, alpha, wc, beta, exec)
+
+    def loss(self, params: torch.Tensor, target: torch.Tensor, exec: Optional[Exec] = None) -> torch.Tensor:
+        d = self.rho_lab(params, exec) - target
+        return (d.real ** 2 + d.imag ** 2).sum()
+
+
+def _rho_lab(prob, H: torch.Tensor, A: np.ndarray, kind: int, p0, p1, beta, exec: Optional[Exec]) -> torch.Tensor:
+    """Lab-basis rho(t), t = 0..steps, of the TCL4 run from rho0 = |0><0| (in
+    the eigenbasis) for a differentiable Hamiltonian H and bath parameters
+    (p0, p1, beta) of spectral density `kind`; buckets and mirror are those of
+    H's current spectrum (held fixed under differentiation)."""
+    e, U = EighFun.apply(H, [A], prob.freq_tol)
+    S = System.build(H.detach().numpy(), [A], prob.freq_tol)
+    om = BucketOmegas.apply(e, S.omegas, S.pair_to_freq)
+    C = BcfFun.apply(p0, p1, beta, (kind, prob.Nt - 1, prob.dt))
+    G = GammaFun.apply(C, om, prob.dt)
+    At = torch.from_numpy(A)
+    Aeig = (U.conj().T @ At @ U).unsqueeze(0)
+    N = S.N
+    r0 = torch.zeros(N * N, dtype=torch.complex128)
+    r0[0] = 1.0
+    meta = (S.mirror, S.pair_to_freq, prob.dt, prob.steps, prob.order, prob.checkpoint_every, exec)
+    r = TCL4Propagate.apply(G, om, Aeig, e, r0, meta)
+    rho = r.reshape(-1, N, N).transpose(1, 2)  # vec column-major: rho[n, m] = r[n + m N]
+    return U @ rho @ U.conj().T
+
+
+def _kron_chain(ops) -> torch.Tensor:
+    out = ops[0]
+    for o in ops[1:]:
+        out = torch.kron(out, o)
+    return out
+
+
+@dataclass
+class TfimChainProblem:
+    """Gradient workload: open n-spin transverse-field Ising chain
+    H = -J sum sz_i sz_{i+1} - h sum sx_i (d = 2^n; 16 for n = 4), coupling
+    A = sum sz_i, Drude-Lorentz bath J(w) = 2 lam gam w / (w^2 + gam^2); RK4
+    of TCL2 + L4 from rho0 = |0><0| in the eigenbasis, loss
+    sum_t ||rho_lab(t) - target(t)||_F^2 over t = 0..steps."""
+    n_spins: int = 4
+    Nt: int = 2 ** 12
+    dt: float = 0.005
+    steps: int = 200
+    checkpoint_every: int = 64
+    order: int = 4
+    freq_tol: float = 1e-9
+
+    names = ("lambda", "gamma", "beta", "J", "h")
+
+    def hamiltonian(self, J, h):
+        n = self.n_spins
+        sz, sx, i2 = (torch.from_numpy(m) for m in (SZ, SX, I2))
+        d = 2 ** n
+        H = torch.zeros((d, d), dtype=torch.complex128)
+        for i in range(n):
+            ops = [i2] * n
+            ops[i] = sx
+            H = H - h * _kron_chain(ops)
+            if i + 1 < n:
+                ops = [i2] * n
+                ops[i] = sz
+                ops[i + 1] = sz
+                H = H - J * _kron_chain(ops)
+        return H
+
+    def coupling(self):
+        n = self.n_spins
+        A = np.zeros((2 ** n, 2 ** n), complex)
+        for i in range(n):
+            ops = [I2] * n
+            ops[i] = SZ
+            m = ops[0]
+            for o in ops[1:]:
+                m = np.kron(m, o)
+            A += m
+        return A
+
+    def rho_lab(self, params: torch.Tensor, exec: Optional[Exec] = None) -> torch.Tensor:
+        """rho_lab(t) [steps+1, d, d] as a differentiable function of
+        params = (lambda, gamma, beta, J, h)."""
+        lam, gam, beta, J, h = params[0], params[1], params[2], params[3], params[4]
+        H = self.hamiltonian(J.to(torch.complex128), h.to(torch.complex128))
+        return _rho_lab(self, H, self.coupling(), 1, lam, gam, beta, exec)
 
     def loss(self, params: torch.Tensor, target: torch.Tensor, exec: Optional[Exec] = None) -> torch.Tensor:
         d = self.rho_lab(params, exec) - target

@@ -444,24 +444,28 @@ struct L4fAdjArgs {
     double2* sbar;        // [T][3][nf3]; zero on entry unless lds_slices
     double2* Abar;        // [T][num_ops][N^2]
     int lds_slices;
+    int lds_tb;           // Tbar in LDS (else after U in the scratch)
     double2* scr;         // [gridDim.x][scr_per]
-    long scr_per;         // 4 N^2 nf + 3 N^2 + N^4
+    long scr_per;         // 4 N^2 nf + 3 N^2 + N^4, + N^4 when !lds_tb
 };
 
 // LDS_SLICES: slice cotangents accumulate in LDS (else in global memory);
-// fixed per instantiation so every access has a known address space.
-template <bool LDS_SLICES>
+// LDS_TB: the N^4 Tbar tile sits in LDS (else in the workgroup's scratch,
+// for N above 8 where it exceeds the 160 KiB of a CU).  Both are fixed per
+// instantiation so every access has a known address space.
+template <bool LDS_SLICES, bool LDS_TB>
 __device__ __forceinline__ void l4f_adjoint_body(double2* sm, const L4fAdjArgs& A) {
     const int N = A.N, N2 = N * N, N4 = N2 * N2, nf = A.nf;
     const long N2nf = static_cast<long>(N2) * nf;
-    double2* Tb = sm;                                   // N^4
-    double2* opsS = Tb + N4;                            // num_ops N^2
+    double2* const scr0 = A.scr + static_cast<long>(blockIdx.x) * A.scr_per;
+    double2* Tb = LDS_TB ? sm : scr0 + 4 * N2nf + 3 * N2 + N4;  // N^4
+    double2* opsS = LDS_TB ? Tb + N4 : sm;              // num_ops N^2
     double2* Ab = opsS + A.num_ops * N2;                // num_ops N^2
     double2* sb = Ab + A.num_ops * N2;                  // 3 nf3 (LDS_SLICES)
     int* f = reinterpret_cast<int*>(sb + (LDS_SLICES ? 3 * A.nf3 : 0));
     for (int e = threadIdx.x; e < A.num_ops * N2; e += kBlock) opsS[e] = A.ops[e];
     for (int e = threadIdx.x; e < N2; e += kBlock) f[e] = A.p2f[e];
-    double2* Q = A.scr + static_cast<long>(blockIdx.x) * A.scr_per;  // Q(a,i,g) at a + i N + g N^2
+    double2* Q = scr0;                                  // Q(a,i,g) at a + i N + g N^2
     double2* S67 = Q + N2nf;
     double2* S67b = S67 + N2nf;
     double2* Qb = S67b + N2nf;
@@ -670,10 +674,17 @@ __device__ __forceinline__ void l4f_adjoint_body(double2* sm, const L4fAdjArgs& 
 
 __global__ __launch_bounds__(kBlock) void k_l4f_adjoint(L4fAdjArgs A) {
     extern __shared__ double2 sm[];
-    if (A.lds_slices)
-        l4f_adjoint_body<true>(sm, A);
-    else
-        l4f_adjoint_body<false>(sm, A);
+    if (A.lds_tb) {
+        if (A.lds_slices)
+            l4f_adjoint_body<true, true>(sm, A);
+        else
+            l4f_adjoint_body<false, true>(sm, A);
+    } else {
+        if (A.lds_slices)
+            l4f_adjoint_body<true, false>(sm, A);
+        else
+            l4f_adjoint_body<false, false>(sm, A);
+    }
 }
 
 // ---------------------------------------------------------------------------

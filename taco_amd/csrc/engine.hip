@@ -2163,9 +2163,9 @@ static void rk4_launch(taco_ctx* ctx, const double2* dLs, const double2* dLh, st
     HIP_OK(hipGetLastError());
 }
 
-static std::size_t l4_adj_lds(int N, int num_ops, std::size_t nf3, bool lds_slices) {
+static std::size_t l4_adj_lds(int N, int num_ops, std::size_t nf3, bool lds_slices, bool lds_tb = true) {
     const std::size_t N2 = static_cast<std::size_t>(N) * N;
-    return (N2 * N2 + 2 * num_ops * N2 + (lds_slices ? 3 * nf3 : 0)) * 16 + N2 * 4;
+    return ((lds_tb ? N2 * N2 : 0) + 2 * num_ops * N2 + (lds_slices ? 3 * nf3 : 0)) * 16 + N2 * 4;
 }
 
 static void set_fn_lds(const void* fn, std::size_t lds) {
@@ -2343,19 +2343,28 @@ static void tape_adjoint(taco_tape& tp, const double* samples_bar, double* gamma
         const FcrJob& J = tp.fcr;
         const long ncomb = J.ncomb;
         // L4 adjoint: slice cotangents [T][3][nf3], operator partials [T][ops N^2]
-        const bool lds_sl = l4_adj_lds(tp.N, tp.num_ops, nf3, true) <= 96 * 1024;
-        const std::size_t lds = l4_adj_lds(tp.N, tp.num_ops, nf3, lds_sl);
-        if (lds > 160 * 1024) throw Error(TACO_ERR_INVALID, "L4 adjoint kernel supports N <= 8");
-        set_fn_lds(reinterpret_cast<const void*>(k_l4_adjoint), lds);
-        const std::size_t sbar_n = T * 3 * nf3, abar_n = T * tp.num_ops * N2;
-        const std::size_t nscr = static_cast<std::size_t>(ncomb) * 11 * T;
         // factorised L4 adjoint (default; TACO_L4_ADJ_KERNEL=direct: the O(N^6) term walk)
         const bool l4_direct = [] {
             const char* v = std::getenv("TACO_L4_ADJ_KERNEL");
             return v && std::string(v) == "direct";
         }();
+        // The N^4 Tbar tile stays in LDS while it fits (N <= 8); the factorised
+        // kernel otherwise keeps it in its scratch (TACO_L4_ADJ_TB=global
+        // forces that placement at any N).
+        const bool tb_global_env = [] {
+            const char* v = std::getenv("TACO_L4_ADJ_TB");
+            return v && std::string(v) == "global";
+        }();
+        const bool lds_tb = l4_direct || (!tb_global_env && l4_adj_lds(tp.N, tp.num_ops, nf3, false) <= 160 * 1024);
+        const bool lds_sl = l4_adj_lds(tp.N, tp.num_ops, nf3, true, lds_tb) <= 96 * 1024;
+        const std::size_t lds = l4_adj_lds(tp.N, tp.num_ops, nf3, lds_sl, lds_tb);
+        if (lds > 160 * 1024) throw Error(TACO_ERR_INVALID, "direct L4 adjoint kernel supports N <= 8");
+        if (l4_direct) set_fn_lds(reinterpret_cast<const void*>(k_l4_adjoint), lds);
+        const std::size_t sbar_n = T * 3 * nf3, abar_n = T * tp.num_ops * N2;
+        const std::size_t nscr = static_cast<std::size_t>(ncomb) * 11 * T;
         const unsigned l4_grid = static_cast<unsigned>(std::min<std::size_t>(T, 1024));
-        const std::size_t l4_per = 4 * N2 * static_cast<std::size_t>(tp.nf) + 3 * N2 + N2 * N2;
+        const std::size_t l4_per =
+            4 * N2 * static_cast<std::size_t>(tp.nf) + 3 * N2 + N2 * N2 + (lds_tb ? 0 : N2 * N2);
         const std::size_t l4_scr = l4_direct ? 0 : l4_grid * l4_per;
         c->inter2.ensure((sbar_n + abar_n + nscr + static_cast<std::size_t>(tp.nf) * T + l4_scr) * 16 +
                          static_cast<std::size_t>(ncomb) * 8 + 64);
@@ -2402,6 +2411,7 @@ static void tape_adjoint(taco_tape& tp, const double* samples_bar, double* gamma
             Af.sbar = sbar;
             Af.Abar = abar;
             Af.lds_slices = A.lds_slices;
+            Af.lds_tb = lds_tb ? 1 : 0;
             Af.scr = l4s;
             Af.scr_per = static_cast<long>(l4_per);
             set_fn_lds(reinterpret_cast<const void*>(k_l4f_adjoint), lds);

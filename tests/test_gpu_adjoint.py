@@ -177,3 +177,117 @@ def test_spin_pair_parameter_gradient_matches_oracle_fd(orc):
         got = float(grad[q])
         print(f"d loss / d {name}: adjoint {got:.12e} fd {ref:.12e} rel {abs(got - ref) / abs(ref):.2e}")
         assert abs(got - ref) <= FD_TOL * abs(ref), name
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_l4_adjoint_tbar_in_scratch_matches_lds(monkeypatch, case):
+    # the placement used above N = 8 (the N^4 Tbar tile in global scratch,
+    # forced here with TACO_L4_ADJ_TB=global) against the LDS placement
+    name, S, G, dt, steps, K = case
+    N = S.N
+    rng = np.random.default_rng(13)
+    Wt = rng.standard_normal((steps + 1, N * N)) + 1j * rng.standard_normal((steps + 1, N * N))
+    out = []
+    for where in ("lds", "global"):
+        if where == "global":
+            monkeypatch.setenv("TACO_L4_ADJ_TB", "global")
+        tp = AD.Tape(G, S.omegas, S.mirror, S.pair_to_freq, S.A_eig, S.eps, dt, steps, r0_for(N), 4, K)
+        out.append(tp.adjoint(Wt))
+        tp.close()
+    a, b = out
+    for key in ("gamma", "omegas", "eps", "r0"):
+        err = np.abs(a[key] - b[key]).max() / np.abs(b[key]).max()
+        print(f"{name} {key}: max diff / max {err:.2e}")
+        assert err <= 1e-12, key
+    for x, y in zip(a["ops"], b["ops"]):
+        assert np.abs(x - y).max() <= 1e-12 * np.abs(y).max()
+
+
+# d = 16: the 4-spin transverse-field Ising chain (nf = 81), a short window
+# with a coarse step and a strong bath so that rho moves appreciably
+D16 = W.tfim_workload(0.7, lam=0.2, dt=0.05, Nt=64, tmax=6)
+D16_STEPS, D16_K = 6, 4
+
+
+def test_d16_tape_forward_matches_oracle(orc):
+    S, G, dt = D16.system, D16.gamma, D16.dt
+    assert S.N == 16 and S.nf == 81
+    tp = AD.Tape(G, S.omegas, S.mirror, S.pair_to_freq, S.A_eig, S.eps, dt, D16_STEPS, r0_for(16), 4, D16_K)
+    got = tp.samples.copy()
+    tp.close()
+    ref = orc.tcl4_propagate(G, S.omegas, S.mirror, S.pair_to_freq, S.eps, S.A_eig, dt, D16_STEPS, r0_for(16))
+    err = np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    moved = np.linalg.norm(ref - ref[0], axis=1).max() / np.linalg.norm(ref[0])
+    print(f"d16 rho per-step rel err max {err.max():.2e}; rho moves by {moved:.2e}")
+    assert moved > 1e-3
+    assert err.max() <= 1e-10
+
+
+def test_d16_vjp_matches_fd_of_forward():
+    """The d = 16 reverse (Tbar in scratch, slice cotangents in global memory,
+    the general RK4 reverse for D = 256) against central differences of the
+    GPU forward, which the test above pins to the oracle."""
+    S, G, dt = D16.system, D16.gamma, D16.dt
+    N, steps, K = 16, D16_STEPS, D16_K
+    rng = np.random.default_rng(21)
+    Wt = rng.standard_normal((steps + 1, N * N)) + 1j * rng.standard_normal((steps + 1, N * N))
+    tp = AD.Tape(G, S.omegas, S.mirror, S.pair_to_freq, S.A_eig, S.eps, dt, steps, r0_for(N), 4, K)
+    g = tp.adjoint(Wt)
+    tp.close()
+
+    def loss(Gx=G, om=S.omegas, ops=S.A_eig, eps=S.eps, r0=r0_for(N)):
+        t = AD.Tape(Gx, om, S.mirror, S.pair_to_freq, ops, eps, dt, steps, r0, 4, K)
+        s = t.samples.copy()
+        t.close()
+        return float(np.real(np.vdot(Wt, s)))
+
+    l0 = abs(loss())
+
+    def check(label, got, f, h):
+        ref = (f(h) - f(-h)) / (2 * h)
+        tol = FD_TOL * abs(ref) + 1e-14 * l0 / h
+        print(f"d16 {label}: vjp {got:.12e} fd {ref:.12e} diff {abs(got - ref):.2e} tol {tol:.2e}")
+        assert abs(got - ref) <= tol, label
+
+    dG = (rng.standard_normal(G.shape) + 1j * rng.standard_normal(G.shape)) * np.abs(G).max()
+    check("gamma", float(np.real(np.vdot(g["gamma"], dG))), lambda s: loss(Gx=G + s * dG), 1e-5)
+    dw = rng.standard_normal(S.nf)
+    check("omegas", float(g["omegas"] @ dw), lambda s: loss(om=S.omegas + s * dw), 1e-4)
+    dA = [rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N)) for _ in S.A_eig]
+    got = sum(float(np.real(np.vdot(a, d))) for a, d in zip(g["ops"], dA))
+    check("ops", got, lambda s: loss(ops=[A + s * d for A, d in zip(S.A_eig, dA)]), 1e-5)
+    de = rng.standard_normal(N)
+    check("eps", float(g["eps"] @ de), lambda s: loss(eps=S.eps + s * de), 1e-4)
+    dr = rng.standard_normal(N * N) + 1j * rng.standard_normal(N * N)
+    check("r0", float(np.real(np.vdot(g["r0"], dr))), lambda s: loss(r0=r0_for(N) + s * dr), 1e-2)
+
+
+def test_tfim_parameter_gradient_matches_fd():
+    """Gradient of the d = 16 chain loss with respect to (lambda, gamma, beta,
+    J, h) through the whole chain (Drude-Lorentz bath correlation, trapz
+    Gamma, eigensystem, TCL4 run and its reverse) against central differences
+    (h = 1e-6 relative) of the same pipeline's forward."""
+    prob = AD.TfimChainProblem(Nt=256, dt=0.02, steps=12, checkpoint_every=4)
+    p0 = np.array([0.1, 5.0, 1.0, 1.0, 0.7])
+    u = np.where(np.random.default_rng(4).uniform(size=5) < 0.5, -1.0, 1.0)
+    with torch.no_grad():
+        target = prob.rho_lab(torch.tensor(p0 * (1 + 0.05 * u), dtype=torch.float64)).detach()
+    params = torch.tensor(p0, dtype=torch.float64, requires_grad=True)
+    loss = prob.loss(params, target)
+    (grad,) = torch.autograd.grad(loss, params)
+
+    def floss(p):
+        with torch.no_grad():
+            return float(prob.loss(torch.tensor(p, dtype=torch.float64), target))
+
+    l0 = float(loss)
+    for q, name in enumerate(prob.names):
+        h = 1e-6 * abs(p0[q])
+        pp, pm = p0.copy(), p0.copy()
+        pp[q] += h
+        pm[q] -= h
+        ref = (floss(pp) - floss(pm)) / (2 * h)
+        got = float(grad[q])
+        tol = FD_TOL * abs(ref) + 1e-14 * l0 / h
+        print(f"d loss / d {name}: adjoint {got:.12e} fd {ref:.12e} rel {abs(got - ref) / abs(ref):.2e}")
+        assert abs(got - ref) <= tol, name
