@@ -482,7 +482,10 @@ def main():
     tf = os.path.join(ROOT, "profiles", "fcr_l4_gue_pair_d4_traffic.json")
     if a.workload == "gue_pair" and not a.reduced and os.path.exists(tf):
         t = json.load(open(tf))
-        if t.get("kernel") == roofline["kernel"]:
+        # only when it was measured on launches of the same size as these
+        same = abs(t.get("algo_bytes_per_launch", 0.0) - roofline["algo_bytes_per_launch"]) <= (
+            0.01 * roofline["algo_bytes_per_launch"])
+        if t.get("kernel") == roofline["kernel"] and same:
             roofline["traffic"] = t["traffic_bytes_per_launch"]
             roofline["traffic_source"] = t["source"]
     from taco_amd import _lib
