@@ -87,6 +87,13 @@ int taco_fft_tile_choice(taco_ctx* ctx, int log2_nfft, int* cfg3);
  * rounded values.  ctx == NULL evaluates on the host, else on the GPU. */
 int taco_cr_sincos(taco_ctx* ctx, const double* x, size_t n, double* s, double* c);
 
+/* Correctly rounded exp of each x[i]: the exponentials of the bath
+ * correlation (the Ohmic factor of J(w) and the KMS factors), for which the
+ * reference calls std::exp (correlation_fft.hpp:154-214) with platform-libm
+ * last bits; the engine fixes them to the correctly rounded values, as it
+ * does the phases.  ctx == NULL evaluates on the host, else on the GPU. */
+int taco_cr_exp(taco_ctx* ctx, const double* x, size_t n, double* out);
+
 /* --------------------------------------------------------------------------
  * Inputs of the hot path (host).  These produce what the reference's callers
  * feed the builders: sys::System (system.hpp:201-226) plus build_map
@@ -108,8 +115,20 @@ int taco_system_build(const double* H, int N, const double* A_lab, int num_ops, 
 /* C(t_n), n = 0..N (N+1 samples); beta may be +inf (zero temperature). */
 int taco_bcf_fft(size_t N, double dt, int kind, double p0, double p1, double beta, double* C_out);
 
-/* Gamma[t + Nc*b] = trapz prefix of exp(i omega_b tau) C(tau), Gamma[0] = 0. */
+/* Gamma[t + Nc*b] = trapz prefix of exp(i omega_b tau) C(tau), Gamma[0] = 0.
+ * The phase step exp(i omega_b dt) is the correctly rounded cos/sin pair. */
 int taco_gamma_trapz(const double* C, size_t Nc, double dt, const double* omegas, int nf, double* gamma_out);
+
+/* GPU twins of the two producers above (bcf_fft_fun, correlation_fft.hpp:154-214;
+ * compute_trapz_prefix_multi_matrix, gamma.hpp:166-213), bit-identical to
+ * them.  taco_bath_gamma_hip: C(t_n), n = 0..N, into C_out (optional) and,
+ * when nf > 0, Gamma[t + (N+1)*b] into gamma_out, without a host round trip
+ * between the two; beta may be +inf.  taco_gamma_trapz_hip: Gamma from a
+ * given C.  Host buffers; synchronous. */
+int taco_bath_gamma_hip(taco_ctx* ctx, size_t N, double dt, int kind, double p0, double p1, double beta,
+                        const double* omegas, int nf, double* C_out, double* gamma_out);
+int taco_gamma_trapz_hip(taco_ctx* ctx, const double* C, size_t Nc, double dt, const double* omegas, int nf,
+                         double* gamma_out);
 
 /* --------------------------------------------------------------------------
  * Hot path (GPU).

@@ -22,6 +22,8 @@ fcr_series (Convolution)    the reference's own Direct method cross-check
 fcr_series (Direct)         brute-force quadrature sums written from tcl4_kernels.m
 triple_kernels              per-triple equality with fcr_series; slicing (keep < Nt)
                             equals the head of the full series bitwise
+bath exp (mode CR)          70-digit decimal exp on random, large and subnormal-range
+                            arguments (correctly rounded exp of J and the KMS factors)
 bcf_fft                     brute-force DFT of the KMS spectrum (bcf_fft_ohmic_simple.m)
 gamma_trapz / gamma_rect    geometric-series closed form for C(t) = exp(-g t)
 mikx                        make_val fixture of cpp/tests/tcl4_mikx_tests.cpp:49-160
@@ -100,6 +102,7 @@ def lib():
             "orc_get_phase_mode": (c_int, []),
             "orc_phases": (None, [c_double, c_double, c_size_t, _dp]),
             "orc_cr_sincos": (None, [_dp, c_size_t, _dp, _dp]),
+            "orc_cr_exp": (None, [_dp, c_size_t, _dp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -169,6 +172,13 @@ class phase_mode:
 def phases(omega: float, dt: float, n: int) -> np.ndarray:
     out = np.zeros(n, np.complex128)
     lib().orc_phases(omega, dt, n, _p(out))
+    return out
+
+
+def cr_exp(x):
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.zeros_like(x)
+    lib().orc_cr_exp(_p(x), x.size, _p(out))
     return out
 
 

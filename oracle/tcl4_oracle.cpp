@@ -100,9 +100,21 @@ std::size_t fcr_nfft(std::size_t N, std::size_t pad) {
 // kind 1: Drude–Lorentz J = 2*lambda*gamma*w/(w^2+gamma^2)
 // Both vanish for w <= 0, as the reference lambdas do.
 // ---------------------------------------------------------------------------
+// Exponential of the bath correlation: std::exp as the reference calls it
+// (phase mode 0) or the correctly rounded value (mode 1, the engine's
+// convention): exp in quad precision rounded once to double (the quad result
+// is within ~2^-112 of exp(x), so the rounding is correct unless exp(x) lies
+// that close to a rounding boundary; also correct in the subnormal range).
+extern int g_phase_mode;
+cd phase(double arg);
+double bath_exp(double x) {
+    if (g_phase_mode == 0) return std::exp(x);
+    return static_cast<double>(expq(static_cast<__float128>(x)));
+}
+
 double spectral_J(int kind, double p0, double p1, double w) {
     if (!(w > 0.0)) return 0.0;
-    if (kind == 0) return p0 * w * std::exp(-w / p1);
+    if (kind == 0) return p0 * w * bath_exp(-w / p1);
     return 2.0 * p0 * p1 * w / (w * w + p1 * p1);
 }
 
@@ -125,7 +137,7 @@ void bcf_fft(std::size_t N, double dt, int kind, double p0, double p1, double be
         S[0] = cd(0.0, 0.0);
         S[Nfft / 2] = cd(0.25 * PI * spectral_J(kind, p0, p1, omegaNy), 0.0);
     } else {
-        const double r = std::exp(-beta * domega);
+        const double r = bath_exp(-beta * domega);
         double KMS = r;
         for (std::size_t k = 1; k <= posCount; ++k) {
             const double w = domega * static_cast<double>(k);
@@ -137,7 +149,7 @@ void bcf_fft(std::size_t N, double dt, int kind, double p0, double p1, double be
         }
         const double ed = r;
         S[0] = cd(0.5 * (PI / 2.0) * spectral_J(kind, p0, p1, domega) * (1.0 + ed) / (1.0 - ed), 0.0);
-        const double eNy = std::exp(beta * omegaNy);
+        const double eNy = bath_exp(beta * omegaNy);
         S[Nfft / 2] = cd(0.25 * PI * spectral_J(kind, p0, p1, omegaNy) * (1.0 + 2.0 / (eNy - 1.0)), 0.0);
     }
     fft_ref(S.data(), Nfft, true);
@@ -152,7 +164,7 @@ void gamma_trapz(const cd* C, std::size_t N, double dt, const double* omegas, in
     for (int j = 0; j < M; ++j) {
         cd* col = G + static_cast<std::size_t>(j) * N;
         col[0] = cd(0.0, 0.0);
-        const cd step = std::exp(cd{0.0, omegas[j] * dt});
+        const cd step = phase(omegas[j] * dt);
         cd phi(1.0, 0.0), acc(0.0, 0.0), prev = C[0];
         for (std::size_t k = 1; k < N; ++k) {
             const cd phi_next = phi * step;
@@ -740,6 +752,11 @@ void orc_phases(double omega, double dt, size_t n, double* out) {
 // Correctly rounded sin/cos of each x (for pinning against quad precision).
 void orc_cr_sincos(const double* x, size_t n, double* s, double* c) {
     for (std::size_t k = 0; k < n; ++k) cr_sincos(x[k], s + k, c + k);
+}
+
+// Correctly rounded exp of each x (quad precision rounded to double).
+void orc_cr_exp(const double* x, size_t n, double* out) {
+    for (std::size_t k = 0; k < n; ++k) out[k] = static_cast<double>(expq(static_cast<__float128>(x[k])));
 }
 
 }  // extern "C"

@@ -42,6 +42,10 @@ SIGNATURES = {
     "taco_ctx_stream": (c_void_p, [c_void_p]),
     "taco_set_fcr_fft_pad_factor": (None, [c_size_t]),
     "taco_cr_sincos": (c_int, [c_void_p, _dp, c_size_t, _dp, _dp]),
+    "taco_cr_exp": (c_int, [c_void_p, _dp, c_size_t, _dp]),
+    "taco_bath_gamma_hip": (c_int, [c_void_p, c_size_t, c_double, c_int, c_double, c_double, c_double, _dp, c_int,
+                                    _dp, _dp]),
+    "taco_gamma_trapz_hip": (c_int, [c_void_p, _dp, c_size_t, c_double, _dp, c_int, _dp]),
     "taco_profile_enable": (c_int, [c_void_p, c_int]),
     "taco_fft_tile_choice": (c_int, [c_void_p, c_int, _ip]),
     "taco_profile_reset": (c_int, [c_void_p]),

@@ -32,7 +32,8 @@ import numpy as np
 import torch
 
 from ._lib import TacoInvalidArgument, check, dptr, iptr, lib
-from .tcl4 import Exec, Ohmic, DrudeLorentz, System, _ctx_for, bcf_fft_fun, compute_trapz_prefix_multi_matrix
+from .tcl4 import (Exec, Ohmic, DrudeLorentz, System, _ctx_for, bath_gamma_hip, bcf_fft_fun,
+                   compute_trapz_prefix_multi_matrix, compute_trapz_prefix_multi_matrix_hip)
 
 __all__ = ["Tape", "TCL4Propagate", "BcfFun", "GammaFun", "EighFun", "BucketOmegas", "SpinPairProblem"]
 
@@ -150,9 +151,16 @@ class BcfFun(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, p0, p1, beta, meta):
-        kind, N, dt = meta
+        # meta = (kind, N, dt[, producer]): producer an Exec runs the device
+        # producer on its GPU, "host" (the default) the host bcf_fft_fun; the
+        # two are bit-identical
+        kind, N, dt = meta[:3]
+        producer = meta[3] if len(meta) > 3 else "host"
         bath = Ohmic(float(p0), float(p1)) if kind == 0 else DrudeLorentz(float(p0), float(p1))
-        _, C = bcf_fft_fun(N, dt, bath, float(beta))
+        if isinstance(producer, Exec):
+            C, _ = bath_gamma_hip(N, dt, bath, float(beta), exec=producer)
+        else:
+            _, C = bcf_fft_fun(N, dt, bath, float(beta))
         ctx.save = (kind, N, dt, float(p0), float(p1), float(beta))
         return torch.from_numpy(C)
 
@@ -208,10 +216,14 @@ class GammaFun(torch.autograd.Function):
     as a function of C [Nt] and w [nf]."""
 
     @staticmethod
-    def forward(ctx, C, omegas, dt):
+    def forward(ctx, C, omegas, dt, producer="host"):
+        # producer: an Exec (device trapz on its GPU) or "host"; bit-identical
         Cn = C.detach().numpy()
         om = omegas.detach().numpy()
-        G = compute_trapz_prefix_multi_matrix(Cn, dt, om)
+        if isinstance(producer, Exec):
+            G = compute_trapz_prefix_multi_matrix_hip(Cn, dt, om, exec=producer)
+        else:
+            G = compute_trapz_prefix_multi_matrix(Cn, dt, om)
         ctx.save = (Cn, om, dt)
         return torch.from_numpy(G)
 
@@ -238,7 +250,7 @@ class GammaFun(torch.autograd.Function):
             Cbar[:T] = (dt / 2.0) * np.sum(np.conj(phi) * Q, axis=1)
             dphi = (1j * (m * dt))[:, None] * phi * Cn[:T, None]
             wbar = (dt / 2.0) * np.real(np.sum(np.conj(Q) * dphi, axis=0))
-        return torch.from_numpy(Cbar), torch.from_numpy(wbar), None
+        return torch.from_numpy(Cbar), torch.from_numpy(wbar), None, None
 
 
 class EighFun(torch.autograd.Function):
@@ -347,8 +359,10 @@ def _rho_lab(prob, H: torch.Tensor, A: np.ndarray, kind: int, p0, p1, beta, exec
     e, U = EighFun.apply(H, [A], prob.freq_tol)
     S = System.build(H.detach().numpy(), [A], prob.freq_tol)
     om = BucketOmegas.apply(e, S.omegas, S.pair_to_freq)
-    C = BcfFun.apply(p0, p1, beta, (kind, prob.Nt - 1, prob.dt))
-    G = GammaFun.apply(C, om, prob.dt)
+    # bath correlation and Gamma from the device producers
+    producer = exec or Exec()
+    C = BcfFun.apply(p0, p1, beta, (kind, prob.Nt - 1, prob.dt, producer))
+    G = GammaFun.apply(C, om, prob.dt, producer)
     At = torch.from_numpy(A)
     Aeig = (U.conj().T @ At @ U).unsqueeze(0)
     N = S.N

@@ -46,6 +46,7 @@
 #pragma clang fp contract(off)
 
 #include "cr_sincos.hpp"
+#include "cr_exp.hpp"
 
 namespace taco_engine {
 
@@ -74,6 +75,8 @@ __device__ __forceinline__ double2 cscale(double2 a, double s) { return make_dou
 
 // sin/cos(j/64) double-double table of the correctly rounded phase routine.
 __constant__ double c_crtab[4 * taco_cr::kTabN];
+// exp(j/64), 1/n! and ln2 table of the correctly rounded exp.
+__constant__ double c_exptab[taco_cr::kExpTabSize];
 
 // exp(i * arg) with correctly rounded cos/sin.
 __device__ __forceinline__ double2 cexpi(double arg) {
@@ -621,6 +624,167 @@ __global__ __launch_bounds__(kBlock) void k_combine(FcrArgs P, CombineArgs O) {
 __global__ void k_cr_sincos(const double* x, long n, double* s, double* c) {
     const long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
     if (i < n) taco_cr::sincos_cr(x[i], s + i, c + i, c_crtab);
+}
+
+__global__ void k_cr_exp(const double* x, long n, double* out) {
+    const long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+    if (i < n) out[i] = taco_cr::exp_cr(x[i], c_exptab);
+}
+
+// ---------------------------------------------------------------------------
+// Bath correlation function and trapz Gamma on the device: the twins of the
+// host producers (bcf_fft_fun, correlation_fft.hpp:154-214, and
+// compute_trapz_prefix_multi_matrix, gamma.hpp:166-213), bit-identical to
+// them.  Their two recurrences (the KMS factor r^k by repeated products and
+// the trapz phase / sum chains) are sequential in the reference's arithmetic,
+// so one lane runs each chain; everything else is parallel.
+// ---------------------------------------------------------------------------
+struct BcfArgs {
+    int kind;
+    double p0, p1, beta;   // beta = +inf: zero temperature
+    double domega, omegaNy, r, eNy;
+    long nfft;
+    const double* kms;     // [nfft / 2] KMS factor used at k (finite beta)
+    double2* S;            // [nfft] sampled spectrum
+};
+
+__device__ __forceinline__ double spectral_J_dev(int kind, double p0, double p1, double w) {
+    if (!(w > 0.0)) return 0.0;
+    if (kind == 0) return p0 * w * taco_cr::exp_cr(-w / p1, c_exptab);
+    return 2.0 * p0 * p1 * w / (w * w + p1 * p1);
+}
+
+// kms[k] = r^k for k = 1..posCount by the reference's recurrence KMS *= r.
+__global__ void k_bcf_kms(double r, long pos_count, double* kms) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double v = r;
+    for (long k = 1; k <= pos_count; ++k) {
+        kms[k] = v;
+        v *= r;
+    }
+}
+
+// S[k], k < nfft: the KMS spectrum of bcf_fft_fun, one entry per thread.
+__global__ __launch_bounds__(kBlock) void k_bcf_spectrum(BcfArgs B) {
+    const long k = blockIdx.x * static_cast<long>(kBlock) + threadIdx.x;
+    if (k >= B.nfft) return;
+    const long half = B.nfft / 2, pos_count = half - 1;
+    constexpr double kPiD = 3.14159265358979323846;
+    double v = 0.0;
+    const bool zero_t = isinf(B.beta);
+    if (k == half) {
+        const double J = spectral_J_dev(B.kind, B.p0, B.p1, B.omegaNy);
+        v = zero_t ? 0.25 * kPiD * J : 0.25 * kPiD * J * (1.0 + 2.0 / (B.eNy - 1.0));
+    } else if (k == 0) {
+        if (!zero_t) {
+            const double ed = B.r;
+            v = 0.5 * (kPiD / 2.0) * spectral_J_dev(B.kind, B.p0, B.p1, B.domega) * (1.0 + ed) / (1.0 - ed);
+        }
+    } else if (k <= pos_count) {
+        const double w = B.domega * static_cast<double>(k);
+        v = zero_t ? (kPiD / 2.0) * spectral_J_dev(B.kind, B.p0, B.p1, w)
+                   : (kPiD / 2.0) * spectral_J_dev(B.kind, B.p0, B.p1, w) / (1.0 - B.kms[k]);
+    } else if (!zero_t) {
+        const long kk = B.nfft - k;  // 1..posCount
+        const double w = B.domega * static_cast<double>(kk);
+        const double Sp = (kPiD / 2.0) * spectral_J_dev(B.kind, B.p0, B.p1, w) / (1.0 - B.kms[kk]);
+        v = Sp * B.kms[kk];
+    }
+    B.S[k] = make_double2(v, 0.0);
+}
+
+// C[n] = scale * X[n], n < nc, from the phase-B spectrum layout
+// spec[k1 * 2^b + p] = X[k1 + p * 2^a].
+__global__ __launch_bounds__(kBlock) void k_bcf_gather(const double2* spec, int a, int b, long nc, double scale,
+                                                       double2* C) {
+    const long n = blockIdx.x * static_cast<long>(kBlock) + threadIdx.x;
+    if (n >= nc) return;
+    const long src = ((n & ((1L << a) - 1)) << b) + (n >> a);
+    const double2 x = spec[src];
+    C[n] = make_double2(x.x * scale, x.y * scale);
+}
+
+// Gamma[j][k] = trapz prefix of exp(i omega_j tau) C(tau), one workgroup per
+// bucket, with the reference's chains (phi_k = phi_{k-1} step; acc_k =
+// acc_{k-1} + dt/2 (phi_{k-1} C_{k-1} + phi_k C_k)) split so that only the
+// two recurrences are sequential.  Per chunk of kTzChunk times, in a
+// pipeline: lane 0 of wave 0 runs the phase chain of chunk c, lane 0 of wave
+// 1 the sum chain of chunk c - 1, waves 2-3 load C of chunk c and write Gamma
+// of chunk c - 2; then every thread forms chunk c's terms, each exactly the
+// reference's expression.
+constexpr int kTzChunk = 1024;
+__global__ __launch_bounds__(kBlock) void k_gamma_trapz(const double2* C, long nc, double dt, const double* omegas,
+                                                        double2* G) {
+    __shared__ double2 cin[2][kTzChunk];   // C of chunk c (c % 2)
+    __shared__ double2 ph[2][kTzChunk];    // phi_k of chunk c
+    __shared__ double2 uu[2][kTzChunk];    // terms of chunk c
+    __shared__ double2 gout[2][kTzChunk];  // Gamma of chunk c
+    __shared__ double2 last_phi[2], last_c[2];
+    double2* col = G + static_cast<long>(blockIdx.x) * nc;
+    const double2 half_dt = make_double2(dt / 2.0, 0.0);
+    double2 step = make_double2(1.0, 0.0), phi = make_double2(1.0, 0.0), acc = make_double2(0.0, 0.0);
+    if (threadIdx.x == 0) step = cexpi(omegas[blockIdx.x] * dt);
+    const long nch = (nc + kTzChunk - 1) / kTzChunk;
+    for (long c = 0; c <= nch + 1; ++c) {
+        const int cb = static_cast<int>(c & 1), pb = cb ^ 1;
+        if (threadIdx.x == 0) {
+            if (c < nch) {  // phase chain of chunk c
+                const long k0 = c * kTzChunk;
+                const int e_end = static_cast<int>(min(static_cast<long>(kTzChunk), nc - k0));
+                int e = 0;
+                if (k0 == 0) {
+                    ph[cb][0] = phi;  // phi_0 = 1
+                    e = 1;
+                }
+                for (; e < e_end; ++e) {
+                    phi = cmul(phi, step);
+                    ph[cb][e] = phi;
+                }
+            }
+        } else if (threadIdx.x == 64) {
+            if (c >= 1 && c - 1 < nch) {  // sum chain of chunk c - 1
+                const long k0 = (c - 1) * kTzChunk;
+                const int e_end = static_cast<int>(min(static_cast<long>(kTzChunk), nc - k0));
+                int e = 0;
+                if (k0 == 0) {
+                    gout[pb][0] = make_double2(0.0, 0.0);
+                    e = 1;
+                }
+#pragma unroll 8
+                for (; e < e_end; ++e) {
+                    acc = cadd(acc, uu[pb][e]);
+                    gout[pb][e] = acc;
+                }
+            }
+        } else if (threadIdx.x >= 128) {
+            if (c < nch)
+                for (int e = threadIdx.x - 128; e < kTzChunk; e += kBlock - 128) {
+                    const long k = c * kTzChunk + e;
+                    cin[cb][e] = k < nc ? C[k] : make_double2(0.0, 0.0);
+                }
+            if (c >= 2)
+                for (int e = threadIdx.x - 128; e < kTzChunk; e += kBlock - 128) {
+                    const long k = (c - 2) * kTzChunk + e;
+                    if (k < nc) col[k] = gout[cb][e];
+                }
+        }
+        __syncthreads();
+        if (c < nch) {  // terms of chunk c
+            const long k0 = c * kTzChunk;
+            const int e_end = static_cast<int>(min(static_cast<long>(kTzChunk), nc - k0));
+            for (int e = threadIdx.x; e < e_end; e += kBlock) {
+                if (k0 + e == 0) continue;
+                const double2 pm = e > 0 ? ph[cb][e - 1] : last_phi[pb];
+                const double2 cm = e > 0 ? cin[cb][e - 1] : last_c[pb];
+                uu[cb][e] = cmul(half_dt, cadd(cmul(pm, cm), cmul(ph[cb][e], cin[cb][e])));
+            }
+            if (threadIdx.x == 0 && e_end == kTzChunk) {
+                last_phi[cb] = ph[cb][kTzChunk - 1];
+                last_c[cb] = cin[cb][kTzChunk - 1];
+            }
+        }
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2539,6 +2703,80 @@ static void tape_adjoint(taco_tape& tp, const double* samples_bar, double* gamma
 
 }  // namespace taco_engine
 
+namespace taco_engine {
+constexpr double kPiHost = 3.141592653589793238462643383279502884;
+
+// Bath correlation C(t_n), n <= N, into device memory dC (N + 1 values): the
+// sampled KMS spectrum, its forward FFT by the three-pass kernels (the
+// reference's radix-2 recurrence FFT) and the scale dw/pi.
+static void bcf_device(taco_ctx* c, std::size_t N, double dt, int kind, double p0, double p1, double beta, double2* dC) {
+    if (!(dt > 0.0)) throw Error(TACO_ERR_INVALID, "bcf_fft: dt must be > 0");
+    if (kind != 0 && kind != 1) throw Error(TACO_ERR_INVALID, "bcf_fft: unknown spectral density kind");
+    const std::size_t Nt = N + 1;
+    std::size_t Nfft = 2 * Nt;
+    if ((Nfft & (Nfft - 1)) != 0) Nfft = next_pow2(Nfft);
+    int L = 0;
+    while ((std::size_t{1} << L) < Nfft) ++L;
+    if (L > 22) throw Error(TACO_ERR_INVALID, "bcf_fft: FFT length above 2^22 (N above 2^21) is not supported");
+    const double tf = dt * static_cast<double>(Nfft);
+    const double domega = 2.0 * kPiHost / tf;
+    const double omegaNy = kPiHost / dt;
+    const bool zero_t = std::isinf(beta);
+    const long half = static_cast<long>(Nfft / 2);
+    c->misc.ensure(Nfft * 16 + static_cast<std::size_t>(half + 1) * 8);
+    BcfArgs B{};
+    B.kind = kind;
+    B.p0 = p0;
+    B.p1 = p1;
+    B.beta = beta;
+    B.domega = domega;
+    B.omegaNy = omegaNy;
+    B.r = zero_t ? 0.0 : taco_cr::exp_cr(-beta * domega, exp_table());
+    B.eNy = zero_t ? 0.0 : taco_cr::exp_cr(beta * omegaNy, exp_table());
+    B.nfft = static_cast<long>(Nfft);
+    B.S = c->misc.as<double2>();
+    double* kms = reinterpret_cast<double*>(B.S + Nfft);
+    B.kms = kms;
+    if (!zero_t) hipLaunchKernelGGL(k_bcf_kms, dim3(1), dim3(64), 0, c->stream, B.r, half - 1, kms);
+    hipLaunchKernelGGL(k_bcf_spectrum, dim3(static_cast<unsigned>((Nfft + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       c->stream, B);
+    const Twiddles& tw = twiddles_for(c, L);
+    FcrArgs P{};
+    P.gamma = B.S;
+    P.Nt = static_cast<long>(Nfft);
+    P.L = L;
+    P.a = (L + 1) / 2;
+    P.b = L - P.a;
+    P.n = static_cast<long>(Nfft);
+    P.twStd = tw.std_.as<double2>();
+    P.twB = tw.b_.as<double2>();
+    P.twC = tw.c_.as<double2>();
+    c->inter1.ensure(Nfft * 16);
+    c->spec.ensure(Nfft * 16);
+    P.inter1 = c->inter1.as<double2>();
+    P.spec = c->spec.as<double2>();
+    P.njobs = 1;
+    auto it = c->fft_choice.find(L);
+    const int cfgA = it != c->fft_choice.end() && it->second[0] >= 0 ? it->second[0] : 2;
+    const int cfgB = it != c->fft_choice.end() && it->second[1] >= 0 ? it->second[1] : 2;
+    launch_phase(cfgA, PH_A_SPEC, P, 1, c->stream);
+    launch_phase(cfgB, PH_B_SPEC, P, 1, c->stream);
+    hipLaunchKernelGGL(k_bcf_gather, dim3(static_cast<unsigned>((Nt + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       c->stream, P.spec, P.a, P.b, static_cast<long>(Nt), domega / kPiHost, dC);
+    HIP_OK(hipGetLastError());
+}
+
+// Trapz Gamma [nf][Nc] from device C [Nc] and device bucket frequencies.
+static void trapz_device(taco_ctx* c, const double2* dC, std::size_t Nc, double dt, const double* d_om, int nf,
+                  double2* dG) {
+    if (!(dt > 0.0)) throw Error(TACO_ERR_INVALID, "gamma_trapz: dt must be > 0");
+    if (Nc == 0 || nf <= 0) return;
+    hipLaunchKernelGGL(k_gamma_trapz, dim3(static_cast<unsigned>(nf)), dim3(kBlock), 0, c->stream, dC,
+                       static_cast<long>(Nc), dt, d_om, dG);
+    HIP_OK(hipGetLastError());
+}
+}  // namespace taco_engine
+
 using namespace taco_engine;
 
 extern "C" {
@@ -2560,6 +2798,8 @@ int taco_ctx_create(int device, taco_ctx** out) {
         c->device = device;
         HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(c_crtab), cr_table(), sizeof(double) * 4 * taco_cr::kTabN, 0,
+                                 hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(c_exptab), exp_table(), sizeof(double) * taco_cr::kExpTabSize, 0,
                                  hipMemcpyHostToDevice));
         HIP_OK(hipEventCreate(&c->ev0));
         HIP_OK(hipEventCreate(&c->ev1));
@@ -2641,6 +2881,70 @@ int taco_cr_sincos(taco_ctx* ctx, const double* x, size_t n, double* s, double* 
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpyAsync(s, dx + n, n * 8, hipMemcpyDeviceToHost, ctx->stream));
         HIP_OK(hipMemcpyAsync(c, dx + 2 * n, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+
+int taco_cr_exp(taco_ctx* ctx, const double* x, size_t n, double* out) {
+    return guarded([&] {
+        if (!ctx) {
+            cr_exp_host(x, n, out);
+            return;
+        }
+        if (n == 0) return;
+        DeviceGuard g(ctx->device);
+        ctx->misc.ensure(2 * n * sizeof(double));
+        double* dx = ctx->misc.as<double>();
+        HIP_OK(hipMemcpyAsync(dx, x, n * 8, hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(k_cr_exp, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, ctx->stream, dx,
+                           static_cast<long>(n), dx + n);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(out, dx + n, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int taco_bath_gamma_hip(taco_ctx* ctx, size_t N, double dt, int kind, double p0, double p1, double beta,
+                        const double* omegas, int nf, double* C_out, double* gamma_out) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (nf < 0) throw Error(TACO_ERR_INVALID, "nf must be >= 0");
+        if (nf > 0 && (!omegas || !gamma_out)) throw Error(TACO_ERR_INVALID, "omegas and gamma_out are required");
+        DeviceGuard g(ctx->device);
+        const std::size_t Nt = N + 1;
+        ctx->inter2.ensure((Nt + static_cast<std::size_t>(nf) * Nt) * 16 + static_cast<std::size_t>(nf + 1) * 8);
+        double2* dC = ctx->inter2.as<double2>();
+        double2* dG = dC + Nt;
+        double* d_om = reinterpret_cast<double*>(dG + static_cast<std::size_t>(nf) * Nt);
+        bcf_device(ctx, N, dt, kind, p0, p1, beta, dC);
+        if (nf > 0) {
+            HIP_OK(hipMemcpyAsync(d_om, omegas, static_cast<std::size_t>(nf) * 8, hipMemcpyHostToDevice, ctx->stream));
+            trapz_device(ctx, dC, Nt, dt, d_om, nf, dG);
+            HIP_OK(hipMemcpyAsync(gamma_out, dG, static_cast<std::size_t>(nf) * Nt * 16, hipMemcpyDeviceToHost,
+                                  ctx->stream));
+        }
+        if (C_out) HIP_OK(hipMemcpyAsync(C_out, dC, Nt * 16, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int taco_gamma_trapz_hip(taco_ctx* ctx, const double* C, size_t Nc, double dt, const double* omegas, int nf,
+                         double* gamma_out) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (nf < 0) throw Error(TACO_ERR_INVALID, "nf must be >= 0");
+        if (Nc == 0 || nf == 0) return;
+        DeviceGuard g(ctx->device);
+        ctx->inter2.ensure((Nc + static_cast<std::size_t>(nf) * Nc) * 16 + static_cast<std::size_t>(nf + 1) * 8);
+        double2* dC = ctx->inter2.as<double2>();
+        double2* dG = dC + Nc;
+        double* d_om = reinterpret_cast<double*>(dG + static_cast<std::size_t>(nf) * Nc);
+        HIP_OK(hipMemcpyAsync(dC, C, Nc * 16, hipMemcpyHostToDevice, ctx->stream));
+        HIP_OK(hipMemcpyAsync(d_om, omegas, static_cast<std::size_t>(nf) * 8, hipMemcpyHostToDevice, ctx->stream));
+        trapz_device(ctx, dC, Nc, dt, d_om, nf, dG);
+        HIP_OK(hipMemcpyAsync(gamma_out, dG, static_cast<std::size_t>(nf) * Nc * 16, hipMemcpyDeviceToHost,
+                              ctx->stream));
         HIP_OK(hipStreamSynchronize(ctx->stream));
     });
 }

@@ -43,6 +43,8 @@ void tcl2_ops_adjoint(const Tcl2Parts& parts, const std::complex<double>* W, con
 void set_last_error(const std::string& msg);
 const double* cr_table();
 void cr_sincos_host(const double* x, std::size_t n, double* s, double* c);
+const double* exp_table();
+void cr_exp_host(const double* x, std::size_t n, double* out);
 
 template <class Fn>
 int guarded(Fn&& fn) {

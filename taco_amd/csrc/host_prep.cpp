@@ -11,6 +11,7 @@
 // is compiled with -ffp-contract=off so no product is fused into an add.
 
 #include "engine_internal.hpp"
+#include "cr_exp.hpp"
 
 #include <quadmath.h>
 
@@ -105,9 +106,12 @@ static void fft_host(std::vector<cd>& a, bool forward) {
     }
 }
 
+// The exponentials of the bath correlation (J's Ohmic factor, the KMS
+// factors) and the trapz phase step are correctly rounded (cr_exp.hpp,
+// cr_sincos.hpp), the values the device producer computes with the same code.
 static double spectral_J(int kind, double p0, double p1, double w) {
     if (!(w > 0.0)) return 0.0;
-    if (kind == 0) return p0 * w * std::exp(-w / p1);
+    if (kind == 0) return p0 * w * taco_cr::exp_cr(-w / p1, exp_table());
     return 2.0 * p0 * p1 * w / (w * w + p1 * p1);
 }
 
@@ -129,7 +133,7 @@ void bcf_fft(std::size_t N, double dt, int kind, double p0, double p1, double be
         }
         S[Nfft / 2] = cd(0.25 * kPi * spectral_J(kind, p0, p1, omegaNy), 0.0);
     } else {
-        const double r = std::exp(-beta * domega);
+        const double r = taco_cr::exp_cr(-beta * domega, exp_table());
         double KMS = r;
         for (std::size_t k = 1; k <= posCount; ++k) {
             const double w = domega * static_cast<double>(k);
@@ -140,7 +144,7 @@ void bcf_fft(std::size_t N, double dt, int kind, double p0, double p1, double be
         }
         const double ed = r;
         S[0] = cd(0.5 * (kPi / 2.0) * spectral_J(kind, p0, p1, domega) * (1.0 + ed) / (1.0 - ed), 0.0);
-        const double eNy = std::exp(beta * omegaNy);
+        const double eNy = taco_cr::exp_cr(beta * omegaNy, exp_table());
         S[Nfft / 2] = cd(0.25 * kPi * spectral_J(kind, p0, p1, omegaNy) * (1.0 + 2.0 / (eNy - 1.0)), 0.0);
     }
     fft_host(S, true);
@@ -160,7 +164,9 @@ void gamma_trapz(const double* Cin, std::size_t N, double dt, const double* omeg
     for (int j = 0; j < M; ++j) {
         cd* col = G + static_cast<std::size_t>(j) * N;
         col[0] = cd(0.0, 0.0);
-        const cd step = std::exp(cd{0.0, omegas[j] * dt});
+        double ss, sc;
+        taco_cr::sincos_cr(omegas[j] * dt, &ss, &sc, cr_table());
+        const cd step(sc, ss);
         cd phi(1.0, 0.0), acc(0.0, 0.0), prev = C[0];
         for (std::size_t k = 1; k < N; ++k) {
             const cd phi_next = phi * step;
@@ -359,6 +365,38 @@ const double* cr_table() {
         return t;
     }();
     return tab.data();
+}
+
+// exp(j/64) (j = -23..23) and 1/n! (n <= 12) as double-doubles from quad
+// precision, and ln2 in three parts.
+const double* exp_table() {
+    static const std::vector<double> tab = [] {
+        std::vector<double> t(taco_cr::kExpTabSize);
+        auto put = [&](int at, __float128 v) {
+            const double h = static_cast<double>(v);
+            t[at] = h;
+            t[at + 1] = static_cast<double>(v - h);
+        };
+        for (int j = -taco_cr::kExpJ; j <= taco_cr::kExpJ; ++j)
+            put(2 * (j + taco_cr::kExpJ), expq(static_cast<__float128>(j) / 64));
+        __float128 f = 1;
+        for (int n = 0; n <= taco_cr::kExpDeg; ++n) {
+            if (n > 0) f *= n;
+            put(taco_cr::kExpTabInv + 2 * n, 1 / f);
+        }
+        // ln2 = L0 + L1 + L2, each the double nearest the remainder (from an
+        // 80-digit evaluation)
+        t[taco_cr::kExpTabLn2] = 0x1.62e42fefa39efp-1;
+        t[taco_cr::kExpTabLn2 + 1] = 0x1.abc9e3b39803fp-56;
+        t[taco_cr::kExpTabLn2 + 2] = 0x1.7b57a079a1934p-111;
+        return t;
+    }();
+    return tab.data();
+}
+
+void cr_exp_host(const double* x, std::size_t n, double* out) {
+    const double* tab = exp_table();
+    for (std::size_t k = 0; k < n; ++k) out[k] = taco_cr::exp_cr(x[k], tab);
 }
 
 void cr_sincos_host(const double* x, std::size_t n, double* s, double* c) {

@@ -34,7 +34,8 @@ from ._lib import TacoInvalidArgument, check, dptr, iptr, lib, sptr
 
 __all__ = [
     "Backend", "Exec", "FCRMethod", "System", "Tcl4Map", "TripleKernelSeries", "Ohmic", "DrudeLorentz",
-    "build_map", "bcf_fft_fun", "compute_trapz_prefix_multi_matrix", "compute_triple_kernels",
+    "build_map", "bcf_fft_fun", "compute_trapz_prefix_multi_matrix", "bath_gamma_hip",
+    "compute_trapz_prefix_multi_matrix_hip", "compute_triple_kernels",
     "build_TCL4_generator_hip_fused", "build_TCL4_generator_hip_fused_batch", "build_correction_series_hip_fused",
     "build_TCL4_generator_cuda_fused", "build_TCL4_generator_cuda_fused_batch", "build_correction_series_cuda_fused",
     "build_TCL4_generator", "build_correction_series", "assemble_l4_from_slices", "propagate_rk4_dense",
@@ -242,6 +243,31 @@ def compute_trapz_prefix_multi_matrix(C, dt: float, omegas) -> np.ndarray:
     om = np.ascontiguousarray(omegas, np.float64)
     G = np.zeros((len(om), C.size), np.complex128)
     check(lib().taco_gamma_trapz(dptr(C), C.size, dt, dptr(om), len(om), dptr(G)))
+    return G.T
+
+
+def bath_gamma_hip(N: int, dt: float, J, beta: float, omegas=None, exec: Optional[Exec] = None):
+    """GPU twin of bcf_fft_fun + compute_trapz_prefix_multi_matrix, bit-identical
+    to them: returns (C[0..N], Gamma [N+1, nf]) (Gamma None without omegas)."""
+    ctx = _ctx_for(exec)
+    C = np.zeros(N + 1, np.complex128)
+    p0, p1 = J.params
+    if omegas is None:
+        check(lib().taco_bath_gamma_hip(ctx.handle, N, dt, J.kind, p0, p1, beta, None, 0, dptr(C), None))
+        return C, None
+    om = np.ascontiguousarray(omegas, np.float64)
+    G = np.zeros((len(om), N + 1), np.complex128)
+    check(lib().taco_bath_gamma_hip(ctx.handle, N, dt, J.kind, p0, p1, beta, dptr(om), len(om), dptr(C), dptr(G)))
+    return C, G.T
+
+
+def compute_trapz_prefix_multi_matrix_hip(C, dt: float, omegas, exec: Optional[Exec] = None) -> np.ndarray:
+    """GPU twin of compute_trapz_prefix_multi_matrix (bit-identical): Gamma [Nt, nf]."""
+    ctx = _ctx_for(exec)
+    C = np.ascontiguousarray(C, np.complex128)
+    om = np.ascontiguousarray(omegas, np.float64)
+    G = np.zeros((len(om), C.size), np.complex128)
+    check(lib().taco_gamma_trapz_hip(ctx.handle, dptr(C), C.size, dt, dptr(om), len(om), dptr(G)))
     return G.T
 
 

@@ -100,3 +100,49 @@ def test_reference_libm_sensitivity_at_early_t(orc):
     e, _ = per_t_rel_err(a, b)
     assert e[0] > 1e-12          # t = 0: pure rounding noise, libm-dependent
     assert e[5:].max() < 1e-13   # later t: libm-independent
+
+
+def decimal_exp_rounded(x: float) -> float:
+    getcontext().prec = 80
+    return float(Decimal(x).exp())  # Decimal -> float rounds correctly (also to subnormals)
+
+
+def exp_args(rng, n_random):
+    return np.concatenate([
+        rng.uniform(-745.2, 709.8, n_random), rng.uniform(-3, 3, n_random // 2),
+        rng.uniform(-745.2, -707.0, n_random // 8),      # subnormal results
+        -rng.uniform(0, 1, n_random // 4) * 200.0,       # -w / omega_c of Ohmic spectra
+        [0.0, -0.0, 1.0, -1.0, 709.782712893384, 709.7827128933841, -745.1332191019412,
+         -745.1332191019411, -744.44007192138126, -708.3964185322641, 1e-300, -1e-300, 5e-324]])
+
+
+def test_oracle_cr_exp_matches_decimal(orc):
+    """The bath correlation's exponentials (the Ohmic factor exp(-w/w_c) and
+    the KMS factors) use the correctly rounded exp; the oracle's (quad
+    precision rounded once) against a 80-digit decimal evaluation."""
+    rng = np.random.default_rng(3)
+    xs = exp_args(rng, 400)
+    got = orc.cr_exp(xs)
+    for x, g in zip(xs, got):
+        assert g == decimal_exp_rounded(float(x)), x
+
+
+def host_exp(x):
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.zeros_like(x)
+    _lib.check(_lib.lib().taco_cr_exp(None, _lib.dptr(x), x.size, _lib.dptr(out)))
+    return out
+
+
+def test_engine_cr_exp_bitwise_equals_oracle(orc):
+    """The engine's double-double exp (host build of the device code) equals
+    the oracle's bit for bit, including the subnormal range, overflow and the
+    arguments of the bench bath (w = k dw up to the Nyquist frequency)."""
+    rng = np.random.default_rng(4)
+    dw = 2 * np.pi / (0.002 * 2 ** 19)
+    grid = -(np.arange(1, 2 ** 18) * dw) / 10.0
+    xs = np.concatenate([exp_args(rng, 400000), grid])
+    a, b = host_exp(xs), orc.cr_exp(xs)
+    bad = np.flatnonzero(a != b)
+    assert bad.size == 0, (xs[bad[:5]], a[bad[:5]], b[bad[:5]])
+    assert host_exp(np.array([710.0]))[0] == np.inf and host_exp(np.array([-746.0]))[0] == 0.0
