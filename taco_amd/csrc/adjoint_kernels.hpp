@@ -8,14 +8,27 @@
 #pragma once
 
 // y = L^H x (column r of L conjugated, column stride ld), summed over c in order.
+// (General D: column blocks of kMvBlk loads issued ahead, as in matvec_rows.)
 template <int DT>
 __device__ __forceinline__ void matvec_h(const double2* __restrict__ L, int ld, const double2* x, double2* y, int D) {
     const int n = DT ? DT : D;
     for (int r = threadIdx.x; r < n; r += blockDim.x) {
         double2 s = make_double2(0.0, 0.0);
         const double2* col = L + static_cast<long>(r) * ld;
+        if constexpr (DT == 0) {
+            int c = 0;
+            for (; c + kMvBlk <= n; c += kMvBlk) {
+                double2 v[kMvBlk];
 #pragma unroll
-        for (int c = 0; c < (DT ? DT : n); ++c) s = cadd(s, cmul(cconj(col[c]), x[c]));
+                for (int j = 0; j < kMvBlk; ++j) v[j] = col[c + j];
+#pragma unroll
+                for (int j = 0; j < kMvBlk; ++j) s = cadd(s, cmul(cconj(v[j]), x[c + j]));
+            }
+            for (; c < n; ++c) s = cadd(s, cmul(cconj(col[c]), x[c]));
+        } else {
+#pragma unroll
+            for (int c = 0; c < DT; ++c) s = cadd(s, cmul(cconj(col[c]), x[c]));
+        }
         y[r] = s;
     }
 }
@@ -28,9 +41,24 @@ __device__ __forceinline__ void matvec_h_half(const double2* __restrict__ L0, co
     for (int r = threadIdx.x; r < n; r += blockDim.x) {
         double2 s = make_double2(0.0, 0.0);
         const long o = static_cast<long>(r) * ld;
+        if constexpr (DT == 0) {
+            int c = 0;
+            for (; c + kMvBlk <= n; c += kMvBlk) {
+                double2 v0[kMvBlk], v1[kMvBlk];
 #pragma unroll
-        for (int c = 0; c < (DT ? DT : n); ++c)
-            s = cadd(s, cmul(cconj(cscale(cadd(L0[o + c], L1[o + c]), 0.5)), x[c]));
+                for (int j = 0; j < kMvBlk; ++j) {
+                    v0[j] = L0[o + c + j];
+                    v1[j] = L1[o + c + j];
+                }
+#pragma unroll
+                for (int j = 0; j < kMvBlk; ++j) s = cadd(s, cmul(cconj(cscale(cadd(v0[j], v1[j]), 0.5)), x[c + j]));
+            }
+            for (; c < n; ++c) s = cadd(s, cmul(cconj(cscale(cadd(L0[o + c], L1[o + c]), 0.5)), x[c]));
+        } else {
+#pragma unroll
+            for (int c = 0; c < DT; ++c)
+                s = cadd(s, cmul(cconj(cscale(cadd(L0[o + c], L1[o + c]), 0.5)), x[c]));
+        }
         y[r] = s;
     }
 }

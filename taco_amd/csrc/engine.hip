@@ -1061,14 +1061,31 @@ __global__ __launch_bounds__(kBlock) void k_l4f(L4fArgs A) {
 // staged and unstaged runs, and every instantiation below, give identical
 // values.  DT > 0 fixes D at compile time (the row loop is unrolled, so its
 // loads issue together); DT = 0 is the general case.
+// In the general case (DT = 0, D a runtime value) the columns are taken in
+// blocks of kMvBlk whose loads are issued before the block's products, so a
+// thread keeps kMvBlk generator loads in flight (the sum order is unchanged).
+constexpr int kMvBlk = 16;
+
 template <int DT>
 __device__ __forceinline__ void matvec_rows(const double2* __restrict__ L, int ld, const double2* x, double2* y,
                                             int D) {
     const int n = DT ? DT : D;
     for (int r = threadIdx.x; r < n; r += blockDim.x) {
         double2 s = make_double2(0.0, 0.0);
+        if constexpr (DT == 0) {
+            int c = 0;
+            for (; c + kMvBlk <= n; c += kMvBlk) {
+                double2 v[kMvBlk];
 #pragma unroll
-        for (int c = 0; c < (DT ? DT : n); ++c) s = cadd(s, cmul(L[r + static_cast<long>(c) * ld], x[c]));
+                for (int j = 0; j < kMvBlk; ++j) v[j] = L[r + static_cast<long>(c + j) * ld];
+#pragma unroll
+                for (int j = 0; j < kMvBlk; ++j) s = cadd(s, cmul(v[j], x[c + j]));
+            }
+            for (; c < n; ++c) s = cadd(s, cmul(L[r + static_cast<long>(c) * ld], x[c]));
+        } else {
+#pragma unroll
+            for (int c = 0; c < DT; ++c) s = cadd(s, cmul(L[r + static_cast<long>(c) * ld], x[c]));
+        }
         y[r] = s;
     }
 }
@@ -1080,11 +1097,30 @@ __device__ __forceinline__ void matvec_half(const double2* __restrict__ L0, cons
     const int n = DT ? DT : D;
     for (int r = threadIdx.x; r < n; r += blockDim.x) {
         double2 s = make_double2(0.0, 0.0);
+        if constexpr (DT == 0) {
+            int c = 0;
+            for (; c + kMvBlk <= n; c += kMvBlk) {
+                double2 v0[kMvBlk], v1[kMvBlk];
 #pragma unroll
-        for (int c = 0; c < (DT ? DT : n); ++c) {
-            const long e = r + static_cast<long>(c) * ld;
-            const double2 h = cscale(cadd(L0[e], L1[e]), 0.5);
-            s = cadd(s, cmul(h, x[c]));
+                for (int j = 0; j < kMvBlk; ++j) {
+                    const long e = r + static_cast<long>(c + j) * ld;
+                    v0[j] = L0[e];
+                    v1[j] = L1[e];
+                }
+#pragma unroll
+                for (int j = 0; j < kMvBlk; ++j) s = cadd(s, cmul(cscale(cadd(v0[j], v1[j]), 0.5), x[c + j]));
+            }
+            for (; c < n; ++c) {
+                const long e = r + static_cast<long>(c) * ld;
+                s = cadd(s, cmul(cscale(cadd(L0[e], L1[e]), 0.5), x[c]));
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < DT; ++c) {
+                const long e = r + static_cast<long>(c) * ld;
+                const double2 h = cscale(cadd(L0[e], L1[e]), 0.5);
+                s = cadd(s, cmul(h, x[c]));
+            }
         }
         y[r] = s;
     }

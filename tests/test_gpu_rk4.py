@@ -11,16 +11,19 @@ from .metrics import max_rel_err
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("D,steps", [(4, 50), (16, 200), (64, 30)])
+@pytest.mark.parametrize("D,steps", [(4, 50), (16, 200), (64, 30), (100, 12), (256, 6)])
 def test_rk4_matches_oracle(orc, D, steps):
     rng = np.random.default_rng(D)
     base = (rng.standard_normal((D, D)) + 1j * rng.standard_normal((D, D))) * 0.3
     Ls = np.array([base * np.cos(0.01 * s) for s in range(steps + 1)])
     r0 = rng.standard_normal(D) + 1j * rng.standard_normal(D)
     got_samples = []
+    # D = 100 and 256 take the general-size path (generators staged in LDS, or
+    # read from HBM in column blocks)
+    every = 7 if steps > 7 else 2
     r = taco_amd.propagate_rk4_dense(Ls, r0, 0.0, 0.01, on_sample=lambda t, v: got_samples.append(v.copy()),
-                                     sample_every=7)
-    rr, ref_samples = orc.propagate(Ls, r0, 0.01, every=7)
+                                     sample_every=every)
+    rr, ref_samples = orc.propagate(Ls, r0, 0.01, every=every)
     assert np.abs(r - rr).max() <= 1e-12 * np.abs(rr).max()
     assert len(got_samples) == len(ref_samples)
     assert max_rel_err(np.array(got_samples), ref_samples)[0] <= 1e-12
