@@ -658,7 +658,18 @@ __device__ __forceinline__ double spectral_J_dev(int kind, double p0, double p1,
 __global__ void k_bcf_kms(double r, long pos_count, double* kms) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     double v = r;
-    for (long k = 1; k <= pos_count; ++k) {
+    long k = 1;
+    for (; k + 8 <= pos_count + 1; k += 8) {  // the same chain, stored 8 at a time
+        double t[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            t[j] = v;
+            v *= r;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) kms[k + j] = t[j];
+    }
+    for (; k <= pos_count; ++k) {
         kms[k] = v;
         v *= r;
     }
@@ -713,6 +724,7 @@ __global__ __launch_bounds__(kBlock) void k_bcf_gather(const double2* spec, int 
 // of chunk c - 2; then every thread forms chunk c's terms, each exactly the
 // reference's expression.
 constexpr int kTzChunk = 1024;
+static_assert(kTzChunk % (kBlock - 128) == 0, "loader waves cover a chunk exactly");
 __global__ __launch_bounds__(kBlock) void k_gamma_trapz(const double2* C, long nc, double dt, const double* omegas,
                                                         double2* G) {
     __shared__ double2 cin[2][kTzChunk];   // C of chunk c (c % 2)
@@ -736,6 +748,7 @@ __global__ __launch_bounds__(kBlock) void k_gamma_trapz(const double2* C, long n
                     ph[cb][0] = phi;  // phi_0 = 1
                     e = 1;
                 }
+#pragma unroll 4
                 for (; e < e_end; ++e) {
                     phi = cmul(phi, step);
                     ph[cb][e] = phi;
@@ -757,16 +770,29 @@ __global__ __launch_bounds__(kBlock) void k_gamma_trapz(const double2* C, long n
                 }
             }
         } else if (threadIdx.x >= 128) {
-            if (c < nch)
-                for (int e = threadIdx.x - 128; e < kTzChunk; e += kBlock - 128) {
-                    const long k = c * kTzChunk + e;
-                    cin[cb][e] = k < nc ? C[k] : make_double2(0.0, 0.0);
+            // kLd values per loader thread, all loads issued before the LDS writes
+            constexpr int kLd = kTzChunk / (kBlock - 128);
+            const int e0 = threadIdx.x - 128;
+            if (c < nch) {
+                double2 v[kLd];
+#pragma unroll
+                for (int q = 0; q < kLd; ++q) {
+                    const long k = c * kTzChunk + e0 + q * (kBlock - 128);
+                    v[q] = k < nc ? C[k] : make_double2(0.0, 0.0);
                 }
-            if (c >= 2)
-                for (int e = threadIdx.x - 128; e < kTzChunk; e += kBlock - 128) {
-                    const long k = (c - 2) * kTzChunk + e;
-                    if (k < nc) col[k] = gout[cb][e];
+#pragma unroll
+                for (int q = 0; q < kLd; ++q) cin[cb][e0 + q * (kBlock - 128)] = v[q];
+            }
+            if (c >= 2) {
+                double2 v[kLd];
+#pragma unroll
+                for (int q = 0; q < kLd; ++q) v[q] = gout[cb][e0 + q * (kBlock - 128)];
+#pragma unroll
+                for (int q = 0; q < kLd; ++q) {
+                    const long k = (c - 2) * kTzChunk + e0 + q * (kBlock - 128);
+                    if (k < nc) col[k] = v[q];
                 }
+            }
         }
         __syncthreads();
         if (c < nch) {  // terms of chunk c
