@@ -1365,16 +1365,65 @@ struct Tcl2Args {
     int N, nf, num_ops;
     double2* L;            // [T][D*D], accumulated into
     int order;             // 0: unitary only; 2/4: full TCL2
+    int use_lists = 1;     // bucket-list path for N <= kTcl2ListN (TACO_TCL2_LISTS=0: all-bucket walk)
 };
 
+// Dynamic LDS of k_tcl2: the general path keeps H (N^2); the bucket-list path
+// (N <= kTcl2ListN) adds the pair map, and per ordered pair (x, y) the
+// ascending list of buckets b with some q such that p2f(q,x) == p2f(q,y) == b.
+constexpr int kTcl2ListN = 16;
+__host__ __device__ inline std::size_t tcl2_lds(int N) {
+    const std::size_t N2 = static_cast<std::size_t>(N) * N;
+    if (N > kTcl2ListN) return 2 * N2 * 16;
+    return N2 * 16 + N2 * 4 + N2 * 4 + N2 * static_cast<std::size_t>(N) * 2;
+}
+
+// L_TCL2(t) added onto L(t), one workgroup per time (generator.cpp:81-177 as
+// composed in TCL4_spin_boson_example.cpp:303-322).  Every entry sums its
+// bucket terms in ascending bucket order; a bucket whose term is exactly zero
+// (no jump, Lamb-shift or anticommutator element of the entry in it) adds
+// nothing, so for N <= 16 each entry visits only the buckets of its pair
+// lists: one bucket at most off the N-diagonal blocks, the merged lists on
+// them.  The values equal the all-bucket walk (general path) up to the sign
+// of an exact zero.
 __global__ __launch_bounds__(kBlock) void k_tcl2(Tcl2Args A) {
     extern __shared__ double2 sh[];
     const int N = A.N, N2 = N * N, D = N2;
     const long t = blockIdx.x;
-    double2* H = sh;                 // N x N effective Hamiltonian
-    double2* diss = H + N2;          // per (alpha) jump sums: kept as dense D x D in global instead
-    (void)diss;
+    double2* H = sh;  // N x N effective Hamiltonian
     double2* L = A.L + t * static_cast<long>(D) * D;
+    const bool lists = A.use_lists && N <= kTcl2ListN;
+    int* P = reinterpret_cast<int*>(H + N2);                        // pair map
+    int* cnt = P + N2;                                              // list lengths per pair
+    unsigned short* bk = reinterpret_cast<unsigned short*>(cnt + N2);  // [pair][N] buckets
+    if (lists) {
+        for (int e = threadIdx.x; e < N2; e += kBlock) P[e] = A.p2f[e];
+        __syncthreads();
+        for (int pr = threadIdx.x; pr < N2; pr += kBlock) {
+            const int x = pr % N, y = pr / N;
+            unsigned short* l = bk + pr * N;
+            int n = 0;
+            for (int q = 0; q < N; ++q) {
+                const int b = P[q + x * N];
+                if (b < 0 || b >= A.nf || b != P[q + y * N]) continue;
+                int i = n;  // insertion into the ascending list, unique
+                while (i > 0 && l[i - 1] > b) --i;
+                if (i > 0 && l[i - 1] == b) continue;
+                for (int m = n; m > i; --m) l[m] = l[m - 1];
+                l[i] = static_cast<unsigned short>(b);
+                ++n;
+            }
+            cnt[pr] = n;
+        }
+        __syncthreads();
+    }
+    // sum_q [p2f(q,x) == p2f(q,y) == b] conj(A(q,x)) A(q,y), q ascending
+    auto qsum = [&](const double2* Ap, int x, int y, int b) {
+        double2 s = make_double2(0, 0);
+        for (int q = 0; q < N; ++q)
+            if (P[q + x * N] == b && P[q + y * N] == b) s = cadd(s, cmul(cconj(Ap[q + x * N]), Ap[q + y * N]));
+        return s;
+    };
     // H_eff = diag(eps) + hermitize(sum_b sum_a S_b A_b^dag A_b) with S_b = Im gamma(t,b).
     for (int e = threadIdx.x; e < N2; e += kBlock) {
         const int r = e % N, c = e / N;
@@ -1382,6 +1431,19 @@ __global__ __launch_bounds__(kBlock) void k_tcl2(Tcl2Args A) {
         if (A.order != 0) {
             for (int al = 0; al < A.num_ops; ++al) {
                 const double2* Ap = A.ops + al * N2;
+                if (lists) {
+                    // (A_b^dag A_b)(r,c) is nonzero only for the buckets of pair (r, c)
+                    const unsigned short* l = bk + (r + c * N) * N;
+                    const int n = cnt[r + c * N];
+                    for (int i = 0; i < n; ++i) {
+                        const int b = l[i];
+                        const double S = A.gamma[static_cast<long>(b) * A.Nt + t].y;
+                        if (S == 0.0) continue;
+                        h = cadd(h, cscale(qsum(Ap, r, c, b), S));
+                        hc = cadd(hc, cscale(qsum(Ap, c, r, b), S));
+                    }
+                    continue;
+                }
                 for (int b = 0; b < A.nf; ++b) {
                     const double S = A.gamma[static_cast<long>(b) * A.Nt + t].y;
                     if (S == 0.0) continue;
@@ -1404,6 +1466,7 @@ __global__ __launch_bounds__(kBlock) void k_tcl2(Tcl2Args A) {
     }
     __syncthreads();
     // L(row=(r1,r2), col=(c1,c2)) with row = r1*N + r2 (kron(I,H) - kron(H^T,I) convention)
+    const double2 zero = make_double2(0, 0);
     for (long e = threadIdx.x; e < static_cast<long>(D) * D; e += kBlock) {
         const int row = static_cast<int>(e % D), col = static_cast<int>(e / D);
         const int r1 = row / N, r2 = row % N, c1 = col / N, c2 = col % N;
@@ -1414,6 +1477,37 @@ __global__ __launch_bounds__(kBlock) void k_tcl2(Tcl2Args A) {
         if (A.order != 0) {
             for (int al = 0; al < A.num_ops; ++al) {
                 const double2* Ap = A.ops + al * N2;
+                if (lists) {
+                    const int b1 = P[r1 + c1 * N], b2 = P[r2 + c2 * N];
+                    const int kInf = 1 << 30;
+                    const int jb = (b1 == b2 && b1 >= 0 && b1 < A.nf) ? b1 : kInf;
+                    // lt: pair (r2, c2) when r1 == c1; rt: pair (c1, r1) when r2 == c2
+                    const unsigned short* la = bk + (r2 + c2 * N) * N;
+                    const unsigned short* lb = bk + (c1 + r1 * N) * N;
+                    const int na = r1 == c1 ? cnt[r2 + c2 * N] : 0, nb = r2 == c2 ? cnt[c1 + r1 * N] : 0;
+                    int ia = 0, ib = 0;
+                    bool jpend = jb != kInf;
+                    while (true) {
+                        int b = jpend ? jb : kInf;
+                        if (ia < na && la[ia] < b) b = la[ia];
+                        if (ib < nb && lb[ib] < b) b = lb[ib];
+                        if (b == kInf) break;
+                        double2 jump = zero, lt = zero, rt = zero;
+                        const bool use_a = ia < na && la[ia] == b, use_b = ib < nb && lb[ib] == b;
+                        const bool use_j = jpend && jb == b;
+                        if (use_a) ++ia;
+                        if (use_b) ++ib;
+                        if (use_j) jpend = false;
+                        const double gam = 2.0 * A.gamma[static_cast<long>(b) * A.Nt + t].x;
+                        if (gam == 0.0) continue;
+                        if (use_j) jump = cmul(cconj(Ap[r1 + c1 * N]), Ap[r2 + c2 * N]);
+                        if (use_a) lt = qsum(Ap, r2, c2, b);
+                        if (use_b) rt = qsum(Ap, c1, r1, b);
+                        const double2 term = csub(csub(jump, cscale(lt, 0.5)), cscale(rt, 0.5));
+                        val = cadd(val, cscale(term, gam));
+                    }
+                    continue;
+                }
                 for (int b = 0; b < A.nf; ++b) {
                     const double gam = 2.0 * A.gamma[static_cast<long>(b) * A.Nt + t].x;
                     if (gam == 0.0) continue;
@@ -1675,6 +1769,12 @@ static void launch_phase(int cfg, Phase ph, const FcrArgs& P, long njobs, hipStr
 static int env_int(const char* name, int dflt) {
     const char* v = std::getenv(name);
     return v ? std::atoi(v) : dflt;
+}
+
+// Read at each launch (a test compares the two paths in one process).
+static bool tcl2_lists_enabled() {
+    const char* e = std::getenv("TACO_TCL2_LISTS");
+    return !(e && std::string(e) == "0");
 }
 
 static std::size_t chunk_budget_bytes() {
@@ -2285,12 +2385,13 @@ static void batch_enqueue_system(taco_ctx* c, BatchSys& S) {
         A2.ops = S.ops.as<double2>();
         A2.p2f = S.p2f.as<int>();
         A2.N = S.N;
+        A2.use_lists = tcl2_lists_enabled() ? 1 : 0;
         A2.nf = S.nf;
         A2.num_ops = S.num_ops;
         A2.L = S.lgen.as<double2>();
         A2.order = S.order;
         timed(c, K_TCL2, static_cast<double>(gen) * 32.0, [&] {
-            hipLaunchKernelGGL(k_tcl2, dim3(static_cast<unsigned>(S.steps + 1)), dim3(kBlock), 2 * N2 * 16, c->stream,
+            hipLaunchKernelGGL(k_tcl2, dim3(static_cast<unsigned>(S.steps + 1)), dim3(kBlock), tcl2_lds(S.N), c->stream,
                                A2);
         });
     }
@@ -2451,11 +2552,12 @@ static void tape_forward(taco_tape& tp, const double* r0, double* samples_out) {
     A2.ops = tp.ops.as<double2>();
     A2.p2f = tp.p2f.as<int>();
     A2.N = tp.N;
+    A2.use_lists = tcl2_lists_enabled() ? 1 : 0;
     A2.nf = tp.nf;
     A2.num_ops = tp.num_ops;
     A2.L = tp.lgen.as<double2>();
     A2.order = tp.order;
-    hipLaunchKernelGGL(k_tcl2, dim3(static_cast<unsigned>(T)), dim3(kBlock), 2 * N2 * 16, c->stream, A2);
+    hipLaunchKernelGGL(k_tcl2, dim3(static_cast<unsigned>(T)), dim3(kBlock), tcl2_lds(A2.N), c->stream, A2);
     HIP_OK(hipGetLastError());
     // RK4 with every state sampled; checkpoints r_{qK} kept on the tape.
     c->misc.ensure((T + 1) * D * 16 + 64);
@@ -3245,11 +3347,12 @@ int taco_tcl4_propagate(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, c
         A2.ops = dops;
         A2.p2f = dpf;
         A2.N = N;
+        A2.use_lists = tcl2_lists_enabled() ? 1 : 0;
         A2.nf = nf;
         A2.num_ops = num_ops;
         A2.L = ctx->l4.as<double2>();
         A2.order = order;
-        hipLaunchKernelGGL(k_tcl2, dim3(static_cast<unsigned>(T)), dim3(kBlock), 2 * N2 * 16, ctx->stream, A2);
+        hipLaunchKernelGGL(k_tcl2, dim3(static_cast<unsigned>(T)), dim3(kBlock), tcl2_lds(A2.N), ctx->stream, A2);
         HIP_OK(hipGetLastError());
         ctx->misc.ensure((T + 1) * N2 * 16 + 64);
         double2* dr = ctx->misc.as<double2>();

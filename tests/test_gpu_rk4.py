@@ -69,3 +69,35 @@ def test_tcl4_propagate_spin_pair_degenerate(orc):
     got = taco_amd.tcl4_propagate(S, w.gamma, w.dt, w.steps, r0)
     ref = orc.tcl4_propagate(w.gamma, S.omegas, S.mirror, S.pair_to_freq, S.eps, S.A_eig, w.dt, w.steps, r0)
     assert max_rel_err(got, ref)[0] <= 1e-10
+
+
+@pytest.mark.parametrize("case", ["gue3_two_ops", "spin_pair", "d8", "tfim_d16"])
+def test_tcl2_bucket_lists_equal_all_bucket_walk(monkeypatch, case):
+    """The TCL2 kernel's bucket-list path (each superoperator entry visits only
+    the buckets with a nonzero term) against the all-bucket walk
+    (TACO_TCL2_LISTS=0): the same values (exact zeros may differ in sign)."""
+    if case == "gue3_two_ops":
+        rng = W.SplitMix64(11)
+        H = W.gue(3, rng)
+        A1, A2 = W.random_hermitian_unit(3, rng), W.random_hermitian_unit(3, rng)
+        w = W.Workload("g3", H, [A1, A2], taco_amd.DrudeLorentz(0.08, 4.0), 0.7, 0.01, 511, list(range(41)),
+                       steps=40)
+        steps = 40
+    elif case == "spin_pair":
+        w, steps = W.spin_pair(Nt=1024, steps=60), 60
+    elif case == "d8":
+        rng = W.SplitMix64(5)
+        H = W.gue(8, rng)
+        A = W.random_hermitian_unit(8, rng)
+        w = W.Workload("g8", H, [A], taco_amd.Ohmic(0.1, 5.0), 1.0, 0.01, 255, list(range(21)), steps=20)
+        steps = 20
+    else:
+        w, steps = W.tfim_workload(0.7, lam=0.2, dt=0.05, Nt=64, tmax=6), 6
+    S, G = w.system, w.gamma
+    r0 = np.zeros(S.N ** 2, complex)
+    r0[0] = 1.0
+    out = []
+    for lists in ("1", "0"):
+        monkeypatch.setenv("TACO_TCL2_LISTS", lists)
+        out.append(taco_amd.tcl4_propagate(S, G, w.dt, steps, r0, order=2))
+    assert np.array_equal(out[0], out[1])
