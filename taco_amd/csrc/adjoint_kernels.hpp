@@ -789,8 +789,9 @@ __global__ __launch_bounds__(kBlock) void k_fcr_adj_scan(FcrAdjArgs P) {
         }
         double v[6] = {a1.x, a1.y, a2.x, a2.y, a3.x, a3.y};
         double tot[6];
+        block_scan_n<6>(v, wsum, tot);
         for (int q = 0; q < 6; ++q) {
-            v[q] = block_scan(v[q], wsum[q], tot[q]) + carry[q];
+            v[q] = v[q] + carry[q];
             carry[q] += tot[q];
         }
         if (!valid) continue;
@@ -827,8 +828,9 @@ __global__ __launch_bounds__(kBlock) void k_fcr_adj_scan(FcrAdjArgs P) {
             v[4] = Pb3[t].x; v[5] = Pb3[t].y; v[6] = Pb4[t].x; v[7] = Pb4[t].y;
         }
         double tot[8];
+        block_scan_n<8>(v, wsum, tot);
         for (int q = 0; q < 8; ++q) {
-            v[q] = block_scan(v[q], wsum[q], tot[q]) + sc[q];
+            v[q] = v[q] + sc[q];
             sc[q] += tot[q];
         }
         if (!valid) continue;

@@ -550,6 +550,31 @@ __device__ __forceinline__ double block_scan(double v, double* wsum, double& tot
     return pre + v;
 }
 
+// The block scan above for Q values at once: the same wave scans, prefix
+// sums and totals (so bit-identical per value), two barriers in all.
+template <int Q>
+__device__ __forceinline__ void block_scan_n(double (&v)[Q], double (*wsum)[4], double (&total)[Q]) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        for (int off = 1; off < 64; off <<= 1) {
+            const double nb = __shfl_up(v[q], off, 64);
+            if (lane >= off) v[q] += nb;
+        }
+    if (lane == 63)
+#pragma unroll
+        for (int q = 0; q < Q; ++q) wsum[q][wid] = v[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        double pre = 0.0;
+        for (int w = 0; w < wid; ++w) pre += wsum[q][w];
+        total[q] = (wsum[q][0] + wsum[q][1]) + (wsum[q][2] + wsum[q][3]);
+        v[q] = pre + v[q];
+    }
+    __syncthreads();
+}
+
 struct CombineArgs {
     int out_mode;        // 0: time-major slices at t2pos; 1: series-major [tri][T]
     const int* t2pos;    // [T], -1 when t is not requested (mode 0)
@@ -596,8 +621,10 @@ __global__ __launch_bounds__(kBlock) void k_combine(FcrArgs P, CombineArgs O) {
         }
         double v[8] = {a1.x, a1.y, a2.x, a2.y, a3.x, a3.y, a4.x, a4.y};
         double tot[8];
+        block_scan_n<8>(v, wsum, tot);
+#pragma unroll
         for (int q = 0; q < 8; ++q) {
-            v[q] = block_scan(v[q], wsum[q], tot[q]) + carry[q];
+            v[q] = v[q] + carry[q];
             carry[q] += tot[q];
         }
         if (!valid) continue;
