@@ -28,7 +28,10 @@ from ctypes import c_void_p
 from dataclasses import dataclass
 from typing import Optional, Sequence
 
+import os
+
 import numpy as np
+import scipy.fft
 import torch
 
 from ._lib import TacoInvalidArgument, check, dptr, iptr, lib
@@ -177,7 +180,8 @@ class BcfFun(torch.autograd.Function):
         wny = np.pi / dt
         cb = np.zeros(Nfft, np.complex128)
         cb[:Nt] = Cbar.detach().numpy()
-        Sbar = (dw / np.pi) * (Nfft * np.fft.ifft(cb)).real
+        # unnormalised inverse FFT (multithreaded on the host cores)
+        Sbar = (dw / np.pi) * scipy.fft.ifft(cb, norm="forward", workers=len(os.sched_getaffinity(0))).real
         k = np.arange(1, Nfft // 2)
         w = dw * k
         J, dJ0, dJ1 = _spectral(kind, p0, p1, w)
