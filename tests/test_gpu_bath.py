@@ -26,7 +26,7 @@ def test_device_cr_exp_bitwise_equals_host():
 CASES = [
     ("ohmic", taco_amd.Ohmic(0.05, 5.0), 1.0, 2 ** 14 - 1, 0.005),
     ("ohmic_bench", taco_amd.Ohmic(0.1, 10.0), 0.5, 200000, 0.002),
-    ("ohmic_cfg4", taco_amd.Ohmic(0.05, 5.0), 1.0, 2 ** 16 - 1, 0.005),
+    ("ohmic_pair_grid", taco_amd.Ohmic(0.05, 5.0), 1.0, 2 ** 16 - 1, 0.005),
     ("drude", taco_amd.DrudeLorentz(0.05, 5.0), 1.0, 3000, 0.005),
     ("ohmic_zero_T", taco_amd.Ohmic(0.1, 3.0), float("inf"), 777, 0.01),
     ("drude_hot", taco_amd.DrudeLorentz(0.2, 20.0), 0.2, 2 ** 12, 0.005),
