@@ -247,7 +247,28 @@ __device__ __forceinline__ void rk4_adjoint_body(double2* sh, const Rk4AdjArgs& 
                 }
             }
         } else {
-            for (long e = threadIdx.x; e < D2; e += blockDim.x) {
+            // in place, kU elements per thread at a time with their loads issued
+            // first (the same expressions as the remainder loop)
+            constexpr int kU = 8;
+            const long stride = blockDim.x;
+            long e0 = threadIdx.x;
+            for (; e0 + (kU - 1) * stride < D2; e0 += kU * stride) {
+                double2 a0[kU], a1[kU];
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    a0[u] = Lb0[e0 + u * stride];
+                    a1[u] = Lb1[e0 + u * stride];
+                }
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    const long e = e0 + u * stride;
+                    const int r = static_cast<int>(e % D), c = static_cast<int>(e / D);
+                    const double2 lh = cadd(cmul(k3[r], cconj(u2[c])), cmul(kb2[r], cconj(u1[c])));
+                    Lb0[e] = cadd(a0[u], cadd(cmul(kb1[r], cconj(x[c])), cscale(lh, 0.5)));
+                    Lb1[e] = cadd(a1[u], cadd(cmul(k4[r], cconj(u3[c])), cscale(lh, 0.5)));
+                }
+            }
+            for (long e = e0; e < D2; e += stride) {
                 const int r = static_cast<int>(e % D), c = static_cast<int>(e / D);
                 const double2 lh = cadd(cmul(k3[r], cconj(u2[c])), cmul(kb2[r], cconj(u1[c])));
                 Lb0[e] = cadd(Lb0[e], cadd(cmul(kb1[r], cconj(x[c])), cscale(lh, 0.5)));
