@@ -1825,12 +1825,17 @@ static void autotune(taco_ctx* c, const FcrArgs& P, long ncls, long nj, std::arr
         int best_cfg = 0;
         for (int cfg = 0; cfg < kNumFftCfg; ++cfg) {
             launch_phase(cfg, phases[ph], P, units[ph], c->stream);  // warm
-            HIP_OK(hipEventRecord(e0, c->stream));
-            launch_phase(cfg, phases[ph], P, units[ph], c->stream);
-            HIP_OK(hipEventRecord(e1, c->stream));
-            HIP_OK(hipEventSynchronize(e1));
+            // the fastest of three runs, so one noisy launch does not decide
             float ms = 0.f;
-            HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+            for (int rep = 0; rep < 3; ++rep) {
+                HIP_OK(hipEventRecord(e0, c->stream));
+                launch_phase(cfg, phases[ph], P, units[ph], c->stream);
+                HIP_OK(hipEventRecord(e1, c->stream));
+                HIP_OK(hipEventSynchronize(e1));
+                float m = 0.f;
+                HIP_OK(hipEventElapsedTime(&m, e0, e1));
+                if (rep == 0 || m < ms) ms = m;
+            }
             if (cfg == 0 || ms < best) {
                 best = ms;
                 best_cfg = cfg;
