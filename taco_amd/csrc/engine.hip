@@ -2908,6 +2908,7 @@ constexpr double kPiHost = 3.141592653589793238462643383279502884;
 static void bcf_device(taco_ctx* c, std::size_t N, double dt, int kind, double p0, double p1, double beta, double2* dC) {
     if (!(dt > 0.0)) throw Error(TACO_ERR_INVALID, "bcf_fft: dt must be > 0");
     if (kind != 0 && kind != 1) throw Error(TACO_ERR_INVALID, "bcf_fft: unknown spectral density kind");
+    if (N < 1) throw Error(TACO_ERR_INVALID, "bcf_fft on the GPU needs N >= 1 (two or more time samples)");
     const std::size_t Nt = N + 1;
     std::size_t Nfft = 2 * Nt;
     if ((Nfft & (Nfft - 1)) != 0) Nfft = next_pow2(Nfft);

@@ -103,3 +103,4 @@ def test_ctx_create_fails_cleanly_without_device():
     h = ctypes.c_void_p()
     rc = _lib.lib().taco_ctx_create(0, ctypes.byref(h))
     assert rc != _lib.TACO_OK
+

@@ -46,3 +46,8 @@ def test_bath_gamma_bitwise_equals_host(case):
     C_only, none = taco_amd.bath_gamma_hip(N, dt, bath, beta)
     assert none is None and np.array_equal(C_only, C_h)
     assert np.array_equal(taco_amd.compute_trapz_prefix_multi_matrix_hip(C_h, dt, om), G_h)
+
+
+def test_device_bath_producer_rejects_a_single_sample():
+    with pytest.raises(ValueError):
+        taco_amd.bath_gamma_hip(0, 0.01, taco_amd.Ohmic(0.1, 1.0), 1.0)
