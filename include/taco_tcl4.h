@@ -62,6 +62,23 @@ void* taco_ctx_stream(taco_ctx* ctx);
 void taco_set_fcr_fft_pad_factor(size_t factor);
 size_t taco_get_fcr_fft_pad_factor(void);
 
+/* Phase and exponential convention, process-global like the pad factor.
+ * TACO_PHASE_LIBM (default): the F/C/R phases exp(+-i*Omega*(k*dt)), the
+ * trapz phase step exp(i*omega_b*dt) and the bath correlation's
+ * exponentials are the platform libm's std::exp values, the reference's own
+ * arithmetic (tcl4_kernels.cpp:116-120, gamma.hpp:185,
+ * correlation_fft.hpp:154-214).  The host evaluates them: the F/C/R phases
+ * as one table row per distinct Omega over t < Nt, cached per context (and
+ * per batch system or tape) and read by the kernels; a table larger than
+ * TACO_LIBM_TABLE_BYTES (default 16 GiB) is refused with TACO_ERR_INVALID.
+ * TACO_PHASE_CR: correctly rounded cos/sin/exp of the same arguments,
+ * evaluated on the device (no table; any size).  Returns TACO_ERR_INVALID
+ * for an unknown mode. */
+#define TACO_PHASE_LIBM 0
+#define TACO_PHASE_CR 1
+int taco_set_phase_mode(int mode);
+int taco_get_phase_mode(void);
+
 /* Per-kernel HIP-event profiler of a context (off by default).  Kernel ids:
  * 0 phase A, 1 phase B, 2 phase C, 3 F/C/R combine, 4 fused L4,
  * 5 per-bucket spectra, 6 TCL2, 7 RK4.  taco_profile_get returns the summed

@@ -24,15 +24,23 @@ triple_kernels              per-triple equality with fcr_series; slicing (keep <
                             equals the head of the full series bitwise
 bath exp (mode CR)          70-digit decimal exp on random, large and subnormal-range
                             arguments (correctly rounded exp of J and the KMS factors)
-bcf_fft                     brute-force DFT of the KMS spectrum (bcf_fft_ohmic_simple.m)
+bcf_fft                     Ohmic closed forms C(t) at zero and finite temperature
+                            (coth series), converging at second order in the frequency
+                            step (tests/test_oracle_pins.py); brute-force DFT of the KMS
+                            spectrum (bcf_fft_ohmic_simple.m)
 gamma_trapz / gamma_rect    geometric-series closed form for C(t) = exp(-g t)
 mikx                        make_val fixture of cpp/tests/tcl4_mikx_tests.cpp:49-160
                             with the code's M1 order (tcl4_mikx.cpp:111-113)
-assemble_gw / gw_to_l4      independent 8-loop numpy restatement of
-                            matlab/NAKZWAN_v9.m on the tests/tcl4_tests.cpp:36-90
-                            synthetic fixture; GW == GW^H; L4 preserves Hermiticity
-tcl2_at                     numpy kron restatement of generator.cpp:81-177;
-                            trace preservation of L2
+assemble_gw / gw_to_l4      with mikx and fcr_series: first-order convergence in dt
+                            to the fourth-order TCL generator evaluated by brute force
+                            (ordered cumulants + Wick, tests/tcl4_bruteforce.py, itself
+                            checked against exact spin + bosonic-mode dynamics) at
+                            N = 2, 3, 4; a sign planted in any of the ten NAKZWAN terms
+                            breaks it; pure dephasing gives L4 == 0 exactly (the
+                            planted signs do not); GW == GW^H; L4 preserves Hermiticity
+tcl2_at                     equals L0 + the secular part of the brute-force K2 for the
+                            Gamma table read at -omega_b (the reference's frequency
+                            convention for A(w)); trace preservation of L2
 rk4_update / propagate      rk4_dense_tests.cpp closed forms (expm, exp(t^2/2));
                             one step == 4th-order Taylor polynomial
 system_build                residual ||HU - U diag(eps)||, unitarity, ascending
@@ -100,6 +108,7 @@ def lib():
                                          _ip, _ip]),
             "orc_set_phase_mode": (None, [c_int]),
             "orc_get_phase_mode": (c_int, []),
+            "orc_set_mutation": (None, [c_int]),
             "orc_phases": (None, [c_double, c_double, c_size_t, _dp]),
             "orc_cr_sincos": (None, [_dp, c_size_t, _dp, _dp]),
             "orc_cr_exp": (None, [_dp, c_size_t, _dp]),
@@ -149,6 +158,21 @@ PHASE_CR = 1    # correctly rounded cos/sin of the same argument
 
 def set_phase_mode(mode: int) -> None:
     lib().orc_set_phase_mode(int(mode))
+
+
+class planted_mistake:
+    """Context manager: NAKZWAN term q (1..10) enters with the wrong sign
+    (tests only: shows that a check catches such a mistake)."""
+
+    def __init__(self, term: int):
+        self.term = term
+
+    def __enter__(self):
+        lib().orc_set_mutation(int(self.term))
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_mutation(0)
 
 
 def get_phase_mode() -> int:

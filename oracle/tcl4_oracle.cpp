@@ -455,6 +455,13 @@ void mikx_at(const cd* Fs, const cd* Cs, const cd* Rs, const int* p2f, int N, in
     }
 }
 
+// Test-only planted mistake: term q of the NAKZWAN sum (1..7 the seven
+// off-diagonal terms t1..t7, 8..10 the three j == m terms) enters with the
+// wrong sign; 0 (the default) leaves the restatement untouched.  Tests use it
+// to show that the closed-form and brute-force checks catch such a mistake.
+int g_mutation = 0;
+static inline cd mut(int q, cd v) { return g_mutation == q ? -v : v; }
+
 // assemble_liouvillian — tcl4_assemble.cpp:29-178 (NAKZWAN T, then T + T^H).
 void assemble_gw(const cd* M, const cd* Ia, const cd* K, const cd* X, const cd* ops, int num_ops,
                  int N, cd* GW) {
@@ -492,7 +499,8 @@ void assemble_gw(const cd* M, const cd* Ia, const cd* K, const cd* X, const cd* 
                         const cd t5 = B(n, a) * A(a, i) * B(j, b) * A(b, m) * X_iajbni;
                         const cd t6 = A(n, a) * A(a, b) * B(b, i) * B(j, m) * X_bajmai;
                         const cd t7 = A(n, a) * B(a, b) * A(b, i) * B(j, m) * X_ibjmai;
-                        res -= (t1 - t2 + t3 + t4 - t5 - t6 + t7);
+                        res -= (mut(1, t1) - mut(2, t2) + mut(3, t3) + mut(4, t4) - mut(5, t5) - mut(6, t6) +
+                                mut(7, t7));
                     }
                 }
                 if (j == m) {
@@ -505,7 +513,7 @@ void assemble_gw(const cd* M, const cd* Ia, const cd* K, const cd* X, const cd* 
                                 const cd add1 = A(n, a) * B(a, b) * A(b, c) * B(c, i) * M_ci_ab;
                                 const cd add2 = A(n, a) * B(a, b) * B(b, c) * A(c, i) * K_ic_ab;
                                 const cd add3 = -A(n, a) * A(a, b) * B(b, c) * B(c, i) * I_ba_ic;
-                                res += (add1 + add2 + add3);
+                                res += (mut(8, add1) + mut(9, add2) + mut(10, add3));
                             }
                 }
             }
@@ -738,6 +746,7 @@ const char* orc_last_error() { return g_err.c_str(); }
 
 // 0: std::exp phases (glibc on this platform); 1: correctly rounded phases.
 void orc_set_phase_mode(int mode) { g_phase_mode = mode; }
+void orc_set_mutation(int term) { g_mutation = term; }
 int orc_get_phase_mode() { return g_phase_mode; }
 
 // Phases exp(i*omega*(k*dt)), k < n, in the current mode (interleaved re, im).

@@ -763,6 +763,9 @@ struct FcrAdjArgs {
     long nf3;
     double2* scr;           // [ncomb][11][T]
     double* om;             // [ncomb] Omega cotangent
+    const double2* ph_tab;  // phase table of the forward pass (TACO_PHASE_LIBM) or null
+    const int* job_row;
+    long ph_len;
 };
 
 __device__ __forceinline__ double block_sum(double v, double* ws) {
@@ -785,6 +788,7 @@ __global__ __launch_bounds__(kBlock) void k_fcr_adj_scan(FcrAdjArgs P) {
     int i, j, k;
     decode_triple(tri, P.nf, i, j, k);
     const double omega = (P.omegas[i] + P.omegas[j]) + P.omegas[k];
+    const double2* prow = P.ph_tab ? P.ph_tab + static_cast<long>(P.job_row[p]) * P.ph_len : nullptr;
     const double2* g1 = P.gamma + static_cast<long>(i) * P.Nt;
     const double2* g2 = P.gamma + static_cast<long>(j) * P.Nt;
     const double2* gm = P.gamma + static_cast<long>(P.mirror[j]) * P.Nt;
@@ -800,7 +804,7 @@ __global__ __launch_bounds__(kBlock) void k_fcr_adj_scan(FcrAdjArgs P) {
         double2 a1 = make_double2(0, 0), a2 = a1, a3 = a1;
         double2 pp = a1, pm = a1, x1 = a1, x2 = a1;
         if (valid) {
-            pp = cexpi(omega * (static_cast<double>(t) * dt));
+            pp = phase_plus(prow, t, omega, dt);
             pm = cconj(pp);
             x1 = g1[t];
             x2 = g2[t];
@@ -858,7 +862,7 @@ __global__ __launch_bounds__(kBlock) void k_fcr_adj_scan(FcrAdjArgs P) {
         const double2 yF = make_double2(v[0] * dt, v[1] * dt), yC = make_double2(v[2] * dt, v[3] * dt);
         const double2 y3 = make_double2(v[4] * dt, v[5] * dt), y4 = make_double2(v[6] * dt, v[7] * dt);
         const double tdt = static_cast<double>(t) * dt;
-        const double2 pp = cexpi(omega * tdt), pm = cconj(pp);
+        const double2 pp = phase_plus(prow, t, omega, dt), pm = cconj(pp);
         const double2 x1 = g1[t], x2 = g2[t], xm = gm[t];
         GM[t] = cmul(yF, cconj(pp));
         G2[t] = cadd(cadd(cmul(cconj(yC), pp), cmul(y3, pp)), cmul(y4, cconj(cmul(x1, pm))));
@@ -942,6 +946,7 @@ __global__ __launch_bounds__(kBlock) void k_fcr_adj_fin(FcrAdjArgs P) {
     int i, j, k;
     decode_triple(tri, P.nf, i, j, k);
     const double omega = (P.omegas[i] + P.omegas[j]) + P.omegas[k];
+    const double2* prow = P.ph_tab ? P.ph_tab + static_cast<long>(P.job_row[p]) * P.ph_len : nullptr;
     const double2* g1 = P.gamma + static_cast<long>(i) * P.Nt;
     const long T = P.T;
     double2* S = P.scr + p * 11 * T;
@@ -950,7 +955,7 @@ __global__ __launch_bounds__(kBlock) void k_fcr_adj_fin(FcrAdjArgs P) {
     double om_part = 0.0;
     for (long t = threadIdx.x; t < T; t += kBlock) {
         const double tdt = static_cast<double>(t) * P.dt;
-        const double2 pp = cexpi(omega * tdt), pm = cconj(pp);
+        const double2 pp = phase_plus(prow, t, omega, P.dt), pm = cconj(pp);
         const double2 wb = WB[t];
         G1[t] = cadd(G1[t], cmul(wb, pp));
         const double2 pmb = cmul(wb, cconj(g1[t]));

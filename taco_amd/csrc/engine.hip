@@ -29,6 +29,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cstdint>
 #include <chrono>
 #include <cmath>
@@ -314,7 +315,18 @@ struct FcrArgs {
     double2* spec;          // [2 nf][2^a][2^b]   spectra of g_b (even) and conj(g_b) (odd)
     double2* conv;          // [2][njobs][T]
     long T;
+    // Phase mode TACO_PHASE_LIBM: host table of exp(+i Omega (t dt)), one row
+    // of ph_len values per distinct Omega, and the row of each sorted
+    // position; null in TACO_PHASE_CR (the kernels evaluate cexpi).
+    const double2* ph_tab;
+    const int* job_row;
+    long ph_len;
 };
+
+// exp(+i Omega (t dt)) of a triple: its table row, or the correctly rounded value.
+__device__ __forceinline__ double2 phase_plus(const double2* row, long t, double omega, double dt) {
+    return row ? row[t] : cexpi(omega * (static_cast<double>(t) * dt));
+}
 
 __device__ __forceinline__ void decode_triple(long tri, int nf, int& i, int& j, int& k) {
     k = static_cast<int>(tri % nf);
@@ -376,6 +388,7 @@ __global__ __launch_bounds__(Cfg::kFT) void k_phase_a(FcrArgs P) {
     int i0, j0, k0;
     decode_triple(P.job_tri[pbeg], P.nf, i0, j0, k0);
     const double omega = (P.omegas[i0] + P.omegas[j0]) + P.omegas[k0];
+    const double2* prow = P.ph_tab ? P.ph_tab + static_cast<long>(P.job_row[pbeg]) * P.ph_len : nullptr;
     const int total = G * gs;
     double2 ph[kPT];
 #pragma unroll
@@ -384,7 +397,7 @@ __global__ __launch_bounds__(Cfg::kFT) void k_phase_a(FcrArgs P) {
         const int grp = Tl.grp_of(e), pos = Tl.pos_of(e);
         const long t = static_cast<long>(r0b + grp) + (static_cast<long>(pos) << P.b);
         ph[q] = make_double2(0.0, 0.0);
-        if (e < total && t < P.Nt) ph[q] = cconj(cexpi(omega * (static_cast<double>(t) * P.dt)));
+        if (e < total && t < P.Nt) ph[q] = cconj(phase_plus(prow, t, omega, P.dt));
     }
     for (long p = pbeg; p < pend; ++p) {
         int i, j, k;
@@ -598,6 +611,7 @@ __global__ __launch_bounds__(kBlock) void k_combine(FcrArgs P, CombineArgs O) {
     int i, j, k;
     decode_triple(tri, P.nf, i, j, k);
     const double omega = (P.omegas[i] + P.omegas[j]) + P.omegas[k];
+    const double2* prow = P.ph_tab ? P.ph_tab + static_cast<long>(P.job_row[cpos]) * P.ph_len : nullptr;
     const double2* g1 = P.gamma + static_cast<long>(i) * P.Nt;
     const double2* g2 = P.gamma + static_cast<long>(j) * P.Nt;
     const double2* g2m = P.gamma + static_cast<long>(P.mirror[j]) * P.Nt;
@@ -609,8 +623,7 @@ __global__ __launch_bounds__(kBlock) void k_combine(FcrArgs P, CombineArgs O) {
         const bool valid = t < P.T;
         double2 a1 = make_double2(0, 0), a2 = a1, a3 = a1, a4 = a1, pm = a1, gg1 = a1;
         if (valid) {
-            const double tk = static_cast<double>(t) * P.dt;
-            const double2 pp = cexpi(omega * tk);
+            const double2 pp = phase_plus(prow, t, omega, P.dt);
             pm = cconj(pp);
             gg1 = g1[t];
             const double2 gg2 = g2[t];
@@ -673,11 +686,16 @@ struct BcfArgs {
     long nfft;
     const double* kms;     // [nfft / 2] KMS factor used at k (finite beta)
     double2* S;            // [nfft] sampled spectrum
+    // TACO_PHASE_LIBM, Ohmic J: the host's std::exp(-w_k / p1) at w_k =
+    // domega k (k <= nfft/2 - 1) and at omegaNy; null otherwise.
+    const double* efac;
+    double efac_ny;
 };
 
-__device__ __forceinline__ double spectral_J_dev(int kind, double p0, double p1, double w) {
+// J(w); ef < 0 asks for the correctly rounded Ohmic factor, else ef is it.
+__device__ __forceinline__ double spectral_J_dev(int kind, double p0, double p1, double w, double ef) {
     if (!(w > 0.0)) return 0.0;
-    if (kind == 0) return p0 * w * taco_cr::exp_cr(-w / p1, c_exptab);
+    if (kind == 0) return p0 * w * (ef >= 0.0 ? ef : taco_cr::exp_cr(-w / p1, c_exptab));
     return 2.0 * p0 * p1 * w / (w * w + p1 * p1);
 }
 
@@ -711,24 +729,56 @@ __global__ __launch_bounds__(kBlock) void k_bcf_spectrum(BcfArgs B) {
     double v = 0.0;
     const bool zero_t = isinf(B.beta);
     if (k == half) {
-        const double J = spectral_J_dev(B.kind, B.p0, B.p1, B.omegaNy);
+        const double J = spectral_J_dev(B.kind, B.p0, B.p1, B.omegaNy, B.efac ? B.efac_ny : -1.0);
         v = zero_t ? 0.25 * kPiD * J : 0.25 * kPiD * J * (1.0 + 2.0 / (B.eNy - 1.0));
     } else if (k == 0) {
         if (!zero_t) {
             const double ed = B.r;
-            v = 0.5 * (kPiD / 2.0) * spectral_J_dev(B.kind, B.p0, B.p1, B.domega) * (1.0 + ed) / (1.0 - ed);
+            v = 0.5 * (kPiD / 2.0) * spectral_J_dev(B.kind, B.p0, B.p1, B.domega, B.efac ? B.efac[1] : -1.0) *
+                (1.0 + ed) / (1.0 - ed);
         }
     } else if (k <= pos_count) {
         const double w = B.domega * static_cast<double>(k);
-        v = zero_t ? (kPiD / 2.0) * spectral_J_dev(B.kind, B.p0, B.p1, w)
-                   : (kPiD / 2.0) * spectral_J_dev(B.kind, B.p0, B.p1, w) / (1.0 - B.kms[k]);
+        const double ef = B.efac ? B.efac[k] : -1.0;
+        v = zero_t ? (kPiD / 2.0) * spectral_J_dev(B.kind, B.p0, B.p1, w, ef)
+                   : (kPiD / 2.0) * spectral_J_dev(B.kind, B.p0, B.p1, w, ef) / (1.0 - B.kms[k]);
     } else if (!zero_t) {
         const long kk = B.nfft - k;  // 1..posCount
         const double w = B.domega * static_cast<double>(kk);
-        const double Sp = (kPiD / 2.0) * spectral_J_dev(B.kind, B.p0, B.p1, w) / (1.0 - B.kms[kk]);
+        const double Sp = (kPiD / 2.0) * spectral_J_dev(B.kind, B.p0, B.p1, w, B.efac ? B.efac[kk] : -1.0) /
+                          (1.0 - B.kms[kk]);
         v = Sp * B.kms[kk];
     }
     B.S[k] = make_double2(v, 0.0);
+}
+
+// The reference's radix-2 FFT (bit reversal, then each stage's butterflies
+// u +- w x with the recurrence twiddles of the host table) on one thread, for
+// the lengths 2 and 4 below the three-pass kernels' range (bath grids with
+// N <= 1).  tw: stage s at offset 2^(s-1) - 1.
+__global__ void k_fft_tiny(double2* a, int L, const double2* tw) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int n = 1 << L;
+    for (int i = 1, j = 0; i < n; ++i) {
+        int bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) {
+            const double2 t = a[i];
+            a[i] = a[j];
+            a[j] = t;
+        }
+    }
+    for (int half = 1; half < n; half <<= 1) {
+        const double2* w = tw + (half - 1);
+        for (int i = 0; i < n; i += 2 * half)
+            for (int k = 0; k < half; ++k) {
+                const double2 u = a[i + k];
+                const double2 v = cmul(w[k], a[i + k + half]);
+                a[i + k] = cadd(u, v);
+                a[i + k + half] = csub(u, v);
+            }
+    }
 }
 
 // C[n] = scale * X[n], n < nc, from the phase-B spectrum layout
@@ -753,7 +803,7 @@ __global__ __launch_bounds__(kBlock) void k_bcf_gather(const double2* spec, int 
 constexpr int kTzChunk = 1024;
 static_assert(kTzChunk % (kBlock - 128) == 0, "loader waves cover a chunk exactly");
 __global__ __launch_bounds__(kBlock) void k_gamma_trapz(const double2* C, long nc, double dt, const double* omegas,
-                                                        double2* G) {
+                                                        const double2* steps, double2* G) {
     __shared__ double2 cin[2][kTzChunk];   // C of chunk c (c % 2)
     __shared__ double2 ph[2][kTzChunk];    // phi_k of chunk c
     __shared__ double2 uu[2][kTzChunk];    // terms of chunk c
@@ -762,7 +812,8 @@ __global__ __launch_bounds__(kBlock) void k_gamma_trapz(const double2* C, long n
     double2* col = G + static_cast<long>(blockIdx.x) * nc;
     const double2 half_dt = make_double2(dt / 2.0, 0.0);
     double2 step = make_double2(1.0, 0.0), phi = make_double2(1.0, 0.0), acc = make_double2(0.0, 0.0);
-    if (threadIdx.x == 0) step = cexpi(omegas[blockIdx.x] * dt);
+    // the phase step exp(i omega_b dt): the host's value (TACO_PHASE_LIBM) or correctly rounded
+    if (threadIdx.x == 0) step = steps ? steps[blockIdx.x] : cexpi(omegas[blockIdx.x] * dt);
     const long nch = (nc + kTzChunk - 1) / kTzChunk;
     for (long c = 0; c <= nch + 1; ++c) {
         const int cb = static_cast<int>(c & 1), pb = cb ^ 1;
@@ -1562,6 +1613,11 @@ __global__ __launch_bounds__(kBlock) void k_tcl2(Tcl2Args A) {
 // ---------------------------------------------------------------------------
 // Context and device workspace
 // ---------------------------------------------------------------------------
+// Counts device reallocations of every DevBuf in the process.  A captured
+// graph holds raw device addresses; a batch replays its graph only while
+// this count is the one it was captured at, and captures again otherwise.
+static std::atomic<unsigned long> g_dev_realloc{0};
+
 struct DevBuf {
     void* p = nullptr;
     std::size_t bytes = 0;
@@ -1570,6 +1626,7 @@ struct DevBuf {
         if (p) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
+        g_dev_realloc.fetch_add(1);
         if (need == 0) return;
         HIP_OK(hipMalloc(&p, need));
         bytes = need;
@@ -1590,17 +1647,37 @@ struct Twiddles {
     int L = -1;
 };
 
+// Host-evaluated phase table of the TACO_PHASE_LIBM mode (rows of
+// exp(+i Omega (t dt)), t < len) and the key it was built for: the bit
+// patterns of its Omegas, len and dt.
+struct PhaseTab {
+    DevBuf tab;
+    std::vector<std::uint64_t> rows;
+    std::size_t len = 0;
+    std::uint64_t dt_bits = 0;
+    bool valid = false;
+    void swap(PhaseTab& o) {
+        tab.swap(o.tab);
+        rows.swap(o.rows);
+        std::swap(len, o.len);
+        std::swap(dt_bits, o.dt_bits);
+        std::swap(valid, o.valid);
+    }
+};
+
 // Device buffers of one tape.  A destroyed tape hands them back to its
 // context, and the next tape on the context takes them over, so repeated
 // gradients run without device allocations once the sizes have settled.
 struct TapeBufs {
-    DevBuf gamma, omegas, mirror, p2f, ops, eps, t2pos, jobtri, jobmask, clsstart, slices, lgen, ck;
+    DevBuf gamma, omegas, mirror, p2f, ops, eps, t2pos, jobtri, jobmask, clsstart, jobrow, slices, lgen, ck;
+    PhaseTab phtab;
     void swap(TapeBufs& o) {
         for (auto [a, b] : {std::pair{&gamma, &o.gamma}, {&omegas, &o.omegas}, {&mirror, &o.mirror}, {&p2f, &o.p2f},
                             {&ops, &o.ops}, {&eps, &o.eps}, {&t2pos, &o.t2pos}, {&jobtri, &o.jobtri},
-                            {&jobmask, &o.jobmask}, {&clsstart, &o.clsstart}, {&slices, &o.slices},
-                            {&lgen, &o.lgen}, {&ck, &o.ck}})
+                            {&jobmask, &o.jobmask}, {&clsstart, &o.clsstart}, {&jobrow, &o.jobrow},
+                            {&slices, &o.slices}, {&lgen, &o.lgen}, {&ck, &o.ck}})
             a->swap(*b);
+        phtab.swap(o.phtab);
     }
 };
 
@@ -1670,7 +1747,8 @@ struct taco_ctx {
     std::map<int, std::array<int, 3>> fft_choice;  // per FFT log-length: tile config of phases A, B, C
     std::map<int, std::unique_ptr<taco_engine::Twiddles>> tw;
     taco_engine::DevBuf gamma, omegas, mirror, p2f, ops, spec, inter1, inter2, conv, slices, t2pos, l4, gw, misc,
-        rk, eps, jobtri, clsstart, jobmask;
+        rk, eps, jobtri, clsstart, jobmask, jobrow;
+    taco_engine::PhaseTab phtab;  // phase table of the last plain-path system (TACO_PHASE_LIBM)
     std::size_t l4_lds_set = 0;  // dynamic LDS limit set on the fused L4 kernel
     std::vector<std::unique_ptr<taco_engine::TapeBufs>> tape_pool;  // buffers of destroyed tapes (at most 2)
 };
@@ -1934,7 +2012,66 @@ struct FcrLists {
     DevBuf* jobtri;
     DevBuf* jobmask;
     DevBuf* clsstart;
+    DevBuf* jobrow;   // per sorted position: phase-table row (TACO_PHASE_LIBM)
+    PhaseTab* phtab;  // the pass's phase table (TACO_PHASE_LIBM)
 };
+
+// TACO_PHASE_LIBM: the platform libm's exp(+i Omega (t dt)) for every
+// distinct Omega of the pass (in the kernels' expression (w_i + w_j) + w_k),
+// t < Nt, evaluated on the host and kept in the owner's table; rebuilt only
+// when the Omegas, Nt or dt change.  Each sorted position gets its row.
+struct PhaseRows {
+    const double2* tab = nullptr;
+    const int* rows = nullptr;
+    long len = 0;
+};
+
+static std::size_t libm_table_budget() {
+    const char* v = std::getenv("TACO_LIBM_TABLE_BYTES");
+    return v ? static_cast<std::size_t>(std::strtoull(v, nullptr, 10)) : (std::size_t{16} << 30);
+}
+
+static PhaseRows phase_rows(taco_ctx* c, const FcrPlan& pl, const std::vector<int>& order,
+                            const std::vector<std::uint64_t>& key, const FcrLists& lists) {
+    PhaseRows out;
+    if (phase_mode() != TACO_PHASE_LIBM || order.empty()) return out;
+    std::vector<std::uint64_t> uniq(order.size());
+    for (std::size_t q = 0; q < order.size(); ++q) uniq[q] = key[static_cast<std::size_t>(order[q])];
+    std::sort(uniq.begin(), uniq.end());
+    uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+    std::vector<int> row(order.size());
+    for (std::size_t q = 0; q < order.size(); ++q)
+        row[q] = static_cast<int>(std::lower_bound(uniq.begin(), uniq.end(), key[static_cast<std::size_t>(order[q])]) -
+                                  uniq.begin());
+    const std::size_t len = pl.Nt;
+    if (uniq.size() > libm_table_budget() / 16 / len)
+        throw Error(TACO_ERR_INVALID, "libm phase table of " + std::to_string(uniq.size()) + " x " + std::to_string(len) +
+                                          " values exceeds TACO_LIBM_TABLE_BYTES; use the TACO_PHASE_CR phase mode");
+    std::uint64_t dt_bits;
+    std::memcpy(&dt_bits, &pl.dt, 8);
+    PhaseTab& T = *lists.phtab;
+    if (!(T.valid && T.len == len && T.dt_bits == dt_bits && T.rows == uniq)) {
+        T.valid = false;
+        std::vector<double> om(uniq.size());
+        for (std::size_t r = 0; r < uniq.size(); ++r) std::memcpy(&om[r], &uniq[r], 8);
+        std::vector<double> host(2 * uniq.size() * len);
+        libm_phase_table(om.data(), uniq.size(), len, pl.dt, host.data());
+        T.tab.ensure(host.size() * sizeof(double));
+        HIP_OK(hipMemcpyAsync(T.tab.p, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));  // the host vector goes out of scope
+        T.rows = std::move(uniq);
+        T.len = len;
+        T.dt_bits = dt_bits;
+        T.valid = true;
+    }
+    lists.jobrow->ensure(row.size() * sizeof(int));
+    HIP_OK(hipMemcpyAsync(lists.jobrow->p, row.data(), row.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    out.tab = T.tab.as<double2>();
+    out.rows = lists.jobrow->as<int>();
+    out.len = static_cast<long>(len);
+    return out;
+}
 
 static FcrJob fcr_prepare(taco_ctx* c, const FcrPlan& pl, const FcrLists& lists) {
     const std::size_t n = fcr_nfft(pl.Nt, g_pad_factor);
@@ -2008,6 +2145,7 @@ static FcrJob fcr_prepare(taco_ctx* c, const FcrPlan& pl, const FcrLists& lists)
     }
     HIP_OK(hipMemcpyAsync(lists.clsstart->p, cls_start.data(), cls_start.size() * sizeof(int),
                           hipMemcpyHostToDevice, c->stream));
+    const PhaseRows prow = phase_rows(c, pl, order, key, lists);
 
     // FFT chunk (intermediates of nj triples) and combine batch (conv slots).
     const long chunk = chunk_len;
@@ -2042,6 +2180,9 @@ static FcrJob fcr_prepare(taco_ctx* c, const FcrPlan& pl, const FcrLists& lists)
     P.cls_start = lists.clsstart->as<int>();
     P.cconv = cconv;
     P.T = static_cast<long>(pl.T);
+    P.ph_tab = prow.tab;
+    P.job_row = prow.rows;
+    P.ph_len = prow.len;
     J.cls_start = std::move(cls_start);
     J.paths.assign(static_cast<std::size_t>(ncomb) + 1, 0);
     J.outs.assign(static_cast<std::size_t>(ncomb) + 1, 0);
@@ -2133,7 +2274,7 @@ static void fcr_launch(taco_ctx* c, FcrJob& J, const CombineArgs& out) {
 }
 
 static void run_fcr(taco_ctx* c, const FcrPlan& pl, const CombineArgs& out) {
-    FcrJob J = fcr_prepare(c, pl, FcrLists{&c->jobtri, &c->jobmask, &c->clsstart});
+    FcrJob J = fcr_prepare(c, pl, FcrLists{&c->jobtri, &c->jobmask, &c->clsstart, &c->jobrow, &c->phtab});
     fcr_launch(c, J, out);
 }
 
@@ -2345,7 +2486,8 @@ struct BatchSys {
     double dt = 0.0;
     long npos = 0;
     DevBuf gamma, omegas, mirror, p2f, ops, eps, r0;
-    DevBuf jobtri, jobmask, clsstart, t2pos, rows_out, rows_gen;
+    DevBuf jobtri, jobmask, clsstart, jobrow, t2pos, rows_out, rows_gen;
+    PhaseTab phtab;  // this system's phase table (TACO_PHASE_LIBM)
     DevBuf l4out;  // [ntau][N^4] at the requested times, request order
     DevBuf lgen;   // [steps + 1][N^4] generator series of the RK4 run
     DevBuf rho;    // [steps + 1][N^2] samples, and the final vector after them
@@ -2361,11 +2503,15 @@ struct BatchRun {
     double rk_bytes = 0.0;  // algorithmic bytes of the RK4 launch (generators read, samples written)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    unsigned long gen = 0;  // g_dev_realloc when the graph was captured
     bool ran_eager = false;
-    ~BatchRun() {
+    void drop_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
     }
+    ~BatchRun() { drop_graph(); }
 };
 
 // Enqueues one system of a batch: F/C/R at its kept times, fused L4 into the
@@ -2546,7 +2692,7 @@ static void tape_forward(taco_tape& tp, const double* r0, double* samples_out) {
         const std::vector<unsigned char> need = needed_triples(tp.h_p2f.data(), tp.N, tp.nf);
         FcrPlan pl{tp.gamma.as<double2>(), tp.Nt, tp.nf, tp.omegas.as<double>(), tp.h_omegas.data(),
                    tp.mirror.as<int>(), tp.dt, T, &need};
-        tp.fcr = fcr_prepare(c, pl, FcrLists{&tp.jobtri, &tp.jobmask, &tp.clsstart});
+        tp.fcr = fcr_prepare(c, pl, FcrLists{&tp.jobtri, &tp.jobmask, &tp.clsstart, &tp.bufs.jobrow, &tp.bufs.phtab});
         if (T > (~std::size_t{0} / 48) / nf3) throw Error(TACO_ERR_OVERFLOW, "FCR buffer too large");
         tp.slices.ensure(3 * nf3 * T * 16);
         CombineArgs out{};
@@ -2796,6 +2942,9 @@ static void tape_adjoint(taco_tape& tp, const double* samples_bar, double* gamma
             P.nf3 = static_cast<long>(nf3);
             P.scr = scr;
             P.om = om;
+            P.ph_tab = J.P.ph_tab;
+            P.job_row = J.P.job_row;
+            P.ph_len = J.P.ph_len;
             const unsigned nub = static_cast<unsigned>((T + kBlock - 1) / kBlock);
             timed(c, K_COMBINE, 0.0, [&] {
                 hipLaunchKernelGGL(k_fcr_adj_scan, dim3(static_cast<unsigned>(ncomb)), dim3(kBlock), 0, c->stream, P);
@@ -2908,7 +3057,6 @@ constexpr double kPiHost = 3.141592653589793238462643383279502884;
 static void bcf_device(taco_ctx* c, std::size_t N, double dt, int kind, double p0, double p1, double beta, double2* dC) {
     if (!(dt > 0.0)) throw Error(TACO_ERR_INVALID, "bcf_fft: dt must be > 0");
     if (kind != 0 && kind != 1) throw Error(TACO_ERR_INVALID, "bcf_fft: unknown spectral density kind");
-    if (N < 1) throw Error(TACO_ERR_INVALID, "bcf_fft on the GPU needs N >= 1 (two or more time samples)");
     const std::size_t Nt = N + 1;
     std::size_t Nfft = 2 * Nt;
     if ((Nfft & (Nfft - 1)) != 0) Nfft = next_pow2(Nfft);
@@ -2920,7 +3068,9 @@ static void bcf_device(taco_ctx* c, std::size_t N, double dt, int kind, double p
     const double omegaNy = kPiHost / dt;
     const bool zero_t = std::isinf(beta);
     const long half = static_cast<long>(Nfft / 2);
-    c->misc.ensure(Nfft * 16 + static_cast<std::size_t>(half + 1) * 8);
+    const bool libm_ef = phase_mode() == TACO_PHASE_LIBM && kind == TACO_J_OHMIC;
+    // misc: S [Nfft], twiddles of the tiny FFT [4], kms [half + 1], Ohmic factors [half]
+    c->misc.ensure((Nfft + 4) * 16 + static_cast<std::size_t>(half + 1) * 8 * (libm_ef ? 2 : 1));
     BcfArgs B{};
     B.kind = kind;
     B.p0 = p0;
@@ -2928,15 +3078,38 @@ static void bcf_device(taco_ctx* c, std::size_t N, double dt, int kind, double p
     B.beta = beta;
     B.domega = domega;
     B.omegaNy = omegaNy;
-    B.r = zero_t ? 0.0 : taco_cr::exp_cr(-beta * domega, exp_table());
-    B.eNy = zero_t ? 0.0 : taco_cr::exp_cr(beta * omegaNy, exp_table());
+    B.r = zero_t ? 0.0 : exp_in_mode(-beta * domega);
+    B.eNy = zero_t ? 0.0 : exp_in_mode(beta * omegaNy);
     B.nfft = static_cast<long>(Nfft);
     B.S = c->misc.as<double2>();
-    double* kms = reinterpret_cast<double*>(B.S + Nfft);
+    double2* d_tw = B.S + Nfft;
+    double* kms = reinterpret_cast<double*>(d_tw + 4);
     B.kms = kms;
+    if (libm_ef) {  // the Ohmic factor of J at every sampled frequency, by the host's libm
+        // entries 1..max(posCount, 1): the DC term reads J(domega) even when posCount = 0
+        std::vector<double> ef(static_cast<std::size_t>(half + 1), 0.0);
+        for (long k = 1; k < std::max(half, 2L); ++k)
+            ef[static_cast<std::size_t>(k)] = std::exp(-(domega * static_cast<double>(k)) / p1);
+        double* d_ef = kms + half + 1;
+        HIP_OK(hipMemcpyAsync(d_ef, ef.data(), ef.size() * 8, hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        B.efac = d_ef;
+        B.efac_ny = std::exp(-omegaNy / p1);
+    }
     if (!zero_t) hipLaunchKernelGGL(k_bcf_kms, dim3(1), dim3(64), 0, c->stream, B.r, half - 1, kms);
     hipLaunchKernelGGL(k_bcf_spectrum, dim3(static_cast<unsigned>((Nfft + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                        c->stream, B);
+    if (L <= 2) {  // Nfft 2 or 4 (N <= 1): the reference's loop on one thread
+        std::vector<double> tab;
+        make_twiddles(L, tab);
+        HIP_OK(hipMemcpyAsync(d_tw, tab.data(), tab.size() * 8, hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_fft_tiny, dim3(1), dim3(64), 0, c->stream, B.S, L, d_tw);
+        hipLaunchKernelGGL(k_bcf_gather, dim3(1), dim3(kBlock), 0, c->stream, B.S, L, 0, static_cast<long>(Nt),
+                           domega / kPiHost, dC);
+        HIP_OK(hipStreamSynchronize(c->stream));  // the host table goes out of scope
+        HIP_OK(hipGetLastError());
+        return;
+    }
     const Twiddles& tw = twiddles_for(c, L);
     FcrArgs P{};
     P.gamma = B.S;
@@ -2964,12 +3137,22 @@ static void bcf_device(taco_ctx* c, std::size_t N, double dt, int kind, double p
 }
 
 // Trapz Gamma [nf][Nc] from device C [Nc] and device bucket frequencies.
-static void trapz_device(taco_ctx* c, const double2* dC, std::size_t Nc, double dt, const double* d_om, int nf,
-                  double2* dG) {
+// d_steps: room for nf complex values; in TACO_PHASE_LIBM it receives the
+// host's phase steps std::exp(i omega_b dt) (gamma.hpp:185).
+static void trapz_device(taco_ctx* c, const double2* dC, std::size_t Nc, double dt, const double* h_om,
+                         const double* d_om, int nf, double2* d_steps, double2* dG) {
     if (!(dt > 0.0)) throw Error(TACO_ERR_INVALID, "gamma_trapz: dt must be > 0");
     if (Nc == 0 || nf <= 0) return;
+    const double2* steps = nullptr;
+    if (phase_mode() == TACO_PHASE_LIBM) {
+        std::vector<std::complex<double>> st(static_cast<std::size_t>(nf));
+        for (int b = 0; b < nf; ++b) st[static_cast<std::size_t>(b)] = expi_in_mode(h_om[b] * dt);
+        HIP_OK(hipMemcpyAsync(d_steps, st.data(), st.size() * 16, hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        steps = d_steps;
+    }
     hipLaunchKernelGGL(k_gamma_trapz, dim3(static_cast<unsigned>(nf)), dim3(kBlock), 0, c->stream, dC,
-                       static_cast<long>(Nc), dt, d_om, dG);
+                       static_cast<long>(Nc), dt, d_om, steps, dG);
     HIP_OK(hipGetLastError());
 }
 }  // namespace taco_engine
@@ -2983,6 +3166,11 @@ const char* taco_version(void) { return "taco-mi355x 0.1.0 (gfx950)"; }
 
 void taco_set_fcr_fft_pad_factor(size_t factor) { g_pad_factor = factor; }
 size_t taco_get_fcr_fft_pad_factor(void) { return g_pad_factor; }
+
+int taco_set_phase_mode(int mode) {
+    return guarded([&] { set_phase_mode(mode); });
+}
+int taco_get_phase_mode(void) { return phase_mode(); }
 
 int taco_ctx_create(int device, taco_ctx** out) {
     return guarded([&] {
@@ -3110,14 +3298,15 @@ int taco_bath_gamma_hip(taco_ctx* ctx, size_t N, double dt, int kind, double p0,
         if (nf > 0 && (!omegas || !gamma_out)) throw Error(TACO_ERR_INVALID, "omegas and gamma_out are required");
         DeviceGuard g(ctx->device);
         const std::size_t Nt = N + 1;
-        ctx->inter2.ensure((Nt + static_cast<std::size_t>(nf) * Nt) * 16 + static_cast<std::size_t>(nf + 1) * 8);
+        ctx->inter2.ensure((Nt + static_cast<std::size_t>(nf) * Nt + nf) * 16 + static_cast<std::size_t>(nf + 1) * 8);
         double2* dC = ctx->inter2.as<double2>();
         double2* dG = dC + Nt;
-        double* d_om = reinterpret_cast<double*>(dG + static_cast<std::size_t>(nf) * Nt);
+        double2* d_steps = dG + static_cast<std::size_t>(nf) * Nt;
+        double* d_om = reinterpret_cast<double*>(d_steps + nf);
         bcf_device(ctx, N, dt, kind, p0, p1, beta, dC);
         if (nf > 0) {
             HIP_OK(hipMemcpyAsync(d_om, omegas, static_cast<std::size_t>(nf) * 8, hipMemcpyHostToDevice, ctx->stream));
-            trapz_device(ctx, dC, Nt, dt, d_om, nf, dG);
+            trapz_device(ctx, dC, Nt, dt, omegas, d_om, nf, d_steps, dG);
             HIP_OK(hipMemcpyAsync(gamma_out, dG, static_cast<std::size_t>(nf) * Nt * 16, hipMemcpyDeviceToHost,
                                   ctx->stream));
         }
@@ -3133,13 +3322,14 @@ int taco_gamma_trapz_hip(taco_ctx* ctx, const double* C, size_t Nc, double dt, c
         if (nf < 0) throw Error(TACO_ERR_INVALID, "nf must be >= 0");
         if (Nc == 0 || nf == 0) return;
         DeviceGuard g(ctx->device);
-        ctx->inter2.ensure((Nc + static_cast<std::size_t>(nf) * Nc) * 16 + static_cast<std::size_t>(nf + 1) * 8);
+        ctx->inter2.ensure((Nc + static_cast<std::size_t>(nf) * Nc + nf) * 16 + static_cast<std::size_t>(nf + 1) * 8);
         double2* dC = ctx->inter2.as<double2>();
         double2* dG = dC + Nc;
-        double* d_om = reinterpret_cast<double*>(dG + static_cast<std::size_t>(nf) * Nc);
+        double2* d_steps = dG + static_cast<std::size_t>(nf) * Nc;
+        double* d_om = reinterpret_cast<double*>(d_steps + nf);
         HIP_OK(hipMemcpyAsync(dC, C, Nc * 16, hipMemcpyHostToDevice, ctx->stream));
         HIP_OK(hipMemcpyAsync(d_om, omegas, static_cast<std::size_t>(nf) * 8, hipMemcpyHostToDevice, ctx->stream));
-        trapz_device(ctx, dC, Nc, dt, d_om, nf, dG);
+        trapz_device(ctx, dC, Nc, dt, omegas, d_om, nf, d_steps, dG);
         HIP_OK(hipMemcpyAsync(gamma_out, dG, static_cast<std::size_t>(nf) * Nc * 16, hipMemcpyDeviceToHost,
                               ctx->stream));
         HIP_OK(hipStreamSynchronize(ctx->stream));
@@ -3465,7 +3655,7 @@ int taco_batch_create(taco_ctx* ctx, const taco_batch_system* systems, int nsys,
                 const std::vector<unsigned char> need = needed_triples(d.pair_to_freq, d.N, d.nf);
                 FcrPlan pl{S->gamma.as<double2>(), d.Nt, d.nf, S->omegas.as<double>(), d.omegas, S->mirror.as<int>(),
                            d.dt, tp.T, &need};
-                S->fcr = fcr_prepare(ctx, pl, FcrLists{&S->jobtri, &S->jobmask, &S->clsstart});
+                S->fcr = fcr_prepare(ctx, pl, FcrLists{&S->jobtri, &S->jobmask, &S->clsstart, &S->jobrow, &S->phtab});
                 const std::size_t nf3 = static_cast<std::size_t>(d.nf) * d.nf * d.nf;
                 if (static_cast<std::size_t>(S->npos) > (~std::size_t{0} / 48) / nf3)
                     throw Error(TACO_ERR_OVERFLOW, "FCR buffer too large");
@@ -3522,6 +3712,10 @@ int taco_batch_run(taco_batch* b, int use_graph) {
             B.ran_eager = true;
             if (!use_graph) return;
         }
+        // The graph holds the addresses of the context's workspace; a call
+        // that has reallocated any device buffer since the capture makes it
+        // stale, so it is captured again.
+        if (B.exec && B.gen != g_dev_realloc.load()) B.drop_graph();
         if (!B.exec) {
             const bool prof = c->prof.on;
             c->prof.on = false;
@@ -3538,6 +3732,7 @@ int taco_batch_run(taco_batch* b, int use_graph) {
             c->prof.on = prof;
             HIP_OK(hipStreamEndCapture(c->stream, &B.graph));
             HIP_OK(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+            B.gen = g_dev_realloc.load();
         }
         HIP_OK(hipGraphLaunch(B.exec, c->stream));
         HIP_OK(hipStreamSynchronize(c->stream));

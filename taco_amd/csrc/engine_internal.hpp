@@ -40,6 +40,20 @@ Tcl2Parts tcl2_parts(const std::complex<double>* ops, int num_ops, int N, const 
 void tcl2_ops_adjoint(const Tcl2Parts& parts, const std::complex<double>* W, const std::complex<double>* V, int N,
                       const int* p2f, std::complex<double>* ops_bar);
 
+// Phase and exponential convention (TACO_PHASE_LIBM or TACO_PHASE_CR),
+// process-global like the pad factor.
+int phase_mode();
+void set_phase_mode(int mode);
+// exp(x) in the current convention: the platform libm's std::exp, or the
+// correctly rounded value.
+double exp_in_mode(double x);
+// exp(i x) = (cos x, sin x) in the current convention.
+std::complex<double> expi_in_mode(double x);
+// out[r][k] = std::exp(i * (omega[r] * (k * dt))), k < len, interleaved
+// (re, im): the platform libm's values of the reference's F/C/R phases,
+// evaluated on all host cores.
+void libm_phase_table(const double* omega, std::size_t nrows, std::size_t len, double dt, double* out);
+
 void set_last_error(const std::string& msg);
 const double* cr_table();
 void cr_sincos_host(const double* x, std::size_t n, double* s, double* c);

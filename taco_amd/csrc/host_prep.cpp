@@ -106,12 +106,53 @@ static void fft_host(std::vector<cd>& a, bool forward) {
     }
 }
 
+static int g_phase_mode = TACO_PHASE_LIBM;
+
+int phase_mode() { return g_phase_mode; }
+
+void set_phase_mode(int mode) {
+    if (mode != TACO_PHASE_LIBM && mode != TACO_PHASE_CR) throw Error(TACO_ERR_INVALID, "unknown phase mode");
+    g_phase_mode = mode;
+}
+
+double exp_in_mode(double x) {
+    return g_phase_mode == TACO_PHASE_LIBM ? std::exp(x) : taco_cr::exp_cr(x, exp_table());
+}
+
+cd expi_in_mode(double x) {
+    if (g_phase_mode == TACO_PHASE_LIBM) return std::exp(cd{0.0, x});
+    double s, c;
+    taco_cr::sincos_cr(x, &s, &c, cr_table());
+    return cd(c, s);
+}
+
+void libm_phase_table(const double* omega, std::size_t nrows, std::size_t len, double dt, double* out) {
+    constexpr std::size_t kPiece = 1 << 14;
+    const std::size_t pieces = (len + kPiece - 1) / kPiece;
+    const long total = static_cast<long>(nrows * pieces);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long w = 0; w < total; ++w) {
+        const std::size_t r = static_cast<std::size_t>(w) / pieces, k0 = (static_cast<std::size_t>(w) % pieces) * kPiece;
+        const std::size_t k1 = std::min(len, k0 + kPiece);
+        const double om = omega[r];
+        double* row = out + 2 * r * len;
+        for (std::size_t k = k0; k < k1; ++k) {
+            const double tk = static_cast<double>(k) * dt;  // tcl4_kernels.cpp:116-120
+            const cd z = std::exp(cd{0.0, om * tk});
+            row[2 * k] = z.real();
+            row[2 * k + 1] = z.imag();
+        }
+    }
+}
+
 // The exponentials of the bath correlation (J's Ohmic factor, the KMS
-// factors) and the trapz phase step are correctly rounded (cr_exp.hpp,
-// cr_sincos.hpp), the values the device producer computes with the same code.
+// factors) and the trapz phase step follow the phase mode: the platform
+// libm's std::exp, as the reference calls, or the correctly rounded values
+// (cr_exp.hpp, cr_sincos.hpp) that the device producer computes with the
+// same code.
 static double spectral_J(int kind, double p0, double p1, double w) {
     if (!(w > 0.0)) return 0.0;
-    if (kind == 0) return p0 * w * taco_cr::exp_cr(-w / p1, exp_table());
+    if (kind == 0) return p0 * w * exp_in_mode(-w / p1);
     return 2.0 * p0 * p1 * w / (w * w + p1 * p1);
 }
 
@@ -133,7 +174,7 @@ void bcf_fft(std::size_t N, double dt, int kind, double p0, double p1, double be
         }
         S[Nfft / 2] = cd(0.25 * kPi * spectral_J(kind, p0, p1, omegaNy), 0.0);
     } else {
-        const double r = taco_cr::exp_cr(-beta * domega, exp_table());
+        const double r = exp_in_mode(-beta * domega);
         double KMS = r;
         for (std::size_t k = 1; k <= posCount; ++k) {
             const double w = domega * static_cast<double>(k);
@@ -144,7 +185,7 @@ void bcf_fft(std::size_t N, double dt, int kind, double p0, double p1, double be
         }
         const double ed = r;
         S[0] = cd(0.5 * (kPi / 2.0) * spectral_J(kind, p0, p1, domega) * (1.0 + ed) / (1.0 - ed), 0.0);
-        const double eNy = taco_cr::exp_cr(beta * omegaNy, exp_table());
+        const double eNy = exp_in_mode(beta * omegaNy);
         S[Nfft / 2] = cd(0.25 * kPi * spectral_J(kind, p0, p1, omegaNy) * (1.0 + 2.0 / (eNy - 1.0)), 0.0);
     }
     fft_host(S, true);
@@ -164,9 +205,7 @@ void gamma_trapz(const double* Cin, std::size_t N, double dt, const double* omeg
     for (int j = 0; j < M; ++j) {
         cd* col = G + static_cast<std::size_t>(j) * N;
         col[0] = cd(0.0, 0.0);
-        double ss, sc;
-        taco_cr::sincos_cr(omegas[j] * dt, &ss, &sc, cr_table());
-        const cd step(sc, ss);
+        const cd step = expi_in_mode(omegas[j] * dt);  // gamma.hpp:185
         cd phi(1.0, 0.0), acc(0.0, 0.0), prev = C[0];
         for (std::size_t k = 1; k < N; ++k) {
             const cd phi_next = phi * step;

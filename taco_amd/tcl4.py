@@ -40,7 +40,8 @@ __all__ = [
     "build_TCL4_generator_cuda_fused", "build_TCL4_generator_cuda_fused_batch", "build_correction_series_cuda_fused",
     "build_TCL4_generator", "build_correction_series", "assemble_l4_from_slices", "propagate_rk4_dense",
     "propagate_rk4_dense_serial", "propagate_rk4_dense_omp", "tcl4_propagate", "set_fcr_fft_pad_factor",
-    "get_fcr_fft_pad_factor", "context", "version", "BatchItem", "Batch", "batched_run",
+    "get_fcr_fft_pad_factor", "set_phase_mode", "get_phase_mode", "phase_mode", "PHASE_LIBM", "PHASE_CR",
+    "context", "version", "BatchItem", "Batch", "batched_run",
 ]
 
 
@@ -80,6 +81,37 @@ def set_fcr_fft_pad_factor(factor: int) -> None:
 
 def get_fcr_fft_pad_factor() -> int:
     return int(lib().taco_get_fcr_fft_pad_factor())
+
+
+PHASE_LIBM = 0  # the platform libm's std::exp values (the reference's arithmetic); default
+PHASE_CR = 1    # correctly rounded cos/sin/exp of the same arguments, evaluated on the device
+
+
+def set_phase_mode(mode: int) -> None:
+    """Process-global phase/exponential convention (include/taco_tcl4.h):
+    PHASE_LIBM reproduces the reference's std::exp calls
+    (tcl4_kernels.cpp:116-120, gamma.hpp:185, correlation_fft.hpp:154-214)
+    through host-evaluated tables; PHASE_CR uses correctly rounded values."""
+    check(lib().taco_set_phase_mode(int(mode)))
+
+
+def get_phase_mode() -> int:
+    return int(lib().taco_get_phase_mode())
+
+
+class phase_mode:
+    """Context manager that switches the phase mode and restores it."""
+
+    def __init__(self, mode: int):
+        self.mode = mode
+
+    def __enter__(self):
+        self.prev = get_phase_mode()
+        set_phase_mode(self.mode)
+        return self
+
+    def __exit__(self, *exc):
+        set_phase_mode(self.prev)
 
 
 # --- contexts: one per (host thread, device), like the reference's thread_local workspace

@@ -97,6 +97,34 @@ def test_graph_replays_equal_eager():
     b.close()
 
 
+def test_graph_replay_after_workspace_growth():
+    """Calls that grow the context's device workspace between replays (a
+    larger plain build, a second and larger batch, a gradient tape) move the
+    buffers the captured graph addressed; the batch captures again and its
+    replays stay bit-identical to the eager run."""
+    items, _ = small_batch()
+    b = taco_amd.Batch(items[:2]).run()
+    eager_l4 = [b.l4(s) for s in range(2)]
+    eager_rho = [b.rho(s) for s in range(2)]
+    b.run(graph=True)
+    big = W.gue_pair(N_bath=30000, tmax=2500)
+    taco_amd.build_TCL4_generator_hip_fused_batch(big.system, big.gamma, big.dt, big.tidx)
+    b2 = taco_amd.Batch(items).run(graph=True)
+    for rep in range(2):
+        b.run(graph=True)
+        for s in range(2):
+            assert np.array_equal(b.l4(s), eager_l4[s]), (rep, s)
+            assert np.array_equal(b.rho(s), eager_rho[s]), (rep, s)
+        b2.run(graph=True)
+        w = W.gue_pair(N_bath=40000 + 20000 * rep, tmax=3000)
+        taco_amd.build_TCL4_generator_hip_fused_batch(w.system, w.gamma, w.dt, w.tidx)
+    for s, it in enumerate(items):
+        plain = taco_amd.build_TCL4_generator_hip_fused_batch(it.system, it.gamma, it.dt, list(it.tidx))
+        assert np.array_equal(b2.l4(s), plain), s
+    b.close()
+    b2.close()
+
+
 def test_batch_error_paths():
     sb = W.spin_boson(Nt=256, tmax=20, tstep=2)
     S, G = sb.system, sb.gamma

@@ -31,13 +31,24 @@ CASES = [
     ("ohmic_zero_T", taco_amd.Ohmic(0.1, 3.0), float("inf"), 777, 0.01),
     ("drude_hot", taco_amd.DrudeLorentz(0.2, 20.0), 0.2, 2 ** 12, 0.005),
     ("tiny", taco_amd.Ohmic(0.3, 1.0), 2.0, 3, 0.1),
+    ("two_samples", taco_amd.Ohmic(0.3, 1.0), 2.0, 1, 0.1),
+    ("one_sample", taco_amd.Ohmic(0.3, 1.0), 2.0, 0, 0.1),
+    ("one_sample_drude", taco_amd.DrudeLorentz(0.1, 3.0), 0.7, 0, 0.05),
 ]
 
 
+@pytest.mark.parametrize("mode", ["PHASE_LIBM", "PHASE_CR"])
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
-def test_bath_gamma_bitwise_equals_host(case):
+def test_bath_gamma_bitwise_equals_host(case, mode):
+    """In each phase convention: TACO_PHASE_LIBM feeds the device the host's
+    std::exp values, TACO_PHASE_CR evaluates them on the device."""
     name, bath, beta, N, dt = case
     om = np.array([0.0, 1.25, -1.25, 2.5, -0.3333, 7.0])
+    with taco_amd.phase_mode(getattr(taco_amd, mode)):
+        check_bath_case(name, bath, beta, N, dt, om)
+
+
+def check_bath_case(name, bath, beta, N, dt, om):
     _, C_h = taco_amd.bcf_fft_fun(N, dt, bath, beta)
     G_h = taco_amd.compute_trapz_prefix_multi_matrix(C_h, dt, om)
     C_d, G_d = taco_amd.bath_gamma_hip(N, dt, bath, beta, om)
@@ -47,7 +58,3 @@ def test_bath_gamma_bitwise_equals_host(case):
     assert none is None and np.array_equal(C_only, C_h)
     assert np.array_equal(taco_amd.compute_trapz_prefix_multi_matrix_hip(C_h, dt, om), G_h)
 
-
-def test_device_bath_producer_rejects_a_single_sample():
-    with pytest.raises(ValueError):
-        taco_amd.bath_gamma_hip(0, 0.01, taco_amd.Ohmic(0.1, 1.0), 1.0)

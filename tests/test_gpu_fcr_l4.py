@@ -103,6 +103,73 @@ def test_l4_gue_pair_reduced_grid(orc):
     l4_check(orc, w, w.tidx)
 
 
+def test_l4_spin_boson_cr_phases(orc_cr):
+    """TACO_PHASE_CR (correctly rounded phases on the device) against the
+    oracle in the same convention."""
+    w = W.spin_boson()
+    l4_check(orc_cr, w, w.tidx)
+
+
+def test_l4_gue_pair_reduced_grid_cr_phases(orc_cr):
+    w = W.gue_pair(N_bath=20000, tmax=1000)
+    l4_check(orc_cr, w, w.tidx)
+
+
+def test_libm_phase_early_t_matches_reference_arithmetic(orc):
+    """In the libm convention the first steps (pure FFT rounding noise, where
+    the two conventions differ by ~1e-9) match the oracle as closely as the
+    later ones."""
+    w = W.gue_pair(N_bath=20000, tmax=40)
+    got, ref = l4_check(orc, w, w.tidx)
+    per_t, _ = per_t_rel_err(got, ref)
+    assert per_t[1:20].max() <= 1e-11, per_t[:20]
+
+
+def test_pure_dephasing_l4_vanishes_on_the_gpu(orc):
+    """[H, A] = 0: the fourth-order TCL generator of a Gaussian bath is zero
+    (the oracle gives exact zeros); the engine's sums leave at most rounding."""
+    H = 0.5 * 1.3 * np.diag([1.0, -1.0]).astype(complex)
+    A = np.diag([1.0, -1.0]).astype(complex)
+    w = W.Workload("dephasing", H, [A], taco_amd.Ohmic(0.05, 5.0), 1.0, 0.01, 4095, list(range(0, 2000, 7)))
+    got = taco_amd.build_TCL4_generator_hip_fused_batch(w.system, w.gamma, w.dt, w.tidx)
+    sb = W.spin_boson(Nt=4096, tmax=2000, tstep=7)
+    scale = np.abs(taco_amd.build_TCL4_generator_hip_fused_batch(sb.system, sb.gamma, sb.dt, sb.tidx)).max()
+    assert np.abs(got).max() <= 1e-14 * scale, np.abs(got).max() / scale
+
+
+def test_gpu_l4_converges_to_bruteforce_tcl4():
+    """The engine's L4 approaches the fourth-order TCL generator evaluated by
+    brute force from its definition (tests/tcl4_bruteforce.py) at first order
+    in dt, like the oracle it matches (exact Gamma tables of an exponential
+    bath, N = 3)."""
+    from . import tcl4_bruteforce as BF
+
+    rng = np.random.default_rng(3)
+    X = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
+    Y = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
+    S = taco_amd.System.build((X + X.conj().T) / 2, [(Y + Y.conj().T) / 4])
+    bath = BF.ExpBath([0.3 + 0.1j, 0.2 - 0.05j], [1.0 + 2.0j, 2.5 - 0.7j])
+    t = 0.9
+    ref = BF.k4(bath, np.asarray(S.eps), np.asarray(S.A_eig[0]), t)
+    e = []
+    for dt in (0.005, 0.0025):
+        k = int(round(t / dt))
+        got = taco_amd.build_TCL4_generator_hip_fused_batch(S, bath.gamma_table(S.omegas, k + 1, dt), dt, [k])
+        e.append(np.linalg.norm(got[0] - ref) / np.linalg.norm(ref))
+    assert e[1] < 1e-2 and 1.9 < e[0] / e[1] < 2.1, e
+
+
+def test_libm_phase_table_budget(orc, monkeypatch):
+    """A libm phase table above TACO_LIBM_TABLE_BYTES is refused with an
+    invalid-argument error; the CR convention needs no table."""
+    w = W.spin_boson(Nt=500)
+    monkeypatch.setenv("TACO_LIBM_TABLE_BYTES", "4096")
+    with pytest.raises(TacoInvalidArgument):
+        taco_amd.build_TCL4_generator_hip_fused_batch(w.system, w.gamma, w.dt, [0, 10])
+    with taco_amd.phase_mode(taco_amd.PHASE_CR):
+        taco_amd.build_TCL4_generator_hip_fused_batch(w.system, w.gamma, w.dt, [0, 10])
+
+
 def test_l4_published_setting_reduced(orc):
     w = W.published(N_bath=20000, tmax=2000)
     l4_check(orc, w, w.tidx)
