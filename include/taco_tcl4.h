@@ -10,7 +10,8 @@
  *   - outputs are caller-owned; every call is synchronous unless the name
  *     ends in _dev, in which case pointers are device pointers and the call
  *     returns after the work is enqueued on the context stream and
- *     completed (it synchronises the stream before returning);
+ *     completed (it synchronises the stream before returning); a name ending
+ *     in _noalloc takes device pointers and a stream and only enqueues;
  *   - every function returns TACO_OK (0) or an error code; taco_last_error()
  *     gives the message of the last failure on the calling thread.
  *
@@ -188,6 +189,26 @@ int taco_l4_single(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, const 
 int taco_l4_from_slices(taco_ctx* ctx, const double* Fs, const double* Cs, const double* Rs,
                         const int* pair_to_freq, int N, int nf, const double* ops, int num_ops,
                         double* L4_out, double* GW_out);
+
+/* Allocation-free twin of taco_l4_from_slices, with no counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Device pointers: Fs, Cs, Rs [npos][nf^3] F/C/R slices (time-major, idx3 =
+ * (i*nf + j)*nf + k), pair_to_freq [N*N] int column-major with entries in
+ * [0, nf) (a time whose map has an entry outside it gets NaN L4),
+ * ops [num_ops][N*N], L4_out [npos][N^4], GW_out [npos][N^4] (nullable).
+ * Enqueued on `stream` (a hipStream_t; NULL = the default stream) of the
+ * current device and not synchronised.  The factorised kernel's per-time
+ * intermediates live in a 64 MiB file-scope device array that each GPU holds
+ * until the process ends; times run in launches whose intermediates fit it,
+ * and a single time that needs more than 64 MiB (N and nf with
+ * (num_ops^2 (2N^2 + 2N^2 nf) + N^2 nf + N^2) * 16 bytes above it) is refused
+ * with TACO_ERR_INVALID.  Calls that overlap on one GPU share the array and
+ * race: serialise them (one stream, or events).  No hipMalloc, hipFree or
+ * other allocation happens in this call. */
+int taco_l4_from_slices_noalloc(const void* Fs, const void* Cs, const void* Rs, const void* pair_to_freq,
+                                int N, int nf, const void* ops, int num_ops, size_t npos, void* L4_out,
+                                void* GW_out, void* stream);
 
 /* propagate_rk4_dense_serial(L_series[, L_half_series], r, t0, dt, on_sample,
  * sample_every) (rk4_dense.hpp:85-100; rk4_dense.cpp:246-320).  L_series:

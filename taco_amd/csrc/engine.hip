@@ -1038,8 +1038,22 @@ __global__ __launch_bounds__(kBlock) void k_l4f(L4fArgs A) {
     double2* opsS = sm;                                          // num_ops * N^2
     int* f = reinterpret_cast<int*>(opsS + A.num_ops * N2);      // N^2
     for (int e = threadIdx.x; e < A.num_ops * N2; e += kBlock) opsS[e] = A.ops[e];
-    for (int e = threadIdx.x; e < N2; e += kBlock) f[e] = A.p2f[e];
-    __syncthreads();
+    int bad = 0;
+    for (int e = threadIdx.x; e < N2; e += kBlock) {
+        f[e] = A.p2f[e];
+        bad |= (f[e] < 0 || f[e] >= nf);
+    }
+    // A bucket index outside [0, nf) (a caller's device map that the host
+    // never saw) would index outside the slices: the time's L4 becomes NaN.
+    if (__syncthreads_or(bad)) {
+        const double nan = __builtin_nan("");
+        const long N4 = static_cast<long>(N2) * N2;
+        for (long idx = threadIdx.x; idx < N4; idx += kBlock) {
+            A.L4[pos * N4 + idx] = make_double2(nan, nan);
+            if (A.GW) A.GW[pos * N4 + idx] = make_double2(nan, nan);
+        }
+        return;
+    }
     const double2* Fp = A.Fs + pos * A.nf3;
     const double2* Cp = A.Cs + pos * A.nf3;
     const double2* Rp = A.Rs + pos * A.nf3;
@@ -3499,6 +3513,82 @@ int taco_l4_from_slices(taco_ctx* ctx, const double* Fs, const double* Cs, const
         HIP_OK(hipMemcpyAsync(L4_out, A.L4, N4 * 16, hipMemcpyDeviceToHost, ctx->stream));
         if (GW_out) HIP_OK(hipMemcpyAsync(GW_out, A.GW, N4 * 16, hipMemcpyDeviceToHost, ctx->stream));
         HIP_OK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+}  // extern "C"
+
+// Scratch of the allocation-free entry point: the factorised L4's per-time
+// intermediates, in a file-scope device array that each GPU holds from
+// module load to process end.  Host code must not pass `scratch` itself to a
+// kernel (on the host it only names the symbol): it uses the device address
+// that hipGetSymbolAddress returns, resolved once per device.
+namespace taco_engine {
+constexpr std::size_t kNoallocScratchBytes = std::size_t{64} << 20;
+static __device__ __align__(256) double2 scratch[kNoallocScratchBytes / sizeof(double2)];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double2* g_scratch_ptr[kMaxDevices] = {};
+
+static double2* scratch_on_current_device() {
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw Error(TACO_ERR_RUNTIME, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw Error(TACO_ERR_RUNTIME, "scratch symbol has no device address");
+        g_scratch_ptr[dev] = static_cast<double2*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace taco_engine
+
+extern "C" {
+
+int taco_l4_from_slices_noalloc(const void* Fs, const void* Cs, const void* Rs, const void* pair_to_freq, int N,
+                                int nf, const void* ops, int num_ops, size_t npos, void* L4_out, void* GW_out,
+                                void* stream) {
+    return guarded([&] {
+        if (N <= 0 || nf <= 0) throw Error(TACO_ERR_INVALID, "N and nf must be > 0");
+        if (num_ops <= 0) throw Error(TACO_ERR_INVALID, "assemble_liouvillian: coupling_ops must be non-empty");
+        if (npos == 0) return;
+        if (!Fs || !Cs || !Rs || !pair_to_freq || !ops || !L4_out)
+            throw Error(TACO_ERR_INVALID, "device pointers must be non-null");
+        const long N2 = static_cast<long>(N) * N;
+        const std::size_t lds = static_cast<std::size_t>(num_ops) * N2 * 16 + static_cast<std::size_t>(N2) * 4;
+        if (lds > 64 * 1024) throw Error(TACO_ERR_INVALID, "factorised L4 kernel: coupling operators exceed 64 KiB of LDS");
+        const long per = l4f_per_pos(N, nf, num_ops);
+        const std::size_t per_bytes = static_cast<std::size_t>(per) * sizeof(double2);
+        if (per_bytes > kNoallocScratchBytes)
+            throw Error(TACO_ERR_INVALID, "one time's L4 intermediates (" + std::to_string(per_bytes) +
+                                              " bytes) exceed the 64 MiB scratch of the allocation-free entry point");
+        double2* scr = scratch_on_current_device();
+        const long chunk = static_cast<long>(kNoallocScratchBytes / per_bytes);
+        const long nf3 = static_cast<long>(nf) * nf * nf;
+        const long N4 = N2 * N2;
+        L4fArgs F{};
+        F.p2f = static_cast<const int*>(pair_to_freq);
+        F.ops = static_cast<const double2*>(ops);
+        F.N = N;
+        F.nf = nf;
+        F.num_ops = num_ops;
+        F.nf3 = nf3;
+        F.scr = scr;
+        F.scr_per = per;
+        const hipStream_t st = static_cast<hipStream_t>(stream);
+        for (long p0 = 0; p0 < static_cast<long>(npos); p0 += chunk) {
+            const long np = std::min(chunk, static_cast<long>(npos) - p0);
+            F.Fs = static_cast<const double2*>(Fs) + p0 * nf3;
+            F.Cs = static_cast<const double2*>(Cs) + p0 * nf3;
+            F.Rs = static_cast<const double2*>(Rs) + p0 * nf3;
+            F.L4 = static_cast<double2*>(L4_out) + p0 * N4;
+            F.GW = GW_out ? static_cast<double2*>(GW_out) + p0 * N4 : nullptr;
+            hipLaunchKernelGGL(k_l4f, dim3(static_cast<unsigned>(np)), dim3(kBlock), lds, st, F);
+            HIP_OK(hipGetLastError());
+        }
     });
 }
 
