@@ -47,6 +47,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / checker leg")
     p.add_argument("--cpu-threads", type=int, default=0)
     p.add_argument("--reduced", action="store_true", help="small grid (for a quick functional run)")
+    p.add_argument("--plain-only", action="store_true",
+                   help="default workload: skip the batched, adjoint, host-pointer and serial-CPU sub-records")
     p.add_argument("--copy-bytes", type=int, default=0,
                    help="after the timed steps, run one plain device copy of this many bytes (the reference "
                         "kernel for scaling hardware traffic counters)")
@@ -129,6 +131,10 @@ def kernel_table(kern: dict, steps: int) -> tuple[dict, dict]:
 
 
 def batched_main(a):
+    print(json.dumps(batched_record(a)), flush=True)
+
+
+def batched_record(a, steps: int | None = None, warmup: int | None = None):
     """Many independent systems (d in {2,3,4,6,8}, Ohmic or Drude-Lorentz,
     Nt = 2^14..2^18, 1000 L4 times at stride 1..16, 1000 RK4 steps of
     TCL2 + L4 each): one batch (inputs resident in HBM, replayed hipGraph)
@@ -168,17 +174,19 @@ def batched_main(a):
         eager_s = time.perf_counter() - t0
         kern = prof.read()
     print(f"[bench] eager batch run {eager_s:.2f} s", file=sys.stderr, flush=True)
-    for _ in range(a.warmup):
+    steps = a.steps if steps is None else steps
+    warmup = a.warmup if warmup is None else warmup
+    for _ in range(warmup):
         b.run(graph=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(a.steps):
+    for _ in range(steps):
         b.run(graph=True)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     n_l4 = sum(len(it.tidx) for it in items)
-    value = n_l4 * a.steps / elapsed
-    print(f"[bench] graph replays: {a.steps} in {elapsed:.2f} s -> {value:.0f} L4/s", file=sys.stderr, flush=True)
+    value = n_l4 * steps / elapsed
+    print(f"[bench] graph replays: {steps} in {elapsed:.2f} s -> {value:.0f} L4/s", file=sys.stderr, flush=True)
     kernels, roofline = kernel_table(kern, 1)
     plain, check = None, None
     if not a.no_plain:
@@ -201,9 +209,9 @@ def batched_main(a):
         "value": value,
         "unit": "L4 time-points/s",
         "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": 1000.0 * elapsed / a.steps,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": 1000.0 * elapsed / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -223,10 +231,15 @@ def batched_main(a):
         "kernels": kernels,
     }
     b.close()
-    print(json.dumps(rec), flush=True)
+    taco_amd.set_phase_mode(taco_amd.PHASE_LIBM)
+    return rec
 
 
 def adjoint_main(a):
+    print(json.dumps(adjoint_record(a)), flush=True)
+
+
+def adjoint_record(a, steps: int | None = None, warmup: int | None = None):
     """Gradient of the spin-pair loss (d = 4, H1 = eps/2 sz + Delta/2 sx,
     A = sz(x)I + I(x)sz, Ohmic bath alpha 0.05, omega_c 5, beta 1, bath grid
     2^16, dt 0.005, 4000 RK4 steps of TCL2 + L4, checkpoints every 64 steps,
@@ -257,17 +270,19 @@ def adjoint_main(a):
         (g,) = torch.autograd.grad(loss, params)
         return float(loss), g.numpy()
 
-    for _ in range(a.warmup):
+    steps = a.steps if steps is None else steps
+    warmup = a.warmup if warmup is None else warmup
+    for _ in range(max(warmup, 1)):
         fwd()
         grad()
     t0 = time.perf_counter()
-    for _ in range(a.steps):
+    for _ in range(steps):
         fwd()
-    t_fwd = (time.perf_counter() - t0) / a.steps
+    t_fwd = (time.perf_counter() - t0) / steps
     t0 = time.perf_counter()
-    for _ in range(a.steps):
+    for _ in range(steps):
         loss, g = grad()
-    t_grad = (time.perf_counter() - t0) / a.steps
+    t_grad = (time.perf_counter() - t0) / steps
     print(f"[bench] forward {t_fwd * 1e3:.1f} ms, forward+reverse {t_grad * 1e3:.1f} ms", file=sys.stderr, flush=True)
     # reverse-pass kernels of one gradient
     from taco_amd import tcl4
@@ -311,8 +326,8 @@ def adjoint_main(a):
         "value": 1.0 / t_grad,
         "unit": "gradients/s",
         "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
+        "steps": steps,
+        "warmup": max(warmup, 1),
         "ms_per_step": 1e3 * t_grad,
         "higher_is_better": True,
         "scaling": "weak",
@@ -331,7 +346,50 @@ def adjoint_main(a):
         "cpu_baseline": cpu,
         "kernels": kernels,
     }
-    print(json.dumps(rec), flush=True)
+    return rec
+
+
+def host_pointer_record(w, steps: int = 3) -> dict:
+    """taco_l4_batch from host buffers: the Gamma table copied in, the L4
+    series copied out and repacked into per-time matrices, as the reference's
+    published timing includes (tests/tcl4_e2e_cuda_compare.cpp:208-214)."""
+    import taco_amd
+
+    S, G = w.system, w.gamma
+    taco_amd.build_TCL4_generator_hip_fused_batch(S, G, w.dt, w.tidx)  # tables, tiles, workspace
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        L = taco_amd.build_TCL4_generator_hip_fused_batch(S, G, w.dt, w.tidx)
+        mats = [np.ascontiguousarray(L[q]) for q in range(L.shape[0])]  # per-time matrices
+    el = (time.perf_counter() - t0) / steps
+    del mats
+    return {"workload": w.name, "d": S.N, "nf": S.nf, "ntau": len(w.tidx), "Nt": int(G.shape[0]),
+            "seconds_per_call": el, "L4_per_s_with_host_copies": len(w.tidx) / el, "calls": steps}
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def serial_cpu_record(w) -> dict:
+    """The oracle on one thread (the reference's serial path) over the whole
+    workload w."""
+    from oracle import oracle
+
+    S, G = w.system, w.gamma
+    with oracle.phase_mode(oracle.PHASE_LIBM):
+        t0 = time.perf_counter()
+        oracle.l4_series(G, S.omegas, S.mirror, S.pair_to_freq, S.A_eig, w.dt, w.tidx, nthreads=1)
+        el = time.perf_counter() - t0
+    return {"value": len(w.tidx) / el, "unit": "L4 time-points/s", "cores": 1, "kind": "port",
+            "sample": f"{w.name}: whole workload (d={S.N}, nf={S.nf}, Nt={G.shape[0]}, {len(w.tidx)} L4 times), "
+                      f"oracle serial, {el:.1f} s", "seconds": el}
 
 
 def tfim_adjoint_main(a):
@@ -545,6 +603,33 @@ def main():
         "parity": parity,
         "kernels": kernels,
     }
+    if cpu is not None:
+        cpu["cpu_model"] = cpu_model()
+    if a.workload == "gue_pair" and not a.reduced and not a.plain_only:
+        # the metric's other terms, each with its own timed count and wall time
+        from taco_amd import workloads as W
+
+        del dsys, out
+        torch.cuda.empty_cache()
+        t0 = time.perf_counter()
+        print("[bench] sub-record: host-pointer calls (d=4 bench workload, published d=2 setting)", file=sys.stderr,
+              flush=True)
+        rec["host_pointer"] = [host_pointer_record(w), host_pointer_record(W.published())]
+        rec["host_pointer_wall_s"] = time.perf_counter() - t0
+        if not a.no_cpu:
+            t0 = time.perf_counter()
+            print("[bench] sub-record: serial CPU (published d=2 setting)", file=sys.stderr, flush=True)
+            rec["cpu_serial"] = serial_cpu_record(W.published())
+            rec["cpu_serial"]["cpu_model"] = cpu_model()
+            rec["cpu_serial_wall_s"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        print("[bench] sub-record: batched, 256 systems", file=sys.stderr, flush=True)
+        rec["batched"] = batched_record(a, steps=1, warmup=1)
+        rec["batched"]["wall_s"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        print("[bench] sub-record: adjoint, d=4 spin pair", file=sys.stderr, flush=True)
+        rec["adjoint"] = adjoint_record(a, steps=5, warmup=1)
+        rec["adjoint"]["wall_s"] = time.perf_counter() - t0
     print(json.dumps(rec), flush=True)
 
 

@@ -99,17 +99,17 @@ int taco_profile_get(taco_ctx* ctx, int kernel, double* total_ms, long* launches
 int taco_fft_tile_choice(taco_ctx* ctx, int log2_nfft, int* cfg3);
 
 /* Correctly rounded sin/cos of each x[i] (|x| < 2^30): the phase routine of
- * the F/C/R kernels, exp(+-i*omega*(k*dt)) = (c, +-s).  The reference calls
- * std::exp of an imaginary argument (tcl4_kernels.cpp:116-120), whose last
- * bits depend on the platform libm; the engine fixes them to the correctly
- * rounded values.  ctx == NULL evaluates on the host, else on the GPU. */
+ * the F/C/R kernels in TACO_PHASE_CR, exp(+-i*omega*(k*dt)) = (c, +-s).  The
+ * reference calls std::exp of an imaginary argument (tcl4_kernels.cpp:116-120),
+ * whose last bits depend on the platform libm (TACO_PHASE_LIBM reproduces
+ * those).  ctx == NULL evaluates on the host, else on the GPU. */
 int taco_cr_sincos(taco_ctx* ctx, const double* x, size_t n, double* s, double* c);
 
 /* Correctly rounded exp of each x[i]: the exponentials of the bath
- * correlation (the Ohmic factor of J(w) and the KMS factors), for which the
- * reference calls std::exp (correlation_fft.hpp:154-214) with platform-libm
- * last bits; the engine fixes them to the correctly rounded values, as it
- * does the phases.  ctx == NULL evaluates on the host, else on the GPU. */
+ * correlation (the Ohmic factor of J(w) and the KMS factors) in
+ * TACO_PHASE_CR; the reference calls std::exp (correlation_fft.hpp:154-214)
+ * with platform-libm last bits, which TACO_PHASE_LIBM reproduces.
+ * ctx == NULL evaluates on the host, else on the GPU. */
 int taco_cr_exp(taco_ctx* ctx, const double* x, size_t n, double* out);
 
 /* --------------------------------------------------------------------------
@@ -134,7 +134,8 @@ int taco_system_build(const double* H, int N, const double* A_lab, int num_ops, 
 int taco_bcf_fft(size_t N, double dt, int kind, double p0, double p1, double beta, double* C_out);
 
 /* Gamma[t + Nc*b] = trapz prefix of exp(i omega_b tau) C(tau), Gamma[0] = 0.
- * The phase step exp(i omega_b dt) is the correctly rounded cos/sin pair. */
+ * The phase step exp(i omega_b dt) and the exponentials of taco_bcf_fft
+ * follow the phase mode (libm std::exp or correctly rounded). */
 int taco_gamma_trapz(const double* C, size_t Nc, double dt, const double* omegas, int nf, double* gamma_out);
 
 /* GPU twins of the two producers above (bcf_fft_fun, correlation_fft.hpp:154-214;
@@ -189,6 +190,41 @@ int taco_l4_single(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, const 
 int taco_l4_from_slices(taco_ctx* ctx, const double* Fs, const double* Cs, const double* Rs,
                         const int* pair_to_freq, int N, int nf, const double* ops, int num_ops,
                         double* L4_out, double* GW_out);
+
+/* The TCL2 generator L_unitary(H_eff) + L_dissipator of build_tcl2_components
+ * (generator.cpp:81-177) in its general form, from channel matrices: gamma_ab
+ * is column-major [Nt x (nf * num_ops^2)], column (b*num_ops + beta)*num_ops
+ * + alpha holding Gamma_b(alpha, beta)(t_k) (num_ops = 1 is the gamma_series
+ * layout of the other entry points).  L_out [ntau][N^4] at the times tidx
+ * (ntau = 0: every t < Nt), each N^2 x N^2 column-major in the reference's
+ * vec convention.  As in the reference, a (b, alpha, beta) whose
+ * A_alpha,b^dag A_beta,b is the zero matrix contributes nothing. */
+int taco_tcl2_generator(taco_ctx* ctx, const double* gamma_ab, size_t Nt, int nf, int num_ops, const double* eps,
+                        const double* ops, const int* pair_to_freq, int N, const size_t* tidx, size_t ntau,
+                        double* L_out);
+
+/* Device-level helpers of the reference's split path, device pointers,
+ * enqueued on `stream` (a hipStream_t; NULL = the default stream) without
+ * synchronising; they allocate nothing.
+ * taco_mikx_device: build_mikx_device (tcl4_mikx_cuda.hpp:44-46) — M, I, K
+ * [N^4] (N^2 x N^2 column-major) and X [N^6] (flat6 j + N(k + N(p + N(q +
+ * N(r + N s))))) from the F/C/R slices [nf^3] at one time; a pair_to_freq
+ * entry outside [0, nf) makes the outputs NaN.
+ * taco_assemble_gw_device: assemble_liouvillian_cuda_device (raw = 0: GW =
+ * T + T^H) and assemble_liouvillian_cuda_device_raw (raw = 1: T), from M, I,
+ * K, X and ops [num_ops][N*N] (tcl4_assemble_cuda.hpp:24-44). */
+int taco_mikx_device(const void* Fs, const void* Cs, const void* Rs, const void* pair_to_freq, int N, int nf,
+                     void* M, void* I, void* K, void* X, void* stream);
+int taco_assemble_gw_device(const void* M, const void* I, const void* K, const void* X, const void* ops, int N,
+                            int num_ops, void* GW, int raw, void* stream);
+
+/* compute_fcr_convolution_batched (tcl4_fcr_kernels_cuda.hpp:88-91): F, C, R
+ * of the triples (i, j, k0 + b), b < batch, for every t < Nt, into device
+ * buffers [batch][Nt] (out[t + Nt*b], time contiguous); device inputs gamma
+ * [nf][Nt] (column-major [Nt x nf]), omegas [nf], mirror [nf].  Runs on the
+ * context's stream and workspace; synchronous like the other _dev calls. */
+int taco_fcr_batched_dev(taco_ctx* ctx, const void* gamma, const void* omegas, const void* mirror, size_t Nt,
+                         int nf, double dt, int i, int j, int k0, int batch, void* F, void* C, void* R);
 
 /* Allocation-free twin of taco_l4_from_slices, with no counterpart in the
  * reference: for callers that cannot allocate device memory (code under HIP

@@ -100,6 +100,7 @@ def lib():
             "orc_l4_series": (c_int, [_dp, c_size_t, c_int, _dp, _ip, _ip, c_int, _dp, c_int, c_double,
                                       c_int, c_size_t, _sp, c_size_t, _dp, c_int]),
             "orc_tcl2_at": (c_int, [_dp, _dp, c_int, c_int, _ip, c_int, _dp, _dp]),
+            "orc_tcl2_general_at": (c_int, [_dp, _dp, c_int, c_int, _ip, c_int, _dp, _dp]),
             "orc_rk4_update": (c_int, [_dp, _dp, _dp, _dp, c_size_t, c_double]),
             "orc_propagate": (c_int, [_dp, _dp, c_size_t, c_size_t, _dp, c_double, c_size_t, _dp, _sp]),
             "orc_tcl4_propagate": (c_int, [_dp, c_size_t, c_int, _dp, _ip, _ip, c_int, _dp, _dp, c_int,
@@ -366,6 +367,21 @@ def tcl2_at(eps, ops, p2f, gamma_row) -> np.ndarray:
     gr = _c(gamma_row)
     L = np.zeros(N ** 4, np.complex128)
     _chk(lib().orc_tcl2_at(_p(e), _p(of), len(ops), N, _pi(pf), gr.size, _p(gr), _p(L)))
+    return L.reshape(N * N, N * N).T
+
+
+def tcl2_general_at(eps, ops, p2f, gamma_ab) -> np.ndarray:
+    """TCL2 generator from channel matrices gamma_ab [nf, num_ops, num_ops]
+    (gamma_ab[b, alpha, beta] = Gamma_b(alpha, beta) at one t)."""
+    N = len(eps)
+    e = np.ascontiguousarray(eps, np.float64)
+    of = _ops_flat(ops)
+    pf = _p2f_flat(p2f)
+    g = np.asarray(gamma_ab, np.complex128)
+    nf = g.shape[0]
+    gf = np.ascontiguousarray(g.transpose(0, 2, 1)).ravel()  # [b][beta][alpha]
+    L = np.zeros(N ** 4, np.complex128)
+    _chk(lib().orc_tcl2_general_at(_p(e), _p(of), len(ops), N, _pi(pf), nf, _p(gf), _p(L)))
     return L.reshape(N * N, N * N).T
 
 

@@ -565,8 +565,12 @@ double sqnorm(const std::vector<cd>& A) {
     return s;
 }
 
-void tcl2_at(const double* eps, const cd* ops_eig, int num_ops, int N, const int* p2f, int nf,
-             const cd* gamma_row /* nf values at t */, cd* Lout) {
+// TCL2 generator (build_tcl2_components, generator.cpp:81-177) from channel
+// matrices: gamma_ab[(b*num_ops + beta)*num_ops + alpha] = Gamma_b(alpha, beta) at t.
+// tcl2_at (below) is the drivers' diagonal case Gamma_b(alpha, beta) =
+// delta_ab gamma_b (TCL4_spin_boson_example.cpp:303-316).
+void tcl2_general_at(const double* eps, const cd* ops_eig, int num_ops, int N, const int* p2f, int nf,
+                     const cd* gamma_ab, cd* Lout) {
     const std::size_t NN = static_cast<std::size_t>(N), N2 = NN * NN;
     std::vector<cd> L(N2 * N2, cd(0.0, 0.0)), Hls(N2, cd(0.0, 0.0));
     for (int b = 0; b < nf; ++b) {
@@ -585,7 +589,7 @@ void tcl2_at(const double* eps, const cd* ops_eig, int num_ops, int N, const int
             if (!has[al]) continue;
             for (int be = 0; be < num_ops; ++be) {
                 if (!has[be]) continue;
-                const cd G = (al == be) ? gamma_row[b] : cd(0.0, 0.0);
+                const cd G = gamma_ab[(static_cast<std::size_t>(b) * num_ops + be) * num_ops + al];
                 const double gam = 2.0 * G.real();
                 const double S = G.imag();
                 const bool need_jump = std::abs(gam) > 0.0;
@@ -634,6 +638,15 @@ void tcl2_at(const double* eps, const cd* ops_eig, int num_ops, int N, const int
                     const cd right = (r2 == c2) ? H[c1 + r1 * NN] : cd(0.0, 0.0);
                     Lout[row + col * N2] = mi * (left - right) + L[row + col * N2];
                 }
+}
+
+void tcl2_at(const double* eps, const cd* ops_eig, int num_ops, int N, const int* p2f, int nf,
+             const cd* gamma_row /* nf values at t */, cd* Lout) {
+    const std::size_t P = static_cast<std::size_t>(num_ops) * num_ops;
+    std::vector<cd> gab(static_cast<std::size_t>(nf) * P, cd(0.0, 0.0));
+    for (int b = 0; b < nf; ++b)
+        for (int a = 0; a < num_ops; ++a) gab[b * P + static_cast<std::size_t>(a) * num_ops + a] = gamma_row[b];
+    tcl2_general_at(eps, ops_eig, num_ops, N, p2f, nf, gab.data(), Lout);
 }
 
 // ---------------------------------------------------------------------------
@@ -936,6 +949,12 @@ int orc_tcl2_at(const double* eps, const double* ops_eig, int num_ops, int N, co
                 const double* gamma_row, double* L) {
     return guard([&] {tcl2_at(eps, reinterpret_cast<const cd*>(ops_eig), num_ops, N, p2f, nf,
                     reinterpret_cast<const cd*>(gamma_row), reinterpret_cast<cd*>(L));});
+}
+
+int orc_tcl2_general_at(const double* eps, const double* ops_eig, int num_ops, int N, const int* p2f, int nf,
+                        const double* gamma_ab, double* L) {
+    return guard([&] {tcl2_general_at(eps, reinterpret_cast<const cd*>(ops_eig), num_ops, N, p2f, nf,
+                    reinterpret_cast<const cd*>(gamma_ab), reinterpret_cast<cd*>(L));});
 }
 
 int orc_rk4_update(const double* L0, const double* Lh, const double* L1, double* r, size_t D, double dt) {

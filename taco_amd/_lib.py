@@ -65,6 +65,12 @@ SIGNATURES = {
     "taco_l4_single": (c_int, [c_void_p, _dp, c_size_t, c_int, _dp, _ip, _ip, c_int, _dp, c_int, c_double,
                                c_int, c_size_t, _dp]),
     "taco_l4_from_slices": (c_int, [c_void_p, _dp, _dp, _dp, _ip, c_int, c_int, _dp, c_int, _dp, _dp]),
+    "taco_tcl2_generator": (c_int, [c_void_p, _dp, c_size_t, c_int, c_int, _dp, _dp, _ip, c_int, _sp, c_size_t,
+                                    _dp]),
+    "taco_mikx_device": (c_int, [c_void_p] * 4 + [c_int, c_int] + [c_void_p] * 5),
+    "taco_assemble_gw_device": (c_int, [c_void_p] * 5 + [c_int, c_int, c_void_p, c_int, c_void_p]),
+    "taco_fcr_batched_dev": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_int, c_double, c_int, c_int,
+                                     c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "taco_l4_from_slices_noalloc": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_int,
                                             c_size_t, c_void_p, c_void_p, c_void_p]),
     "taco_rk4_propagate": (c_int, [c_void_p, _dp, _dp, c_size_t, c_int, _dp, c_double, c_size_t, _dp, _sp]),

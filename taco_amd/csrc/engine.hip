@@ -288,6 +288,22 @@ __device__ __forceinline__ void xcd_map(int bid, int bpj, int& job, int& gb) {
     }
 }
 
+// Job-fastest variant for phase B of the triples: each XCD keeps a fixed
+// eighth of the group blocks as above, and walks all jobs of a group block
+// before the next, so the workgroups it runs at one time share a group
+// block; jobs are visited grouped by j, so they read the same spectrum
+// tiles (of g_mirror(j) and conj g_j) and twiddles, from its L2.
+__device__ __forceinline__ void xcd_map_job_fast(int bid, int bpj, int njobs, int& job, int& gb) {
+    if ((bpj & 7) == 0) {
+        const int x = bid & 7, i = bid >> 3;
+        gb = x + 8 * (i / njobs);
+        job = i % njobs;
+    } else {
+        gb = bid / njobs;
+        job = bid % njobs;
+    }
+}
+
 struct FcrArgs {
     const double2* gamma;   // [nf][Nt]
     const double* omegas;   // [nf]
@@ -321,6 +337,7 @@ struct FcrArgs {
     const double2* ph_tab;
     const int* job_row;
     long ph_len;
+    int b_job_fast;         // phase B of the triples: job-fastest workgroup order
 };
 
 // exp(+i Omega (t dt)) of a triple: its table row, or the correctly rounded value.
@@ -434,7 +451,10 @@ __global__ __launch_bounds__(Cfg::kFT) TACO_FFT_OCC(Cfg) void k_phase_b(FcrArgs 
     const TileLayout Tl(P.b, tile_groups<Cfg>(P.b, ngroups));
     const int G = Tl.G, gs = Tl.gs;
     int job, gb;
-    xcd_map(blockIdx.x, ngroups / G, job, gb);
+    if (mode == 0 && P.b_job_fast)
+        xcd_map_job_fast(blockIdx.x, ngroups / G, P.njobs, job, gb);
+    else
+        xcd_map(blockIdx.x, ngroups / G, job, gb);
     if (mode == 0) job = P.bperm[P.job0 + job] - static_cast<int>(P.job0);
     const int k1b = gb * G;
     const int total = G * gs;
@@ -596,6 +616,7 @@ struct CombineArgs {
     double2* Fo;
     double2* Co;
     double2* Ro;
+    long tri0;           // mode 1: series of triple tri go to row tri - tri0
 };
 
 // F/C/R assembly for t < T (tcl4_kernels.cpp:105-175): prefix integrals by
@@ -647,7 +668,7 @@ __global__ __launch_bounds__(kBlock) void k_combine(FcrArgs P, CombineArgs O) {
             if (pos < 0) continue;
             dst = static_cast<long>(pos) * O.nf3 + tri;
         } else {
-            dst = tri * P.T + t;
+            dst = (tri - O.tri0) * P.T + t;
         }
         const double dt = P.dt;
         const double2 pF = make_double2(v[0] * dt, v[1] * dt);
@@ -998,6 +1019,128 @@ __global__ __launch_bounds__(kBlock) void k_l4(L4Args A) {
         const int n = r % N, i = r / N, m = c % N, j = c / N;
         L[(n + m * N) + (i + j * N) * N2] = g;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Device-level helpers of the reference's split path, for callers that keep
+// it: the explicit M/I/K (N^4) and X (N^6) tensors at one time from the nf^3
+// slices (build_mikx_device, tcl4_mikx_cuda.hpp:44-46; the gather of
+// tcl4_mikx.cpp:101-132 with the code's M1 order) and GW from them
+// (assemble_liouvillian_cuda_device[_raw], tcl4_assemble_cuda.hpp:24-44).
+// The fused kernels above never materialise X; these exist for drop-in use.
+// ---------------------------------------------------------------------------
+// One thread per X element (j,k,p,q,r,s), flat6 = j + N(k + N(p + N(q + N(r + N s))));
+// the (r,s) = (0,0) thread also writes M, I, K at (row j + N k, col p + N q).
+__global__ __launch_bounds__(kBlock) void k_mikx(const double2* Fs, const double2* Cs, const double2* Rs,
+                                                 const int* p2f, int N, int nf, double2* M, double2* I, double2* K,
+                                                 double2* X) {
+    extern __shared__ int fmap[];
+    const int N2 = N * N;
+    int bad = 0;
+    for (int e = threadIdx.x; e < N2; e += kBlock) {
+        fmap[e] = p2f[e];
+        bad |= (fmap[e] < 0 || fmap[e] >= nf);
+    }
+    const bool invalid = __syncthreads_or(bad);
+    const long N4 = static_cast<long>(N2) * N2;
+    const long x = blockIdx.x * static_cast<long>(kBlock) + threadIdx.x;
+    if (x >= N4 * N2) return;
+    const double nan = __builtin_nan("");
+    const long outer = x % N4;      // j + N k + N^2 (p + N q)
+    const int rs = static_cast<int>(x / N4);
+    const int j = static_cast<int>(outer % N), k = static_cast<int>((outer / N) % N);
+    const int p = static_cast<int>((outer / N2) % N), q = static_cast<int>(outer / (static_cast<long>(N2) * N));
+    const int r = rs % N, s = rs / N;
+    if (invalid) {  // a bucket index outside [0, nf): NaN instead of reads outside the slices
+        X[x] = make_double2(nan, nan);
+        if (rs == 0) M[outer] = I[outer] = K[outer] = make_double2(nan, nan);
+        return;
+    }
+    auto f = [&](int a, int b) { return static_cast<long>(fmap[a + b * N]); };
+    auto id3 = [nf](long a, long b, long c) { return (a * nf + b) * nf + c; };
+    const long e3 = id3(f(j, k), f(p, q), f(r, s));
+    X[x] = cadd(Cs[e3], Rs[e3]);
+    if (rs == 0) {
+        M[outer] = csub(Fs[id3(f(j, q), f(j, k), f(p, j))], Rs[id3(f(j, q), f(p, q), f(q, k))]);
+        I[outer] = Fs[id3(f(j, k), f(q, p), f(k, q))];
+        K[outer] = Rs[id3(f(j, k), f(p, q), f(q, j))];
+    }
+}
+
+// T(n,i;m,j) of the NAKZWAN sums (tcl4_assemble.cpp:29-178) from explicit
+// M/I/K/X, one thread per element, the terms in the reference's order.
+__global__ __launch_bounds__(kBlock) void k_gw_raw(const double2* M, const double2* I, const double2* K,
+                                                   const double2* X, const double2* ops, int N, int num_ops,
+                                                   double2* GW) {
+    const int N2 = N * N;
+    const long idx = blockIdx.x * static_cast<long>(kBlock) + threadIdx.x;
+    if (idx >= static_cast<long>(N2) * N2) return;
+    const int row_u = static_cast<int>(idx % N2), col_u = static_cast<int>(idx / N2);
+    const int n = row_u % N, i = row_u / N, m = col_u % N, j = col_u / N;
+    auto mat = [N2](const double2* Mx, int r, int c) { return Mx[r + static_cast<long>(c) * N2]; };
+    auto l2 = [N](int r, int c) { return r + c * N; };
+    auto l6 = [N](int a, int b, int c, int d, int e, int f) {
+        return a + static_cast<long>(N) * (b + static_cast<long>(N) * (c + static_cast<long>(N) *
+               (d + static_cast<long>(N) * (e + static_cast<long>(N) * f))));
+    };
+    double2 res = make_double2(0.0, 0.0);
+    for (int iA = 0; iA < num_ops; ++iA) {
+        const double2* Ap = ops + static_cast<long>(iA) * N2;
+        for (int iB = 0; iB < num_ops; ++iB) {
+            const double2* Bp = ops + static_cast<long>(iB) * N2;
+#define AA(r, c) Ap[(r) + (c) * N]
+#define BB(r, c) Bp[(r) + (c) * N]
+            for (int a = 0; a < N; ++a)
+                for (int b = 0; b < N; ++b) {
+                    const double2 t1 = cmul(cmul(cmul(cmul(BB(n, a), AA(a, b)), BB(b, i)), AA(j, m)),
+                                            mat(M, l2(b, i), l2(n, a)));
+                    const double2 t2 = cmul(cmul(cmul(cmul(AA(n, a), BB(a, b)), BB(b, i)), AA(j, m)),
+                                            mat(I, l2(a, n), l2(i, b)));
+                    const double2 t3 = cmul(cmul(cmul(cmul(BB(n, a), BB(a, b)), AA(b, i)), AA(j, m)),
+                                            mat(K, l2(i, b), l2(n, a)));
+                    const double2 t4 = cmul(cmul(cmul(cmul(AA(n, a), BB(a, i)), BB(j, b)), AA(b, m)),
+                                            X[l6(a, n, j, b, n, i)]);
+                    const double2 t5 = cmul(cmul(cmul(cmul(BB(n, a), AA(a, i)), BB(j, b)), AA(b, m)),
+                                            X[l6(i, a, j, b, n, i)]);
+                    const double2 t6 = cmul(cmul(cmul(cmul(AA(n, a), AA(a, b)), BB(b, i)), BB(j, m)),
+                                            X[l6(b, a, j, m, a, i)]);
+                    const double2 t7 = cmul(cmul(cmul(cmul(AA(n, a), BB(a, b)), AA(b, i)), BB(j, m)),
+                                            X[l6(i, b, j, m, a, i)]);
+                    res = csub(res, cadd(csub(csub(cadd(cadd(csub(t1, t2), t3), t4), t5), t6), t7));
+                }
+            if (j == m)
+                for (int a = 0; a < N; ++a)
+                    for (int b = 0; b < N; ++b)
+                        for (int c = 0; c < N; ++c) {
+                            const double2 add1 = cmul(cmul(cmul(cmul(AA(n, a), BB(a, b)), AA(b, c)), BB(c, i)),
+                                                      mat(M, l2(c, i), l2(a, b)));
+                            const double2 add2 = cmul(cmul(cmul(cmul(AA(n, a), BB(a, b)), BB(b, c)), AA(c, i)),
+                                                      mat(K, l2(i, c), l2(a, b)));
+                            const double2 nA = make_double2(-AA(n, a).x, -AA(n, a).y);
+                            const double2 add3 = cmul(cmul(cmul(cmul(nA, AA(a, b)), BB(b, c)), BB(c, i)),
+                                                      mat(I, l2(b, a), l2(i, c)));
+                            res = cadd(res, cadd(cadd(add1, add2), add3));
+                        }
+#undef AA
+#undef BB
+        }
+    }
+    GW[idx] = res;
+}
+
+// GW = T + T^H in place, one thread per unordered pair (r <= c).
+__global__ __launch_bounds__(kBlock) void k_gw_sym(double2* GW, int N2) {
+    const long e = blockIdx.x * static_cast<long>(kBlock) + threadIdx.x;
+    const long npairs = static_cast<long>(N2) * (N2 + 1) / 2;
+    if (e >= npairs) return;
+    // pair e -> (r, c) with c the column: e = c (c + 1) / 2 + r, r <= c
+    long c = static_cast<long>((sqrt(8.0 * static_cast<double>(e) + 1.0) - 1.0) / 2.0);
+    while (c * (c + 1) / 2 > e) --c;
+    while ((c + 1) * (c + 2) / 2 <= e) ++c;
+    const long r = e - c * (c + 1) / 2;
+    const double2 g = cadd(GW[r + c * N2], cconj(GW[c + r * N2]));
+    GW[r + c * N2] = g;
+    if (r != c) GW[c + r * N2] = cconj(g);
 }
 
 // ---------------------------------------------------------------------------
@@ -1458,6 +1601,13 @@ struct Tcl2Args {
     double2* L;            // [T][D*D], accumulated into
     int order;             // 0: unitary only; 2/4: full TCL2
     int use_lists = 1;     // bucket-list path for N <= kTcl2ListN (TACO_TCL2_LISTS=0: all-bucket walk)
+    // Channel matrices (generator.cpp:92-162): when non-null, Gamma_b(alpha, beta)
+    // at column (b num_ops + beta) num_ops + alpha, time contiguous ([cols][Nt]),
+    // replaces the diagonal gamma; skip[col] marks the (b, alpha, beta) whose
+    // A_alpha,b^dag A_beta,b is the zero matrix (the reference skips them).
+    const double2* gamma_ab = nullptr;
+    const unsigned char* skip = nullptr;
+    const long* tmap = nullptr;  // block -> Gamma row (null: the block index)
 };
 
 // Dynamic LDS of k_tcl2: the general path keeps H (N^2); the bucket-list path
@@ -1468,6 +1618,71 @@ __host__ __device__ inline std::size_t tcl2_lds(int N) {
     const std::size_t N2 = static_cast<std::size_t>(N) * N;
     if (N > kTcl2ListN) return 2 * N2 * 16;
     return N2 * 16 + N2 * 4 + N2 * 4 + N2 * static_cast<std::size_t>(N) * 2;
+}
+
+// L_TCL2(t) from channel matrices Gamma_b(alpha, beta) (build_tcl2_components,
+// generator.cpp:81-177, in its general form): every entry walks (b, alpha,
+// beta) in the reference's order; a (b, alpha, beta) with A_alpha,b^dag
+// A_beta,b == 0, or with a zero jump rate 2 Re Gamma / Lamb-shift rate Im Gamma,
+// contributes nothing to the respective part, as there.
+__device__ void tcl2_channels(const Tcl2Args& A, long t, double2* H, double2* L) {
+    const int N = A.N, N2 = N * N, D = N2, no = A.num_ops;
+    auto G = [&](int b, int al, int be) {
+        return A.gamma_ab[(static_cast<long>(b * no + be) * no + al) * A.Nt + t];
+    };
+    auto skipped = [&](int b, int al, int be) { return A.skip[(b * no + be) * no + al] != 0; };
+    // (A_al,b^dag A_be,b)(r, c) = sum_q [p2f(q,r) == p2f(q,c) == b] conj(A_al(q,r)) A_be(q,c)
+    auto adag_a = [&](const double2* Aa, const double2* Ab, int r, int c, int b) {
+        double2 s = make_double2(0, 0);
+        for (int q = 0; q < N; ++q)
+            if (A.p2f[q + r * N] == b && A.p2f[q + c * N] == b) s = cadd(s, cmul(cconj(Aa[q + r * N]), Ab[q + c * N]));
+        return s;
+    };
+    for (int e = threadIdx.x; e < N2; e += kBlock) {
+        const int r = e % N, c = e / N;
+        double2 h = make_double2(0.0, 0.0), hc = make_double2(0.0, 0.0);
+        if (A.order != 0)
+            for (int b = 0; b < A.nf; ++b)
+                for (int al = 0; al < no; ++al)
+                    for (int be = 0; be < no; ++be) {
+                        if (skipped(b, al, be)) continue;
+                        const double S = G(b, al, be).y;
+                        if (S == 0.0) continue;
+                        const double2* Aa = A.ops + al * N2;
+                        const double2* Ab = A.ops + be * N2;
+                        h = cadd(h, cscale(adag_a(Aa, Ab, r, c, b), S));
+                        hc = cadd(hc, cscale(adag_a(Aa, Ab, c, r, b), S));
+                    }
+        double2 v = cscale(cadd(h, cconj(hc)), 0.5);
+        if (r == c) v.x += A.eps[r];
+        H[e] = v;
+    }
+    __syncthreads();
+    for (long e = threadIdx.x; e < static_cast<long>(D) * D; e += kBlock) {
+        const int row = static_cast<int>(e % D), col = static_cast<int>(e / D);
+        const int r1 = row / N, r2 = row % N, c1 = col / N, c2 = col % N;
+        const double2 left = (r1 == c1) ? H[r2 + c2 * N] : make_double2(0, 0);
+        const double2 right = (r2 == c2) ? H[c1 + r1 * N] : make_double2(0, 0);
+        const double2 d = csub(left, right);
+        double2 diss = make_double2(0.0, 0.0);
+        if (A.order != 0)
+            for (int b = 0; b < A.nf; ++b)
+                for (int al = 0; al < no; ++al)
+                    for (int be = 0; be < no; ++be) {
+                        if (skipped(b, al, be)) continue;
+                        const double gam = 2.0 * G(b, al, be).x;
+                        if (gam == 0.0) continue;
+                        const double2* Aa = A.ops + al * N2;
+                        const double2* Ab = A.ops + be * N2;
+                        double2 jump = make_double2(0, 0);
+                        if (A.p2f[r1 + c1 * N] == b && A.p2f[r2 + c2 * N] == b)
+                            jump = cmul(cconj(Aa[r1 + c1 * N]), Ab[r2 + c2 * N]);
+                        const double2 lt = (r1 == c1) ? adag_a(Aa, Ab, r2, c2, b) : make_double2(0, 0);
+                        const double2 rt = (r2 == c2) ? adag_a(Aa, Ab, c1, r1, b) : make_double2(0, 0);
+                        diss = cadd(diss, cscale(csub(csub(jump, cscale(lt, 0.5)), cscale(rt, 0.5)), gam));
+                    }
+        L[e] = cadd(L[e], cadd(make_double2(d.y, -d.x), diss));  // -i d + L_diss
+    }
 }
 
 // L_TCL2(t) added onto L(t), one workgroup per time (generator.cpp:81-177 as
@@ -1481,9 +1696,13 @@ __host__ __device__ inline std::size_t tcl2_lds(int N) {
 __global__ __launch_bounds__(kBlock) void k_tcl2(Tcl2Args A) {
     extern __shared__ double2 sh[];
     const int N = A.N, N2 = N * N, D = N2;
-    const long t = blockIdx.x;
+    const long t = A.tmap ? A.tmap[blockIdx.x] : static_cast<long>(blockIdx.x);
     double2* H = sh;  // N x N effective Hamiltonian
-    double2* L = A.L + t * static_cast<long>(D) * D;
+    double2* L = A.L + static_cast<long>(blockIdx.x) * D * D;
+    if (A.gamma_ab) {
+        tcl2_channels(A, t, H, L);
+        return;
+    }
     const bool lists = A.use_lists && N <= kTcl2ListN;
     int* P = reinterpret_cast<int*>(H + N2);                        // pair map
     int* cnt = P + N2;                                              // list lengths per pair
@@ -2197,6 +2416,7 @@ static FcrJob fcr_prepare(taco_ctx* c, const FcrPlan& pl, const FcrLists& lists)
     P.ph_tab = prow.tab;
     P.job_row = prow.rows;
     P.ph_len = prow.len;
+    P.b_job_fast = env_int("TACO_PHASEB_JOB_FAST", 0);
     J.cls_start = std::move(cls_start);
     J.paths.assign(static_cast<std::size_t>(ncomb) + 1, 0);
     J.outs.assign(static_cast<std::size_t>(ncomb) + 1, 0);
@@ -3589,6 +3809,159 @@ int taco_l4_from_slices_noalloc(const void* Fs, const void* Cs, const void* Rs, 
             hipLaunchKernelGGL(k_l4f, dim3(static_cast<unsigned>(np)), dim3(kBlock), lds, st, F);
             HIP_OK(hipGetLastError());
         }
+    });
+}
+
+int taco_mikx_device(const void* Fs, const void* Cs, const void* Rs, const void* pair_to_freq, int N, int nf,
+                     void* M, void* I, void* K, void* X, void* stream) {
+    return guarded([&] {
+        if (N <= 0 || nf <= 0) throw Error(TACO_ERR_INVALID, "N and nf must be > 0");
+        if (!Fs || !Cs || !Rs || !pair_to_freq || !M || !I || !K || !X)
+            throw Error(TACO_ERR_INVALID, "device pointers must be non-null");
+        const long N2 = static_cast<long>(N) * N, N6 = N2 * N2 * N2;
+        if (N2 * 4 > 64 * 1024) throw Error(TACO_ERR_INVALID, "build_mikx_device: N above 128");
+        hipLaunchKernelGGL(k_mikx, dim3(static_cast<unsigned>((N6 + kBlock - 1) / kBlock)), dim3(kBlock),
+                           static_cast<std::size_t>(N2) * 4, static_cast<hipStream_t>(stream),
+                           static_cast<const double2*>(Fs), static_cast<const double2*>(Cs),
+                           static_cast<const double2*>(Rs), static_cast<const int*>(pair_to_freq), N, nf,
+                           static_cast<double2*>(M), static_cast<double2*>(I), static_cast<double2*>(K),
+                           static_cast<double2*>(X));
+        HIP_OK(hipGetLastError());
+    });
+}
+
+int taco_assemble_gw_device(const void* M, const void* I, const void* K, const void* X, const void* ops, int N,
+                            int num_ops, void* GW, int raw, void* stream) {
+    return guarded([&] {
+        if (N <= 0) throw Error(TACO_ERR_INVALID, "N must be > 0");
+        if (num_ops <= 0) throw Error(TACO_ERR_INVALID, "assemble_liouvillian: coupling_ops must be non-empty");
+        if (!M || !I || !K || !X || !ops || !GW) throw Error(TACO_ERR_INVALID, "device pointers must be non-null");
+        const long N2 = static_cast<long>(N) * N, N4 = N2 * N2;
+        const hipStream_t st = static_cast<hipStream_t>(stream);
+        hipLaunchKernelGGL(k_gw_raw, dim3(static_cast<unsigned>((N4 + kBlock - 1) / kBlock)), dim3(kBlock), 0, st,
+                           static_cast<const double2*>(M), static_cast<const double2*>(I),
+                           static_cast<const double2*>(K), static_cast<const double2*>(X),
+                           static_cast<const double2*>(ops), N, num_ops, static_cast<double2*>(GW));
+        HIP_OK(hipGetLastError());
+        if (!raw) {
+            const long npairs = N2 * (N2 + 1) / 2;
+            hipLaunchKernelGGL(k_gw_sym, dim3(static_cast<unsigned>((npairs + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                               st, static_cast<double2*>(GW), static_cast<int>(N2));
+            HIP_OK(hipGetLastError());
+        }
+    });
+}
+
+int taco_fcr_batched_dev(taco_ctx* ctx, const void* gamma, const void* omegas, const void* mirror, size_t Nt,
+                         int nf, double dt, int i, int j, int k0, int batch, void* F, void* C, void* R) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!(dt > 0.0)) throw Error(TACO_ERR_INVALID, "dt must be > 0");
+        if (nf <= 0 || Nt == 0) throw Error(TACO_ERR_INVALID, "empty gamma_series");
+        if (batch <= 0) return;
+        if (i < 0 || i >= nf || j < 0 || j >= nf || k0 < 0 || k0 + batch > nf)
+            throw Error(TACO_ERR_RANGE, "compute_fcr_convolution_batched: (i, j, k0 + batch) outside the buckets");
+        if (!gamma || !omegas || !mirror || !F || !C || !R)
+            throw Error(TACO_ERR_INVALID, "device pointers must be non-null");
+        DeviceGuard g(ctx->device);
+        // the pass groups triples by Omega on the host and validates the mirror
+        std::vector<double> h_om(static_cast<std::size_t>(nf));
+        std::vector<int> h_mi(static_cast<std::size_t>(nf));
+        HIP_OK(hipMemcpyAsync(h_om.data(), omegas, h_om.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipMemcpyAsync(h_mi.data(), mirror, h_mi.size() * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        for (int b = 0; b < nf; ++b)
+            if (h_mi[static_cast<std::size_t>(b)] < 0 || h_mi[static_cast<std::size_t>(b)] >= nf)
+                throw Error(TACO_ERR_INVALID, "mirror index out of range");
+        const std::size_t nf3 = static_cast<std::size_t>(nf) * nf * nf;
+        std::vector<unsigned char> need(nf3, 0);
+        const long tri0 = (static_cast<long>(i) * nf + j) * nf + k0;
+        for (int b = 0; b < batch; ++b) need[static_cast<std::size_t>(tri0 + b)] = 7;
+        CombineArgs out{};
+        out.out_mode = 1;
+        out.nf3 = static_cast<long>(nf3);
+        out.Fo = static_cast<double2*>(F);
+        out.Co = static_cast<double2*>(C);
+        out.Ro = static_cast<double2*>(R);
+        out.tri0 = tri0;
+        FcrPlan pl{static_cast<const double2*>(gamma), Nt, nf, static_cast<const double*>(omegas), h_om.data(),
+                   static_cast<const int*>(mirror), dt, Nt, &need};
+        run_fcr(ctx, pl, out);
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int taco_tcl2_generator(taco_ctx* ctx, const double* gamma_ab, size_t Nt, int nf, int num_ops, const double* eps,
+                        const double* ops, const int* pair_to_freq, int N, const size_t* tidx, size_t ntau,
+                        double* L_out) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (N <= 0 || nf <= 0 || Nt == 0) throw Error(TACO_ERR_INVALID, "N, nf and Nt must be > 0");
+        if (num_ops <= 0) throw Error(TACO_ERR_INVALID, "coupling_ops must be non-empty");
+        if (!gamma_ab || !eps || !ops || !pair_to_freq || !L_out || (ntau && !tidx))
+            throw Error(TACO_ERR_INVALID, "inputs must be non-null");
+        const std::size_t N2 = static_cast<std::size_t>(N) * N;
+        for (std::size_t e = 0; e < N2; ++e)
+            if (pair_to_freq[e] < -1 || pair_to_freq[e] >= nf) throw Error(TACO_ERR_INVALID, "pair_to_freq entry >= nf");
+        const std::size_t nt = ntau ? ntau : Nt;
+        std::vector<long> tm(nt);
+        for (std::size_t q = 0; q < nt; ++q) {
+            const std::size_t t = ntau ? tidx[q] : q;
+            if (t >= Nt) throw Error(TACO_ERR_RANGE, "time_index out of range");
+            tm[q] = static_cast<long>(t);
+        }
+        // (b, alpha, beta) the reference skips: a zero part, or A_al,b^dag A_be,b == 0
+        const std::size_t P = static_cast<std::size_t>(num_ops) * num_ops;
+        const std::complex<double>* Ah = reinterpret_cast<const std::complex<double>*>(ops);
+        std::vector<unsigned char> skip(static_cast<std::size_t>(nf) * P, 0);
+        for (int b = 0; b < nf; ++b)
+            for (int be = 0; be < num_ops; ++be)
+                for (int al = 0; al < num_ops; ++al) {
+                    bool zero = true;
+                    for (int r = 0; r < N && zero; ++r)
+                        for (int c = 0; c < N && zero; ++c) {
+                            std::complex<double> s(0.0, 0.0);
+                            for (int q = 0; q < N; ++q)
+                                if (pair_to_freq[q + r * N] == b && pair_to_freq[q + c * N] == b)
+                                    s += std::conj(Ah[static_cast<std::size_t>(al) * N2 + q + r * N]) *
+                                         Ah[static_cast<std::size_t>(be) * N2 + q + c * N];
+                            zero = s.real() == 0.0 && s.imag() == 0.0;
+                        }
+                    skip[(static_cast<std::size_t>(b) * num_ops + be) * num_ops + al] = zero ? 1 : 0;
+                }
+        DeviceGuard g(ctx->device);
+        const std::size_t cols = static_cast<std::size_t>(nf) * P;
+        const double2* dg = upload(ctx, ctx->gamma, reinterpret_cast<const double2*>(gamma_ab), Nt * cols);
+        const double* de = upload(ctx, ctx->eps, eps, static_cast<std::size_t>(N));
+        const double2* dops = upload(ctx, ctx->ops, reinterpret_cast<const double2*>(ops), N2 * num_ops);
+        const int* dpf = upload(ctx, ctx->p2f, pair_to_freq, N2);
+        ctx->misc.ensure(cols + nt * sizeof(long) + 16);
+        unsigned char* dskip = ctx->misc.as<unsigned char>();
+        long* dtm = reinterpret_cast<long*>(ctx->misc.as<unsigned char>() + ((cols + 15) / 16) * 16);
+        HIP_OK(hipMemcpyAsync(dskip, skip.data(), cols, hipMemcpyHostToDevice, ctx->stream));
+        HIP_OK(hipMemcpyAsync(dtm, tm.data(), nt * sizeof(long), hipMemcpyHostToDevice, ctx->stream));
+        ctx->l4.ensure(nt * N2 * N2 * 16);
+        const long cnt = static_cast<long>(nt * N2 * N2);
+        hipLaunchKernelGGL(k_fill_zero, dim3(static_cast<unsigned>((cnt + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                           ctx->stream, ctx->l4.as<double2>(), cnt);
+        Tcl2Args A2{};
+        A2.Nt = static_cast<long>(Nt);
+        A2.eps = de;
+        A2.ops = dops;
+        A2.p2f = dpf;
+        A2.N = N;
+        A2.nf = nf;
+        A2.num_ops = num_ops;
+        A2.L = ctx->l4.as<double2>();
+        A2.order = 2;
+        A2.use_lists = 0;
+        A2.gamma_ab = dg;
+        A2.skip = dskip;
+        A2.tmap = dtm;
+        hipLaunchKernelGGL(k_tcl2, dim3(static_cast<unsigned>(nt)), dim3(kBlock), tcl2_lds(N), ctx->stream, A2);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(L_out, ctx->l4.p, nt * N2 * N2 * 16, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
     });
 }
 

@@ -38,7 +38,8 @@ __all__ = [
     "compute_trapz_prefix_multi_matrix_hip", "compute_triple_kernels",
     "build_TCL4_generator_hip_fused", "build_TCL4_generator_hip_fused_batch", "build_correction_series_hip_fused",
     "build_TCL4_generator_cuda_fused", "build_TCL4_generator_cuda_fused_batch", "build_correction_series_cuda_fused",
-    "build_TCL4_generator", "build_correction_series", "assemble_l4_from_slices", "propagate_rk4_dense",
+    "build_TCL4_generator", "build_correction_series", "assemble_l4_from_slices", "build_tcl2_generator",
+    "propagate_rk4_dense",
     "propagate_rk4_dense_serial", "propagate_rk4_dense_omp", "tcl4_propagate", "set_fcr_fft_pad_factor",
     "get_fcr_fft_pad_factor", "set_phase_mode", "get_phase_mode", "phase_mode", "PHASE_LIBM", "PHASE_CR",
     "context", "version", "BatchItem", "Batch", "batched_run",
@@ -408,6 +409,31 @@ def build_TCL4_generator(system: System, gamma_series, dt: float, time_index: in
 def build_correction_series(system: System, gamma_series, dt: float, method: FCRMethod = FCRMethod.Convolution,
                             exec: Optional[Exec] = None) -> np.ndarray:
     return build_correction_series_hip_fused(system, gamma_series, dt, method, exec)
+
+
+def build_tcl2_generator(system: System, gamma_ab, time_indices: Sequence[int] = (),
+                         exec: Optional[Exec] = None) -> np.ndarray:
+    """L_unitary + L_dissipator of build_tcl2_components (generator.cpp:81-177)
+    from channel matrices gamma_ab [Nt, nf, num_ops, num_ops]
+    (gamma_ab[t, b, alpha, beta] = Gamma_b(alpha, beta)(t)); returns
+    [ntau, N^2, N^2] at time_indices (empty: every t)."""
+    g = np.asarray(gamma_ab, np.complex128)
+    Nt, nf, no = g.shape[0], g.shape[1], g.shape[2]
+    if nf != system.nf or no != len(system.A_eig) or g.shape[3] != no:
+        raise TacoInvalidArgument(1, "gamma_ab must be [Nt, nf, num_ops, num_ops]")
+    N = system.N
+    cols = np.ascontiguousarray(g.transpose(1, 3, 2, 0)).reshape(-1)  # [(b*no + beta)*no + alpha][t]
+    ops = _ops_flat(system.A_eig)
+    p2f = np.ascontiguousarray(np.asarray(system.pair_to_freq, np.int32).T.reshape(-1))
+    eps = np.ascontiguousarray(system.eps, np.float64)
+    tid = np.ascontiguousarray(list(time_indices), np.uint64)
+    nt = len(tid) if len(tid) else Nt
+    L = np.zeros((nt, N ** 4), np.complex128)
+    ctx = _ctx_for(exec)
+    check(lib().taco_tcl2_generator(ctx.handle, dptr(cols.view(np.float64)), Nt, nf, no, dptr(eps),
+                                    dptr(ops.view(np.float64)), iptr(p2f), N, sptr(tid) if len(tid) else None,
+                                    len(tid), dptr(L.view(np.float64))))
+    return L.reshape(nt, N * N, N * N).transpose(0, 2, 1)
 
 
 def assemble_l4_from_slices(Fs, Cs, Rs, pair_to_freq, ops, exec: Optional[Exec] = None):

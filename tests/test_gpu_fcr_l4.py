@@ -159,6 +159,51 @@ def test_gpu_l4_converges_to_bruteforce_tcl4():
     assert e[1] < 1e-2 and 1.9 < e[0] / e[1] < 2.1, e
 
 
+@pytest.mark.parametrize("wl", ["spin_boson", "d3"])
+def test_device_mikx_and_gw_helpers_match_oracle(orc, wl):
+    """taco_mikx_device (build_mikx_device) gathers M, I, K, X exactly as the
+    oracle's mikx, and taco_assemble_gw_device (assemble_liouvillian_cuda_device
+    [_raw]) sums the NAKZWAN terms in the oracle's order: equal bit for bit."""
+    import torch
+
+    from taco_amd import _lib
+
+    if wl == "spin_boson":
+        w = W.spin_boson(Nt=1024, tmax=300)
+    else:
+        w = W.heterogeneous_system(1, dims=(3,), n_time=40, log2_range=(10, 10))
+    S, G = w.system, w.gamma
+    N, nf, t = S.N, S.nf, 37
+    F, C, R = orc.triple_kernels(G, S.omegas, S.mirror, w.dt, keep=t + 1)
+    Fs, Cs, Rs = (np.ascontiguousarray(x[..., t]).reshape(-1) for x in (F, C, R))
+    M, I, K, X = orc.mikx(Fs, Cs, Rs, S.pair_to_freq, N)
+    ops = np.stack([np.asarray(a) for a in S.A_eig])
+    GW = orc.assemble_gw(M, I, K, X, list(ops), N)
+    dev = torch.device("cuda:0")
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    dF, dC, dR = T(Fs), T(Cs), T(Rs)
+    dpf = T(np.asarray(S.pair_to_freq, np.int32).T.reshape(-1))
+    dops = T(np.stack([a.T for a in ops]).reshape(-1))
+    N4, N6 = N ** 4, N ** 6
+    out = [torch.zeros(n, dtype=torch.complex128, device=dev) for n in (N4, N4, N4, N6, N4, N4)]
+    dM, dI, dK, dX, dGW, dGWr = out
+    torch.cuda.synchronize()
+    L = _lib.lib()
+    _lib.check(L.taco_mikx_device(dF.data_ptr(), dC.data_ptr(), dR.data_ptr(), dpf.data_ptr(), N, nf, dM.data_ptr(),
+                                  dI.data_ptr(), dK.data_ptr(), dX.data_ptr(), None))
+    _lib.check(L.taco_assemble_gw_device(dM.data_ptr(), dI.data_ptr(), dK.data_ptr(), dX.data_ptr(), dops.data_ptr(),
+                                         N, len(ops), dGW.data_ptr(), 0, None))
+    _lib.check(L.taco_assemble_gw_device(dM.data_ptr(), dI.data_ptr(), dK.data_ptr(), dX.data_ptr(), dops.data_ptr(),
+                                         N, len(ops), dGWr.data_ptr(), 1, None))
+    torch.cuda.synchronize()
+    col = lambda d: d.cpu().numpy().reshape(N * N, N * N).T  # noqa: E731
+    assert np.array_equal(col(dM), M) and np.array_equal(col(dI), I) and np.array_equal(col(dK), K)
+    assert np.array_equal(dX.cpu().numpy(), X)
+    assert np.array_equal(col(dGW), GW)
+    raw = col(dGWr)
+    assert np.array_equal(raw + raw.conj().T, GW)
+
+
 def test_libm_phase_table_budget(orc, monkeypatch):
     """A libm phase table above TACO_LIBM_TABLE_BYTES is refused with an
     invalid-argument error; the CR convention needs no table."""

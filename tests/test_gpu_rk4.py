@@ -101,3 +101,67 @@ def test_tcl2_bucket_lists_equal_all_bucket_walk(monkeypatch, case):
         monkeypatch.setenv("TACO_TCL2_LISTS", lists)
         out.append(taco_amd.tcl4_propagate(S, G, w.dt, steps, r0, order=2))
     assert np.array_equal(out[0], out[1])
+
+
+def _two_channel_system(seed=7):
+    rng = np.random.default_rng(seed)
+    N = 3
+    X = rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N))
+    ops = []
+    for _ in range(2):
+        Y = rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N))
+        ops.append((Y + Y.conj().T) / 4)
+    return taco_amd.System.build((X + X.conj().T) / 2, ops), rng
+
+
+def test_tcl2_channel_matrices_match_oracle(orc):
+    """build_tcl2_components in its general form (generator.cpp:81-177): two
+    non-commuting coupling operators and full complex channel matrices
+    Gamma_b(alpha, beta) with off-diagonal entries, at several times."""
+    S, rng = _two_channel_system()
+    Nt, nf = 40, S.nf
+    G = rng.normal(size=(Nt, nf, 2, 2)) + 1j * rng.normal(size=(Nt, nf, 2, 2))
+    G[:, 1, 0, 1] = 0.0          # exact zeros are skipped, as in the reference
+    G[:, 2, 1, 1] = 0.3          # real: no Lamb-shift part
+    tidx = [0, 3, 17, 39]
+    got = taco_amd.build_tcl2_generator(S, G, tidx)
+    for q, t in enumerate(tidx):
+        ref = orc.tcl2_general_at(S.eps, S.A_eig, S.pair_to_freq, G[t])
+        assert np.abs(got[q] - ref).max() <= 1e-13 * np.abs(ref).max(), t
+
+
+def test_tcl2_diagonal_channels_match_the_drivers_generator(orc):
+    """Diagonal channel matrices Gamma_b(alpha, beta) = delta_ab gamma_b are the
+    drivers' case: equal to the oracle's tcl2_at on the spin-boson Gamma table."""
+    w = W.spin_boson(Nt=256, tmax=20)
+    S = w.system
+    G = np.zeros((256, S.nf, 1, 1), complex)
+    G[:, :, 0, 0] = w.gamma
+    got = taco_amd.build_tcl2_generator(S, G, [5, 100])
+    for q, t in enumerate([5, 100]):
+        ref = orc.tcl2_at(S.eps, S.A_eig, S.pair_to_freq, w.gamma[t])
+        assert np.abs(got[q] - ref).max() <= 1e-14 * np.abs(ref).max()
+
+
+def test_tcl2_skips_channels_whose_adag_a_vanishes(orc):
+    """The reference skips a (b, alpha, beta) whose A_alpha,b^dag A_beta,b is
+    the zero matrix, even where kron(conj(A_alpha,b), A_beta,b) is not: with
+    equally spaced levels the pairs (0,1) and (1,2) share the bucket -1, and
+    operators on either pair meet no common row q."""
+    H = np.diag([0.0, 1.0, 2.0]).astype(complex)
+    A1 = np.zeros((3, 3), complex)
+    A2 = np.zeros((3, 3), complex)
+    A1[0, 1] = A1[1, 0] = 1.0
+    A2[1, 2] = A2[2, 1] = 0.5
+    S = taco_amd.System.build(H, [A1, A2])
+    p2f = np.asarray(S.pair_to_freq)
+    b = p2f[0, 1]
+    assert p2f[1, 2] == b
+    P1, P2 = (np.where(p2f == b, np.asarray(a), 0) for a in S.A_eig)
+    assert not np.any(P1.conj().T @ P2) and np.any(np.kron(P1.conj(), P2))
+    rng = np.random.default_rng(3)
+    G = rng.normal(size=(8, S.nf, 2, 2)) + 1j * rng.normal(size=(8, S.nf, 2, 2))
+    got = taco_amd.build_tcl2_generator(S, G, [2, 7])
+    for q, t in enumerate([2, 7]):
+        ref = orc.tcl2_general_at(S.eps, S.A_eig, S.pair_to_freq, G[t])
+        assert np.abs(got[q] - ref).max() <= 1e-13 * np.abs(ref).max()

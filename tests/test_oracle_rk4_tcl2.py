@@ -86,3 +86,59 @@ def test_tcl2_matches_kron_restatement_and_preserves_trace(orc):
     N = 2
     diag_rows = [n + n * N for n in range(N)]
     assert np.abs(L[diag_rows].sum(axis=0)).max() < 1e-14 * np.abs(L).max() * 10
+
+
+def _two_channel_system(orc, seed=7):
+    rng = np.random.default_rng(seed)
+    N = 3
+    X = rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N))
+    H = (X + X.conj().T) / 2
+    ops = []
+    for _ in range(2):
+        Y = rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N))
+        ops.append((Y + Y.conj().T) / 4)
+    return orc.system_build(H, ops), rng
+
+
+def test_tcl2_channels_single_entry_is_the_lindblad_form(orc):
+    """With one channel entry Gamma_b(alpha, beta) = g (real) and all others
+    zero, the generator minus L0 acts on rho as
+    g (A_beta,b rho A_alpha,b^dag - 1/2 {A_alpha,b^dag A_beta,b, rho}) (the
+    master-equation form of generator.cpp:92-162, evaluated with matrix
+    products, not Kronecker products): pins the channel indices, the operand
+    order and the conjugation."""
+    s, rng = _two_channel_system(orc)
+    eps, N, nf = np.asarray(s["eps"]), 3, s["nf"]
+    ops = [np.asarray(a) for a in s["A_eig"]]
+    p2f = np.asarray(s["pair_to_freq"])
+    L0 = orc.tcl2_general_at(eps, ops, p2f, np.zeros((nf, 2, 2), complex))
+    rho = rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N))
+    for b in range(nf):
+        parts = [np.where(p2f == b, a, 0) for a in ops]
+        for al in range(2):
+            for be in range(2):
+                if not np.any(parts[al].conj().T @ parts[be]):
+                    continue
+                G = np.zeros((nf, 2, 2), complex)
+                G[b, al, be] = 0.5
+                L = orc.tcl2_general_at(eps, ops, p2f, G) - L0
+                got = (L @ rho.reshape(-1, order="F")).reshape(N, N, order="F")
+                Aa, Ab = parts[al], parts[be]
+                AdA = Aa.conj().T @ Ab
+                ref = 1.0 * (Ab @ rho @ Aa.conj().T - 0.5 * (AdA @ rho + rho @ AdA))
+                assert np.abs(got - ref).max() <= 1e-14 * max(1.0, np.abs(ref).max()), (b, al, be)
+
+
+def test_tcl2_channels_of_a_correlated_pair_equal_one_channel(orc):
+    """B_2 = c B_1 (real c): Gamma_b(alpha, beta) = Gamma_b c_alpha c_beta with
+    operators A_1, A_2 equals one channel with A_1 + c A_2 and Gamma_b."""
+    s, rng = _two_channel_system(orc, seed=8)
+    eps, nf = np.asarray(s["eps"]), s["nf"]
+    ops = [np.asarray(a) for a in s["A_eig"]]
+    p2f = np.asarray(s["pair_to_freq"])
+    c = np.array([1.0, -0.7])
+    gam = rng.normal(size=nf) + 1j * rng.normal(size=nf)
+    G = gam[:, None, None] * np.outer(c, c)[None]
+    two = orc.tcl2_general_at(eps, ops, p2f, G)
+    one = orc.tcl2_at(eps, [ops[0] + c[1] * ops[1]], p2f, gam)
+    assert np.abs(two - one).max() <= 1e-13 * np.abs(one).max()
