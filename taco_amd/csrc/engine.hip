@@ -319,11 +319,15 @@ struct FcrArgs {
     const int* job_tri;     // sorted position -> triple id (triples grouped by equal Omega)
     const unsigned char* job_mask;  // per sorted position: bit0 F needed, bit1 C needed, bit2 R needed
     long nfft_jobs;         // positions [0, nfft_jobs) need FFTs; the rest only R
-    const int* cls_start;   // [ncls + 1] first sorted position of each Omega class
-    const int* bperm;       // phase B processing order: grid slot (per chunk) -> sorted position
+    const int* ajob_start;  // [nA + 1] first sorted position of each A-job (the triples
+                            // of one Omega class with the same i: identical phase-A input)
+    const int* cls_start;   // [ncls + 1] first A-job of each Omega-class piece
+    const int* bperm;       // phase B processing order: grid slot (per chunk) -> A-job
     long job0;              // first sorted position of the FFT chunk
     int njobs;              // triples in the FFT chunk
-    int cls0;               // first class touching the chunk
+    long a0;                // first A-job of the FFT chunk
+    int na;                 // A-jobs in the FFT chunk
+    int cls0;               // first class piece touching the chunk
     long comb0;             // first sorted position of the combine batch
     long cconv;             // combine-batch capacity (conv slots)
     double2* inter1;        // [njobs][2^a][2^b]
@@ -384,9 +388,11 @@ __global__ __launch_bounds__(Cfg::kFT) void k_phase_a_spec(FcrArgs P) {
     }
 }
 
-// Phase A of the triples: one workgroup per (Omega class, group block).  The
-// phases exp(-i Omega t_k) of the tile are computed once and applied to the
-// Gamma column g_i of every triple of the class inside the chunk.
+// Phase A of the triples: one workgroup per (Omega-class piece, group block).
+// The phases exp(-i Omega t_k) of the tile are computed once and applied to
+// the Gamma column g_i of every A-job of the piece inside the chunk; triples
+// of a class that share i share the input g_i exp(-i Omega t), so one A-job
+// (its first triple) computes their common half-transformed signal.
 template <class Cfg>
 __global__ __launch_bounds__(Cfg::kFT) void k_phase_a(FcrArgs P) {
     constexpr int kFT = Cfg::kFT;
@@ -398,14 +404,15 @@ __global__ __launch_bounds__(Cfg::kFT) void k_phase_a(FcrArgs P) {
     int cj, gb;
     xcd_map(blockIdx.x, ngroups / G, cj, gb);
     const int cls = P.cls0 + cj;
-    const long pbeg = max(static_cast<long>(P.cls_start[cls]), P.job0);
-    const long pend = min(static_cast<long>(P.cls_start[cls + 1]), P.job0 + P.njobs);
-    if (pbeg >= pend) return;
+    const long abeg = max(static_cast<long>(P.cls_start[cls]), P.a0);
+    const long aend = min(static_cast<long>(P.cls_start[cls + 1]), P.a0 + P.na);
+    if (abeg >= aend) return;
     const int r0b = gb * G;
+    const long pfirst = P.ajob_start[abeg];
     int i0, j0, k0;
-    decode_triple(P.job_tri[pbeg], P.nf, i0, j0, k0);
+    decode_triple(P.job_tri[pfirst], P.nf, i0, j0, k0);
     const double omega = (P.omegas[i0] + P.omegas[j0]) + P.omegas[k0];
-    const double2* prow = P.ph_tab ? P.ph_tab + static_cast<long>(P.job_row[pbeg]) * P.ph_len : nullptr;
+    const double2* prow = P.ph_tab ? P.ph_tab + static_cast<long>(P.job_row[pfirst]) * P.ph_len : nullptr;
     const int total = G * gs;
     double2 ph[kPT];
 #pragma unroll
@@ -416,9 +423,9 @@ __global__ __launch_bounds__(Cfg::kFT) void k_phase_a(FcrArgs P) {
         ph[q] = make_double2(0.0, 0.0);
         if (e < total && t < P.Nt) ph[q] = cconj(phase_plus(prow, t, omega, P.dt));
     }
-    for (long p = pbeg; p < pend; ++p) {
+    for (long aj = abeg; aj < aend; ++aj) {
         int i, j, k;
-        decode_triple(P.job_tri[p], P.nf, i, j, k);
+        decode_triple(P.job_tri[P.ajob_start[aj]], P.nf, i, j, k);
         const double2* src = P.gamma + static_cast<long>(i) * P.Nt;
 #pragma unroll
         for (int q = 0; q < kPT; ++q) {
@@ -431,7 +438,7 @@ __global__ __launch_bounds__(Cfg::kFT) void k_phase_a(FcrArgs P) {
         }
         __syncthreads();
         tile_fft<Cfg>(buf, Tl, P.twStd, 0, false);
-        double2* out = P.inter1 + (p - P.job0) * P.n;
+        double2* out = P.inter1 + (aj - P.a0) * P.n;
         for (int e = threadIdx.x; e < total; e += kFT) {
             const int grp = Tl.grp_of(e), pos = Tl.pos_of(e);
             out[(static_cast<long>(pos) << P.b) + r0b + grp] = buf[Tl.at(grp, pos)];
@@ -452,10 +459,10 @@ __global__ __launch_bounds__(Cfg::kFT) TACO_FFT_OCC(Cfg) void k_phase_b(FcrArgs 
     const int G = Tl.G, gs = Tl.gs;
     int job, gb;
     if (mode == 0 && P.b_job_fast)
-        xcd_map_job_fast(blockIdx.x, ngroups / G, P.njobs, job, gb);
+        xcd_map_job_fast(blockIdx.x, ngroups / G, P.na, job, gb);
     else
         xcd_map(blockIdx.x, ngroups / G, job, gb);
-    if (mode == 0) job = P.bperm[P.job0 + job] - static_cast<int>(P.job0);
+    if (mode == 0) job = P.bperm[P.a0 + job] - static_cast<int>(P.a0);  // A-job within the chunk
     const int k1b = gb * G;
     const int total = G * gs;
     const double2* in = P.inter1 + static_cast<long>(job) * P.n + static_cast<long>(k1b) * gs;
@@ -475,27 +482,30 @@ __global__ __launch_bounds__(Cfg::kFT) TACO_FFT_OCC(Cfg) void k_phase_b(FcrArgs 
         if (e < total) X[q] = buf[Tl.at(e >> Tl.lg, e & (gs - 1))];
     }
     __syncthreads();
-    int i, j, k;
-    decode_triple(P.job_tri[P.job0 + job], P.nf, i, j, k);
-    const int bF = P.mirror[j];
-    const int mask = P.job_mask[P.job0 + job];
-    for (int path = 0; path < 2; ++path) {
-        if (!(mask & (1 << path))) continue;
-        const double2* sp = P.spec + static_cast<long>(path == 0 ? 2 * bF : 2 * j + 1) * P.n +
-                            static_cast<long>(k1b) * gs;
+    const long pend = P.ajob_start[P.a0 + job + 1];
+    for (long p = P.ajob_start[P.a0 + job]; p < pend; ++p) {
+        int i, j, k;
+        decode_triple(P.job_tri[p], P.nf, i, j, k);
+        const int bF = P.mirror[j];
+        const int mask = P.job_mask[p];
+        for (int path = 0; path < 2; ++path) {
+            if (!(mask & (1 << path))) continue;
+            const double2* sp = P.spec + static_cast<long>(path == 0 ? 2 * bF : 2 * j + 1) * P.n +
+                                static_cast<long>(k1b) * gs;
 #pragma unroll
-        for (int q = 0; q < kPT; ++q) {
-            const int e = threadIdx.x + q * kFT;
-            if (e < total) buf[Tl.at(e >> Tl.lg, e & (gs - 1))] = cmul(X[q], sp[e]);
+            for (int q = 0; q < kPT; ++q) {
+                const int e = threadIdx.x + q * kFT;
+                if (e < total) buf[Tl.at(e >> Tl.lg, e & (gs - 1))] = cmul(X[q], sp[e]);
+            }
+            __syncthreads();
+            tile_fft<Cfg, true>(buf, Tl, P.twStd, 0, true);
+            double2* out = P.inter2 + (static_cast<long>(path) * P.njobs + (p - P.job0)) * P.n;
+            for (int e = threadIdx.x; e < total; e += kFT) {
+                const int grp = Tl.grp_of(e), pos = Tl.pos_of(e);
+                out[(static_cast<long>(pos) << P.a) + k1b + grp] = buf[Tl.at(grp, pos)];
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        tile_fft<Cfg, true>(buf, Tl, P.twStd, 0, true);
-        double2* out = P.inter2 + (static_cast<long>(path) * P.njobs + job) * P.n;
-        for (int e = threadIdx.x; e < total; e += kFT) {
-            const int grp = Tl.grp_of(e), pos = Tl.pos_of(e);
-            out[(static_cast<long>(pos) << P.a) + k1b + grp] = buf[Tl.at(grp, pos)];
-        }
-        __syncthreads();
     }
 }
 
@@ -2125,9 +2135,9 @@ static std::size_t chunk_budget_bytes() {
 
 // Times every tile configuration on this chunk, phase by phase in pipeline
 // order (each run rewrites the same values), and records the fastest.
-static void autotune(taco_ctx* c, const FcrArgs& P, long ncls, long nj, std::array<int, 3>& choice) {
+static void autotune(taco_ctx* c, const FcrArgs& P, long ncls, long na, long nj, std::array<int, 3>& choice) {
     const Phase phases[3] = {PH_A_TRIPLE, PH_B_TRIPLE, PH_C};
-    const long units[3] = {ncls, nj, nj};
+    const long units[3] = {ncls, na, nj};
     hipEvent_t e0, e1;
     HIP_OK(hipEventCreate(&e0));
     HIP_OK(hipEventCreate(&e1));
@@ -2230,7 +2240,12 @@ static std::size_t env_bytes(const char* name, std::size_t dflt) {
 // context at launch time, so passes prepared one after another can share it.
 struct FcrJob {
     FcrArgs P{};
-    std::vector<int> cls_start;  // host copy of the Omega-class starts
+    std::vector<int> cls_start;  // host copy of the Omega-class piece starts (A-job indices)
+    std::vector<int> ajs;        // host copy of the A-job starts (sorted positions)
+    std::vector<long> chunk_a;   // FFT chunks: A-jobs [chunk_a[c], chunk_a[c + 1])
+    // combine batches: positions [comb0, comb0 + cb) holding chunks [c0, c1)
+    struct Batch { long comb0, cb, c0, c1; };
+    std::vector<Batch> batches;
     std::vector<long> paths;     // [ncomb + 1] prefix count of the F and C paths (traffic accounting)
     std::vector<long> outs;      // [ncomb + 1] prefix count of the F, C and R outputs
     std::vector<int> order;      // host copy of the combine list (position -> triple)
@@ -2340,32 +2355,54 @@ static FcrJob fcr_prepare(taco_ctx* c, const FcrPlan& pl, const FcrLists& lists)
     mask.resize(order.size());
     for (std::size_t q = 0; q < order.size(); ++q)
         mask[q] = pl.need ? (*pl.need)[static_cast<std::size_t>(order[q])] : 7;
-    // Class starts; a class longer than TACO_CLASS_MAX triples (default 4) is
-    // cut into pieces, each recomputing the same phases, so no phase-A
-    // workgroup loops much longer than the others.
+    // A-jobs: within a class the positions are in ascending triple order, so
+    // the triples with the same i (the same phase-A input g_i exp(-i Omega t))
+    // are consecutive; each run is one A-job.
+    std::vector<int> ajs;
+    for (long p = 0; p < nfft_jobs; ++p) {
+        const int i = order[p] / (nf * nf);
+        if (p == 0 || key[order[p]] != key[order[p - 1]] || i != order[p - 1] / (nf * nf))
+            ajs.push_back(static_cast<int>(p));
+    }
+    const long nA = static_cast<long>(ajs.size());
+    ajs.push_back(static_cast<int>(nfft_jobs));
+    // Class pieces over A-jobs; a class with more than TACO_CLASS_MAX A-jobs
+    // (default 4) is cut into pieces, each recomputing the same phases, so no
+    // phase-A workgroup loops much longer than the others.
     const int cmax = std::max(1, env_int("TACO_CLASS_MAX", 4));
     std::vector<int> cls_start;
-    for (long p = 0, run = 0; p < nfft_jobs; ++p, ++run) {
-        if (p == 0 || key[order[p]] != key[order[p - 1]]) run = 0;
-        if (run % cmax == 0) cls_start.push_back(static_cast<int>(p));
+    for (long a = 0, run = 0; a < nA; ++a, ++run) {
+        if (a == 0 || key[order[ajs[a]]] != key[order[ajs[a - 1]]]) run = 0;
+        if (run % cmax == 0) cls_start.push_back(static_cast<int>(a));
     }
-    cls_start.push_back(static_cast<int>(nfft_jobs));
+    cls_start.push_back(static_cast<int>(nA));
     // Phase B visits the FFT jobs of each chunk grouped by j, so the triples
     // running at the same time read the same two spectra (of g_mirror(j) and
     // conj(g_j)) and share them through L2.  The chunk length is fixed here
     // (chunks start at multiples of it).
     const std::size_t per_job_b = 3 * n * 16;
     long chunk_len = static_cast<long>(std::max<std::size_t>(1, chunk_budget_bytes() / per_job_b));
-    chunk_len = std::max(1L, std::min(chunk_len, nfft_jobs));
-    std::vector<int> bperm(static_cast<std::size_t>(nfft_jobs));
-    for (long c0 = 0; c0 < nfft_jobs; c0 += chunk_len) {
-        const long c1 = std::min(nfft_jobs, c0 + chunk_len);
+    chunk_len = std::max(1L, std::min(chunk_len, std::max(nfft_jobs, 1L)));
+    // FFT chunks: whole A-jobs, at most chunk_len triples (a longer A-job alone)
+    std::vector<long> chunk_a{0};
+    for (long a = 0; a < nA;) {
+        long a1 = a + 1;
+        while (a1 < nA && ajs[a1 + 1] - ajs[chunk_a.back()] <= chunk_len) ++a1;
+        chunk_a.push_back(a1);
+        a = a1;
+    }
+    // Phase B visits the A-jobs of each chunk grouped by the j of their first
+    // triple, so the workgroups running at one time mostly read the same
+    // spectra (of g_mirror(j) and conj g_j), from L2.
+    std::vector<int> bperm(static_cast<std::size_t>(nA));
+    for (std::size_t c = 0; c + 1 < chunk_a.size(); ++c) {
+        const long c0 = chunk_a[c], c1 = chunk_a[c + 1];
         std::iota(bperm.begin() + c0, bperm.begin() + c1, static_cast<int>(c0));
         std::stable_sort(bperm.begin() + c0, bperm.begin() + c1, [&](int x, int y) {
-            return (order[x] / nf) % nf < (order[y] / nf) % nf;
+            return (order[ajs[x]] / nf) % nf < (order[ajs[y]] / nf) % nf;
         });
     }
-    lists.jobtri->ensure(std::max<std::size_t>(order.size() + bperm.size(), 1) * sizeof(int));
+    lists.jobtri->ensure(std::max<std::size_t>(order.size() + bperm.size() + ajs.size(), 1) * sizeof(int));
     lists.jobmask->ensure(std::max<std::size_t>(mask.size(), 1));
     lists.clsstart->ensure(cls_start.size() * sizeof(int));
     if (!order.empty()) {
@@ -2376,16 +2413,36 @@ static FcrJob fcr_prepare(taco_ctx* c, const FcrPlan& pl, const FcrLists& lists)
             HIP_OK(hipMemcpyAsync(lists.jobtri->as<int>() + order.size(), bperm.data(), bperm.size() * sizeof(int),
                                   hipMemcpyHostToDevice, c->stream));
     }
+    HIP_OK(hipMemcpyAsync(lists.jobtri->as<int>() + order.size() + bperm.size(), ajs.data(), ajs.size() * sizeof(int),
+                          hipMemcpyHostToDevice, c->stream));
     HIP_OK(hipMemcpyAsync(lists.clsstart->p, cls_start.data(), cls_start.size() * sizeof(int),
                           hipMemcpyHostToDevice, c->stream));
     const PhaseRows prow = phase_rows(c, pl, order, key, lists);
 
-    // FFT chunk (intermediates of nj triples) and combine batch (conv slots).
-    const long chunk = chunk_len;
+    // FFT chunk (intermediates of at most chunk triples) and combine batches
+    // (conv slots): consecutive chunks while they fit cconv slots, then the
+    // R-only triples in batches of cconv.
+    long chunk = 1;
+    for (std::size_t c = 0; c + 1 < chunk_a.size(); ++c)
+        chunk = std::max(chunk, static_cast<long>(ajs[chunk_a[c + 1]] - ajs[chunk_a[c]]));
     const std::size_t Tpad = std::max<std::size_t>(pl.T, 1);
     long cconv = static_cast<long>(env_bytes("TACO_CONV_BYTES", std::size_t{4} << 30) / (2 * Tpad * 16));
-    cconv = std::max(chunk, (cconv / chunk) * chunk);
-    cconv = std::max(1L, std::min(cconv, ncomb));
+    cconv = std::max(cconv, chunk);
+    cconv = std::max(1L, std::min(cconv, std::max(ncomb, chunk)));
+    std::vector<FcrJob::Batch> batches;
+    {
+        std::size_t c = 0;
+        const std::size_t nch = chunk_a.size() - 1;
+        while (c < nch) {
+            const long comb0 = ajs[chunk_a[c]];
+            std::size_t c1 = c + 1;
+            while (c1 < nch && ajs[chunk_a[c1 + 1]] - comb0 <= cconv) ++c1;
+            batches.push_back({comb0, ajs[chunk_a[c1]] - comb0, static_cast<long>(c), static_cast<long>(c1)});
+            c = c1;
+        }
+        for (long p = nfft_jobs; p < ncomb; p += cconv)
+            batches.push_back({p, std::min(cconv, ncomb - p), static_cast<long>(nch), static_cast<long>(nch)});
+    }
     const long spec_jobs = 2L * nf;
     c->spec.ensure(static_cast<std::size_t>(spec_jobs) * n * 16);
     c->inter1.ensure(static_cast<std::size_t>(std::max(chunk, spec_jobs)) * n * 16);
@@ -2408,6 +2465,7 @@ static FcrJob fcr_prepare(taco_ctx* c, const FcrPlan& pl, const FcrLists& lists)
     P.twC = tw.c_.as<double2>();
     P.job_tri = lists.jobtri->as<int>();
     P.bperm = P.job_tri + order.size();
+    P.ajob_start = P.bperm + bperm.size();
     P.job_mask = lists.jobmask->as<unsigned char>();
     P.nfft_jobs = nfft_jobs;
     P.cls_start = lists.clsstart->as<int>();
@@ -2418,6 +2476,9 @@ static FcrJob fcr_prepare(taco_ctx* c, const FcrPlan& pl, const FcrLists& lists)
     P.ph_len = prow.len;
     P.b_job_fast = env_int("TACO_PHASEB_JOB_FAST", 0);
     J.cls_start = std::move(cls_start);
+    J.ajs = std::move(ajs);
+    J.chunk_a = std::move(chunk_a);
+    J.batches = std::move(batches);
     J.paths.assign(static_cast<std::size_t>(ncomb) + 1, 0);
     J.outs.assign(static_cast<std::size_t>(ncomb) + 1, 0);
     for (long p = 0; p < ncomb; ++p) {
@@ -2449,7 +2510,6 @@ static void fcr_launch(taco_ctx* c, FcrJob& J, const CombineArgs& out) {
     P.spec = c->spec.as<double2>();
     P.conv = c->conv.as<double2>();
     const int L = J.L;
-    const long nfft_jobs = J.nfft_jobs, ncomb = J.ncomb, chunk = J.chunk, cconv = J.cconv;
     const long spec_jobs = J.spec_jobs;
     const std::vector<int>& cls_start = J.cls_start;
     // Tile configuration per phase: TACO_FFT_TILE=<0..6> forces one; otherwise
@@ -2470,27 +2530,32 @@ static void fcr_launch(taco_ctx* c, FcrJob& J, const CombineArgs& out) {
         launch_phase(choice[1] >= 0 ? choice[1] : 0, PH_B_SPEC, P, spec_jobs, c->stream);
     });
     HIP_OK(hipGetLastError());
-    auto class_of = [&](long pos) {
-        return static_cast<int>(std::upper_bound(cls_start.begin(), cls_start.end(), static_cast<int>(pos)) -
+    auto class_of = [&](long aj) {
+        return static_cast<int>(std::upper_bound(cls_start.begin(), cls_start.end(), static_cast<int>(aj)) -
                                 cls_start.begin()) - 1;
     };
-    for (long comb0 = 0; comb0 < ncomb; comb0 += cconv) {
-        const long cb = std::min(cconv, ncomb - comb0);
+    for (const FcrJob::Batch& bt : J.batches) {
+        const long comb0 = bt.comb0, cb = bt.cb;
         P.comb0 = comb0;
-        const long fft_end = std::min(comb0 + cb, nfft_jobs);
-        for (long job0 = comb0; job0 < fft_end; job0 += chunk) {
-            const long nj = std::min(chunk, fft_end - job0);
+        for (long ci = bt.c0; ci < bt.c1; ++ci) {
+            const long a0 = J.chunk_a[static_cast<std::size_t>(ci)], a1 = J.chunk_a[static_cast<std::size_t>(ci) + 1];
+            const long job0 = J.ajs[static_cast<std::size_t>(a0)], nj = J.ajs[static_cast<std::size_t>(a1)] - job0;
             P.job0 = job0;
             P.njobs = static_cast<int>(nj);
-            P.cls0 = class_of(job0);
-            const long ncls = class_of(job0 + nj - 1) - P.cls0 + 1;
-            if (choice[0] < 0 || choice[1] < 0 || choice[2] < 0) autotune(c, P, ncls, nj, choice);
-            // phase A: read the Gamma column, write the half-transformed signal
-            timed(c, K_PHASE_A, nj * (cNt + cn), [&] { launch_phase(choice[0], PH_A_TRIPLE, P, ncls, c->stream); });
+            P.a0 = a0;
+            P.na = static_cast<int>(a1 - a0);
+            P.cls0 = class_of(a0);
+            const long ncls = class_of(a1 - 1) - P.cls0 + 1;
+            const double na = static_cast<double>(a1 - a0);
+            if (choice[0] < 0 || choice[1] < 0 || choice[2] < 0) autotune(c, P, ncls, a1 - a0, nj, choice);
+            // phase A: read the Gamma column, write the half-transformed signal, once per A-job
+            timed(c, K_PHASE_A, na * (cNt + cn), [&] { launch_phase(choice[0], PH_A_TRIPLE, P, ncls, c->stream); });
             // F and C paths of the chunk (one or both per triple)
             const double np_ = static_cast<double>(J.paths[job0 + nj] - J.paths[job0]);
-            // phase B: read it and one spectrum per path, write one half-inverse product per path
-            timed(c, K_PHASE_B, nj * cn + np_ * 2 * cn, [&] { launch_phase(choice[1], PH_B_TRIPLE, P, nj, c->stream); });
+            // phase B: read each A-job's signal once and one spectrum per path, write one
+            // half-inverse product per path
+            timed(c, K_PHASE_B, na * cn + np_ * 2 * cn,
+                  [&] { launch_phase(choice[1], PH_B_TRIPLE, P, a1 - a0, c->stream); });
             // phase C: read each path's product, write its convolution for t < T
             timed(c, K_PHASE_C, np_ * (cn + cT), [&] { launch_phase(choice[2], PH_C, P, nj, c->stream); });
             HIP_OK(hipGetLastError());
