@@ -101,7 +101,7 @@ def cpu_baseline_leg(w, L4_gpu: np.ndarray, threads: int) -> tuple[dict, dict]:
         "oracle_libm_vs_cr_phases_max_per_t": float(e_ref_libm_vs_cr.max()),
         "pass_1e-10": bool(e_libm.max() <= 1e-10),
     }
-    return baseline, parity
+    return baseline, parity, ref_libm
 
 
 def kernel_table(kern: dict, steps: int) -> tuple[dict, dict]:
@@ -349,6 +349,55 @@ def adjoint_record(a, steps: int | None = None, warmup: int | None = None):
     return rec
 
 
+def window_record(w, ref_libm, threads: int, steps: int = 5) -> dict:
+    """The same workload in the window FFT mode (rows t < T only, Nfft =
+    next_pow2(2T - 1), exact twiddles): L4 time-points/s from inputs resident
+    in HBM, its per-t error against the parity-mode oracle (the reference's
+    recurrence-twiddle FFT) and against the reference's Direct O(T^2) method
+    on t <= 500.  A separate labelled line, never the parity headline."""
+    import torch
+
+    import taco_amd
+    from taco_amd import tcl4
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from metrics import per_t_rel_err
+    from oracle import oracle
+
+    S, G = w.system, w.gamma
+    ntau = len(w.tidx)
+    with taco_amd.fft_mode(taco_amd.FFT_WINDOW):
+        dsys = tcl4.DeviceSystem(S, G, 0)
+        out = torch.empty((ntau, S.N ** 4), dtype=torch.complex128, device="cuda:0")
+        tcl4.build_l4_dev(dsys, w.dt, w.tidx, out)
+        tcl4.build_l4_dev(dsys, w.dt, w.tidx, out)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            tcl4.build_l4_dev(dsys, w.dt, w.tidx, out)
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / steps
+        L4 = out.cpu().numpy().reshape(ntau, S.N ** 2, S.N ** 2).transpose(0, 2, 1)
+        short = [t for t in w.tidx if t <= 500]
+        L4s = taco_amd.build_TCL4_generator_hip_fused_batch(S, G, w.dt, short)
+    rec = {"workload": f"{w.name} in the window FFT mode, Nfft {1 << int(np.ceil(np.log2(2 * (max(w.tidx) + 1) - 1)))}",
+           "value": ntau / el, "unit": "L4 time-points/s", "ms_per_step": 1e3 * el, "steps": steps}
+    if ref_libm is not None:
+        e, _ = per_t_rel_err(L4, ref_libm)
+        rec["max_per_t_rel_err_vs_parity_oracle"] = float(e.max())
+        rec["max_per_t_rel_err_vs_parity_oracle_t_ge_20"] = float(e[20:].max()) if ntau > 20 else None
+        t0 = time.perf_counter()
+        with oracle.phase_mode(oracle.PHASE_LIBM):
+            direct = oracle.l4_series(G, S.omegas, S.mirror, S.pair_to_freq, S.A_eig, w.dt, short, method="direct",
+                                      nthreads=threads)
+        ed, _ = per_t_rel_err(L4s, direct)
+        ep, _ = per_t_rel_err(ref_libm[: len(short)], direct)
+        rec["max_per_t_rel_err_vs_direct_t_le_500"] = float(ed.max())
+        rec["parity_oracle_vs_direct_t_le_500"] = float(ep.max())
+        rec["direct_seconds"] = time.perf_counter() - t0
+    return rec
+
+
 def host_pointer_record(w, steps: int = 3) -> dict:
     """taco_l4_batch from host buffers: the Gamma table copied in, the L4
     series copied out and repacked into per-time matrices, as the reference's
@@ -570,11 +619,11 @@ def main():
     tiles = ["8192/512", "4096/256", "4096/512", "2048/256", "4096/1024", "2048/512", "8192/1024"]
     fft_tiles = {ph: tiles[c] if c >= 0 else None for ph, c in zip("ABC", choice)}
     L4 = out.cpu().numpy().reshape(ntau, S.N ** 2, S.N ** 2).transpose(0, 2, 1)
-    cpu, parity = None, None
+    cpu, parity, ref_libm = None, None, None
     if not a.no_cpu:
         threads = a.cpu_threads or len(os.sched_getaffinity(0))
         print(f"[bench] cpu baseline / checker leg on {threads} threads", file=sys.stderr, flush=True)
-        cpu, parity = cpu_baseline_leg(w, L4, threads)
+        cpu, parity, ref_libm = cpu_baseline_leg(w, L4, threads)
         print(f"[bench] cpu leg done in {cpu['seconds']:.1f} s", file=sys.stderr, flush=True)
     rec = {
         "metric": METRIC,
@@ -616,6 +665,11 @@ def main():
               flush=True)
         rec["host_pointer"] = [host_pointer_record(w), host_pointer_record(W.published())]
         rec["host_pointer_wall_s"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        print("[bench] sub-record: window FFT mode (d=4 bench workload)", file=sys.stderr, flush=True)
+        rec["window"] = window_record(w, None if a.no_cpu else ref_libm,
+                                      a.cpu_threads or len(os.sched_getaffinity(0)))
+        rec["window"]["wall_s"] = time.perf_counter() - t0
         if not a.no_cpu:
             t0 = time.perf_counter()
             print("[bench] sub-record: serial CPU (published d=2 setting)", file=sys.stderr, flush=True)

@@ -80,6 +80,22 @@ size_t taco_get_fcr_fft_pad_factor(void);
 int taco_set_phase_mode(int mode);
 int taco_get_phase_mode(void);
 
+/* F/C/R FFT mode, process-global.  TACO_FFT_PARITY (default): the
+ * reference's radix-2 FFT with its recurrence twiddles and its Nfft rule over
+ * the whole Gamma table (tcl4_kernels.cpp:31-68; bit-identical spectra).
+ * TACO_FFT_WINDOW (no reference counterpart): only the Gamma rows the
+ * requested window reads (t < T = max time + 1) enter the convolutions,
+ * Nfft = next_pow2(max(2T - 1, pad*T)), and the twiddles are exact
+ * (each rounded once from a quad-precision evaluation).  The causal
+ * convolutions at t < T are the same sums, so the results differ from the
+ * parity mode only by rounding, and are closer to the exact convolution
+ * (the recurrence twiddles drift by ~1e-11 at 2^19); the FFT work shrinks
+ * with Nfft.  Returns TACO_ERR_INVALID for an unknown mode. */
+#define TACO_FFT_PARITY 0
+#define TACO_FFT_WINDOW 1
+int taco_set_fft_mode(int mode);
+int taco_get_fft_mode(void);
+
 /* Per-kernel HIP-event profiler of a context (off by default).  Kernel ids:
  * 0 phase A, 1 phase B, 2 phase C, 3 F/C/R combine, 4 fused L4,
  * 5 per-bucket spectra, 6 TCL2, 7 RK4.  taco_profile_get returns the summed

@@ -475,10 +475,12 @@ __global__ __launch_bounds__(kBlock) void k_l4_adjoint(L4AdjArgs A) {
 // I = F(f(a,n),f(b,i),f(n,b)), K = R(f(i,b),f(n,a),f(a,i)) (the E
 // intermediate of k_l4f equals -S123, and its Wd equals Q).  The reverse
 // walks these back: Tbar -> (S123bar, S67bar, Dbar, U = A^H-contracted
-// Tbar) -> Qbar, S123bar -> slice and operator cotangents.  Slice cotangents
-// accumulate with atomic adds (LDS when 3 nf^3 fits, else global); operator
-// cotangents with LDS atomic adds of per-thread partial sums.  Each block
-// walks times blockIdx.x, blockIdx.x + gridDim.x, ... with its own scratch.
+// Tbar) -> Qbar, S123bar -> slice and operator cotangents.  Every cotangent
+// entry has one owner thread per stage that sums its terms in a fixed order
+// (gathers over host-built term lists for the slices), so gradients are
+// reproducible bit for bit.  Slice cotangents sit in LDS when 3 nf^3 fits,
+// else in global memory.  Each block walks times blockIdx.x, blockIdx.x +
+// gridDim.x, ... with its own scratch.
 // ---------------------------------------------------------------------------
 struct L4fAdjArgs {
     const double2* Lbar;  // [T][N^4]
@@ -496,6 +498,22 @@ struct L4fAdjArgs {
     int lds_tb;           // Tbar in LDS (else after U in the scratch)
     double2* scr;         // [gridDim.x][scr_per]
     long scr_per;         // 4 N^2 nf + 3 N^2 + N^4, + N^4 when !lds_tb
+    // Gather lists (host-built from the pair map) that give every cotangent
+    // entry one owner thread summing its terms in a fixed order:
+    // X-form pairs (x, z) with their (a, i, b, form) terms; F and R slice
+    // targets with their (n, i, a, b, kind) terms.  Packed: 8 bits per index.
+    const int* xp_xz;     // [nxp] x + z * 65536
+    const int* xp_start;  // [nxp + 1]
+    const int* xp_ent;    // a | i << 8 | b << 16 | form << 24
+    int nxp;
+    const int* f_tgt;     // [nft] slice index of F targets
+    const int* f_start;   // [nft + 1]
+    const int* f_ent;     // n | i << 8 | a << 16 | b << 24 (kind in bit 31)
+    int nft;
+    const int* r_tgt;
+    const int* r_start;
+    const int* r_ent;
+    int nrt;
 };
 
 // LDS_SLICES: slice cotangents accumulate in LDS (else in global memory);
@@ -613,7 +631,7 @@ __device__ __forceinline__ void l4f_adjoint_body(double2* sm, const L4fAdjArgs& 
                     const int j = e % N, m = e / N;
                     double2 s = make_double2(0.0, 0.0);
                     for (int r = 0; r < N2; ++r) s = cadd(s, cmul(cconj(S123[r]), Tb[r + (m + j * N) * N2]));
-                    atomic_cadd(AbA + j + m * N, s);
+                    AbA[j + m * N] = cadd(AbA[j + m * N], s);
                 }
                 __syncthreads();
                 // 3: Qbar; S123bar gets the D term; operator cotangents of the S67,
@@ -639,16 +657,34 @@ __device__ __forceinline__ void l4f_adjoint_body(double2* sm, const L4fAdjArgs& 
                     const long go = static_cast<long>(F2(j, m)) * N2;
                     double2 s = make_double2(0.0, 0.0);
                     for (int r = 0; r < N2; ++r) s = cadd(s, cmul(cconj(S67[go + r]), Tb[r + (m + j * N) * N2]));
-                    atomic_cadd(AbB + j + m * N, s);
+                    AbB[j + m * N] = cadd(AbB[j + m * N], s);
                 }
+                __syncthreads();
+                // v(j,b,m) = sum_r conj(Q(r, f(j,b))) Tbar(r; m,j), in U's space (U is done)
+                double2* V = U;
                 for (int e = threadIdx.x; e < N2 * N; e += kBlock) {
-                    const int j = e % N, b = (e / N) % N, m = e / N2;
-                    const long go = static_cast<long>(F2(j, b)) * N2;
+                    const int j = e % N, m = e / N2;
+                    const long go = static_cast<long>(F2(j, (e / N) % N)) * N2;
                     double2 v = make_double2(0.0, 0.0);
                     for (int r = 0; r < N2; ++r) v = cadd(v, cmul(cconj(Q[go + r]), Tb[r + (m + j * N) * N2]));
-                    atomic_cadd(AbA + b + m * N, cscale(cmul(cconj(BB(j, b)), v), -1.0));
-                    atomic_cadd(AbB + j + b * N, cscale(cmul(cconj(AA(b, m)), v), -1.0));
+                    V[e] = v;
                 }
+                __syncthreads();
+                // the B(j,b) A(b,m) Q term: each cotangent entry sums its v's in j (or m) order
+                for (int e = threadIdx.x; e < N2; e += kBlock) {
+                    const int b = e % N, m = e / N;
+                    double2 s = make_double2(0.0, 0.0);
+                    for (int j = 0; j < N; ++j) s = cadd(s, cmul(cconj(BB(j, b)), V[j + b * N + m * N2]));
+                    AbA[b + m * N] = csub(AbA[b + m * N], s);
+                }
+                __syncthreads();
+                for (int e = threadIdx.x; e < N2; e += kBlock) {
+                    const int j = e % N, b = e / N;
+                    double2 s = make_double2(0.0, 0.0);
+                    for (int m = 0; m < N; ++m) s = cadd(s, cmul(cconj(AA(b, m)), V[j + b * N + m * N2]));
+                    AbB[j + b * N] = csub(AbB[j + b * N], s);
+                }
+                __syncthreads();
                 for (int e = threadIdx.x; e < N2; e += kBlock) {
                     const int n = e % N, a = e / N;
                     double2 s = make_double2(0.0, 0.0);
@@ -657,58 +693,119 @@ __device__ __forceinline__ void l4f_adjoint_body(double2* sm, const L4fAdjArgs& 
                         for (int g = 0; g < nf; ++g)
                             s = cadd(s, cmul(cconj(Q[a + i * N + g * N2]), S67b[n + i * N + g * N2]));
                     }
-                    atomic_cadd(AbA + n + a * N, s);
+                    AbA[n + a * N] = cadd(AbA[n + a * N], s);
                 }
                 __syncthreads();
-                // 4: slice and operator cotangents of Q and of S123
-                for (long e = threadIdx.x; e < N2nf; e += kBlock) {
-                    const double2 qb = Qb[e];
-                    if (qb.x == 0.0 && qb.y == 0.0) continue;
-                    const int a = static_cast<int>(e % N), i = static_cast<int>((e / N) % N);
-                    const int g = static_cast<int>(e / N2);
-                    for (int b = 0; b < N; ++b) {
-                        const long x1 = id3(F2(b, a), g, F2(a, i)), x2 = id3(F2(i, b), g, F2(a, i));
-                        const double2 X1 = cadd(Cp[x1], Rp[x1]), X2 = cadd(Cp[x2], Rp[x2]);
-                        const double2 s1 = cmul(cconj(cmul(AA(a, b), BB(b, i))), qb);
-                        const double2 s2 = cscale(cmul(cconj(cmul(BB(a, b), AA(b, i))), qb), -1.0);
-                        atomic_cadd(sC + x1, s1);
-                        atomic_cadd(sR + x1, s1);
-                        atomic_cadd(sC + x2, s2);
-                        atomic_cadd(sR + x2, s2);
-                        atomic_cadd(AbA + a + b * N, cmul(cconj(cmul(BB(b, i), X1)), qb));
-                        atomic_cadd(AbB + b + i * N, cmul(cconj(cmul(AA(a, b), X1)), qb));
-                        atomic_cadd(AbB + a + b * N, cscale(cmul(cconj(cmul(AA(b, i), X2)), qb), -1.0));
-                        atomic_cadd(AbA + b + i * N, cscale(cmul(cconj(cmul(BB(a, b), X2)), qb), -1.0));
+                // 4: slice and operator cotangents of Q and of S123, each entry owned by
+                // one thread that sums its terms in a fixed order (no atomics)
+                auto Xv = [&](long x) { return cadd(Cp[x], Rp[x]); };
+                // X = C + R slices: pair (x, z) and bucket g gather the Q-bar terms
+                for (long e = threadIdx.x; e < static_cast<long>(A.nxp) * nf; e += kBlock) {
+                    const int pp = static_cast<int>(e % A.nxp), g = static_cast<int>(e / A.nxp);
+                    double2 acc = make_double2(0.0, 0.0);
+                    for (int q = A.xp_start[pp]; q < A.xp_start[pp + 1]; ++q) {
+                        const int w = A.xp_ent[q];
+                        const int a = w & 255, i = (w >> 8) & 255, b = (w >> 16) & 255;
+                        const double2 qb = Qb[a + i * N + g * N2];
+                        if (w >> 24)
+                            acc = csub(acc, cmul(cconj(cmul(BB(a, b), AA(b, i))), qb));
+                        else
+                            acc = cadd(acc, cmul(cconj(cmul(AA(a, b), BB(b, i))), qb));
                     }
+                    const int xz = A.xp_xz[pp];
+                    const long x = id3(xz & 65535, g, xz >> 16);
+                    sC[x] = cadd(sC[x], acc);
+                    sR[x] = cadd(sR[x], acc);
                 }
+                __syncthreads();
+                // F and R slices of the M, I, K terms of S123
+                for (int e = threadIdx.x; e < A.nft; e += kBlock) {
+                    double2 acc = make_double2(0.0, 0.0);
+                    for (int q = A.f_start[e]; q < A.f_start[e + 1]; ++q) {
+                        const unsigned w = static_cast<unsigned>(A.f_ent[q]);
+                        const int n = w & 255, i = (w >> 8) & 255, a = (w >> 16) & 255, b = (w >> 24) & 127;
+                        const double2 sv = S123b[n + i * N];
+                        if (w >> 31)  // +A(n,a) B(a,b) B(b,i) I
+                            acc = cadd(acc, cmul(cconj(cmul(cmul(AA(n, a), BB(a, b)), BB(b, i))), sv));
+                        else          // -B(n,a) A(a,b) B(b,i) M, M = F - R
+                            acc = csub(acc, cmul(cconj(cmul(cmul(BB(n, a), AA(a, b)), BB(b, i))), sv));
+                    }
+                    sF[A.f_tgt[e]] = cadd(sF[A.f_tgt[e]], acc);
+                }
+                for (int e = threadIdx.x; e < A.nrt; e += kBlock) {
+                    double2 acc = make_double2(0.0, 0.0);
+                    for (int q = A.r_start[e]; q < A.r_start[e + 1]; ++q) {
+                        const unsigned w = static_cast<unsigned>(A.r_ent[q]);
+                        const int n = w & 255, i = (w >> 8) & 255, a = (w >> 16) & 255, b = (w >> 24) & 127;
+                        const double2 sv = S123b[n + i * N];
+                        if (w >> 31)  // -B(n,a) B(a,b) A(b,i) K
+                            acc = csub(acc, cmul(cconj(cmul(cmul(BB(n, a), BB(a, b)), AA(b, i))), sv));
+                        else          // the -R part of M
+                            acc = cadd(acc, cmul(cconj(cmul(cmul(BB(n, a), AA(a, b)), BB(b, i))), sv));
+                    }
+                    sR[A.r_tgt[e]] = cadd(sR[A.r_tgt[e]], acc);
+                }
+                __syncthreads();
+                auto Mv = [&](int n, int i, int a, int b) {
+                    return csub(Fp[id3(F2(b, a), F2(b, i), F2(n, b))], Rp[id3(F2(b, a), F2(n, a), F2(a, i))]);
+                };
+                auto Iv = [&](int n, int i, int a, int b) { return Fp[id3(F2(a, n), F2(b, i), F2(n, b))]; };
+                auto Kv = [&](int n, int i, int a, int b) { return Rp[id3(F2(i, b), F2(n, a), F2(a, i))]; };
+                // operator entries at (a,b) of the Q and S123 terms
                 for (int e = threadIdx.x; e < N2; e += kBlock) {
-                    const double2 sv = S123b[e];
-                    if (sv.x == 0.0 && sv.y == 0.0) continue;
-                    const int n = e % N, i = e / N;
-                    for (int a = 0; a < N; ++a)
-                        for (int b = 0; b < N; ++b) {
-                            const long p1 = id3(F2(b, a), F2(b, i), F2(n, b)), q1 = id3(F2(b, a), F2(n, a), F2(a, i));
-                            const long p2 = id3(F2(a, n), F2(b, i), F2(n, b));
-                            const long q3 = id3(F2(i, b), F2(n, a), F2(a, i));
-                            const double2 Mv = csub(Fp[p1], Rp[q1]), Iv = Fp[p2], Kv = Rp[q3];
-                            // -B(n,a) A(a,b) B(b,i) M
-                            const double2 w1 = cmul(cconj(cmul(cmul(BB(n, a), AA(a, b)), BB(b, i))), sv);
-                            atomic_cadd(sF + p1, cscale(w1, -1.0));
-                            atomic_cadd(sR + q1, w1);
-                            atomic_cadd(AbB + n + a * N, cscale(cmul(cconj(cmul(cmul(AA(a, b), BB(b, i)), Mv)), sv), -1.0));
-                            atomic_cadd(AbA + a + b * N, cscale(cmul(cconj(cmul(cmul(BB(n, a), BB(b, i)), Mv)), sv), -1.0));
-                            atomic_cadd(AbB + b + i * N, cscale(cmul(cconj(cmul(cmul(BB(n, a), AA(a, b)), Mv)), sv), -1.0));
-                            // +A(n,a) B(a,b) B(b,i) I
-                            atomic_cadd(sF + p2, cmul(cconj(cmul(cmul(AA(n, a), BB(a, b)), BB(b, i))), sv));
-                            atomic_cadd(AbA + n + a * N, cmul(cconj(cmul(cmul(BB(a, b), BB(b, i)), Iv)), sv));
-                            atomic_cadd(AbB + a + b * N, cmul(cconj(cmul(cmul(AA(n, a), BB(b, i)), Iv)), sv));
-                            atomic_cadd(AbB + b + i * N, cmul(cconj(cmul(cmul(AA(n, a), BB(a, b)), Iv)), sv));
-                            // -B(n,a) B(a,b) A(b,i) K
-                            atomic_cadd(sR + q3, cscale(cmul(cconj(cmul(cmul(BB(n, a), BB(a, b)), AA(b, i))), sv), -1.0));
-                            atomic_cadd(AbB + n + a * N, cscale(cmul(cconj(cmul(cmul(BB(a, b), AA(b, i)), Kv)), sv), -1.0));
-                            atomic_cadd(AbB + a + b * N, cscale(cmul(cconj(cmul(cmul(BB(n, a), AA(b, i)), Kv)), sv), -1.0));
-                            atomic_cadd(AbA + b + i * N, cscale(cmul(cconj(cmul(cmul(BB(n, a), BB(a, b)), Kv)), sv), -1.0));
+                    const int a = e % N, b = e / N;
+                    double2 sa = make_double2(0.0, 0.0), sbb = sa;
+                    for (int i = 0; i < N; ++i)
+                        for (int g = 0; g < nf; ++g) {
+                            const double2 qb = Qb[a + i * N + g * N2];
+                            sa = cadd(sa, cmul(cconj(cmul(BB(b, i), Xv(id3(F2(b, a), g, F2(a, i))))), qb));
+                            sbb = csub(sbb, cmul(cconj(cmul(AA(b, i), Xv(id3(F2(i, b), g, F2(a, i))))), qb));
                         }
+                    for (int n = 0; n < N; ++n)
+                        for (int i = 0; i < N; ++i) {
+                            const double2 sv = S123b[n + i * N];
+                            sa = csub(sa, cmul(cconj(cmul(cmul(BB(n, a), BB(b, i)), Mv(n, i, a, b))), sv));
+                            sbb = cadd(sbb, cmul(cconj(cmul(cmul(AA(n, a), BB(b, i)), Iv(n, i, a, b))), sv));
+                            sbb = csub(sbb, cmul(cconj(cmul(cmul(BB(n, a), AA(b, i)), Kv(n, i, a, b))), sv));
+                        }
+                    AbA[a + b * N] = cadd(AbA[a + b * N], sa);
+                    AbB[a + b * N] = cadd(AbB[a + b * N], sbb);
+                }
+                __syncthreads();
+                // operator entries at (b,i)
+                for (int e = threadIdx.x; e < N2; e += kBlock) {
+                    const int b = e % N, i = e / N;
+                    double2 sa = make_double2(0.0, 0.0), sbb = sa;
+                    for (int a = 0; a < N; ++a)
+                        for (int g = 0; g < nf; ++g) {
+                            const double2 qb = Qb[a + i * N + g * N2];
+                            sbb = cadd(sbb, cmul(cconj(cmul(AA(a, b), Xv(id3(F2(b, a), g, F2(a, i))))), qb));
+                            sa = csub(sa, cmul(cconj(cmul(BB(a, b), Xv(id3(F2(i, b), g, F2(a, i))))), qb));
+                        }
+                    for (int n = 0; n < N; ++n)
+                        for (int a = 0; a < N; ++a) {
+                            const double2 sv = S123b[n + i * N];
+                            sbb = csub(sbb, cmul(cconj(cmul(cmul(BB(n, a), AA(a, b)), Mv(n, i, a, b))), sv));
+                            sbb = cadd(sbb, cmul(cconj(cmul(cmul(AA(n, a), BB(a, b)), Iv(n, i, a, b))), sv));
+                            sa = csub(sa, cmul(cconj(cmul(cmul(BB(n, a), BB(a, b)), Kv(n, i, a, b))), sv));
+                        }
+                    AbA[b + i * N] = cadd(AbA[b + i * N], sa);
+                    AbB[b + i * N] = cadd(AbB[b + i * N], sbb);
+                }
+                __syncthreads();
+                // operator entries at (n,a)
+                for (int e = threadIdx.x; e < N2; e += kBlock) {
+                    const int n = e % N, a = e / N;
+                    double2 sa = make_double2(0.0, 0.0), sbb = sa;
+                    for (int i = 0; i < N; ++i)
+                        for (int b = 0; b < N; ++b) {
+                            const double2 sv = S123b[n + i * N];
+                            sbb = csub(sbb, cmul(cconj(cmul(cmul(AA(a, b), BB(b, i)), Mv(n, i, a, b))), sv));
+                            sa = cadd(sa, cmul(cconj(cmul(cmul(BB(a, b), BB(b, i)), Iv(n, i, a, b))), sv));
+                            sbb = csub(sbb, cmul(cconj(cmul(cmul(BB(a, b), AA(b, i)), Kv(n, i, a, b))), sv));
+                        }
+                    AbA[n + a * N] = cadd(AbA[n + a * N], sa);
+                    AbB[n + a * N] = cadd(AbB[n + a * N], sbb);
                 }
                 __syncthreads();
 #undef AA

@@ -288,28 +288,13 @@ __device__ __forceinline__ void xcd_map(int bid, int bpj, int& job, int& gb) {
     }
 }
 
-// Job-fastest variant for phase B of the triples: each XCD keeps a fixed
-// eighth of the group blocks as above, and walks all jobs of a group block
-// before the next, so the workgroups it runs at one time share a group
-// block; jobs are visited grouped by j, so they read the same spectrum
-// tiles (of g_mirror(j) and conj g_j) and twiddles, from its L2.
-__device__ __forceinline__ void xcd_map_job_fast(int bid, int bpj, int njobs, int& job, int& gb) {
-    if ((bpj & 7) == 0) {
-        const int x = bid & 7, i = bid >> 3;
-        gb = x + 8 * (i / njobs);
-        job = i % njobs;
-    } else {
-        gb = bid / njobs;
-        job = bid % njobs;
-    }
-}
-
 struct FcrArgs {
     const double2* gamma;   // [nf][Nt]
     const double* omegas;   // [nf]
     const int* mirror;      // [nf]
     int nf;
-    long Nt;
+    long Nt;                // Gamma column stride
+    long Nin;               // Gamma rows that enter the FFT inputs (Nt, or T in the window mode)
     double dt;
     int L, a, b;
     long n;
@@ -341,7 +326,6 @@ struct FcrArgs {
     const double2* ph_tab;
     const int* job_row;
     long ph_len;
-    int b_job_fast;         // phase B of the triples: job-fastest workgroup order
 };
 
 // exp(+i Omega (t dt)) of a triple: its table row, or the correctly rounded value.
@@ -373,7 +357,7 @@ __global__ __launch_bounds__(Cfg::kFT) void k_phase_a_spec(FcrArgs P) {
         const int grp = Tl.grp_of(e), pos = Tl.pos_of(e);
         const long t = static_cast<long>(r0b + grp) + (static_cast<long>(pos) << P.b);
         double2 x = make_double2(0.0, 0.0);
-        if (t < P.Nt) {
+        if (t < P.Nin) {
             x = src[t];
             if (conj_in) x.y = -x.y;
         }
@@ -421,7 +405,7 @@ __global__ __launch_bounds__(Cfg::kFT) void k_phase_a(FcrArgs P) {
         const int grp = Tl.grp_of(e), pos = Tl.pos_of(e);
         const long t = static_cast<long>(r0b + grp) + (static_cast<long>(pos) << P.b);
         ph[q] = make_double2(0.0, 0.0);
-        if (e < total && t < P.Nt) ph[q] = cconj(phase_plus(prow, t, omega, P.dt));
+        if (e < total && t < P.Nin) ph[q] = cconj(phase_plus(prow, t, omega, P.dt));
     }
     for (long aj = abeg; aj < aend; ++aj) {
         int i, j, k;
@@ -433,7 +417,7 @@ __global__ __launch_bounds__(Cfg::kFT) void k_phase_a(FcrArgs P) {
             if (e < total) {
                 const int grp = Tl.grp_of(e), pos = Tl.pos_of(e);
                 const long t = static_cast<long>(r0b + grp) + (static_cast<long>(pos) << P.b);
-                buf[Tl.at(grp, pos)] = (t < P.Nt) ? cmul(src[t], ph[q]) : make_double2(0.0, 0.0);
+                buf[Tl.at(grp, pos)] = (t < P.Nin) ? cmul(src[t], ph[q]) : make_double2(0.0, 0.0);
             }
         }
         __syncthreads();
@@ -458,10 +442,7 @@ __global__ __launch_bounds__(Cfg::kFT) TACO_FFT_OCC(Cfg) void k_phase_b(FcrArgs 
     const TileLayout Tl(P.b, tile_groups<Cfg>(P.b, ngroups));
     const int G = Tl.G, gs = Tl.gs;
     int job, gb;
-    if (mode == 0 && P.b_job_fast)
-        xcd_map_job_fast(blockIdx.x, ngroups / G, P.na, job, gb);
-    else
-        xcd_map(blockIdx.x, ngroups / G, job, gb);
+    xcd_map(blockIdx.x, ngroups / G, job, gb);
     if (mode == 0) job = P.bperm[P.a0 + job] - static_cast<int>(P.a0);  // A-job within the chunk
     const int k1b = gb * G;
     const int total = G * gs;
@@ -630,66 +611,127 @@ struct CombineArgs {
 };
 
 // F/C/R assembly for t < T (tcl4_kernels.cpp:105-175): prefix integrals by
-// block scans carried across 256-sample chunks, combined with the FFT
-// convolution terms of phase C.
+// block scans carried across 256-sample blocks, combined with the FFT
+// convolution terms of phase C.  combine_block handles one block of one
+// triple: the eight scanned quantities (F: g2m pp; C: conj(g2) pp; R: g2 pm
+// and g1 g2 pm), their block totals, and (WRITE) the outputs from the
+// carried prefix.
+struct CombineTriple {
+    long cpos, tri;
+    int mask;
+    bool has_conv;
+    const double2 *g1, *g2, *g2m, *convF, *convC, *prow;
+    double omega;
+};
+
+__device__ __forceinline__ CombineTriple combine_triple(const FcrArgs& P, int slot) {
+    CombineTriple c;
+    c.cpos = P.comb0 + slot;
+    c.tri = P.job_tri[c.cpos];
+    c.mask = P.job_mask[c.cpos];
+    c.has_conv = c.cpos < P.nfft_jobs;
+    int i, j, k;
+    decode_triple(c.tri, P.nf, i, j, k);
+    c.omega = (P.omegas[i] + P.omegas[j]) + P.omegas[k];
+    c.prow = P.ph_tab ? P.ph_tab + static_cast<long>(P.job_row[c.cpos]) * P.ph_len : nullptr;
+    c.g1 = P.gamma + static_cast<long>(i) * P.Nt;
+    c.g2 = P.gamma + static_cast<long>(j) * P.Nt;
+    c.g2m = P.gamma + static_cast<long>(P.mirror[j]) * P.Nt;
+    c.convF = P.conv + static_cast<long>(slot) * P.T;
+    c.convC = P.conv + (P.cconv + slot) * P.T;
+    return c;
+}
+
+template <bool WRITE>
+__device__ __forceinline__ void combine_block(const FcrArgs& P, const CombineArgs& O, const CombineTriple& c, long t0,
+                                              const double (&carry)[8], double (&tot)[8], double (*wsum)[4]) {
+    const long t = t0 + threadIdx.x;
+    const bool valid = t < P.T;
+    double2 a1 = make_double2(0, 0), a2 = a1, a3 = a1, a4 = a1, pm = a1, gg1 = a1;
+    if (valid) {
+        const double2 pp = phase_plus(c.prow, t, c.omega, P.dt);
+        pm = cconj(pp);
+        gg1 = c.g1[t];
+        const double2 gg2 = c.g2[t];
+        a1 = cmul(c.g2m[t], pp);        // F: g2m * phase_plus
+        a2 = cmul(cconj(gg2), pp);      // C: conj(g2) * phase_plus
+        a3 = cmul(gg2, pm);             // R: g2 * phase_minus
+        a4 = cmul(cmul(gg1, gg2), pm);  // R: (g1 g2) * phase_minus
+    }
+    double v[8] = {a1.x, a1.y, a2.x, a2.y, a3.x, a3.y, a4.x, a4.y};
+    block_scan_n<8>(v, wsum, tot);
+    if constexpr (!WRITE) return;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = v[q] + carry[q];
+    if (!valid) return;
+    long dst = -1;
+    if (O.out_mode == 0) {
+        const int pos = O.t2pos[t];
+        if (pos < 0) return;
+        dst = static_cast<long>(pos) * O.nf3 + c.tri;
+    } else {
+        dst = (c.tri - O.tri0) * P.T + t;
+    }
+    const double dt = P.dt;
+    const double2 pF = make_double2(v[0] * dt, v[1] * dt);
+    const double2 pC = make_double2(v[2] * dt, v[3] * dt);
+    const double2 pR = make_double2(v[4] * dt, v[5] * dt);
+    const double2 pP = make_double2(v[6] * dt, v[7] * dt);
+    const double2 g1p = cmul(gg1, pm);
+    if (c.has_conv && (c.mask & 1)) O.Fo[dst] = csub(cmul(g1p, pF), c.convF[t]);
+    if (c.has_conv && (c.mask & 2)) O.Co[dst] = csub(cmul(g1p, pC), c.convC[t]);
+    if (c.mask & 4) O.Ro[dst] = csub(cmul(gg1, pR), pP);
+}
+
+// One workgroup per triple walks the blocks t0 = 0, 256, ... with the carry.
 __global__ __launch_bounds__(kBlock) void k_combine(FcrArgs P, CombineArgs O) {
     __shared__ double wsum[8][4];
-    const int slot = blockIdx.x;
-    const long cpos = P.comb0 + slot;
-    const long tri = P.job_tri[cpos];
-    const int mask = P.job_mask[cpos];
-    const bool has_conv = cpos < P.nfft_jobs;
-    int i, j, k;
-    decode_triple(tri, P.nf, i, j, k);
-    const double omega = (P.omegas[i] + P.omegas[j]) + P.omegas[k];
-    const double2* prow = P.ph_tab ? P.ph_tab + static_cast<long>(P.job_row[cpos]) * P.ph_len : nullptr;
-    const double2* g1 = P.gamma + static_cast<long>(i) * P.Nt;
-    const double2* g2 = P.gamma + static_cast<long>(j) * P.Nt;
-    const double2* g2m = P.gamma + static_cast<long>(P.mirror[j]) * P.Nt;
-    const double2* convF = P.conv + static_cast<long>(slot) * P.T;
-    const double2* convC = P.conv + (P.cconv + slot) * P.T;
+    const CombineTriple c = combine_triple(P, static_cast<int>(blockIdx.x));
     double carry[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (long t0 = 0; t0 < P.T; t0 += kBlock) {
-        const long t = t0 + threadIdx.x;
-        const bool valid = t < P.T;
-        double2 a1 = make_double2(0, 0), a2 = a1, a3 = a1, a4 = a1, pm = a1, gg1 = a1;
-        if (valid) {
-            const double2 pp = phase_plus(prow, t, omega, P.dt);
-            pm = cconj(pp);
-            gg1 = g1[t];
-            const double2 gg2 = g2[t];
-            a1 = cmul(g2m[t], pp);           // F: g2m * phase_plus
-            a2 = cmul(cconj(gg2), pp);       // C: conj(g2) * phase_plus
-            a3 = cmul(gg2, pm);              // R: g2 * phase_minus
-            a4 = cmul(cmul(gg1, gg2), pm);   // R: (g1 g2) * phase_minus
-        }
-        double v[8] = {a1.x, a1.y, a2.x, a2.y, a3.x, a3.y, a4.x, a4.y};
         double tot[8];
-        block_scan_n<8>(v, wsum, tot);
+        combine_block<true>(P, O, c, t0, carry, tot, wsum);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            v[q] = v[q] + carry[q];
-            carry[q] += tot[q];
-        }
-        if (!valid) continue;
-        long dst = -1;
-        if (O.out_mode == 0) {
-            const int pos = O.t2pos[t];
-            if (pos < 0) continue;
-            dst = static_cast<long>(pos) * O.nf3 + tri;
-        } else {
-            dst = (tri - O.tri0) * P.T + t;
-        }
-        const double dt = P.dt;
-        const double2 pF = make_double2(v[0] * dt, v[1] * dt);
-        const double2 pC = make_double2(v[2] * dt, v[3] * dt);
-        const double2 pR = make_double2(v[4] * dt, v[5] * dt);
-        const double2 pP = make_double2(v[6] * dt, v[7] * dt);
-        const double2 g1p = cmul(gg1, pm);
-        if (has_conv && (mask & 1)) O.Fo[dst] = csub(cmul(g1p, pF), convF[t]);
-        if (has_conv && (mask & 2)) O.Co[dst] = csub(cmul(g1p, pC), convC[t]);
-        if (mask & 4) O.Ro[dst] = csub(cmul(gg1, pR), pP);
+        for (int q = 0; q < 8; ++q) carry[q] += tot[q];
     }
+}
+
+// The same in three launches when the triples are too few to fill the chip:
+// block totals (a workgroup per (triple, block)), the carries into every
+// block by the sequential sum above (a thread per (triple, quantity)), then
+// every block with its carry.  Each value is formed by the same operations
+// in the same order, so the outputs equal k_combine's bit for bit.
+__global__ __launch_bounds__(kBlock) void k_combine_tot(FcrArgs P, CombineArgs O, double* tot_out, int nblk) {
+    __shared__ double wsum[8][4];
+    const CombineTriple c = combine_triple(P, static_cast<int>(blockIdx.x));
+    const double carry[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double tot[8];
+    combine_block<false>(P, O, c, static_cast<long>(blockIdx.y) * kBlock, carry, tot, wsum);
+    if (threadIdx.x < 8) tot_out[(static_cast<long>(blockIdx.x) * nblk + blockIdx.y) * 8 + threadIdx.x] = tot[threadIdx.x];
+}
+
+__global__ void k_combine_carry(double* tot_carry, long cb, int nblk) {
+    const long e = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;  // (slot, quantity)
+    if (e >= cb * 8) return;
+    const long slot = e / 8, q = e % 8;
+    double* p = tot_carry + slot * nblk * 8 + q;
+    double carry = 0.0;
+    for (int b = 0; b < nblk; ++b) {
+        const double tb = p[static_cast<long>(b) * 8];
+        p[static_cast<long>(b) * 8] = carry;
+        carry += tb;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_combine_blocks(FcrArgs P, CombineArgs O, const double* carry_in, int nblk) {
+    __shared__ double wsum[8][4];
+    const CombineTriple c = combine_triple(P, static_cast<int>(blockIdx.x));
+    double carry[8];
+    const double* ci = carry_in + (static_cast<long>(blockIdx.x) * nblk + blockIdx.y) * 8;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) carry[q] = ci[q];
+    double tot[8];
+    combine_block<true>(P, O, c, static_cast<long>(blockIdx.y) * kBlock, carry, tot, wsum);
 }
 
 __global__ void k_cr_sincos(const double* x, long n, double* s, double* c) {
@@ -1912,13 +1954,13 @@ struct PhaseTab {
 // context, and the next tape on the context takes them over, so repeated
 // gradients run without device allocations once the sizes have settled.
 struct TapeBufs {
-    DevBuf gamma, omegas, mirror, p2f, ops, eps, t2pos, jobtri, jobmask, clsstart, jobrow, slices, lgen, ck;
+    DevBuf gamma, omegas, mirror, p2f, ops, eps, t2pos, jobtri, jobmask, clsstart, jobrow, slices, lgen, ck, adjl;
     PhaseTab phtab;
     void swap(TapeBufs& o) {
         for (auto [a, b] : {std::pair{&gamma, &o.gamma}, {&omegas, &o.omegas}, {&mirror, &o.mirror}, {&p2f, &o.p2f},
                             {&ops, &o.ops}, {&eps, &o.eps}, {&t2pos, &o.t2pos}, {&jobtri, &o.jobtri},
                             {&jobmask, &o.jobmask}, {&clsstart, &o.clsstart}, {&jobrow, &o.jobrow},
-                            {&slices, &o.slices}, {&lgen, &o.lgen}, {&ck, &o.ck}})
+                            {&slices, &o.slices}, {&lgen, &o.lgen}, {&ck, &o.ck}, {&adjl, &o.adjl}})
             a->swap(*b);
         phtab.swap(o.phtab);
     }
@@ -1990,7 +2032,7 @@ struct taco_ctx {
     std::map<int, std::array<int, 3>> fft_choice;  // per FFT log-length: tile config of phases A, B, C
     std::map<int, std::unique_ptr<taco_engine::Twiddles>> tw;
     taco_engine::DevBuf gamma, omegas, mirror, p2f, ops, spec, inter1, inter2, conv, slices, t2pos, l4, gw, misc,
-        rk, eps, jobtri, clsstart, jobmask, jobrow;
+        rk, eps, jobtri, clsstart, jobmask, jobrow, comb;
     taco_engine::PhaseTab phtab;  // phase table of the last plain-path system (TACO_PHASE_LIBM)
     std::size_t l4_lds_set = 0;  // dynamic LDS limit set on the fused L4 kernel
     std::vector<std::unique_ptr<taco_engine::TapeBufs>> tape_pool;  // buffers of destroyed tapes (at most 2)
@@ -2022,13 +2064,19 @@ static void timed(taco_ctx* c, int id, double bytes, Fn&& launch) {
     c->prof.pending.push_back({id, a, b, bytes});
 }
 
-static const Twiddles& twiddles_for(taco_ctx* c, int L) {
-    auto it = c->tw.find(L);
+// Twiddle tables of length 2^L: the reference's recurrence values, or (exact)
+// each entry rounded once from quad precision (the window mode).
+static const Twiddles& twiddles_for(taco_ctx* c, int L, bool exact = false) {
+    const int key = L + (exact ? 64 : 0);
+    auto it = c->tw.find(key);
     if (it != c->tw.end()) return *it->second;
     auto t = std::make_unique<Twiddles>();
     t->L = L;
     std::vector<double> tab;
-    make_twiddles(L, tab);  // stage-major standard table, 2^L - 1 entries
+    if (exact)
+        make_twiddles_exact(L, tab);
+    else
+        make_twiddles(L, tab);  // stage-major standard table, 2^L - 1 entries
     const int a = (L + 1) / 2, b = L - a;
     const std::size_t nstd = (std::size_t{1} << std::max(a, b)) - 1;
     // Phase B forward stages s = a+1..L for group k1 < 2^a: T_s[k1 + v 2^a], v < 2^(s-1-a).
@@ -2064,7 +2112,7 @@ static const Twiddles& twiddles_for(taco_ctx* c, int L) {
     t->c_.ensure(tc.size() * 8);
     HIP_OK(hipMemcpy(t->c_.p, tc.data(), tc.size() * 8, hipMemcpyHostToDevice));
     auto& ref = *t;
-    c->tw.emplace(L, std::move(t));
+    c->tw.emplace(key, std::move(t));
     return ref;
 }
 
@@ -2321,13 +2369,30 @@ static PhaseRows phase_rows(taco_ctx* c, const FcrPlan& pl, const std::vector<in
     return out;
 }
 
+// A combine batch of cb triples runs split over its 256-sample blocks (three
+// launches) when the triples alone would leave most of the chip idle; the
+// block totals and carries take cb * nblk * 8 doubles in the context's
+// `comb` buffer.
+// TACO_COMBINE_SPLIT=0 / 1 forces the single-launch / split form where it fits.
+static long combine_split_blocks(long cb, std::size_t T) {
+    const long nblk = static_cast<long>((T + kBlock - 1) / kBlock);
+    if (nblk < 2 || cb * nblk > (1L << 20)) return 0;
+    const int force = env_int("TACO_COMBINE_SPLIT", -1);
+    if (force >= 0) return force ? nblk : 0;
+    return cb < 512 ? nblk : 0;
+}
+
 static FcrJob fcr_prepare(taco_ctx* c, const FcrPlan& pl, const FcrLists& lists) {
-    const std::size_t n = fcr_nfft(pl.Nt, g_pad_factor);
+    // window mode: only the rows t < T enter the convolutions (the same causal
+    // sums at t < T), with exact twiddles
+    const bool window = fft_mode() == TACO_FFT_WINDOW;
+    const std::size_t Nin = window ? std::max<std::size_t>(pl.T, 1) : pl.Nt;
+    const std::size_t n = fcr_nfft(Nin, g_pad_factor);
     int L = 0;
     while ((std::size_t{1} << L) < n) ++L;
     if (L > 22) throw Error(TACO_ERR_INVALID, "FFT length above 2^22 (Nt above 2^21) is not supported");
     const int a = (L + 1) / 2, b = L - a;
-    const Twiddles& tw = twiddles_for(c, L);
+    const Twiddles& tw = twiddles_for(c, L, window);
     const int nf = pl.nf;
     const long nf3 = static_cast<long>(nf) * nf * nf;
 
@@ -2448,6 +2513,9 @@ static FcrJob fcr_prepare(taco_ctx* c, const FcrPlan& pl, const FcrLists& lists)
     c->inter1.ensure(static_cast<std::size_t>(std::max(chunk, spec_jobs)) * n * 16);
     c->inter2.ensure(static_cast<std::size_t>(2 * chunk) * n * 16);
     c->conv.ensure(static_cast<std::size_t>(2 * cconv) * Tpad * 16);
+    for (const FcrJob::Batch& bt : batches)
+        if (const long nblk = combine_split_blocks(bt.cb, pl.T))
+            c->comb.ensure(static_cast<std::size_t>(bt.cb * nblk) * 8 * sizeof(double));
     FcrJob J;
     FcrArgs& P = J.P;
     P.gamma = pl.gamma;
@@ -2455,6 +2523,7 @@ static FcrJob fcr_prepare(taco_ctx* c, const FcrPlan& pl, const FcrLists& lists)
     P.mirror = pl.mirror;
     P.nf = nf;
     P.Nt = static_cast<long>(pl.Nt);
+    P.Nin = static_cast<long>(Nin);
     P.dt = pl.dt;
     P.L = L;
     P.a = a;
@@ -2474,7 +2543,6 @@ static FcrJob fcr_prepare(taco_ctx* c, const FcrPlan& pl, const FcrLists& lists)
     P.ph_tab = prow.tab;
     P.job_row = prow.rows;
     P.ph_len = prow.len;
-    P.b_job_fast = env_int("TACO_PHASEB_JOB_FAST", 0);
     J.cls_start = std::move(cls_start);
     J.ajs = std::move(ajs);
     J.chunk_a = std::move(chunk_a);
@@ -2565,8 +2633,19 @@ static void fcr_launch(taco_ctx* c, FcrJob& J, const CombineArgs& out) {
         const double kept = out.out_mode == 0 ? static_cast<double>(out.npos) : static_cast<double>(J.T);
         const double cpaths = static_cast<double>(J.paths[comb0 + cb] - J.paths[comb0]);
         const double couts = static_cast<double>(J.outs[comb0 + cb] - J.outs[comb0]);
+        const long nblk = combine_split_blocks(cb, J.T);
         timed(c, K_COMBINE, cb * 3 * cT + cpaths * cT + couts * 16.0 * kept, [&] {
-            hipLaunchKernelGGL(k_combine, dim3(static_cast<unsigned>(cb)), dim3(kBlock), 0, c->stream, P, out);
+            if (!nblk) {
+                hipLaunchKernelGGL(k_combine, dim3(static_cast<unsigned>(cb)), dim3(kBlock), 0, c->stream, P, out);
+                return;
+            }
+            double* tc = c->comb.as<double>();
+            const dim3 grid(static_cast<unsigned>(cb), static_cast<unsigned>(nblk));
+            hipLaunchKernelGGL(k_combine_tot, grid, dim3(kBlock), 0, c->stream, P, out, tc, static_cast<int>(nblk));
+            hipLaunchKernelGGL(k_combine_carry, dim3(static_cast<unsigned>((cb * 8 + 63) / 64)), dim3(64), 0, c->stream,
+                               tc, cb, static_cast<int>(nblk));
+            hipLaunchKernelGGL(k_combine_blocks, grid, dim3(kBlock), 0, c->stream, P, out,
+                               static_cast<const double*>(tc), static_cast<int>(nblk));
         });
         HIP_OK(hipGetLastError());
     }
@@ -3055,6 +3134,56 @@ static void tape_forward(taco_tape& tp, const double* r0, double* samples_out) {
     HIP_OK(hipStreamSynchronize(c->stream));
 }
 
+// Term lists of the factorised L4 reverse (gathers instead of scatter-adds):
+// every X = C + R slice pair (x, z) = (f(b,a), f(a,i)) or (f(i,b), f(a,i))
+// with its (a, i, b, form) terms; every F slice index with its M and I terms
+// (n, i, a, b, kind) and every R slice index with its M and K terms, in a
+// fixed order.  Packed 8 bits per index (N <= 128).
+struct L4AdjLists {
+    std::vector<int> xp_xz, xp_start, xp_ent, f_tgt, f_start, f_ent, r_tgt, r_start, r_ent;
+};
+
+static L4AdjLists l4_adjoint_lists(const int* p2f, int N, int nf) {
+    if (N > 128) throw Error(TACO_ERR_INVALID, "factorised L4 reverse supports N <= 128");
+    auto f = [&](int r, int c) { return p2f[r + c * N]; };
+    const long nfl = nf;
+    auto id3 = [nfl](long x, long y, long z) { return (x * nfl + y) * nfl + z; };
+    auto csr = [](std::map<long, std::vector<int>>& m, std::vector<int>* keys, std::vector<int>& start,
+                  std::vector<int>& ent, bool pair_key) {
+        start.assign(1, 0);
+        for (auto& kv : m) {
+            keys->push_back(pair_key ? static_cast<int>(kv.first) : static_cast<int>(kv.first));
+            ent.insert(ent.end(), kv.second.begin(), kv.second.end());
+            start.push_back(static_cast<int>(ent.size()));
+        }
+    };
+    L4AdjLists L;
+    std::map<long, std::vector<int>> xm, fm, rm;
+    for (int a = 0; a < N; ++a)
+        for (int i = 0; i < N; ++i)
+            for (int b = 0; b < N; ++b) {
+                const int w = a | (i << 8) | (b << 16);
+                xm[f(b, a) + 65536L * f(a, i)].push_back(w);
+                xm[f(i, b) + 65536L * f(a, i)].push_back(w | (1 << 24));
+            }
+    for (int n = 0; n < N; ++n)
+        for (int i = 0; i < N; ++i)
+            for (int a = 0; a < N; ++a)
+                for (int b = 0; b < N; ++b) {
+                    const unsigned w = static_cast<unsigned>(n) | (static_cast<unsigned>(i) << 8) |
+                                       (static_cast<unsigned>(a) << 16) | (static_cast<unsigned>(b) << 24);
+                    const int w0 = static_cast<int>(w), w1 = static_cast<int>(w | (1u << 31));
+                    fm[id3(f(b, a), f(b, i), f(n, b))].push_back(w0);  // M: F part
+                    fm[id3(f(a, n), f(b, i), f(n, b))].push_back(w1);  // I
+                    rm[id3(f(b, a), f(n, a), f(a, i))].push_back(w0);  // M: R part
+                    rm[id3(f(i, b), f(n, a), f(a, i))].push_back(w1);  // K
+                }
+    csr(xm, &L.xp_xz, L.xp_start, L.xp_ent, true);
+    csr(fm, &L.f_tgt, L.f_start, L.f_ent, false);
+    csr(rm, &L.r_tgt, L.r_start, L.r_ent, false);
+    return L;
+}
+
 static void tape_adjoint(taco_tape& tp, const double* samples_bar, double* gamma_bar, double* omega_bar,
                          double* ops_bar, double* eps_bar, double* r0_bar) {
     taco_ctx* c = tp.ctx;
@@ -3219,6 +3348,34 @@ static void tape_adjoint(taco_tape& tp, const double* samples_bar, double* gamma
             Af.lds_tb = lds_tb ? 1 : 0;
             Af.scr = l4s;
             Af.scr_per = static_cast<long>(l4_per);
+            const L4AdjLists LL = l4_adjoint_lists(tp.h_p2f.data(), tp.N, tp.nf);
+            const std::vector<int>* parts_l[9] = {&LL.xp_xz, &LL.xp_start, &LL.xp_ent, &LL.f_tgt, &LL.f_start,
+                                                  &LL.f_ent, &LL.r_tgt, &LL.r_start, &LL.r_ent};
+            std::size_t tot = 0;
+            for (auto* v : parts_l) tot += v->size();
+            tp.bufs.adjl.ensure(std::max<std::size_t>(tot, 1) * sizeof(int));
+            int* dl = tp.bufs.adjl.as<int>();
+            const int* dp[9];
+            for (int q = 0; q < 9; ++q) {
+                dp[q] = dl;
+                if (!parts_l[q]->empty())
+                    HIP_OK(hipMemcpyAsync(dl, parts_l[q]->data(), parts_l[q]->size() * sizeof(int),
+                                          hipMemcpyHostToDevice, c->stream));
+                dl += parts_l[q]->size();
+            }
+            Af.xp_xz = dp[0];
+            Af.xp_start = dp[1];
+            Af.xp_ent = dp[2];
+            Af.nxp = static_cast<int>(LL.xp_xz.size());
+            Af.f_tgt = dp[3];
+            Af.f_start = dp[4];
+            Af.f_ent = dp[5];
+            Af.nft = static_cast<int>(LL.f_tgt.size());
+            Af.r_tgt = dp[6];
+            Af.r_start = dp[7];
+            Af.r_ent = dp[8];
+            Af.nrt = static_cast<int>(LL.r_tgt.size());
+            HIP_OK(hipStreamSynchronize(c->stream));  // the host lists go out of scope
             set_fn_lds(reinterpret_cast<const void*>(k_l4f_adjoint), lds);
             timed(c, K_L4, 0.0, [&] {
                 hipLaunchKernelGGL(k_l4f_adjoint, dim3(l4_grid), dim3(kBlock), lds, c->stream, Af);
@@ -3413,6 +3570,7 @@ static void bcf_device(taco_ctx* c, std::size_t N, double dt, int kind, double p
     FcrArgs P{};
     P.gamma = B.S;
     P.Nt = static_cast<long>(Nfft);
+    P.Nin = static_cast<long>(Nfft);
     P.L = L;
     P.a = (L + 1) / 2;
     P.b = L - P.a;
@@ -3470,6 +3628,11 @@ int taco_set_phase_mode(int mode) {
     return guarded([&] { set_phase_mode(mode); });
 }
 int taco_get_phase_mode(void) { return phase_mode(); }
+
+int taco_set_fft_mode(int mode) {
+    return guarded([&] { set_fft_mode(mode); });
+}
+int taco_get_fft_mode(void) { return fft_mode(); }
 
 int taco_ctx_create(int device, taco_ctx** out) {
     return guarded([&] {

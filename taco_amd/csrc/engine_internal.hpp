@@ -20,6 +20,12 @@ struct Error : std::runtime_error {
 std::size_t next_pow2(std::size_t n);
 std::size_t fcr_nfft(std::size_t Nt, std::size_t pad);
 void make_twiddles(int L, std::vector<double>& out);
+// The same table with every entry exp(-2 pi i k / 2^s) rounded once from a
+// quad-precision evaluation (the window mode's accurate twiddles).
+void make_twiddles_exact(int L, std::vector<double>& out);
+// F/C/R FFT mode (process-global): TACO_FFT_PARITY or TACO_FFT_WINDOW.
+int fft_mode();
+void set_fft_mode(int mode);
 void bcf_fft(std::size_t N, double dt, int kind, double p0, double p1, double beta, double* C_out);
 void gamma_trapz(const double* C, std::size_t N, double dt, const double* omegas, int M, double* G);
 void mirror_indices(const double* omegas, int nf, double fidx_tol, int* mirror);

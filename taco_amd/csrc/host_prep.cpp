@@ -106,6 +106,29 @@ static void fft_host(std::vector<cd>& a, bool forward) {
     }
 }
 
+void make_twiddles_exact(int L, std::vector<double>& out) {
+    const std::size_t total = (std::size_t{1} << L) - 1;
+    out.assign(2 * total, 0.0);
+    const __float128 two_pi = 2 * acosq(static_cast<__float128>(-1));
+    for (int s = 1; s <= L; ++s) {
+        const std::size_t half = std::size_t{1} << (s - 1), off = half - 1;
+        for (std::size_t k = 0; k < half; ++k) {
+            const __float128 ang = -two_pi * static_cast<__float128>(k) / static_cast<__float128>(2 * half);
+            out[2 * (off + k)] = static_cast<double>(cosq(ang));
+            out[2 * (off + k) + 1] = static_cast<double>(sinq(ang));
+        }
+    }
+}
+
+static int g_fft_mode = TACO_FFT_PARITY;
+
+int fft_mode() { return g_fft_mode; }
+
+void set_fft_mode(int mode) {
+    if (mode != TACO_FFT_PARITY && mode != TACO_FFT_WINDOW) throw Error(TACO_ERR_INVALID, "unknown FFT mode");
+    g_fft_mode = mode;
+}
+
 static int g_phase_mode = TACO_PHASE_LIBM;
 
 int phase_mode() { return g_phase_mode; }

@@ -42,6 +42,7 @@ __all__ = [
     "propagate_rk4_dense",
     "propagate_rk4_dense_serial", "propagate_rk4_dense_omp", "tcl4_propagate", "set_fcr_fft_pad_factor",
     "get_fcr_fft_pad_factor", "set_phase_mode", "get_phase_mode", "phase_mode", "PHASE_LIBM", "PHASE_CR",
+    "set_fft_mode", "get_fft_mode", "fft_mode", "FFT_PARITY", "FFT_WINDOW",
     "context", "version", "BatchItem", "Batch", "batched_run",
 ]
 
@@ -98,6 +99,37 @@ def set_phase_mode(mode: int) -> None:
 
 def get_phase_mode() -> int:
     return int(lib().taco_get_phase_mode())
+
+
+FFT_PARITY = 0  # the reference's recurrence-twiddle FFT over the whole Gamma table (default)
+FFT_WINDOW = 1  # only the rows t < T enter; Nfft from T; exact twiddles (no reference counterpart)
+
+
+def set_fft_mode(mode: int) -> None:
+    """Process-global F/C/R FFT mode (include/taco_tcl4.h): FFT_PARITY
+    reproduces the reference's spectra bit for bit; FFT_WINDOW computes the
+    same causal convolutions at t < T from the rows t < T only, with an FFT of
+    next_pow2(2T - 1) and exact twiddles."""
+    check(lib().taco_set_fft_mode(int(mode)))
+
+
+def get_fft_mode() -> int:
+    return int(lib().taco_get_fft_mode())
+
+
+class fft_mode:
+    """Context manager that switches the FFT mode and restores it."""
+
+    def __init__(self, mode: int):
+        self.mode = mode
+
+    def __enter__(self):
+        self.prev = get_fft_mode()
+        set_fft_mode(self.mode)
+        return self
+
+    def __exit__(self, *exc):
+        set_fft_mode(self.prev)
 
 
 class phase_mode:

@@ -391,3 +391,63 @@ def test_l4_tfim_d16_matches_oracle(orc, h):
     S = w.system
     assert S.N == 16 and S.nf == (57 if h == 1.0 else 81)
     l4_check(orc, w, [0, 5, 19])
+
+
+@pytest.mark.parametrize("wl", ["spin_boson", "gue_pair"])
+def test_window_mode_matches_direct_method(orc, wl):
+    """TACO_FFT_WINDOW (rows t < T only, Nfft = next_pow2(2T - 1), exact
+    twiddles) against the reference's own Direct O(T^2) method
+    (tcl4_kernels.cpp:188-281, tcl4_tests.cpp:188-212), which has no FFT
+    rounding: per-t relative error <= 1e-10 from t = 20 on and <= 1e-8 at the
+    first steps (where L4 is a small difference of FFT-rounded terms); the
+    parity mode reproduces the reference's recurrence-twiddle FFT, whose
+    early-t error against Direct is orders of magnitude larger (1e-4 for the
+    spin-boson window)."""
+    w = W.spin_boson(Nt=2 ** 14, tmax=300, tstep=1) if wl == "spin_boson" else W.gue_pair(N_bath=20000, tmax=150)
+    S, G = w.system, w.gamma
+    ref = orc.l4_series(G, S.omegas, S.mirror, S.pair_to_freq, S.A_eig, w.dt, w.tidx, method="direct")
+    with taco_amd.fft_mode(taco_amd.FFT_WINDOW):
+        win = taco_amd.build_TCL4_generator_hip_fused_batch(S, G, w.dt, w.tidx)
+    par = taco_amd.build_TCL4_generator_hip_fused_batch(S, G, w.dt, w.tidx)
+    e_win, _ = max_rel_err(win, ref)
+    e_par, _ = max_rel_err(par, ref)
+    pt_win, _ = per_t_rel_err(win, ref)
+    print(f"{wl}: vs Direct: window {e_win:.3e} (t >= 20: {pt_win[20:].max():.3e}), parity {e_par:.3e}")
+    assert e_win <= 1e-8 and pt_win[20:].max() <= 1e-10
+    assert e_win <= e_par
+    assert taco_amd.get_fft_mode() == taco_amd.FFT_PARITY
+
+
+def test_window_mode_batch_and_tape_equal_plain(orc):
+    """The window mode applies to every F/C/R pass: the batch equals the plain
+    builder bit for bit in it, and TCL4 propagation through it stays within
+    the bar of the parity-mode oracle."""
+    sb = W.spin_boson(Nt=2048, tmax=300, tstep=3)
+    r0 = np.zeros(4, complex)
+    r0[0] = 1.0
+    with taco_amd.fft_mode(taco_amd.FFT_WINDOW):
+        b = taco_amd.Batch([taco_amd.BatchItem(sb.system, sb.gamma, sb.dt, sb.tidx, steps=300, r0=r0)]).run()
+        plain = taco_amd.build_TCL4_generator_hip_fused_batch(sb.system, sb.gamma, sb.dt, sb.tidx)
+        rho = taco_amd.tcl4_propagate(sb.system, sb.gamma, sb.dt, 300, r0)
+        assert np.array_equal(b.l4(0), plain) and np.array_equal(b.rho(0), rho)
+        b.close()
+    S, G = sb.system, sb.gamma
+    rr = orc.tcl4_propagate(G, S.omegas, S.mirror, S.pair_to_freq, S.eps, S.A_eig, sb.dt, 300, r0)
+    assert max_rel_err(rho, rr)[0] <= 1e-10
+
+
+@pytest.mark.parametrize("wl", ["spin_boson", "gue_pair"])
+def test_combine_split_equals_single_launch(monkeypatch, wl):
+    """The F/C/R assembly split over 256-sample blocks (block totals, carries,
+    blocks) forms every value by the same operations in the same order as the
+    one-workgroup-per-triple walk: bit-identical slices, series and L4."""
+    w = W.spin_boson(Nt=3000, tmax=1500) if wl == "spin_boson" else W.gue_pair(N_bath=5000, tmax=700)
+    S, G = w.system, w.gamma
+    res = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("TACO_COMBINE_SPLIT", mode)
+        ser = taco_amd.compute_triple_kernels(S, G, w.dt, keep=1200)
+        L4 = taco_amd.build_TCL4_generator_hip_fused_batch(S, G, w.dt, w.tidx)
+        res[mode] = (ser.F, ser.C, ser.R, L4)
+    for a, b in zip(res["0"], res["1"]):
+        assert np.array_equal(a, b)
