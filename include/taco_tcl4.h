@@ -96,6 +96,17 @@ int taco_get_phase_mode(void);
 int taco_set_fft_mode(int mode);
 int taco_get_fft_mode(void);
 
+/* Bytes of F/C/R slices (3 nf^3 complex values per kept time) one pass of
+ * the batch builders (taco_l4_batch[_dev], taco_l4_single) may hold; 0
+ * restores the default (TACO_SLICE_BYTES, else 32 GiB).  A call whose kept
+ * times need more is run as windows of consecutive kept times, each an F/C/R
+ * pass up to its last time followed by the L4 of its times: the time
+ * decomposition of the reference's distributed builder
+ * (backend/cpu/tcl4_mpi_omp.cpp:62-156), with bit-identical results and the
+ * FFT work repeated per window.  Per context. */
+int taco_set_slice_budget(taco_ctx* ctx, size_t bytes);
+size_t taco_get_slice_budget(const taco_ctx* ctx);
+
 /* Per-kernel HIP-event profiler of a context (off by default).  Kernel ids:
  * 0 phase A, 1 phase B, 2 phase C, 3 F/C/R combine, 4 fused L4,
  * 5 per-bucket spectra, 6 TCL2, 7 RK4.  taco_profile_get returns the summed

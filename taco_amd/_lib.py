@@ -55,6 +55,8 @@ SIGNATURES = {
     "taco_get_phase_mode": (c_int, []),
     "taco_set_fft_mode": (c_int, [c_int]),
     "taco_get_fft_mode": (c_int, []),
+    "taco_set_slice_budget": (c_int, [c_void_p, c_size_t]),
+    "taco_get_slice_budget": (c_size_t, [c_void_p]),
     "taco_system_build": (c_int, [_dp, c_int, _dp, c_int, c_double, _dp, _dp, _dp, _ip, _dp, _ip, _ip, _ip]),
     "taco_bcf_fft": (c_int, [c_size_t, c_double, c_int, c_double, c_double, c_double, _dp]),
     "taco_gamma_trapz": (c_int, [_dp, c_size_t, c_double, _dp, c_int, _dp]),

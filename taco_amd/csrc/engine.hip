@@ -2035,6 +2035,7 @@ struct taco_ctx {
         rk, eps, jobtri, clsstart, jobmask, jobrow, comb;
     taco_engine::PhaseTab phtab;  // phase table of the last plain-path system (TACO_PHASE_LIBM)
     std::size_t l4_lds_set = 0;  // dynamic LDS limit set on the fused L4 kernel
+    std::size_t slice_budget = 0;  // bytes of F/C/R slices per pass of the batch builders (0: default)
     std::vector<std::unique_ptr<taco_engine::TapeBufs>> tape_pool;  // buffers of destroyed tapes (at most 2)
 };
 
@@ -2229,6 +2230,7 @@ struct FcrPlan {
     double dt;
     std::size_t T;
     const std::vector<unsigned char>* need;  // per triple: bit0 F, bit1 C, bit2 R (null: all)
+    std::size_t T_fft = 0;  // window mode: rows entering the convolutions (0: T)
 };
 
 // Triples the L4 assembly reads (tcl4_mikx.cpp:101-132 gathers through the
@@ -2386,7 +2388,7 @@ static FcrJob fcr_prepare(taco_ctx* c, const FcrPlan& pl, const FcrLists& lists)
     // window mode: only the rows t < T enter the convolutions (the same causal
     // sums at t < T), with exact twiddles
     const bool window = fft_mode() == TACO_FFT_WINDOW;
-    const std::size_t Nin = window ? std::max<std::size_t>(pl.T, 1) : pl.Nt;
+    const std::size_t Nin = window ? std::max<std::size_t>(std::max(pl.T, pl.T_fft), 1) : pl.Nt;
     const std::size_t n = fcr_nfft(Nin, g_pad_factor);
     int L = 0;
     while ((std::size_t{1} << L) < n) ++L;
@@ -2785,8 +2787,29 @@ static TimePlan plan_times(const std::size_t* tidx, std::size_t ntau, std::size_
     return tp;
 }
 
+// Bytes of F/C/R slices one pass of the batch builders may hold:
+// taco_set_slice_budget, else TACO_SLICE_BYTES, else 32 GiB.
+static std::size_t slice_budget(const taco_ctx* c) {
+    if (c->slice_budget) return c->slice_budget;
+    static const std::size_t env = [] {
+        const char* v = std::getenv("TACO_SLICE_BYTES");
+        return v ? static_cast<std::size_t>(std::strtoull(v, nullptr, 10)) : std::size_t{0};
+    }();
+    return env ? env : (std::size_t{32} << 30);
+}
+
 // Core of the batch builders on device-resident inputs.  Writes L4 for the
 // unique times into c->l4 and returns the plan; fcr_ms optional.
+//
+// The slices of one pass hold 3 nf^3 values per kept time.  When all kept
+// times would exceed the slice budget, the kept times are cut into windows of
+// consecutive positions and each window is one F/C/R pass over t < (its last
+// time + 1) followed by the L4 of its times -- the time decomposition of the
+// reference's distributed builder (tcl4_mpi_omp.cpp:62-156, each rank slicing
+// its own times out of full series).  Every kept value is the same arithmetic
+// as in one pass: the convolutions, prefix scans and phases at time t do not
+// depend on how far the pass runs (window FFT mode keeps Nfft from the last
+// kept time of the whole call), so windowed results are bit-identical.
 static TimePlan l4_core(taco_ctx* c, const double2* d_gamma, std::size_t Nt, int nf, const double* d_omegas,
                         const double* h_omegas, const int* d_mirror, const int* d_p2f, const int* h_p2f, int N,
                         const double2* d_ops, int num_ops, double dt, const std::size_t* tidx, std::size_t ntau,
@@ -2798,44 +2821,66 @@ static TimePlan l4_core(taco_ctx* c, const double2* d_gamma, std::size_t Nt, int
     const std::size_t N4 = static_cast<std::size_t>(N) * N * N * N;
     if (nf3 > 0 && static_cast<std::size_t>(npos) > (~std::size_t{0} / 48) / nf3)
         throw Error(TACO_ERR_OVERFLOW, "FCR buffer too large");
-    c->slices.ensure(3 * nf3 * static_cast<std::size_t>(npos) * 16);
-    c->t2pos.ensure(tp.T * sizeof(int));
-    HIP_OK(hipMemcpyAsync(c->t2pos.p, tp.t2pos.data(), tp.T * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    const std::size_t per_pos = 3 * nf3 * 16;
+    const long win = static_cast<long>(
+        std::max<std::size_t>(1, std::min<std::size_t>(static_cast<std::size_t>(npos), slice_budget(c) / per_pos)));
+    const long nwin = (npos + win - 1) / win;
+    c->slices.ensure(per_pos * static_cast<std::size_t>(win));
     c->l4.ensure(N4 * static_cast<std::size_t>(npos) * 16);
-    CombineArgs out{};
-    out.out_mode = 0;
-    out.t2pos = c->t2pos.as<int>();
-    out.npos = npos;
-    out.nf3 = static_cast<long>(nf3);
-    double2* sl = c->slices.as<double2>();
-    out.Fo = sl;
-    out.Co = sl + nf3 * npos;
-    out.Ro = sl + 2 * nf3 * npos;
+    // t -> position within its window, one [T_w] block per window
+    std::vector<int> t2w;
+    std::vector<std::size_t> woff(static_cast<std::size_t>(nwin) + 1, 0);
+    for (long w = 0; w < nwin; ++w) {
+        const long p0 = w * win, p1 = std::min(npos, p0 + win);
+        const std::size_t Tw = tp.uniq[static_cast<std::size_t>(p1 - 1)] + 1;
+        woff[static_cast<std::size_t>(w)] = t2w.size();
+        for (std::size_t t = 0; t < Tw; ++t) {
+            const int q = tp.t2pos[t];
+            t2w.push_back(q >= p0 && q < p1 ? static_cast<int>(q - p0) : -1);
+        }
+    }
+    woff[static_cast<std::size_t>(nwin)] = t2w.size();
+    c->t2pos.ensure(t2w.size() * sizeof(int));
+    HIP_OK(hipMemcpyAsync(c->t2pos.p, t2w.data(), t2w.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
     const std::vector<unsigned char> need = needed_triples(h_p2f, N, nf);
-    if (fcr_ms) HIP_OK(hipEventRecord(c->ev0, c->stream));
-    if (env_int("TACO_DEBUG_POISON", 0))  // NaN-fill the slices: an entry L4 reads but nobody wrote shows up
-        HIP_OK(hipMemsetAsync(c->slices.p, 0xff, 3 * nf3 * static_cast<std::size_t>(npos) * 16, c->stream));
-    FcrPlan pl{d_gamma, Nt, nf, d_omegas, h_omegas, d_mirror, dt, tp.T, &need};
-    run_fcr(c, pl, out);
-    if (fcr_ms) HIP_OK(hipEventRecord(c->ev1, c->stream));
-    L4Args A{};
-    A.Fs = out.Fo;
-    A.Cs = out.Co;
-    A.Rs = out.Ro;
-    A.p2f = d_p2f;
-    A.ops = d_ops;
-    A.N = N;
-    A.nf = nf;
-    A.num_ops = num_ops;
-    A.nf3 = static_cast<long>(nf3);
-    A.L4 = c->l4.as<double2>();
-    A.GW = nullptr;
-    launch_l4(c, A, npos);
-    if (fcr_ms) {
-        HIP_OK(hipEventSynchronize(c->ev1));
-        float ms = 0.f;
-        HIP_OK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-        *fcr_ms = ms;
+    if (fcr_ms) *fcr_ms = 0.0;
+    for (long w = 0; w < nwin; ++w) {
+        const long p0 = w * win, np = std::min(npos, p0 + win) - p0;
+        const std::size_t Tw = woff[static_cast<std::size_t>(w) + 1] - woff[static_cast<std::size_t>(w)];
+        CombineArgs out{};
+        out.out_mode = 0;
+        out.t2pos = c->t2pos.as<int>() + woff[static_cast<std::size_t>(w)];
+        out.npos = np;
+        out.nf3 = static_cast<long>(nf3);
+        double2* sl = c->slices.as<double2>();
+        out.Fo = sl;
+        out.Co = sl + nf3 * np;
+        out.Ro = sl + 2 * nf3 * np;
+        if (fcr_ms) HIP_OK(hipEventRecord(c->ev0, c->stream));
+        if (env_int("TACO_DEBUG_POISON", 0))  // NaN-fill the slices: an entry L4 reads but nobody wrote shows up
+            HIP_OK(hipMemsetAsync(c->slices.p, 0xff, per_pos * static_cast<std::size_t>(np), c->stream));
+        FcrPlan pl{d_gamma, Nt, nf, d_omegas, h_omegas, d_mirror, dt, Tw, &need, tp.T};
+        run_fcr(c, pl, out);
+        if (fcr_ms) HIP_OK(hipEventRecord(c->ev1, c->stream));
+        L4Args A{};
+        A.Fs = out.Fo;
+        A.Cs = out.Co;
+        A.Rs = out.Ro;
+        A.p2f = d_p2f;
+        A.ops = d_ops;
+        A.N = N;
+        A.nf = nf;
+        A.num_ops = num_ops;
+        A.nf3 = static_cast<long>(nf3);
+        A.L4 = c->l4.as<double2>() + static_cast<std::size_t>(p0) * N4;
+        A.GW = nullptr;
+        launch_l4(c, A, np);
+        if (fcr_ms) {
+            HIP_OK(hipEventSynchronize(c->ev1));
+            float ms = 0.f;
+            HIP_OK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            *fcr_ms += ms;
+        }
     }
     return tp;
 }
@@ -3633,6 +3678,14 @@ int taco_set_fft_mode(int mode) {
     return guarded([&] { set_fft_mode(mode); });
 }
 int taco_get_fft_mode(void) { return fft_mode(); }
+
+int taco_set_slice_budget(taco_ctx* ctx, size_t bytes) {
+    return guarded([&] {
+        check_ctx(ctx);
+        ctx->slice_budget = bytes;
+    });
+}
+size_t taco_get_slice_budget(const taco_ctx* ctx) { return ctx ? slice_budget(ctx) : 0; }
 
 int taco_ctx_create(int device, taco_ctx** out) {
     return guarded([&] {

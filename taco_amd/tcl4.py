@@ -42,7 +42,7 @@ __all__ = [
     "propagate_rk4_dense",
     "propagate_rk4_dense_serial", "propagate_rk4_dense_omp", "tcl4_propagate", "set_fcr_fft_pad_factor",
     "get_fcr_fft_pad_factor", "set_phase_mode", "get_phase_mode", "phase_mode", "PHASE_LIBM", "PHASE_CR",
-    "set_fft_mode", "get_fft_mode", "fft_mode", "FFT_PARITY", "FFT_WINDOW",
+    "set_fft_mode", "get_fft_mode", "fft_mode", "FFT_PARITY", "FFT_WINDOW", "set_slice_budget", "get_slice_budget",
     "context", "version", "BatchItem", "Batch", "batched_run",
 ]
 
@@ -145,6 +145,18 @@ class phase_mode:
 
     def __exit__(self, *exc):
         set_phase_mode(self.prev)
+
+
+def set_slice_budget(nbytes: int, device: int = 0) -> None:
+    """Bytes of F/C/R slices one pass of the L4 builders may hold on this
+    thread's context (0: default, 32 GiB).  Longer time sets run as windows
+    of consecutive kept times, bit-identical to one pass
+    (backend/cpu/tcl4_mpi_omp.cpp:62-156 splits the times the same way)."""
+    check(lib().taco_set_slice_budget(context(device).handle, int(nbytes)))
+
+
+def get_slice_budget(device: int = 0) -> int:
+    return int(lib().taco_get_slice_budget(context(device).handle))
 
 
 # --- contexts: one per (host thread, device), like the reference's thread_local workspace

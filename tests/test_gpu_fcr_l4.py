@@ -451,3 +451,33 @@ def test_combine_split_equals_single_launch(monkeypatch, wl):
         res[mode] = (ser.F, ser.C, ser.R, L4)
     for a, b in zip(res["0"], res["1"]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("fft", ["parity", "window"])
+def test_time_windows_equal_one_pass(orc, fft):
+    """A slice budget below the call's kept times runs the builder as windows
+    of consecutive kept times (an F/C/R pass and the L4 per window, the
+    reference's time decomposition, tcl4_mpi_omp.cpp:62-156): L4 is
+    bit-identical to the one-pass build for an unsorted time list with
+    repeats, in both FFT modes, and the budget is per context."""
+    w = W.gue_pair(N_bath=3000, tmax=400)
+    S, G = w.system, w.gamma
+    nf = len(S.omegas)
+    rng = np.random.default_rng(11)
+    tidx = [int(t) for t in rng.integers(0, 1500, 60)] + [0, 1499, 7, 7]
+    mode = taco_amd.FFT_PARITY if fft == "parity" else taco_amd.FFT_WINDOW
+    default = taco_amd.get_slice_budget()
+    with taco_amd.fft_mode(mode):
+        one = taco_amd.build_TCL4_generator_hip_fused_batch(S, G, w.dt, tidx)
+        try:
+            for per in (1, 7, 25):
+                taco_amd.set_slice_budget(per * 3 * nf ** 3 * 16)
+                assert taco_amd.get_slice_budget() == per * 3 * nf ** 3 * 16
+                got = taco_amd.build_TCL4_generator_hip_fused_batch(S, G, w.dt, tidx)
+                assert np.array_equal(got, one), per
+        finally:
+            taco_amd.set_slice_budget(0)
+    assert taco_amd.get_slice_budget() == default
+    if fft == "parity":
+        ref = orc.l4_series(G, S.omegas, S.mirror, S.pair_to_freq, S.A_eig, w.dt, tidx)
+        assert max_rel_err(one, ref)[0] <= TOL
