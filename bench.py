@@ -42,7 +42,7 @@ def parse():
     p.add_argument("--workload", default="gue_pair",
                    choices=["gue_pair", "published", "spin_boson", "batched", "adjoint", "tfim_adjoint"])
     p.add_argument("--systems", type=int, default=None,
-                   help="batched workload: number of systems (256); tfim_adjoint: parameter sets (4)")
+                   help="batched workload: number of systems (256); tfim_adjoint: parameter sets (64)")
     p.add_argument("--no-plain", action="store_true", help="batched workload: skip the plain per-system leg")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / checker leg")
     p.add_argument("--cpu-threads", type=int, default=0)
@@ -445,8 +445,9 @@ def tfim_adjoint_main(a):
     """Gradients of the d = 16 loss (open 4-spin transverse-field Ising chain,
     J = 1, h drawn from U[0.5, 1.5] (seed 5), A = sum sz_i, Drude-Lorentz
     bath lambda 0.05, gamma 5, beta 1) with respect to (lambda, gamma, beta,
-    J, h), one parameter set after another (--systems sets, default 4).  The
-    window is reduced from the 2^20-point grid and 20001 times of the full
+    J, h), for --systems parameter sets (default 64) in one multi-set tape
+    (their RK4 recursions one launch each way), and set by set for the first
+    four (the single-tape path, timed for comparison).  The window is reduced from the 2^20-point grid and 20001 times of the full
     configuration (its forward F/C/R pass alone moves ~10 TB per parameter
     set) to a 2^12-point grid and 200 RK4 steps of TCL2 + L4, checkpoints
     every 64.  One step = one gradient per parameter set (forward + reverse);
@@ -464,7 +465,7 @@ def tfim_adjoint_main(a):
     # correctly rounded phases: each parameter set has its own ~10^5 Omega
     # classes, whose libm table the host would rebuild for every gradient
     taco_amd.set_phase_mode(taco_amd.PHASE_CR)
-    nsets = a.systems or 4
+    nsets = a.systems or 64
     rng = np.random.default_rng(5)
     hs = rng.uniform(0.5, 1.5, size=nsets)
     prob = AD.TfimChainProblem(Nt=2 ** 12, dt=0.005, steps=200, checkpoint_every=64)
@@ -484,21 +485,39 @@ def tfim_adjoint_main(a):
         (g,) = torch.autograd.grad(loss, params)
         return float(loss), g.numpy()
 
+    def grad_many():
+        params = [torch.tensor(p, dtype=torch.float64, requires_grad=True) for p in P0]
+        losses = prob.losses_many(params, targets)
+        gs = torch.autograd.grad(sum(losses), params)
+        return [(float(l), g.numpy()) for l, g in zip(losses, gs)]
+
     for _ in range(a.warmup):
         fwd(0)
         grad(0)
+    # set by set (a few sets: each is a whole forward + reverse)
+    nseq = min(nsets, 4)
     t0 = time.perf_counter()
     for _ in range(a.steps):
-        for q in range(nsets):
+        for q in range(nseq):
             fwd(q)
-    t_fwd = (time.perf_counter() - t0) / (a.steps * nsets)
+    t_fwd = (time.perf_counter() - t0) / (a.steps * nseq)
+    t0 = time.perf_counter()
+    seq = []
+    for _ in range(a.steps):
+        seq = [grad(q) for q in range(nseq)]
+    t_seq = (time.perf_counter() - t0) / (a.steps * nseq)
+    print(f"[bench] set by set: forward {t_fwd * 1e3:.1f} ms, forward+reverse {t_seq * 1e3:.1f} ms per set",
+          file=sys.stderr, flush=True)
+    # every set in one multi-set tape
+    grad_many()
     t0 = time.perf_counter()
     grads = []
     for _ in range(a.steps):
-        grads = [grad(q) for q in range(nsets)]
+        grads = grad_many()
     t_grad = (time.perf_counter() - t0) / (a.steps * nsets)
-    print(f"[bench] forward {t_fwd * 1e3:.1f} ms, forward+reverse {t_grad * 1e3:.1f} ms per set", file=sys.stderr,
-          flush=True)
+    same = sum(1 for q in range(nseq) if grads[q][0] == seq[q][0] and np.array_equal(grads[q][1], seq[q][1]))
+    print(f"[bench] multi-set tape: {t_grad * 1e3:.1f} ms per set over {nsets} sets; {same}/{nseq} sets bit-equal "
+          f"to the set-by-set gradients", file=sys.stderr, flush=True)
     from taco_amd import tcl4
     with tcl4.Profile(0) as prof:
         grad(0)
@@ -530,13 +549,18 @@ def tfim_adjoint_main(a):
         "dtype": "c128",
         "data": "synthetic",
         "config": {
-            "workload": f"tfim_adjoint (reduced window): d=16 open TFIM chain (nf=81), J=1, h~U[0.5,1.5] seed 5, "
+            "workload": f"tfim_adjoint (reduced window): d=16 open TFIM chain (nf=81 at generic h), J=1, "
+                        f"h~U[0.5,1.5] seed 5, "
                         f"{nsets} parameter sets, Drude-Lorentz (0.05, 5, 1), Nt=2^12, dt=0.005, 200 RK4 steps of "
                         f"TCL2+L4, checkpoints every 64, 5 parameters (lambda, gamma, beta, J, h)",
-            "path": "tape forward + hand-written reverse, one parameter set after another",
+            "path": "multi-set tape: every set's F/C/R, L4, TCL2 and their reverses in turn, the forward RK4 "
+                    "and the reverse RK4 (recompute, walk) of all sets one launch each; F/C/R slices "
+                    "recomputed in the reverse pass when the sets' slices exceed 32 GiB; correctly rounded phases",
         },
         "forward_ms": 1e3 * t_fwd,
         "gradient_over_forward": t_grad / t_fwd,
+        "set_by_set": {"sets": nseq, "ms_per_gradient": 1e3 * t_seq, "forward_ms": 1e3 * t_fwd,
+                       "multi_set_speedup": t_seq / t_grad, "bit_equal_sets": same},
         "check": {"params": list(prob.names), "h": float(hs[0]), "adjoint": g0.tolist(), "gpu_fd": fdg.tolist(),
                   "max_rel_diff": float(rel.max()), "loss": loss0},
         "roofline": roofline,

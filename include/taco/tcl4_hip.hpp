@@ -183,6 +183,8 @@ std::vector<Matrix> build_TCL4_generator_hip_fused_batch(const System& system, c
                                                          const std::vector<std::size_t>& time_indices,
                                                          FCRMethodT method, const ExecT& exec,
                                                          double* fcr_ms = nullptr) {
+    if (static_cast<int>(method) != TACO_FCR_CONVOLUTION)  // tcl4_fused_cuda.cu:820-822
+        throw std::invalid_argument("build_TCL4_generator_cuda_fused: only FCRMethod::Convolution is supported");
     const auto f = taco::hip::flatten(system);
     const std::size_t Nt = static_cast<std::size_t>(gamma_series.rows());
     if (static_cast<int>(gamma_series.cols()) != f.nf)

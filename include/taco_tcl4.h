@@ -153,6 +153,18 @@ int taco_system_build(const double* H, int N, const double* A_lab, int num_ops, 
                       double* eps, double* U, double* A_eig, int* nf, double* omegas, int* mirror,
                       int* pair_to_freq, int* zero_index);
 
+/* taco_system_build on the GPU for nsys Hamiltonians at once (N <= 64), a
+ * workgroup per system running the same steps as the host builder (cyclic
+ * complex Jacobi with algebraic rotations, ascending eigenvalues, the
+ * largest entry of each eigenvector real positive, the reference's bucket
+ * scan, mirror and zero bucket, A_eig = U^H A U): every output equals
+ * taco_system_build's bit for bit.  Host arrays, per system s at offsets
+ * s*N (eps), s*N*N (H, U, omegas, mirror, pair_to_freq), s*num_ops*N*N
+ * (A_lab, A_eig), s (nf, zero_index); synchronous. */
+int taco_system_build_hip(taco_ctx* ctx, int nsys, const double* H, int N, const double* A_lab, int num_ops,
+                          double freq_tol, double* eps, double* U, double* A_eig, int* nf, double* omegas,
+                          int* mirror, int* pair_to_freq, int* zero_index);
+
 /* Spectral density kinds for taco_bcf_fft. */
 #define TACO_J_OHMIC 0          /* J = p0 * w * exp(-w / p1)       (alpha, omega_c) */
 #define TACO_J_DRUDE_LORENTZ 1  /* J = 2 p0 p1 w / (w^2 + p1^2)    (lambda, gamma)  */
@@ -164,6 +176,27 @@ int taco_bcf_fft(size_t N, double dt, int kind, double p0, double p1, double bet
  * The phase step exp(i omega_b dt) and the exponentials of taco_bcf_fft
  * follow the phase mode (libm std::exp or correctly rounded). */
 int taco_gamma_trapz(const double* C, size_t Nc, double dt, const double* omegas, int nf, double* gamma_out);
+
+/* Spectral density given as an expression of w (the configuration files'
+ * J_expr, expression.hpp:14-26): + - * / ^ (unary minus binds tighter than
+ * * / and looser than ^; ^ groups to the right), exp log sqrt abs sin cos
+ * tan step/heaviside, pow max min, the constants pi and e, and the named
+ * parameters given here.  Compile errors (unknown identifier or function,
+ * unbalanced parentheses, wrong argument counts, an expression that does not
+ * reduce to one value) return TACO_ERR_INVALID with the message in
+ * taco_last_error.  Evaluation calls the platform libm. */
+typedef struct taco_spectral taco_spectral;
+int taco_spectral_create(const char* source, const char* const* names, const double* values, int nparams,
+                         taco_spectral** out);
+int taco_spectral_eval(const taco_spectral* J, const double* w, size_t n, double* out);
+int taco_spectral_destroy(taco_spectral* J);
+/* taco_bcf_fft for an expression spectral density (bcf_fft_fun with a
+ * callable J, correlation_fft.hpp:154-214, as tcl_driver.cpp builds C(t)). */
+int taco_bcf_fft_spectral(size_t N, double dt, const taco_spectral* J, double beta, double* C_out);
+/* Rectangle-rule Gamma (compute_rect_prefix_multi_matrix, gamma.hpp:127-160):
+ * Gamma[k + Nc*b] = sum_{m<=k} (dt phi_m) C[m], phi_m = exp(i omega_b dt)^m;
+ * the phase step follows the phase mode. */
+int taco_gamma_rect(const double* C, size_t Nc, double dt, const double* omegas, int nf, double* gamma_out);
 
 /* GPU twins of the two producers above (bcf_fft_fun, correlation_fft.hpp:154-214;
  * compute_trapz_prefix_multi_matrix, gamma.hpp:166-213), bit-identical to
@@ -185,7 +218,9 @@ int taco_gamma_trapz_hip(taco_ctx* ctx, const double* C, size_t Nc, double dt, c
  * `keep` samples (keep <= Nt; 0 means Nt) of every F/C/R series:
  * F[(idx3 * keep) + t], idx3 = (i*nf + j)*nf + k.  The FFT length follows the
  * full Nt, so the kept samples equal the reference's full series.
- * Only TACO_FCR_CONVOLUTION is supported on the GPU (as in the reference). */
+ * method: TACO_FCR_CONVOLUTION (the FFT route, tcl4_kernels.cpp:105-175) or
+ * TACO_FCR_DIRECT (the O(T^2) sums of tcl4_kernels.cpp:188-281, summed in
+ * the reference's order; for checks and short series). */
 int taco_triple_kernels(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, const double* omegas,
                         const int* mirror, double dt, int method, size_t keep, double* F, double* C,
                         double* R);
@@ -193,7 +228,10 @@ int taco_triple_kernels(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, c
 /* build_TCL4_generator_cuda_fused_batch (tcl4_fused_cuda.hpp:25-41):
  * L4(t) for t in tidx[0..ntau) (ntau == 0 means every t < Nt), each an
  * N^2 x N^2 column-major block of L4_out.  fcr_ms (nullable) receives the
- * device time of the F/C/R stage, like the reference's cuda_fcr_ms. */
+ * device time of the F/C/R stage, like the reference's cuda_fcr_ms.
+ * method as for taco_triple_kernels (the reference's fused CUDA builders
+ * accept Convolution only; its CPU build_TCL4_generator takes both, and the
+ * C++ wrappers of the fused names keep the CUDA rule). */
 int taco_l4_batch(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, const double* omegas,
                   const int* mirror, const int* pair_to_freq, int N, const double* ops, int num_ops,
                   double dt, int method, const size_t* tidx, size_t ntau, double* L4_out, double* fcr_ms);
@@ -361,6 +399,42 @@ int taco_tape_forward(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, con
 int taco_tape_adjoint(taco_tape* tape, const double* samples_bar, double* gamma_bar, double* omega_bar,
                       double* ops_bar, double* eps_bar, double* r0_bar);
 int taco_tape_destroy(taco_tape* tape);
+/* Multi-set tape (no reference counterpart): S parameter sets of one model
+ * size -- the same N and steps; Gamma, buckets, operators, eigenvalues, dt,
+ * order and r0 per set -- forward and reverse together.  Each set's F/C/R,
+ * L4 and TCL2 passes run in turn; the forward RK4 of all sets is one launch
+ * (a workgroup per set) and so are the reverse RK4's recompute and walk.
+ * Every set's samples and cotangents equal those of taco_tape_forward /
+ * taco_tape_adjoint on that set alone, bit for bit.  The sets' F/C/R slices
+ * stay on the device while their total fits the context's slice budget
+ * (taco_set_slice_budget); otherwise the reverse pass recomputes each set's
+ * slices in one shared buffer (taco_mtape_keeps_slices tells which).
+ * samples_out / samples_bar / the cotangent arrays hold one pointer per set
+ * (layouts as in taco_tape_*); an output array or entry may be NULL. */
+typedef struct taco_tape_set {
+    const double* gamma;        /* [Nt x nf] Gamma table, column-major */
+    size_t Nt;
+    int nf;
+    const double* omegas;       /* [nf] */
+    const int* mirror;          /* [nf] */
+    const int* pair_to_freq;    /* N x N column-major */
+    int N;
+    const double* eps;          /* [N] */
+    const double* ops;          /* num_ops N x N blocks (A_eig) */
+    int num_ops;
+    double dt;
+    size_t steps;
+    int order;                  /* 4 (TCL2 + L4), 2, 0 */
+    const double* r0;           /* [N^2] */
+} taco_tape_set;
+typedef struct taco_mtape taco_mtape;
+int taco_mtape_forward(taco_ctx* ctx, const taco_tape_set* sets, int nsets, size_t checkpoint_every,
+                       double* const* samples_out, taco_mtape** out);
+int taco_mtape_adjoint(taco_mtape* mt, const double* const* samples_bar, double* const* gamma_bar,
+                       double* const* omega_bar, double* const* ops_bar, double* const* eps_bar,
+                       double* const* r0_bar);
+int taco_mtape_keeps_slices(const taco_mtape* mt);
+int taco_mtape_destroy(taco_mtape* mt);
 /* One-shot taco_tape_forward + taco_tape_adjoint + taco_tape_destroy. */
 int taco_adjoint(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, const double* omegas, const int* mirror,
                  const int* pair_to_freq, int N, const double* eps, const double* ops, int num_ops, double dt,

@@ -33,6 +33,17 @@ class BatchSystem(ctypes.Structure):
     ]
 
 
+class TapeSet(ctypes.Structure):
+    """taco_tape_set of include/taco_tcl4.h."""
+    _fields_ = [
+        ("gamma", _dp), ("Nt", c_size_t), ("nf", c_int), ("omegas", _dp), ("mirror", _ip),
+        ("pair_to_freq", _ip), ("N", c_int), ("eps", _dp), ("ops", _dp), ("num_ops", c_int), ("dt", c_double),
+        ("steps", c_size_t), ("order", c_int), ("r0", _dp),
+    ]
+
+
+_dpp = POINTER(_dp)
+
 # name -> (restype, argtypes); every symbol declared in include/taco_tcl4.h.
 SIGNATURES = {
     "taco_last_error": (c_char_p, []),
@@ -58,8 +69,15 @@ SIGNATURES = {
     "taco_set_slice_budget": (c_int, [c_void_p, c_size_t]),
     "taco_get_slice_budget": (c_size_t, [c_void_p]),
     "taco_system_build": (c_int, [_dp, c_int, _dp, c_int, c_double, _dp, _dp, _dp, _ip, _dp, _ip, _ip, _ip]),
+    "taco_system_build_hip": (c_int, [c_void_p, c_int, _dp, c_int, _dp, c_int, c_double, _dp, _dp, _dp, _ip, _dp,
+                                      _ip, _ip, _ip]),
     "taco_bcf_fft": (c_int, [c_size_t, c_double, c_int, c_double, c_double, c_double, _dp]),
     "taco_gamma_trapz": (c_int, [_dp, c_size_t, c_double, _dp, c_int, _dp]),
+    "taco_spectral_create": (c_int, [c_char_p, POINTER(c_char_p), _dp, c_int, POINTER(c_void_p)]),
+    "taco_spectral_eval": (c_int, [c_void_p, _dp, c_size_t, _dp]),
+    "taco_spectral_destroy": (c_int, [c_void_p]),
+    "taco_bcf_fft_spectral": (c_int, [c_size_t, c_double, c_void_p, c_double, _dp]),
+    "taco_gamma_rect": (c_int, [_dp, c_size_t, c_double, _dp, c_int, _dp]),
     "taco_triple_kernels": (c_int, [c_void_p, _dp, c_size_t, c_int, _dp, _ip, c_double, c_int, c_size_t,
                                     _dp, _dp, _dp]),
     "taco_l4_batch": (c_int, [c_void_p, _dp, c_size_t, c_int, _dp, _ip, _ip, c_int, _dp, c_int, c_double,
@@ -90,6 +108,10 @@ SIGNATURES = {
                                   c_size_t, c_int, _dp, c_size_t, _dp, POINTER(c_void_p)]),
     "taco_tape_adjoint": (c_int, [c_void_p, _dp, _dp, _dp, _dp, _dp, _dp]),
     "taco_tape_destroy": (c_int, [c_void_p]),
+    "taco_mtape_forward": (c_int, [c_void_p, POINTER(TapeSet), c_int, c_size_t, _dpp, POINTER(c_void_p)]),
+    "taco_mtape_adjoint": (c_int, [c_void_p, _dpp, _dpp, _dpp, _dpp, _dpp, _dpp]),
+    "taco_mtape_keeps_slices": (c_int, [c_void_p]),
+    "taco_mtape_destroy": (c_int, [c_void_p]),
     "taco_adjoint": (c_int, [c_void_p, _dp, c_size_t, c_int, _dp, _ip, _ip, c_int, _dp, _dp, c_int, c_double,
                              c_size_t, c_int, _dp, c_size_t, _dp, _dp, _dp, _dp, _dp, _dp]),
 }

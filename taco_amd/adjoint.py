@@ -38,7 +38,7 @@ from ._lib import TacoInvalidArgument, check, dptr, iptr, lib
 from .tcl4 import (Exec, Ohmic, DrudeLorentz, System, _ctx_for, bath_gamma_hip, bcf_fft_fun,
                    compute_trapz_prefix_multi_matrix, compute_trapz_prefix_multi_matrix_hip)
 
-__all__ = ["Tape", "TCL4Propagate", "BcfFun", "GammaFun", "EighFun", "BucketOmegas", "SpinPairProblem"]
+__all__ = ["Tape", "MultiTape", "TCL4Propagate", "TCL4PropagateMany", "BcfFun", "GammaFun", "EighFun", "BucketOmegas", "SpinPairProblem"]
 
 
 def _cm(a) -> np.ndarray:
@@ -103,6 +103,89 @@ class Tape:
             pass
 
 
+class MultiTape:
+    """Several parameter sets of one model size (the same N and steps) run
+    forward and reverse together (taco_mtape_*): each set's samples and
+    cotangents equal a Tape of that set alone, bit for bit, while the RK4
+    recursions of all sets share one launch each way.  `sets` holds dicts
+    with the Tape arguments (gamma, omegas, mirror, pair_to_freq, ops, eps,
+    dt, steps, r0, optional order)."""
+
+    def __init__(self, sets, checkpoint_every: int = 64, exec: Optional[Exec] = None):
+        from ._lib import TapeSet
+        self.ctx = _ctx_for(exec)
+        self.meta, self._keep = [], []
+        arr = (TapeSet * len(sets))()
+        for k, st in enumerate(sets):
+            g = np.asarray(st["gamma"], np.complex128)
+            Nt, nf = g.shape
+            p2f = np.asarray(st["pair_to_freq"], np.int32)
+            N = p2f.shape[0]
+            ops = st["ops"]
+            steps = int(st["steps"])
+            buf = dict(
+                gc=np.ascontiguousarray(g.T).ravel(), om=np.ascontiguousarray(st["omegas"], np.float64),
+                mi=np.ascontiguousarray(st["mirror"], np.int32), pf=np.ascontiguousarray(p2f.T).ravel(),
+                of=np.concatenate([_cm(o) for o in ops]), ev=np.ascontiguousarray(st["eps"], np.float64),
+                r=np.ascontiguousarray(np.asarray(st["r0"], np.complex128).ravel()))
+            if buf["r"].size != N * N:
+                raise TacoInvalidArgument(1, "r0 must have N^2 entries")
+            self._keep.append(buf)
+            self.meta.append(dict(Nt=Nt, nf=nf, N=N, num_ops=len(ops), steps=steps))
+            arr[k] = TapeSet(dptr(buf["gc"]), Nt, nf, dptr(buf["om"]), iptr(buf["mi"]), iptr(buf["pf"]), N,
+                             dptr(buf["ev"]), dptr(buf["of"]), len(ops), float(st["dt"]), steps,
+                             int(st.get("order", 4)), dptr(buf["r"]))
+        self.samples = [np.zeros((m["steps"] + 1, m["N"] ** 2), np.complex128) for m in self.meta]
+        h = c_void_p()
+        check(lib().taco_mtape_forward(self.ctx.handle, arr, len(sets), int(checkpoint_every),
+                                       _ptrs(self.samples), ctypes.byref(h)))
+        self.handle = h
+        self._keep = None  # inputs are only read during the forward call
+
+    @property
+    def keeps_slices(self) -> bool:
+        return bool(lib().taco_mtape_keeps_slices(self.handle))
+
+    def adjoint(self, samples_bar) -> list:
+        sb = [np.ascontiguousarray(np.asarray(b, np.complex128).reshape(m["steps"] + 1, m["N"] ** 2))
+              for b, m in zip(samples_bar, self.meta)]
+        gb = [np.zeros(m["Nt"] * m["nf"], np.complex128) for m in self.meta]
+        ob = [np.zeros(m["nf"]) for m in self.meta]
+        ab = [np.zeros(m["num_ops"] * m["N"] ** 2, np.complex128) for m in self.meta]
+        eb = [np.zeros(m["N"]) for m in self.meta]
+        rb = [np.zeros(m["N"] ** 2, np.complex128) for m in self.meta]
+        check(lib().taco_mtape_adjoint(self.handle, _ptrs(sb), _ptrs(gb), _ptrs(ob), _ptrs(ab), _ptrs(eb),
+                                       _ptrs(rb)))
+        out = []
+        for k, m in enumerate(self.meta):
+            N = m["N"]
+            out.append({
+                "gamma": gb[k].reshape(m["nf"], m["Nt"]).T.copy(),
+                "omegas": ob[k],
+                "ops": [ab[k][q * N * N:(q + 1) * N * N].reshape(N, N).T.copy() for q in range(m["num_ops"])],
+                "eps": eb[k],
+                "r0": rb[k],
+            })
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None):
+            check(lib().taco_mtape_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _ptrs(arrays):
+    """double** over numpy arrays (complex arrays as interleaved doubles)."""
+    from ._lib import _dp
+    return (_dp * len(arrays))(*[dptr(a) for a in arrays])
+
+
 class TCL4Propagate(torch.autograd.Function):
     """r_0..r_steps of the TCL4 run as a differentiable function of
     (gamma [Nt, nf], omegas [nf], ops [num_ops, N, N], eps [N], r0 [N^2]).
@@ -123,6 +206,39 @@ class TCL4Propagate(torch.autograd.Function):
         return (torch.from_numpy(res["gamma"]), torch.from_numpy(res["omegas"]),
                 torch.from_numpy(np.stack(res["ops"])), torch.from_numpy(res["eps"]), torch.from_numpy(res["r0"]),
                 None)
+
+
+class TCL4PropagateMany(torch.autograd.Function):
+    """TCL4Propagate of several parameter sets of one model size at once
+    (MultiTape: one RK4 launch for all sets each way).  Inputs after ``meta``
+    are the sets' (gamma, omegas, ops, eps, r0) in turn; ``meta`` = (per-set
+    (mirror, pair_to_freq, dt, order) tuples, steps, checkpoint_every, exec).
+    Returns each set's r_0..r_steps; every set's values and cotangents equal
+    TCL4Propagate on that set alone."""
+
+    @staticmethod
+    def forward(ctx, meta, *tensors):
+        per, steps, K, ex = meta
+        sets = []
+        for q, (mirror, p2f, dt, order) in enumerate(per):
+            g, om, ops, e, r0 = tensors[5 * q:5 * q + 5]
+            sets.append(dict(gamma=g.detach().numpy(), omegas=om.detach().numpy(), mirror=mirror,
+                             pair_to_freq=p2f, ops=list(ops.detach().numpy()), eps=e.detach().numpy(), dt=dt,
+                             steps=steps, r0=r0.detach().numpy(), order=order))
+        mt = MultiTape(sets, K, ex)
+        ctx.mt = mt
+        return tuple(torch.from_numpy(x.copy()) for x in mt.samples)
+
+    @staticmethod
+    def backward(ctx, *gs):
+        bars = [np.zeros(m.shape, np.complex128) if g is None else g.detach().numpy()
+                for g, m in zip(gs, ctx.mt.samples)]
+        out = [None]
+        for res in ctx.mt.adjoint(bars):
+            out += [torch.from_numpy(res["gamma"]), torch.from_numpy(res["omegas"]),
+                    torch.from_numpy(np.stack(res["ops"])), torch.from_numpy(res["eps"]),
+                    torch.from_numpy(res["r0"])]
+        return tuple(out)
 
 
 def _spectral(kind: int, p0: float, p1: float, w: np.ndarray):
@@ -378,6 +494,33 @@ def _rho_lab(prob, H: torch.Tensor, A: np.ndarray, kind: int, p0, p1, beta, exec
     return U @ rho @ U.conj().T
 
 
+def _rho_lab_many(prob, Hs, A: np.ndarray, kind: int, baths, exec: Optional[Exec]) -> list:
+    """_rho_lab for several parameter sets (Hamiltonians Hs, bath parameters
+    baths[q] = (p0, p1, beta)) of one model size, with the TCL4 runs of all
+    sets in one multi-set tape."""
+    producer = exec or Exec()
+    per, flat, Us, Ns = [], [], [], []
+    for H, (p0, p1, beta) in zip(Hs, baths):
+        e, U = EighFun.apply(H, [A], prob.freq_tol)
+        S = System.build(H.detach().numpy(), [A], prob.freq_tol)
+        om = BucketOmegas.apply(e, S.omegas, S.pair_to_freq)
+        C = BcfFun.apply(p0, p1, beta, (kind, prob.Nt - 1, prob.dt, producer))
+        G = GammaFun.apply(C, om, prob.dt, producer)
+        Aeig = (U.conj().T @ torch.from_numpy(A) @ U).unsqueeze(0)
+        r0 = torch.zeros(S.N * S.N, dtype=torch.complex128)
+        r0[0] = 1.0
+        per.append((S.mirror, S.pair_to_freq, prob.dt, prob.order))
+        flat += [G, om, Aeig, e, r0]
+        Us.append(U)
+        Ns.append(S.N)
+    rs = TCL4PropagateMany.apply((per, prob.steps, prob.checkpoint_every, exec), *flat)
+    out = []
+    for r, U, N in zip(rs, Us, Ns):
+        rho = r.reshape(-1, N, N).transpose(1, 2)
+        out.append(U @ rho @ U.conj().T)
+    return out
+
+
 def _kron_chain(ops) -> torch.Tensor:
     out = ops[0]
     for o in ops[1:]:
@@ -440,3 +583,18 @@ class TfimChainProblem:
     def loss(self, params: torch.Tensor, target: torch.Tensor, exec: Optional[Exec] = None) -> torch.Tensor:
         d = self.rho_lab(params, exec) - target
         return (d.real ** 2 + d.imag ** 2).sum()
+
+    def losses_many(self, params: Sequence[torch.Tensor], targets: Sequence[torch.Tensor],
+                    exec: Optional[Exec] = None) -> list:
+        """loss() of several parameter sets, their TCL4 runs (forward and
+        reverse) in one multi-set tape; equal to loss() set by set."""
+        Hs, baths = [], []
+        for p in params:
+            Hs.append(self.hamiltonian(p[3].to(torch.complex128), p[4].to(torch.complex128)))
+            baths.append((p[0], p[1], p[2]))
+        rhos = _rho_lab_many(self, Hs, self.coupling(), 1, baths, exec)
+        out = []
+        for rho, tgt in zip(rhos, targets):
+            d = rho - tgt
+            out.append((d.real ** 2 + d.imag ** 2).sum())
+        return out

@@ -308,6 +308,54 @@ __global__ __launch_bounds__(kBlock) void k_rk4_adjoint(Rk4AdjArgs A) {
     }
 }
 
+// Many tapes of one D, steps and K in one launch: the recompute with
+// blockIdx.y choosing the tape, the walk with one workgroup per tape; each
+// workgroup runs the single-tape body on its own arguments.
+template <int DT>
+__global__ __launch_bounds__(64 * RowGroup<DT>::W) void k_rk4_recompute_rows_multi(const Rk4AdjArgs* As) {
+    extern __shared__ double2 sh[];
+    const Rk4AdjArgs A = As[blockIdx.y];
+    const long sg = blockIdx.x;
+    const long s0 = sg * A.K, s1 = min(s0 + A.K, A.steps);
+    rk4_rows<DT, true>(sh, A.Ls, s0, s1, A.ck + sg * DT, nullptr, A.dt, 1, nullptr, nullptr, A.seg);
+}
+
+__global__ __launch_bounds__(kBlock) void k_rk4_recompute_multi(const Rk4AdjArgs* As) {
+    extern __shared__ double2 sh[];
+    const Rk4AdjArgs A = As[blockIdx.y];
+    const long sg = blockIdx.x;
+    if (A.nstage >= 2) {
+#define TACO_RK4_CALL(DT) rk4_recompute_body<DT, true>(sh, A, sg)
+        TACO_RK4_DISPATCH(A.D, TACO_RK4_CALL)
+#undef TACO_RK4_CALL
+    } else {
+#define TACO_RK4_CALL(DT) rk4_recompute_body<DT, false>(sh, A, sg)
+        TACO_RK4_DISPATCH(A.D, TACO_RK4_CALL)
+#undef TACO_RK4_CALL
+    }
+}
+
+template <int DT>
+__global__ __launch_bounds__(64) void k_rk4_adjoint_rows_multi(const Rk4AdjArgs* As) {
+    extern __shared__ double2 sh[];
+    const Rk4AdjArgs A = As[blockIdx.x];
+    rk4_rows_adjoint<DT>(sh, A);
+}
+
+__global__ __launch_bounds__(kBlock) void k_rk4_adjoint_multi(const Rk4AdjArgs* As) {
+    extern __shared__ double2 sh[];
+    const Rk4AdjArgs A = As[blockIdx.x];
+    if (A.nstage >= 2) {
+#define TACO_RK4_CALL(DT) rk4_adjoint_body<DT, true>(sh, A)
+        TACO_RK4_DISPATCH(A.D, TACO_RK4_CALL)
+#undef TACO_RK4_CALL
+    } else {
+#define TACO_RK4_CALL(DT) rk4_adjoint_body<DT, false>(sh, A)
+        TACO_RK4_DISPATCH(A.D, TACO_RK4_CALL)
+#undef TACO_RK4_CALL
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Fused L4 adjoint, one workgroup per time t.  From Lbar(t) (the RK4 layout of
 // L4(n,m;i,j) = GW(n,i;m,j)) it forms Tbar = GWbar + GWbar^H (GW = T + T^H)

@@ -43,6 +43,7 @@
 #include <vector>
 
 #include "engine_internal.hpp"
+#include "system_prep.hpp"
 
 #pragma clang fp contract(off)
 
@@ -681,6 +682,81 @@ __device__ __forceinline__ void combine_block(const FcrArgs& P, const CombineArg
     if (c.has_conv && (c.mask & 1)) O.Fo[dst] = csub(cmul(g1p, pF), c.convF[t]);
     if (c.has_conv && (c.mask & 2)) O.Co[dst] = csub(cmul(g1p, pC), c.convC[t]);
     if (c.mask & 4) O.Ro[dst] = csub(cmul(gg1, pR), pP);
+}
+
+// Direct method (tcl4_kernels.cpp:188-281), the reference's O(T^2)
+// cross-check of the convolution route: one thread per (triple, t) forms
+// F(t), C(t) and R(t) from their defining sums, each sum taken in the
+// reference's order (m = 0, 1, ..., t) with the reference's grouping of
+// products, so the values are those of the sequential code.
+struct DirectArgs {
+    const double2* gamma;  // [nf][Nt]
+    long Nt;
+    int nf;
+    const double* omegas;
+    const int* mirror;
+    double dt;
+    long T;                // times t < T
+    const int* tri;        // [ntri] triples, ascending
+    const unsigned char* mask;
+    const double2* ph_tab;  // TACO_PHASE_LIBM rows (null: correctly rounded on the device)
+    const int* row;
+    long ph_len;
+};
+
+__global__ __launch_bounds__(kBlock) void k_fcr_direct(DirectArgs D, CombineArgs O) {
+    const long q = blockIdx.x;
+    const long t = static_cast<long>(blockIdx.y) * kBlock + threadIdx.x;
+    if (t >= D.T) return;
+    long dst = -1;
+    const int tri = D.tri[q];
+    if (O.out_mode == 0) {
+        const int pos = O.t2pos[t];
+        if (pos < 0) return;
+        dst = static_cast<long>(pos) * O.nf3 + tri;
+    } else {
+        dst = (tri - O.tri0) * D.T + t;
+    }
+    const int mask = D.mask[q];
+    int i, j, k;
+    decode_triple(tri, D.nf, i, j, k);
+    const double omega = (D.omegas[i] + D.omegas[j]) + D.omegas[k];
+    const double2* prow = D.ph_tab ? D.ph_tab + static_cast<long>(D.row[q]) * D.ph_len : nullptr;
+    const double2* g1 = D.gamma + static_cast<long>(i) * D.Nt;
+    const double2* g2 = D.gamma + static_cast<long>(j) * D.Nt;
+    const double2* g2m = D.gamma + static_cast<long>(D.mirror[j]) * D.Nt;
+    const double dt = D.dt;
+    const double2 pm_t = cconj(phase_plus(prow, t, omega, dt));
+    const double2 z = make_double2(0.0, 0.0);
+    if (mask & 3) {
+        // F: g2 at mirror(j); C: conj(g2) at j (tcl4.cpp:128-150)
+        double2 preF = z, preC = z, secF = z, secC = z;
+        for (long m = 0; m <= t; ++m) {
+            const double2 pp = phase_plus(prow, m, omega, dt);
+            const double2 pmk = cconj(phase_plus(prow, t - m, omega, dt));
+            const double2 a = g1[t - m];
+            if (mask & 1) {
+                preF = cadd(preF, cscale(cmul(g2m[m], pp), dt));
+                secF = cadd(secF, cmul(cmul(a, g2m[m]), pmk));
+            }
+            if (mask & 2) {
+                const double2 b = cconj(g2[m]);
+                preC = cadd(preC, cscale(cmul(b, pp), dt));
+                secC = cadd(secC, cmul(cmul(a, b), pmk));
+            }
+        }
+        if (mask & 1) O.Fo[dst] = csub(cmul(cmul(g1[t], preF), pm_t), cscale(secF, dt));
+        if (mask & 2) O.Co[dst] = csub(cmul(cmul(g1[t], preC), pm_t), cscale(secC, dt));
+    }
+    if (mask & 4) {
+        double2 p1 = z, p2 = z;
+        for (long m = 0; m <= t; ++m) {
+            const double2 pm = cconj(phase_plus(prow, m, omega, dt));
+            p1 = cadd(p1, cscale(cmul(g2[m], pm), dt));
+            p2 = cadd(p2, cscale(cmul(cmul(g1[m], g2[m]), pm), dt));
+        }
+        O.Ro[dst] = csub(cmul(g1[t], p1), p2);
+    }
 }
 
 // One workgroup per triple walks the blocks t0 = 0, 256, ... with the carry.
@@ -2032,7 +2108,7 @@ struct taco_ctx {
     std::map<int, std::array<int, 3>> fft_choice;  // per FFT log-length: tile config of phases A, B, C
     std::map<int, std::unique_ptr<taco_engine::Twiddles>> tw;
     taco_engine::DevBuf gamma, omegas, mirror, p2f, ops, spec, inter1, inter2, conv, slices, t2pos, l4, gw, misc,
-        rk, eps, jobtri, clsstart, jobmask, jobrow, comb;
+        rk, eps, jobtri, clsstart, jobmask, jobrow, comb, t2scr;
     taco_engine::PhaseTab phtab;  // phase table of the last plain-path system (TACO_PHASE_LIBM)
     std::size_t l4_lds_set = 0;  // dynamic LDS limit set on the fused L4 kernel
     std::size_t slice_budget = 0;  // bytes of F/C/R slices per pass of the batch builders (0: default)
@@ -2231,6 +2307,7 @@ struct FcrPlan {
     std::size_t T;
     const std::vector<unsigned char>* need;  // per triple: bit0 F, bit1 C, bit2 R (null: all)
     std::size_t T_fft = 0;  // window mode: rows entering the convolutions (0: T)
+    int method = TACO_FCR_CONVOLUTION;  // TACO_FCR_DIRECT: the O(T^2) sums of k_fcr_direct
 };
 
 // Triples the L4 assembly reads (tcl4_mikx.cpp:101-132 gathers through the
@@ -2653,7 +2730,59 @@ static void fcr_launch(taco_ctx* c, FcrJob& J, const CombineArgs& out) {
     }
 }
 
+// The Direct method over the pass's triples (every one, or the ones the L4
+// assembly reads), phases from the libm table or correctly rounded.
+static void fcr_direct(taco_ctx* c, const FcrPlan& pl, const CombineArgs& out) {
+    const int nf = pl.nf;
+    const long nf3 = static_cast<long>(nf) * nf * nf;
+    std::vector<int> order;
+    std::vector<unsigned char> mask;
+    std::vector<std::uint64_t> key(static_cast<std::size_t>(nf3));
+    for (long t = 0; t < nf3; ++t) {
+        const int k = static_cast<int>(t % nf), j = static_cast<int>((t / nf) % nf), i = static_cast<int>(t / (nf * nf));
+        const double om = (pl.h_omegas[i] + pl.h_omegas[j]) + pl.h_omegas[k];
+        std::memcpy(&key[static_cast<std::size_t>(t)], &om, 8);
+        const unsigned char m = pl.need ? (*pl.need)[static_cast<std::size_t>(t)] : 7;
+        if (m) {
+            order.push_back(static_cast<int>(t));
+            mask.push_back(m);
+        }
+    }
+    if (order.empty() || pl.T == 0) return;
+    const PhaseRows prow = phase_rows(c, pl, order, key, FcrLists{&c->jobtri, &c->jobmask, &c->clsstart, &c->jobrow,
+                                                                 &c->phtab});
+    c->jobtri.ensure(order.size() * sizeof(int));
+    c->jobmask.ensure(mask.size());
+    HIP_OK(hipMemcpyAsync(c->jobtri.p, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipMemcpyAsync(c->jobmask.p, mask.data(), mask.size(), hipMemcpyHostToDevice, c->stream));
+    DirectArgs D{};
+    D.gamma = pl.gamma;
+    D.Nt = static_cast<long>(pl.Nt);
+    D.nf = nf;
+    D.omegas = pl.omegas;
+    D.mirror = pl.mirror;
+    D.dt = pl.dt;
+    D.T = static_cast<long>(pl.T);
+    D.tri = c->jobtri.as<int>();
+    D.mask = c->jobmask.as<unsigned char>();
+    D.ph_tab = prow.tab;
+    D.row = prow.rows;
+    D.ph_len = prow.len;
+    const double T = static_cast<double>(pl.T);
+    timed(c, K_COMBINE, static_cast<double>(order.size()) * 3.0 * T * 16.0, [&] {
+        hipLaunchKernelGGL(k_fcr_direct, dim3(static_cast<unsigned>(order.size()),
+                                              static_cast<unsigned>((pl.T + kBlock - 1) / kBlock)),
+                           dim3(kBlock), 0, c->stream, D, out);
+    });
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(c->stream));  // the host lists go out of scope
+}
+
 static void run_fcr(taco_ctx* c, const FcrPlan& pl, const CombineArgs& out) {
+    if (pl.method == TACO_FCR_DIRECT) {
+        fcr_direct(c, pl, out);
+        return;
+    }
     FcrJob J = fcr_prepare(c, pl, FcrLists{&c->jobtri, &c->jobmask, &c->clsstart, &c->jobrow, &c->phtab});
     fcr_launch(c, J, out);
 }
@@ -2741,8 +2870,8 @@ static void launch_l4(taco_ctx* c, const L4Args& A, long npos) {
 // ---------------------------------------------------------------------------
 static void validate_system(std::size_t Nt, int nf, const int* mirror, const int* p2f, int N, int num_ops,
                             double dt, int method) {
-    if (method != TACO_FCR_CONVOLUTION)
-        throw Error(TACO_ERR_INVALID, "build_TCL4_generator_hip_fused: only FCRMethod::Convolution is supported");
+    if (method != TACO_FCR_CONVOLUTION && method != TACO_FCR_DIRECT)
+        throw Error(TACO_ERR_INVALID, "unknown FCRMethod");
     if (!(dt > 0.0)) throw Error(TACO_ERR_INVALID, "dt must be > 0");
     if (nf <= 0 || Nt == 0) throw Error(TACO_ERR_INVALID, "empty gamma_series");
     if (N <= 0) throw Error(TACO_ERR_INVALID, "map.N must be > 0");
@@ -2813,7 +2942,7 @@ static std::size_t slice_budget(const taco_ctx* c) {
 static TimePlan l4_core(taco_ctx* c, const double2* d_gamma, std::size_t Nt, int nf, const double* d_omegas,
                         const double* h_omegas, const int* d_mirror, const int* d_p2f, const int* h_p2f, int N,
                         const double2* d_ops, int num_ops, double dt, const std::size_t* tidx, std::size_t ntau,
-                        double* fcr_ms) {
+                        double* fcr_ms, int method = TACO_FCR_CONVOLUTION) {
     TimePlan tp = plan_times(tidx, ntau, Nt);
     const long npos = static_cast<long>(tp.uniq.size());
     if (npos == 0) return tp;
@@ -2859,7 +2988,7 @@ static TimePlan l4_core(taco_ctx* c, const double2* d_gamma, std::size_t Nt, int
         if (fcr_ms) HIP_OK(hipEventRecord(c->ev0, c->stream));
         if (env_int("TACO_DEBUG_POISON", 0))  // NaN-fill the slices: an entry L4 reads but nobody wrote shows up
             HIP_OK(hipMemsetAsync(c->slices.p, 0xff, per_pos * static_cast<std::size_t>(np), c->stream));
-        FcrPlan pl{d_gamma, Nt, nf, d_omegas, h_omegas, d_mirror, dt, Tw, &need, tp.T};
+        FcrPlan pl{d_gamma, Nt, nf, d_omegas, h_omegas, d_mirror, dt, Tw, &need, tp.T, method};
         run_fcr(c, pl, out);
         if (fcr_ms) HIP_OK(hipEventRecord(c->ev1, c->stream));
         L4Args A{};
@@ -3034,6 +3163,7 @@ struct taco_tape {
                         &jobmask = bufs.jobmask, &clsstart = bufs.clsstart, &slices = bufs.slices, &lgen = bufs.lgen,
                         &ck = bufs.ck;
     taco_engine::FcrJob fcr;
+    bool fcr_ready = false;  // fcr prepared (job lists and phase table on the device)
 };
 
 namespace taco_engine {
@@ -3101,13 +3231,13 @@ static void set_fn_lds(const void* fn, std::size_t lds) {
         HIP_OK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
 }
 
-static void tape_forward(taco_tape& tp, const double* r0, double* samples_out) {
+// F/C/R slices of every t < T into sl ([3][T][nf3]): the tape's pass,
+// prepared on its first call and only relaunched after it.
+static void tape_slices(taco_tape& tp, double2* sl) {
     taco_ctx* c = tp.ctx;
-    const std::size_t N2 = static_cast<std::size_t>(tp.N) * tp.N, D = N2, N4 = N2 * N2;
     const std::size_t T = tp.T;
     const std::size_t nf3 = static_cast<std::size_t>(tp.nf) * tp.nf * tp.nf;
-    tp.lgen.ensure(T * N4 * 16);
-    if (tp.order == 4) {
+    if (!tp.fcr_ready) {
         std::vector<int> t2pos(T);
         std::iota(t2pos.begin(), t2pos.end(), 0);
         tp.t2pos.ensure(T * sizeof(int));
@@ -3116,22 +3246,40 @@ static void tape_forward(taco_tape& tp, const double* r0, double* samples_out) {
         FcrPlan pl{tp.gamma.as<double2>(), tp.Nt, tp.nf, tp.omegas.as<double>(), tp.h_omegas.data(),
                    tp.mirror.as<int>(), tp.dt, T, &need};
         tp.fcr = fcr_prepare(c, pl, FcrLists{&tp.jobtri, &tp.jobmask, &tp.clsstart, &tp.bufs.jobrow, &tp.bufs.phtab});
-        if (T > (~std::size_t{0} / 48) / nf3) throw Error(TACO_ERR_OVERFLOW, "FCR buffer too large");
-        tp.slices.ensure(3 * nf3 * T * 16);
-        CombineArgs out{};
-        out.out_mode = 0;
-        out.t2pos = tp.t2pos.as<int>();
-        out.npos = static_cast<long>(T);
-        out.nf3 = static_cast<long>(nf3);
-        double2* sl = tp.slices.as<double2>();
-        out.Fo = sl;
-        out.Co = sl + nf3 * T;
-        out.Ro = sl + 2 * nf3 * T;
-        fcr_launch(c, tp.fcr, out);
+        tp.fcr_ready = true;
+    }
+    CombineArgs out{};
+    out.out_mode = 0;
+    out.t2pos = tp.t2pos.as<int>();
+    out.npos = static_cast<long>(T);
+    out.nf3 = static_cast<long>(nf3);
+    out.Fo = sl;
+    out.Co = sl + nf3 * T;
+    out.Ro = sl + 2 * nf3 * T;
+    fcr_launch(c, tp.fcr, out);
+}
+
+static std::size_t tape_slice_bytes(const taco_tape& tp) {
+    const std::size_t nf3 = static_cast<std::size_t>(tp.nf) * tp.nf * tp.nf;
+    if (tp.order != 4) return 0;
+    if (nf3 > 0 && tp.T > (~std::size_t{0} / 48) / nf3) throw Error(TACO_ERR_OVERFLOW, "FCR buffer too large");
+    return 3 * nf3 * tp.T * 16;
+}
+
+// Generators L(t) = L_TCL2(t) (+ L4(t)) of every t < T into tp.lgen, with
+// the F/C/R slices written to sl.
+static void tape_generator(taco_tape& tp, double2* sl) {
+    taco_ctx* c = tp.ctx;
+    const std::size_t N2 = static_cast<std::size_t>(tp.N) * tp.N, N4 = N2 * N2;
+    const std::size_t T = tp.T;
+    const std::size_t nf3 = static_cast<std::size_t>(tp.nf) * tp.nf * tp.nf;
+    tp.lgen.ensure(T * N4 * 16);
+    if (tp.order == 4) {
+        tape_slices(tp, sl);
         L4Args A{};
-        A.Fs = out.Fo;
-        A.Cs = out.Co;
-        A.Rs = out.Ro;
+        A.Fs = sl;
+        A.Cs = sl + nf3 * T;
+        A.Rs = sl + 2 * nf3 * T;
         A.p2f = tp.p2f.as<int>();
         A.ops = tp.ops.as<double2>();
         A.N = tp.N;
@@ -3160,12 +3308,12 @@ static void tape_forward(taco_tape& tp, const double* r0, double* samples_out) {
     A2.order = tp.order;
     hipLaunchKernelGGL(k_tcl2, dim3(static_cast<unsigned>(T)), dim3(kBlock), tcl2_lds(A2.N), c->stream, A2);
     HIP_OK(hipGetLastError());
-    // RK4 with every state sampled; checkpoints r_{qK} kept on the tape.
-    c->misc.ensure((T + 1) * D * 16 + 64);
-    double2* dr = c->misc.as<double2>();
-    double2* ds = dr + D;
-    HIP_OK(hipMemcpyAsync(dr, r0, D * 16, hipMemcpyHostToDevice, c->stream));
-    rk4_launch(c, tp.lgen.as<double2>(), nullptr, tp.steps, static_cast<int>(D), dr, tp.dt, 1, ds, nullptr);
+}
+
+// Checkpoints r_{qK} of the sampled states ds ([T][D]) onto the tape.
+static void tape_checkpoints(taco_tape& tp, const double2* ds) {
+    taco_ctx* c = tp.ctx;
+    const std::size_t D = static_cast<std::size_t>(tp.N) * tp.N;
     const std::size_t nseg = (tp.steps + tp.K - 1) / tp.K;
     std::vector<int> rows(nseg);
     for (std::size_t q = 0; q < nseg; ++q) rows[q] = static_cast<int>(q * tp.K);
@@ -3175,6 +3323,22 @@ static void tape_forward(taco_tape& tp, const double* r0, double* samples_out) {
     hipLaunchKernelGGL(k_gather_rows, dim3(static_cast<unsigned>(nseg)), dim3(kBlock), 0, c->stream, ds, drows,
                        static_cast<long>(D), tp.ck.as<double2>());
     HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(c->stream));  // the host row list goes out of scope
+}
+
+static void tape_forward(taco_tape& tp, const double* r0, double* samples_out) {
+    taco_ctx* c = tp.ctx;
+    const std::size_t D = static_cast<std::size_t>(tp.N) * tp.N;
+    const std::size_t T = tp.T;
+    tp.slices.ensure(std::max<std::size_t>(tape_slice_bytes(tp), 16));
+    tape_generator(tp, tp.slices.as<double2>());
+    // RK4 with every state sampled; checkpoints r_{qK} kept on the tape.
+    c->misc.ensure((T + 1) * D * 16 + 64);
+    double2* dr = c->misc.as<double2>();
+    double2* ds = dr + D;
+    HIP_OK(hipMemcpyAsync(dr, r0, D * 16, hipMemcpyHostToDevice, c->stream));
+    rk4_launch(c, tp.lgen.as<double2>(), nullptr, tp.steps, static_cast<int>(D), dr, tp.dt, 1, ds, nullptr);
+    tape_checkpoints(tp, ds);
     if (samples_out) HIP_OK(hipMemcpyAsync(samples_out, ds, T * D * 16, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
 }
@@ -3229,15 +3393,133 @@ static L4AdjLists l4_adjoint_lists(const int* p2f, int N, int nf) {
     return L;
 }
 
+// Reverse RK4 state of one tape: dLoss/dr_n (gb, [T][D]), the generator
+// cotangents (Lb, [T][D^2]), the recomputed states and stage points of every
+// step (seg, [steps][4][D]) and dLoss/dr_0 (rb, [D]).
+struct RkRev {
+    double2 *gb, *Lb, *seg, *rb;
+};
+
+static std::size_t rk_rev_values(const taco_tape& tp) {
+    const std::size_t D = static_cast<std::size_t>(tp.N) * tp.N;
+    return tp.T * D + tp.T * D * D + 4 * tp.steps * D + D;
+}
+
+static RkRev rk_rev_at(const taco_tape& tp, double2* base) {
+    const std::size_t D = static_cast<std::size_t>(tp.N) * tp.N;
+    RkRev b{};
+    b.gb = base;
+    b.Lb = b.gb + tp.T * D;
+    b.seg = b.Lb + tp.T * D * D;
+    b.rb = b.seg + 4 * tp.steps * D;
+    return b;
+}
+
+// Launch shape of the reverse RK4 kernels of a tape (the same for every
+// tape of a D, steps and K): recompute (Rf) and walk (R) arguments, their
+// LDS, and whether the walk keeps the next step's L0 part in registers
+// (carry) instead of accumulating Lbar in place over a zeroed buffer.
+struct RkRevShape {
+    Rk4AdjArgs Rf, R;
+    std::size_t lds_f = 0, lds_b = 0, nseg = 0;
+    bool carry = false, rows_f = false, rows_b = false;
+    double bytes = 0.0;
+};
+
+static RkRevShape rk_rev_shape(const taco_tape& tp, const RkRev& b) {
+    const std::size_t D = static_cast<std::size_t>(tp.N) * tp.N, D2 = D * D;
+    const std::size_t K = static_cast<std::size_t>(tp.K), steps = tp.steps;
+    RkRevShape sh;
+    Rk4AdjArgs R{};
+    R.Ls = tp.lgen.as<double2>();
+    R.ck = tp.ck.as<double2>();
+    R.gbar = b.gb;
+    R.Lbar = b.Lb;
+    R.seg = b.seg;
+    R.r0bar = b.rb;
+    R.steps = static_cast<long>(steps);
+    R.D = static_cast<int>(D);
+    R.K = tp.K;
+    R.dt = tp.dt;
+    sh.carry = D2 <= static_cast<std::size_t>(kRkCarry) * kBlock;
+    sh.nseg = (steps + K - 1) / K;
+    const int Di = static_cast<int>(D);
+    sh.Rf = R;
+    sh.Rf.nstage = rk4_nstage(Di, 6, static_cast<long>(K) + 1);
+    sh.lds_f = rk4_lds(Di, 6, sh.Rf.nstage);
+    sh.R = R;
+    sh.R.nstage = rk4_nstage(Di, 12, static_cast<long>(steps) + 1);
+    sh.lds_b = rk4_lds(Di, 12, sh.R.nstage);
+    if (sh.lds_b > kRkLds || sh.lds_f > kRkLds) throw Error(TACO_ERR_INVALID, "adjoint RK4 kernel supports D <= 853");
+    sh.rows_f = rk4_rows_threads(Di) > 0;
+    sh.rows_b = Di == 4 || Di == 9 || Di == 16;
+    // algorithmic bytes of the recompute and the walk: generators read twice,
+    // stage points written and read, gbar read, Lbar written
+    sh.bytes = 2.0 * static_cast<double>(steps + 1) * D2 * 16.0 + 8.0 * steps * D * 16.0 +
+               static_cast<double>(tp.T) * (D + D2) * 16.0;
+    return sh;
+}
+
+// Reverse RK4 of one tape: gb filled, Lb and rb written.
+static void rk_rev_run(taco_tape& tp, const RkRev& b) {
+    taco_ctx* c = tp.ctx;
+    const std::size_t D = static_cast<std::size_t>(tp.N) * tp.N, D2 = D * D;
+    const RkRevShape sh = rk_rev_shape(tp, b);
+    if (!sh.carry) {
+        const long cntL = static_cast<long>(tp.T * D2);
+        hipLaunchKernelGGL(k_fill_zero, dim3(static_cast<unsigned>((cntL + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                           c->stream, b.Lb, cntL);
+    }
+    const int Di = static_cast<int>(D);
+    timed(c, K_RK4, sh.bytes, [&] {
+        // every segment's states and stage points from its checkpoint, all segments at once
+        if (sh.rows_f) {
+            rows_dispatch(Di, [&](auto dc) {
+                constexpr int DT = decltype(dc)::value;
+                hipLaunchKernelGGL(k_rk4_recompute_rows<DT>, dim3(static_cast<unsigned>(sh.nseg)),
+                                   dim3(rk4_rows_threads(DT)), 3 * DT * 16, c->stream, sh.Rf);
+            });
+        } else {
+            hipLaunchKernelGGL(k_rk4_recompute, dim3(static_cast<unsigned>(sh.nseg)), dim3(kBlock), sh.lds_f,
+                               c->stream, sh.Rf);
+        }
+        if (sh.rows_b) {
+            rows_dispatch(Di, [&](auto dc) {
+                constexpr int DT = decltype(dc)::value;
+                if constexpr (DT <= 16)
+                    hipLaunchKernelGGL(k_rk4_adjoint_rows<DT>, dim3(1), dim3(64), 5 * DT * 16, c->stream, sh.R);
+            });
+        } else {
+            hipLaunchKernelGGL(k_rk4_adjoint, dim3(1), dim3(kBlock), sh.lds_b, c->stream, sh.R);
+        }
+    });
+    HIP_OK(hipGetLastError());
+}
+
+static void tape_adjoint_tail(taco_tape& tp, const RkRev& rv, const double2* sl, double* gamma_bar,
+                              double* omega_bar, double* ops_bar, double* eps_bar, double* r0_bar);
+
 static void tape_adjoint(taco_tape& tp, const double* samples_bar, double* gamma_bar, double* omega_bar,
                          double* ops_bar, double* eps_bar, double* r0_bar) {
+    taco_ctx* c = tp.ctx;
+    const std::size_t D = static_cast<std::size_t>(tp.N) * tp.N;
+    c->rk.ensure(rk_rev_values(tp) * 16);
+    const RkRev rv = rk_rev_at(tp, c->rk.as<double2>());
+    HIP_OK(hipMemcpyAsync(rv.gb, samples_bar, tp.T * D * 16, hipMemcpyHostToDevice, c->stream));
+    rk_rev_run(tp, rv);
+    tape_adjoint_tail(tp, rv, tp.slices.as<double2>(), gamma_bar, omega_bar, ops_bar, eps_bar, r0_bar);
+}
+
+// Everything after the reverse RK4: the L4 reverse over the slices sl, the
+// F/C/R and TCL2 reverses, and the cotangents to the host.
+static void tape_adjoint_tail(taco_tape& tp, const RkRev& rv, const double2* sl, double* gamma_bar,
+                              double* omega_bar, double* ops_bar, double* eps_bar, double* r0_bar) {
     taco_ctx* c = tp.ctx;
     const std::size_t N2 = static_cast<std::size_t>(tp.N) * tp.N, D = N2, D2 = D * D;
     const std::size_t T = tp.T;
     const std::size_t nf3 = static_cast<std::size_t>(tp.nf) * tp.nf * tp.nf;
-    const std::size_t K = static_cast<std::size_t>(tp.K);
-    // device layout in the context's rk buffer: gbar, Lbar, seg, r0bar
-    const std::size_t steps = tp.steps;
+    double2* Lb = rv.Lb;
+    double2* rb = rv.rb;
     // TCL2 reverse: per-(operator, bucket) parts and the device time sums
     const Tcl2Parts parts = tcl2_parts(tp.h_ops.data(), tp.num_ops, tp.N, tp.h_p2f.data(), tp.nf);
     const std::size_t npart = parts.al.size();
@@ -3246,12 +3528,8 @@ static void tape_adjoint(taco_tape& tp, const double* samples_bar, double* gamma
     const int nchunk = static_cast<int>(std::max<std::size_t>(1, std::min<std::size_t>(64, (std::size_t{1} << 22) / nout)));
     const std::size_t n_tcl2 = T * N2 + (T * tp.N + 1) / 2 + nfz * T + nchunk * nout + nout + npart * (D2 + N2) +
                                (npart + 1) / 4 + 1;
-    c->rk.ensure((T * D + T * D2 + 4 * steps * D + D + n_tcl2) * 16);
-    double2* gb = c->rk.as<double2>();
-    double2* Lb = gb + T * D;
-    double2* seg = Lb + T * D2;
-    double2* rb = seg + 4 * steps * D;
-    double2* t2X = rb + D;
+    c->t2scr.ensure(n_tcl2 * 16);
+    double2* t2X = c->t2scr.as<double2>();
     double* t2eps = reinterpret_cast<double*>(t2X + T * N2);
     double2* t2g = t2X + T * N2 + (T * tp.N + 1) / 2;
     double2* t2part = t2g + nfz * T;
@@ -3264,57 +3542,6 @@ static void tape_adjoint(taco_tape& tp, const double* samples_bar, double* gamma
         HIP_OK(hipMemcpyAsync(t2A, parts.AdA.data(), npart * N2 * 16, hipMemcpyHostToDevice, c->stream));
         HIP_OK(hipMemcpyAsync(t2b, parts.b.data(), npart * sizeof(int), hipMemcpyHostToDevice, c->stream));
     }
-    HIP_OK(hipMemcpyAsync(gb, samples_bar, T * D * 16, hipMemcpyHostToDevice, c->stream));
-    Rk4AdjArgs R{};
-    R.Ls = tp.lgen.as<double2>();
-    R.ck = tp.ck.as<double2>();
-    R.gbar = gb;
-    R.Lbar = Lb;
-    R.seg = seg;
-    R.r0bar = rb;
-    R.steps = static_cast<long>(steps);
-    R.D = static_cast<int>(D);
-    R.K = tp.K;
-    R.dt = tp.dt;
-    const bool carry = D2 <= static_cast<std::size_t>(kRkCarry) * kBlock;
-    if (!carry) {
-        const long cntL = static_cast<long>(T * D2);
-        hipLaunchKernelGGL(k_fill_zero, dim3(static_cast<unsigned>((cntL + kBlock - 1) / kBlock)), dim3(kBlock), 0,
-                           c->stream, Lb, cntL);
-    }
-    const std::size_t nseg = (steps + K - 1) / K;
-    // row-group kernels where D has them (no generator staging), else the general ones
-    const int Di = static_cast<int>(D);
-    Rk4AdjArgs Rf = R;
-    Rf.nstage = rk4_nstage(Di, 6, static_cast<long>(K) + 1);
-    const std::size_t lds_f = rk4_lds(Di, 6, Rf.nstage);
-    R.nstage = rk4_nstage(Di, 12, static_cast<long>(steps) + 1);
-    const std::size_t lds_b = rk4_lds(Di, 12, R.nstage);
-    if (lds_b > kRkLds || lds_f > kRkLds) throw Error(TACO_ERR_INVALID, "adjoint RK4 kernel supports D <= 853");
-    // algorithmic bytes of the recompute and the walk: generators read twice,
-    // stage points written and read, gbar read, Lbar written
-    const double rev_bytes = 2.0 * static_cast<double>(steps + 1) * D2 * 16.0 + 8.0 * steps * D * 16.0 +
-                             static_cast<double>(T) * (D + D2) * 16.0;
-    timed(c, K_RK4, rev_bytes, [&] {
-        // every segment's states and stage points from its checkpoint, all segments at once
-        const bool rows_f = rows_dispatch(Di, [&](auto dc) {
-            constexpr int DT = decltype(dc)::value;
-            hipLaunchKernelGGL(k_rk4_recompute_rows<DT>, dim3(static_cast<unsigned>(nseg)), dim3(rk4_rows_threads(DT)),
-                               3 * DT * 16, c->stream, Rf);
-        });
-        if (!rows_f)
-            hipLaunchKernelGGL(k_rk4_recompute, dim3(static_cast<unsigned>(nseg)), dim3(kBlock), lds_f, c->stream, Rf);
-        if (Di == 4 || Di == 9 || Di == 16) {
-            rows_dispatch(Di, [&](auto dc) {
-                constexpr int DT = decltype(dc)::value;
-                if constexpr (DT <= 16)
-                    hipLaunchKernelGGL(k_rk4_adjoint_rows<DT>, dim3(1), dim3(64), 5 * DT * 16, c->stream, R);
-            });
-        } else {
-            hipLaunchKernelGGL(k_rk4_adjoint, dim3(1), dim3(kBlock), lds_b, c->stream, R);
-        }
-    });
-    HIP_OK(hipGetLastError());
     std::vector<std::complex<double>> gbar_h(static_cast<std::size_t>(tp.nf) * T, {0.0, 0.0});
     std::vector<std::complex<double>> opsb(static_cast<std::size_t>(tp.num_ops) * N2, {0.0, 0.0});
     std::vector<double> omb(static_cast<std::size_t>(tp.nf), 0.0), epsb(static_cast<std::size_t>(tp.N), 0.0);
@@ -3358,7 +3585,7 @@ static void tape_adjoint(taco_tape& tp, const double* samples_bar, double* gamma
                                0, c->stream, sbar, static_cast<long>(sbar_n));
         L4AdjArgs A{};
         A.Lbar = Lb;
-        A.Fs = tp.slices.as<double2>();
+        A.Fs = sl;
         A.Cs = A.Fs + nf3 * T;
         A.Rs = A.Fs + 2 * nf3 * T;
         A.p2f = tp.p2f.as<int>();
@@ -3659,6 +3886,294 @@ static void trapz_device(taco_ctx* c, const double2* dC, std::size_t Nc, double 
 }
 }  // namespace taco_engine
 
+namespace taco_engine {
+
+// Validates one tape's inputs and uploads them (the tape keeps host copies of
+// what its reverse pass reads on the host).
+static std::unique_ptr<taco_tape> tape_make(taco_ctx* ctx, const taco_tape_set& in, std::size_t checkpoint_every) {
+    const std::size_t Nt = in.Nt, steps = in.steps;
+    const int nf = in.nf, N = in.N, num_ops = in.num_ops;
+    if (!(in.order == 0 || in.order == 2 || in.order == 4)) throw Error(TACO_ERR_INVALID, "order must be 0, 2, or 4");
+    validate_system(Nt, nf, in.mirror, in.pair_to_freq, N, num_ops, in.dt, TACO_FCR_CONVOLUTION);
+    if (!in.gamma || !in.omegas || !in.mirror || !in.pair_to_freq || !in.ops || !in.eps || !in.r0)
+        throw Error(TACO_ERR_INVALID, "tape inputs must be non-null");
+    if (steps < 1) throw Error(TACO_ERR_INVALID, "steps must be >= 1");
+    if (steps + 1 > Nt) throw Error(TACO_ERR_RANGE, "steps exceed gamma_series rows");
+    if (checkpoint_every < 1) throw Error(TACO_ERR_INVALID, "checkpoint_every must be >= 1");
+    auto tp = std::make_unique<taco_tape>();
+    tp->ctx = ctx;
+    if (!ctx->tape_pool.empty()) {
+        tp->bufs.swap(*ctx->tape_pool.back());
+        ctx->tape_pool.pop_back();
+    }
+    tp->Nt = Nt;
+    tp->T = steps + 1;
+    tp->steps = steps;
+    tp->nf = nf;
+    tp->N = N;
+    tp->num_ops = num_ops;
+    tp->order = in.order;
+    tp->K = static_cast<int>(std::min<std::size_t>(checkpoint_every, steps));
+    tp->dt = in.dt;
+    const std::size_t N2 = static_cast<std::size_t>(N) * N;
+    const auto* gc = reinterpret_cast<const std::complex<double>*>(in.gamma);
+    tp->h_gamma.resize(static_cast<std::size_t>(nf) * tp->T);
+    for (int b = 0; b < nf; ++b)
+        std::copy(gc + static_cast<std::size_t>(b) * Nt, gc + static_cast<std::size_t>(b) * Nt + tp->T,
+                  tp->h_gamma.begin() + static_cast<std::size_t>(b) * tp->T);
+    tp->h_eps.assign(in.eps, in.eps + N);
+    tp->h_omegas.assign(in.omegas, in.omegas + nf);
+    tp->h_p2f.assign(in.pair_to_freq, in.pair_to_freq + N2);
+    tp->h_mirror.assign(in.mirror, in.mirror + nf);
+    const auto* oc = reinterpret_cast<const std::complex<double>*>(in.ops);
+    tp->h_ops.assign(oc, oc + N2 * num_ops);
+    auto put = [&](DevBuf& buf, const void* host, std::size_t bytes) {
+        buf.ensure(std::max<std::size_t>(bytes, 16));
+        HIP_OK(hipMemcpy(buf.p, host, bytes, hipMemcpyHostToDevice));
+    };
+    put(tp->gamma, in.gamma, Nt * nf * 16);
+    put(tp->omegas, in.omegas, nf * sizeof(double));
+    put(tp->mirror, in.mirror, nf * sizeof(int));
+    put(tp->p2f, in.pair_to_freq, N2 * sizeof(int));
+    put(tp->ops, in.ops, N2 * num_ops * 16);
+    put(tp->eps, in.eps, N * sizeof(double));
+    return tp;
+}
+
+}  // namespace taco_engine
+
+// Several tapes of one N, step count and checkpoint interval (parameter
+// sets of one model): each set's F/C/R, L4 and TCL2 passes run in turn, the
+// forward RK4 of every set is one launch (a workgroup per set), and so are
+// the reverse RK4's recompute and walk.  The sets' F/C/R slices stay on the
+// device while they fit the context's slice budget; otherwise one shared
+// buffer holds one set's slices at a time and the reverse pass recomputes
+// them (the same pass, the same values).
+struct taco_mtape {
+    taco_ctx* ctx = nullptr;
+    std::vector<std::unique_ptr<taco_tape>> sets;
+    bool keep = true;
+    taco_engine::DevBuf shared;  // one set's slices when !keep
+    taco_engine::DevBuf samp;    // [S][T][D] sampled states
+    taco_engine::DevBuf rv;      // [S][rk_rev_values] reverse RK4 state
+    taco_engine::DevBuf args;    // launch arguments
+};
+
+namespace taco_engine {
+
+static double2* mtape_slices(taco_mtape& mt, std::size_t s) {
+    taco_tape& tp = *mt.sets[s];
+    return mt.keep ? tp.slices.as<double2>() : mt.shared.as<double2>();
+}
+
+static void mtape_forward(taco_mtape& mt, const taco_tape_set* in, double* const* samples_out) {
+    taco_ctx* c = mt.ctx;
+    const std::size_t S = mt.sets.size();
+    const taco_tape& t0 = *mt.sets[0];
+    const std::size_t D = static_cast<std::size_t>(t0.N) * t0.N, T = t0.T;
+    std::size_t total = 0, most = 0;
+    for (auto& tp : mt.sets) {
+        const std::size_t b = tape_slice_bytes(*tp);
+        total += b;
+        most = std::max(most, b);
+    }
+    mt.keep = total <= slice_budget(c);
+    if (mt.keep) {
+        for (auto& tp : mt.sets) tp->slices.ensure(std::max<std::size_t>(tape_slice_bytes(*tp), 16));
+    } else {
+        mt.shared.ensure(std::max<std::size_t>(most, 16));
+    }
+    for (std::size_t s = 0; s < S; ++s) tape_generator(*mt.sets[s], mtape_slices(mt, s));
+    // one RK4 launch over every set: r0 in, samples out ([S][T][D]), final r after them
+    mt.samp.ensure((S * T + 2 * S) * D * 16);
+    double2* samp = mt.samp.as<double2>();
+    double2* r0d = samp + S * T * D;
+    double2* rend = r0d + S * D;
+    for (std::size_t s = 0; s < S; ++s)
+        HIP_OK(hipMemcpyAsync(r0d + s * D, in[s].r0, D * 16, hipMemcpyHostToDevice, c->stream));
+    const int Di = static_cast<int>(D);
+    const int rows_thr = rk4_rows_threads(Di);
+    const int nstage = rows_thr > 0 ? 0 : rk4_nstage(Di, 6, static_cast<long>(t0.steps) + 1);
+    const std::size_t lds = rows_thr > 0 ? static_cast<std::size_t>(3) * D * 16 : rk4_lds(Di, 6, nstage);
+    if (lds > kRkLds) throw Error(TACO_ERR_INVALID, "RK4 kernel supports D <= 1706");
+    std::vector<Rk4Sys> sys(S);
+    for (std::size_t s = 0; s < S; ++s) {
+        const taco_tape& tp = *mt.sets[s];
+        sys[s] = Rk4Sys{tp.lgen.as<double2>(), nstage, r0d + s * D, rend + s * D, samp + s * T * D,
+                        static_cast<long>(tp.steps), tp.dt, Di};
+    }
+    mt.args.ensure(std::max(S * sizeof(Rk4Sys), 2 * S * sizeof(Rk4AdjArgs)));
+    HIP_OK(hipMemcpyAsync(mt.args.p, sys.data(), S * sizeof(Rk4Sys), hipMemcpyHostToDevice, c->stream));
+    const double rk_bytes = static_cast<double>(S) * ((t0.steps + 1) * D * D * 16.0 + (t0.steps + 1) * D * 16.0);
+    timed(c, K_RK4, rk_bytes, [&] {
+        hipLaunchKernelGGL(k_rk4_batch, dim3(static_cast<unsigned>(S)), dim3(rows_thr > 0 ? rows_thr : rk4_block(Di)),
+                           lds, c->stream, mt.args.as<const Rk4Sys>());
+    });
+    HIP_OK(hipGetLastError());
+    for (std::size_t s = 0; s < S; ++s) tape_checkpoints(*mt.sets[s], samp + s * T * D);
+    for (std::size_t s = 0; s < S; ++s)
+        if (samples_out && samples_out[s])
+            HIP_OK(hipMemcpyAsync(samples_out[s], samp + s * T * D, T * D * 16, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+}
+
+static void mtape_adjoint(taco_mtape& mt, const double* const* samples_bar, double* const* gamma_bar,
+                          double* const* omega_bar, double* const* ops_bar, double* const* eps_bar,
+                          double* const* r0_bar) {
+    taco_ctx* c = mt.ctx;
+    const std::size_t S = mt.sets.size();
+    const taco_tape& t0 = *mt.sets[0];
+    const std::size_t D = static_cast<std::size_t>(t0.N) * t0.N, D2 = D * D, T = t0.T;
+    const std::size_t per = rk_rev_values(t0);
+    mt.rv.ensure(S * per * 16);
+    std::vector<RkRev> rv(S);
+    std::vector<Rk4AdjArgs> args(2 * S);
+    RkRevShape sh{};
+    for (std::size_t s = 0; s < S; ++s) {
+        rv[s] = rk_rev_at(*mt.sets[s], mt.rv.as<double2>() + s * per);
+        HIP_OK(hipMemcpyAsync(rv[s].gb, samples_bar[s], T * D * 16, hipMemcpyHostToDevice, c->stream));
+        sh = rk_rev_shape(*mt.sets[s], rv[s]);
+        args[s] = sh.Rf;
+        args[S + s] = sh.R;
+        if (!sh.carry) {
+            const long cntL = static_cast<long>(T * D2);
+            hipLaunchKernelGGL(k_fill_zero, dim3(static_cast<unsigned>((cntL + kBlock - 1) / kBlock)), dim3(kBlock),
+                               0, c->stream, rv[s].Lb, cntL);
+        }
+    }
+    mt.args.ensure(std::max(S * sizeof(Rk4Sys), 2 * S * sizeof(Rk4AdjArgs)));
+    const Rk4AdjArgs* dargs = mt.args.as<const Rk4AdjArgs>();
+    HIP_OK(hipMemcpyAsync(mt.args.p, args.data(), 2 * S * sizeof(Rk4AdjArgs), hipMemcpyHostToDevice, c->stream));
+    const int Di = static_cast<int>(D);
+    timed(c, K_RK4, sh.bytes * static_cast<double>(S), [&] {
+        if (sh.rows_f) {
+            rows_dispatch(Di, [&](auto dc) {
+                constexpr int DT = decltype(dc)::value;
+                hipLaunchKernelGGL(k_rk4_recompute_rows_multi<DT>,
+                                   dim3(static_cast<unsigned>(sh.nseg), static_cast<unsigned>(S)),
+                                   dim3(rk4_rows_threads(DT)), 3 * DT * 16, c->stream, dargs);
+            });
+        } else {
+            hipLaunchKernelGGL(k_rk4_recompute_multi, dim3(static_cast<unsigned>(sh.nseg), static_cast<unsigned>(S)),
+                               dim3(kBlock), sh.lds_f, c->stream, dargs);
+        }
+        if (sh.rows_b) {
+            rows_dispatch(Di, [&](auto dc) {
+                constexpr int DT = decltype(dc)::value;
+                if constexpr (DT <= 16)
+                    hipLaunchKernelGGL(k_rk4_adjoint_rows_multi<DT>, dim3(static_cast<unsigned>(S)), dim3(64),
+                                       5 * DT * 16, c->stream, dargs + S);
+            });
+        } else {
+            hipLaunchKernelGGL(k_rk4_adjoint_multi, dim3(static_cast<unsigned>(S)), dim3(kBlock), sh.lds_b, c->stream,
+                               dargs + S);
+        }
+    });
+    HIP_OK(hipGetLastError());
+    for (std::size_t s = 0; s < S; ++s) {
+        taco_tape& tp = *mt.sets[s];
+        if (!mt.keep && tp.order == 4) tape_slices(tp, mt.shared.as<double2>());
+        auto pick = [&](double* const* v) { return v ? v[s] : nullptr; };
+        tape_adjoint_tail(tp, rv[s], mtape_slices(mt, s), pick(gamma_bar), pick(omega_bar), pick(ops_bar),
+                          pick(eps_bar), pick(r0_bar));
+    }
+}
+
+}  // namespace taco_engine
+
+namespace taco_engine {
+// ---------------------------------------------------------------------------
+// System preparation on the device (system.hpp:37-157, tcl4.cpp:40-101): a
+// workgroup per Hamiltonian runs the Jacobi sweeps of system_prep.hpp with
+// each rotation's independent row and column updates spread over its
+// threads, then the ordering, the gauge, the buckets with mirror and zero
+// bucket, and A_eig = U^H A U; the host builder runs the same steps, so the
+// results are equal bit for bit.
+// ---------------------------------------------------------------------------
+constexpr int kSysMaxN = 64;
+
+struct SysArgs {
+    const double2* H;     // [nsys][N*N] column-major
+    const double2* Alab;  // [nsys][num_ops][N*N]
+    int N, num_ops;
+    double tol;
+    double* eps;          // [nsys][N]
+    double2* U;           // [nsys][N*N]
+    double2* Aeig;        // [nsys][num_ops][N*N]
+    int* nf;              // [nsys]
+    double* omegas;       // [nsys][N*N]
+    int* mirror;          // [nsys][N*N]
+    int* p2f;             // [nsys][N*N]
+    int* zero;            // [nsys]
+    double* rep;          // scratch [nsys][N*N]
+    int* idx;             // scratch [nsys][3][N*N]
+};
+
+__global__ __launch_bounds__(kBlock) void k_system_build(SysArgs S) {
+    using namespace taco_sys;
+    extern __shared__ double2 shs[];
+    __shared__ int flag;
+    __shared__ int order[kSysMaxN];
+    const int N = S.N;
+    const long NN = static_cast<long>(N) * N;
+    const long sys = blockIdx.x;
+    Cx* A = reinterpret_cast<Cx*>(shs);
+    Cx* V = A + NN;
+    const double2* H = S.H + sys * NN;
+    for (long e = threadIdx.x; e < NN; e += blockDim.x) {
+        A[e] = cx(H[e].x, H[e].y);
+        V[e] = cx((e % N) == (e / N) ? 1.0 : 0.0, 0.0);
+    }
+    __syncthreads();
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        if (threadIdx.x == 0) flag = converged(A, N) ? 1 : 0;
+        __syncthreads();
+        if (flag) break;
+        for (int p = 0; p < N; ++p)
+            for (int q = p + 1; q < N; ++q) {
+                const Rot g = rotation(A, N, p, q);  // the same inputs in every thread
+                __syncthreads();
+                if (g.skip) continue;
+                for (int r = threadIdx.x; r < N; r += blockDim.x) rot_cols(A, N, p, q, g, r);
+                __syncthreads();
+                for (int c = threadIdx.x; c < N; c += blockDim.x) rot_rows(A, N, p, q, g, c);
+                __syncthreads();
+                if (threadIdx.x == 0) rot_fix(A, N, p, q);
+                for (int r = threadIdx.x; r < N; r += blockDim.x) rot_cols(V, N, p, q, g, r);
+                __syncthreads();
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sort_diag(A, N, order);
+    __syncthreads();
+    double* eps = S.eps + sys * N;
+    Cx* U = reinterpret_cast<Cx*>(S.U + sys * NN);
+    for (int c = threadIdx.x; c < N; c += blockDim.x) {
+        eps[c] = A[static_cast<long>(order[c]) * (N + 1)].re;
+        gauge_column(V, N, order[c], U + static_cast<long>(c) * N);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int* idx = S.idx + sys * 3 * NN;
+        S.nf[sys] = buckets(eps, N, S.tol, S.rep + sys * NN, idx, idx + NN, idx + 2 * NN, S.omegas + sys * NN,
+                            S.mirror + sys * NN, S.p2f + sys * NN, S.zero + sys);
+    }
+    // A_eig = U^H A U, the product U^H A staged in the LDS rows A used
+    for (int o = 0; o < S.num_ops; ++o) {
+        const Cx* Al = reinterpret_cast<const Cx*>(S.Alab + (sys * S.num_ops + o) * NN);
+        for (long e = threadIdx.x; e < NN; e += blockDim.x)
+            A[e] = uha(U, Al, N, static_cast<int>(e % N), static_cast<int>(e / N));
+        __syncthreads();
+        double2* E = S.Aeig + (sys * S.num_ops + o) * NN;
+        for (long e = threadIdx.x; e < NN; e += blockDim.x) {
+            const Cx v = tu(A, U, N, static_cast<int>(e % N), static_cast<int>(e / N));
+            E[e] = make_double2(v.re, v.im);
+        }
+        __syncthreads();
+    }
+}
+}  // namespace taco_engine
+
 using namespace taco_engine;
 
 extern "C" {
@@ -3705,7 +4220,9 @@ int taco_ctx_create(int device, taco_ctx** out) {
         HIP_OK(hipEventCreate(&c->ev1));
         for (const void* fn : {reinterpret_cast<const void*>(k_rk4), reinterpret_cast<const void*>(k_rk4_batch),
                                reinterpret_cast<const void*>(k_rk4_adjoint),
-                               reinterpret_cast<const void*>(k_rk4_recompute)})
+                               reinterpret_cast<const void*>(k_rk4_recompute),
+                               reinterpret_cast<const void*>(k_rk4_adjoint_multi),
+                               reinterpret_cast<const void*>(k_rk4_recompute_multi)})
             HIP_OK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kRkLds)));
         *out = c.release();
     });
@@ -3860,6 +4377,66 @@ int taco_system_build(const double* H, int N, const double* A_lab, int num_ops, 
     });
 }
 
+int taco_system_build_hip(taco_ctx* ctx, int nsys, const double* H, int N, const double* A_lab, int num_ops,
+                          double freq_tol, double* eps, double* U, double* A_eig, int* nf, double* omegas,
+                          int* mirror, int* pair_to_freq, int* zero_index) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (nsys < 0) throw Error(TACO_ERR_INVALID, "nsys must be >= 0");
+        if (N <= 0) throw Error(TACO_ERR_INVALID, "Eigensystem: Hamiltonian dimension must be > 0");
+        if (N > kSysMaxN) throw Error(TACO_ERR_INVALID, "device system build supports N <= 64");
+        if (num_ops < 0) throw Error(TACO_ERR_INVALID, "system_build: num_ops must be >= 0");
+        if (nsys == 0) return;
+        if (!H || !eps || !U || !nf || !omegas || !mirror || !pair_to_freq || !zero_index ||
+            (num_ops > 0 && (!A_lab || !A_eig)))
+            throw Error(TACO_ERR_INVALID, "null array");
+        const std::size_t NN = static_cast<std::size_t>(N) * N, S = static_cast<std::size_t>(nsys);
+        const auto* Hc = reinterpret_cast<const std::complex<double>*>(H);
+        for (std::size_t s = 0; s < S; ++s)
+            for (std::size_t r = 0; r < static_cast<std::size_t>(N); ++r)
+                for (std::size_t c = 0; c < static_cast<std::size_t>(N); ++c) {
+                    const auto a = Hc[s * NN + r + c * N], b = Hc[s * NN + c + r * N];
+                    if (std::abs(a - std::conj(b)) > 1e-12 * (1.0 + std::abs(a)) ||
+                        (r == c && std::abs(a.imag()) > 1e-12 * (1.0 + std::abs(a.real()))))
+                        throw Error(TACO_ERR_INVALID, "Eigensystem: Hamiltonian must be Hermitian");
+                }
+        DeviceGuard g(ctx->device);
+        const std::size_t nops = static_cast<std::size_t>(num_ops);
+        // device layout in misc: H, A_lab, U, A_eig (complex); eps, omegas, rep (double); ints
+        const std::size_t nc = S * NN * (2 + 2 * nops), nd = S * (N + 2 * NN), ni = S * (2 + 5 * NN);
+        ctx->misc.ensure(nc * 16 + nd * 8 + ni * 4 + 64);
+        double2* dH = ctx->misc.as<double2>();
+        double2* dA = dH + S * NN;
+        double2* dU = dA + S * nops * NN;
+        double2* dE = dU + S * NN;
+        double* deps = reinterpret_cast<double*>(dE + S * nops * NN);
+        double* dom = deps + S * N;
+        double* drep = dom + S * NN;
+        int* dnf = reinterpret_cast<int*>(drep + S * NN);
+        int* dzero = dnf + S;
+        int* dmir = dzero + S;
+        int* dp2f = dmir + S * NN;
+        int* didx = dp2f + S * NN;
+        HIP_OK(hipMemcpyAsync(dH, H, S * NN * 16, hipMemcpyHostToDevice, ctx->stream));
+        if (nops) HIP_OK(hipMemcpyAsync(dA, A_lab, S * nops * NN * 16, hipMemcpyHostToDevice, ctx->stream));
+        SysArgs A{dH, dA, N, num_ops, std::max(freq_tol, std::numeric_limits<double>::epsilon()), deps, dU, dE, dnf,
+                  dom, dmir, dp2f, dzero, drep, didx};
+        const std::size_t lds = 2 * NN * 16;
+        set_fn_lds(reinterpret_cast<const void*>(k_system_build), lds);
+        hipLaunchKernelGGL(k_system_build, dim3(static_cast<unsigned>(S)), dim3(kBlock), lds, ctx->stream, A);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(eps, deps, S * N * 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipMemcpyAsync(U, dU, S * NN * 16, hipMemcpyDeviceToHost, ctx->stream));
+        if (nops) HIP_OK(hipMemcpyAsync(A_eig, dE, S * nops * NN * 16, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipMemcpyAsync(nf, dnf, S * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipMemcpyAsync(omegas, dom, S * NN * 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipMemcpyAsync(mirror, dmir, S * NN * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipMemcpyAsync(pair_to_freq, dp2f, S * NN * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipMemcpyAsync(zero_index, dzero, S * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
 int taco_bcf_fft(size_t N, double dt, int kind, double p0, double p1, double beta, double* C_out) {
     return guarded([&] { bcf_fft(N, dt, kind, p0, p1, beta, C_out); });
 }
@@ -3872,8 +4449,8 @@ int taco_triple_kernels(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, c
                         const int* mirror, double dt, int method, size_t keep, double* F, double* C, double* R) {
     return guarded([&] {
         check_ctx(ctx);
-        if (method != TACO_FCR_CONVOLUTION)
-            throw Error(TACO_ERR_INVALID, "compute_triple_kernels_hip: only FCRMethod::Convolution is supported");
+        if (method != TACO_FCR_CONVOLUTION && method != TACO_FCR_DIRECT)
+            throw Error(TACO_ERR_INVALID, "compute_triple_kernels: unknown FCRMethod");
         if (!(dt > 0.0)) throw Error(TACO_ERR_INVALID, "dt must be > 0");
         if (nf <= 0 || Nt == 0) throw Error(TACO_ERR_INVALID, "empty gamma_series");
         for (int b = 0; b < nf; ++b)
@@ -3894,7 +4471,7 @@ int taco_triple_kernels(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, c
         out.Fo = sl;
         out.Co = sl + nf3 * keep;
         out.Ro = sl + 2 * nf3 * keep;
-        FcrPlan pl{dg, Nt, nf, dom, omegas, dmi, dt, keep, nullptr};
+        FcrPlan pl{dg, Nt, nf, dom, omegas, dmi, dt, keep, nullptr, 0, method};
         run_fcr(ctx, pl, out);
         HIP_OK(hipMemcpyAsync(F, out.Fo, nf3 * keep * 16, hipMemcpyDeviceToHost, ctx->stream));
         HIP_OK(hipMemcpyAsync(C, out.Co, nf3 * keep * 16, hipMemcpyDeviceToHost, ctx->stream));
@@ -3918,7 +4495,7 @@ int taco_l4_batch(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, const d
         const int* dpf = upload(ctx, ctx->p2f, pair_to_freq, N2);
         const double2* dops = upload(ctx, ctx->ops, reinterpret_cast<const double2*>(ops), N2 * num_ops);
         TimePlan tp = l4_core(ctx, dg, Nt, nf, dom, omegas, dmi, dpf, pair_to_freq, N, dops, num_ops, dt, tidx, ntau,
-                              fcr_ms);
+                              fcr_ms, method);
         if (tp.uniq.empty()) return;
         std::vector<double> host(N4 * tp.uniq.size() * 2);
         HIP_OK(hipMemcpyAsync(host.data(), ctx->l4.p, host.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -3947,7 +4524,7 @@ int taco_l4_batch_dev(taco_ctx* ctx, const void* d_gamma, size_t Nt, int nf, con
         TimePlan tp = l4_core(ctx, static_cast<const double2*>(d_gamma), Nt, nf, static_cast<const double*>(d_omegas),
                               hom.data(), static_cast<const int*>(d_mirror), static_cast<const int*>(d_pair_to_freq),
                               hp.data(), N,
-                              static_cast<const double2*>(d_ops), num_ops, dt, tidx, ntau, fcr_ms);
+                              static_cast<const double2*>(d_ops), num_ops, dt, tidx, ntau, fcr_ms, method);
         if (tp.uniq.empty()) return;
         double2* out = static_cast<double2*>(d_L4_out);
         const bool identity = tp.req2u.size() == tp.uniq.size() &&
@@ -4541,53 +5118,54 @@ int taco_tape_forward(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, con
         check_ctx(ctx);
         if (!out) throw Error(TACO_ERR_INVALID, "null output handle");
         *out = nullptr;
-        if (!(order == 0 || order == 2 || order == 4)) throw Error(TACO_ERR_INVALID, "order must be 0, 2, or 4");
-        validate_system(Nt, nf, mirror, pair_to_freq, N, num_ops, dt, TACO_FCR_CONVOLUTION);
-        if (!gamma || !omegas || !mirror || !pair_to_freq || !ops || !eps || !r0)
-            throw Error(TACO_ERR_INVALID, "tape inputs must be non-null");
-        if (steps < 1) throw Error(TACO_ERR_INVALID, "steps must be >= 1");
-        if (steps + 1 > Nt) throw Error(TACO_ERR_RANGE, "steps exceed gamma_series rows");
-        if (checkpoint_every < 1) throw Error(TACO_ERR_INVALID, "checkpoint_every must be >= 1");
         DeviceGuard g(ctx->device);
-        auto tp = std::make_unique<taco_tape>();
-        tp->ctx = ctx;
-        if (!ctx->tape_pool.empty()) {
-            tp->bufs.swap(*ctx->tape_pool.back());
-            ctx->tape_pool.pop_back();
-        }
-        tp->Nt = Nt;
-        tp->T = steps + 1;
-        tp->steps = steps;
-        tp->nf = nf;
-        tp->N = N;
-        tp->num_ops = num_ops;
-        tp->order = order;
-        tp->K = static_cast<int>(std::min<std::size_t>(checkpoint_every, steps));
-        tp->dt = dt;
-        const std::size_t N2 = static_cast<std::size_t>(N) * N;
-        const auto* gc = reinterpret_cast<const std::complex<double>*>(gamma);
-        tp->h_gamma.resize(static_cast<std::size_t>(nf) * tp->T);
-        for (int b = 0; b < nf; ++b)
-            std::copy(gc + static_cast<std::size_t>(b) * Nt, gc + static_cast<std::size_t>(b) * Nt + tp->T,
-                      tp->h_gamma.begin() + static_cast<std::size_t>(b) * tp->T);
-        tp->h_eps.assign(eps, eps + N);
-        tp->h_omegas.assign(omegas, omegas + nf);
-        tp->h_p2f.assign(pair_to_freq, pair_to_freq + N2);
-        tp->h_mirror.assign(mirror, mirror + nf);
-        const auto* oc = reinterpret_cast<const std::complex<double>*>(ops);
-        tp->h_ops.assign(oc, oc + N2 * num_ops);
-        auto put = [&](DevBuf& buf, const void* host, std::size_t bytes) {
-            buf.ensure(std::max<std::size_t>(bytes, 16));
-            HIP_OK(hipMemcpy(buf.p, host, bytes, hipMemcpyHostToDevice));
-        };
-        put(tp->gamma, gamma, Nt * nf * 16);
-        put(tp->omegas, omegas, nf * sizeof(double));
-        put(tp->mirror, mirror, nf * sizeof(int));
-        put(tp->p2f, pair_to_freq, N2 * sizeof(int));
-        put(tp->ops, ops, N2 * num_ops * 16);
-        put(tp->eps, eps, N * sizeof(double));
+        const taco_tape_set in{gamma, Nt, nf, omegas, mirror, pair_to_freq, N, eps, ops, num_ops, dt, steps, order, r0};
+        auto tp = tape_make(ctx, in, checkpoint_every);
         tape_forward(*tp, r0, samples_out);
         *out = tp.release();
+    });
+}
+
+int taco_mtape_forward(taco_ctx* ctx, const taco_tape_set* sets, int nsets, size_t checkpoint_every,
+                       double* const* samples_out, taco_mtape** out) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!out) throw Error(TACO_ERR_INVALID, "null output handle");
+        *out = nullptr;
+        if (!sets || nsets < 1) throw Error(TACO_ERR_INVALID, "need at least one parameter set");
+        for (int s = 1; s < nsets; ++s)
+            if (sets[s].N != sets[0].N || sets[s].steps != sets[0].steps)
+                throw Error(TACO_ERR_INVALID, "parameter sets of a multi-set tape must share N and steps");
+        DeviceGuard g(ctx->device);
+        auto mt = std::make_unique<taco_mtape>();
+        mt->ctx = ctx;
+        for (int s = 0; s < nsets; ++s) mt->sets.push_back(tape_make(ctx, sets[s], checkpoint_every));
+        mtape_forward(*mt, sets, samples_out);
+        *out = mt.release();
+    });
+}
+
+int taco_mtape_adjoint(taco_mtape* mt, const double* const* samples_bar, double* const* gamma_bar,
+                       double* const* omega_bar, double* const* ops_bar, double* const* eps_bar,
+                       double* const* r0_bar) {
+    return guarded([&] {
+        if (!mt) throw Error(TACO_ERR_INVALID, "null taco_mtape");
+        if (!samples_bar) throw Error(TACO_ERR_INVALID, "null samples_bar");
+        for (std::size_t s = 0; s < mt->sets.size(); ++s)
+            if (!samples_bar[s]) throw Error(TACO_ERR_INVALID, "null samples_bar entry");
+        DeviceGuard g(mt->ctx->device);
+        mtape_adjoint(*mt, samples_bar, gamma_bar, omega_bar, ops_bar, eps_bar, r0_bar);
+    });
+}
+
+int taco_mtape_keeps_slices(const taco_mtape* mt) { return mt && mt->keep ? 1 : 0; }
+
+int taco_mtape_destroy(taco_mtape* mt) {
+    return guarded([&] {
+        if (!mt) return;
+        DeviceGuard g(mt->ctx->device);
+        (void)hipStreamSynchronize(mt->ctx->stream);
+        delete mt;
     });
 }
 

@@ -3,6 +3,7 @@
 
 #include <complex>
 #include <cstddef>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -27,6 +28,11 @@ void make_twiddles_exact(int L, std::vector<double>& out);
 int fft_mode();
 void set_fft_mode(int mode);
 void bcf_fft(std::size_t N, double dt, int kind, double p0, double p1, double beta, double* C_out);
+// C(t_n), n = 0..N, of correlation_fft.hpp:154-214 for any spectral density J
+// (called at the sampled frequencies w > 0 only).
+void bcf_fft_fn(std::size_t N, double dt, const std::function<double(double)>& J, double beta, double* C_out);
+// Rectangle-rule Gamma (gamma.hpp:127-160), columns of N rows per omega.
+void gamma_rect(const double* C, std::size_t N, double dt, const double* omegas, int M, double* G);
 void gamma_trapz(const double* C, std::size_t N, double dt, const double* omegas, int M, double* G);
 void mirror_indices(const double* omegas, int nf, double fidx_tol, int* mirror);
 void system_build(const double* H, int N, const double* A_lab, int num_ops, double freq_tol, double* eps,

@@ -16,10 +16,12 @@
 #include <quadmath.h>
 
 #include "cr_sincos.hpp"
+#include "system_prep.hpp"
 
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <functional>
 #include <limits>
 #include <stdexcept>
 #include <vector>
@@ -182,6 +184,12 @@ static double spectral_J(int kind, double p0, double p1, double w) {
 void bcf_fft(std::size_t N, double dt, int kind, double p0, double p1, double beta, double* C_out) {
     if (!(dt > 0.0)) throw Error(TACO_ERR_INVALID, "bcf_fft: dt must be > 0");
     if (kind != 0 && kind != 1) throw Error(TACO_ERR_INVALID, "bcf_fft: unknown spectral density kind");
+    bcf_fft_fn(N, dt, [=](double w) { return spectral_J(kind, p0, p1, w); }, beta, C_out);
+}
+
+// The same for any spectral density J (sampled at w > 0 only).
+void bcf_fft_fn(std::size_t N, double dt, const std::function<double(double)>& J, double beta, double* C_out) {
+    if (!(dt > 0.0)) throw Error(TACO_ERR_INVALID, "bcf_fft: dt must be > 0");
     const std::size_t Nt = N + 1;
     std::size_t Nfft = 2 * Nt;
     if ((Nfft & (Nfft - 1)) != 0) Nfft = next_pow2(Nfft);
@@ -193,23 +201,23 @@ void bcf_fft(std::size_t N, double dt, int kind, double p0, double p1, double be
     if (std::isinf(beta)) {
         for (std::size_t k = 1; k <= posCount; ++k) {
             const double w = domega * static_cast<double>(k);
-            S[k] = cd((kPi / 2.0) * spectral_J(kind, p0, p1, w), 0.0);
+            S[k] = cd((kPi / 2.0) * J(w), 0.0);
         }
-        S[Nfft / 2] = cd(0.25 * kPi * spectral_J(kind, p0, p1, omegaNy), 0.0);
+        S[Nfft / 2] = cd(0.25 * kPi * J(omegaNy), 0.0);
     } else {
         const double r = exp_in_mode(-beta * domega);
         double KMS = r;
         for (std::size_t k = 1; k <= posCount; ++k) {
             const double w = domega * static_cast<double>(k);
-            const double Sp = (kPi / 2.0) * spectral_J(kind, p0, p1, w) / (1.0 - KMS);
+            const double Sp = (kPi / 2.0) * J(w) / (1.0 - KMS);
             S[k] = cd(Sp, 0.0);
             S[Nfft - k] = cd(Sp * KMS, 0.0);
             KMS *= r;
         }
         const double ed = r;
-        S[0] = cd(0.5 * (kPi / 2.0) * spectral_J(kind, p0, p1, domega) * (1.0 + ed) / (1.0 - ed), 0.0);
+        S[0] = cd(0.5 * (kPi / 2.0) * J(domega) * (1.0 + ed) / (1.0 - ed), 0.0);
         const double eNy = exp_in_mode(beta * omegaNy);
-        S[Nfft / 2] = cd(0.25 * kPi * spectral_J(kind, p0, p1, omegaNy) * (1.0 + 2.0 / (eNy - 1.0)), 0.0);
+        S[Nfft / 2] = cd(0.25 * kPi * J(omegaNy) * (1.0 + 2.0 / (eNy - 1.0)), 0.0);
     }
     fft_host(S, true);
     const double scale = domega / kPi;
@@ -240,98 +248,37 @@ void gamma_trapz(const double* Cin, std::size_t N, double dt, const double* omeg
     }
 }
 
-// Cyclic complex Jacobi for Hermitian matrices; eigenvalues ascending,
-// eigenvectors normalised with their largest-magnitude entry real positive.
-static void eigh(const cd* Hin, int n, double* w, cd* V) {
+// Hermitian eigensystem (system_prep.hpp: cyclic complex Jacobi, eigenvalues
+// ascending, gauge fixed), the same steps the device builder runs.
+static void eigh(const cd* Hin, int n, double* w, cd* Vout) {
+    using namespace taco_sys;
     const std::size_t N = static_cast<std::size_t>(n);
-    std::vector<cd> A(Hin, Hin + N * N);
+    std::vector<Cx> A(N * N), V(N * N, cx(0.0, 0.0)), col(N);
+    for (std::size_t e = 0; e < N * N; ++e) A[e] = cx(Hin[e].real(), Hin[e].imag());
     for (std::size_t r = 0; r < N; ++r) {
-        if (std::abs(A[r + r * N].imag()) > 1e-12 * (1.0 + std::abs(A[r + r * N].real())))
+        if (std::abs(A[r + r * N].im) > 1e-12 * (1.0 + std::abs(A[r + r * N].re)))
             throw Error(TACO_ERR_INVALID, "Eigensystem: Hamiltonian must be Hermitian");
-        for (std::size_t c = 0; c < N; ++c) V[r + c * N] = (r == c) ? cd(1.0, 0.0) : cd(0.0, 0.0);
+        V[r + r * N] = cx(1.0, 0.0);
     }
     for (int sweep = 0; sweep < 100; ++sweep) {
-        double off = 0.0, tot = 0.0;
-        for (std::size_t c = 0; c < N; ++c)
-            for (std::size_t r = 0; r < N; ++r) {
-                tot += std::norm(A[r + c * N]);
-                if (r != c) off += std::norm(A[r + c * N]);
-            }
-        if (off <= 1e-30 * std::max(tot, 1e-300)) break;
-        for (std::size_t p = 0; p < N; ++p)
-            for (std::size_t q = p + 1; q < N; ++q) {
-                const cd apq = A[p + q * N];
-                const double mag = std::abs(apq);
-                if (mag < 1e-300) continue;
-                const double app = A[p + p * N].real(), aqq = A[q + q * N].real();
-                const cd ph = apq / mag;
-                const double theta = 0.5 * std::atan2(2.0 * mag, aqq - app);
-                const double c = std::cos(theta), s = std::sin(theta);
-                const cd gpp(c, 0.0), gpq = s * ph, gqp = -s * std::conj(ph), gqq(c, 0.0);
-                for (std::size_t r = 0; r < N; ++r) {
-                    const cd arp = A[r + p * N], arq = A[r + q * N];
-                    A[r + p * N] = arp * gpp + arq * gqp;
-                    A[r + q * N] = arp * gpq + arq * gqq;
-                }
-                for (std::size_t cc = 0; cc < N; ++cc) {
-                    const cd apc = A[p + cc * N], aqc = A[q + cc * N];
-                    A[p + cc * N] = std::conj(gpp) * apc + std::conj(gqp) * aqc;
-                    A[q + cc * N] = std::conj(gpq) * apc + std::conj(gqq) * aqc;
-                }
-                A[p + q * N] = cd(0.0, 0.0);
-                A[q + p * N] = cd(0.0, 0.0);
-                A[p + p * N] = cd(A[p + p * N].real(), 0.0);
-                A[q + q * N] = cd(A[q + q * N].real(), 0.0);
-                for (std::size_t r = 0; r < N; ++r) {
-                    const cd vrp = V[r + p * N], vrq = V[r + q * N];
-                    V[r + p * N] = vrp * gpp + vrq * gqp;
-                    V[r + q * N] = vrp * gpq + vrq * gqq;
-                }
+        if (converged(A.data(), n)) break;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const Rot g = rotation(A.data(), n, p, q);
+                if (g.skip) continue;
+                for (int r = 0; r < n; ++r) rot_cols(A.data(), n, p, q, g, r);
+                for (int c = 0; c < n; ++c) rot_rows(A.data(), n, p, q, g, c);
+                rot_fix(A.data(), n, p, q);
+                for (int r = 0; r < n; ++r) rot_cols(V.data(), n, p, q, g, r);
             }
     }
-    std::vector<std::size_t> order(N);
-    for (std::size_t i = 0; i < N; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](std::size_t a, std::size_t b) { return A[a + a * N].real() < A[b + b * N].real(); });
-    std::vector<cd> Vs(N * N);
+    std::vector<int> order(N);
+    sort_diag(A.data(), n, order.data());
     for (std::size_t c = 0; c < N; ++c) {
-        const std::size_t src = order[c];
-        w[c] = A[src + src * N].real();
-        std::size_t imax = 0;
-        double vmax = -1.0;
-        for (std::size_t r = 0; r < N; ++r)
-            if (std::abs(V[r + src * N]) > vmax + 1e-14) {
-                vmax = std::abs(V[r + src * N]);
-                imax = r;
-            }
-        const cd v0 = V[imax + src * N];
-        const cd phase = (std::abs(v0) > 0) ? std::conj(v0) / std::abs(v0) : cd(1.0, 0.0);
-        double nrm = 0.0;
-        for (std::size_t r = 0; r < N; ++r) nrm += std::norm(V[r + src * N]);
-        nrm = std::sqrt(nrm);
-        for (std::size_t r = 0; r < N; ++r) Vs[r + c * N] = V[r + src * N] * phase / nrm;
-    }
-    std::copy(Vs.begin(), Vs.end(), V);
-}
-
-void mirror_indices(const double* omegas, int nf, double fidx_tol, int* mirror) {
-    const double tol = std::max(1e-12, fidx_tol);
-    for (int j = 0; j < nf; ++j) {
-        const double w = omegas[j];
-        if (std::abs(w) <= tol) {
-            mirror[j] = j;
-            continue;
-        }
-        double best = std::numeric_limits<double>::infinity();
-        int best_idx = -1;
-        for (int jp = 0; jp < nf; ++jp) {
-            const double dw = std::abs(omegas[jp] - (-w));
-            if (dw < best) {
-                best = dw;
-                best_idx = jp;
-            }
-        }
-        mirror[j] = best_idx >= 0 ? best_idx : j;
+        const int src = order[c];
+        w[c] = A[static_cast<std::size_t>(src) * (N + 1)].re;
+        gauge_column(V.data(), n, src, col.data());
+        for (std::size_t r = 0; r < N; ++r) Vout[r + c * N] = cd(col[r].re, col[r].im);
     }
 }
 
@@ -348,65 +295,28 @@ void system_build(const double* Hin, int N, const double* A_lab, int num_ops, do
                 throw Error(TACO_ERR_INVALID, "Eigensystem: Hamiltonian must be Hermitian");
     cd* U = reinterpret_cast<cd*>(Uout);
     eigh(H, N, eps, U);
-    // build_frequency_buckets (system.hpp:138-157): scan (m, n), first bucket
-    // within tol, representative = first omega seen, then sort by omega.
+    // build_frequency_buckets (system.hpp:138-157), mirror and zero bucket
     const double tol = std::max(freq_tol, std::numeric_limits<double>::epsilon());
-    struct Bucket {
-        double omega;
-        std::vector<std::pair<int, int>> pairs;
-    };
-    std::vector<Bucket> bk;
-    for (int m = 0; m < N; ++m)
-        for (int n = 0; n < N; ++n) {
-            const double w = eps[m] - eps[n];
-            std::size_t idx = bk.size();
-            for (std::size_t q = 0; q < bk.size(); ++q)
-                if (std::abs(bk[q].omega - w) <= tol) {
-                    idx = q;
-                    break;
-                }
-            if (idx == bk.size()) bk.push_back(Bucket{w, {}});
-            bk[idx].pairs.emplace_back(m, n);
-        }
-    std::sort(bk.begin(), bk.end(), [](const Bucket& a, const Bucket& b) { return a.omega < b.omega; });
-    const int nf = static_cast<int>(bk.size());
-    *nf_out = nf;
-    for (std::size_t e = 0; e < NN * NN; ++e) p2f[e] = -1;
-    for (int b = 0; b < nf; ++b) {
-        omegas[b] = bk[static_cast<std::size_t>(b)].omega;
-        for (const auto& pr : bk[static_cast<std::size_t>(b)].pairs) p2f[pr.first + pr.second * N] = b;
-    }
-    mirror_indices(omegas, nf, tol, mirror);
-    const double mtol = std::max(1e-12, tol);
-    int zi = -1;
-    for (int b = 0; b < nf && zi < 0; ++b)
-        if (std::abs(omegas[b]) <= mtol) zi = b;
-    if (zi < 0) {
-        double best = std::numeric_limits<double>::infinity();
-        for (int b = 0; b < nf; ++b)
-            if (std::abs(omegas[b]) < best) {
-                best = std::abs(omegas[b]);
-                zi = b;
-            }
-    }
-    *zero_index = zi;
+    std::vector<double> rep(NN * NN);
+    std::vector<int> idx(NN * NN), perm(NN * NN), inv(NN * NN);
+    *nf_out = taco_sys::buckets(eps, N, tol, rep.data(), idx.data(), perm.data(), inv.data(), omegas, mirror, p2f,
+                                zero_index);
+    // A_eig = U^H A U
+    using taco_sys::Cx;
+    std::vector<Cx> Uc(NN * NN), Ac(NN * NN), T(NN * NN);
+    for (std::size_t e = 0; e < NN * NN; ++e) Uc[e] = taco_sys::cx(U[e].real(), U[e].imag());
     const cd* Al = reinterpret_cast<const cd*>(A_lab);
     cd* Ae = reinterpret_cast<cd*>(A_eig);
-    std::vector<cd> tmp(NN * NN);
     for (int o = 0; o < num_ops; ++o) {
         const cd* A = Al + static_cast<std::size_t>(o) * NN * NN;
+        for (std::size_t e = 0; e < NN * NN; ++e) Ac[e] = taco_sys::cx(A[e].real(), A[e].imag());
+        for (int c = 0; c < N; ++c)
+            for (int r = 0; r < N; ++r) T[r + c * NN] = taco_sys::uha(Uc.data(), Ac.data(), N, r, c);
         cd* E = Ae + static_cast<std::size_t>(o) * NN * NN;
-        for (std::size_t r = 0; r < NN; ++r)
-            for (std::size_t c = 0; c < NN; ++c) {
-                cd s(0.0, 0.0);
-                for (std::size_t q = 0; q < NN; ++q) s += std::conj(U[q + r * NN]) * A[q + c * NN];
-                tmp[r + c * NN] = s;
-            }
-        for (std::size_t r = 0; r < NN; ++r)
-            for (std::size_t c = 0; c < NN; ++c) {
-                cd s(0.0, 0.0);
-                for (std::size_t q = 0; q < NN; ++q) s += tmp[r + q * NN] * U[q + c * NN];
-                E[r + c * NN] = s;
+        for (int c = 0; c < N; ++c)
+            for (int r = 0; r < N; ++r) {
+                const Cx v = taco_sys::tu(T.data(), Uc.data(), N, r, c);
+                E[r + c * NN] = cd(v.re, v.im);
             }
     }
 }

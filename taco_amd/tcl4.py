@@ -34,7 +34,8 @@ from ._lib import TacoInvalidArgument, check, dptr, iptr, lib, sptr
 
 __all__ = [
     "Backend", "Exec", "FCRMethod", "System", "Tcl4Map", "TripleKernelSeries", "Ohmic", "DrudeLorentz",
-    "build_map", "bcf_fft_fun", "compute_trapz_prefix_multi_matrix", "bath_gamma_hip",
+    "build_map", "bcf_fft_fun", "compute_trapz_prefix_multi_matrix", "bath_gamma_hip", "SpectralExpr",
+    "compute_rect_prefix_multi_matrix",
     "compute_trapz_prefix_multi_matrix_hip", "compute_triple_kernels",
     "build_TCL4_generator_hip_fused", "build_TCL4_generator_hip_fused_batch", "build_correction_series_hip_fused",
     "build_TCL4_generator_cuda_fused", "build_TCL4_generator_cuda_fused_batch", "build_correction_series_cuda_fused",
@@ -270,6 +271,55 @@ class System:
                    omegas=om[:n].copy(), mirror=mi[:n].copy(), pair_to_freq=pf.reshape(N, N).T.copy(),
                    zero_index=zi.value, freq_tol=freq_tol)
 
+    @classmethod
+    def build_many(cls, Hs, jump_ops_lab, freq_tol: float = 1e-9, exec: Optional[Exec] = None) -> list:
+        """System.build of several Hamiltonians of one size on the GPU (a
+        workgroup each); every System equals System.build's bit for bit.
+        jump_ops_lab: one list of operators shared by all, or one list per H."""
+        Hs = [np.asarray(H, np.complex128) for H in Hs]
+        if not Hs:
+            return []
+        N = Hs[0].shape[0]
+        for H in Hs:
+            if H.shape != (N, N):
+                raise TacoInvalidArgument(1, "Eigensystem: Hamiltonians must be square and of one size")
+        per = jump_ops_lab if (len(jump_ops_lab) and isinstance(jump_ops_lab[0], (list, tuple))) else \
+            [jump_ops_lab] * len(Hs)
+        if len(per) != len(Hs) or len({len(o) for o in per}) != 1:
+            raise TacoInvalidArgument(1, "one operator list per Hamiltonian, all of one length")
+        no = len(per[0])
+        S = len(Hs)
+        ops = [[np.asarray(o, np.complex128) for o in ol] for ol in per]
+        for ol in ops:
+            for o in ol:
+                if o.shape != (N, N):
+                    raise TacoInvalidArgument(1, "to_eigen: dimension mismatch")
+        Hf = np.concatenate([_colmajor(H).ravel() for H in Hs])
+        of = np.concatenate([_ops_flat(ol) for ol in ops]) if no else np.zeros(1, np.complex128)
+        eps = np.zeros(S * N)
+        U = np.zeros(S * N * N, np.complex128)
+        Ae = np.zeros(max(no, 1) * S * N * N, np.complex128)
+        nf = np.zeros(S, np.int32)
+        om = np.zeros(S * N * N)
+        mi = np.zeros(S * N * N, np.int32)
+        pf = np.zeros(S * N * N, np.int32)
+        zi = np.zeros(S, np.int32)
+        ctx = _ctx_for(exec)
+        check(lib().taco_system_build_hip(ctx.handle, S, dptr(Hf), N, dptr(of), no, freq_tol, dptr(eps), dptr(U),
+                                          dptr(Ae), iptr(nf), dptr(om), iptr(mi), iptr(pf), iptr(zi)))
+        out = []
+        NN = N * N
+        for s in range(S):
+            n = int(nf[s])
+            out.append(cls(H=Hs[s], A_lab=ops[s], eps=eps[s * N:(s + 1) * N].copy(),
+                           U=U[s * NN:(s + 1) * NN].reshape(N, N).T.copy(),
+                           A_eig=[Ae[(s * no + k) * NN:(s * no + k + 1) * NN].reshape(N, N).T.copy()
+                                  for k in range(no)],
+                           omegas=om[s * NN:s * NN + n].copy(), mirror=mi[s * NN:s * NN + n].copy(),
+                           pair_to_freq=pf[s * NN:(s + 1) * NN].reshape(N, N).T.copy(),
+                           zero_index=int(zi[s]), freq_tol=freq_tol))
+        return out
+
     def to_lab(self, A_eig) -> np.ndarray:
         return self.U @ np.asarray(A_eig) @ self.U.conj().T
 
@@ -306,9 +356,44 @@ class DrudeLorentz:
         return self.lam, self.gamma
 
 
+class SpectralExpr:
+    """A spectral density J(w) written as an expression, as the reference's
+    configuration files give it (bath.J_expr with bath.params;
+    expression.hpp:14-26), compiled once by the engine's host code:
+    SpectralExpr("w * exp(-w/omega_c) * step(w)", omega_c=10.0)(w)."""
+
+    def __init__(self, source: str, **params):
+        names = list(params)
+        arr = (ctypes.c_char_p * max(len(names), 1))(*[n.encode() for n in names])
+        vals = np.ascontiguousarray([float(params[n]) for n in names] or [0.0], np.float64)
+        h = c_void_p()
+        check(lib().taco_spectral_create(source.encode(), arr, dptr(vals), len(names), ctypes.byref(h)))
+        self.handle = h
+        self.source = source
+        self.params = dict(params)
+
+    def __call__(self, w):
+        w = np.ascontiguousarray(np.atleast_1d(np.asarray(w, np.float64)))
+        out = np.zeros_like(w)
+        check(lib().taco_spectral_eval(self.handle, dptr(w), w.size, dptr(out)))
+        return out
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                lib().taco_spectral_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
 def bcf_fft_fun(N: int, dt: float, J, beta: float):
-    """correlation_fft.hpp:154-214: returns (t[0..N], C[0..N])."""
+    """correlation_fft.hpp:154-214: returns (t[0..N], C[0..N]).  J is an
+    Ohmic, a DrudeLorentz or a SpectralExpr."""
     C = np.zeros(N + 1, np.complex128)
+    if isinstance(J, SpectralExpr):
+        check(lib().taco_bcf_fft_spectral(N, dt, J.handle, beta, dptr(C)))
+        return np.arange(N + 1) * dt, C
     p0, p1 = J.params
     check(lib().taco_bcf_fft(N, dt, J.kind, p0, p1, beta, dptr(C)))
     return np.arange(N + 1) * dt, C
@@ -320,6 +405,15 @@ def compute_trapz_prefix_multi_matrix(C, dt: float, omegas) -> np.ndarray:
     om = np.ascontiguousarray(omegas, np.float64)
     G = np.zeros((len(om), C.size), np.complex128)
     check(lib().taco_gamma_trapz(dptr(C), C.size, dt, dptr(om), len(om), dptr(G)))
+    return G.T
+
+
+def compute_rect_prefix_multi_matrix(C, dt: float, omegas) -> np.ndarray:
+    """gamma.hpp:127-160 (rectangle rule, the tcl_driver default): Gamma [Nt, nf]."""
+    C = np.ascontiguousarray(C, np.complex128)
+    om = np.ascontiguousarray(omegas, np.float64)
+    G = np.zeros((len(om), C.size), np.complex128)
+    check(lib().taco_gamma_rect(dptr(C), C.size, dt, dptr(om), len(om), dptr(G)))
     return G.T
 
 
@@ -400,7 +494,15 @@ def build_TCL4_generator_hip_fused_batch(system: System, gamma_series, dt: float
                                          time_indices: Sequence[int] = (),
                                          method: FCRMethod = FCRMethod.Convolution, exec: Optional[Exec] = None,
                                          fcr_ms: Optional[list] = None) -> np.ndarray:
-    """L4(t) for each t in time_indices (empty = all t < Nt): [ntau, N^2, N^2]."""
+    """L4(t) for each t in time_indices (empty = all t < Nt): [ntau, N^2, N^2].
+    As the reference's fused CUDA builders (tcl4_fused_cuda.cu:820-822), only
+    FCRMethod.Convolution is accepted; build_TCL4_generator takes Direct too."""
+    if FCRMethod(method) != FCRMethod.Convolution:
+        raise TacoInvalidArgument(1, "build_TCL4_generator_cuda_fused: only FCRMethod::Convolution is supported")
+    return _l4_batch(system, gamma_series, dt, time_indices, method, exec, fcr_ms)
+
+
+def _l4_batch(system: System, gamma_series, dt: float, time_indices, method, exec, fcr_ms=None) -> np.ndarray:
     ctx = _ctx_for(exec)
     g = _gamma_checked(system, gamma_series)
     Nt, nf = g.shape
@@ -444,15 +546,26 @@ build_TCL4_generator_cuda_fused_batch = build_TCL4_generator_hip_fused_batch
 build_correction_series_cuda_fused = build_correction_series_hip_fused
 
 
-def build_TCL4_generator(system: System, gamma_series, dt: float, time_index: int,
+def build_TCL4_generator(system: System, gamma_series, dt: float, time_index,
                          method: FCRMethod = FCRMethod.Convolution, exec: Optional[Exec] = None) -> np.ndarray:
-    """tcl4.cpp:293-320 (Exec dispatch) — always the fused GPU path here."""
-    return build_TCL4_generator_hip_fused(system, gamma_series, dt, time_index, method, exec)
+    """tcl4.cpp:293-320 (Exec dispatch), every backend on the GPU.  Unlike the
+    fused CUDA names it accepts FCRMethod.Direct, as the reference's CPU
+    backends do: the F/C/R values then come from the O(T^2) sums of
+    tcl4_kernels.cpp:188-281.  time_index: one index (returns N^2 x N^2) or a
+    sequence of indices (returns [ntau, N^2, N^2])."""
+    g = _gamma_checked(system, gamma_series)
+    if np.ndim(time_index) == 0:
+        t = int(time_index)
+        if t < 0 or t >= g.shape[0]:
+            from ._lib import TacoOutOfRange
+            raise TacoOutOfRange(2, "build_TCL4_generator: time_index out of range")
+        return _l4_batch(system, g, dt, [t], method, exec)[0]
+    return _l4_batch(system, g, dt, list(time_index), method, exec)
 
 
 def build_correction_series(system: System, gamma_series, dt: float, method: FCRMethod = FCRMethod.Convolution,
                             exec: Optional[Exec] = None) -> np.ndarray:
-    return build_correction_series_hip_fused(system, gamma_series, dt, method, exec)
+    return _l4_batch(system, gamma_series, dt, (), method, exec)
 
 
 def build_tcl2_generator(system: System, gamma_ab, time_indices: Sequence[int] = (),

@@ -300,3 +300,85 @@ def test_tfim_parameter_gradient_matches_fd():
         tol = FD_TOL * abs(ref) + 1e-14 * l0 / h
         print(f"d loss / d {name}: adjoint {got:.12e} fd {ref:.12e} rel {abs(got - ref) / abs(ref):.2e}")
         assert abs(got - ref) <= tol, name
+
+
+def _mt_sets(kind):
+    """Parameter sets of one model size for the multi-set tape."""
+    if kind == "spin_pair_d4":
+        ws = [W.spin_pair(eps=e, delta=d, alpha=a, Nt=1024, steps=120)
+              for e, d, a in ((1.0, 1.0, 0.05), (0.8, 1.3, 0.08), (1.2, 0.6, 0.03))]
+        return [dict(gamma=w.gamma, omegas=w.system.omegas, mirror=w.system.mirror,
+                     pair_to_freq=w.system.pair_to_freq, ops=w.system.A_eig, eps=w.system.eps, dt=w.dt,
+                     steps=120, r0=r0_for(4)) for w in ws], 32
+    if kind == "gue3_mixed_order":
+        out = []
+        for k, order in enumerate((4, 2, 4)):
+            rng = W.SplitMix64(40 + k)
+            H = W.gue(3, rng)
+            A1, A2 = W.random_hermitian_unit(3, rng), W.random_hermitian_unit(3, rng)
+            w = W.Workload("g3", H, [A1, A2], taco_amd.DrudeLorentz(0.08, 4.0), 0.7, 0.01, 300 + 50 * k,
+                           list(range(81)), steps=80)
+            r0 = np.zeros(9, complex)
+            r0[0], r0[4] = 0.6, 0.4
+            out.append(dict(gamma=w.gamma, omegas=w.system.omegas, mirror=w.system.mirror,
+                            pair_to_freq=w.system.pair_to_freq, ops=w.system.A_eig, eps=w.system.eps, dt=w.dt,
+                            steps=80, r0=r0, order=order))
+        return out, 16
+    # d = 16 TFIM chain, two transverse fields (nf = 81 and 57)
+    ws = [W.tfim_workload(h, lam=0.2, dt=0.05, Nt=64, tmax=6) for h in (0.7, 1.0)]
+    return [dict(gamma=w.gamma, omegas=w.system.omegas, mirror=w.system.mirror, pair_to_freq=w.system.pair_to_freq,
+                 ops=w.system.A_eig, eps=w.system.eps, dt=w.dt, steps=6, r0=r0_for(16)) for w in ws], 4
+
+
+@pytest.mark.parametrize("kind", ["spin_pair_d4", "gue3_mixed_order", "tfim_d16"])
+@pytest.mark.parametrize("budget", ["keep", "recompute"])
+def test_multi_set_tape_equals_single_tapes(kind, budget):
+    """Every set of a multi-set tape (one RK4 launch forward, one recompute
+    and one walk launch in reverse for all sets) has the samples and the
+    cotangents of a single tape of that set, bit for bit -- with the slices
+    kept, and with a slice budget that forces their recomputation."""
+    sets, K = _mt_sets(kind)
+    rng = np.random.default_rng(31)
+    bars = [rng.standard_normal((s["steps"] + 1, len(s["r0"]))) + 1j * rng.standard_normal(
+        (s["steps"] + 1, len(s["r0"]))) for s in sets]
+    if budget == "recompute":
+        taco_amd.set_slice_budget(16)
+    try:
+        mt = AD.MultiTape(sets, checkpoint_every=K)
+        assert mt.keeps_slices == (budget == "keep")
+        got = mt.adjoint(bars)
+        again = mt.adjoint(bars)
+        samples = [s.copy() for s in mt.samples]
+        mt.close()
+    finally:
+        taco_amd.set_slice_budget(0)
+    for k, st in enumerate(sets):
+        tp = AD.Tape(st["gamma"], st["omegas"], st["mirror"], st["pair_to_freq"], st["ops"], st["eps"], st["dt"],
+                     st["steps"], st["r0"], st.get("order", 4), K)
+        ref = tp.adjoint(bars[k])
+        assert np.array_equal(samples[k], tp.samples), (kind, k)
+        tp.close()
+        for key in ("gamma", "omegas", "eps", "r0"):
+            assert np.array_equal(got[k][key], ref[key]), (kind, k, key)
+            assert np.array_equal(again[k][key], ref[key]), (kind, k, key)
+        for x, y in zip(got[k]["ops"], ref["ops"]):
+            assert np.array_equal(x, y), (kind, k)
+
+
+def test_tfim_losses_many_equal_set_by_set():
+    """The d = 16 chain's multi-set loss path (one tape for all parameter
+    sets) gives every set the loss and parameter gradient of the set-by-set
+    path, bit for bit."""
+    prob = AD.TfimChainProblem(Nt=256, dt=0.02, steps=12, checkpoint_every=4)
+    P = [np.array([0.1, 5.0, 1.0, 1.0, h]) for h in (0.7, 1.0, 1.3)]
+    with torch.no_grad():
+        targets = [prob.rho_lab(torch.tensor(p * 1.03, dtype=torch.float64)).detach() for p in P]
+    params = [torch.tensor(p, dtype=torch.float64, requires_grad=True) for p in P]
+    losses = prob.losses_many(params, targets)
+    grads = torch.autograd.grad(sum(losses), params)
+    for q, p in enumerate(P):
+        x = torch.tensor(p, dtype=torch.float64, requires_grad=True)
+        l1 = prob.loss(x, targets[q])
+        (g1,) = torch.autograd.grad(l1, x)
+        assert float(l1) == float(losses[q])
+        assert np.array_equal(g1.numpy(), grads[q].numpy()), q

@@ -481,3 +481,25 @@ def test_time_windows_equal_one_pass(orc, fft):
     if fft == "parity":
         ref = orc.l4_series(G, S.omegas, S.mirror, S.pair_to_freq, S.A_eig, w.dt, tidx)
         assert max_rel_err(one, ref)[0] <= TOL
+
+
+@pytest.mark.parametrize("wl", ["spin_boson", "gue_pair"])
+def test_direct_method_matches_oracle_direct(orc, wl):
+    """FCRMethod.Direct on the GPU (tcl4_kernels.cpp:188-281: each sum in the
+    reference's order) equals the oracle's Direct series bit for bit, and the
+    L4 built from it matches the oracle's Direct L4; the fused CUDA names keep
+    rejecting Direct as the reference's do (tcl4_fused_cuda.cu:820-822)."""
+    w = W.spin_boson(Nt=600) if wl == "spin_boson" else W.gue_pair(N_bath=400, tmax=400)
+    S, G, dt = w.system, w.gamma, w.dt
+    got = taco_amd.compute_triple_kernels(S, G, dt, method=taco_amd.FCRMethod.Direct)
+    ref = orc.triple_kernels(G, S.omegas, S.mirror, dt, method="direct")
+    for name, a, b in zip("FCR", (got.F, got.C, got.R), ref):
+        assert np.array_equal(a, b), name
+    tidx = [0, 1, 7, 150, len(G) - 1]
+    L = taco_amd.build_TCL4_generator(S, G, dt, tidx, method=taco_amd.FCRMethod.Direct)
+    Lr = orc.l4_series(G, S.omegas, S.mirror, S.pair_to_freq, S.A_eig, dt, tidx, method="direct")
+    assert max_rel_err(L, Lr)[0] <= TOL
+    one = taco_amd.build_TCL4_generator(S, G, dt, 150, method=taco_amd.FCRMethod.Direct)
+    assert np.array_equal(one, L[3])
+    with pytest.raises(TacoInvalidArgument):
+        taco_amd.build_TCL4_generator_cuda_fused(S, G, dt, 3, method=taco_amd.FCRMethod.Direct)

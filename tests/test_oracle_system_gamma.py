@@ -86,3 +86,39 @@ def test_bcf_matches_bruteforce_dft_of_kms_spectrum(orc, beta):
     ref = (np.exp(-2j * np.pi * np.outer(n, n) / Nfft) @ S)[:Nt] * domega / np.pi
     got = orc.bcf_fft(N, dt, beta, 0, alpha, wc)
     assert np.abs(got - ref).max() < 1e-13 * np.abs(ref).max()
+
+
+def test_gamma_rect_geometric_closed_form(orc):
+    # C(t) = exp(-g t): rect sum is dt (1 - z^(k+1)) / (1 - z), z = e^{(i w - g) dt} (gamma.hpp:127-160)
+    g, dt, n = 0.7, 0.01, 4000
+    C = np.exp(-g * np.arange(n) * dt).astype(complex)
+    om = np.array([-2.0, 0.0, 3.5])
+    G = orc.gamma_rect(C, dt, om)
+    k = np.arange(n)
+    for b, w in enumerate(om):
+        z = np.exp((1j * w - g) * dt)
+        ref = dt * (1 - z ** (k + 1)) / (1 - z)
+        assert np.abs(G[:, b] - ref).max() < 1e-12 * np.abs(ref).max()
+    assert np.all(G[0] == dt * C[0])
+
+
+@pytest.mark.parametrize("d", [2, 5, 16, 40])
+def test_host_jacobi_eigensystem(d):
+    """The engine's host eigensolver (algebraic Jacobi rotations): residual,
+    unitarity, ascending order and the gauge (largest entry real positive),
+    also on a matrix with an exactly degenerate pair."""
+    import taco_amd
+    rng = W.SplitMix64(90 + d)
+    H = W.gue(d, rng)
+    for M in (H, H @ H):  # H^2 has near-degenerate pairs from +-lambda
+        S = taco_amd.System.build(M, [np.eye(d, dtype=complex)])
+        U, e = S.U, S.eps
+        assert np.abs(M @ U - U @ np.diag(e)).max() < 1e-12 * max(1.0, np.abs(e).max())
+        assert np.abs(U.conj().T @ U - np.eye(d)).max() < 1e-13
+        assert np.all(np.diff(e) >= 0)
+        for c in range(d):  # the first entry within 1e-14 of the column's largest magnitude is real positive
+            k = int(np.flatnonzero(np.abs(U[:, c]) >= np.abs(U[:, c]).max() - 1e-14)[0])
+            assert abs(U[k, c].imag) <= 1e-15 and U[k, c].real > 0
+    D = np.diag([1.0, 0.5, 1.0, -2.0] + [0.0] * (d - 4) if d >= 4 else [1.0, 1.0][:d]).astype(complex)
+    S = taco_amd.System.build(D, [np.eye(d, dtype=complex)])
+    assert np.array_equal(S.eps, np.sort(np.diag(D).real))
