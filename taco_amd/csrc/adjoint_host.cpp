@@ -24,7 +24,7 @@ namespace taco_engine {
 using cd = std::complex<double>;
 
 Tcl2Parts tcl2_parts(const cd* ops, int num_ops, int N, const int* p2f, int nf) {
-    const std::size_t NN = static_cast<std::size_t>(N), N2 = NN * NN, D = N2, D2 = D * D;
+    const std::size_t NN = static_cast<std::size_t>(N), N2 = NN * NN;
     Tcl2Parts out;
     for (int al = 0; al < num_ops; ++al)
         for (int b = 0; b < nf; ++b) {
@@ -44,18 +44,6 @@ Tcl2Parts tcl2_parts(const cd* ops, int num_ops, int N, const int* p2f, int nf) 
                     for (std::size_t m = 0; m < NN; ++m) s += std::conj(P[m + r * NN]) * P[m + c * NN];
                     AdA[r + c * NN] = s;
                 }
-            const std::size_t k0 = out.K.size();
-            out.K.resize(k0 + D2);
-            for (std::size_t c1 = 0; c1 < NN; ++c1)
-                for (std::size_t c2 = 0; c2 < NN; ++c2)
-                    for (std::size_t r1 = 0; r1 < NN; ++r1)
-                        for (std::size_t r2 = 0; r2 < NN; ++r2) {
-                            const std::size_t row = r1 * NN + r2, col = c1 * NN + c2;
-                            cd v = std::conj(P[r1 + c1 * NN]) * P[r2 + c2 * NN];
-                            if (r1 == c1) v -= 0.5 * AdA[r2 + c2 * NN];
-                            if (r2 == c2) v -= 0.5 * AdA[c1 + r1 * NN];
-                            out.K[k0 + row + col * D] = v;
-                        }
             out.P.insert(out.P.end(), P.begin(), P.end());
             out.AdA.insert(out.AdA.end(), AdA.begin(), AdA.end());
             out.al.push_back(al);

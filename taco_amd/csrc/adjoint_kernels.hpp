@@ -76,8 +76,12 @@ __device__ __forceinline__ void matvec_h_half(const double2* __restrict__ L0, co
 // L_{n+1}.  LDS: vectors of D, then (nstage >= 2) a window of generators
 // staged with column stride D + 1.  Lbar(t) collects the L1 part of step t-1
 // and the L0 part of step t: the latter is carried in registers
-// (D^2 <= 16 x 256) and each Lbar(t) stored once; otherwise Lbar (zero on
-// entry) is accumulated in place.
+// (D^2 <= 16 x 256) and each Lbar(t) stored once.  Larger D (generators read
+// from global memory): the walk reads the conjugate transposes LsH (written
+// beforehand by k_conj_transpose, so every product is a row-coalesced
+// matrix-vector product with the same terms in the same order), stores only
+// the step's cotangent vectors kb1..kb4, and k_rk4_lbar forms every Lbar(t)
+// afterwards, all times at once, with the walk's own sums.
 // ---------------------------------------------------------------------------
 struct Rk4AdjArgs {
     const double2* Ls;    // [steps + 1][D * D] generators
@@ -91,6 +95,8 @@ struct Rk4AdjArgs {
     int K;
     double dt;
     int nstage;
+    const double2* LsH = nullptr;  // [steps + 1][D * D] conjugate transposes (walk without staging)
+    double2* vec = nullptr;        // [steps][4][D] kb1..kb4 of every step (Lbar formed by k_rk4_lbar)
 };
 
 constexpr int kRkCarry = 16;
@@ -191,6 +197,8 @@ __device__ __forceinline__ void rk4_adjoint_body(double2* sh, const Rk4AdjArgs& 
     for (long n = A.steps - 1; n >= 0; --n) {
         const double2* L0;
         const double2* L1;
+        const double2* H0 = nullptr;  // conjugate transposes (LsH path)
+        const double2* H1 = nullptr;
         if constexpr (STAGED) {
             w.need(M, A.Ls, n, n + 1, max(0L, n + 2 - A.nstage), n + 1, D);
             L0 = M + (n - w.lo) * D * ld;
@@ -198,6 +206,10 @@ __device__ __forceinline__ void rk4_adjoint_body(double2* sh, const Rk4AdjArgs& 
         } else {
             L0 = A.Ls + n * D2;
             L1 = L0 + D2;
+            if (A.LsH) {
+                H0 = A.LsH + n * D2;
+                H1 = H0 + D2;
+            }
         }
         const double2* row = A.seg + n * 4 * D;
         for (int e = threadIdx.x; e < D; e += blockDim.x) {
@@ -209,33 +221,41 @@ __device__ __forceinline__ void rk4_adjoint_body(double2* sh, const Rk4AdjArgs& 
         }
         __syncthreads();
         // reverse: r' = r + s (k1 + 2 k2 + 2 k3 + k4)
-        matvec_h<DT>(L1, ld, k4, tmp, D);                                                  // ub3
+        if (H1) matvec_rows<DT>(H1, D, k4, tmp, D); else matvec_h<DT>(L1, ld, k4, tmp, D);  // ub3
         __syncthreads();
         for (int e = threadIdx.x; e < D; e += blockDim.x) {
             k3[e] = cadd(cscale(lam[e], 2.0 * s6), cscale(tmp[e], A.dt));                  // kb3
             rb[e] = cadd(lam[e], tmp[e]);
         }
         __syncthreads();
-        matvec_h_half<DT>(L0, L1, ld, k3, tmp, D);                                         // ub2
+        if (H0) matvec_half<DT>(H0, H1, D, k3, tmp, D); else matvec_h_half<DT>(L0, L1, ld, k3, tmp, D);  // ub2
         __syncthreads();
         for (int e = threadIdx.x; e < D; e += blockDim.x) {
             kb2[e] = cadd(cscale(lam[e], 2.0 * s6), cscale(tmp[e], h));
             rb[e] = cadd(rb[e], tmp[e]);
         }
         __syncthreads();
-        matvec_h_half<DT>(L0, L1, ld, kb2, tmp, D);                                        // ub1
+        if (H0) matvec_half<DT>(H0, H1, D, kb2, tmp, D); else matvec_h_half<DT>(L0, L1, ld, kb2, tmp, D);  // ub1
         __syncthreads();
         for (int e = threadIdx.x; e < D; e += blockDim.x) {
             kb1[e] = cadd(cscale(lam[e], s6), cscale(tmp[e], h));
             rb[e] = cadd(rb[e], tmp[e]);
         }
         __syncthreads();
-        matvec_h<DT>(L0, ld, kb1, tmp, D);
+        if (H0) matvec_rows<DT>(H0, D, kb1, tmp, D); else matvec_h<DT>(L0, ld, kb1, tmp, D);
         __syncthreads();
         // generator cotangents: Lbar_h = kb3 u2^H + kb2 u1^H, split half/half
         double2* Lb0 = A.Lbar + n * D2;
         double2* Lb1 = A.Lbar + (n + 1) * D2;
-        if (carry) {
+        if (!carry && A.vec) {
+            double2* v = A.vec + n * 4 * D;
+            for (int e = threadIdx.x; e < D; e += blockDim.x) {
+                v[e] = kb1[e];
+                v[D + e] = kb2[e];
+                v[2 * D + e] = k3[e];
+                v[3 * D + e] = k4[e];
+            }
+        } else if (carry) {
 #pragma unroll
             for (int q = 0; q < kRkCarry; ++q) {
                 const long e = threadIdx.x + static_cast<long>(q) * blockDim.x;
@@ -306,6 +326,58 @@ __global__ __launch_bounds__(kBlock) void k_rk4_adjoint(Rk4AdjArgs A) {
         TACO_RK4_DISPATCH(A.D, TACO_RK4_CALL)
 #undef TACO_RK4_CALL
     }
+}
+
+// dst[t] = src[t]^H for count D x D matrices (tiles of 16 x 16 through LDS,
+// coalesced reads and writes).  Grid: (tiles of one matrix, count).
+__global__ __launch_bounds__(kBlock) void k_conj_transpose(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                           int D, long count) {
+    __shared__ double2 tile[16][17];
+    const int tiles = (D + 15) / 16;
+    const int ti = static_cast<int>(blockIdx.x) % tiles, tj = static_cast<int>(blockIdx.x) / tiles;
+    const long m = blockIdx.y;
+    if (m >= count) return;
+    const long D2 = static_cast<long>(D) * D;
+    const int lr = threadIdx.x % 16, lc = threadIdx.x / 16;
+    const int r = ti * 16 + lr, c = tj * 16 + lc;  // source element (r, c)
+    if (r < D && c < D) tile[lc][lr] = src[m * D2 + r + static_cast<long>(c) * D];
+    __syncthreads();
+    const int r2 = tj * 16 + lr, c2 = ti * 16 + lc;  // destination element (r2, c2) = conj(src(c2, r2))
+    if (r2 < D && c2 < D) dst[m * D2 + r2 + static_cast<long>(c2) * D] = cconj(tile[lr][lc]);
+}
+
+// Lbar(t), t = 0..steps, from the walk's vectors: the L0 part of step t
+// (kb1 x^H + (kb3 u2^H + kb2 u1^H)/2) added to zero, then the L1 part of
+// step t-1 (kb4 u3^H + (kb3 u2^H + kb2 u1^H)/2): the sums the in-place walk
+// forms, in its order.  Grid: (element blocks of D^2, steps + 1[, tapes]).
+__device__ __forceinline__ void rk4_lbar_body(const Rk4AdjArgs& A) {
+    const int D = A.D;
+    const long D2 = static_cast<long>(D) * D;
+    const long e = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+    const long t = blockIdx.y;
+    if (e >= D2 || t > A.steps) return;
+    const int r = static_cast<int>(e % D), c = static_cast<int>(e / D);
+    double2 acc = make_double2(0.0, 0.0);
+    if (t < A.steps) {
+        const double2* v = A.vec + t * 4 * D;
+        const double2* sg = A.seg + t * 4 * D;
+        const double2 lh = cadd(cmul(v[2 * D + r], cconj(sg[2 * D + c])), cmul(v[D + r], cconj(sg[D + c])));
+        acc = cadd(acc, cadd(cmul(v[r], cconj(sg[c])), cscale(lh, 0.5)));
+    }
+    if (t >= 1) {
+        const double2* v = A.vec + (t - 1) * 4 * D;
+        const double2* sg = A.seg + (t - 1) * 4 * D;
+        const double2 lh = cadd(cmul(v[2 * D + r], cconj(sg[2 * D + c])), cmul(v[D + r], cconj(sg[D + c])));
+        acc = cadd(acc, cadd(cmul(v[3 * D + r], cconj(sg[3 * D + c])), cscale(lh, 0.5)));
+    }
+    A.Lbar[t * D2 + e] = acc;
+}
+
+__global__ __launch_bounds__(kBlock) void k_rk4_lbar(Rk4AdjArgs A) { rk4_lbar_body(A); }
+
+__global__ __launch_bounds__(kBlock) void k_rk4_lbar_multi(const Rk4AdjArgs* As) {
+    const Rk4AdjArgs A = As[blockIdx.z];
+    rk4_lbar_body(A);
 }
 
 // Many tapes of one D, steps and K in one launch: the recompute with
@@ -1157,6 +1229,26 @@ struct Tcl2AdjArgs {
     double2* part;         // [nchunk][nout]
     double2* wsum;         // [nout]: W_b, V_b, eps sums (in .x)
 };
+
+// K of every TCL2 part (d L / d gam_b for the part's masked operator P):
+// K[(r1 N + r2) + (c1 N + c2) D] = conj(P(r1,c1)) P(r2,c2)
+//   - [r1 = c1] AdA(r2,c2)/2 - [r2 = c2] AdA(c1,r1)/2.
+// Grid: (element blocks of D^2, parts).
+__global__ __launch_bounds__(kBlock) void k_tcl2_part_k(const double2* P, const double2* AdA, int N, double2* K) {
+    const int N2 = N * N, D = N2;
+    const long D2 = static_cast<long>(D) * D;
+    const long e = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (e >= D2) return;
+    const long q = blockIdx.y;
+    const int row = static_cast<int>(e % D), col = static_cast<int>(e / D);
+    const int r1 = row / N, r2 = row % N, c1 = col / N, c2 = col % N;
+    const double2* Pq = P + q * N2;
+    const double2* Aq = AdA + q * N2;
+    double2 v = cmul(cconj(Pq[r1 + c1 * N]), Pq[r2 + c2 * N]);
+    if (r1 == c1) v = csub(v, cscale(Aq[r2 + c2 * N], 0.5));
+    if (r2 == c2) v = csub(v, cscale(Aq[c1 + r1 * N], 0.5));
+    K[q * D2 + e] = v;
+}
 
 __global__ __launch_bounds__(kBlock) void k_tcl2_adj_t(Tcl2AdjArgs A) {
     __shared__ double wred[4];

@@ -2100,6 +2100,10 @@ struct Profiler {
 };
 }  // namespace taco_engine
 
+namespace taco_engine {
+struct L4AdjLists;
+}
+
 struct taco_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -2112,6 +2116,8 @@ struct taco_ctx {
     taco_engine::PhaseTab phtab;  // phase table of the last plain-path system (TACO_PHASE_LIBM)
     std::size_t l4_lds_set = 0;  // dynamic LDS limit set on the fused L4 kernel
     std::size_t slice_budget = 0;  // bytes of F/C/R slices per pass of the batch builders (0: default)
+    // term lists of the factorised L4 reverse per (pair_to_freq, N, nf)
+    std::vector<std::pair<std::vector<int>, std::shared_ptr<taco_engine::L4AdjLists>>> adj_lists;
     std::vector<std::unique_ptr<taco_engine::TapeBufs>> tape_pool;  // buffers of destroyed tapes (at most 2)
 };
 
@@ -3398,11 +3404,25 @@ static L4AdjLists l4_adjoint_lists(const int* p2f, int N, int nf) {
 // step (seg, [steps][4][D]) and dLoss/dr_0 (rb, [D]).
 struct RkRev {
     double2 *gb, *Lb, *seg, *rb;
+    double2 *LsH = nullptr, *vec = nullptr;  // deferred walk: conjugate transposes, cotangent vectors
 };
+
+// The walk reads transposed generators and leaves Lbar to k_rk4_lbar when it
+// neither has a row-group kernel nor stages generators in LDS nor carries
+// the L0 parts in registers (large D).
+// TACO_RK4_DEFER=0 keeps the in-place walk (a test compares the two).
+static bool rk_rev_deferred(const taco_tape& tp) {
+    if (env_int("TACO_RK4_DEFER", 1) == 0) return false;
+    const int D = tp.N * tp.N;
+    if (D == 4 || D == 9 || D == 16) return false;
+    if (static_cast<long>(D) * D <= static_cast<long>(kRkCarry) * kBlock) return false;
+    return rk4_nstage(D, 12, static_cast<long>(tp.steps) + 1) < 2;
+}
 
 static std::size_t rk_rev_values(const taco_tape& tp) {
     const std::size_t D = static_cast<std::size_t>(tp.N) * tp.N;
-    return tp.T * D + tp.T * D * D + 4 * tp.steps * D + D;
+    const std::size_t base = tp.T * D + tp.T * D * D + 4 * tp.steps * D + D;
+    return rk_rev_deferred(tp) ? base + tp.T * D * D + 4 * tp.steps * D : base;
 }
 
 static RkRev rk_rev_at(const taco_tape& tp, double2* base) {
@@ -3412,6 +3432,10 @@ static RkRev rk_rev_at(const taco_tape& tp, double2* base) {
     b.Lb = b.gb + tp.T * D;
     b.seg = b.Lb + tp.T * D * D;
     b.rb = b.seg + 4 * tp.steps * D;
+    if (rk_rev_deferred(tp)) {
+        b.LsH = b.rb + D;
+        b.vec = b.LsH + tp.T * D * D;
+    }
     return b;
 }
 
@@ -3441,6 +3465,8 @@ static RkRevShape rk_rev_shape(const taco_tape& tp, const RkRev& b) {
     R.D = static_cast<int>(D);
     R.K = tp.K;
     R.dt = tp.dt;
+    R.LsH = b.LsH;
+    R.vec = b.vec;
     sh.carry = D2 <= static_cast<std::size_t>(kRkCarry) * kBlock;
     sh.nseg = (steps + K - 1) / K;
     const int Di = static_cast<int>(D);
@@ -3465,12 +3491,16 @@ static void rk_rev_run(taco_tape& tp, const RkRev& b) {
     taco_ctx* c = tp.ctx;
     const std::size_t D = static_cast<std::size_t>(tp.N) * tp.N, D2 = D * D;
     const RkRevShape sh = rk_rev_shape(tp, b);
-    if (!sh.carry) {
+    const int Di = static_cast<int>(D);
+    if (b.LsH) {
+        const int tiles = (Di + 15) / 16;
+        hipLaunchKernelGGL(k_conj_transpose, dim3(static_cast<unsigned>(tiles * tiles), static_cast<unsigned>(tp.T)),
+                           dim3(kBlock), 0, c->stream, tp.lgen.as<double2>(), b.LsH, Di, static_cast<long>(tp.T));
+    } else if (!sh.carry) {
         const long cntL = static_cast<long>(tp.T * D2);
         hipLaunchKernelGGL(k_fill_zero, dim3(static_cast<unsigned>((cntL + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                            c->stream, b.Lb, cntL);
     }
-    const int Di = static_cast<int>(D);
     timed(c, K_RK4, sh.bytes, [&] {
         // every segment's states and stage points from its checkpoint, all segments at once
         if (sh.rows_f) {
@@ -3492,12 +3522,29 @@ static void rk_rev_run(taco_tape& tp, const RkRev& b) {
         } else {
             hipLaunchKernelGGL(k_rk4_adjoint, dim3(1), dim3(kBlock), sh.lds_b, c->stream, sh.R);
         }
+        if (b.vec)
+            hipLaunchKernelGGL(k_rk4_lbar, dim3(static_cast<unsigned>((D2 + kBlock - 1) / kBlock),
+                                                static_cast<unsigned>(tp.T)),
+                               dim3(kBlock), 0, c->stream, sh.R);
     });
     HIP_OK(hipGetLastError());
 }
 
 static void tape_adjoint_tail(taco_tape& tp, const RkRev& rv, const double2* sl, double* gamma_bar,
                               double* omega_bar, double* ops_bar, double* eps_bar, double* r0_bar);
+
+// The term lists depend on (N, nf, pair_to_freq) only: built once per
+// context and structure (the last 8 kept), reused by every later reverse.
+static const L4AdjLists& l4_adjoint_lists_cached(taco_ctx* c, const int* p2f, int N, int nf) {
+    std::vector<int> key(p2f, p2f + static_cast<std::size_t>(N) * N);
+    key.push_back(N);
+    key.push_back(nf);
+    for (auto& e : c->adj_lists)
+        if (e.first == key) return *e.second;
+    if (c->adj_lists.size() >= 8) c->adj_lists.erase(c->adj_lists.begin());
+    c->adj_lists.emplace_back(std::move(key), std::make_shared<L4AdjLists>(l4_adjoint_lists(p2f, N, nf)));
+    return *c->adj_lists.back().second;
+}
 
 static void tape_adjoint(taco_tape& tp, const double* samples_bar, double* gamma_bar, double* omega_bar,
                          double* ops_bar, double* eps_bar, double* r0_bar) {
@@ -3526,7 +3573,7 @@ static void tape_adjoint_tail(taco_tape& tp, const RkRev& rv, const double2* sl,
     const std::size_t nfz = static_cast<std::size_t>(tp.nf);
     const std::size_t nout = nfz * D2 + nfz * N2 + static_cast<std::size_t>(tp.N);
     const int nchunk = static_cast<int>(std::max<std::size_t>(1, std::min<std::size_t>(64, (std::size_t{1} << 22) / nout)));
-    const std::size_t n_tcl2 = T * N2 + (T * tp.N + 1) / 2 + nfz * T + nchunk * nout + nout + npart * (D2 + N2) +
+    const std::size_t n_tcl2 = T * N2 + (T * tp.N + 1) / 2 + nfz * T + nchunk * nout + nout + npart * (D2 + 2 * N2) +
                                (npart + 1) / 4 + 1;
     c->t2scr.ensure(n_tcl2 * 16);
     double2* t2X = c->t2scr.as<double2>();
@@ -3536,11 +3583,16 @@ static void tape_adjoint_tail(taco_tape& tp, const RkRev& rv, const double2* sl,
     double2* t2w = t2part + nchunk * nout;
     double2* t2K = t2w + nout;
     double2* t2A = t2K + npart * D2;
-    int* t2b = reinterpret_cast<int*>(t2A + npart * N2);
+    double2* t2P = t2A + npart * N2;
+    int* t2b = reinterpret_cast<int*>(t2P + npart * N2);
     if (npart) {
-        HIP_OK(hipMemcpyAsync(t2K, parts.K.data(), npart * D2 * 16, hipMemcpyHostToDevice, c->stream));
         HIP_OK(hipMemcpyAsync(t2A, parts.AdA.data(), npart * N2 * 16, hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipMemcpyAsync(t2P, parts.P.data(), npart * N2 * 16, hipMemcpyHostToDevice, c->stream));
         HIP_OK(hipMemcpyAsync(t2b, parts.b.data(), npart * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_tcl2_part_k, dim3(static_cast<unsigned>((D2 + kBlock - 1) / kBlock),
+                                               static_cast<unsigned>(npart)),
+                           dim3(kBlock), 0, c->stream, t2P, t2A, tp.N, t2K);
+        HIP_OK(hipGetLastError());
     }
     std::vector<std::complex<double>> gbar_h(static_cast<std::size_t>(tp.nf) * T, {0.0, 0.0});
     std::vector<std::complex<double>> opsb(static_cast<std::size_t>(tp.num_ops) * N2, {0.0, 0.0});
@@ -3620,7 +3672,7 @@ static void tape_adjoint_tail(taco_tape& tp, const RkRev& rv, const double2* sl,
             Af.lds_tb = lds_tb ? 1 : 0;
             Af.scr = l4s;
             Af.scr_per = static_cast<long>(l4_per);
-            const L4AdjLists LL = l4_adjoint_lists(tp.h_p2f.data(), tp.N, tp.nf);
+            const L4AdjLists& LL = l4_adjoint_lists_cached(c, tp.h_p2f.data(), tp.N, tp.nf);
             const std::vector<int>* parts_l[9] = {&LL.xp_xz, &LL.xp_start, &LL.xp_ent, &LL.f_tgt, &LL.f_start,
                                                   &LL.f_ent, &LL.r_tgt, &LL.r_start, &LL.r_ent};
             std::size_t tot = 0;
@@ -4035,7 +4087,12 @@ static void mtape_adjoint(taco_mtape& mt, const double* const* samples_bar, doub
         sh = rk_rev_shape(*mt.sets[s], rv[s]);
         args[s] = sh.Rf;
         args[S + s] = sh.R;
-        if (!sh.carry) {
+        if (rv[s].LsH) {
+            const int tiles = (static_cast<int>(D) + 15) / 16;
+            hipLaunchKernelGGL(k_conj_transpose, dim3(static_cast<unsigned>(tiles * tiles), static_cast<unsigned>(T)),
+                               dim3(kBlock), 0, c->stream, mt.sets[s]->lgen.as<double2>(), rv[s].LsH,
+                               static_cast<int>(D), static_cast<long>(T));
+        } else if (!sh.carry) {
             const long cntL = static_cast<long>(T * D2);
             hipLaunchKernelGGL(k_fill_zero, dim3(static_cast<unsigned>((cntL + kBlock - 1) / kBlock)), dim3(kBlock),
                                0, c->stream, rv[s].Lb, cntL);
@@ -4068,6 +4125,10 @@ static void mtape_adjoint(taco_mtape& mt, const double* const* samples_bar, doub
             hipLaunchKernelGGL(k_rk4_adjoint_multi, dim3(static_cast<unsigned>(S)), dim3(kBlock), sh.lds_b, c->stream,
                                dargs + S);
         }
+        if (rv[0].vec)
+            hipLaunchKernelGGL(k_rk4_lbar_multi, dim3(static_cast<unsigned>((D2 + kBlock - 1) / kBlock),
+                                                      static_cast<unsigned>(T), static_cast<unsigned>(S)),
+                               dim3(kBlock), 0, c->stream, dargs + S);
     });
     HIP_OK(hipGetLastError());
     for (std::size_t s = 0; s < S; ++s) {

@@ -40,10 +40,11 @@ void system_build(const double* H, int N, const double* A_lab, int num_ops, doub
                   int* zero_index);
 
 // Reverse of the TCL2 generator, host side (adjoint_host.cpp).  Parts: one
-// per (operator, bucket) with a nonzero masked operator P: K = dL/dgam
-// [D^2], AdA = P^H P [N^2], P [N^2]; complex values column-major.
+// per (operator, bucket) with a nonzero masked operator P: AdA = P^H P
+// [N^2], P [N^2]; complex values column-major.  The device forms each part's
+// K = dL/dgam [D^2] from P and AdA (k_tcl2_part_k).
 struct Tcl2Parts {
-    std::vector<std::complex<double>> K, AdA, P;
+    std::vector<std::complex<double>> AdA, P;
     std::vector<int> al, b;
 };
 Tcl2Parts tcl2_parts(const std::complex<double>* ops, int num_ops, int N, const int* p2f, int nf);

@@ -382,3 +382,39 @@ def test_tfim_losses_many_equal_set_by_set():
         (g1,) = torch.autograd.grad(l1, x)
         assert float(l1) == float(losses[q])
         assert np.array_equal(g1.numpy(), grads[q].numpy()), q
+
+
+@pytest.mark.parametrize("kind", ["d16", "d10"])
+def test_deferred_reverse_walk_equals_in_place_walk(monkeypatch, kind):
+    """Large D: the reverse RK4 walk over conjugate-transposed generators with
+    the generator cotangents formed afterwards by k_rk4_lbar gives the
+    cotangents of the in-place walk bit for bit (single and multi-set)."""
+    if kind == "d16":
+        S, G, dt = D16.system, D16.gamma, D16.dt
+        steps, K = D16_STEPS, D16_K
+    else:
+        rng = W.SplitMix64(17)
+        H = W.gue(10, rng)
+        w = W.Workload("g10", H, [W.random_hermitian_unit(10, rng)], taco_amd.DrudeLorentz(0.1, 3.0), 0.8, 0.02, 127,
+                       list(range(21)), steps=20)
+        S, G, dt, steps, K = w.system, w.gamma, w.dt, 20, 8
+    N = S.N
+    rng = np.random.default_rng(5)
+    bar = rng.standard_normal((steps + 1, N * N)) + 1j * rng.standard_normal((steps + 1, N * N))
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("TACO_RK4_DEFER", mode)
+        tp = AD.Tape(G, S.omegas, S.mirror, S.pair_to_freq, S.A_eig, S.eps, dt, steps, r0_for(N), 4, K)
+        out[mode] = tp.adjoint(bar)
+        tp.close()
+        st = dict(gamma=G, omegas=S.omegas, mirror=S.mirror, pair_to_freq=S.pair_to_freq, ops=S.A_eig, eps=S.eps,
+                  dt=dt, steps=steps, r0=r0_for(N))
+        mt = AD.MultiTape([st, st], checkpoint_every=K)
+        out[mode + "m"] = mt.adjoint([bar, bar])
+        mt.close()
+    for key in ("gamma", "omegas", "eps", "r0"):
+        assert np.array_equal(out["0"][key], out["1"][key]), key
+        for q in range(2):
+            assert np.array_equal(out["0m"][q][key], out["1"][key]), key
+    for x, y in zip(out["0"]["ops"], out["1"]["ops"]):
+        assert np.array_equal(x, y)
