@@ -1230,6 +1230,51 @@ struct Tcl2AdjArgs {
     double2* wsum;         // [nout]: W_b, V_b, eps sums (in .x)
 };
 
+// Operator cotangent of every TCL2 part from the time sums W_b (D^2) and
+// V_b (N^2) of its bucket: the transposes of the part's K and AdA = P^H P
+// terms applied to W_b and V_b,
+//   AdAbar(a, c) = V_b(a, c) - 1/2 sum_k W_b[(k N + a) + (k N + c) D]
+//                            - 1/2 sum_k W_b[(c N + k) + (a N + k) D],
+//   Pbar(x, y) = sum_{r,c} W_b[(r N + x) + (c N + y) D] P(r, c)
+//              + sum_{r,c} conj(W_b[(x N + r) + (y N + c) D]) P(r, c)
+//              + sum_r AdAbar(r, y) P(x, r) + sum_c conj(AdAbar(y, c)) P(x, c),
+// written per part (the host adds Pbar where pair_to_freq = b).  One
+// workgroup per part; N^2 <= 1024.
+__global__ __launch_bounds__(kBlock) void k_tcl2_ops_parts(const double2* W, const double2* V, const double2* P,
+                                                           const int* part_b, int N, double2* Pbar) {
+    extern __shared__ double2 shp[];
+    const int N2 = N * N, D = N2;
+    const long D2 = static_cast<long>(D) * D;
+    const long q = blockIdx.x;
+    const int b = part_b[q];
+    const double2* Wb = W + static_cast<long>(b) * D2;
+    const double2* Vb = V + static_cast<long>(b) * N2;
+    const double2* Pq = P + q * N2;
+    double2* Ps = shp;        // [N2]
+    double2* Ab = shp + N2;   // [N2] AdAbar
+    for (int e = threadIdx.x; e < N2; e += kBlock) Ps[e] = Pq[e];
+    for (int e = threadIdx.x; e < N2; e += kBlock) {
+        const int a = e % N, cc = e / N;
+        double2 v = Vb[e];
+        for (int k = 0; k < N; ++k) v = csub(v, cscale(Wb[(k * N + a) + static_cast<long>(k * N + cc) * D], 0.5));
+        for (int k = 0; k < N; ++k) v = csub(v, cscale(Wb[(cc * N + k) + static_cast<long>(a * N + k) * D], 0.5));
+        Ab[e] = v;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < N2; e += kBlock) {
+        const int x = e % N, y = e / N;
+        double2 s = make_double2(0.0, 0.0);
+        for (int c = 0; c < N; ++c)
+            for (int r = 0; r < N; ++r) s = cadd(s, cmul(Wb[(r * N + x) + static_cast<long>(c * N + y) * D], Ps[r + c * N]));
+        for (int c = 0; c < N; ++c)
+            for (int r = 0; r < N; ++r)
+                s = cadd(s, cmul(cconj(Wb[(x * N + r) + static_cast<long>(y * N + c) * D]), Ps[r + c * N]));
+        for (int r = 0; r < N; ++r) s = cadd(s, cmul(Ab[r + y * N], Ps[x + r * N]));
+        for (int c = 0; c < N; ++c) s = cadd(s, cmul(cconj(Ab[y + c * N]), Ps[x + c * N]));
+        Pbar[q * N2 + e] = s;
+    }
+}
+
 // K of every TCL2 part (d L / d gam_b for the part's masked operator P):
 // K[(r1 N + r2) + (c1 N + c2) D] = conj(P(r1,c1)) P(r2,c2)
 //   - [r1 = c1] AdA(r2,c2)/2 - [r2 = c2] AdA(c1,r1)/2.

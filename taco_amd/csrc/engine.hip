@@ -2495,7 +2495,7 @@ static FcrJob fcr_prepare(taco_ctx* c, const FcrPlan& pl, const FcrLists& lists)
         const unsigned char m = pl.need ? (*pl.need)[static_cast<std::size_t>(t)] : 7;
         if (m & 3) order.push_back(static_cast<int>(t));
     }
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key[x] < key[y]; });
+    stable_sort_by_key(order, key.data());
     const long nfft_jobs = static_cast<long>(order.size());
     for (long t = 0; t < nf3; ++t) {
         const unsigned char m = pl.need ? (*pl.need)[static_cast<std::size_t>(t)] : 7;
@@ -3573,7 +3573,7 @@ static void tape_adjoint_tail(taco_tape& tp, const RkRev& rv, const double2* sl,
     const std::size_t nfz = static_cast<std::size_t>(tp.nf);
     const std::size_t nout = nfz * D2 + nfz * N2 + static_cast<std::size_t>(tp.N);
     const int nchunk = static_cast<int>(std::max<std::size_t>(1, std::min<std::size_t>(64, (std::size_t{1} << 22) / nout)));
-    const std::size_t n_tcl2 = T * N2 + (T * tp.N + 1) / 2 + nfz * T + nchunk * nout + nout + npart * (D2 + 2 * N2) +
+    const std::size_t n_tcl2 = T * N2 + (T * tp.N + 1) / 2 + nfz * T + nchunk * nout + nout + npart * (D2 + 3 * N2) +
                                (npart + 1) / 4 + 1;
     c->t2scr.ensure(n_tcl2 * 16);
     double2* t2X = c->t2scr.as<double2>();
@@ -3584,7 +3584,8 @@ static void tape_adjoint_tail(taco_tape& tp, const RkRev& rv, const double2* sl,
     double2* t2K = t2w + nout;
     double2* t2A = t2K + npart * D2;
     double2* t2P = t2A + npart * N2;
-    int* t2b = reinterpret_cast<int*>(t2P + npart * N2);
+    double2* t2Pb = t2P + npart * N2;
+    int* t2b = reinterpret_cast<int*>(t2Pb + npart * N2);
     if (npart) {
         HIP_OK(hipMemcpyAsync(t2A, parts.AdA.data(), npart * N2 * 16, hipMemcpyHostToDevice, c->stream));
         HIP_OK(hipMemcpyAsync(t2P, parts.P.data(), npart * N2 * 16, hipMemcpyHostToDevice, c->stream));
@@ -3802,16 +3803,30 @@ static void tape_adjoint_tail(taco_tape& tp, const RkRev& rv, const double2* sl,
     hipLaunchKernelGGL(k_tcl2_adj_w, dim3(nob, static_cast<unsigned>(nchunk)), dim3(kBlock), 0, c->stream, Q);
     hipLaunchKernelGGL(k_tcl2_adj_wsum, dim3(nob), dim3(kBlock), 0, c->stream, Q);
     HIP_OK(hipGetLastError());
-    std::vector<std::complex<double>> g2(nfz * T), wv(nout), r0b(D);
-    HIP_OK(hipMemcpyAsync(g2.data(), t2g, g2.size() * 16, hipMemcpyDeviceToHost, c->stream));
-    HIP_OK(hipMemcpyAsync(wv.data(), t2w, nout * 16, hipMemcpyDeviceToHost, c->stream));
-    HIP_OK(hipMemcpyAsync(r0b.data(), rb, D * 16, hipMemcpyDeviceToHost, c->stream));
-    HIP_OK(hipStreamSynchronize(c->stream));
     const std::size_t nW = nfz * D2, nV = nfz * N2;
-    for (int n = 0; n < tp.N; ++n) epsb[static_cast<std::size_t>(n)] += wv[nW + nV + static_cast<std::size_t>(n)].real();
+    // operator cotangents of the TCL2 part, per part on the device
+    if (tp.order != 0 && npart) {
+        if (N2 > 1024) throw Error(TACO_ERR_INVALID, "TCL2 reverse supports N <= 32");
+        hipLaunchKernelGGL(k_tcl2_ops_parts, dim3(static_cast<unsigned>(npart)), dim3(kBlock), 2 * N2 * 16, c->stream,
+                           t2w, t2w + nW, t2P, t2b, tp.N, t2Pb);
+        HIP_OK(hipGetLastError());
+    }
+    std::vector<std::complex<double>> g2(nfz * T), wv(nV + static_cast<std::size_t>(tp.N)), r0b(D), pb(npart * N2);
+    HIP_OK(hipMemcpyAsync(g2.data(), t2g, g2.size() * 16, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipMemcpyAsync(wv.data(), t2w + nW, wv.size() * 16, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipMemcpyAsync(r0b.data(), rb, D * 16, hipMemcpyDeviceToHost, c->stream));
+    if (tp.order != 0 && npart)
+        HIP_OK(hipMemcpyAsync(pb.data(), t2Pb, pb.size() * 16, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    for (int n = 0; n < tp.N; ++n) epsb[static_cast<std::size_t>(n)] += wv[nV + static_cast<std::size_t>(n)].real();
     if (tp.order != 0) {
         for (std::size_t e = 0; e < g2.size(); ++e) gbar_h[e] += g2[e];
-        tcl2_ops_adjoint(parts, wv.data(), wv.data() + nW, tp.N, tp.h_p2f.data(), opsb.data());
+        // each operator entry takes the Pbar of the part of its bucket (one part per entry)
+        for (std::size_t q = 0; q < npart; ++q) {
+            std::complex<double>* Ab = opsb.data() + static_cast<std::size_t>(parts.al[q]) * N2;
+            for (std::size_t e = 0; e < N2; ++e)
+                if (tp.h_p2f[e] == parts.b[q]) Ab[e] += pb[q * N2 + e];
+        }
     }
     if (gamma_bar) {
         auto* g = reinterpret_cast<std::complex<double>*>(gamma_bar);

@@ -3,6 +3,7 @@
 
 #include <complex>
 #include <cstddef>
+#include <cstdint>
 #include <functional>
 #include <stdexcept>
 #include <string>
@@ -35,6 +36,8 @@ void bcf_fft_fn(std::size_t N, double dt, const std::function<double(double)>& J
 void gamma_rect(const double* C, std::size_t N, double dt, const double* omegas, int M, double* G);
 void gamma_trapz(const double* C, std::size_t N, double dt, const double* omegas, int M, double* G);
 void mirror_indices(const double* omegas, int nf, double fidx_tol, int* mirror);
+// std::stable_sort(order, key[x] < key[y]) on all host cores (same result).
+void stable_sort_by_key(std::vector<int>& order, const std::uint64_t* key);
 void system_build(const double* H, int N, const double* A_lab, int num_ops, double freq_tol, double* eps,
                   double* U, double* A_eig, int* nf_out, double* omegas, int* mirror, int* p2f,
                   int* zero_index);

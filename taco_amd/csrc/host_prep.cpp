@@ -19,6 +19,10 @@
 #include "system_prep.hpp"
 
 #include <algorithm>
+#include <cstdint>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <cmath>
 #include <complex>
 #include <functional>
@@ -374,6 +378,41 @@ void cr_exp_host(const double* x, std::size_t n, double* out) {
 void cr_sincos_host(const double* x, std::size_t n, double* s, double* c) {
     const double* tab = cr_table();
     for (std::size_t k = 0; k < n; ++k) taco_cr::sincos_cr(x[k], s + k, c + k, tab);
+}
+
+}  // namespace taco_engine
+
+namespace taco_engine {
+
+// std::stable_sort of order by key[order[.]] on all host cores: chunks sorted
+// in parallel, then merged pairwise (inplace_merge keeps equal keys of the
+// left run first), so the result is the unique stable order.
+void stable_sort_by_key(std::vector<int>& order, const std::uint64_t* key) {
+    const long n = static_cast<long>(order.size());
+    auto less = [key](int x, int y) { return key[x] < key[y]; };
+    int P = 1;
+#ifdef _OPENMP
+    P = std::min(16, omp_get_max_threads());
+#endif
+    if (n < 65536 || P < 2) {
+        std::stable_sort(order.begin(), order.end(), less);
+        return;
+    }
+    std::vector<long> b(static_cast<std::size_t>(P) + 1);
+    for (int q = 0; q <= P; ++q) b[static_cast<std::size_t>(q)] = n * q / P;
+#pragma omp parallel for schedule(static, 1) num_threads(P)
+    for (int q = 0; q < P; ++q)
+        std::stable_sort(order.begin() + b[static_cast<std::size_t>(q)], order.begin() + b[static_cast<std::size_t>(q) + 1],
+                         less);
+    for (int w = 1; w < P; w *= 2) {
+#pragma omp parallel for schedule(static, 1) num_threads(P)
+        for (int q = 0; q < P; q += 2 * w) {
+            if (q + w >= P) continue;
+            const long lo = b[static_cast<std::size_t>(q)], mid = b[static_cast<std::size_t>(q + w)],
+                       hi = b[static_cast<std::size_t>(std::min(q + 2 * w, P))];
+            std::inplace_merge(order.begin() + lo, order.begin() + mid, order.begin() + hi, less);
+        }
+    }
 }
 
 }  // namespace taco_engine

@@ -4,7 +4,9 @@
 // and the sanitizers (no device code), each entry exercised on ordinary,
 // edge and error inputs.  Exit status 0 and "host_sanitize: ok" when every
 // check holds; a sanitizer report aborts the run.
+#include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <complex>
 #include <cstdio>
 #include <cstring>
@@ -194,6 +196,18 @@ int main() {
             check(t1.size() == t2.size() && !t1.empty(), "twiddle tables");
         }
         check(next_pow2(1000) == 1024 && fcr_nfft(1000, 0) >= 1999, "FFT sizes");
+    }
+
+    // the parallel stable sort used to group triples equals std::stable_sort
+    for (long n : {1000L, 300000L}) {
+        std::vector<std::uint64_t> key(static_cast<std::size_t>(n));
+        for (auto& k : key) k = g() % 4000;
+        std::vector<int> o1(static_cast<std::size_t>(n));
+        for (long q = 0; q < n; ++q) o1[static_cast<std::size_t>(q)] = static_cast<int>(n - 1 - q);
+        std::vector<int> o2 = o1;
+        std::stable_sort(o1.begin(), o1.end(), [&](int x, int y) { return key[x] < key[y]; });
+        stable_sort_by_key(o2, key.data());
+        check(o1 == o2, "parallel stable sort");
     }
 
     // TCL2 reverse helpers: parts and operator cotangents on a d = 3 system
