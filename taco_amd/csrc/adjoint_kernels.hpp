@@ -96,6 +96,7 @@ struct Rk4AdjArgs {
     double dt;
     int nstage;
     const double2* LsH = nullptr;  // [steps + 1][D * D] conjugate transposes (walk without staging)
+    const double2* LhH = nullptr;  // [steps][D * D] their midpoints (null: formed on the fly)
     double2* vec = nullptr;        // [steps][4][D] kb1..kb4 of every step (Lbar formed by k_rk4_lbar)
 };
 
@@ -228,14 +229,18 @@ __device__ __forceinline__ void rk4_adjoint_body(double2* sh, const Rk4AdjArgs& 
             rb[e] = cadd(lam[e], tmp[e]);
         }
         __syncthreads();
-        if (H0) matvec_half<DT>(H0, H1, D, k3, tmp, D); else matvec_h_half<DT>(L0, L1, ld, k3, tmp, D);  // ub2
+        if (A.LhH && H0) matvec_rows<DT>(A.LhH + n * D2, D, k3, tmp, D);                             // ub2
+        else if (H0) matvec_half<DT>(H0, H1, D, k3, tmp, D);
+        else matvec_h_half<DT>(L0, L1, ld, k3, tmp, D);
         __syncthreads();
         for (int e = threadIdx.x; e < D; e += blockDim.x) {
             kb2[e] = cadd(cscale(lam[e], 2.0 * s6), cscale(tmp[e], h));
             rb[e] = cadd(rb[e], tmp[e]);
         }
         __syncthreads();
-        if (H0) matvec_half<DT>(H0, H1, D, kb2, tmp, D); else matvec_h_half<DT>(L0, L1, ld, kb2, tmp, D);  // ub1
+        if (A.LhH && H0) matvec_rows<DT>(A.LhH + n * D2, D, kb2, tmp, D);                            // ub1
+        else if (H0) matvec_half<DT>(H0, H1, D, kb2, tmp, D);
+        else matvec_h_half<DT>(L0, L1, ld, kb2, tmp, D);
         __syncthreads();
         for (int e = threadIdx.x; e < D; e += blockDim.x) {
             kb1[e] = cadd(cscale(lam[e], s6), cscale(tmp[e], h));

@@ -1694,12 +1694,13 @@ struct Rk4Sys {
     long steps;
     double dt;
     int D;
+    const double2* Lh = nullptr;  // [steps][D x D] midpoints (large D), else formed on the fly
 };
 
 __global__ __launch_bounds__(kRkBlockMax) void k_rk4_batch(const Rk4Sys* sys) {
     extern __shared__ double2 sh[];
     const Rk4Sys S = sys[blockIdx.x];
-#define TACO_RK4_CALL(DT) rk4_any<DT>(sh, S.nstage, S.Ls, nullptr, S.steps, S.D, S.r0, S.r, S.dt, 1, S.samples, nullptr)
+#define TACO_RK4_CALL(DT) rk4_any<DT>(sh, S.nstage, S.Ls, S.Lh, S.steps, S.D, S.r0, S.r, S.dt, 1, S.samples, nullptr)
     TACO_RK4_DISPATCH(S.D, TACO_RK4_CALL)
 #undef TACO_RK4_CALL
 }
@@ -1709,6 +1710,16 @@ __global__ __launch_bounds__(kBlock) void k_gather_rows(const double2* src, cons
     const double2* s = src + static_cast<long>(rows[blockIdx.x]) * len;
     double2* d = dst + static_cast<long>(blockIdx.x) * len;
     for (long e = threadIdx.x; e < len; e += kBlock) d[e] = s[e];
+}
+
+// out[n] = (L[n] + L[n + 1]) / 2 elementwise for n < count (midpoint
+// generators, formed with the expression the RK4 products use on the fly).
+__global__ __launch_bounds__(kBlock) void k_midpoints(const double2* __restrict__ L, long count, long D2,
+                                                      double2* __restrict__ out) {
+    const long total = count * D2;
+    for (long i = blockIdx.x * static_cast<long>(kBlock) + threadIdx.x; i < total;
+         i += static_cast<long>(gridDim.x) * kBlock)
+        out[i] = cscale(cadd(L[i], L[i + D2]), 0.5);
 }
 
 __global__ __launch_bounds__(kBlock) void k_fill_zero(double2* dst, long count) {
@@ -2112,7 +2123,7 @@ struct taco_ctx {
     std::map<int, std::array<int, 3>> fft_choice;  // per FFT log-length: tile config of phases A, B, C
     std::map<int, std::unique_ptr<taco_engine::Twiddles>> tw;
     taco_engine::DevBuf gamma, omegas, mirror, p2f, ops, spec, inter1, inter2, conv, slices, t2pos, l4, gw, misc,
-        rk, eps, jobtri, clsstart, jobmask, jobrow, comb, t2scr;
+        rk, eps, jobtri, clsstart, jobmask, jobrow, comb, t2scr, rkmid;
     taco_engine::PhaseTab phtab;  // phase table of the last plain-path system (TACO_PHASE_LIBM)
     std::size_t l4_lds_set = 0;  // dynamic LDS limit set on the fused L4 kernel
     std::size_t slice_budget = 0;  // bytes of F/C/R slices per pass of the batch builders (0: default)
@@ -3201,6 +3212,21 @@ static int rk4_block(int D) {
     return kBlock;
 }
 
+// Midpoint series of large-D generators kept while it fits 2 GiB.
+static bool rk4_midpoints_fit(int D, std::size_t steps) {
+    return steps * static_cast<std::size_t>(D) * D * 16 <= (std::size_t{2} << 30);
+}
+static const double2* rk4_midpoints(taco_ctx* c, DevBuf& buf, const double2* Ls, std::size_t steps, int D) {
+    const long D2 = static_cast<long>(D) * D;
+    buf.ensure(steps * static_cast<std::size_t>(D2) * 16);
+    const long total = static_cast<long>(steps) * D2;
+    const unsigned grid = static_cast<unsigned>(std::min<long>((total + kBlock - 1) / kBlock, 65536));
+    hipLaunchKernelGGL(k_midpoints, dim3(grid), dim3(kBlock), 0, c->stream, Ls, static_cast<long>(steps), D2,
+                       buf.as<double2>());
+    HIP_OK(hipGetLastError());
+    return buf.as<double2>();
+}
+
 static void rk4_launch(taco_ctx* ctx, const double2* dLs, const double2* dLh, std::size_t steps, int D, double2* dr,
                        double dt, std::size_t every, double2* dsamples, long* dns) {
     // algorithmic bytes: every generator read once, every sample written once
@@ -3217,7 +3243,13 @@ static void rk4_launch(taco_ctx* ctx, const double2* dLs, const double2* dLh, st
         HIP_OK(hipGetLastError());
         return;
     }
-    const int nstage = dLh ? 0 : rk4_nstage(D, 6, static_cast<long>(steps) + 1);
+    int nstage = dLh ? 0 : rk4_nstage(D, 6, static_cast<long>(steps) + 1);
+    if (!dLh && nstage < 2 && steps > 0 && rk4_midpoints_fit(D, steps)) {
+        // generators read from global memory: the midpoints once, so each
+        // step reads four matrices instead of six (the same values)
+        dLh = rk4_midpoints(ctx, ctx->rkmid, dLs, steps, D);
+        nstage = 0;
+    }
     const std::size_t lds = rk4_lds(D, 6, nstage);
     if (lds > kRkLds) throw Error(TACO_ERR_INVALID, "RK4 kernel supports D <= 1706");
     timed(ctx, K_RK4, rk_bytes, [&] {
@@ -3405,6 +3437,7 @@ static L4AdjLists l4_adjoint_lists(const int* p2f, int N, int nf) {
 struct RkRev {
     double2 *gb, *Lb, *seg, *rb;
     double2 *LsH = nullptr, *vec = nullptr;  // deferred walk: conjugate transposes, cotangent vectors
+    double2* LhH = nullptr;                  // and their midpoints
 };
 
 // The walk reads transposed generators and leaves Lbar to k_rk4_lbar when it
@@ -3422,7 +3455,7 @@ static bool rk_rev_deferred(const taco_tape& tp) {
 static std::size_t rk_rev_values(const taco_tape& tp) {
     const std::size_t D = static_cast<std::size_t>(tp.N) * tp.N;
     const std::size_t base = tp.T * D + tp.T * D * D + 4 * tp.steps * D + D;
-    return rk_rev_deferred(tp) ? base + tp.T * D * D + 4 * tp.steps * D : base;
+    return rk_rev_deferred(tp) ? base + tp.T * D * D + 4 * tp.steps * D + tp.steps * D * D : base;
 }
 
 static RkRev rk_rev_at(const taco_tape& tp, double2* base) {
@@ -3435,6 +3468,7 @@ static RkRev rk_rev_at(const taco_tape& tp, double2* base) {
     if (rk_rev_deferred(tp)) {
         b.LsH = b.rb + D;
         b.vec = b.LsH + tp.T * D * D;
+        b.LhH = b.vec + 4 * tp.steps * D;
     }
     return b;
 }
@@ -3467,6 +3501,7 @@ static RkRevShape rk_rev_shape(const taco_tape& tp, const RkRev& b) {
     R.dt = tp.dt;
     R.LsH = b.LsH;
     R.vec = b.vec;
+    R.LhH = b.LhH;
     sh.carry = D2 <= static_cast<std::size_t>(kRkCarry) * kBlock;
     sh.nseg = (steps + K - 1) / K;
     const int Di = static_cast<int>(D);
@@ -3496,6 +3531,10 @@ static void rk_rev_run(taco_tape& tp, const RkRev& b) {
         const int tiles = (Di + 15) / 16;
         hipLaunchKernelGGL(k_conj_transpose, dim3(static_cast<unsigned>(tiles * tiles), static_cast<unsigned>(tp.T)),
                            dim3(kBlock), 0, c->stream, tp.lgen.as<double2>(), b.LsH, Di, static_cast<long>(tp.T));
+        const long D2l = static_cast<long>(D2), total = static_cast<long>(tp.steps) * D2l;
+        hipLaunchKernelGGL(k_midpoints, dim3(static_cast<unsigned>(std::min<long>((total + kBlock - 1) / kBlock, 65536))),
+                           dim3(kBlock), 0, c->stream, static_cast<const double2*>(b.LsH), static_cast<long>(tp.steps),
+                           D2l, b.LhH);
     } else if (!sh.carry) {
         const long cntL = static_cast<long>(tp.T * D2);
         hipLaunchKernelGGL(k_fill_zero, dim3(static_cast<unsigned>((cntL + kBlock - 1) / kBlock)), dim3(kBlock), 0,
@@ -4024,6 +4063,7 @@ struct taco_mtape {
     taco_engine::DevBuf samp;    // [S][T][D] sampled states
     taco_engine::DevBuf rv;      // [S][rk_rev_values] reverse RK4 state
     taco_engine::DevBuf args;    // launch arguments
+    taco_engine::DevBuf mid;     // [S][steps][D^2] midpoint generators of the forward RK4 (large D)
 };
 
 namespace taco_engine {
@@ -4064,10 +4104,20 @@ static void mtape_forward(taco_mtape& mt, const taco_tape_set* in, double* const
     const std::size_t lds = rows_thr > 0 ? static_cast<std::size_t>(3) * D * 16 : rk4_lds(Di, 6, nstage);
     if (lds > kRkLds) throw Error(TACO_ERR_INVALID, "RK4 kernel supports D <= 1706");
     std::vector<Rk4Sys> sys(S);
+    const bool mids = rows_thr == 0 && nstage < 2 && rk4_midpoints_fit(Di, t0.steps * S);
+    if (mids) mt.mid.ensure(S * t0.steps * D * D * 16);
     for (std::size_t s = 0; s < S; ++s) {
         const taco_tape& tp = *mt.sets[s];
         sys[s] = Rk4Sys{tp.lgen.as<double2>(), nstage, r0d + s * D, rend + s * D, samp + s * T * D,
                         static_cast<long>(tp.steps), tp.dt, Di};
+        if (mids) {
+            const long D2 = static_cast<long>(D) * D, total = static_cast<long>(tp.steps) * D2;
+            double2* m = mt.mid.as<double2>() + s * tp.steps * D * D;
+            hipLaunchKernelGGL(k_midpoints, dim3(static_cast<unsigned>(std::min<long>((total + kBlock - 1) / kBlock,
+                                                                                     65536))),
+                               dim3(kBlock), 0, c->stream, tp.lgen.as<double2>(), static_cast<long>(tp.steps), D2, m);
+            sys[s].Lh = m;
+        }
     }
     mt.args.ensure(std::max(S * sizeof(Rk4Sys), 2 * S * sizeof(Rk4AdjArgs)));
     HIP_OK(hipMemcpyAsync(mt.args.p, sys.data(), S * sizeof(Rk4Sys), hipMemcpyHostToDevice, c->stream));
@@ -4107,6 +4157,11 @@ static void mtape_adjoint(taco_mtape& mt, const double* const* samples_bar, doub
             hipLaunchKernelGGL(k_conj_transpose, dim3(static_cast<unsigned>(tiles * tiles), static_cast<unsigned>(T)),
                                dim3(kBlock), 0, c->stream, mt.sets[s]->lgen.as<double2>(), rv[s].LsH,
                                static_cast<int>(D), static_cast<long>(T));
+            const long D2l = static_cast<long>(D2), total = static_cast<long>(T - 1) * D2l;
+            hipLaunchKernelGGL(k_midpoints,
+                               dim3(static_cast<unsigned>(std::min<long>((total + kBlock - 1) / kBlock, 65536))),
+                               dim3(kBlock), 0, c->stream, static_cast<const double2*>(rv[s].LsH),
+                               static_cast<long>(T - 1), D2l, rv[s].LhH);
         } else if (!sh.carry) {
             const long cntL = static_cast<long>(T * D2);
             hipLaunchKernelGGL(k_fill_zero, dim3(static_cast<unsigned>((cntL + kBlock - 1) / kBlock)), dim3(kBlock),

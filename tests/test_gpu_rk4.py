@@ -165,3 +165,18 @@ def test_tcl2_skips_channels_whose_adag_a_vanishes(orc):
     for q, t in enumerate([2, 7]):
         ref = orc.tcl2_general_at(S.eps, S.A_eig, S.pair_to_freq, G[t])
         assert np.abs(got[q] - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("D", [100, 256])
+def test_large_d_midpoints_equal_explicit_half_series(D):
+    """At large D the forward RK4 reads midpoint generators formed once on
+    the device; they are the values (L_n + L_{n+1})/2 the products form on
+    the fly, so the run equals the L_half_series overload given the same
+    midpoints, bit for bit."""
+    rng = np.random.default_rng(D)
+    steps = 12
+    L = (rng.standard_normal((steps + 1, D, D)) + 1j * rng.standard_normal((steps + 1, D, D))) * (0.5 / np.sqrt(D))
+    r0 = rng.standard_normal(D) + 1j * rng.standard_normal(D)
+    a = taco_amd.propagate_rk4_dense(L, r0, 0.0, 0.01)
+    b = taco_amd.propagate_rk4_dense(L, r0, 0.0, 0.01, L_half_series=(L[:-1] + L[1:]) * 0.5)
+    assert np.array_equal(a, b)
