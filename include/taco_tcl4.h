@@ -399,6 +399,24 @@ int taco_tape_forward(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, con
 int taco_tape_adjoint(taco_tape* tape, const double* samples_bar, double* gamma_bar, double* omega_bar,
                       double* ops_bar, double* eps_bar, double* r0_bar);
 int taco_tape_destroy(taco_tape* tape);
+/* Bath-sourced tape (no reference counterpart; for gradient callers that
+ * differentiate through the bath): the Gamma table of taco_tape_forward is
+ * made on the device from the bath instead of read from the host -- the
+ * correlation C(t_n), n < Nt, of bcf_fft_fun (correlation_fft.hpp:154-214)
+ * for spectral density `kind` (TACO_J_*) with parameters (p0, p1) at inverse
+ * temperature beta > 0, then its trapz Gamma
+ * (compute_trapz_prefix_multi_matrix, gamma.hpp:166-213), bit-identical to
+ * taco_bath_gamma_hip(Nt - 1, ...) -- and C and Gamma stay in device memory.
+ * taco_tape_adjoint_bath returns bath_bar[3] = dLoss/d(p0, p1, beta) (beta
+ * finite) and omega_bar including the trapz Gamma's dependence on the bucket
+ * frequencies; the other outputs are those of taco_tape_adjoint (nullable).
+ * taco_tape_adjoint and taco_tape_destroy accept such a tape as well. */
+int taco_tape_forward_bath(taco_ctx* ctx, int kind, double p0, double p1, double beta, size_t Nt, int nf,
+                           const double* omegas, const int* mirror, const int* pair_to_freq, int N, const double* eps,
+                           const double* ops, int num_ops, double dt, size_t steps, int order, const double* r0,
+                           size_t checkpoint_every, double* samples_out, taco_tape** out);
+int taco_tape_adjoint_bath(taco_tape* tape, const double* samples_bar, double* bath_bar, double* omega_bar,
+                           double* ops_bar, double* eps_bar, double* r0_bar);
 /* Multi-set tape (no reference counterpart): S parameter sets of one model
  * size -- the same N and steps; Gamma, buckets, operators, eigenvalues, dt,
  * order and r0 per set -- forward and reverse together.  Each set's F/C/R,

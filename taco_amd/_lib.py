@@ -108,6 +108,10 @@ SIGNATURES = {
                                   c_size_t, c_int, _dp, c_size_t, _dp, POINTER(c_void_p)]),
     "taco_tape_adjoint": (c_int, [c_void_p, _dp, _dp, _dp, _dp, _dp, _dp]),
     "taco_tape_destroy": (c_int, [c_void_p]),
+    "taco_tape_forward_bath": (c_int, [c_void_p, c_int, c_double, c_double, c_double, c_size_t, c_int, _dp, _ip, _ip,
+                                       c_int, _dp, _dp, c_int, c_double, c_size_t, c_int, _dp, c_size_t, _dp,
+                                       POINTER(c_void_p)]),
+    "taco_tape_adjoint_bath": (c_int, [c_void_p, _dp, _dp, _dp, _dp, _dp, _dp]),
     "taco_mtape_forward": (c_int, [c_void_p, POINTER(TapeSet), c_int, c_size_t, _dpp, POINTER(c_void_p)]),
     "taco_mtape_adjoint": (c_int, [c_void_p, _dpp, _dpp, _dpp, _dpp, _dpp, _dpp]),
     "taco_mtape_keeps_slices": (c_int, [c_void_p]),

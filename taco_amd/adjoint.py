@@ -11,9 +11,13 @@ no derivatives).
 * ``BcfFun``, ``GammaFun``, ``EighFun`` and ``BucketOmegas`` carry the
   parameter chain in front of it: bath spectral density -> C(t)
   (correlation_fft.hpp:154-214), trapz Gamma (gamma.hpp:166-213), the
-  eigensystem of H (system.hpp:37-157).  Their forwards are the engine's host
+  eigensystem of H (system.hpp:37-157).  Their forwards are the engine's
   routines (bit-identical to the oracle's); their backwards are written out
   below.
+* ``BathTape`` / ``BathTCL4Propagate`` fuse BcfFun -> GammaFun ->
+  TCL4Propagate into one device tape (taco_tape_forward_bath): C(t) and
+  Gamma are made, kept and reversed in device memory, and the reverse
+  returns the bath-parameter cotangents directly.
 * ``SpinPairProblem`` is the differentiation workload: H = H1 (x) I + I (x) H1,
   H1 = eps/2 sz + Delta/2 sx, A = sz (x) I + I (x) sz, Ohmic bath
   (alpha, omega_c, beta), loss sum_t ||rho_lab(t) - rho_target(t)||_F^2.
@@ -38,7 +42,8 @@ from ._lib import TacoInvalidArgument, check, dptr, iptr, lib
 from .tcl4 import (Exec, Ohmic, DrudeLorentz, System, _ctx_for, bath_gamma_hip, bcf_fft_fun,
                    compute_trapz_prefix_multi_matrix, compute_trapz_prefix_multi_matrix_hip)
 
-__all__ = ["Tape", "MultiTape", "TCL4Propagate", "TCL4PropagateMany", "BcfFun", "GammaFun", "EighFun", "BucketOmegas", "SpinPairProblem"]
+__all__ = ["Tape", "BathTape", "MultiTape", "TCL4Propagate", "BathTCL4Propagate", "TCL4PropagateMany", "BcfFun",
+           "GammaFun", "EighFun", "BucketOmegas", "SpinPairProblem", "TfimChainProblem"]
 
 
 def _cm(a) -> np.ndarray:
@@ -101,6 +106,57 @@ class Tape:
             self.close()
         except Exception:
             pass
+
+
+class BathTape(Tape):
+    """A Tape whose Gamma table the device makes from a bath
+    (taco_tape_forward_bath): C(t_n), n < Nt, of spectral density `kind`
+    (0 Ohmic, 1 Drude-Lorentz) with parameters (p0, p1) at inverse temperature
+    beta, and its trapz Gamma, both kept in device memory.  `adjoint` returns
+    the cotangents of (p0, p1, beta) under "bath" instead of a Gamma
+    cotangent, and omega_bar includes the trapz Gamma's part."""
+
+    def __init__(self, kind: int, p0: float, p1: float, beta: float, Nt: int, omegas, mirror, pair_to_freq, ops,
+                 eps, dt: float, steps: int, r0, order: int = 4, checkpoint_every: int = 64,
+                 exec: Optional[Exec] = None):
+        self.ctx = _ctx_for(exec)
+        om = np.ascontiguousarray(omegas, np.float64)
+        self.Nt, self.nf = int(Nt), om.size
+        p2f = np.asarray(pair_to_freq, np.int32)
+        self.N = p2f.shape[0]
+        self.num_ops = len(ops)
+        self.steps = int(steps)
+        mi = np.ascontiguousarray(mirror, np.int32)
+        pf = np.ascontiguousarray(p2f.T).ravel()
+        of = np.concatenate([_cm(o) for o in ops])
+        ev = np.ascontiguousarray(eps, np.float64)
+        r = np.ascontiguousarray(np.asarray(r0, np.complex128).ravel())
+        if r.size != self.N ** 2:
+            raise TacoInvalidArgument(1, "r0 must have N^2 entries")
+        self.samples = np.zeros((self.steps + 1, self.N ** 2), np.complex128)
+        h = c_void_p()
+        check(lib().taco_tape_forward_bath(self.ctx.handle, int(kind), float(p0), float(p1), float(beta), self.Nt,
+                                           self.nf, dptr(om), iptr(mi), iptr(pf), self.N, dptr(ev), dptr(of),
+                                           self.num_ops, float(dt), self.steps, int(order), dptr(r),
+                                           int(checkpoint_every), dptr(self.samples), ctypes.byref(h)))
+        self.handle = h
+
+    def adjoint(self, samples_bar) -> dict:
+        sb = np.ascontiguousarray(np.asarray(samples_bar, np.complex128).reshape(self.steps + 1, self.N ** 2))
+        bb = np.zeros(3)
+        ob = np.zeros(self.nf)
+        ab = np.zeros(self.num_ops * self.N ** 2, np.complex128)
+        eb = np.zeros(self.N)
+        rb = np.zeros(self.N ** 2, np.complex128)
+        check(lib().taco_tape_adjoint_bath(self.handle, dptr(sb), dptr(bb), dptr(ob), dptr(ab), dptr(eb), dptr(rb)))
+        N = self.N
+        return {
+            "bath": bb,
+            "omegas": ob,
+            "ops": [ab[k * N * N:(k + 1) * N * N].reshape(N, N).T.copy() for k in range(self.num_ops)],
+            "eps": eb,
+            "r0": rb,
+        }
 
 
 class MultiTape:
@@ -204,6 +260,33 @@ class TCL4Propagate(torch.autograd.Function):
     def backward(ctx, gs):
         res = ctx.tape.adjoint(gs.detach().numpy())
         return (torch.from_numpy(res["gamma"]), torch.from_numpy(res["omegas"]),
+                torch.from_numpy(np.stack(res["ops"])), torch.from_numpy(res["eps"]), torch.from_numpy(res["r0"]),
+                None)
+
+
+class BathTCL4Propagate(torch.autograd.Function):
+    """BcfFun -> GammaFun -> TCL4Propagate in one step on the device
+    (BathTape): r_0..r_steps as a differentiable function of the bath
+    parameters (p0, p1, beta) and (omegas [nf], ops [num_ops, N, N], eps [N],
+    r0 [N^2]), with C(t) and Gamma made and reversed in device memory.
+    ``meta`` = (kind, Nt, mirror, pair_to_freq, dt, steps, order,
+    checkpoint_every, exec)."""
+
+    @staticmethod
+    def forward(ctx, p0, p1, beta, omegas, ops, eps, r0, meta):
+        kind, Nt, mirror, p2f, dt, steps, order, K, ex = meta
+        tape = BathTape(kind, float(p0), float(p1), float(beta), Nt, omegas.detach().numpy(), mirror, p2f,
+                        list(ops.detach().numpy()), eps.detach().numpy(), dt, steps, r0.detach().numpy(), order, K,
+                        ex)
+        ctx.tape = tape
+        return torch.from_numpy(tape.samples.copy())
+
+    @staticmethod
+    def backward(ctx, gs):
+        res = ctx.tape.adjoint(gs.detach().numpy())
+        t = lambda v: torch.tensor(float(v), dtype=torch.float64)
+        pb = res["bath"]
+        return (t(pb[0]), t(pb[1]), t(pb[2]), torch.from_numpy(res["omegas"]),
                 torch.from_numpy(np.stack(res["ops"])), torch.from_numpy(res["eps"]), torch.from_numpy(res["r0"]),
                 None)
 
@@ -445,6 +528,9 @@ class SpinPairProblem:
     checkpoint_every: int = 64
     order: int = 4
     freq_tol: float = 1e-9
+    # "device": bath chain and TCL4 run in one device tape (BathTCL4Propagate);
+    # "split": BcfFun -> GammaFun -> TCL4Propagate (host reverses of the chain)
+    bath_chain: str = "device"
 
     names = ("alpha", "omega_c", "beta", "eps", "delta")
 
@@ -479,17 +565,24 @@ def _rho_lab(prob, H: torch.Tensor, A: np.ndarray, kind: int, p0, p1, beta, exec
     e, U = EighFun.apply(H, [A], prob.freq_tol)
     S = System.build(H.detach().numpy(), [A], prob.freq_tol)
     om = BucketOmegas.apply(e, S.omegas, S.pair_to_freq)
-    # bath correlation and Gamma from the device producers
-    producer = exec or Exec()
-    C = BcfFun.apply(p0, p1, beta, (kind, prob.Nt - 1, prob.dt, producer))
-    G = GammaFun.apply(C, om, prob.dt, producer)
     At = torch.from_numpy(A)
     Aeig = (U.conj().T @ At @ U).unsqueeze(0)
     N = S.N
     r0 = torch.zeros(N * N, dtype=torch.complex128)
     r0[0] = 1.0
-    meta = (S.mirror, S.pair_to_freq, prob.dt, prob.steps, prob.order, prob.checkpoint_every, exec)
-    r = TCL4Propagate.apply(G, om, Aeig, e, r0, meta)
+    if prob.bath_chain == "device":
+        # C(t) and Gamma made, kept and reversed in device memory (BathTape)
+        meta = (kind, prob.Nt, S.mirror, S.pair_to_freq, prob.dt, prob.steps, prob.order, prob.checkpoint_every,
+                exec)
+        r = BathTCL4Propagate.apply(p0, p1, beta, om, Aeig, e, r0, meta)
+    else:
+        # the same values through the separate steps: C and Gamma from the
+        # device producers, their reverses on the host
+        producer = exec or Exec()
+        C = BcfFun.apply(p0, p1, beta, (kind, prob.Nt - 1, prob.dt, producer))
+        G = GammaFun.apply(C, om, prob.dt, producer)
+        meta = (S.mirror, S.pair_to_freq, prob.dt, prob.steps, prob.order, prob.checkpoint_every, exec)
+        r = TCL4Propagate.apply(G, om, Aeig, e, r0, meta)
     rho = r.reshape(-1, N, N).transpose(1, 2)  # vec column-major: rho[n, m] = r[n + m N]
     return U @ rho @ U.conj().T
 
@@ -542,6 +635,7 @@ class TfimChainProblem:
     checkpoint_every: int = 64
     order: int = 4
     freq_tol: float = 1e-9
+    bath_chain: str = "device"  # as SpinPairProblem.bath_chain
 
     names = ("lambda", "gamma", "beta", "J", "h")
 

@@ -1392,3 +1392,159 @@ __global__ __launch_bounds__(kBlock) void k_tcl2_adj_wsum(Tcl2AdjArgs A) {
     for (int c = 0; c < A.nchunk; ++c) s = cadd(s, A.part[static_cast<long>(c) * A.nout + o]);
     A.wsum[o] = s;
 }
+
+// ---------------------------------------------------------------------------
+// Reverse of the bath chain in front of a tape whose Gamma table the device
+// produced: Gamma_b(t_k) = (dt/2) sum_{m=1..k} (phi^{m-1} C_{m-1} + phi^m C_m),
+// phi = exp(i w_b dt) (gamma.hpp:166-213), of C = (dw/pi) FFT(S), S the
+// sampled KMS spectrum of J(w) (correlation_fft.hpp:154-214).  Every sum runs
+// in a fixed order, so the cotangents repeat bit for bit.
+// ---------------------------------------------------------------------------
+
+// out = a + b elementwise (either may be null: that part is zero).
+__global__ __launch_bounds__(kBlock) void k_gbar_sum(const double2* a, const double2* b, long n, double2* out) {
+    const long i = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    double2 s = a ? a[i] : make_double2(0.0, 0.0);
+    if (b) s = cadd(s, b[i]);
+    out[i] = s;
+}
+
+// One workgroup per bucket b.  Gamma_b(k) weighs phi^m C_m by dt/2 for m = 0
+// (k >= 1) and m = k, by dt for 0 < m < k, so its cotangent is (dt/2) Q_b(m)
+// with Q(0) = sum_{k>=1} Gbar(k) and Q(m) = Gbar(m) + 2 sum_{k>m} Gbar(k).
+// Writes Q_b [T] and omega_bar_b = (dt/2) Re sum_m conj(Q(m)) (i m dt) phi^m C_m.
+// Each thread owns a contiguous run of m; the runs' suffix offsets come from
+// one serial pass over the run totals.
+__global__ __launch_bounds__(kBlock) void k_trapz_adj_q(const double2* gbar, long T, const double2* C, double dt,
+                                                        const double* omegas, double2* Q, double* wbar) {
+    __shared__ double2 part[kBlock];
+    __shared__ double red[kBlock / 64];
+    const int b = blockIdx.x;
+    const double2* g = gbar + static_cast<long>(b) * T;
+    double2* q = Q + static_cast<long>(b) * T;
+    const long seg = (T + kBlock - 1) / kBlock;
+    const long m0 = min(T, static_cast<long>(threadIdx.x) * seg), m1 = min(T, m0 + seg);
+    double2 s = make_double2(0.0, 0.0);
+    for (long m = m1 - 1; m >= m0; --m) s = cadd(s, g[m]);
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double2 run = make_double2(0.0, 0.0);
+        for (int i = kBlock - 1; i >= 0; --i) {
+            const double2 p = part[i];
+            part[i] = run;
+            run = cadd(run, p);
+        }
+    }
+    __syncthreads();
+    double2 run = part[threadIdx.x];  // sum_{k >= m1} Gbar(k)
+    const double w = omegas[b];
+    double acc = 0.0;
+    for (long m = m1 - 1; m >= m0; --m) {
+        const double2 gm = g[m];
+        const double2 qm = m == 0 ? run : make_double2(gm.x + 2.0 * run.x, gm.y + 2.0 * run.y);
+        q[m] = qm;
+        const double md = static_cast<double>(m) * dt;
+        const double2 pc = cmul(cexpi(md * w), C[m]);  // phi^m C_m
+        acc += md * (qm.y * pc.x - qm.x * pc.y);        // Re(conj(Q) i pc) m dt
+        run = cadd(run, gm);
+    }
+    const double tot = block_sum(acc, red);
+    if (threadIdx.x == 0) wbar[b] = 0.5 * dt * tot;
+}
+
+// x[m] = conj(C_bar(m)) for m < T, C_bar(m) = (dt/2) sum_b conj(phi_b^m) Q_b(m)
+// (buckets in order), and 0 for T <= m < n: the input of the forward FFT that
+// gives sum_m C_bar(m) exp(+2 pi i k m / n) as its conjugate.
+__global__ __launch_bounds__(kBlock) void k_trapz_adj_c(const double2* Q, long T, int nf, double dt,
+                                                        const double* omegas, long n, double2* x) {
+    const long m = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (m >= n) return;
+    double2 s = make_double2(0.0, 0.0);
+    if (m < T) {
+        const double md = static_cast<double>(m) * dt;
+        for (int b = 0; b < nf; ++b) s = cadd(s, cmul(cconj(cexpi(md * omegas[b])), Q[static_cast<long>(b) * T + m]));
+    }
+    x[m] = make_double2(0.5 * dt * s.x, -(0.5 * dt * s.y));
+}
+
+struct BcfAdjArgs {
+    const double2* spec;  // X[k] at ((k mod 2^a) << b) + (k >> a)
+    int a, b;
+    long nfft;
+    double scale;         // dw / pi
+    double domega, beta, p0, p1;
+    int kind;
+    double* part;         // [gridDim.x][3]
+};
+
+__device__ __forceinline__ double bcf_adj_sbar(const BcfAdjArgs& A, long k) {
+    const long idx = ((k & ((1L << A.a) - 1)) << A.b) + (k >> A.a);
+    return A.scale * A.spec[idx].x;
+}
+
+// J(w) and dJ/dp0, dJ/dp1 for w > 0.
+__device__ __forceinline__ void spectral_J_grad(int kind, double p0, double p1, double w, double& J, double& d0,
+                                                double& d1) {
+    if (kind == 0) {  // p0 w exp(-w / p1)
+        const double e = taco_cr::exp_cr(-w / p1, c_exptab);
+        J = p0 * w * e;
+        d0 = w * e;
+        d1 = J * w / (p1 * p1);
+    } else {  // 2 p0 p1 w / (w^2 + p1^2)
+        const double den = w * w + p1 * p1;
+        J = 2.0 * p0 * p1 * w / den;
+        d0 = 2.0 * p1 * w / den;
+        d1 = 2.0 * p0 * w * (w * w - p1 * p1) / (den * den);
+    }
+}
+
+// Per k = 1..nfft/2-1 (a thread each): with Sbar = (dw/pi) Re X the spectrum
+// cotangent, kms = exp(-beta w), f = (pi/2) / (1 - kms), S_k = f J and
+// S_{nfft-k} = f J kms, the terms of dLoss/d(p0, p1, beta); per-workgroup
+// sums (block_sum: a fixed shuffle tree) into part.
+__global__ __launch_bounds__(kBlock) void k_bcf_adj_terms(BcfAdjArgs A) {
+    __shared__ double red[kBlock / 64];
+    constexpr double kPiD = 3.14159265358979323846;
+    const long half = A.nfft / 2;
+    const long k = 1 + static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+    double g0 = 0.0, g1 = 0.0, gb = 0.0;
+    if (k < half) {
+        const double sk = bcf_adj_sbar(A, k), sm = bcf_adj_sbar(A, A.nfft - k);
+        const double w = A.domega * static_cast<double>(k);
+        double J, d0, d1;
+        spectral_J_grad(A.kind, A.p0, A.p1, w, J, d0, d1);
+        const double kms = taco_cr::exp_cr(-A.beta * w, c_exptab);
+        const double f = (kPiD / 2.0) / (1.0 - kms);
+        const double sp = f * J;
+        const double dsp_db = (kPiD / 2.0) * J * (-w * kms) / ((1.0 - kms) * (1.0 - kms));
+        g0 = sk * f * d0 + sm * f * kms * d0;
+        g1 = sk * f * d1 + sm * f * kms * d1;
+        gb = sk * dsp_db + sm * (dsp_db * kms + sp * (-w * kms));
+    }
+    g0 = block_sum(g0, red);
+    g1 = block_sum(g1, red);
+    gb = block_sum(gb, red);
+    if (threadIdx.x == 0) {
+        A.part[3L * blockIdx.x + 0] = g0;
+        A.part[3L * blockIdx.x + 1] = g1;
+        A.part[3L * blockIdx.x + 2] = gb;
+    }
+}
+
+// out[0..2] = the workgroup sums in order; out[3] = Sbar_0, out[4] = Sbar_{nfft/2}.
+__global__ void k_bcf_adj_final(BcfAdjArgs A, int nblocks, double* out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int i = 0; i < nblocks; ++i) {
+        s0 += A.part[3L * i + 0];
+        s1 += A.part[3L * i + 1];
+        s2 += A.part[3L * i + 2];
+    }
+    out[0] = s0;
+    out[1] = s1;
+    out[2] = s2;
+    out[3] = bcf_adj_sbar(A, 0);
+    out[4] = bcf_adj_sbar(A, A.nfft / 2);
+}

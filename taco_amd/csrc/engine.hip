@@ -2042,12 +2042,14 @@ struct PhaseTab {
 // gradients run without device allocations once the sizes have settled.
 struct TapeBufs {
     DevBuf gamma, omegas, mirror, p2f, ops, eps, t2pos, jobtri, jobmask, clsstart, jobrow, slices, lgen, ck, adjl;
+    DevBuf bath;  // bath-sourced tapes: C(t), phase steps and the bath reverse's scratch
     PhaseTab phtab;
     void swap(TapeBufs& o) {
         for (auto [a, b] : {std::pair{&gamma, &o.gamma}, {&omegas, &o.omegas}, {&mirror, &o.mirror}, {&p2f, &o.p2f},
                             {&ops, &o.ops}, {&eps, &o.eps}, {&t2pos, &o.t2pos}, {&jobtri, &o.jobtri},
                             {&jobmask, &o.jobmask}, {&clsstart, &o.clsstart}, {&jobrow, &o.jobrow},
-                            {&slices, &o.slices}, {&lgen, &o.lgen}, {&ck, &o.ck}, {&adjl, &o.adjl}})
+                            {&slices, &o.slices}, {&lgen, &o.lgen}, {&ck, &o.ck}, {&adjl, &o.adjl},
+                            {&bath, &o.bath}})
             a->swap(*b);
         phtab.swap(o.phtab);
     }
@@ -3170,7 +3172,11 @@ struct taco_tape {
     std::size_t Nt = 0, T = 0, steps = 0;
     int nf = 0, N = 0, num_ops = 0, order = 4, K = 64;
     double dt = 0.0;
-    std::vector<std::complex<double>> h_gamma;  // [nf][T] rows t < T
+    // Gamma made on the device from a bath (taco_tape_forward_bath): the
+    // spectral density and its parameters, for the bath reverse
+    bool bath = false;
+    int bath_kind = 0;
+    double bath_p0 = 0.0, bath_p1 = 0.0, bath_beta = 0.0;
     std::vector<double> h_eps, h_omegas;
     std::vector<int> h_p2f, h_mirror;
     std::vector<std::complex<double>> h_ops;
@@ -3570,7 +3576,10 @@ static void rk_rev_run(taco_tape& tp, const RkRev& b) {
 }
 
 static void tape_adjoint_tail(taco_tape& tp, const RkRev& rv, const double2* sl, double* gamma_bar,
-                              double* omega_bar, double* ops_bar, double* eps_bar, double* r0_bar);
+                              double* omega_bar, double* ops_bar, double* eps_bar, double* r0_bar,
+                              double* bath_bar = nullptr);
+static void tape_bath_adjoint(taco_tape& tp, const double2* gb4, const double2* g2, double* bath_bar,
+                              std::vector<double>& omb);
 
 // The term lists depend on (N, nf, pair_to_freq) only: built once per
 // context and structure (the last 8 kept), reused by every later reverse.
@@ -3586,20 +3595,22 @@ static const L4AdjLists& l4_adjoint_lists_cached(taco_ctx* c, const int* p2f, in
 }
 
 static void tape_adjoint(taco_tape& tp, const double* samples_bar, double* gamma_bar, double* omega_bar,
-                         double* ops_bar, double* eps_bar, double* r0_bar) {
+                         double* ops_bar, double* eps_bar, double* r0_bar, double* bath_bar = nullptr) {
     taco_ctx* c = tp.ctx;
     const std::size_t D = static_cast<std::size_t>(tp.N) * tp.N;
     c->rk.ensure(rk_rev_values(tp) * 16);
     const RkRev rv = rk_rev_at(tp, c->rk.as<double2>());
     HIP_OK(hipMemcpyAsync(rv.gb, samples_bar, tp.T * D * 16, hipMemcpyHostToDevice, c->stream));
     rk_rev_run(tp, rv);
-    tape_adjoint_tail(tp, rv, tp.slices.as<double2>(), gamma_bar, omega_bar, ops_bar, eps_bar, r0_bar);
+    tape_adjoint_tail(tp, rv, tp.slices.as<double2>(), gamma_bar, omega_bar, ops_bar, eps_bar, r0_bar, bath_bar);
 }
 
 // Everything after the reverse RK4: the L4 reverse over the slices sl, the
-// F/C/R and TCL2 reverses, and the cotangents to the host.
+// F/C/R and TCL2 reverses, and the cotangents to the host (for a
+// bath-sourced tape with bath_bar set, also the bath chain's reverse).
 static void tape_adjoint_tail(taco_tape& tp, const RkRev& rv, const double2* sl, double* gamma_bar,
-                              double* omega_bar, double* ops_bar, double* eps_bar, double* r0_bar) {
+                              double* omega_bar, double* ops_bar, double* eps_bar, double* r0_bar,
+                              double* bath_bar) {
     taco_ctx* c = tp.ctx;
     const std::size_t N2 = static_cast<std::size_t>(tp.N) * tp.N, D = N2, D2 = D * D;
     const std::size_t T = tp.T;
@@ -3637,6 +3648,8 @@ static void tape_adjoint_tail(taco_tape& tp, const RkRev& rv, const double2* sl,
     std::vector<std::complex<double>> gbar_h(static_cast<std::size_t>(tp.nf) * T, {0.0, 0.0});
     std::vector<std::complex<double>> opsb(static_cast<std::size_t>(tp.num_ops) * N2, {0.0, 0.0});
     std::vector<double> omb(static_cast<std::size_t>(tp.nf), 0.0), epsb(static_cast<std::size_t>(tp.N), 0.0);
+    const double2* gb4_dev = nullptr;  // Gamma_bar of the L4 part on the device, when there is one
+    if (bath_bar && !tp.bath) throw Error(TACO_ERR_INVALID, "bath cotangents need a bath-sourced tape");
     if (tp.order == 4) {
         const FcrJob& J = tp.fcr;
         const long ncomb = J.ncomb;
@@ -3796,6 +3809,7 @@ static void tape_adjoint_tail(taco_tape& tp, const RkRev& rv, const double2* sl,
             hipLaunchKernelGGL(k_bucket_reduce, dim3(nub, static_cast<unsigned>(tp.nf)), dim3(kBlock), 0, c->stream,
                                scr, static_cast<long>(T), dstart, dent, static_cast<long>(T), gb4);
             HIP_OK(hipGetLastError());
+            gb4_dev = gb4;
             HIP_OK(hipMemcpyAsync(gbar_h.data(), gb4, gbar_h.size() * 16, hipMemcpyDeviceToHost, c->stream));
             std::vector<double> omp(static_cast<std::size_t>(ncomb));
             HIP_OK(hipMemcpyAsync(omp.data(), om, omp.size() * 8, hipMemcpyDeviceToHost, c->stream));
@@ -3850,6 +3864,7 @@ static void tape_adjoint_tail(taco_tape& tp, const RkRev& rv, const double2* sl,
                            t2w, t2w + nW, t2P, t2b, tp.N, t2Pb);
         HIP_OK(hipGetLastError());
     }
+    if (bath_bar) tape_bath_adjoint(tp, gb4_dev, tp.order != 0 ? t2g : nullptr, bath_bar, omb);
     std::vector<std::complex<double>> g2(nfz * T), wv(nV + static_cast<std::size_t>(tp.N)), r0b(D), pb(npart * N2);
     HIP_OK(hipMemcpyAsync(g2.data(), t2g, g2.size() * 16, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipMemcpyAsync(wv.data(), t2w + nW, wv.size() * 16, hipMemcpyDeviceToHost, c->stream));
@@ -3884,6 +3899,51 @@ static void tape_adjoint_tail(taco_tape& tp, const RkRev& rv, const double2* sl,
 
 namespace taco_engine {
 constexpr double kPiHost = 3.141592653589793238462643383279502884;
+
+// Forward FFT of x [2^L values] on the device, the reference's radix-2
+// recurrence FFT bit for bit: for L <= 2 in place on one thread (tw: room for
+// 4 complex twiddles), else by the spectrum-mode phase kernels into c->spec.
+// X[k] sits at ((k mod 2^a) << b) + (k >> a).
+struct SpecOut {
+    const double2* X;
+    int a, b;
+};
+static SpecOut fft_forward_dev(taco_ctx* c, double2* x, int L, double2* tw_tiny) {
+    const std::size_t n = std::size_t{1} << L;
+    if (L <= 2) {
+        std::vector<double> tab;
+        make_twiddles(L, tab);
+        HIP_OK(hipMemcpyAsync(tw_tiny, tab.data(), tab.size() * 8, hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_fft_tiny, dim3(1), dim3(64), 0, c->stream, x, L, tw_tiny);
+        HIP_OK(hipStreamSynchronize(c->stream));  // the host table goes out of scope
+        HIP_OK(hipGetLastError());
+        return {x, L, 0};
+    }
+    const Twiddles& tw = twiddles_for(c, L);
+    FcrArgs P{};
+    P.gamma = x;
+    P.Nt = static_cast<long>(n);
+    P.Nin = static_cast<long>(n);
+    P.L = L;
+    P.a = (L + 1) / 2;
+    P.b = L - P.a;
+    P.n = static_cast<long>(n);
+    P.twStd = tw.std_.as<double2>();
+    P.twB = tw.b_.as<double2>();
+    P.twC = tw.c_.as<double2>();
+    c->inter1.ensure(n * 16);
+    c->spec.ensure(n * 16);
+    P.inter1 = c->inter1.as<double2>();
+    P.spec = c->spec.as<double2>();
+    P.njobs = 1;
+    auto it = c->fft_choice.find(L);
+    const int cfgA = it != c->fft_choice.end() && it->second[0] >= 0 ? it->second[0] : 2;
+    const int cfgB = it != c->fft_choice.end() && it->second[1] >= 0 ? it->second[1] : 2;
+    launch_phase(cfgA, PH_A_SPEC, P, 1, c->stream);
+    launch_phase(cfgB, PH_B_SPEC, P, 1, c->stream);
+    HIP_OK(hipGetLastError());
+    return {P.spec, P.a, P.b};
+}
 
 // Bath correlation C(t_n), n <= N, into device memory dC (N + 1 values): the
 // sampled KMS spectrum, its forward FFT by the three-pass kernels (the
@@ -3933,41 +3993,9 @@ static void bcf_device(taco_ctx* c, std::size_t N, double dt, int kind, double p
     if (!zero_t) hipLaunchKernelGGL(k_bcf_kms, dim3(1), dim3(64), 0, c->stream, B.r, half - 1, kms);
     hipLaunchKernelGGL(k_bcf_spectrum, dim3(static_cast<unsigned>((Nfft + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                        c->stream, B);
-    if (L <= 2) {  // Nfft 2 or 4 (N <= 1): the reference's loop on one thread
-        std::vector<double> tab;
-        make_twiddles(L, tab);
-        HIP_OK(hipMemcpyAsync(d_tw, tab.data(), tab.size() * 8, hipMemcpyHostToDevice, c->stream));
-        hipLaunchKernelGGL(k_fft_tiny, dim3(1), dim3(64), 0, c->stream, B.S, L, d_tw);
-        hipLaunchKernelGGL(k_bcf_gather, dim3(1), dim3(kBlock), 0, c->stream, B.S, L, 0, static_cast<long>(Nt),
-                           domega / kPiHost, dC);
-        HIP_OK(hipStreamSynchronize(c->stream));  // the host table goes out of scope
-        HIP_OK(hipGetLastError());
-        return;
-    }
-    const Twiddles& tw = twiddles_for(c, L);
-    FcrArgs P{};
-    P.gamma = B.S;
-    P.Nt = static_cast<long>(Nfft);
-    P.Nin = static_cast<long>(Nfft);
-    P.L = L;
-    P.a = (L + 1) / 2;
-    P.b = L - P.a;
-    P.n = static_cast<long>(Nfft);
-    P.twStd = tw.std_.as<double2>();
-    P.twB = tw.b_.as<double2>();
-    P.twC = tw.c_.as<double2>();
-    c->inter1.ensure(Nfft * 16);
-    c->spec.ensure(Nfft * 16);
-    P.inter1 = c->inter1.as<double2>();
-    P.spec = c->spec.as<double2>();
-    P.njobs = 1;
-    auto it = c->fft_choice.find(L);
-    const int cfgA = it != c->fft_choice.end() && it->second[0] >= 0 ? it->second[0] : 2;
-    const int cfgB = it != c->fft_choice.end() && it->second[1] >= 0 ? it->second[1] : 2;
-    launch_phase(cfgA, PH_A_SPEC, P, 1, c->stream);
-    launch_phase(cfgB, PH_B_SPEC, P, 1, c->stream);
+    const SpecOut X = fft_forward_dev(c, B.S, L, d_tw);
     hipLaunchKernelGGL(k_bcf_gather, dim3(static_cast<unsigned>((Nt + kBlock - 1) / kBlock)), dim3(kBlock), 0,
-                       c->stream, P.spec, P.a, P.b, static_cast<long>(Nt), domega / kPiHost, dC);
+                       c->stream, X.X, X.a, X.b, static_cast<long>(Nt), domega / kPiHost, dC);
     HIP_OK(hipGetLastError());
 }
 
@@ -3994,15 +4022,55 @@ static void trapz_device(taco_ctx* c, const double2* dC, std::size_t Nc, double 
 
 namespace taco_engine {
 
+// A bath whose correlation and trapz Gamma the device makes for a tape.
+struct BathSrc {
+    int kind;
+    double p0, p1, beta;
+};
+
+// Device layout of a bath-sourced tape's `bath` buffer, in complex slots:
+// C [Nt], phase steps [nf], Gamma_bar [nf][T], Q [nf][T], the reverse FFT's
+// input [nfft] and twiddles [4], omega_bar [nf], workgroup sums [3 nblk] and
+// five results (doubles, two per slot).
+struct BathLayout {
+    std::size_t nfft = 0, nblk = 0;
+    int L = 0;
+    std::size_t steps = 0, gbar = 0, q = 0, x = 0, tw = 0, wbar = 0, part = 0, out = 0, total = 0;
+};
+static BathLayout bath_layout(std::size_t Nt, int nf, std::size_t T) {
+    BathLayout B;
+    B.nfft = 2 * Nt;
+    if ((B.nfft & (B.nfft - 1)) != 0) B.nfft = next_pow2(B.nfft);
+    while ((std::size_t{1} << B.L) < B.nfft) ++B.L;
+    B.nblk = std::max<std::size_t>(1, (B.nfft / 2 - 1 + kBlock - 1) / kBlock);
+    const std::size_t nfz = static_cast<std::size_t>(nf);
+    B.steps = Nt;
+    B.gbar = B.steps + nfz;
+    B.q = B.gbar + nfz * T;
+    B.x = B.q + nfz * T;
+    B.tw = B.x + B.nfft;
+    B.wbar = B.tw + 4;
+    B.part = B.wbar + (nfz + 1) / 2;
+    B.out = B.part + (3 * B.nblk + 1) / 2;
+    B.total = B.out + 3;
+    return B;
+}
+
 // Validates one tape's inputs and uploads them (the tape keeps host copies of
-// what its reverse pass reads on the host).
-static std::unique_ptr<taco_tape> tape_make(taco_ctx* ctx, const taco_tape_set& in, std::size_t checkpoint_every) {
+// what its reverse pass reads on the host).  With a bath, the Gamma table is
+// made on the device instead (bcf_fft_fun and the trapz Gamma, bit-identical
+// to taco_bath_gamma_hip) and in.gamma is not read.
+static std::unique_ptr<taco_tape> tape_make(taco_ctx* ctx, const taco_tape_set& in, std::size_t checkpoint_every,
+                                            const BathSrc* bath = nullptr) {
     const std::size_t Nt = in.Nt, steps = in.steps;
     const int nf = in.nf, N = in.N, num_ops = in.num_ops;
     if (!(in.order == 0 || in.order == 2 || in.order == 4)) throw Error(TACO_ERR_INVALID, "order must be 0, 2, or 4");
     validate_system(Nt, nf, in.mirror, in.pair_to_freq, N, num_ops, in.dt, TACO_FCR_CONVOLUTION);
-    if (!in.gamma || !in.omegas || !in.mirror || !in.pair_to_freq || !in.ops || !in.eps || !in.r0)
+    if ((!bath && !in.gamma) || !in.omegas || !in.mirror || !in.pair_to_freq || !in.ops || !in.eps || !in.r0)
         throw Error(TACO_ERR_INVALID, "tape inputs must be non-null");
+    if (bath && bath->kind != TACO_J_OHMIC && bath->kind != TACO_J_DRUDE_LORENTZ)
+        throw Error(TACO_ERR_INVALID, "unknown spectral density kind");
+    if (bath && !(bath->beta > 0.0)) throw Error(TACO_ERR_INVALID, "beta must be > 0");
     if (steps < 1) throw Error(TACO_ERR_INVALID, "steps must be >= 1");
     if (steps + 1 > Nt) throw Error(TACO_ERR_RANGE, "steps exceed gamma_series rows");
     if (checkpoint_every < 1) throw Error(TACO_ERR_INVALID, "checkpoint_every must be >= 1");
@@ -4022,11 +4090,6 @@ static std::unique_ptr<taco_tape> tape_make(taco_ctx* ctx, const taco_tape_set& 
     tp->K = static_cast<int>(std::min<std::size_t>(checkpoint_every, steps));
     tp->dt = in.dt;
     const std::size_t N2 = static_cast<std::size_t>(N) * N;
-    const auto* gc = reinterpret_cast<const std::complex<double>*>(in.gamma);
-    tp->h_gamma.resize(static_cast<std::size_t>(nf) * tp->T);
-    for (int b = 0; b < nf; ++b)
-        std::copy(gc + static_cast<std::size_t>(b) * Nt, gc + static_cast<std::size_t>(b) * Nt + tp->T,
-                  tp->h_gamma.begin() + static_cast<std::size_t>(b) * tp->T);
     tp->h_eps.assign(in.eps, in.eps + N);
     tp->h_omegas.assign(in.omegas, in.omegas + nf);
     tp->h_p2f.assign(in.pair_to_freq, in.pair_to_freq + N2);
@@ -4037,13 +4100,114 @@ static std::unique_ptr<taco_tape> tape_make(taco_ctx* ctx, const taco_tape_set& 
         buf.ensure(std::max<std::size_t>(bytes, 16));
         HIP_OK(hipMemcpy(buf.p, host, bytes, hipMemcpyHostToDevice));
     };
-    put(tp->gamma, in.gamma, Nt * nf * 16);
+    if (!bath) put(tp->gamma, in.gamma, Nt * nf * 16);
     put(tp->omegas, in.omegas, nf * sizeof(double));
     put(tp->mirror, in.mirror, nf * sizeof(int));
     put(tp->p2f, in.pair_to_freq, N2 * sizeof(int));
     put(tp->ops, in.ops, N2 * num_ops * 16);
     put(tp->eps, in.eps, N * sizeof(double));
+    if (bath) {
+        tp->bath = true;
+        tp->bath_kind = bath->kind;
+        tp->bath_p0 = bath->p0;
+        tp->bath_p1 = bath->p1;
+        tp->bath_beta = bath->beta;
+        const BathLayout Ly = bath_layout(Nt, nf, tp->T);
+        tp->bufs.bath.ensure(Ly.total * 16);
+        tp->gamma.ensure(Nt * nf * 16);
+        double2* dC = tp->bufs.bath.as<double2>();
+        bcf_device(ctx, Nt - 1, in.dt, bath->kind, bath->p0, bath->p1, bath->beta, dC);
+        trapz_device(ctx, dC, Nt, in.dt, in.omegas, tp->omegas.as<double>(), nf, dC + Ly.steps,
+                     tp->gamma.as<double2>());
+    }
     return tp;
+}
+
+// Reverse of a bath-sourced tape's bath chain: Gamma_bar = gb4 + g2 ([nf][T]
+// on the device, either null for a zero part) -> dLoss/d(p0, p1, beta) into
+// bath_bar[3], and the trapz Gamma's part of omega_bar added to omb.
+static void tape_bath_adjoint(taco_tape& tp, const double2* gb4, const double2* g2, double* bath_bar,
+                              std::vector<double>& omb) {
+    taco_ctx* c = tp.ctx;
+    if (!std::isfinite(tp.bath_beta))
+        throw Error(TACO_ERR_INVALID, "the bath cotangents need a finite beta");
+    const std::size_t T = tp.T, nfz = static_cast<std::size_t>(tp.nf);
+    const BathLayout Ly = bath_layout(tp.Nt, tp.nf, T);
+    double2* base = tp.bufs.bath.as<double2>();
+    const double2* dC = base;
+    double2* gbar = base + Ly.gbar;
+    double2* Q = base + Ly.q;
+    double2* x = base + Ly.x;
+    double* wbar = reinterpret_cast<double*>(base + Ly.wbar);
+    double* part = reinterpret_cast<double*>(base + Ly.part);
+    double* out = reinterpret_cast<double*>(base + Ly.out);
+    const long ng = static_cast<long>(nfz * T);
+    hipLaunchKernelGGL(k_gbar_sum, dim3(static_cast<unsigned>((ng + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       c->stream, gb4, g2, ng, gbar);
+    hipLaunchKernelGGL(k_trapz_adj_q, dim3(static_cast<unsigned>(nfz)), dim3(kBlock), 0, c->stream, gbar,
+                       static_cast<long>(T), dC, tp.dt, tp.omegas.as<double>(), Q, wbar);
+    hipLaunchKernelGGL(k_trapz_adj_c, dim3(static_cast<unsigned>((Ly.nfft + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       c->stream, Q, static_cast<long>(T), tp.nf, tp.dt, tp.omegas.as<double>(),
+                       static_cast<long>(Ly.nfft), x);
+    HIP_OK(hipGetLastError());
+    const SpecOut X = fft_forward_dev(c, x, Ly.L, base + Ly.tw);
+    const double domega = 2.0 * kPiHost / (tp.dt * static_cast<double>(Ly.nfft));
+    BcfAdjArgs A{};
+    A.spec = X.X;
+    A.a = X.a;
+    A.b = X.b;
+    A.nfft = static_cast<long>(Ly.nfft);
+    A.scale = domega / kPiHost;
+    A.domega = domega;
+    A.beta = tp.bath_beta;
+    A.p0 = tp.bath_p0;
+    A.p1 = tp.bath_p1;
+    A.kind = tp.bath_kind;
+    A.part = part;
+    hipLaunchKernelGGL(k_bcf_adj_terms, dim3(static_cast<unsigned>(Ly.nblk)), dim3(kBlock), 0, c->stream, A);
+    hipLaunchKernelGGL(k_bcf_adj_final, dim3(1), dim3(64), 0, c->stream, A, static_cast<int>(Ly.nblk), out);
+    HIP_OK(hipGetLastError());
+    double r[5];
+    std::vector<double> wb(nfz);
+    HIP_OK(hipMemcpyAsync(r, out, sizeof r, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipMemcpyAsync(wb.data(), wbar, nfz * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    for (std::size_t b = 0; b < nfz; ++b) omb[b] += wb[b];
+    // S_0 = (pi/4) J(dw) (1 + e) / (1 - e), e = exp(-beta dw); S_{nfft/2} =
+    // (pi/4) J(w_Ny) (1 + 2 xN / (1 - xN)), xN = exp(-beta w_Ny)
+    const double beta = tp.bath_beta, p0 = tp.bath_p0, p1 = tp.bath_p1;
+    auto spec = [&](double w, double& J, double& d0, double& d1) {
+        if (tp.bath_kind == TACO_J_OHMIC) {
+            const double e = std::exp(-w / p1);
+            J = p0 * w * e;
+            d0 = w * e;
+            d1 = J * w / (p1 * p1);
+        } else {
+            const double den = w * w + p1 * p1;
+            J = 2.0 * p0 * p1 * w / den;
+            d0 = 2.0 * p1 * w / den;
+            d1 = 2.0 * p0 * w * (w * w - p1 * p1) / (den * den);
+        }
+    };
+    double g0 = r[0], g1 = r[1], gb = r[2];
+    double J0, a0, a1;
+    spec(domega, J0, a0, a1);
+    const double e = std::exp(-beta * domega);
+    const double q0 = 0.5 * (kPiHost / 2.0) * (1.0 + e) / (1.0 - e);
+    g0 += r[3] * q0 * a0;
+    g1 += r[3] * q0 * a1;
+    gb += r[3] * 0.5 * (kPiHost / 2.0) * J0 * (-2.0 * domega * e / ((1.0 - e) * (1.0 - e)));
+    const double wny = kPiHost / tp.dt;
+    double Jn, n0, n1;
+    spec(wny, Jn, n0, n1);
+    const double xN = std::exp(-beta * wny);
+    const double qn = 0.25 * kPiHost * (1.0 + 2.0 * xN / (1.0 - xN));
+    g0 += r[4] * qn * n0;
+    g1 += r[4] * qn * n1;
+    gb += r[4] * 0.25 * kPiHost * Jn * (-2.0 * wny * xN / ((1.0 - xN) * (1.0 - xN)));
+    bath_bar[0] = g0;
+    bath_bar[1] = g1;
+    bath_bar[2] = gb;
 }
 
 }  // namespace taco_engine
@@ -5307,6 +5471,33 @@ int taco_tape_adjoint(taco_tape* tape, const double* samples_bar, double* gamma_
         if (!samples_bar) throw Error(TACO_ERR_INVALID, "null samples_bar");
         DeviceGuard g(tape->ctx->device);
         tape_adjoint(*tape, samples_bar, gamma_bar, omega_bar, ops_bar, eps_bar, r0_bar);
+    });
+}
+
+int taco_tape_forward_bath(taco_ctx* ctx, int kind, double p0, double p1, double beta, size_t Nt, int nf,
+                           const double* omegas, const int* mirror, const int* pair_to_freq, int N, const double* eps,
+                           const double* ops, int num_ops, double dt, size_t steps, int order, const double* r0,
+                           size_t checkpoint_every, double* samples_out, taco_tape** out) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!out) throw Error(TACO_ERR_INVALID, "null output handle");
+        *out = nullptr;
+        DeviceGuard g(ctx->device);
+        const taco_tape_set in{nullptr, Nt, nf, omegas, mirror, pair_to_freq, N, eps, ops, num_ops, dt, steps, order, r0};
+        const BathSrc bath{kind, p0, p1, beta};
+        auto tp = tape_make(ctx, in, checkpoint_every, &bath);
+        tape_forward(*tp, r0, samples_out);
+        *out = tp.release();
+    });
+}
+
+int taco_tape_adjoint_bath(taco_tape* tape, const double* samples_bar, double* bath_bar, double* omega_bar,
+                           double* ops_bar, double* eps_bar, double* r0_bar) {
+    return guarded([&] {
+        if (!tape) throw Error(TACO_ERR_INVALID, "null taco_tape");
+        if (!samples_bar || !bath_bar) throw Error(TACO_ERR_INVALID, "null samples_bar or bath_bar");
+        DeviceGuard g(tape->ctx->device);
+        tape_adjoint(*tape, samples_bar, nullptr, omega_bar, ops_bar, eps_bar, r0_bar, bath_bar);
     });
 }
 

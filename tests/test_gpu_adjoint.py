@@ -18,6 +18,7 @@ import pytest
 import torch
 
 import taco_amd
+from taco_amd import _lib
 from taco_amd import adjoint as AD
 from taco_amd import workloads as W
 
@@ -418,3 +419,66 @@ def test_deferred_reverse_walk_equals_in_place_walk(monkeypatch, kind):
             assert np.array_equal(out["0m"][q][key], out["1"][key]), key
     for x, y in zip(out["0"]["ops"], out["1"]["ops"]):
         assert np.array_equal(x, y)
+
+
+def _chain_grad(prob, p, target):
+    params = torch.tensor(p, dtype=torch.float64, requires_grad=True)
+    rho = prob.rho_lab(params)
+    d = rho - target
+    loss = (d.real ** 2 + d.imag ** 2).sum()
+    (g,) = torch.autograd.grad(loss, params)
+    return rho.detach().numpy(), float(loss), g.numpy()
+
+
+@pytest.mark.parametrize("kind", ["spin_pair_ohmic", "spin_pair_order2", "tfim_drude", "tiny_grid"])
+def test_device_bath_chain_equals_split_chain(kind):
+    """The fused device chain (BathTape: C(t) and Gamma made, kept and
+    reversed in device memory) against the split chain (BcfFun -> GammaFun ->
+    TCL4Propagate, whose chain reverses run on the host in numpy): the same
+    rho(t) bit for bit (the device producers are bit-identical), and the same
+    parameter gradient up to the rounding of the two reverse implementations
+    (different FFTs and summation orders: bar 1e-10 of the largest entry)."""
+    if kind == "tfim_drude":
+        mk = lambda chain: AD.TfimChainProblem(Nt=256, dt=0.02, steps=12, checkpoint_every=4, bath_chain=chain)
+        p0 = np.array([0.1, 5.0, 1.0, 1.0, 0.7])
+    elif kind == "tiny_grid":  # Nt = 2: a length-4 FFT on one thread
+        mk = lambda chain: AD.SpinPairProblem(Nt=2, dt=0.01, steps=1, checkpoint_every=1, bath_chain=chain)
+        p0 = np.array([0.05, 5.0, 1.0, 1.0, 1.0])
+    else:
+        order = 2 if kind == "spin_pair_order2" else 4
+        mk = lambda chain: AD.SpinPairProblem(Nt=2 ** 12, steps=400, checkpoint_every=64, order=order,
+                                              bath_chain=chain)
+        p0 = np.array([0.05, 5.0, 1.0, 1.0, 1.0])
+    dev, split = mk("device"), mk("split")
+    u = np.where(np.random.default_rng(4).uniform(size=5) < 0.5, -1.0, 1.0)
+    with torch.no_grad():
+        target = split.rho_lab(torch.tensor(p0 * (1 + 0.05 * u), dtype=torch.float64)).detach()
+    rho_d, loss_d, g_d = _chain_grad(dev, p0, target)
+    rho_s, loss_s, g_s = _chain_grad(split, p0, target)
+    assert np.array_equal(rho_d, rho_s)
+    assert loss_d == loss_s
+    scale = np.abs(g_s).max()
+    for q, name in enumerate(dev.names):
+        print(f"{kind} d loss / d {name}: device {g_d[q]:.15e} split {g_s[q]:.15e} "
+              f"diff/max {abs(g_d[q] - g_s[q]) / scale:.2e}")
+    assert np.abs(g_d - g_s).max() <= 1e-10 * scale
+    # the device reverse repeats bit for bit
+    _, _, g_d2 = _chain_grad(dev, p0, target)
+    assert np.array_equal(g_d, g_d2)
+
+
+def test_bath_tape_rejects_infinite_beta_cotangents():
+    sp = W.spin_pair(Nt=256, steps=20)
+    S = sp.system
+    tp = AD.BathTape(0, 0.05, 5.0, float("inf"), 256, S.omegas, S.mirror, S.pair_to_freq, S.A_eig, S.eps, sp.dt, 20,
+                     r0_for(4), 4, 8)
+    # the forward runs at zero temperature (the producers support it) ...
+    _, C = taco_amd.bcf_fft_fun(255, sp.dt, taco_amd.Ohmic(0.05, 5.0), float("inf"))
+    G = taco_amd.compute_trapz_prefix_multi_matrix(C, sp.dt, S.omegas)
+    ref = AD.Tape(G, S.omegas, S.mirror, S.pair_to_freq, S.A_eig, S.eps, sp.dt, 20, r0_for(4), 4, 8)
+    assert np.array_equal(tp.samples, ref.samples)
+    ref.close()
+    # ... and its bath cotangents are refused
+    with pytest.raises(_lib.TacoInvalidArgument, match="finite beta"):
+        tp.adjoint(np.ones((21, 16), complex))
+    tp.close()
