@@ -451,6 +451,21 @@ int taco_mtape_forward(taco_ctx* ctx, const taco_tape_set* sets, int nsets, size
 int taco_mtape_adjoint(taco_mtape* mt, const double* const* samples_bar, double* const* gamma_bar,
                        double* const* omega_bar, double* const* ops_bar, double* const* eps_bar,
                        double* const* r0_bar);
+/* Multi-set tape whose sets take their Gamma tables from baths on the device
+ * (as taco_tape_forward_bath; sets[s].gamma is not read, sets[s].Nt is the
+ * grid): bath_bar[s][3] = dLoss/d(p0, p1, beta) of set s; every set's
+ * samples and cotangents equal those of taco_tape_forward_bath /
+ * taco_tape_adjoint_bath on that set alone, bit for bit. */
+typedef struct taco_bath_params {
+    int kind;          /* TACO_J_* */
+    double p0, p1;     /* spectral density parameters */
+    double beta;       /* inverse temperature, > 0 */
+} taco_bath_params;
+int taco_mtape_forward_bath(taco_ctx* ctx, const taco_tape_set* sets, const taco_bath_params* baths, int nsets,
+                            size_t checkpoint_every, double* const* samples_out, taco_mtape** out);
+int taco_mtape_adjoint_bath(taco_mtape* mt, const double* const* samples_bar, double* const* bath_bar,
+                            double* const* omega_bar, double* const* ops_bar, double* const* eps_bar,
+                            double* const* r0_bar);
 int taco_mtape_keeps_slices(const taco_mtape* mt);
 int taco_mtape_destroy(taco_mtape* mt);
 /* One-shot taco_tape_forward + taco_tape_adjoint + taco_tape_destroy. */

@@ -42,6 +42,11 @@ class TapeSet(ctypes.Structure):
     ]
 
 
+class BathParams(ctypes.Structure):
+    """taco_bath_params of include/taco_tcl4.h."""
+    _fields_ = [("kind", c_int), ("p0", c_double), ("p1", c_double), ("beta", c_double)]
+
+
 _dpp = POINTER(_dp)
 
 # name -> (restype, argtypes); every symbol declared in include/taco_tcl4.h.
@@ -114,6 +119,9 @@ SIGNATURES = {
     "taco_tape_adjoint_bath": (c_int, [c_void_p, _dp, _dp, _dp, _dp, _dp, _dp]),
     "taco_mtape_forward": (c_int, [c_void_p, POINTER(TapeSet), c_int, c_size_t, _dpp, POINTER(c_void_p)]),
     "taco_mtape_adjoint": (c_int, [c_void_p, _dpp, _dpp, _dpp, _dpp, _dpp, _dpp]),
+    "taco_mtape_forward_bath": (c_int, [c_void_p, POINTER(TapeSet), POINTER(BathParams), c_int, c_size_t, _dpp,
+                                        POINTER(c_void_p)]),
+    "taco_mtape_adjoint_bath": (c_int, [c_void_p, _dpp, _dpp, _dpp, _dpp, _dpp, _dpp]),
     "taco_mtape_keeps_slices": (c_int, [c_void_p]),
     "taco_mtape_destroy": (c_int, [c_void_p]),
     "taco_adjoint": (c_int, [c_void_p, _dp, c_size_t, c_int, _dp, _ip, _ip, c_int, _dp, _dp, c_int, c_double,

@@ -42,7 +42,8 @@ from ._lib import TacoInvalidArgument, check, dptr, iptr, lib
 from .tcl4 import (Exec, Ohmic, DrudeLorentz, System, _ctx_for, bath_gamma_hip, bcf_fft_fun,
                    compute_trapz_prefix_multi_matrix, compute_trapz_prefix_multi_matrix_hip)
 
-__all__ = ["Tape", "BathTape", "MultiTape", "TCL4Propagate", "BathTCL4Propagate", "TCL4PropagateMany", "BcfFun",
+__all__ = ["Tape", "BathTape", "MultiTape", "TCL4Propagate", "BathTCL4Propagate", "TCL4PropagateMany",
+           "BathTCL4PropagateMany", "BcfFun",
            "GammaFun", "EighFun", "BucketOmegas", "SpinPairProblem", "TfimChainProblem"]
 
 
@@ -165,22 +166,31 @@ class MultiTape:
     cotangents equal a Tape of that set alone, bit for bit, while the RK4
     recursions of all sets share one launch each way.  `sets` holds dicts
     with the Tape arguments (gamma, omegas, mirror, pair_to_freq, ops, eps,
-    dt, steps, r0, optional order)."""
+    dt, steps, r0, optional order).  With `baths` (one (kind, p0, p1, beta)
+    per set) the sets' Gamma tables come from their baths on the device, as
+    in BathTape: each dict gives the grid length "Nt" instead of "gamma", and
+    `adjoint` returns each set's "bath" cotangents instead of "gamma"."""
 
-    def __init__(self, sets, checkpoint_every: int = 64, exec: Optional[Exec] = None):
-        from ._lib import TapeSet
+    def __init__(self, sets, checkpoint_every: int = 64, exec: Optional[Exec] = None, baths=None):
+        from ._lib import BathParams, TapeSet
         self.ctx = _ctx_for(exec)
         self.meta, self._keep = [], []
+        self.bath = baths is not None
         arr = (TapeSet * len(sets))()
         for k, st in enumerate(sets):
-            g = np.asarray(st["gamma"], np.complex128)
-            Nt, nf = g.shape
+            if self.bath:
+                g = None
+                Nt, nf = int(st["Nt"]), len(st["omegas"])
+            else:
+                g = np.asarray(st["gamma"], np.complex128)
+                Nt, nf = g.shape
             p2f = np.asarray(st["pair_to_freq"], np.int32)
             N = p2f.shape[0]
             ops = st["ops"]
             steps = int(st["steps"])
             buf = dict(
-                gc=np.ascontiguousarray(g.T).ravel(), om=np.ascontiguousarray(st["omegas"], np.float64),
+                gc=None if g is None else np.ascontiguousarray(g.T).ravel(),
+                om=np.ascontiguousarray(st["omegas"], np.float64),
                 mi=np.ascontiguousarray(st["mirror"], np.int32), pf=np.ascontiguousarray(p2f.T).ravel(),
                 of=np.concatenate([_cm(o) for o in ops]), ev=np.ascontiguousarray(st["eps"], np.float64),
                 r=np.ascontiguousarray(np.asarray(st["r0"], np.complex128).ravel()))
@@ -188,13 +198,20 @@ class MultiTape:
                 raise TacoInvalidArgument(1, "r0 must have N^2 entries")
             self._keep.append(buf)
             self.meta.append(dict(Nt=Nt, nf=nf, N=N, num_ops=len(ops), steps=steps))
-            arr[k] = TapeSet(dptr(buf["gc"]), Nt, nf, dptr(buf["om"]), iptr(buf["mi"]), iptr(buf["pf"]), N,
-                             dptr(buf["ev"]), dptr(buf["of"]), len(ops), float(st["dt"]), steps,
+            arr[k] = TapeSet(None if g is None else dptr(buf["gc"]), Nt, nf, dptr(buf["om"]), iptr(buf["mi"]),
+                             iptr(buf["pf"]), N, dptr(buf["ev"]), dptr(buf["of"]), len(ops), float(st["dt"]), steps,
                              int(st.get("order", 4)), dptr(buf["r"]))
         self.samples = [np.zeros((m["steps"] + 1, m["N"] ** 2), np.complex128) for m in self.meta]
         h = c_void_p()
-        check(lib().taco_mtape_forward(self.ctx.handle, arr, len(sets), int(checkpoint_every),
-                                       _ptrs(self.samples), ctypes.byref(h)))
+        if self.bath:
+            bp = (BathParams * len(sets))()
+            for k, (kind, p0, p1, beta) in enumerate(baths):
+                bp[k] = BathParams(int(kind), float(p0), float(p1), float(beta))
+            check(lib().taco_mtape_forward_bath(self.ctx.handle, arr, bp, len(sets), int(checkpoint_every),
+                                                _ptrs(self.samples), ctypes.byref(h)))
+        else:
+            check(lib().taco_mtape_forward(self.ctx.handle, arr, len(sets), int(checkpoint_every),
+                                           _ptrs(self.samples), ctypes.byref(h)))
         self.handle = h
         self._keep = None  # inputs are only read during the forward call
 
@@ -205,18 +222,24 @@ class MultiTape:
     def adjoint(self, samples_bar) -> list:
         sb = [np.ascontiguousarray(np.asarray(b, np.complex128).reshape(m["steps"] + 1, m["N"] ** 2))
               for b, m in zip(samples_bar, self.meta)]
-        gb = [np.zeros(m["Nt"] * m["nf"], np.complex128) for m in self.meta]
         ob = [np.zeros(m["nf"]) for m in self.meta]
         ab = [np.zeros(m["num_ops"] * m["N"] ** 2, np.complex128) for m in self.meta]
         eb = [np.zeros(m["N"]) for m in self.meta]
         rb = [np.zeros(m["N"] ** 2, np.complex128) for m in self.meta]
-        check(lib().taco_mtape_adjoint(self.handle, _ptrs(sb), _ptrs(gb), _ptrs(ob), _ptrs(ab), _ptrs(eb),
-                                       _ptrs(rb)))
+        if self.bath:
+            bb = [np.zeros(3) for _ in self.meta]
+            check(lib().taco_mtape_adjoint_bath(self.handle, _ptrs(sb), _ptrs(bb), _ptrs(ob), _ptrs(ab), _ptrs(eb),
+                                                _ptrs(rb)))
+        else:
+            gb = [np.zeros(m["Nt"] * m["nf"], np.complex128) for m in self.meta]
+            check(lib().taco_mtape_adjoint(self.handle, _ptrs(sb), _ptrs(gb), _ptrs(ob), _ptrs(ab), _ptrs(eb),
+                                           _ptrs(rb)))
         out = []
         for k, m in enumerate(self.meta):
             N = m["N"]
+            first = {"bath": bb[k]} if self.bath else {"gamma": gb[k].reshape(m["nf"], m["Nt"]).T.copy()}
             out.append({
-                "gamma": gb[k].reshape(m["nf"], m["Nt"]).T.copy(),
+                **first,
                 "omegas": ob[k],
                 "ops": [ab[k][q * N * N:(q + 1) * N * N].reshape(N, N).T.copy() for q in range(m["num_ops"])],
                 "eps": eb[k],
@@ -319,6 +342,42 @@ class TCL4PropagateMany(torch.autograd.Function):
         out = [None]
         for res in ctx.mt.adjoint(bars):
             out += [torch.from_numpy(res["gamma"]), torch.from_numpy(res["omegas"]),
+                    torch.from_numpy(np.stack(res["ops"])), torch.from_numpy(res["eps"]),
+                    torch.from_numpy(res["r0"])]
+        return tuple(out)
+
+
+class BathTCL4PropagateMany(torch.autograd.Function):
+    """BathTCL4Propagate of several parameter sets of one model size at once
+    (MultiTape with baths).  Inputs after ``meta`` are the sets' (p0, p1,
+    beta, omegas, ops, eps, r0) in turn; ``meta`` = (per-set (kind, Nt,
+    mirror, pair_to_freq, dt, order) tuples, steps, checkpoint_every, exec).
+    Every set's values and cotangents equal BathTCL4Propagate on that set
+    alone."""
+
+    @staticmethod
+    def forward(ctx, meta, *tensors):
+        per, steps, K, ex = meta
+        sets, baths = [], []
+        for q, (kind, Nt, mirror, p2f, dt, order) in enumerate(per):
+            p0, p1, beta, om, ops, e, r0 = tensors[7 * q:7 * q + 7]
+            sets.append(dict(Nt=Nt, omegas=om.detach().numpy(), mirror=mirror, pair_to_freq=p2f,
+                             ops=list(ops.detach().numpy()), eps=e.detach().numpy(), dt=dt, steps=steps,
+                             r0=r0.detach().numpy(), order=order))
+            baths.append((kind, float(p0), float(p1), float(beta)))
+        mt = MultiTape(sets, K, ex, baths=baths)
+        ctx.mt = mt
+        return tuple(torch.from_numpy(x.copy()) for x in mt.samples)
+
+    @staticmethod
+    def backward(ctx, *gs):
+        bars = [np.zeros(m.shape, np.complex128) if g is None else g.detach().numpy()
+                for g, m in zip(gs, ctx.mt.samples)]
+        t = lambda v: torch.tensor(float(v), dtype=torch.float64)
+        out = [None]
+        for res in ctx.mt.adjoint(bars):
+            pb = res["bath"]
+            out += [t(pb[0]), t(pb[1]), t(pb[2]), torch.from_numpy(res["omegas"]),
                     torch.from_numpy(np.stack(res["ops"])), torch.from_numpy(res["eps"]),
                     torch.from_numpy(res["r0"])]
         return tuple(out)
@@ -592,21 +651,27 @@ def _rho_lab_many(prob, Hs, A: np.ndarray, kind: int, baths, exec: Optional[Exec
     baths[q] = (p0, p1, beta)) of one model size, with the TCL4 runs of all
     sets in one multi-set tape."""
     producer = exec or Exec()
+    device = prob.bath_chain == "device"
     per, flat, Us, Ns = [], [], [], []
     for H, (p0, p1, beta) in zip(Hs, baths):
         e, U = EighFun.apply(H, [A], prob.freq_tol)
         S = System.build(H.detach().numpy(), [A], prob.freq_tol)
         om = BucketOmegas.apply(e, S.omegas, S.pair_to_freq)
-        C = BcfFun.apply(p0, p1, beta, (kind, prob.Nt - 1, prob.dt, producer))
-        G = GammaFun.apply(C, om, prob.dt, producer)
         Aeig = (U.conj().T @ torch.from_numpy(A) @ U).unsqueeze(0)
         r0 = torch.zeros(S.N * S.N, dtype=torch.complex128)
         r0[0] = 1.0
-        per.append((S.mirror, S.pair_to_freq, prob.dt, prob.order))
-        flat += [G, om, Aeig, e, r0]
+        if device:
+            per.append((kind, prob.Nt, S.mirror, S.pair_to_freq, prob.dt, prob.order))
+            flat += [p0, p1, beta, om, Aeig, e, r0]
+        else:
+            C = BcfFun.apply(p0, p1, beta, (kind, prob.Nt - 1, prob.dt, producer))
+            G = GammaFun.apply(C, om, prob.dt, producer)
+            per.append((S.mirror, S.pair_to_freq, prob.dt, prob.order))
+            flat += [G, om, Aeig, e, r0]
         Us.append(U)
         Ns.append(S.N)
-    rs = TCL4PropagateMany.apply((per, prob.steps, prob.checkpoint_every, exec), *flat)
+    meta = (per, prob.steps, prob.checkpoint_every, exec)
+    rs = BathTCL4PropagateMany.apply(meta, *flat) if device else TCL4PropagateMany.apply(meta, *flat)
     out = []
     for r, U, N in zip(rs, Us, Ns):
         rho = r.reshape(-1, N, N).transpose(1, 2)

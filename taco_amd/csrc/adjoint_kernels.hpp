@@ -645,7 +645,7 @@ struct L4fAdjArgs {
 // LDS_TB: the N^4 Tbar tile sits in LDS (else in the workgroup's scratch,
 // for N above 8 where it exceeds the 160 KiB of a CU).  Both are fixed per
 // instantiation so every access has a known address space.
-template <bool LDS_SLICES, bool LDS_TB>
+template <bool LDS_SLICES, bool LDS_TB, int BS>
 __device__ __forceinline__ void l4f_adjoint_body(double2* sm, const L4fAdjArgs& A) {
     const int N = A.N, N2 = N * N, N4 = N2 * N2, nf = A.nf;
     const long N2nf = static_cast<long>(N2) * nf;
@@ -655,8 +655,8 @@ __device__ __forceinline__ void l4f_adjoint_body(double2* sm, const L4fAdjArgs& 
     double2* Ab = opsS + A.num_ops * N2;                // num_ops N^2
     double2* sb = Ab + A.num_ops * N2;                  // 3 nf3 (LDS_SLICES)
     int* f = reinterpret_cast<int*>(sb + (LDS_SLICES ? 3 * A.nf3 : 0));
-    for (int e = threadIdx.x; e < A.num_ops * N2; e += kBlock) opsS[e] = A.ops[e];
-    for (int e = threadIdx.x; e < N2; e += kBlock) f[e] = A.p2f[e];
+    for (int e = threadIdx.x; e < A.num_ops * N2; e += BS) opsS[e] = A.ops[e];
+    for (int e = threadIdx.x; e < N2; e += BS) f[e] = A.p2f[e];
     double2* Q = scr0;                                  // Q(a,i,g) at a + i N + g N^2
     double2* S67 = Q + N2nf;
     double2* S67b = S67 + N2nf;
@@ -671,14 +671,14 @@ __device__ __forceinline__ void l4f_adjoint_body(double2* sm, const L4fAdjArgs& 
     for (long t = blockIdx.x; t < A.T; t += gridDim.x) {
         __syncthreads();
         const double2* Lb = A.Lbar + t * N4;
-        for (int idx = threadIdx.x; idx < N4; idx += kBlock) {
+        for (int idx = threadIdx.x; idx < N4; idx += BS) {
             const int ru = idx % N2, cu = idx / N2;
             const int n = ru % N, i = ru / N, m = cu % N, j = cu / N;
             Tb[idx] = cadd(Lb[(n + m * N) + (i + j * N) * N2], cconj(Lb[(m + n * N) + (j + i * N) * N2]));
         }
-        for (int e = threadIdx.x; e < A.num_ops * N2; e += kBlock) Ab[e] = make_double2(0.0, 0.0);
+        for (int e = threadIdx.x; e < A.num_ops * N2; e += BS) Ab[e] = make_double2(0.0, 0.0);
         if constexpr (LDS_SLICES)
-            for (long e = threadIdx.x; e < 3 * A.nf3; e += kBlock) sb[e] = make_double2(0.0, 0.0);
+            for (long e = threadIdx.x; e < 3 * A.nf3; e += BS) sb[e] = make_double2(0.0, 0.0);
         __syncthreads();
         const double2* Fp = A.Fs + t * A.nf3;
         const double2* Cp = A.Cs + t * A.nf3;
@@ -696,7 +696,7 @@ __device__ __forceinline__ void l4f_adjoint_body(double2* sm, const L4fAdjArgs& 
 #define BB(r, c) Bp[(r) + (c) * N]
 #define TB(n, i, m, j) Tb[((n) + (i) * N) + ((m) + (j) * N) * N2]
                 // 1: forward Q and S123
-                for (long e = threadIdx.x; e < N2nf; e += kBlock) {
+                for (long e = threadIdx.x; e < N2nf; e += BS) {
                     const int a = static_cast<int>(e % N), i = static_cast<int>((e / N) % N);
                     const int g = static_cast<int>(e / N2);
                     double2 y = make_double2(0.0, 0.0);
@@ -707,7 +707,7 @@ __device__ __forceinline__ void l4f_adjoint_body(double2* sm, const L4fAdjArgs& 
                     }
                     Q[e] = y;
                 }
-                for (int e = threadIdx.x; e < N2; e += kBlock) {
+                for (int e = threadIdx.x; e < N2; e += BS) {
                     const int n = e % N, i = e / N;
                     double2 s = make_double2(0.0, 0.0);
                     for (int a = 0; a < N; ++a)
@@ -724,7 +724,7 @@ __device__ __forceinline__ void l4f_adjoint_body(double2* sm, const L4fAdjArgs& 
                 }
                 __syncthreads();
                 // 2: S67; S123bar, Dbar, S67bar, U from Tbar; A-bar of the A(j,m) S123 term
-                for (long e = threadIdx.x; e < N2nf; e += kBlock) {
+                for (long e = threadIdx.x; e < N2nf; e += BS) {
                     const int n = static_cast<int>(e % N);
                     const long rest = e - n;
                     double2 s = make_double2(0.0, 0.0);
@@ -737,7 +737,7 @@ __device__ __forceinline__ void l4f_adjoint_body(double2* sm, const L4fAdjArgs& 
                             if (F2(j, m) == g) sb7 = cadd(sb7, cmul(cconj(BB(j, m)), Tb[ni + (m + j * N) * N2]));
                     S67b[e] = sb7;
                 }
-                for (int e = threadIdx.x; e < N2; e += kBlock) {
+                for (int e = threadIdx.x; e < N2; e += BS) {
                     double2 s = make_double2(0.0, 0.0), d = s;
                     for (int j = 0; j < N; ++j) {
                         for (int m = 0; m < N; ++m) s = cadd(s, cmul(cconj(AA(j, m)), Tb[e + (m + j * N) * N2]));
@@ -746,13 +746,13 @@ __device__ __forceinline__ void l4f_adjoint_body(double2* sm, const L4fAdjArgs& 
                     S123b[e] = s;
                     Db[e] = d;
                 }
-                for (int e = threadIdx.x; e < N4; e += kBlock) {
+                for (int e = threadIdx.x; e < N4; e += BS) {
                     const int r = e % N2, c = e / N2, b = c % N, j = c / N;
                     double2 s = make_double2(0.0, 0.0);
                     for (int m = 0; m < N; ++m) s = cadd(s, cmul(cconj(AA(b, m)), Tb[r + (m + j * N) * N2]));
                     U[e] = s;
                 }
-                for (int e = threadIdx.x; e < N2; e += kBlock) {
+                for (int e = threadIdx.x; e < N2; e += BS) {
                     const int j = e % N, m = e / N;
                     double2 s = make_double2(0.0, 0.0);
                     for (int r = 0; r < N2; ++r) s = cadd(s, cmul(cconj(S123[r]), Tb[r + (m + j * N) * N2]));
@@ -761,7 +761,7 @@ __device__ __forceinline__ void l4f_adjoint_body(double2* sm, const L4fAdjArgs& 
                 __syncthreads();
                 // 3: Qbar; S123bar gets the D term; operator cotangents of the S67,
                 // Q and D terms of T and of S67 = A Q and D = -A S123
-                for (long e = threadIdx.x; e < N2nf; e += kBlock) {
+                for (long e = threadIdx.x; e < N2nf; e += BS) {
                     const int a = static_cast<int>(e % N), i = static_cast<int>((e / N) % N);
                     const int g = static_cast<int>(e / N2);
                     double2 s = make_double2(0.0, 0.0);
@@ -771,13 +771,13 @@ __device__ __forceinline__ void l4f_adjoint_body(double2* sm, const L4fAdjArgs& 
                             if (F2(j, b) == g) s = csub(s, cmul(cconj(BB(j, b)), U[(a + i * N) + (b + j * N) * N2]));
                     Qb[e] = s;
                 }
-                for (int e = threadIdx.x; e < N2; e += kBlock) {
+                for (int e = threadIdx.x; e < N2; e += BS) {
                     const int a = e % N, i = e / N;
                     double2 s = S123b[e];
                     for (int n = 0; n < N; ++n) s = csub(s, cmul(cconj(AA(n, a)), Db[n + i * N]));
                     S123b[e] = s;
                 }
-                for (int e = threadIdx.x; e < N2; e += kBlock) {
+                for (int e = threadIdx.x; e < N2; e += BS) {
                     const int j = e % N, m = e / N;
                     const long go = static_cast<long>(F2(j, m)) * N2;
                     double2 s = make_double2(0.0, 0.0);
@@ -787,7 +787,7 @@ __device__ __forceinline__ void l4f_adjoint_body(double2* sm, const L4fAdjArgs& 
                 __syncthreads();
                 // v(j,b,m) = sum_r conj(Q(r, f(j,b))) Tbar(r; m,j), in U's space (U is done)
                 double2* V = U;
-                for (int e = threadIdx.x; e < N2 * N; e += kBlock) {
+                for (int e = threadIdx.x; e < N2 * N; e += BS) {
                     const int j = e % N, m = e / N2;
                     const long go = static_cast<long>(F2(j, (e / N) % N)) * N2;
                     double2 v = make_double2(0.0, 0.0);
@@ -796,21 +796,21 @@ __device__ __forceinline__ void l4f_adjoint_body(double2* sm, const L4fAdjArgs& 
                 }
                 __syncthreads();
                 // the B(j,b) A(b,m) Q term: each cotangent entry sums its v's in j (or m) order
-                for (int e = threadIdx.x; e < N2; e += kBlock) {
+                for (int e = threadIdx.x; e < N2; e += BS) {
                     const int b = e % N, m = e / N;
                     double2 s = make_double2(0.0, 0.0);
                     for (int j = 0; j < N; ++j) s = cadd(s, cmul(cconj(BB(j, b)), V[j + b * N + m * N2]));
                     AbA[b + m * N] = csub(AbA[b + m * N], s);
                 }
                 __syncthreads();
-                for (int e = threadIdx.x; e < N2; e += kBlock) {
+                for (int e = threadIdx.x; e < N2; e += BS) {
                     const int j = e % N, b = e / N;
                     double2 s = make_double2(0.0, 0.0);
                     for (int m = 0; m < N; ++m) s = cadd(s, cmul(cconj(AA(b, m)), V[j + b * N + m * N2]));
                     AbB[j + b * N] = csub(AbB[j + b * N], s);
                 }
                 __syncthreads();
-                for (int e = threadIdx.x; e < N2; e += kBlock) {
+                for (int e = threadIdx.x; e < N2; e += BS) {
                     const int n = e % N, a = e / N;
                     double2 s = make_double2(0.0, 0.0);
                     for (int i = 0; i < N; ++i) {
@@ -825,7 +825,7 @@ __device__ __forceinline__ void l4f_adjoint_body(double2* sm, const L4fAdjArgs& 
                 // one thread that sums its terms in a fixed order (no atomics)
                 auto Xv = [&](long x) { return cadd(Cp[x], Rp[x]); };
                 // X = C + R slices: pair (x, z) and bucket g gather the Q-bar terms
-                for (long e = threadIdx.x; e < static_cast<long>(A.nxp) * nf; e += kBlock) {
+                for (long e = threadIdx.x; e < static_cast<long>(A.nxp) * nf; e += BS) {
                     const int pp = static_cast<int>(e % A.nxp), g = static_cast<int>(e / A.nxp);
                     double2 acc = make_double2(0.0, 0.0);
                     for (int q = A.xp_start[pp]; q < A.xp_start[pp + 1]; ++q) {
@@ -844,7 +844,7 @@ __device__ __forceinline__ void l4f_adjoint_body(double2* sm, const L4fAdjArgs& 
                 }
                 __syncthreads();
                 // F and R slices of the M, I, K terms of S123
-                for (int e = threadIdx.x; e < A.nft; e += kBlock) {
+                for (int e = threadIdx.x; e < A.nft; e += BS) {
                     double2 acc = make_double2(0.0, 0.0);
                     for (int q = A.f_start[e]; q < A.f_start[e + 1]; ++q) {
                         const unsigned w = static_cast<unsigned>(A.f_ent[q]);
@@ -857,7 +857,7 @@ __device__ __forceinline__ void l4f_adjoint_body(double2* sm, const L4fAdjArgs& 
                     }
                     sF[A.f_tgt[e]] = cadd(sF[A.f_tgt[e]], acc);
                 }
-                for (int e = threadIdx.x; e < A.nrt; e += kBlock) {
+                for (int e = threadIdx.x; e < A.nrt; e += BS) {
                     double2 acc = make_double2(0.0, 0.0);
                     for (int q = A.r_start[e]; q < A.r_start[e + 1]; ++q) {
                         const unsigned w = static_cast<unsigned>(A.r_ent[q]);
@@ -877,7 +877,7 @@ __device__ __forceinline__ void l4f_adjoint_body(double2* sm, const L4fAdjArgs& 
                 auto Iv = [&](int n, int i, int a, int b) { return Fp[id3(F2(a, n), F2(b, i), F2(n, b))]; };
                 auto Kv = [&](int n, int i, int a, int b) { return Rp[id3(F2(i, b), F2(n, a), F2(a, i))]; };
                 // operator entries at (a,b) of the Q and S123 terms
-                for (int e = threadIdx.x; e < N2; e += kBlock) {
+                for (int e = threadIdx.x; e < N2; e += BS) {
                     const int a = e % N, b = e / N;
                     double2 sa = make_double2(0.0, 0.0), sbb = sa;
                     for (int i = 0; i < N; ++i)
@@ -898,7 +898,7 @@ __device__ __forceinline__ void l4f_adjoint_body(double2* sm, const L4fAdjArgs& 
                 }
                 __syncthreads();
                 // operator entries at (b,i)
-                for (int e = threadIdx.x; e < N2; e += kBlock) {
+                for (int e = threadIdx.x; e < N2; e += BS) {
                     const int b = e % N, i = e / N;
                     double2 sa = make_double2(0.0, 0.0), sbb = sa;
                     for (int a = 0; a < N; ++a)
@@ -919,7 +919,7 @@ __device__ __forceinline__ void l4f_adjoint_body(double2* sm, const L4fAdjArgs& 
                 }
                 __syncthreads();
                 // operator entries at (n,a)
-                for (int e = threadIdx.x; e < N2; e += kBlock) {
+                for (int e = threadIdx.x; e < N2; e += BS) {
                     const int n = e % N, a = e / N;
                     double2 sa = make_double2(0.0, 0.0), sbb = sa;
                     for (int i = 0; i < N; ++i)
@@ -937,24 +937,29 @@ __device__ __forceinline__ void l4f_adjoint_body(double2* sm, const L4fAdjArgs& 
 #undef BB
 #undef TB
             }
-        for (int e = threadIdx.x; e < A.num_ops * N2; e += kBlock) A.Abar[t * A.num_ops * N2 + e] = Ab[e];
+        for (int e = threadIdx.x; e < A.num_ops * N2; e += BS) A.Abar[t * A.num_ops * N2 + e] = Ab[e];
         if constexpr (LDS_SLICES)
-            for (long e = threadIdx.x; e < 3 * A.nf3; e += kBlock) A.sbar[t * 3 * A.nf3 + e] = sb[e];
+            for (long e = threadIdx.x; e < 3 * A.nf3; e += BS) A.sbar[t * 3 * A.nf3 + e] = sb[e];
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_l4f_adjoint(L4fAdjArgs A) {
+// 1024 threads per time: every entry is still summed by one thread in the
+// same order (the block size only changes which thread owns it, so results
+// are identical at any size), and the sixteen waves of a CU hide the latency
+// of the scattered slice gathers that four waves left exposed (d = 16: 4x).
+constexpr int kL4AdjBlock = 1024;
+__global__ __launch_bounds__(kL4AdjBlock) void k_l4f_adjoint(L4fAdjArgs A) {
     extern __shared__ double2 sm[];
     if (A.lds_tb) {
         if (A.lds_slices)
-            l4f_adjoint_body<true, true>(sm, A);
+            l4f_adjoint_body<true, true, kL4AdjBlock>(sm, A);
         else
-            l4f_adjoint_body<false, true>(sm, A);
+            l4f_adjoint_body<false, true, kL4AdjBlock>(sm, A);
     } else {
         if (A.lds_slices)
-            l4f_adjoint_body<true, false>(sm, A);
+            l4f_adjoint_body<true, false, kL4AdjBlock>(sm, A);
         else
-            l4f_adjoint_body<false, false>(sm, A);
+            l4f_adjoint_body<false, false, kL4AdjBlock>(sm, A);
     }
 }
 

@@ -465,6 +465,60 @@ __global__ __launch_bounds__(Cfg::kFT) TACO_FFT_OCC(Cfg) void k_phase_b(FcrArgs 
     }
     __syncthreads();
     const long pend = P.ajob_start[P.a0 + job + 1];
+    if constexpr (kPT <= 4) {
+        // The A-job's live (triple, path) pairs in order (path 0, F: the
+        // spectrum of g_mirror(j); path 1, C: that of conj(g_j)); the next
+        // pair's spectrum slice is loaded into registers while the current
+        // product is transformed (tiles of 4 values per thread have the
+        // registers for it).
+        auto next_live = [&](long& p, int& path) {
+            while (p < pend) {
+                const int mask = P.job_mask[p];
+                while (path < 2 && !(mask & (1 << path))) ++path;
+                if (path < 2) return;
+                ++p;
+                path = 0;
+            }
+        };
+        auto load_spec = [&](double2 (&Y)[kPT], long p, int path) {
+            int i, j, k;
+            decode_triple(P.job_tri[p], P.nf, i, j, k);
+            const double2* sp = P.spec + static_cast<long>(path == 0 ? 2 * P.mirror[j] : 2 * j + 1) * P.n +
+                                static_cast<long>(k1b) * gs;
+#pragma unroll
+            for (int q = 0; q < kPT; ++q) {
+                const int e = threadIdx.x + q * kFT;
+                if (e < total) Y[q] = sp[e];
+            }
+        };
+        long p = P.ajob_start[P.a0 + job];
+        int path = 0;
+        next_live(p, path);
+        double2 Y[kPT];
+        if (p < pend) load_spec(Y, p, path);
+        while (p < pend) {
+            long pn = p;
+            int pathn = path + 1;
+            next_live(pn, pathn);
+#pragma unroll
+            for (int q = 0; q < kPT; ++q) {
+                const int e = threadIdx.x + q * kFT;
+                if (e < total) buf[Tl.at(e >> Tl.lg, e & (gs - 1))] = cmul(X[q], Y[q]);
+            }
+            __syncthreads();
+            if (pn < pend) load_spec(Y, pn, pathn);
+            tile_fft<Cfg, true>(buf, Tl, P.twStd, 0, true);
+            double2* out = P.inter2 + (static_cast<long>(path) * P.njobs + (p - P.job0)) * P.n;
+            for (int e = threadIdx.x; e < total; e += kFT) {
+                const int grp = Tl.grp_of(e), pos = Tl.pos_of(e);
+                out[(static_cast<long>(pos) << P.a) + k1b + grp] = buf[Tl.at(grp, pos)];
+            }
+            __syncthreads();
+            p = pn;
+            path = pathn;
+        }
+        return;
+    }
     for (long p = P.ajob_start[P.a0 + job]; p < pend; ++p) {
         int i, j, k;
         decode_triple(P.job_tri[p], P.nf, i, j, k);
@@ -3788,7 +3842,7 @@ static void tape_adjoint_tail(taco_tape& tp, const RkRev& rv, const double2* sl,
             HIP_OK(hipStreamSynchronize(c->stream));  // the host lists go out of scope
             set_fn_lds(reinterpret_cast<const void*>(k_l4f_adjoint), lds);
             timed(c, K_L4, 0.0, [&] {
-                hipLaunchKernelGGL(k_l4f_adjoint, dim3(l4_grid), dim3(kBlock), lds, c->stream, Af);
+                hipLaunchKernelGGL(k_l4f_adjoint, dim3(l4_grid), dim3(kL4AdjBlock), lds, c->stream, Af);
             });
         }
         HIP_OK(hipGetLastError());
@@ -4333,7 +4387,7 @@ static void mtape_forward(taco_mtape& mt, const taco_tape_set* in, double* const
 
 static void mtape_adjoint(taco_mtape& mt, const double* const* samples_bar, double* const* gamma_bar,
                           double* const* omega_bar, double* const* ops_bar, double* const* eps_bar,
-                          double* const* r0_bar) {
+                          double* const* r0_bar, double* const* bath_bar = nullptr) {
     taco_ctx* c = mt.ctx;
     const std::size_t S = mt.sets.size();
     const taco_tape& t0 = *mt.sets[0];
@@ -4403,7 +4457,7 @@ static void mtape_adjoint(taco_mtape& mt, const double* const* samples_bar, doub
         if (!mt.keep && tp.order == 4) tape_slices(tp, mt.shared.as<double2>());
         auto pick = [&](double* const* v) { return v ? v[s] : nullptr; };
         tape_adjoint_tail(tp, rv[s], mtape_slices(mt, s), pick(gamma_bar), pick(omega_bar), pick(ops_bar),
-                          pick(eps_bar), pick(r0_bar));
+                          pick(eps_bar), pick(r0_bar), pick(bath_bar));
     }
 }
 
@@ -5483,6 +5537,41 @@ int taco_mtape_adjoint(taco_mtape* mt, const double* const* samples_bar, double*
             if (!samples_bar[s]) throw Error(TACO_ERR_INVALID, "null samples_bar entry");
         DeviceGuard g(mt->ctx->device);
         mtape_adjoint(*mt, samples_bar, gamma_bar, omega_bar, ops_bar, eps_bar, r0_bar);
+    });
+}
+
+int taco_mtape_forward_bath(taco_ctx* ctx, const taco_tape_set* sets, const taco_bath_params* baths, int nsets,
+                            size_t checkpoint_every, double* const* samples_out, taco_mtape** out) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!out) throw Error(TACO_ERR_INVALID, "null output handle");
+        *out = nullptr;
+        if (!sets || !baths || nsets < 1) throw Error(TACO_ERR_INVALID, "need at least one parameter set and bath");
+        for (int s = 1; s < nsets; ++s)
+            if (sets[s].N != sets[0].N || sets[s].steps != sets[0].steps)
+                throw Error(TACO_ERR_INVALID, "parameter sets of a multi-set tape must share N and steps");
+        DeviceGuard g(ctx->device);
+        auto mt = std::make_unique<taco_mtape>();
+        mt->ctx = ctx;
+        for (int s = 0; s < nsets; ++s) {
+            const BathSrc bath{baths[s].kind, baths[s].p0, baths[s].p1, baths[s].beta};
+            mt->sets.push_back(tape_make(ctx, sets[s], checkpoint_every, &bath));
+        }
+        mtape_forward(*mt, sets, samples_out);
+        *out = mt.release();
+    });
+}
+
+int taco_mtape_adjoint_bath(taco_mtape* mt, const double* const* samples_bar, double* const* bath_bar,
+                            double* const* omega_bar, double* const* ops_bar, double* const* eps_bar,
+                            double* const* r0_bar) {
+    return guarded([&] {
+        if (!mt) throw Error(TACO_ERR_INVALID, "null taco_mtape");
+        if (!samples_bar || !bath_bar) throw Error(TACO_ERR_INVALID, "null samples_bar or bath_bar");
+        for (std::size_t s = 0; s < mt->sets.size(); ++s)
+            if (!samples_bar[s] || !bath_bar[s]) throw Error(TACO_ERR_INVALID, "null samples_bar or bath_bar entry");
+        DeviceGuard g(mt->ctx->device);
+        mtape_adjoint(*mt, samples_bar, nullptr, omega_bar, ops_bar, eps_bar, r0_bar, bath_bar);
     });
 }
 

@@ -366,11 +366,47 @@ def test_multi_set_tape_equals_single_tapes(kind, budget):
             assert np.array_equal(x, y), (kind, k)
 
 
-def test_tfim_losses_many_equal_set_by_set():
+@pytest.mark.parametrize("budget", ["keep", "recompute"])
+def test_multi_set_bath_tape_equals_single_bath_tapes(budget):
+    """Multi-set tape with device-made Gamma tables (taco_mtape_*_bath): every
+    set's samples and cotangents, the bath-parameter ones included, equal a
+    BathTape of that set alone, bit for bit (slices kept or recomputed)."""
+    ws = [W.spin_pair(eps=e, delta=d, alpha=a, Nt=1024, steps=120)
+          for e, d, a in ((1.0, 1.0, 0.05), (0.8, 1.3, 0.08), (1.2, 0.6, 0.03))]
+    baths = [(0, 0.05, 5.0, 1.0), (0, 0.08, 4.0, 0.7), (1, 0.03, 3.0, 1.5)]
+    sets = [dict(Nt=1024, omegas=w.system.omegas, mirror=w.system.mirror, pair_to_freq=w.system.pair_to_freq,
+                 ops=w.system.A_eig, eps=w.system.eps, dt=w.dt, steps=120, r0=r0_for(4)) for w in ws]
+    rng = np.random.default_rng(32)
+    bars = [rng.standard_normal((121, 16)) + 1j * rng.standard_normal((121, 16)) for _ in sets]
+    if budget == "recompute":
+        taco_amd.set_slice_budget(16)
+    try:
+        mt = AD.MultiTape(sets, checkpoint_every=32, baths=baths)
+        assert mt.keeps_slices == (budget == "keep")
+        got = mt.adjoint(bars)
+        samples = [x.copy() for x in mt.samples]
+        mt.close()
+    finally:
+        taco_amd.set_slice_budget(0)
+    for k, (st, b) in enumerate(zip(sets, baths)):
+        tp = AD.BathTape(*b, st["Nt"], st["omegas"], st["mirror"], st["pair_to_freq"], st["ops"], st["eps"],
+                         st["dt"], st["steps"], st["r0"], 4, 32)
+        ref = tp.adjoint(bars[k])
+        assert np.array_equal(samples[k], tp.samples), k
+        tp.close()
+        for key in ("bath", "omegas", "eps", "r0"):
+            assert np.array_equal(got[k][key], ref[key]), (k, key)
+        for x, y in zip(got[k]["ops"], ref["ops"]):
+            assert np.array_equal(x, y), k
+
+
+@pytest.mark.parametrize("chain", ["device", "split"])
+def test_tfim_losses_many_equal_set_by_set(chain):
     """The d = 16 chain's multi-set loss path (one tape for all parameter
     sets) gives every set the loss and parameter gradient of the set-by-set
-    path, bit for bit."""
-    prob = AD.TfimChainProblem(Nt=256, dt=0.02, steps=12, checkpoint_every=4)
+    path, bit for bit, with the bath chain on the device (BathTCL4Propagate
+    and its multi-set form) or in separate steps."""
+    prob = AD.TfimChainProblem(Nt=256, dt=0.02, steps=12, checkpoint_every=4, bath_chain=chain)
     P = [np.array([0.1, 5.0, 1.0, 1.0, h]) for h in (0.7, 1.0, 1.3)]
     with torch.no_grad():
         targets = [prob.rho_lab(torch.tensor(p * 1.03, dtype=torch.float64)).detach() for p in P]
