@@ -943,23 +943,27 @@ __device__ __forceinline__ void l4f_adjoint_body(double2* sm, const L4fAdjArgs& 
     }
 }
 
-// 1024 threads per time: every entry is still summed by one thread in the
-// same order (the block size only changes which thread owns it, so results
-// are identical at any size), and the sixteen waves of a CU hide the latency
-// of the scattered slice gathers that four waves left exposed (d = 16: 4x).
-constexpr int kL4AdjBlock = 1024;
-__global__ __launch_bounds__(kL4AdjBlock) void k_l4f_adjoint(L4fAdjArgs A) {
+// Every entry is summed by one thread in a fixed order, so the block size
+// only changes which thread owns it and the results are identical at any
+// size.  Large N (d >= 12: tens of thousands of entries per time, scattered
+// slice gathers) runs 1024 threads per time, whose sixteen waves per CU hide
+// the gathers' latency (d = 16: 23.2 -> 19.0 ms); small N keeps 256 threads
+// (d = 4: 0.43 ms, twice that at 1024).
+constexpr int kL4AdjBlockLarge = 1024;
+constexpr int kL4AdjLargeN = 12;
+template <int BS>
+__global__ __launch_bounds__(BS) void k_l4f_adjoint(L4fAdjArgs A) {
     extern __shared__ double2 sm[];
     if (A.lds_tb) {
         if (A.lds_slices)
-            l4f_adjoint_body<true, true, kL4AdjBlock>(sm, A);
+            l4f_adjoint_body<true, true, BS>(sm, A);
         else
-            l4f_adjoint_body<false, true, kL4AdjBlock>(sm, A);
+            l4f_adjoint_body<false, true, BS>(sm, A);
     } else {
         if (A.lds_slices)
-            l4f_adjoint_body<true, false, kL4AdjBlock>(sm, A);
+            l4f_adjoint_body<true, false, BS>(sm, A);
         else
-            l4f_adjoint_body<false, false, kL4AdjBlock>(sm, A);
+            l4f_adjoint_body<false, false, BS>(sm, A);
     }
 }
 

@@ -3840,9 +3840,16 @@ static void tape_adjoint_tail(taco_tape& tp, const RkRev& rv, const double2* sl,
             Af.r_ent = dp[8];
             Af.nrt = static_cast<int>(LL.r_tgt.size());
             HIP_OK(hipStreamSynchronize(c->stream));  // the host lists go out of scope
-            set_fn_lds(reinterpret_cast<const void*>(k_l4f_adjoint), lds);
+            const bool large = tp.N >= kL4AdjLargeN;
+            set_fn_lds(large ? reinterpret_cast<const void*>(k_l4f_adjoint<kL4AdjBlockLarge>)
+                             : reinterpret_cast<const void*>(k_l4f_adjoint<kBlock>),
+                       lds);
             timed(c, K_L4, 0.0, [&] {
-                hipLaunchKernelGGL(k_l4f_adjoint, dim3(l4_grid), dim3(kL4AdjBlock), lds, c->stream, Af);
+                if (large)
+                    hipLaunchKernelGGL(k_l4f_adjoint<kL4AdjBlockLarge>, dim3(l4_grid), dim3(kL4AdjBlockLarge), lds,
+                                       c->stream, Af);
+                else
+                    hipLaunchKernelGGL(k_l4f_adjoint<kBlock>, dim3(l4_grid), dim3(kBlock), lds, c->stream, Af);
             });
         }
         HIP_OK(hipGetLastError());
