@@ -1199,17 +1199,35 @@ __global__ __launch_bounds__(kBlock) void k_fcr_adj_fin(FcrAdjArgs P) {
 
 // Gamma cotangent per bucket: gbar[b][t] = sum over the bucket's entries
 // (position, role) of series role (4 g1bar, 5 g2bar, 6 gmbar), in list order.
+// Per bucket b (blockIdx.y), its list of (position, role) entries in
+// gridDim.z contiguous pieces: piece z (blockIdx.z) sums its entries in list
+// order into part[(z nf + b) T + t]; k_bucket_reduce_pieces adds the pieces
+// in order.  Enough workgroups to hide the gathers' latency (a bucket can
+// hold thousands of entries), and a fixed order.
+constexpr int kBredPieces = 32;
 __global__ __launch_bounds__(kBlock) void k_bucket_reduce(const double2* scr, long T, const int* ent_start,
-                                                          const int* ent, long Nt, double2* gbar) {
-    const int b = blockIdx.y;
+                                                          const int* ent, double2* part) {
+    const int b = blockIdx.y, nf = gridDim.y, z = blockIdx.z, P = gridDim.z;
     const long t = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
     if (t >= T) return;
+    const long q0 = ent_start[b], len = ent_start[b + 1] - q0;
+    const long lo = q0 + len * z / P, hi = q0 + len * (z + 1) / P;
     double2 s = make_double2(0.0, 0.0);
-    for (int q = ent_start[b]; q < ent_start[b + 1]; ++q) {
+    for (long q = lo; q < hi; ++q) {
         const int code = ent[q];
         const long p = code / 3, role = 4 + code % 3;
         s = cadd(s, scr[(p * 11 + role) * T + t]);
     }
+    part[(static_cast<long>(z) * nf + b) * T + t] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bucket_reduce_pieces(const double2* part, long T, long Nt,
+                                                                 double2* gbar) {
+    const int b = blockIdx.y, nf = gridDim.y;
+    const long t = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (t >= T) return;
+    double2 s = make_double2(0.0, 0.0);
+    for (int z = 0; z < kBredPieces; ++z) s = cadd(s, part[(static_cast<long>(z) * nf + b) * T + t]);
     gbar[static_cast<long>(b) * Nt + t] = s;
 }
 

@@ -2212,7 +2212,7 @@ struct taco_ctx {
     std::map<int, std::array<int, 3>> fft_choice;  // per FFT log-length: tile config of phases A, B, C
     std::map<int, std::unique_ptr<taco_engine::Twiddles>> tw;
     taco_engine::DevBuf gamma, omegas, mirror, p2f, ops, spec, inter1, inter2, conv, slices, t2pos, l4, gw, misc,
-        rk, eps, jobtri, clsstart, jobmask, jobrow, comb, t2scr, rkmid;
+        rk, eps, jobtri, clsstart, jobmask, jobrow, comb, t2scr, rkmid, bred;
     taco_engine::PhaseTab phtab;  // phase table of the last plain-path system (TACO_PHASE_LIBM)
     std::size_t l4_lds_set = 0;  // dynamic LDS limit set on the fused L4 kernel
     std::size_t slice_budget = 0;  // bytes of F/C/R slices per pass of the batch builders (0: default)
@@ -2588,10 +2588,8 @@ static FcrJob fcr_prepare(taco_ctx* c, const FcrPlan& pl, const FcrLists& lists)
     std::vector<int> order;
     std::vector<unsigned char> mask;
     std::vector<std::uint64_t> key(static_cast<std::size_t>(nf3));
+    triple_omega_keys(pl.h_omegas, nf, key.data());
     for (long t = 0; t < nf3; ++t) {
-        const int k = static_cast<int>(t % nf), j = static_cast<int>((t / nf) % nf), i = static_cast<int>(t / (nf * nf));
-        const double om = (pl.h_omegas[i] + pl.h_omegas[j]) + pl.h_omegas[k];
-        std::memcpy(&key[static_cast<std::size_t>(t)], &om, 8);
         const unsigned char m = pl.need ? (*pl.need)[static_cast<std::size_t>(t)] : 7;
         if (m & 3) order.push_back(static_cast<int>(t));
     }
@@ -2644,13 +2642,19 @@ static FcrJob fcr_prepare(taco_ctx* c, const FcrPlan& pl, const FcrLists& lists)
     // Phase B visits the A-jobs of each chunk grouped by the j of their first
     // triple, so the workgroups running at one time mostly read the same
     // spectra (of g_mirror(j) and conj g_j), from L2.
+    // (a counting sort by j: stable, linear in the A-jobs)
     std::vector<int> bperm(static_cast<std::size_t>(nA));
-    for (std::size_t c = 0; c + 1 < chunk_a.size(); ++c) {
-        const long c0 = chunk_a[c], c1 = chunk_a[c + 1];
-        std::iota(bperm.begin() + c0, bperm.begin() + c1, static_cast<int>(c0));
-        std::stable_sort(bperm.begin() + c0, bperm.begin() + c1, [&](int x, int y) {
-            return (order[ajs[x]] / nf) % nf < (order[ajs[y]] / nf) % nf;
-        });
+    {
+        std::vector<int> cnt(static_cast<std::size_t>(nf) + 1);
+        auto j_of = [&](long x) { return (order[ajs[x]] / nf) % nf; };
+        for (std::size_t c = 0; c + 1 < chunk_a.size(); ++c) {
+            const long c0 = chunk_a[c], c1 = chunk_a[c + 1];
+            std::fill(cnt.begin(), cnt.end(), 0);
+            for (long x = c0; x < c1; ++x) ++cnt[static_cast<std::size_t>(j_of(x)) + 1];
+            for (int q = 0; q < nf; ++q) cnt[static_cast<std::size_t>(q) + 1] += cnt[static_cast<std::size_t>(q)];
+            for (long x = c0; x < c1; ++x)
+                bperm[static_cast<std::size_t>(c0 + cnt[static_cast<std::size_t>(j_of(x))]++)] = static_cast<int>(x);
+        }
     }
     lists.jobtri->ensure(std::max<std::size_t>(order.size() + bperm.size() + ajs.size(), 1) * sizeof(int));
     lists.jobmask->ensure(std::max<std::size_t>(mask.size(), 1));
@@ -3900,8 +3904,13 @@ static void tape_adjoint_tail(taco_tape& tp, const RkRev& rv, const double2* sl,
             int* dent = dstart + start.size();
             HIP_OK(hipMemcpyAsync(dstart, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
             HIP_OK(hipMemcpyAsync(dent, ent.data(), ent.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
-            hipLaunchKernelGGL(k_bucket_reduce, dim3(nub, static_cast<unsigned>(tp.nf)), dim3(kBlock), 0, c->stream,
-                               scr, static_cast<long>(T), dstart, dent, static_cast<long>(T), gb4);
+            // each bucket's list in kBredPieces contiguous pieces (partial sums in
+            // list order), the pieces then summed in order
+            c->bred.ensure(static_cast<std::size_t>(kBredPieces) * tp.nf * T * 16);
+            hipLaunchKernelGGL(k_bucket_reduce, dim3(nub, static_cast<unsigned>(tp.nf), kBredPieces), dim3(kBlock), 0,
+                               c->stream, scr, static_cast<long>(T), dstart, dent, c->bred.as<double2>());
+            hipLaunchKernelGGL(k_bucket_reduce_pieces, dim3(nub, static_cast<unsigned>(tp.nf)), dim3(kBlock), 0,
+                               c->stream, c->bred.as<double2>(), static_cast<long>(T), static_cast<long>(T), gb4);
             HIP_OK(hipGetLastError());
             gb4_dev = gb4;
             HIP_OK(hipMemcpyAsync(gbar_h.data(), gb4, gbar_h.size() * 16, hipMemcpyDeviceToHost, c->stream));

@@ -38,6 +38,8 @@ void gamma_trapz(const double* C, std::size_t N, double dt, const double* omegas
 void mirror_indices(const double* omegas, int nf, double fidx_tol, int* mirror);
 // std::stable_sort(order, key[x] < key[y]) on all host cores (same result).
 void stable_sort_by_key(std::vector<int>& order, const std::uint64_t* key);
+// key[t] = the bit pattern of (w_i + w_j) + w_k for t = (i nf + j) nf + k (OpenMP).
+void triple_omega_keys(const double* omegas, int nf, std::uint64_t* key);
 void system_build(const double* H, int N, const double* A_lab, int num_ops, double freq_tol, double* eps,
                   double* U, double* A_eig, int* nf_out, double* omegas, int* mirror, int* p2f,
                   int* zero_index);

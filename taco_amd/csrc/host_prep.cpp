@@ -28,6 +28,7 @@
 #include <functional>
 #include <limits>
 #include <stdexcept>
+#include <cstring>
 #include <vector>
 
 namespace taco_engine {
@@ -387,6 +388,16 @@ namespace taco_engine {
 // std::stable_sort of order by key[order[.]] on all host cores: chunks sorted
 // in parallel, then merged pairwise (inplace_merge keeps equal keys of the
 // left run first), so the result is the unique stable order.
+void triple_omega_keys(const double* omegas, int nf, std::uint64_t* key) {
+    const long nf3 = static_cast<long>(nf) * nf * nf;
+#pragma omp parallel for schedule(static)
+    for (long t = 0; t < nf3; ++t) {
+        const long k = t % nf, j = (t / nf) % nf, i = t / (static_cast<long>(nf) * nf);
+        const double om = (omegas[i] + omegas[j]) + omegas[k];
+        std::memcpy(&key[t], &om, 8);
+    }
+}
+
 void stable_sort_by_key(std::vector<int>& order, const std::uint64_t* key) {
     const long n = static_cast<long>(order.size());
     auto less = [key](int x, int y) { return key[x] < key[y]; };
