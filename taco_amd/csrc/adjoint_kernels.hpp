@@ -1357,31 +1357,78 @@ __global__ __launch_bounds__(kBlock) void k_tcl2_adj_t(Tcl2AdjArgs A) {
         A.X[t * N2 + e] = x;
     }
     __syncthreads();
+    // Gamma_b cotangent, imaginary part (the real part, 2 sum_parts
+    // Re<Lbar,K>, comes from k_tcl2_adj_kdot and k_tcl2_adj_gam)
     for (int b = 0; b < A.nf; ++b) {
-        double gp = 0.0, sp = 0.0;
+        double sp = 0.0;
         bool any = false;
         for (int pi = 0; pi < A.npart; ++pi) {
             if (A.part_b[pi] != b) continue;
             any = true;
-            const double2* K = A.K + pi * D2;
-            for (long e = threadIdx.x; e < D2; e += kBlock) {
-                const double2 l = Lb[e], k = K[e];
-                gp += l.x * k.x + l.y * k.y;
-            }
             const double2* M = A.AdA + static_cast<long>(pi) * N2;
             for (int e = threadIdx.x; e < N2; e += kBlock) {
                 const double2 x = Xb[e], m = M[e];
                 sp += x.x * m.x + x.y * m.y;
             }
         }
-        double2 out = make_double2(0.0, 0.0);
-        if (any) {
-            const double g = block_sum(gp, wred);
-            const double sv = block_sum(sp, wred);
-            out = make_double2(2.0 * g, sv);
-        }
-        if (threadIdx.x == 0) A.gbar[static_cast<long>(b) * A.T + t] = out;
+        double sv = 0.0;
+        if (any) sv = block_sum(sp, wred);
+        if (threadIdx.x == 0) A.gbar[static_cast<long>(b) * A.T + t] = make_double2(0.0, sv);
     }
+}
+
+// G[pi][t] = sum_e Re(conj(K_pi[e]) Lbar_t[e]) over e-chunks (blockIdx.z):
+// a workgroup stages 8 parts' and 8 times' rows of 256 entries in LDS and
+// each of its 64 (part, time) pairs is summed by 4 threads (entries c = sub
+// + 4m of each staged row, in order) that meet in a fixed order; the chunk
+// sums land in part[(z npart + pi) T + t].  Each Lbar_t and K_pi row is
+// read once per tile of 8 parts / 8 times instead of once per part / time.
+constexpr int kKdTile = 8, kKdE = 256;
+__global__ __launch_bounds__(kBlock) void k_tcl2_adj_kdot(const double2* K, const double2* Lbar, long D2, int npart,
+                                                          long T, long echunk, double* part) {
+    __shared__ double2 sk[kKdTile][kKdE], sl[kKdTile][kKdE];
+    __shared__ double red[4][64];
+    const int p0 = blockIdx.x * kKdTile;
+    const long t0 = static_cast<long>(blockIdx.y) * kKdTile;
+    const long e0 = static_cast<long>(blockIdx.z) * echunk, e1 = min(D2, e0 + echunk);
+    const int pair = threadIdx.x & 63, sub = threadIdx.x >> 6;
+    const int pp = pair / kKdTile, tt = pair % kKdTile;
+    const double2 zero = make_double2(0.0, 0.0);
+    double acc = 0.0;
+    for (long eb = e0; eb < e1; eb += kKdE) {
+        for (int q = threadIdx.x; q < kKdTile * kKdE; q += kBlock) {
+            const int r = q / kKdE, c = q % kKdE;
+            const long e = eb + c;
+            sk[r][c] = (p0 + r < npart && e < e1) ? K[static_cast<long>(p0 + r) * D2 + e] : zero;
+            sl[r][c] = (t0 + r < T && e < e1) ? Lbar[(t0 + r) * D2 + e] : zero;
+        }
+        __syncthreads();
+        for (int c = sub; c < kKdE; c += 4) {
+            const double2 k = sk[pp][c], l = sl[tt][c];
+            acc += l.x * k.x + l.y * k.y;
+        }
+        __syncthreads();
+    }
+    red[sub][pair] = acc;
+    __syncthreads();
+    if (sub == 0 && p0 + pp < npart && t0 + tt < T)
+        part[(static_cast<long>(blockIdx.z) * npart + p0 + pp) * T + t0 + tt] =
+            (red[0][pair] + red[1][pair]) + (red[2][pair] + red[3][pair]);
+}
+
+// gbar[b][t].x = 2 sum over the parts of bucket b (in order) of their
+// chunk sums (in order).
+__global__ __launch_bounds__(kBlock) void k_tcl2_adj_gam(const double* part, int npart, int nchunk, long T,
+                                                         const int* part_b, double2* gbar) {
+    const int b = blockIdx.y;
+    const long t = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (t >= T) return;
+    double g = 0.0;
+    for (int pi = 0; pi < npart; ++pi) {
+        if (part_b[pi] != b) continue;
+        for (int z = 0; z < nchunk; ++z) g += part[(static_cast<long>(z) * npart + pi) * T + t];
+    }
+    gbar[static_cast<long>(b) * T + t].x = 2.0 * g;
 }
 
 // Chunk sums over times of W_b, V_b and the eps parts (grid: outputs x chunks).

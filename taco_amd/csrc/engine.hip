@@ -3956,6 +3956,22 @@ static void tape_adjoint_tail(taco_tape& tp, const RkRev& rv, const double2* sl,
     Q.wsum = t2w;
     const unsigned nob = static_cast<unsigned>((nout + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_tcl2_adj_t, dim3(static_cast<unsigned>(T)), dim3(kBlock), 2 * N2 * 16, c->stream, Q);
+    if (tp.order != 0 && npart) {
+        // sum_e Re(conj(K_part) Lbar_t): tiles of 8 parts x 8 times over e-chunks of 16384
+        const long echunk = 16384;
+        const int nzc = static_cast<int>((D2 + echunk - 1) / echunk);
+        c->bred.ensure(static_cast<std::size_t>(nzc) * npart * T * sizeof(double));
+        double* kd = reinterpret_cast<double*>(c->bred.as<double2>());
+        hipLaunchKernelGGL(k_tcl2_adj_kdot,
+                           dim3(static_cast<unsigned>((npart + kKdTile - 1) / kKdTile),
+                                static_cast<unsigned>((T + kKdTile - 1) / kKdTile), static_cast<unsigned>(nzc)),
+                           dim3(kBlock), 0, c->stream, t2K, Lb, static_cast<long>(D2), static_cast<int>(npart),
+                           static_cast<long>(T), echunk, kd);
+        hipLaunchKernelGGL(k_tcl2_adj_gam,
+                           dim3(static_cast<unsigned>((T + kBlock - 1) / kBlock), static_cast<unsigned>(tp.nf)),
+                           dim3(kBlock), 0, c->stream, kd, static_cast<int>(npart), nzc, static_cast<long>(T), t2b,
+                           t2g);
+    }
     hipLaunchKernelGGL(k_tcl2_adj_w, dim3(nob, static_cast<unsigned>(nchunk)), dim3(kBlock), 0, c->stream, Q);
     hipLaunchKernelGGL(k_tcl2_adj_wsum, dim3(nob), dim3(kBlock), 0, c->stream, Q);
     HIP_OK(hipGetLastError());
