@@ -424,9 +424,10 @@ int taco_tape_adjoint_bath(taco_tape* tape, const double* samples_bar, double* b
  * (a workgroup per set) and so are the reverse RK4's recompute and walk.
  * Every set's samples and cotangents equal those of taco_tape_forward /
  * taco_tape_adjoint on that set alone, bit for bit.  The sets' F/C/R slices
- * stay on the device while their total fits the context's slice budget
- * (taco_set_slice_budget); otherwise the reverse pass recomputes each set's
- * slices in one shared buffer (taco_mtape_keeps_slices tells which).
+ * stay on the device, set by set in order, while their running total fits
+ * the context's slice budget (taco_set_slice_budget); the reverse pass
+ * recomputes the slices of the sets past it in one shared buffer
+ * (taco_mtape_keeps_slices: 1 when every set keeps its slices).
  * samples_out / samples_bar / the cotangent arrays hold one pointer per set
  * (layouts as in taco_tape_*); an output array or entry may be NULL. */
 typedef struct taco_tape_set {

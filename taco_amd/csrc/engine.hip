@@ -2131,6 +2131,13 @@ struct TapeBufs {
     DevBuf gamma, omegas, mirror, p2f, ops, eps, t2pos, jobtri, jobmask, clsstart, jobrow, slices, lgen, ck, adjl;
     DevBuf bath;  // bath-sourced tapes: C(t), phase steps and the bath reverse's scratch
     PhaseTab phtab;
+    std::size_t bytes() const {
+        std::size_t n = phtab.tab.bytes;
+        for (const DevBuf* b : {&gamma, &omegas, &mirror, &p2f, &ops, &eps, &t2pos, &jobtri, &jobmask, &clsstart,
+                                &jobrow, &slices, &lgen, &ck, &adjl, &bath})
+            n += b->bytes;
+        return n;
+    }
     void swap(TapeBufs& o) {
         for (auto [a, b] : {std::pair{&gamma, &o.gamma}, {&omegas, &o.omegas}, {&mirror, &o.mirror}, {&p2f, &o.p2f},
                             {&ops, &o.ops}, {&eps, &o.eps}, {&t2pos, &o.t2pos}, {&jobtri, &o.jobtri},
@@ -2140,6 +2147,16 @@ struct TapeBufs {
             a->swap(*b);
         phtab.swap(o.phtab);
     }
+};
+
+// Buffers of a multi-set tape.  The context keeps one set that a live
+// multi-set tape at a time uses and later ones reuse (tens of GB at d = 16 with
+// 64 sets, whose allocation would otherwise recur on every gradient); a tape
+// created while another holds it gets buffers of its own.
+struct MtapeArena {
+    DevBuf slices;  // the F/C/R slices of the sets within the slice budget, one after another
+    DevBuf shared;  // one set's slices at a time, for the sets past the budget
+    DevBuf samp, rv, args, mid;
 };
 
 // Reverse-mode kernels (device code, same namespace as the forward kernels).
@@ -2218,7 +2235,9 @@ struct taco_ctx {
     std::size_t slice_budget = 0;  // bytes of F/C/R slices per pass of the batch builders (0: default)
     // term lists of the factorised L4 reverse per (pair_to_freq, N, nf)
     std::vector<std::pair<std::vector<int>, std::shared_ptr<taco_engine::L4AdjLists>>> adj_lists;
-    std::vector<std::unique_ptr<taco_engine::TapeBufs>> tape_pool;  // buffers of destroyed tapes (at most 2)
+    std::vector<std::unique_ptr<taco_engine::TapeBufs>> tape_pool;  // buffers of destroyed tapes
+    taco_engine::MtapeArena marena;     // multi-set tape buffers, reused across tapes
+    const void* marena_owner = nullptr;  // the live multi-set tape using marena
 };
 
 namespace taco_engine {
@@ -3487,29 +3506,52 @@ struct L4AdjLists {
     std::vector<int> xp_xz, xp_start, xp_ent, f_tgt, f_start, f_ent, r_tgt, r_start, r_ent;
 };
 
+// CSR of (key, word) pairs: the distinct keys ascending, each key's words in
+// insertion order (a stable counting sort over keys 0..nkeys-1).
+static void csr_by_key(const std::vector<int>& key, const std::vector<int>& word, long nkeys, std::vector<int>& keys,
+                       std::vector<int>& start, std::vector<int>& ent) {
+    std::vector<int> cnt(static_cast<std::size_t>(nkeys) + 1, 0);
+    for (int k : key) ++cnt[static_cast<std::size_t>(k) + 1];
+    for (long k = 0; k < nkeys; ++k) cnt[static_cast<std::size_t>(k) + 1] += cnt[static_cast<std::size_t>(k)];
+    ent.assign(word.size(), 0);
+    std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+    for (std::size_t q = 0; q < key.size(); ++q) ent[static_cast<std::size_t>(pos[static_cast<std::size_t>(key[q])]++)] = word[q];
+    keys.clear();
+    start.assign(1, 0);
+    for (long k = 0; k < nkeys; ++k)
+        if (cnt[static_cast<std::size_t>(k) + 1] > cnt[static_cast<std::size_t>(k)]) {
+            keys.push_back(static_cast<int>(k));
+            start.push_back(cnt[static_cast<std::size_t>(k) + 1]);
+        }
+}
+
 static L4AdjLists l4_adjoint_lists(const int* p2f, int N, int nf) {
     if (N > 128) throw Error(TACO_ERR_INVALID, "factorised L4 reverse supports N <= 128");
     auto f = [&](int r, int c) { return p2f[r + c * N]; };
     const long nfl = nf;
-    auto id3 = [nfl](long x, long y, long z) { return (x * nfl + y) * nfl + z; };
-    auto csr = [](std::map<long, std::vector<int>>& m, std::vector<int>* keys, std::vector<int>& start,
-                  std::vector<int>& ent, bool pair_key) {
-        start.assign(1, 0);
-        for (auto& kv : m) {
-            keys->push_back(pair_key ? static_cast<int>(kv.first) : static_cast<int>(kv.first));
-            ent.insert(ent.end(), kv.second.begin(), kv.second.end());
-            start.push_back(static_cast<int>(ent.size()));
-        }
-    };
+    auto id3 = [nfl](long x, long y, long z) { return static_cast<int>((x * nfl + y) * nfl + z); };
     L4AdjLists L;
-    std::map<long, std::vector<int>> xm, fm, rm;
+    // X-form pairs keyed z nf + x (the order of x + 65536 z), stored as x + 65536 z
+    std::vector<int> xk, xw;
+    xk.reserve(2 * static_cast<std::size_t>(N) * N * N);
+    xw.reserve(xk.capacity());
     for (int a = 0; a < N; ++a)
         for (int i = 0; i < N; ++i)
             for (int b = 0; b < N; ++b) {
                 const int w = a | (i << 8) | (b << 16);
-                xm[f(b, a) + 65536L * f(a, i)].push_back(w);
-                xm[f(i, b) + 65536L * f(a, i)].push_back(w | (1 << 24));
+                xk.push_back(f(a, i) * nf + f(b, a));
+                xw.push_back(w);
+                xk.push_back(f(a, i) * nf + f(i, b));
+                xw.push_back(w | (1 << 24));
             }
+    csr_by_key(xk, xw, nfl * nfl, L.xp_xz, L.xp_start, L.xp_ent);
+    for (int& k : L.xp_xz) k = (k % nf) + 65536 * (k / nf);
+    std::vector<int> fk, fw, rk, rw;
+    const std::size_t n4 = static_cast<std::size_t>(N) * N * N * N;
+    fk.reserve(2 * n4);
+    fw.reserve(2 * n4);
+    rk.reserve(2 * n4);
+    rw.reserve(2 * n4);
     for (int n = 0; n < N; ++n)
         for (int i = 0; i < N; ++i)
             for (int a = 0; a < N; ++a)
@@ -3517,14 +3559,17 @@ static L4AdjLists l4_adjoint_lists(const int* p2f, int N, int nf) {
                     const unsigned w = static_cast<unsigned>(n) | (static_cast<unsigned>(i) << 8) |
                                        (static_cast<unsigned>(a) << 16) | (static_cast<unsigned>(b) << 24);
                     const int w0 = static_cast<int>(w), w1 = static_cast<int>(w | (1u << 31));
-                    fm[id3(f(b, a), f(b, i), f(n, b))].push_back(w0);  // M: F part
-                    fm[id3(f(a, n), f(b, i), f(n, b))].push_back(w1);  // I
-                    rm[id3(f(b, a), f(n, a), f(a, i))].push_back(w0);  // M: R part
-                    rm[id3(f(i, b), f(n, a), f(a, i))].push_back(w1);  // K
+                    fk.push_back(id3(f(b, a), f(b, i), f(n, b)));  // M: F part
+                    fw.push_back(w0);
+                    fk.push_back(id3(f(a, n), f(b, i), f(n, b)));  // I
+                    fw.push_back(w1);
+                    rk.push_back(id3(f(b, a), f(n, a), f(a, i)));  // M: R part
+                    rw.push_back(w0);
+                    rk.push_back(id3(f(i, b), f(n, a), f(a, i)));  // K
+                    rw.push_back(w1);
                 }
-    csr(xm, &L.xp_xz, L.xp_start, L.xp_ent, true);
-    csr(fm, &L.f_tgt, L.f_start, L.f_ent, false);
-    csr(rm, &L.r_tgt, L.r_start, L.r_ent, false);
+    csr_by_key(fk, fw, nfl * nfl * nfl, L.f_tgt, L.f_start, L.f_ent);
+    csr_by_key(rk, rw, nfl * nfl * nfl, L.r_tgt, L.r_start, L.r_ent);
     return L;
 }
 
@@ -3673,14 +3718,15 @@ static void tape_bath_adjoint(taco_tape& tp, const double2* gb4, const double2* 
                               std::vector<double>& omb);
 
 // The term lists depend on (N, nf, pair_to_freq) only: built once per
-// context and structure (the last 8 kept), reused by every later reverse.
+// context and structure (the last 128 kept: a multi-set tape's parameter
+// sets may each have their own bucket map), reused by every later reverse.
 static const L4AdjLists& l4_adjoint_lists_cached(taco_ctx* c, const int* p2f, int N, int nf) {
     std::vector<int> key(p2f, p2f + static_cast<std::size_t>(N) * N);
     key.push_back(N);
     key.push_back(nf);
     for (auto& e : c->adj_lists)
         if (e.first == key) return *e.second;
-    if (c->adj_lists.size() >= 8) c->adj_lists.erase(c->adj_lists.begin());
+    if (c->adj_lists.size() >= 128) c->adj_lists.erase(c->adj_lists.begin());
     c->adj_lists.emplace_back(std::move(key), std::make_shared<L4AdjLists>(l4_adjoint_lists(p2f, N, nf)));
     return *c->adj_lists.back().second;
 }
@@ -4141,6 +4187,19 @@ static void trapz_device(taco_ctx* c, const double2* dC, std::size_t Nc, double 
 
 namespace taco_engine {
 
+// Buffers of a destroyed tape go back to the context's pool (at most 256
+// entries and 32 GiB) for the next tapes, so gradients of the same shape
+// stop allocating after the first.
+static void tape_pool_return(taco_tape& tp) {
+    auto& pool = tp.ctx->tape_pool;
+    std::size_t held = 0;
+    for (auto& b : pool) held += b->bytes();
+    const std::size_t add = tp.bufs.bytes();
+    if (pool.size() >= 256 || held + add > (std::size_t{32} << 30)) return;
+    pool.push_back(std::make_unique<TapeBufs>());
+    pool.back()->swap(tp.bufs);
+}
+
 // A bath whose correlation and trapz Gamma the device makes for a tape.
 struct BathSrc {
     int kind;
@@ -4335,25 +4394,44 @@ static void tape_bath_adjoint(taco_tape& tp, const double2* gb4, const double2* 
 // sets of one model): each set's F/C/R, L4 and TCL2 passes run in turn, the
 // forward RK4 of every set is one launch (a workgroup per set), and so are
 // the reverse RK4's recompute and walk.  The sets' F/C/R slices stay on the
-// device while they fit the context's slice budget; otherwise one shared
-// buffer holds one set's slices at a time and the reverse pass recomputes
-// them (the same pass, the same values).
+// device, set by set in order, while their running total fits the context's
+// slice budget; the sets past it share one buffer that holds one set's
+// slices at a time, and the reverse pass recomputes theirs (the same pass,
+// the same values).
 struct taco_mtape {
     taco_ctx* ctx = nullptr;
     std::vector<std::unique_ptr<taco_tape>> sets;
-    bool keep = true;
-    taco_engine::DevBuf shared;  // one set's slices when !keep
-    taco_engine::DevBuf samp;    // [S][T][D] sampled states
-    taco_engine::DevBuf rv;      // [S][rk_rev_values] reverse RK4 state
-    taco_engine::DevBuf args;    // launch arguments
-    taco_engine::DevBuf mid;     // [S][steps][D^2] midpoint generators of the forward RK4 (large D)
+    bool keep = true;             // every set keeps its slices
+    std::vector<char> keep_set;   // per set
+    std::vector<std::size_t> off; // kept sets: offset of their slices in A->slices (complex values)
+    // A->samp [S][T][D] sampled states, A->rv [S][rk_rev_values] reverse RK4
+    // state, A->args launch arguments, A->mid [S][steps][D^2] midpoint
+    // generators of the forward RK4 (large D)
+    taco_engine::MtapeArena own;
+    taco_engine::MtapeArena* A = nullptr;
+    ~taco_mtape() {
+        if (ctx && ctx->marena_owner == this) ctx->marena_owner = nullptr;
+    }
 };
 
 namespace taco_engine {
 
+// The context's multi-set buffers when no other live multi-set tape holds
+// them, else buffers of the tape's own.
+static void mtape_claim(taco_mtape& mt) {
+    taco_ctx* c = mt.ctx;
+    if (!c->marena_owner) {
+        c->marena_owner = &mt;
+        mt.A = &c->marena;
+    } else {
+        mt.A = &mt.own;
+    }
+}
+
 static double2* mtape_slices(taco_mtape& mt, std::size_t s) {
     taco_tape& tp = *mt.sets[s];
-    return mt.keep ? tp.slices.as<double2>() : mt.shared.as<double2>();
+    (void)tp;
+    return mt.keep_set[s] ? mt.A->slices.as<double2>() + mt.off[s] : mt.A->shared.as<double2>();
 }
 
 static void mtape_forward(taco_mtape& mt, const taco_tape_set* in, double* const* samples_out) {
@@ -4361,22 +4439,28 @@ static void mtape_forward(taco_mtape& mt, const taco_tape_set* in, double* const
     const std::size_t S = mt.sets.size();
     const taco_tape& t0 = *mt.sets[0];
     const std::size_t D = static_cast<std::size_t>(t0.N) * t0.N, T = t0.T;
+    const std::size_t budget = slice_budget(c);
     std::size_t total = 0, most = 0;
-    for (auto& tp : mt.sets) {
-        const std::size_t b = tape_slice_bytes(*tp);
-        total += b;
-        most = std::max(most, b);
+    mt.keep_set.assign(S, 0);
+    mt.off.assign(S, 0);
+    mt.keep = true;
+    for (std::size_t s = 0; s < S; ++s) {
+        const std::size_t b = tape_slice_bytes(*mt.sets[s]);
+        if (total + b <= budget) {
+            mt.off[s] = total / 16;
+            total += b;
+            mt.keep_set[s] = 1;
+        } else {
+            mt.keep = false;
+            most = std::max(most, b);
+        }
     }
-    mt.keep = total <= slice_budget(c);
-    if (mt.keep) {
-        for (auto& tp : mt.sets) tp->slices.ensure(std::max<std::size_t>(tape_slice_bytes(*tp), 16));
-    } else {
-        mt.shared.ensure(std::max<std::size_t>(most, 16));
-    }
+    mt.A->slices.ensure(std::max<std::size_t>(total, 16));
+    if (!mt.keep) mt.A->shared.ensure(std::max<std::size_t>(most, 16));
     for (std::size_t s = 0; s < S; ++s) tape_generator(*mt.sets[s], mtape_slices(mt, s));
     // one RK4 launch over every set: r0 in, samples out ([S][T][D]), final r after them
-    mt.samp.ensure((S * T + 2 * S) * D * 16);
-    double2* samp = mt.samp.as<double2>();
+    mt.A->samp.ensure((S * T + 2 * S) * D * 16);
+    double2* samp = mt.A->samp.as<double2>();
     double2* r0d = samp + S * T * D;
     double2* rend = r0d + S * D;
     for (std::size_t s = 0; s < S; ++s)
@@ -4388,26 +4472,26 @@ static void mtape_forward(taco_mtape& mt, const taco_tape_set* in, double* const
     if (lds > kRkLds) throw Error(TACO_ERR_INVALID, "RK4 kernel supports D <= 1706");
     std::vector<Rk4Sys> sys(S);
     const bool mids = rows_thr == 0 && nstage < 2 && rk4_midpoints_fit(Di, t0.steps * S);
-    if (mids) mt.mid.ensure(S * t0.steps * D * D * 16);
+    if (mids) mt.A->mid.ensure(S * t0.steps * D * D * 16);
     for (std::size_t s = 0; s < S; ++s) {
         const taco_tape& tp = *mt.sets[s];
         sys[s] = Rk4Sys{tp.lgen.as<double2>(), nstage, r0d + s * D, rend + s * D, samp + s * T * D,
                         static_cast<long>(tp.steps), tp.dt, Di};
         if (mids) {
             const long D2 = static_cast<long>(D) * D, total = static_cast<long>(tp.steps) * D2;
-            double2* m = mt.mid.as<double2>() + s * tp.steps * D * D;
+            double2* m = mt.A->mid.as<double2>() + s * tp.steps * D * D;
             hipLaunchKernelGGL(k_midpoints, dim3(static_cast<unsigned>(std::min<long>((total + kBlock - 1) / kBlock,
                                                                                      65536))),
                                dim3(kBlock), 0, c->stream, tp.lgen.as<double2>(), static_cast<long>(tp.steps), D2, m);
             sys[s].Lh = m;
         }
     }
-    mt.args.ensure(std::max(S * sizeof(Rk4Sys), 2 * S * sizeof(Rk4AdjArgs)));
-    HIP_OK(hipMemcpyAsync(mt.args.p, sys.data(), S * sizeof(Rk4Sys), hipMemcpyHostToDevice, c->stream));
+    mt.A->args.ensure(std::max(S * sizeof(Rk4Sys), 2 * S * sizeof(Rk4AdjArgs)));
+    HIP_OK(hipMemcpyAsync(mt.A->args.p, sys.data(), S * sizeof(Rk4Sys), hipMemcpyHostToDevice, c->stream));
     const double rk_bytes = static_cast<double>(S) * ((t0.steps + 1) * D * D * 16.0 + (t0.steps + 1) * D * 16.0);
     timed(c, K_RK4, rk_bytes, [&] {
         hipLaunchKernelGGL(k_rk4_batch, dim3(static_cast<unsigned>(S)), dim3(rows_thr > 0 ? rows_thr : rk4_block(Di)),
-                           lds, c->stream, mt.args.as<const Rk4Sys>());
+                           lds, c->stream, mt.A->args.as<const Rk4Sys>());
     });
     HIP_OK(hipGetLastError());
     for (std::size_t s = 0; s < S; ++s) tape_checkpoints(*mt.sets[s], samp + s * T * D);
@@ -4425,12 +4509,12 @@ static void mtape_adjoint(taco_mtape& mt, const double* const* samples_bar, doub
     const taco_tape& t0 = *mt.sets[0];
     const std::size_t D = static_cast<std::size_t>(t0.N) * t0.N, D2 = D * D, T = t0.T;
     const std::size_t per = rk_rev_values(t0);
-    mt.rv.ensure(S * per * 16);
+    mt.A->rv.ensure(S * per * 16);
     std::vector<RkRev> rv(S);
     std::vector<Rk4AdjArgs> args(2 * S);
     RkRevShape sh{};
     for (std::size_t s = 0; s < S; ++s) {
-        rv[s] = rk_rev_at(*mt.sets[s], mt.rv.as<double2>() + s * per);
+        rv[s] = rk_rev_at(*mt.sets[s], mt.A->rv.as<double2>() + s * per);
         HIP_OK(hipMemcpyAsync(rv[s].gb, samples_bar[s], T * D * 16, hipMemcpyHostToDevice, c->stream));
         sh = rk_rev_shape(*mt.sets[s], rv[s]);
         args[s] = sh.Rf;
@@ -4451,9 +4535,9 @@ static void mtape_adjoint(taco_mtape& mt, const double* const* samples_bar, doub
                                0, c->stream, rv[s].Lb, cntL);
         }
     }
-    mt.args.ensure(std::max(S * sizeof(Rk4Sys), 2 * S * sizeof(Rk4AdjArgs)));
-    const Rk4AdjArgs* dargs = mt.args.as<const Rk4AdjArgs>();
-    HIP_OK(hipMemcpyAsync(mt.args.p, args.data(), 2 * S * sizeof(Rk4AdjArgs), hipMemcpyHostToDevice, c->stream));
+    mt.A->args.ensure(std::max(S * sizeof(Rk4Sys), 2 * S * sizeof(Rk4AdjArgs)));
+    const Rk4AdjArgs* dargs = mt.A->args.as<const Rk4AdjArgs>();
+    HIP_OK(hipMemcpyAsync(mt.A->args.p, args.data(), 2 * S * sizeof(Rk4AdjArgs), hipMemcpyHostToDevice, c->stream));
     const int Di = static_cast<int>(D);
     timed(c, K_RK4, sh.bytes * static_cast<double>(S), [&] {
         if (sh.rows_f) {
@@ -4486,7 +4570,7 @@ static void mtape_adjoint(taco_mtape& mt, const double* const* samples_bar, doub
     HIP_OK(hipGetLastError());
     for (std::size_t s = 0; s < S; ++s) {
         taco_tape& tp = *mt.sets[s];
-        if (!mt.keep && tp.order == 4) tape_slices(tp, mt.shared.as<double2>());
+        if (!mt.keep_set[s] && tp.order == 4) tape_slices(tp, mt.A->shared.as<double2>());
         auto pick = [&](double* const* v) { return v ? v[s] : nullptr; };
         tape_adjoint_tail(tp, rv[s], mtape_slices(mt, s), pick(gamma_bar), pick(omega_bar), pick(ops_bar),
                           pick(eps_bar), pick(r0_bar), pick(bath_bar));
@@ -5553,6 +5637,7 @@ int taco_mtape_forward(taco_ctx* ctx, const taco_tape_set* sets, int nsets, size
         DeviceGuard g(ctx->device);
         auto mt = std::make_unique<taco_mtape>();
         mt->ctx = ctx;
+        mtape_claim(*mt);
         for (int s = 0; s < nsets; ++s) mt->sets.push_back(tape_make(ctx, sets[s], checkpoint_every));
         mtape_forward(*mt, sets, samples_out);
         *out = mt.release();
@@ -5585,6 +5670,7 @@ int taco_mtape_forward_bath(taco_ctx* ctx, const taco_tape_set* sets, const taco
         DeviceGuard g(ctx->device);
         auto mt = std::make_unique<taco_mtape>();
         mt->ctx = ctx;
+        mtape_claim(*mt);
         for (int s = 0; s < nsets; ++s) {
             const BathSrc bath{baths[s].kind, baths[s].p0, baths[s].p1, baths[s].beta};
             mt->sets.push_back(tape_make(ctx, sets[s], checkpoint_every, &bath));
@@ -5614,6 +5700,7 @@ int taco_mtape_destroy(taco_mtape* mt) {
         if (!mt) return;
         DeviceGuard g(mt->ctx->device);
         (void)hipStreamSynchronize(mt->ctx->stream);
+        for (auto& tp : mt->sets) tape_pool_return(*tp);
         delete mt;
     });
 }
@@ -5660,11 +5747,7 @@ int taco_tape_destroy(taco_tape* tape) {
         if (!tape) return;
         DeviceGuard g(tape->ctx->device);
         (void)hipStreamSynchronize(tape->ctx->stream);
-        auto& pool = tape->ctx->tape_pool;
-        if (pool.size() < 2) {
-            pool.push_back(std::make_unique<TapeBufs>());
-            pool.back()->swap(tape->bufs);
-        }
+        tape_pool_return(*tape);
         delete tape;
     });
 }

@@ -332,18 +332,22 @@ def _mt_sets(kind):
 
 
 @pytest.mark.parametrize("kind", ["spin_pair_d4", "gue3_mixed_order", "tfim_d16"])
-@pytest.mark.parametrize("budget", ["keep", "recompute"])
+@pytest.mark.parametrize("budget", ["keep", "partial", "recompute"])
 def test_multi_set_tape_equals_single_tapes(kind, budget):
     """Every set of a multi-set tape (one RK4 launch forward, one recompute
     and one walk launch in reverse for all sets) has the samples and the
     cotangents of a single tape of that set, bit for bit -- with the slices
-    kept, and with a slice budget that forces their recomputation."""
+    kept, with a slice budget that keeps only the first set's, and with one
+    that forces the recomputation of all."""
     sets, K = _mt_sets(kind)
     rng = np.random.default_rng(31)
     bars = [rng.standard_normal((s["steps"] + 1, len(s["r0"]))) + 1j * rng.standard_normal(
         (s["steps"] + 1, len(s["r0"]))) for s in sets]
     if budget == "recompute":
         taco_amd.set_slice_budget(16)
+    elif budget == "partial":  # room for the first set's slices only
+        nf0 = len(sets[0]["omegas"])
+        taco_amd.set_slice_budget(3 * nf0 ** 3 * (sets[0]["steps"] + 1) * 16)
     try:
         mt = AD.MultiTape(sets, checkpoint_every=K)
         assert mt.keeps_slices == (budget == "keep")
@@ -364,6 +368,34 @@ def test_multi_set_tape_equals_single_tapes(kind, budget):
             assert np.array_equal(again[k][key], ref[key]), (kind, k, key)
         for x, y in zip(got[k]["ops"], ref["ops"]):
             assert np.array_equal(x, y), (kind, k)
+
+
+def test_two_live_multi_set_tapes_keep_their_own_state():
+    """The context lends its multi-set buffers to one live multi-set tape at a
+    time: a second tape created while the first is alive runs on buffers of
+    its own, so reversing the first afterwards still gives its single-tape
+    cotangents, and a third tape after both are closed reuses the context's."""
+    sets, K = _mt_sets("spin_pair_d4")
+    other = [dict(st, r0=np.roll(st["r0"], 5)) for st in sets]
+    rng = np.random.default_rng(33)
+    bars = [rng.standard_normal((s_["steps"] + 1, 16)) + 0j for s_ in sets]
+    a = AD.MultiTape(sets, checkpoint_every=K)
+    b = AD.MultiTape(other, checkpoint_every=K)
+    got_b = b.adjoint(bars)
+    got_a = a.adjoint(bars)
+    a.close()
+    b.close()
+    c = AD.MultiTape(sets, checkpoint_every=K)
+    got_c = c.adjoint(bars)
+    c.close()
+    for k, (st, ot) in enumerate(zip(sets, other)):
+        for src, res in ((st, got_a[k]), (ot, got_b[k]), (st, got_c[k])):
+            tp = AD.Tape(src["gamma"], src["omegas"], src["mirror"], src["pair_to_freq"], src["ops"], src["eps"],
+                         src["dt"], src["steps"], src["r0"], 4, K)
+            ref = tp.adjoint(bars[k])
+            tp.close()
+            for key in ("gamma", "omegas", "eps", "r0"):
+                assert np.array_equal(res[key], ref[key]), (k, key)
 
 
 @pytest.mark.parametrize("budget", ["keep", "recompute"])
