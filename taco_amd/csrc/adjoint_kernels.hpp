@@ -622,7 +622,7 @@ struct L4fAdjArgs {
     int lds_slices;
     int lds_tb;           // Tbar in LDS (else after U in the scratch)
     double2* scr;         // [gridDim.x][scr_per]
-    long scr_per;         // 4 N^2 nf + 3 N^2 + N^4, + N^4 when !lds_tb, + 6 N^2
+    long scr_per;         // 4 N^2 nf + 3 N^2 + N^4, + N^4 when !lds_tb
     // Gather lists (host-built from the pair map) that give every cotangent
     // entry one owner thread summing its terms in a fixed order:
     // X-form pairs (x, z) with their (a, i, b, form) terms; F and R slice
@@ -665,7 +665,6 @@ __device__ __forceinline__ void l4f_adjoint_body(double2* sm, const L4fAdjArgs& 
     double2* S123b = S123 + N2;
     double2* Db = S123b + N2;
     double2* U = Db + N2;                               // U(n,i,b,j) at (n + i N) + (b + j N) N^2
-    double2* R3 = U + N4 + (LDS_TB ? 0 : N4);           // [3][2][N^2] entry-block partials
     const long nfl = nf;
     auto id3 = [nfl](int x, int y, int z) { return (static_cast<long>(x) * nfl + y) * nfl + z; };
     auto F2 = [f, N](int r, int c) { return f[r + c * N]; };
@@ -877,12 +876,10 @@ __device__ __forceinline__ void l4f_adjoint_body(double2* sm, const L4fAdjArgs& 
                 };
                 auto Iv = [&](int n, int i, int a, int b) { return Fp[id3(F2(a, n), F2(b, i), F2(n, b))]; };
                 auto Kv = [&](int n, int i, int a, int b) { return Rp[id3(F2(i, b), F2(n, a), F2(a, i))]; };
-                // operator entries at (a,b), (b,i) and (n,a) of the Q and S123 terms
-                // (entry e = r + c N of each block: (a,b) = (r,c), (b,i) = (r,c), (n,a) = (r,c))
-                auto ent_ab = [&](int e, double2& sa, double2& sbb) {
+                // operator entries at (a,b) of the Q and S123 terms
+                for (int e = threadIdx.x; e < N2; e += BS) {
                     const int a = e % N, b = e / N;
-                    sa = make_double2(0.0, 0.0);
-                    sbb = sa;
+                    double2 sa = make_double2(0.0, 0.0), sbb = sa;
                     for (int i = 0; i < N; ++i)
                         for (int g = 0; g < nf; ++g) {
                             const double2 qb = Qb[a + i * N + g * N2];
@@ -896,11 +893,14 @@ __device__ __forceinline__ void l4f_adjoint_body(double2* sm, const L4fAdjArgs& 
                             sbb = cadd(sbb, cmul(cconj(cmul(cmul(AA(n, a), BB(b, i)), Iv(n, i, a, b))), sv));
                             sbb = csub(sbb, cmul(cconj(cmul(cmul(BB(n, a), AA(b, i)), Kv(n, i, a, b))), sv));
                         }
-                };
-                auto ent_bi = [&](int e, double2& sa, double2& sbb) {
+                    AbA[a + b * N] = cadd(AbA[a + b * N], sa);
+                    AbB[a + b * N] = cadd(AbB[a + b * N], sbb);
+                }
+                __syncthreads();
+                // operator entries at (b,i)
+                for (int e = threadIdx.x; e < N2; e += BS) {
                     const int b = e % N, i = e / N;
-                    sa = make_double2(0.0, 0.0);
-                    sbb = sa;
+                    double2 sa = make_double2(0.0, 0.0), sbb = sa;
                     for (int a = 0; a < N; ++a)
                         for (int g = 0; g < nf; ++g) {
                             const double2 qb = Qb[a + i * N + g * N2];
@@ -914,11 +914,14 @@ __device__ __forceinline__ void l4f_adjoint_body(double2* sm, const L4fAdjArgs& 
                             sbb = cadd(sbb, cmul(cconj(cmul(cmul(AA(n, a), BB(a, b)), Iv(n, i, a, b))), sv));
                             sa = csub(sa, cmul(cconj(cmul(cmul(BB(n, a), BB(a, b)), Kv(n, i, a, b))), sv));
                         }
-                };
-                auto ent_na = [&](int e, double2& sa, double2& sbb) {
+                    AbA[b + i * N] = cadd(AbA[b + i * N], sa);
+                    AbB[b + i * N] = cadd(AbB[b + i * N], sbb);
+                }
+                __syncthreads();
+                // operator entries at (n,a)
+                for (int e = threadIdx.x; e < N2; e += BS) {
                     const int n = e % N, a = e / N;
-                    sa = make_double2(0.0, 0.0);
-                    sbb = sa;
+                    double2 sa = make_double2(0.0, 0.0), sbb = sa;
                     for (int i = 0; i < N; ++i)
                         for (int b = 0; b < N; ++b) {
                             const double2 sv = S123b[n + i * N];
@@ -926,53 +929,10 @@ __device__ __forceinline__ void l4f_adjoint_body(double2* sm, const L4fAdjArgs& 
                             sa = cadd(sa, cmul(cconj(cmul(cmul(BB(a, b), BB(b, i)), Iv(n, i, a, b))), sv));
                             sbb = csub(sbb, cmul(cconj(cmul(cmul(BB(a, b), AA(b, i)), Kv(n, i, a, b))), sv));
                         }
-                };
-                if (3 * N2 <= BS) {
-                    // the three blocks at once, a thread group each, into R3; then
-                    // each entry adds its three contributions in the blocks' order
-                    // (the same additions, in the same order, as block by block)
-                    const int grp = static_cast<int>(threadIdx.x) / N2, e = static_cast<int>(threadIdx.x) % N2;
-                    if (grp < 3) {
-                        double2 sa, sbb;
-                        if (grp == 0)
-                            ent_ab(e, sa, sbb);
-                        else if (grp == 1)
-                            ent_bi(e, sa, sbb);
-                        else
-                            ent_na(e, sa, sbb);
-                        R3[(2 * grp) * N2 + e] = sa;
-                        R3[(2 * grp + 1) * N2 + e] = sbb;
-                    }
-                    __syncthreads();
-                    for (int e2 = threadIdx.x; e2 < N2; e2 += BS)
-                        for (int g3 = 0; g3 < 3; ++g3) {
-                            AbA[e2] = cadd(AbA[e2], R3[(2 * g3) * N2 + e2]);
-                            AbB[e2] = cadd(AbB[e2], R3[(2 * g3 + 1) * N2 + e2]);
-                        }
-                    __syncthreads();
-                } else {
-                    for (int e = threadIdx.x; e < N2; e += BS) {
-                        double2 sa, sbb;
-                        ent_ab(e, sa, sbb);
-                        AbA[e] = cadd(AbA[e], sa);
-                        AbB[e] = cadd(AbB[e], sbb);
-                    }
-                    __syncthreads();
-                    for (int e = threadIdx.x; e < N2; e += BS) {
-                        double2 sa, sbb;
-                        ent_bi(e, sa, sbb);
-                        AbA[e] = cadd(AbA[e], sa);
-                        AbB[e] = cadd(AbB[e], sbb);
-                    }
-                    __syncthreads();
-                    for (int e = threadIdx.x; e < N2; e += BS) {
-                        double2 sa, sbb;
-                        ent_na(e, sa, sbb);
-                        AbA[e] = cadd(AbA[e], sa);
-                        AbB[e] = cadd(AbB[e], sbb);
-                    }
-                    __syncthreads();
+                    AbA[n + a * N] = cadd(AbA[n + a * N], sa);
+                    AbB[n + a * N] = cadd(AbB[n + a * N], sbb);
                 }
+                __syncthreads();
 #undef AA
 #undef BB
 #undef TB

@@ -3812,7 +3812,7 @@ static void tape_adjoint_tail(taco_tape& tp, const RkRev& rv, const double2* sl,
         const std::size_t nscr = static_cast<std::size_t>(ncomb) * 11 * T;
         const unsigned l4_grid = static_cast<unsigned>(std::min<std::size_t>(T, 1024));
         const std::size_t l4_per =
-            4 * N2 * static_cast<std::size_t>(tp.nf) + 3 * N2 + N2 * N2 + (lds_tb ? 0 : N2 * N2) + 6 * N2;
+            4 * N2 * static_cast<std::size_t>(tp.nf) + 3 * N2 + N2 * N2 + (lds_tb ? 0 : N2 * N2);
         const std::size_t l4_scr = l4_direct ? 0 : l4_grid * l4_per;
         c->inter2.ensure((sbar_n + abar_n + nscr + static_cast<std::size_t>(tp.nf) * T + l4_scr) * 16 +
                          static_cast<std::size_t>(ncomb) * 8 + 64);
