@@ -317,7 +317,10 @@ __device__ __forceinline__ void rk4_adjoint_body(double2* sh, const Rk4AdjArgs& 
 template <int DT>
 __global__ __launch_bounds__(64) void k_rk4_adjoint_rows(Rk4AdjArgs A) {
     extern __shared__ double2 sh[];
-    rk4_rows_adjoint<DT>(sh, A);
+    if (A.vec)
+        rk4_rows_adjoint<DT, Rk4AdjArgs, true>(sh, A);
+    else
+        rk4_rows_adjoint<DT>(sh, A);
 }
 
 __global__ __launch_bounds__(kBlock) void k_rk4_adjoint(Rk4AdjArgs A) {
@@ -416,7 +419,10 @@ template <int DT>
 __global__ __launch_bounds__(64) void k_rk4_adjoint_rows_multi(const Rk4AdjArgs* As) {
     extern __shared__ double2 sh[];
     const Rk4AdjArgs A = As[blockIdx.x];
-    rk4_rows_adjoint<DT>(sh, A);
+    if (A.vec)
+        rk4_rows_adjoint<DT, Rk4AdjArgs, true>(sh, A);
+    else
+        rk4_rows_adjoint<DT>(sh, A);
 }
 
 __global__ __launch_bounds__(kBlock) void k_rk4_adjoint_multi(const Rk4AdjArgs* As) {

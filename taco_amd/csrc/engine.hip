@@ -3594,9 +3594,18 @@ static bool rk_rev_deferred(const taco_tape& tp) {
     return rk4_nstage(D, 12, static_cast<long>(tp.steps) + 1) < 2;
 }
 
+// The row-group walk (D = 4, 9, 16) stores kb1..kb4 per step and leaves
+// Lbar to k_rk4_lbar as well (TACO_RK4_DEFER=0: Lbar formed in the walk).
+static bool rk_rev_rows_deferred(const taco_tape& tp) {
+    if (env_int("TACO_RK4_DEFER", 1) == 0) return false;
+    const int D = tp.N * tp.N;
+    return D == 4 || D == 9 || D == 16;
+}
+
 static std::size_t rk_rev_values(const taco_tape& tp) {
     const std::size_t D = static_cast<std::size_t>(tp.N) * tp.N;
     const std::size_t base = tp.T * D + tp.T * D * D + 4 * tp.steps * D + D;
+    if (rk_rev_rows_deferred(tp)) return base + 4 * tp.steps * D;
     return rk_rev_deferred(tp) ? base + tp.T * D * D + 4 * tp.steps * D + tp.steps * D * D : base;
 }
 
@@ -3607,7 +3616,9 @@ static RkRev rk_rev_at(const taco_tape& tp, double2* base) {
     b.Lb = b.gb + tp.T * D;
     b.seg = b.Lb + tp.T * D * D;
     b.rb = b.seg + 4 * tp.steps * D;
-    if (rk_rev_deferred(tp)) {
+    if (rk_rev_rows_deferred(tp)) {
+        b.vec = b.rb + D;
+    } else if (rk_rev_deferred(tp)) {
         b.LsH = b.rb + D;
         b.vec = b.LsH + tp.T * D * D;
         b.LhH = b.vec + 4 * tp.steps * D;

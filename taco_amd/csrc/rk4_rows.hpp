@@ -259,7 +259,7 @@ __device__ __forceinline__ void load_points(double2 (&p)[4], double2& gb, const 
 //   Lbar_n     += kb1 r_n^H + (kb3 u2^H + kb2 u1^H) / 2,
 // the L0 part carried in registers to the next (earlier) step.  Generators,
 // stage points and gbar are prefetched S - 1 steps ahead.  LDS: 5 D values.
-template <int DT, class Args>
+template <int DT, class Args, bool DEFER = false>
 struct RowsAdjoint {
     using RG = RowGroup<DT>;
     static constexpr int CK = RG::CK, S = RG::SA, G = RG::G;
@@ -278,6 +278,7 @@ struct RowsAdjoint {
     __device__ __forceinline__ void step(long n, double2 (&L1)[CK], const double2 (&L0)[CK], double2 (&Sg)[4],
                                          double2& Gb) {
         const double2* K4p = K4 + par * DT;
+        const double2 kb4 = cscale(lam, s6);  // = K4p[c]
         // ub3 = L1^H kb4
         const double2 ub3 = group_dot<DT, true>(L1, K4p, g);
         const double2 kb3 = cadd(cscale(lam, 2.0 * s6), cscale(ub3, A->dt));
@@ -303,16 +304,28 @@ struct RowsAdjoint {
         __syncthreads();
         // ub0 = L0^H kb1
         const double2 ub0 = group_dot<DT, true>(L0, K1, g);
-        // generator cotangents of column c, rows g + k G
-        const double2 xc = Sg[0], u1c = Sg[1], u2c = Sg[2], u3c = Sg[3];
-        double2* Lb1 = A->Lbar + (n + 1) * D2;
+        if constexpr (DEFER) {
+            // the step's kb1..kb4 for k_rk4_lbar, which forms every Lbar(t)
+            // afterwards with these sums (all times in parallel)
+            if (own) {
+                double2* v = A->vec + n * 4 * DT;
+                v[c] = kb1;
+                v[DT + c] = kb2;
+                v[2 * DT + c] = kb3;
+                v[3 * DT + c] = kb4;
+            }
+        } else {
+            // generator cotangents of column c, rows g + k G
+            const double2 xc = Sg[0], u1c = Sg[1], u2c = Sg[2], u3c = Sg[3];
+            double2* Lb1 = A->Lbar + (n + 1) * D2;
 #pragma unroll
-        for (int k = 0; k < CK; ++k) {
-            const int rr = g + k * G;
-            if (col && rr < DT) {
-                const double2 lh = cadd(cmul(K3[rr], cconj(u2c)), cmul(K2[rr], cconj(u1c)));
-                Lb1[rr + c * DT] = cadd(Lc[k], cadd(cmul(K4p[rr], cconj(u3c)), cscale(lh, 0.5)));
-                Lc[k] = cadd(cmul(K1[rr], cconj(xc)), cscale(lh, 0.5));
+            for (int k = 0; k < CK; ++k) {
+                const int rr = g + k * G;
+                if (col && rr < DT) {
+                    const double2 lh = cadd(cmul(K3[rr], cconj(u2c)), cmul(K2[rr], cconj(u1c)));
+                    Lb1[rr + c * DT] = cadd(Lc[k], cadd(cmul(K4p[rr], cconj(u3c)), cscale(lh, 0.5)));
+                    Lc[k] = cadd(cmul(K1[rr], cconj(xc)), cscale(lh, 0.5));
+                }
             }
         }
         lam = cadd(cadd(rb, ub0), Gb);
@@ -324,14 +337,17 @@ struct RowsAdjoint {
     }
 };
 
-template <int DT, class Args>
+// DEFER: the walk stores each step's kb1..kb4 (A.vec) instead of forming
+// Lbar; k_rk4_lbar then forms Lbar(t) from them and the stage points, with
+// the walk's own sums (the same values; off the walk's sequential path).
+template <int DT, class Args, bool DEFER = false>
 __device__ __forceinline__ void rk4_rows_adjoint(double2* sh, const Args& A) {
     using RG = RowGroup<DT>;
     static_assert(RG::W == 1, "one wave");
     constexpr int CK = RG::CK, S = RG::SA, G = RG::G;
     constexpr long D2 = static_cast<long>(DT) * DT;
     const int lane = static_cast<int>(threadIdx.x & 63);
-    RowsAdjoint<DT, Args> B;
+    RowsAdjoint<DT, Args, DEFER> B;
     B.A = &A;
     B.K4 = sh;  // two buffers, alternating by step
     B.K3 = sh + 2 * DT;
@@ -369,10 +385,12 @@ __device__ __forceinline__ void rk4_rows_adjoint(double2* sh, const Args& A) {
 #pragma unroll
     for (int j = 0; j + 1 < S; ++j)
         if (nb - j >= 0) B.step(nb - j, Lr[j], Lr[(j + 1) % S], Sg[j], Gb[j]);
+    if constexpr (!DEFER) {
 #pragma unroll
-    for (int k = 0; k < CK; ++k) {
-        const int rr = B.g + k * G;
-        if (B.col && rr < DT) A.Lbar[rr + B.c * DT] = B.Lc[k];
+        for (int k = 0; k < CK; ++k) {
+            const int rr = B.g + k * G;
+            if (B.col && rr < DT) A.Lbar[rr + B.c * DT] = B.Lc[k];
+        }
     }
     if (B.own) A.r0bar[B.c] = B.lam;
 }

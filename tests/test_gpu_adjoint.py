@@ -453,14 +453,19 @@ def test_tfim_losses_many_equal_set_by_set(chain):
         assert np.array_equal(g1.numpy(), grads[q].numpy()), q
 
 
-@pytest.mark.parametrize("kind", ["d16", "d10"])
+@pytest.mark.parametrize("kind", ["d16", "d10", "d4_rows", "d3_rows"])
 def test_deferred_reverse_walk_equals_in_place_walk(monkeypatch, kind):
-    """Large D: the reverse RK4 walk over conjugate-transposed generators with
-    the generator cotangents formed afterwards by k_rk4_lbar gives the
-    cotangents of the in-place walk bit for bit (single and multi-set)."""
+    """The reverse RK4 walk with the generator cotangents formed afterwards by
+    k_rk4_lbar (large D: over conjugate-transposed generators; D = 4, 9, 16:
+    the row-group walk storing kb1..kb4) gives the cotangents of the walk that
+    forms them in place, bit for bit (single and multi-set)."""
     if kind == "d16":
         S, G, dt = D16.system, D16.gamma, D16.dt
         steps, K = D16_STEPS, D16_K
+    elif kind == "d4_rows":
+        name, S, G, dt, steps, K = CASES[1]
+    elif kind == "d3_rows":
+        name, S, G, dt, steps, K = CASES[2]
     else:
         rng = W.SplitMix64(17)
         H = W.gue(10, rng)
