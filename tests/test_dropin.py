@@ -2,6 +2,7 @@
 System-like types, as the reference's callers would use it."""
 import os
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -79,3 +80,16 @@ def test_dropin_matches_python_path():
     assert complex(mk[0], mk[1]) == M[0, 0] and complex(mk[2], mk[3]) == X[5]
     # compute_fcr_convolution_batched equals the full series bit for bit
     assert lines["fcr_batched_equal"][1] == "1"
+
+
+def test_python_taco_package_mirrors_reference_name():
+    """python/taco (the reference's package location) imports on the engine
+    and exposes version() like the reference's bindings, plus the engine's
+    host interface."""
+    code = ("import sys; sys.path.insert(0, 'python'); import taco; "
+            "print(taco.version()); print(hasattr(taco, 'System'), hasattr(taco.adjoint, 'BathTape'))")
+    out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    lines = out.stdout.split("\n")
+    assert lines[0].startswith("taco-mi355x")
+    assert lines[1] == "True True"
