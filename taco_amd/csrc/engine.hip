@@ -1389,15 +1389,16 @@ struct L4fArgs {
     long scr_per;        // complex values per position
 };
 
-__global__ __launch_bounds__(kBlock) void k_l4f(L4fArgs A) {
+template <int BS>
+__global__ __launch_bounds__(BS) void k_l4f(L4fArgs A) {
     extern __shared__ double2 sm[];
     const int N = A.N, N2 = N * N, nf = A.nf, P = A.num_ops * A.num_ops;
     const long pos = blockIdx.x;
     double2* opsS = sm;                                          // num_ops * N^2
     int* f = reinterpret_cast<int*>(opsS + A.num_ops * N2);      // N^2
-    for (int e = threadIdx.x; e < A.num_ops * N2; e += kBlock) opsS[e] = A.ops[e];
+    for (int e = threadIdx.x; e < A.num_ops * N2; e += BS) opsS[e] = A.ops[e];
     int bad = 0;
-    for (int e = threadIdx.x; e < N2; e += kBlock) {
+    for (int e = threadIdx.x; e < N2; e += BS) {
         f[e] = A.p2f[e];
         bad |= (f[e] < 0 || f[e] >= nf);
     }
@@ -1406,7 +1407,7 @@ __global__ __launch_bounds__(kBlock) void k_l4f(L4fArgs A) {
     if (__syncthreads_or(bad)) {
         const double nan = __builtin_nan("");
         const long N4 = static_cast<long>(N2) * N2;
-        for (long idx = threadIdx.x; idx < N4; idx += kBlock) {
+        for (long idx = threadIdx.x; idx < N4; idx += BS) {
             A.L4[pos * N4 + idx] = make_double2(nan, nan);
             if (A.GW) A.GW[pos * N4 + idx] = make_double2(nan, nan);
         }
@@ -1436,7 +1437,7 @@ __global__ __launch_bounds__(kBlock) void k_l4f(L4fArgs A) {
             double2* S67 = pb + 2 * N2;
             double2* Wd = S67 + static_cast<long>(N2) * nf;
             // stage 1: Yd(a,i,beta), Wd(n,i,g), S123(n,i), E(a,i)
-            for (long e = threadIdx.x; e < static_cast<long>(N2) * nf; e += kBlock) {
+            for (long e = threadIdx.x; e < static_cast<long>(N2) * nf; e += BS) {
                 const int a = static_cast<int>(e % N), i = static_cast<int>((e / N) % N);
                 const int g = static_cast<int>(e / N2);
                 double2 y = make_double2(0.0, 0.0), w = y;
@@ -1452,7 +1453,7 @@ __global__ __launch_bounds__(kBlock) void k_l4f(L4fArgs A) {
                 Yd[e] = y;
                 Wd[e] = w;
             }
-            for (int e = threadIdx.x; e < N2; e += kBlock) {
+            for (int e = threadIdx.x; e < N2; e += BS) {
                 const int n = e % N, i = e / N;
                 double2 s = make_double2(0.0, 0.0), ev = s;
                 for (int a = 0; a < N; ++a)
@@ -1478,14 +1479,14 @@ __global__ __launch_bounds__(kBlock) void k_l4f(L4fArgs A) {
             }
             __syncthreads();
             // stage 2: S67(n,i,beta) = sum_a A(n,a) Yd(a,i,beta); D(n,i) = sum_a A(n,a) E(a,i)
-            for (long e = threadIdx.x; e < static_cast<long>(N2) * nf; e += kBlock) {
+            for (long e = threadIdx.x; e < static_cast<long>(N2) * nf; e += BS) {
                 const int n = static_cast<int>(e % N);
                 const long rest = e - n;  // (i, beta) part of the index
                 double2 s = make_double2(0.0, 0.0);
                 for (int a = 0; a < N; ++a) s = cadd(s, cmul(AA(n, a), Yd[rest + a]));
                 S67[e] = s;
             }
-            for (int e = threadIdx.x; e < N2; e += kBlock) {
+            for (int e = threadIdx.x; e < N2; e += BS) {
                 const int n = e % N, i = e / N;
                 double2 s = make_double2(0.0, 0.0);
                 for (int a = 0; a < N; ++a) s = cadd(s, cmul(AA(n, a), Ev[a + i * N]));
@@ -1516,13 +1517,27 @@ __global__ __launch_bounds__(kBlock) void k_l4f(L4fArgs A) {
     };
     const long N4 = static_cast<long>(N2) * N2;
     double2* L = A.L4 + pos * N4;
-    for (long idx = threadIdx.x; idx < N4; idx += kBlock) {
+    for (long idx = threadIdx.x; idx < N4; idx += BS) {
         const int r = static_cast<int>(idx % N2), c = static_cast<int>(idx / N2);  // r = (n, i), c = (m, j)
         const int n = r % N, i = r / N, m = c % N, j = c / N;
         const double2 g = cadd(Tval(n, i, m, j), cconj(Tval(m, j, n, i)));
         if (A.GW) A.GW[pos * N4 + idx] = g;
         L[(n + m * N) + static_cast<long>(i + j * N) * N2] = g;
     }
+}
+
+// Per-time workgroups of the forward L4 and TCL2 kernels: every entry is
+// formed by one thread, so the size only decides which thread owns it (the
+// results are identical at any size).  Large N (d >= 12: one workgroup per
+// time over tens of thousands of entries) runs 1024 threads, whose sixteen
+// waves per CU hide the scattered slice and Gamma reads; small N keeps 256.
+constexpr int kFwdBlockLarge = 1024;
+constexpr int kFwdLargeN = 12;
+static void launch_k_l4f(unsigned grid, int N, std::size_t lds, hipStream_t st, const L4fArgs& F) {
+    if (N >= kFwdLargeN)
+        hipLaunchKernelGGL(k_l4f<kFwdBlockLarge>, dim3(grid), dim3(kFwdBlockLarge), lds, st, F);
+    else
+        hipLaunchKernelGGL(k_l4f<kBlock>, dim3(grid), dim3(kBlock), lds, st, F);
 }
 
 // ---------------------------------------------------------------------------
@@ -1851,6 +1866,7 @@ __host__ __device__ inline std::size_t tcl2_lds(int N) {
 // beta) in the reference's order; a (b, alpha, beta) with A_alpha,b^dag
 // A_beta,b == 0, or with a zero jump rate 2 Re Gamma / Lamb-shift rate Im Gamma,
 // contributes nothing to the respective part, as there.
+template <int BS>
 __device__ void tcl2_channels(const Tcl2Args& A, long t, double2* H, double2* L) {
     const int N = A.N, N2 = N * N, D = N2, no = A.num_ops;
     auto G = [&](int b, int al, int be) {
@@ -1864,7 +1880,7 @@ __device__ void tcl2_channels(const Tcl2Args& A, long t, double2* H, double2* L)
             if (A.p2f[q + r * N] == b && A.p2f[q + c * N] == b) s = cadd(s, cmul(cconj(Aa[q + r * N]), Ab[q + c * N]));
         return s;
     };
-    for (int e = threadIdx.x; e < N2; e += kBlock) {
+    for (int e = threadIdx.x; e < N2; e += BS) {
         const int r = e % N, c = e / N;
         double2 h = make_double2(0.0, 0.0), hc = make_double2(0.0, 0.0);
         if (A.order != 0)
@@ -1884,7 +1900,7 @@ __device__ void tcl2_channels(const Tcl2Args& A, long t, double2* H, double2* L)
         H[e] = v;
     }
     __syncthreads();
-    for (long e = threadIdx.x; e < static_cast<long>(D) * D; e += kBlock) {
+    for (long e = threadIdx.x; e < static_cast<long>(D) * D; e += BS) {
         const int row = static_cast<int>(e % D), col = static_cast<int>(e / D);
         const int r1 = row / N, r2 = row % N, c1 = col / N, c2 = col % N;
         const double2 left = (r1 == c1) ? H[r2 + c2 * N] : make_double2(0, 0);
@@ -1919,14 +1935,15 @@ __device__ void tcl2_channels(const Tcl2Args& A, long t, double2* H, double2* L)
 // lists: one bucket at most off the N-diagonal blocks, the merged lists on
 // them.  The values equal the all-bucket walk (general path) up to the sign
 // of an exact zero.
-__global__ __launch_bounds__(kBlock) void k_tcl2(Tcl2Args A) {
+template <int BS>
+__global__ __launch_bounds__(BS) void k_tcl2(Tcl2Args A) {
     extern __shared__ double2 sh[];
     const int N = A.N, N2 = N * N, D = N2;
     const long t = A.tmap ? A.tmap[blockIdx.x] : static_cast<long>(blockIdx.x);
     double2* H = sh;  // N x N effective Hamiltonian
     double2* L = A.L + static_cast<long>(blockIdx.x) * D * D;
     if (A.gamma_ab) {
-        tcl2_channels(A, t, H, L);
+        tcl2_channels<BS>(A, t, H, L);
         return;
     }
     const bool lists = A.use_lists && N <= kTcl2ListN;
@@ -1934,9 +1951,9 @@ __global__ __launch_bounds__(kBlock) void k_tcl2(Tcl2Args A) {
     int* cnt = P + N2;                                              // list lengths per pair
     unsigned short* bk = reinterpret_cast<unsigned short*>(cnt + N2);  // [pair][N] buckets
     if (lists) {
-        for (int e = threadIdx.x; e < N2; e += kBlock) P[e] = A.p2f[e];
+        for (int e = threadIdx.x; e < N2; e += BS) P[e] = A.p2f[e];
         __syncthreads();
-        for (int pr = threadIdx.x; pr < N2; pr += kBlock) {
+        for (int pr = threadIdx.x; pr < N2; pr += BS) {
             const int x = pr % N, y = pr / N;
             unsigned short* l = bk + pr * N;
             int n = 0;
@@ -1962,7 +1979,7 @@ __global__ __launch_bounds__(kBlock) void k_tcl2(Tcl2Args A) {
         return s;
     };
     // H_eff = diag(eps) + hermitize(sum_b sum_a S_b A_b^dag A_b) with S_b = Im gamma(t,b).
-    for (int e = threadIdx.x; e < N2; e += kBlock) {
+    for (int e = threadIdx.x; e < N2; e += BS) {
         const int r = e % N, c = e / N;
         double2 h = make_double2(0.0, 0.0), hc = make_double2(0.0, 0.0);
         if (A.order != 0) {
@@ -2004,7 +2021,7 @@ __global__ __launch_bounds__(kBlock) void k_tcl2(Tcl2Args A) {
     __syncthreads();
     // L(row=(r1,r2), col=(c1,c2)) with row = r1*N + r2 (kron(I,H) - kron(H^T,I) convention)
     const double2 zero = make_double2(0, 0);
-    for (long e = threadIdx.x; e < static_cast<long>(D) * D; e += kBlock) {
+    for (long e = threadIdx.x; e < static_cast<long>(D) * D; e += BS) {
         const int row = static_cast<int>(e % D), col = static_cast<int>(e / D);
         const int r1 = row / N, r2 = row % N, c1 = col / N, c2 = col % N;
         double2 left = (r1 == c1) ? H[r2 + c2 * N] : make_double2(0, 0);
@@ -2067,6 +2084,13 @@ __global__ __launch_bounds__(kBlock) void k_tcl2(Tcl2Args A) {
         }
         L[e] = cadd(L[e], val);
     }
+}
+
+static void launch_tcl2(unsigned grid, int N, std::size_t lds, hipStream_t st, const Tcl2Args& A) {
+    if (N >= kFwdLargeN)
+        hipLaunchKernelGGL(k_tcl2<kFwdBlockLarge>, dim3(grid), dim3(kFwdBlockLarge), lds, st, A);
+    else
+        hipLaunchKernelGGL(k_tcl2<kBlock>, dim3(grid), dim3(kBlock), lds, st, A);
 }
 
 // ---------------------------------------------------------------------------
@@ -2968,7 +2992,7 @@ static void launch_l4f(taco_ctx* c, const L4Args& A, long npos) {
         F.L4 = A.L4 + p0 * static_cast<long>(N4);
         if (A.GW) F.GW = A.GW + p0 * static_cast<long>(N4);
         timed(c, K_L4, np * (3.0 * A.nf3 + N4) * 16.0, [&] {
-            hipLaunchKernelGGL(k_l4f, dim3(static_cast<unsigned>(np)), dim3(kBlock), lds, c->stream, F);
+            launch_k_l4f(static_cast<unsigned>(np), F.N, lds, c->stream, F);
         });
         HIP_OK(hipGetLastError());
     }
@@ -3250,8 +3274,7 @@ static void batch_enqueue_system(taco_ctx* c, BatchSys& S) {
         A2.L = S.lgen.as<double2>();
         A2.order = S.order;
         timed(c, K_TCL2, static_cast<double>(gen) * 32.0, [&] {
-            hipLaunchKernelGGL(k_tcl2, dim3(static_cast<unsigned>(S.steps + 1)), dim3(kBlock), tcl2_lds(S.N), c->stream,
-                               A2);
+            launch_tcl2(static_cast<unsigned>(S.steps + 1), S.N, tcl2_lds(S.N), c->stream, A2);
         });
     }
     HIP_OK(hipGetLastError());
@@ -3460,7 +3483,7 @@ static void tape_generator(taco_tape& tp, double2* sl) {
     A2.num_ops = tp.num_ops;
     A2.L = tp.lgen.as<double2>();
     A2.order = tp.order;
-    hipLaunchKernelGGL(k_tcl2, dim3(static_cast<unsigned>(T)), dim3(kBlock), tcl2_lds(A2.N), c->stream, A2);
+    launch_tcl2(static_cast<unsigned>(T), A2.N, tcl2_lds(A2.N), c->stream, A2);
     HIP_OK(hipGetLastError());
 }
 
@@ -5173,7 +5196,7 @@ int taco_l4_from_slices_noalloc(const void* Fs, const void* Cs, const void* Rs, 
             F.Rs = static_cast<const double2*>(Rs) + p0 * nf3;
             F.L4 = static_cast<double2*>(L4_out) + p0 * N4;
             F.GW = GW_out ? static_cast<double2*>(GW_out) + p0 * N4 : nullptr;
-            hipLaunchKernelGGL(k_l4f, dim3(static_cast<unsigned>(np)), dim3(kBlock), lds, st, F);
+            launch_k_l4f(static_cast<unsigned>(np), F.N, lds, st, F);
             HIP_OK(hipGetLastError());
         }
     });
@@ -5325,7 +5348,7 @@ int taco_tcl2_generator(taco_ctx* ctx, const double* gamma_ab, size_t Nt, int nf
         A2.gamma_ab = dg;
         A2.skip = dskip;
         A2.tmap = dtm;
-        hipLaunchKernelGGL(k_tcl2, dim3(static_cast<unsigned>(nt)), dim3(kBlock), tcl2_lds(N), ctx->stream, A2);
+        launch_tcl2(static_cast<unsigned>(nt), N, tcl2_lds(N), ctx->stream, A2);
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpyAsync(L_out, ctx->l4.p, nt * N2 * N2 * 16, hipMemcpyDeviceToHost, ctx->stream));
         HIP_OK(hipStreamSynchronize(ctx->stream));
@@ -5405,7 +5428,7 @@ int taco_tcl4_propagate(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, c
         A2.num_ops = num_ops;
         A2.L = ctx->l4.as<double2>();
         A2.order = order;
-        hipLaunchKernelGGL(k_tcl2, dim3(static_cast<unsigned>(T)), dim3(kBlock), tcl2_lds(A2.N), ctx->stream, A2);
+        launch_tcl2(static_cast<unsigned>(T), A2.N, tcl2_lds(A2.N), ctx->stream, A2);
         HIP_OK(hipGetLastError());
         ctx->misc.ensure((T + 1) * N2 * 16 + 64);
         double2* dr = ctx->misc.as<double2>();
