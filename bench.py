@@ -465,8 +465,9 @@ def tfim_adjoint_main(a):
     # correctly rounded phases: each parameter set has its own ~10^5 Omega
     # classes, whose libm table the host would rebuild for every gradient
     taco_amd.set_phase_mode(taco_amd.PHASE_CR)
-    # F/C/R slices (5.1 GB per set at nf = 81) kept for the first sets within
-    # 160 GiB of the 288 GB HBM; the multi-set reverse recomputes the rest
+    # F/C/R slices (5.1 GB per set at nf = 81 whole, 1.5 GB packed to the
+    # entries L4 reads) kept within 160 GiB of the 288 GB HBM; the multi-set
+    # reverse recomputes those of any set past it
     slice_budget = 160 << 30
     taco_amd.set_slice_budget(slice_budget)
     nsets = a.systems or 64
@@ -520,8 +521,11 @@ def tfim_adjoint_main(a):
         grads = grad_many()
     t_grad = (time.perf_counter() - t0) / (a.steps * nsets)
     same = sum(1 for q in range(nseq) if grads[q][0] == seq[q][0] and np.array_equal(grads[q][1], seq[q][1]))
+    modes = AD.MultiTape.last_slice_modes
+    slice_modes = {m: sum(1 for x in modes if x[0] == m) for m in ("whole", "packed", "recompute")}
+    slice_gib = sum(x[1] for x in modes) / 2 ** 30
     print(f"[bench] multi-set tape: {t_grad * 1e3:.1f} ms per set over {nsets} sets; {same}/{nseq} sets bit-equal "
-          f"to the set-by-set gradients", file=sys.stderr, flush=True)
+          f"to the set-by-set gradients; slices {slice_modes}, {slice_gib:.1f} GiB", file=sys.stderr, flush=True)
     from taco_amd import tcl4
     with tcl4.Profile(0) as prof:
         grad(0)
@@ -559,9 +563,10 @@ def tfim_adjoint_main(a):
                         f"TCL2+L4, checkpoints every 64, 5 parameters (lambda, gamma, beta, J, h)",
             "path": "multi-set tape: every set's F/C/R, L4, TCL2 and their reverses in turn, the forward RK4 "
                     "and the reverse RK4 (recompute, walk) of all sets one launch each; F/C/R slices kept "
-                    f"for the sets within a {slice_budget >> 30} GiB slice budget, recomputed in the reverse "
-                    "pass for the rest; bath correlation and Gamma made and reversed on the device; correctly "
-                    "rounded phases",
+                    f"within a {slice_budget >> 30} GiB slice budget (whole, or packed to the entries L4 reads), "
+                    "recomputed in the reverse pass past it; bath correlation and Gamma made and reversed on the "
+                    "device; correctly rounded phases",
+            "slices": {"sets": slice_modes, "GiB": round(slice_gib, 2)},
         },
         "forward_ms": 1e3 * t_fwd,
         "gradient_over_forward": t_grad / t_fwd,

@@ -424,8 +424,10 @@ int taco_tape_adjoint_bath(taco_tape* tape, const double* samples_bar, double* b
  * (a workgroup per set) and so are the reverse RK4's recompute and walk.
  * Every set's samples and cotangents equal those of taco_tape_forward /
  * taco_tape_adjoint on that set alone, bit for bit.  The sets' F/C/R slices
- * stay on the device, set by set in order, while their running total fits
- * the context's slice budget (taco_set_slice_budget); the reverse pass
+ * stay on the device while they fit the context's slice budget
+ * (taco_set_slice_budget): whole when all sets' slices fit, else packed to
+ * the entries the L4 passes read (about a third of the whole at d = 16), set
+ * by set in order while their running total fits; the reverse pass
  * recomputes the slices of the sets past it in one shared buffer
  * (taco_mtape_keeps_slices: 1 when every set keeps its slices).
  * samples_out / samples_bar / the cotangent arrays hold one pointer per set
@@ -468,6 +470,10 @@ int taco_mtape_adjoint_bath(taco_mtape* mt, const double* const* samples_bar, do
                             double* const* omega_bar, double* const* ops_bar, double* const* eps_bar,
                             double* const* r0_bar);
 int taco_mtape_keeps_slices(const taco_mtape* mt);
+/* How set s keeps its F/C/R slices: 0 recomputed in the reverse pass, 1
+ * whole, 2 packed; *bytes (if not NULL) = the device bytes they take (0 when
+ * recomputed).  -1 for a NULL tape or s out of range. */
+int taco_mtape_set_slices(const taco_mtape* mt, size_t s, size_t* bytes);
 int taco_mtape_destroy(taco_mtape* mt);
 /* One-shot taco_tape_forward + taco_tape_adjoint + taco_tape_destroy. */
 int taco_adjoint(taco_ctx* ctx, const double* gamma, size_t Nt, int nf, const double* omegas, const int* mirror,

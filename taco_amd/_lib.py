@@ -123,6 +123,7 @@ SIGNATURES = {
                                         POINTER(c_void_p)]),
     "taco_mtape_adjoint_bath": (c_int, [c_void_p, _dpp, _dpp, _dpp, _dpp, _dpp, _dpp]),
     "taco_mtape_keeps_slices": (c_int, [c_void_p]),
+    "taco_mtape_set_slices": (c_int, [c_void_p, c_size_t, POINTER(c_size_t)]),
     "taco_mtape_destroy": (c_int, [c_void_p]),
     "taco_adjoint": (c_int, [c_void_p, _dp, c_size_t, c_int, _dp, _ip, _ip, c_int, _dp, _dp, c_int, c_double,
                              c_size_t, c_int, _dp, c_size_t, _dp, _dp, _dp, _dp, _dp, _dp]),

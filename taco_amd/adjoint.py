@@ -171,6 +171,8 @@ class MultiTape:
     in BathTape: each dict gives the grid length "Nt" instead of "gamma", and
     `adjoint` returns each set's "bath" cotangents instead of "gamma"."""
 
+    last_slice_modes: list = []  # slice_modes of the most recently made multi-set tape
+
     def __init__(self, sets, checkpoint_every: int = 64, exec: Optional[Exec] = None, baths=None):
         from ._lib import BathParams, TapeSet
         self.ctx = _ctx_for(exec)
@@ -214,10 +216,24 @@ class MultiTape:
                                            _ptrs(self.samples), ctypes.byref(h)))
         self.handle = h
         self._keep = None  # inputs are only read during the forward call
+        MultiTape.last_slice_modes = self.slice_modes
 
     @property
     def keeps_slices(self) -> bool:
         return bool(lib().taco_mtape_keeps_slices(self.handle))
+
+    @property
+    def slice_modes(self) -> list:
+        """Per set: ("recompute" | "whole" | "packed", device bytes of its slices)."""
+        names = {0: "recompute", 1: "whole", 2: "packed"}
+        out = []
+        for s in range(len(self.meta)):
+            b = ctypes.c_size_t(0)
+            m = lib().taco_mtape_set_slices(self.handle, s, ctypes.byref(b))
+            if m < 0:
+                raise ValueError("multi-set tape is closed")
+            out.append((names[m], int(b.value)))
+        return out
 
     def adjoint(self, samples_bar) -> list:
         sb = [np.ascontiguousarray(np.asarray(b, np.complex128).reshape(m["steps"] + 1, m["N"] ** 2))

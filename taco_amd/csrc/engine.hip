@@ -2179,7 +2179,8 @@ struct TapeBufs {
 // created while another holds it gets buffers of its own.
 struct MtapeArena {
     DevBuf slices;  // the F/C/R slices of the sets within the slice budget, one after another
-    DevBuf shared;  // one set's slices at a time, for the sets past the budget
+    DevBuf shared;  // one set's slices at a time, for the sets past the budget and packed sets
+    DevBuf pkidx;   // packed sets' triple lists (F, C, R back to back per distinct bucket map)
     DevBuf samp, rv, args, mid;
 };
 
@@ -4422,22 +4423,50 @@ static void tape_bath_adjoint(taco_tape& tp, const double2* gb4, const double2* 
     bath_bar[2] = gb;
 }
 
+// Whole slices ([3][T][nf3]: F, C, R) to and from a packed copy holding only
+// the entries the L4 passes read: F at its needed triples, then C, then R,
+// each [T][count]; idx holds the three ascending triple lists back to back.
+// PACK copies whole -> packed, otherwise packed -> whole (the other whole
+// entries are left as they are: nothing reads them).
+template <bool PACK>
+__global__ __launch_bounds__(kBlock) void k_slices_pack(double2* whole, double2* pk, const int* idx, long nF, long nC,
+                                                          long nR, long nf3, long T) {
+    const long q = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (q >= nF + nC + nR) return;
+    const int comp = q < nF ? 0 : (q < nF + nC ? 1 : 2);
+    const long first = comp == 0 ? 0 : (comp == 1 ? nF : nF + nC);
+    const long cnt = comp == 0 ? nF : (comp == 1 ? nC : nR);
+    const long tri = idx[q];
+    for (long t = blockIdx.y; t < T; t += gridDim.y) {
+        const long w = (comp * T + t) * nf3 + tri;
+        const long p = first * T + t * cnt + (q - first);
+        if constexpr (PACK)
+            pk[p] = whole[w];
+        else
+            whole[w] = pk[p];
+    }
+}
+
 }  // namespace taco_engine
 
 // Several tapes of one N, step count and checkpoint interval (parameter
 // sets of one model): each set's F/C/R, L4 and TCL2 passes run in turn, the
 // forward RK4 of every set is one launch (a workgroup per set), and so are
 // the reverse RK4's recompute and walk.  The sets' F/C/R slices stay on the
-// device, set by set in order, while their running total fits the context's
-// slice budget; the sets past it share one buffer that holds one set's
-// slices at a time, and the reverse pass recomputes theirs (the same pass,
-// the same values).
+// device while they fit the context's slice budget: whole when every set's
+// do, else packed to the entries the L4 passes read (about a third of
+// 3 nf^3 at d = 16), set by set in order while the running total fits.  The
+// sets past it, and the packed sets while they are unpacked or made, share
+// one buffer of one set's whole slices, and the reverse pass recomputes the
+// slices of the sets past the budget (the same pass, the same values).
 struct taco_mtape {
     taco_ctx* ctx = nullptr;
     std::vector<std::unique_ptr<taco_tape>> sets;
     bool keep = true;             // every set keeps its slices
-    std::vector<char> keep_set;   // per set
+    std::vector<char> keep_set;   // per set: 0 recomputed, 1 kept whole, 2 kept packed
     std::vector<std::size_t> off; // kept sets: offset of their slices in A->slices (complex values)
+    std::vector<std::size_t> pk_at;          // packed sets: offset of their triple lists in A->pkidx (ints)
+    std::vector<std::array<long, 3>> pk_n;   // packed sets: F, C and R entry counts per time
     // A->samp [S][T][D] sampled states, A->rv [S][rk_rev_values] reverse RK4
     // state, A->args launch arguments, A->mid [S][steps][D^2] midpoint
     // generators of the forward RK4 (large D)
@@ -4462,10 +4491,102 @@ static void mtape_claim(taco_mtape& mt) {
     }
 }
 
+// Where set s's whole slices are read and written: its own region when kept
+// whole, else the shared buffer.
 static double2* mtape_slices(taco_mtape& mt, std::size_t s) {
-    taco_tape& tp = *mt.sets[s];
-    (void)tp;
-    return mt.keep_set[s] ? mt.A->slices.as<double2>() + mt.off[s] : mt.A->shared.as<double2>();
+    return mt.keep_set[s] == 1 ? mt.A->slices.as<double2>() + mt.off[s] : mt.A->shared.as<double2>();
+}
+
+// Packs set s's whole slices from the shared buffer into its region
+// (PACK), or unpacks them back: every entry the L4 passes read is copied,
+// bit for bit.
+template <bool PACK>
+static void mtape_pack(taco_mtape& mt, std::size_t s) {
+    const taco_tape& tp = *mt.sets[s];
+    const long nf3 = static_cast<long>(tp.nf) * tp.nf * tp.nf, T = static_cast<long>(tp.T);
+    const std::array<long, 3>& n = mt.pk_n[s];
+    const long ntot = n[0] + n[1] + n[2];
+    if (ntot == 0 || T == 0) return;
+    const dim3 grid(static_cast<unsigned>((ntot + kBlock - 1) / kBlock), static_cast<unsigned>(std::min<long>(T, 65535)));
+    hipLaunchKernelGGL(k_slices_pack<PACK>, grid, dim3(kBlock), 0, mt.ctx->stream, mt.A->shared.as<double2>(),
+                       mt.A->slices.as<double2>() + mt.off[s], mt.A->pkidx.as<const int>() + mt.pk_at[s], n[0], n[1],
+                       n[2], nf3, T);
+    HIP_OK(hipGetLastError());
+}
+
+// Chooses each set's slice mode within the slice budget (whole for all, else
+// packed in order while they fit, the rest recomputed) and uploads the
+// packed sets' triple lists, one copy per distinct bucket map.
+static void mtape_plan(taco_mtape& mt) {
+    const std::size_t S = mt.sets.size();
+    const std::size_t budget = slice_budget(mt.ctx);
+    mt.keep_set.assign(S, 1);
+    mt.off.assign(S, 0);
+    mt.pk_at.assign(S, 0);
+    mt.pk_n.assign(S, std::array<long, 3>{0, 0, 0});
+    mt.keep = true;
+    std::size_t total = 0, most = 0;
+    for (std::size_t s = 0; s < S; ++s) {
+        const std::size_t b = tape_slice_bytes(*mt.sets[s]);
+        most = std::max(most, b);
+        total += b;
+    }
+    if (total <= budget) {
+        total = 0;
+        for (std::size_t s = 0; s < S; ++s) {
+            mt.off[s] = total / 16;
+            total += tape_slice_bytes(*mt.sets[s]);
+        }
+        mt.A->slices.ensure(std::max<std::size_t>(total, 16));
+        return;
+    }
+    std::vector<int> lists;
+    std::vector<std::pair<std::vector<int>, std::size_t>> seen;  // bucket-map key -> set using those lists
+    total = 0;
+    for (std::size_t s = 0; s < S; ++s) {
+        const taco_tape& tp = *mt.sets[s];
+        if (tp.order != 4) continue;  // no slices
+        std::vector<int> key(tp.h_p2f);
+        key.push_back(tp.N);
+        key.push_back(tp.nf);
+        auto it = std::find_if(seen.begin(), seen.end(), [&](const auto& e) { return e.first == key; });
+        if (it != seen.end()) {
+            mt.pk_at[s] = mt.pk_at[it->second];
+            mt.pk_n[s] = mt.pk_n[it->second];
+        } else {
+            const std::vector<unsigned char> need = needed_triples(tp.h_p2f.data(), tp.N, tp.nf);
+            mt.pk_at[s] = lists.size();
+            for (int bit = 0; bit < 3; ++bit) {
+                long cnt = 0;
+                for (std::size_t t = 0; t < need.size(); ++t)
+                    if (need[t] & (1 << bit)) {
+                        lists.push_back(static_cast<int>(t));
+                        ++cnt;
+                    }
+                mt.pk_n[s][bit] = cnt;
+            }
+            seen.emplace_back(std::move(key), s);
+        }
+        const std::size_t b =
+            static_cast<std::size_t>(mt.pk_n[s][0] + mt.pk_n[s][1] + mt.pk_n[s][2]) * tp.T * 16;
+        if (b <= budget - std::min(budget, total)) {
+            mt.keep_set[s] = 2;
+            mt.off[s] = total / 16;
+            total += b;
+        } else {
+            mt.keep_set[s] = 0;
+            mt.keep = false;
+        }
+    }
+    mt.A->slices.ensure(std::max<std::size_t>(total, 16));
+    mt.A->shared.ensure(std::max<std::size_t>(most, 16));
+    mt.A->pkidx.ensure(std::max<std::size_t>(lists.size(), 1) * sizeof(int));
+    if (!lists.empty()) {
+        // in stream order after the arena's earlier users; lists dies on return
+        HIP_OK(hipMemcpyAsync(mt.A->pkidx.p, lists.data(), lists.size() * sizeof(int), hipMemcpyHostToDevice,
+                              mt.ctx->stream));
+        HIP_OK(hipStreamSynchronize(mt.ctx->stream));
+    }
 }
 
 static void mtape_forward(taco_mtape& mt, const taco_tape_set* in, double* const* samples_out) {
@@ -4473,25 +4594,11 @@ static void mtape_forward(taco_mtape& mt, const taco_tape_set* in, double* const
     const std::size_t S = mt.sets.size();
     const taco_tape& t0 = *mt.sets[0];
     const std::size_t D = static_cast<std::size_t>(t0.N) * t0.N, T = t0.T;
-    const std::size_t budget = slice_budget(c);
-    std::size_t total = 0, most = 0;
-    mt.keep_set.assign(S, 0);
-    mt.off.assign(S, 0);
-    mt.keep = true;
+    mtape_plan(mt);
     for (std::size_t s = 0; s < S; ++s) {
-        const std::size_t b = tape_slice_bytes(*mt.sets[s]);
-        if (total + b <= budget) {
-            mt.off[s] = total / 16;
-            total += b;
-            mt.keep_set[s] = 1;
-        } else {
-            mt.keep = false;
-            most = std::max(most, b);
-        }
+        tape_generator(*mt.sets[s], mtape_slices(mt, s));
+        if (mt.keep_set[s] == 2) mtape_pack<true>(mt, s);
     }
-    mt.A->slices.ensure(std::max<std::size_t>(total, 16));
-    if (!mt.keep) mt.A->shared.ensure(std::max<std::size_t>(most, 16));
-    for (std::size_t s = 0; s < S; ++s) tape_generator(*mt.sets[s], mtape_slices(mt, s));
     // one RK4 launch over every set: r0 in, samples out ([S][T][D]), final r after them
     mt.A->samp.ensure((S * T + 2 * S) * D * 16);
     double2* samp = mt.A->samp.as<double2>();
@@ -4604,7 +4711,8 @@ static void mtape_adjoint(taco_mtape& mt, const double* const* samples_bar, doub
     HIP_OK(hipGetLastError());
     for (std::size_t s = 0; s < S; ++s) {
         taco_tape& tp = *mt.sets[s];
-        if (!mt.keep_set[s] && tp.order == 4) tape_slices(tp, mt.A->shared.as<double2>());
+        if (tp.order == 4 && mt.keep_set[s] == 0) tape_slices(tp, mt.A->shared.as<double2>());
+        if (tp.order == 4 && mt.keep_set[s] == 2) mtape_pack<false>(mt, s);
         auto pick = [&](double* const* v) { return v ? v[s] : nullptr; };
         tape_adjoint_tail(tp, rv[s], mtape_slices(mt, s), pick(gamma_bar), pick(omega_bar), pick(ops_bar),
                           pick(eps_bar), pick(r0_bar), pick(bath_bar));
@@ -5728,6 +5836,19 @@ int taco_mtape_adjoint_bath(taco_mtape* mt, const double* const* samples_bar, do
 }
 
 int taco_mtape_keeps_slices(const taco_mtape* mt) { return mt && mt->keep ? 1 : 0; }
+
+int taco_mtape_set_slices(const taco_mtape* mt, size_t s, size_t* bytes) {
+    if (!mt || s >= mt->sets.size()) return -1;
+    const int mode = mt->keep_set[s];
+    if (bytes) {
+        const taco_tape& tp = *mt->sets[s];
+        const auto& n = mt->pk_n[s];
+        *bytes = mode == 1 ? tape_slice_bytes(tp)
+                 : mode == 2 ? static_cast<std::size_t>(n[0] + n[1] + n[2]) * tp.T * 16
+                             : 0;
+    }
+    return mode;
+}
 
 int taco_mtape_destroy(taco_mtape* mt) {
     return guarded([&] {

@@ -331,26 +331,51 @@ def _mt_sets(kind):
                  ops=w.system.A_eig, eps=w.system.eps, dt=w.dt, steps=6, r0=r0_for(16)) for w in ws], 4
 
 
+def _whole_slice_bytes(st):
+    return 3 * len(st["omegas"]) ** 3 * (st["steps"] + 1) * 16 if st.get("order", 4) == 4 else 0
+
+
 @pytest.mark.parametrize("kind", ["spin_pair_d4", "gue3_mixed_order", "tfim_d16"])
-@pytest.mark.parametrize("budget", ["keep", "partial", "recompute"])
+@pytest.mark.parametrize("budget", ["keep", "packed", "partial", "recompute"])
 def test_multi_set_tape_equals_single_tapes(kind, budget):
     """Every set of a multi-set tape (one RK4 launch forward, one recompute
     and one walk launch in reverse for all sets) has the samples and the
     cotangents of a single tape of that set, bit for bit -- with the slices
-    kept, with a slice budget that keeps only the first set's, and with one
-    that forces the recomputation of all."""
+    kept whole, kept packed (a budget just short of the whole slices), with a
+    budget that packs only the first set's, and with one that forces the
+    recomputation of all."""
     sets, K = _mt_sets(kind)
     rng = np.random.default_rng(31)
     bars = [rng.standard_normal((s["steps"] + 1, len(s["r0"]))) + 1j * rng.standard_normal(
         (s["steps"] + 1, len(s["r0"]))) for s in sets]
+    sliced = [k for k, st in enumerate(sets) if st.get("order", 4) == 4]
+    packed_budget = sum(_whole_slice_bytes(st) for st in sets) - 16
     if budget == "recompute":
         taco_amd.set_slice_budget(16)
-    elif budget == "partial":  # room for the first set's slices only
-        nf0 = len(sets[0]["omegas"])
-        taco_amd.set_slice_budget(3 * nf0 ** 3 * (sets[0]["steps"] + 1) * 16)
+    elif budget == "packed":
+        taco_amd.set_slice_budget(packed_budget)
+    elif budget == "partial":  # room for the first set's packed slices only
+        taco_amd.set_slice_budget(packed_budget)
+        try:
+            probe = AD.MultiTape(sets, checkpoint_every=K)
+            first = probe.slice_modes[sliced[0]]
+            probe.close()
+        finally:
+            taco_amd.set_slice_budget(0)
+        assert first[0] == "packed" and 0 < first[1] < _whole_slice_bytes(sets[sliced[0]])
+        taco_amd.set_slice_budget(first[1] + 16)
     try:
         mt = AD.MultiTape(sets, checkpoint_every=K)
-        assert mt.keeps_slices == (budget == "keep")
+        modes = mt.slice_modes
+        want = {"keep": "whole", "packed": "packed", "recompute": "recompute"}
+        for k in sliced:
+            if budget == "partial":
+                assert modes[k][0] == ("packed" if k == sliced[0] else "recompute"), (kind, modes)
+            else:
+                assert modes[k][0] == want[budget], (kind, modes)
+        if budget == "keep":
+            assert [m[1] for m in modes] == [_whole_slice_bytes(st) for st in sets]
+        assert mt.keeps_slices == (budget in ("keep", "packed"))
         got = mt.adjoint(bars)
         again = mt.adjoint(bars)
         samples = [s.copy() for s in mt.samples]
@@ -398,7 +423,7 @@ def test_two_live_multi_set_tapes_keep_their_own_state():
                 assert np.array_equal(res[key], ref[key]), (k, key)
 
 
-@pytest.mark.parametrize("budget", ["keep", "recompute"])
+@pytest.mark.parametrize("budget", ["keep", "packed", "recompute"])
 def test_multi_set_bath_tape_equals_single_bath_tapes(budget):
     """Multi-set tape with device-made Gamma tables (taco_mtape_*_bath): every
     set's samples and cotangents, the bath-parameter ones included, equal a
@@ -412,9 +437,13 @@ def test_multi_set_bath_tape_equals_single_bath_tapes(budget):
     bars = [rng.standard_normal((121, 16)) + 1j * rng.standard_normal((121, 16)) for _ in sets]
     if budget == "recompute":
         taco_amd.set_slice_budget(16)
+    elif budget == "packed":
+        taco_amd.set_slice_budget(sum(_whole_slice_bytes(st) for st in sets) - 16)
     try:
         mt = AD.MultiTape(sets, checkpoint_every=32, baths=baths)
-        assert mt.keeps_slices == (budget == "keep")
+        assert mt.keeps_slices == (budget != "recompute")
+        assert {m[0] for m in mt.slice_modes} == {"keep": {"whole"}, "packed": {"packed"},
+                                                  "recompute": {"recompute"}}[budget]
         got = mt.adjoint(bars)
         samples = [x.copy() for x in mt.samples]
         mt.close()
