@@ -34,6 +34,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -3820,6 +3821,23 @@ static void tape_adjoint_tail(taco_tape& tp, const RkRev& rv, const double2* sl,
     std::vector<std::complex<double>> gbar_h(static_cast<std::size_t>(tp.nf) * T, {0.0, 0.0});
     std::vector<std::complex<double>> opsb(static_cast<std::size_t>(tp.num_ops) * N2, {0.0, 0.0});
     std::vector<double> omb(static_cast<std::size_t>(tp.nf), 0.0), epsb(static_cast<std::size_t>(tp.N), 0.0);
+    // The L4 part's host sums wait only for its copies (event l4_done), after
+    // the TCL2 reverse is queued: the device runs that meanwhile.  Host
+    // buffers of those copies live until then.
+    std::vector<int> bl_start, bl_ent;        // bucket lists uploaded for k_bucket_reduce
+    std::vector<double> omp;                  // per combine position omega partials
+    std::vector<std::complex<double>> ap;     // per-time operator partials
+    const std::vector<int>* comb_order = nullptr;
+    hipEvent_t l4_done = nullptr;
+    // on an error, the copies still queued finish before those buffers go
+    struct SyncOnUnwind {
+        hipStream_t s;
+        hipEvent_t* e;
+        ~SyncOnUnwind() {
+            if (std::uncaught_exceptions() > 0) (void)hipStreamSynchronize(s);
+            if (*e) (void)hipEventDestroy(*e);
+        }
+    } unwind_guard{c->stream, &l4_done};
     const double2* gb4_dev = nullptr;  // Gamma_bar of the L4 part on the device, when there is one
     if (bath_bar && !tp.bath) throw Error(TACO_ERR_INVALID, "bath cotangents need a bath-sourced tape");
     if (tp.order == 4) {
@@ -3975,7 +3993,9 @@ static void tape_adjoint_tail(taco_tape& tp, const RkRev& rv, const double2* sl,
                 lists[static_cast<std::size_t>(tp.h_mirror[static_cast<std::size_t>(j)])].push_back(
                     static_cast<int>(q * 3 + 2));
             }
-            std::vector<int> start(1, 0), ent;
+            std::vector<int>& start = bl_start;
+            std::vector<int>& ent = bl_ent;
+            start.assign(1, 0);
             for (auto& l : lists) {
                 ent.insert(ent.end(), l.begin(), l.end());
                 start.push_back(static_cast<int>(ent.size()));
@@ -3995,24 +4015,15 @@ static void tape_adjoint_tail(taco_tape& tp, const RkRev& rv, const double2* sl,
             HIP_OK(hipGetLastError());
             gb4_dev = gb4;
             HIP_OK(hipMemcpyAsync(gbar_h.data(), gb4, gbar_h.size() * 16, hipMemcpyDeviceToHost, c->stream));
-            std::vector<double> omp(static_cast<std::size_t>(ncomb));
+            omp.resize(static_cast<std::size_t>(ncomb));
             HIP_OK(hipMemcpyAsync(omp.data(), om, omp.size() * 8, hipMemcpyDeviceToHost, c->stream));
-            HIP_OK(hipStreamSynchronize(c->stream));
-            for (long q = 0; q < ncomb; ++q) {
-                const int tri = J.order[static_cast<std::size_t>(q)];
-                const int k = tri % tp.nf, j = (tri / tp.nf) % tp.nf, i = tri / (tp.nf * tp.nf);
-                const double v = omp[static_cast<std::size_t>(q)];
-                omb[static_cast<std::size_t>(i)] += v;
-                omb[static_cast<std::size_t>(j)] += v;
-                omb[static_cast<std::size_t>(k)] += v;
-            }
+            comb_order = &J.order;
         }
-        // operator cotangents of the L4 part: per-time partials summed in time order
-        std::vector<std::complex<double>> ap(abar_n);
+        // operator cotangents of the L4 part: per-time partials, summed on the host below
+        ap.resize(abar_n);
         HIP_OK(hipMemcpyAsync(ap.data(), abar, abar_n * 16, hipMemcpyDeviceToHost, c->stream));
-        HIP_OK(hipStreamSynchronize(c->stream));
-        for (std::size_t t = 0; t < T; ++t)
-            for (std::size_t e = 0; e < opsb.size(); ++e) opsb[e] += ap[t * opsb.size() + e];
+        HIP_OK(hipEventCreateWithFlags(&l4_done, hipEventDisableTiming));
+        HIP_OK(hipEventRecord(l4_done, c->stream));
     }
     // TCL2 part: per-time reverse and time sums on the device, operator
     // cotangents on the host from the sums
@@ -4063,6 +4074,24 @@ static void tape_adjoint_tail(taco_tape& tp, const RkRev& rv, const double2* sl,
         hipLaunchKernelGGL(k_tcl2_ops_parts, dim3(static_cast<unsigned>(npart)), dim3(kBlock), 2 * N2 * 16, c->stream,
                            t2w, t2w + nW, t2P, t2b, tp.N, t2Pb);
         HIP_OK(hipGetLastError());
+    }
+    if (l4_done) {
+        // the L4 part's host sums (omega partials in combine order, operator
+        // partials in time order) while the TCL2 reverse runs
+        const hipError_t we = hipEventSynchronize(l4_done);
+        (void)hipEventDestroy(l4_done);
+        l4_done = nullptr;
+        HIP_OK(we);
+        for (std::size_t q = 0; q < omp.size(); ++q) {
+            const int tri = (*comb_order)[q];
+            const int k = tri % tp.nf, j = (tri / tp.nf) % tp.nf, i = tri / (tp.nf * tp.nf);
+            const double v = omp[q];
+            omb[static_cast<std::size_t>(i)] += v;
+            omb[static_cast<std::size_t>(j)] += v;
+            omb[static_cast<std::size_t>(k)] += v;
+        }
+        for (std::size_t t = 0; t < T; ++t)
+            for (std::size_t e = 0; e < opsb.size(); ++e) opsb[e] += ap[t * opsb.size() + e];
     }
     if (bath_bar) tape_bath_adjoint(tp, gb4_dev, tp.order != 0 ? t2g : nullptr, bath_bar, omb);
     std::vector<std::complex<double>> g2(nfz * T), wv(nV + static_cast<std::size_t>(tp.N)), r0b(D), pb(npart * N2);
